@@ -1,0 +1,111 @@
+/*
+ * ksig_hip.h — C ABI of the MI355X (gfx950) signature-kernel library
+ * (libksig_hip.so).  Plain pointers and sizes only; every pointer argument
+ * named x, y, pairs, out, grad_* or M is a DEVICE pointer, and `stream` is a
+ * hipStream_t (NULL = the default stream).  All calls are asynchronous on
+ * `stream`; none allocates device memory or synchronises.
+ *
+ * Sequence sets are packed rows: sequence a occupies rows
+ * [off[a], off[a] + len[a]) of a row-major [total_rows, d] array.  A dense
+ * [n, L, d] array is the case off[a] = a*L, len[a] = L.
+ *
+ * Return value: 0 on success, otherwise a KSIG_E* code or a hipError_t;
+ * ksig_last_error() describes the last failure of the calling thread.
+ */
+#ifndef KSIG_HIP_H_
+#define KSIG_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* dtype codes */
+#define KSIG_F32 0
+#define KSIG_F64 1
+
+/* static kernels */
+#define KSIG_STATIC_RBF 0    /* ksig/static/kernels.py:245-260, param = bandwidth */
+#define KSIG_STATIC_LINEAR 1 /* ksig/static/kernels.py:134-166, param = scale */
+
+/* error codes (besides hipError_t values) */
+#define KSIG_EINVAL 1001      /* bad argument (shape, dtype, kind, null pointer) */
+#define KSIG_EUNSUPPORTED 1002 /* (n_levels, order) or d outside the built set */
+#define KSIG_ELDS 1003        /* sequence too long for the per-pair LDS state */
+
+/* Library version string. */
+const char* ksig_version(void);
+
+/* Description of the last error raised on the calling thread. */
+const char* ksig_last_error(void);
+
+/* 1 if (n_levels, order) is compiled into the DP kernels (order already
+ * normalised as in ksig/algorithms.py:37), else 0. */
+int ksig_dp_supported(int n_levels, int order);
+
+/*
+ * Truncated signature kernel, level values, for a list of sequence pairs.
+ *
+ * Replaces, per pair, the composition used by SignatureKernel._compute_kernel
+ * (ksig/kernels.py:147-164): the static Gram of ksig/static/kernels.py:248-260
+ * (RBF) or :145-155 (linear) followed by signature_kern(M, n_levels, order,
+ * difference, return_levels=True) of ksig/algorithms.py:22-140.
+ *
+ *   out[p, m] = K_m(x_{pairs[p,0]}, y_{pairs[p,1]}),  m = 0..n_levels,
+ *   with K_0 = 1, as float64.
+ *
+ * max_len_x / max_len_y bound the sequence lengths (they size LDS).
+ * dtype selects float32 or float64 inputs and arithmetic.
+ */
+int ksig_signature_levels(int dtype, const void* x, const int64_t* x_off,
+                          const int32_t* x_len, int max_len_x, const void* y,
+                          const int64_t* y_off, const int32_t* y_len,
+                          int max_len_y, int d, const int32_t* pairs,
+                          int64_t n_pairs, int static_kind, double static_param,
+                          int n_levels, int order, int difference, double* out,
+                          void* stream);
+
+/*
+ * Truncated signature kernel on caller-supplied static-kernel matrices.
+ *
+ * Replaces signature_kern(M, n_levels, order, difference, return_levels=True)
+ * of ksig/algorithms.py:22-44 for M of shape [n_mats, lx, ly] (contiguous,
+ * any leading shape flattened): out[p, m] = K_m of matrix p, float64.
+ */
+int ksig_signature_levels_gram(int dtype, const void* M, int64_t n_mats,
+                               int lx, int ly, int n_levels, int order,
+                               int difference, double* out, void* stream);
+
+/*
+ * Signature-PDE kernel for a list of sequence pairs.
+ *
+ * Replaces, per pair, SignaturePDEKernel._compute_kernel
+ * (ksig/kernels.py:242-256): the static Gram followed by
+ * signature_kern_pde(M, difference) of ksig/algorithms.py:402-446
+ * (device kernel :344-399).  dyadic_order refines each ΔΔ cell into
+ * 2^λ × 2^λ cells carrying M/4^λ (λ = 0 is the reference scheme).
+ *   out[p] = K(x_{pairs[p,0]}, y_{pairs[p,1]}) as float64.
+ */
+int ksig_signature_pde(int dtype, const void* x, const int64_t* x_off,
+                       const int32_t* x_len, int max_len_x, const void* y,
+                       const int64_t* y_off, const int32_t* y_len,
+                       int max_len_y, int d, const int32_t* pairs,
+                       int64_t n_pairs, int static_kind, double static_param,
+                       int dyadic_order, int difference, double* out,
+                       void* stream);
+
+/*
+ * Signature-PDE kernel on caller-supplied matrices M [n_mats, lx, ly].
+ * Replaces signature_kern_pde(M, difference) of ksig/algorithms.py:402-446.
+ */
+int ksig_signature_pde_gram(int dtype, const void* M, int64_t n_mats, int lx,
+                            int ly, int dyadic_order, int difference,
+                            double* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KSIG_HIP_H_ */

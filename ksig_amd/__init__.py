@@ -1,0 +1,16 @@
+"""ksig_amd — MI355X (gfx950) signature-kernel hot path with the ksig API.
+
+Drop-in for ``ksig.kernels.SignatureKernel`` / ``SignaturePDEKernel`` with
+``ksig.static.kernels.RBFKernel`` / ``LinearKernel`` and the functional
+``ksig.algorithms.signature_kern`` / ``signature_kern_pde``::
+
+    import ksig_amd as ksig
+    K = ksig.kernels.SignatureKernel(5, static_kernel=ksig.static.kernels.RBFKernel())(X)
+"""
+
+from . import utils
+from . import static
+from . import algorithms
+from . import kernels
+
+__version__ = "0.1.0"

@@ -1,0 +1,57 @@
+// Internal launch interface of the DP and PDE sweeps (not part of the C-ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ksig_common.h"
+
+// (n_levels, order) combinations compiled into the library.  order is the
+// normalised order of ksig/algorithms.py:37 (1 <= order <= n_levels).
+#define KSIG_DP_LEVEL_ORDER_TABLE(X)                                              \
+  X(1, 1)                                                                         \
+  X(2, 1) X(2, 2)                                                                 \
+  X(3, 1) X(3, 2) X(3, 3)                                                         \
+  X(4, 1) X(4, 2) X(4, 3) X(4, 4)                                                 \
+  X(5, 1) X(5, 2) X(5, 3) X(5, 4) X(5, 5)                                         \
+  X(6, 1) X(6, 2) X(6, 3) X(6, 6)                                                 \
+  X(7, 1) X(7, 2) X(7, 3)                                                         \
+  X(8, 1) X(8, 2) X(8, 3)                                                         \
+  X(9, 1) X(9, 2)                                                                 \
+  X(10, 1) X(10, 2)                                                               \
+  X(12, 1) X(12, 2)
+
+namespace ksig {
+
+struct DpFwdArgs {
+  SeqSet x, y;
+  int d;
+  const int32_t* pairs;  // [n_pairs, 2] (index into x, index into y)
+  int64_t n_pairs;
+  int kind;              // StaticKind
+  double xscale, yscale; // per-side input scaling folded into the static kernel
+  int n_levels, order, difference;
+  const void* gram;      // kind == kGram: [n_pairs, gram_lx, gram_ly]
+  int gram_lx, gram_ly;
+  double* out;           // [n_pairs, n_levels + 1]
+};
+
+struct PdeFwdArgs {
+  SeqSet x, y;
+  int d;
+  const int32_t* pairs;
+  int64_t n_pairs;
+  int kind;
+  double xscale, yscale;
+  int dyadic, difference;
+  const void* gram;
+  int gram_lx, gram_ly;
+  double* out;           // [n_pairs]
+};
+
+int dp_flow_count(int nlev, int order);
+bool dp_supported(int n_levels, int order);
+hipError_t dp_forward_launch(const DpFwdArgs& a, int dtype, size_t lds, hipStream_t s);
+hipError_t pde_forward_launch(const PdeFwdArgs& a, int dtype, size_t lds, hipStream_t s);
+
+}  // namespace ksig

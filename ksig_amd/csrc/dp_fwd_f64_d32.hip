@@ -1,0 +1,14 @@
+// DP forward instantiations: T = double, fused static kernel, d <= 32.
+#include "dp_forward_impl.h"
+
+namespace ksig {
+
+#define KSIG_CASE(L, O) \
+  if (a.n_levels == L && a.order == O) return launch_dp<double, L, O, 32, false>(a, lds, s);
+
+hipError_t dp_fwd_f64_d32(const DpFwdArgs& a, size_t lds, hipStream_t s) {
+  KSIG_DP_LEVEL_ORDER_TABLE(KSIG_CASE)
+  return hipErrorInvalidValue;
+}
+
+}  // namespace ksig

@@ -1,0 +1,224 @@
+// C-ABI entry points of libksig_hip.so (declared in include/ksig_hip.h):
+// argument validation, LDS sizing and dispatch to the gfx950 kernels.
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "ksig_common.h"
+#include "ksig_hip.h"
+#include "dp_forward.h"
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+int fail(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* where) {
+  return fail((int)e, "%s: %s", where, hipGetErrorString(e));
+}
+
+size_t max_lds_bytes() {
+  int dev = 0, v = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 64 * 1024;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+    return 64 * 1024;
+  return (size_t)v;
+}
+
+int dmax_for(int d) { return d <= 8 ? 8 : (d <= 16 ? 16 : 32); }
+
+// Scale factors folded into the inputs: RBF uses exp2(-|x'-y'|^2) with
+// x' = sqrt(log2(e) / max(2 sigma^2, 1e-12)) x (ksig/static/kernels.py:258-260,
+// ksig/utils.py:16); linear uses <scale x, y> (ksig/static/kernels.py:155).
+int static_scales(int kind, double param, double* xs, double* ys) {
+  if (!(param > 0.0))
+    return fail(KSIG_EINVAL, "static kernel parameter must be positive (got %g)", param);
+  if (kind == ksig::kRBF) {
+    double den = 2.0 * param * param;
+    if (den < 1e-12) den = 1e-12;
+    const double s = sqrt(1.4426950408889634 / den);
+    *xs = s;
+    *ys = s;
+    return 0;
+  }
+  if (kind == ksig::kLinear) {
+    *xs = param;
+    *ys = 1.0;
+    return 0;
+  }
+  return fail(KSIG_EINVAL, "unknown static kernel kind %d", kind);
+}
+
+int check_common(int dtype, const void* x, const void* y, int d, const int32_t* pairs,
+                 int64_t n_pairs, double* out) {
+  if (dtype != KSIG_F32 && dtype != KSIG_F64) return fail(KSIG_EINVAL, "bad dtype %d", dtype);
+  if (n_pairs < 0) return fail(KSIG_EINVAL, "n_pairs < 0");
+  if (n_pairs > 0x7fffffffLL) return fail(KSIG_EINVAL, "n_pairs exceeds 2^31-1");
+  if (n_pairs > 0 && (!x || !y || !pairs || !out))
+    return fail(KSIG_EINVAL, "null device pointer");
+  if (d < 1) return fail(KSIG_EINVAL, "d must be >= 1 (got %d)", d);
+  if (d > 32) return fail(KSIG_EUNSUPPORTED, "d = %d > 32 is not built into the fused kernels", d);
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ksig_version(void) { return "ksig_hip 0.1.0 (gfx950)"; }
+
+const char* ksig_last_error(void) { return g_err; }
+
+int ksig_dp_supported(int n_levels, int order) {
+  return ksig::dp_supported(n_levels, order) ? 1 : 0;
+}
+
+static int dp_levels_common(int dtype, ksig::DpFwdArgs& a, int cols, int dmax,
+                            void* stream) {
+  const size_t esz = dtype == KSIG_F32 ? 4 : 8;
+  const int nf = ksig::dp_flow_count(a.n_levels, a.order);
+  const int pc = (a.difference ? cols : cols + 1) > ksig::kWave
+                     ? (a.difference ? cols : cols + 1) : ksig::kWave;
+  const bool fused = a.kind != ksig::kGram;
+  const size_t ycols = fused ? (size_t)cols : 0;
+  const size_t xrows = fused ? 2 * (size_t)ksig::kWave : 0;
+  const size_t lds = (xrows * (dmax + 1) + ycols * (dmax + 1) + (size_t)pc * nf) * esz;
+  if (lds > max_lds_bytes())
+    return fail(KSIG_ELDS, "per-pair LDS state of %zu bytes exceeds the device limit "
+                "(shorter side length %d)", lds, cols);
+  hipError_t e = ksig::dp_forward_launch(a, dtype, lds, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "dp_forward launch");
+  return 0;
+}
+
+int ksig_signature_levels(int dtype, const void* x, const int64_t* x_off,
+                          const int32_t* x_len, int max_len_x, const void* y,
+                          const int64_t* y_off, const int32_t* y_len,
+                          int max_len_y, int d, const int32_t* pairs,
+                          int64_t n_pairs, int static_kind, double static_param,
+                          int n_levels, int order, int difference, double* out,
+                          void* stream) {
+  int rc = check_common(dtype, x, y, d, pairs, n_pairs, out);
+  if (rc) return rc;
+  if (n_levels < 1) return fail(KSIG_EINVAL, "n_levels must be positive");
+  if (order <= 0 || order >= n_levels) order = n_levels;  // ksig/algorithms.py:37
+  if (!ksig::dp_supported(n_levels, order))
+    return fail(KSIG_EUNSUPPORTED, "(n_levels=%d, order=%d) is not built", n_levels, order);
+  ksig::DpFwdArgs a{};
+  rc = static_scales(static_kind, static_param, &a.xscale, &a.yscale);
+  if (rc) return rc;
+  if (n_pairs == 0) return 0;
+  if (max_len_x < 0 || max_len_y < 0) return fail(KSIG_EINVAL, "negative max length");
+  a.x = {x, x_off, x_len};
+  a.y = {y, y_off, y_len};
+  a.d = d;
+  a.pairs = pairs;
+  a.n_pairs = n_pairs;
+  a.kind = static_kind;
+  a.n_levels = n_levels;
+  a.order = order;
+  a.difference = difference ? 1 : 0;
+  a.out = out;
+  const int cols = max_len_x < max_len_y ? max_len_x : max_len_y;
+  return dp_levels_common(dtype, a, cols < 1 ? 1 : cols, dmax_for(d), stream);
+}
+
+int ksig_signature_levels_gram(int dtype, const void* M, int64_t n_mats, int lx,
+                               int ly, int n_levels, int order, int difference,
+                               double* out, void* stream) {
+  if (dtype != KSIG_F32 && dtype != KSIG_F64) return fail(KSIG_EINVAL, "bad dtype %d", dtype);
+  if (n_mats < 0 || n_mats > 0x7fffffffLL) return fail(KSIG_EINVAL, "bad n_mats");
+  if (lx < 0 || ly < 0) return fail(KSIG_EINVAL, "negative matrix shape");
+  if (n_levels < 1) return fail(KSIG_EINVAL, "n_levels must be positive");
+  if (order <= 0 || order >= n_levels) order = n_levels;
+  if (!ksig::dp_supported(n_levels, order))
+    return fail(KSIG_EUNSUPPORTED, "(n_levels=%d, order=%d) is not built", n_levels, order);
+  if (n_mats == 0) return 0;
+  if (!M || !out) return fail(KSIG_EINVAL, "null device pointer");
+  ksig::DpFwdArgs a{};
+  a.kind = ksig::kGram;
+  a.d = 1;
+  a.n_pairs = n_mats;
+  a.n_levels = n_levels;
+  a.order = order;
+  a.difference = difference ? 1 : 0;
+  a.gram = M;
+  a.gram_lx = lx;
+  a.gram_ly = ly;
+  a.out = out;
+  return dp_levels_common(dtype, a, ly < 1 ? 1 : ly, 8, stream);
+}
+
+static int pde_common(int dtype, ksig::PdeFwdArgs& a, int cols, int dmax, void* stream) {
+  const size_t esz = dtype == KSIG_F32 ? 4 : 8;
+  if (a.dyadic < 0 || a.dyadic > 6) return fail(KSIG_EINVAL, "dyadic_order must be in [0, 6]");
+  const int nc = a.difference ? cols - 1 : cols;
+  long pc = (long)(nc + 1) << a.dyadic;
+  if (pc < ksig::kWave) pc = ksig::kWave;
+  const size_t ycols = a.kind == ksig::kGram ? 0 : (size_t)cols;
+  const size_t lds = (ycols * dmax + ycols + (size_t)pc * 3) * esz;
+  if (lds > max_lds_bytes())
+    return fail(KSIG_ELDS, "per-pair LDS state of %zu bytes exceeds the device limit", lds);
+  hipError_t e = ksig::pde_forward_launch(a, dtype, lds, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "pde_forward launch");
+  return 0;
+}
+
+int ksig_signature_pde(int dtype, const void* x, const int64_t* x_off,
+                       const int32_t* x_len, int max_len_x, const void* y,
+                       const int64_t* y_off, const int32_t* y_len,
+                       int max_len_y, int d, const int32_t* pairs,
+                       int64_t n_pairs, int static_kind, double static_param,
+                       int dyadic_order, int difference, double* out,
+                       void* stream) {
+  int rc = check_common(dtype, x, y, d, pairs, n_pairs, out);
+  if (rc) return rc;
+  ksig::PdeFwdArgs a{};
+  rc = static_scales(static_kind, static_param, &a.xscale, &a.yscale);
+  if (rc) return rc;
+  if (n_pairs == 0) return 0;
+  a.x = {x, x_off, x_len};
+  a.y = {y, y_off, y_len};
+  a.d = d;
+  a.pairs = pairs;
+  a.n_pairs = n_pairs;
+  a.kind = static_kind;
+  a.dyadic = dyadic_order;
+  a.difference = difference ? 1 : 0;
+  a.out = out;
+  const int cols = max_len_x < max_len_y ? max_len_x : max_len_y;
+  return pde_common(dtype, a, cols < 1 ? 1 : cols, dmax_for(d), stream);
+}
+
+int ksig_signature_pde_gram(int dtype, const void* M, int64_t n_mats, int lx,
+                            int ly, int dyadic_order, int difference,
+                            double* out, void* stream) {
+  if (dtype != KSIG_F32 && dtype != KSIG_F64) return fail(KSIG_EINVAL, "bad dtype %d", dtype);
+  if (n_mats < 0 || n_mats > 0x7fffffffLL) return fail(KSIG_EINVAL, "bad n_mats");
+  if (lx < 0 || ly < 0) return fail(KSIG_EINVAL, "negative matrix shape");
+  if (n_mats == 0) return 0;
+  if (!M || !out) return fail(KSIG_EINVAL, "null device pointer");
+  ksig::PdeFwdArgs a{};
+  a.kind = ksig::kGram;
+  a.d = 1;
+  a.n_pairs = n_mats;
+  a.dyadic = dyadic_order;
+  a.difference = difference ? 1 : 0;
+  a.gram = M;
+  a.gram_lx = lx;
+  a.gram_ly = ly;
+  a.out = out;
+  return pde_common(dtype, a, ly < 1 ? 1 : ly, 8, stream);
+}
+
+}  // extern "C"

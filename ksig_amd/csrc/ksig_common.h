@@ -1,0 +1,63 @@
+// Shared device helpers for the signature-kernel HIP library (gfx950, wave64).
+//
+// Every hot kernel in this library is a skewed systolic sweep over one
+// sequence pair: lane i of a 64-lane wavefront owns row (64*band + i) of the
+// doubly-differenced Gram grid and walks the columns left to right, one column
+// per step, one step behind lane i-1.  Values that flow down a column (column
+// prefix sums, Gram values of the row above, PDE solution values) move from
+// lane i-1 to lane i with a DPP wave_shr:1 at every step; lane 63's outputs are
+// parked in an LDS FIFO (one slot per column) and read back by lane 0 when it
+// starts the next 64-row band.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ksig {
+
+constexpr int kWave = 64;
+
+enum StaticKind : int { kRBF = 0, kLinear = 1, kGram = 2 };
+
+// ---------------------------------------------------------------------------
+// Lane shift: lane i receives lane i-1's value; lane 0 receives `fill`.
+// DPP wave_shr:1 (dpp_ctrl 0x138) with bound_ctrl off keeps the `old` operand
+// in lane 0, so the FIFO value for lane 0 rides in as `old`.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float shift_down(float v, float fill) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(
+      __float_as_int(fill), __float_as_int(v), 0x138, 0xf, 0xf, false));
+}
+
+__device__ __forceinline__ double shift_down(double v, double fill) {
+  const int2 a = *reinterpret_cast<const int2*>(&v);
+  const int2 f = *reinterpret_cast<const int2*>(&fill);
+  int2 r;
+  r.x = __builtin_amdgcn_update_dpp(f.x, a.x, 0x138, 0xf, 0xf, false);
+  r.y = __builtin_amdgcn_update_dpp(f.y, a.y, 0x138, 0xf, 0xf, false);
+  return *reinterpret_cast<const double*>(&r);
+}
+
+// Full-wave sum in double (used once per pair for the level sums).
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+// v_exp_f32 (about 1 ulp); results below 2^-126 flush to zero, which only
+// affects static-kernel values that are already negligible.
+__device__ __forceinline__ float fast_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
+__device__ __forceinline__ double fast_exp2(double x) { return exp2(x); }
+
+// ---------------------------------------------------------------------------
+// Sequence-set descriptor: packed rows [total_rows, d] plus per-sequence row
+// offsets and lengths (dense [n, L, d] arrays are the case off = a*L, len = L).
+// ---------------------------------------------------------------------------
+struct SeqSet {
+  const void* data;
+  const int64_t* off;
+  const int32_t* len;
+};
+
+}  // namespace ksig

@@ -1,0 +1,203 @@
+"""Signature kernels for sequential data: drop-in counterparts of
+``ksig.kernels.SignatureKernel`` and ``ksig.kernels.SignaturePDEKernel``
+(ksig/kernels.py:21-256) computed by the gfx950 HIP library.
+
+With an ``RBFKernel`` or ``LinearKernel`` static kernel the static Gram, its
+double difference and the recursion are fused into one sweep per sequence
+pair (nothing of size L x L is materialised).  Any other ``StaticKernel`` is
+evaluated as in the reference (``static_kernel(X', Y', return_on_gpu=True)``)
+and its tensor is fed to the same sweep.
+"""
+
+from __future__ import annotations
+
+import warnings
+from abc import ABCMeta
+from typing import Optional
+
+import torch
+
+from . import _ops, algorithms, static, utils
+from .static.kernels import Kernel, StaticKernel
+from .utils import _EPS
+
+
+class SignatureBase(Kernel, metaclass=ABCMeta):
+  """Base class for signature kernels (ksig/kernels.py:25-90)."""
+
+  def __init__(self, n_levels: int = 5, order: int = 1, difference: bool = True,
+               normalize: bool = True, n_features: Optional[int] = None):
+    self.n_levels = utils.check_positive_value(n_levels, 'n_levels')
+    self.order = (self.n_levels if order <= 0 or order >= self.n_levels
+                  else order)
+    self.normalize = normalize
+    self.difference = difference
+    self.n_features = (utils.check_positive_value(n_features, 'n_features')
+                       if n_features is not None else None)
+
+  def _validate_data(self, X, reset: Optional[bool] = False):
+    """ksig/kernels.py:56-90.  For ndim == 2 inputs without ``n_features`` the
+    reference intends a warning (its malformed ``warnings.warn`` call raises
+    TypeError instead); this build emits the warning and proceeds with
+    univariate sequences."""
+    n_features = (self.n_features_ if hasattr(self, 'n_features_')
+                  and self.n_features_ is not None else self.n_features)
+    if X.ndim == 2:
+      if n_features is None or reset:
+        warnings.warn(
+          '`X` has ndim==2. Assuming inputs are univariate time series. '
+          'Recommend passing an `n_features` parameter during init when using '
+          'flattened arrays of ndim==2.')
+        n_features = 1
+    elif X.ndim == 3:
+      if n_features is None or reset:
+        n_features = X.shape[-1]
+      elif X.shape[-1] != n_features:
+        raise ValueError(
+          'Number of features in `X` does not match saved `n_features` param.')
+    else:
+      raise ValueError(
+        'Only input sequence arrays with ndim==2 or ndim==3 are supported.')
+    X = X.reshape([X.shape[0], -1, n_features])
+    if reset:
+      self.n_features_ = n_features
+    return X
+
+
+def _normalise(K: torch.Tensor, Kxd: torch.Tensor, Kyd: torch.Tensor) -> torch.Tensor:
+  """ksig/kernels.py:194-207 with ksig/utils.py:173-182."""
+  sx = torch.clamp(utils.robust_sqrt(Kxd), min=_EPS)
+  sy = torch.clamp(utils.robust_sqrt(Kyd), min=_EPS)
+  return K / (sx[..., :, None] * sy[..., None, :])
+
+
+class SignatureKernel(SignatureBase):
+  """Full-rank signature kernel (ksig/kernels.py:97-214)."""
+
+  def __init__(self, n_levels: int = 5, order: int = 1, difference: bool = True,
+               normalize: bool = True, n_features: Optional[int] = None,
+               static_kernel: Optional[StaticKernel] = None):
+    super().__init__(n_levels=n_levels, order=order, difference=difference,
+                     normalize=normalize, n_features=n_features)
+    self.static_kernel = static_kernel or static.kernels.RBFKernel()
+
+  # -- embedding (generic static kernels only) --------------------------------
+  def _compute_embedding(self, X, Y=None, diag: bool = False) -> torch.Tensor:
+    """ksig/kernels.py:124-145: pairwise static-kernel evaluations."""
+    if diag:
+      return self.static_kernel(X[..., None, :], return_on_gpu=True)
+    if Y is None:
+      return self.static_kernel(X[:, None, :, None, :], X[None, :, :, None, :],
+                                return_on_gpu=True)
+    return self.static_kernel(X[:, None, :, None, :], Y[None, :, :, None, :],
+                              return_on_gpu=True)
+
+  # -- level values ------------------------------------------------------------
+  def _levels(self, X, Y=None, diag: bool = False) -> torch.Tensor:
+    """Per-level kernel values [n_levels+1, n_X, n_Y] (or [n_levels+1, n]
+    for ``diag``) in float64."""
+    code = _ops.static_code(self.static_kernel)
+    if code is None:
+      M = self._compute_embedding(X, Y, diag=diag)
+      return algorithms.signature_kern(
+          M, self.n_levels, order=self.order, difference=self.difference,
+          return_levels=True).to(torch.float64)
+    kind, param = code
+    dev = X.device
+    Xs = _ops.SeqSet.dense(X)
+    if diag:
+      pairs = _ops.diag_pairs(X.shape[0], dev)
+      lev = _ops.signature_levels(Xs, Xs, pairs, self.n_levels, self.order,
+                                  self.difference, kind, param)
+      return lev.t()
+    if Y is None:
+      n = X.shape[0]
+      pairs = _ops.triu_pairs(n, dev)
+      lev = _ops.signature_levels(Xs, Xs, pairs, self.n_levels, self.order,
+                                  self.difference, kind, param)
+      K = torch.empty((self.n_levels + 1, n, n), device=dev, dtype=torch.float64)
+      i, j = pairs[:, 0].long(), pairs[:, 1].long()
+      K[:, i, j] = lev.t()
+      K[:, j, i] = lev.t()
+      return K
+    Ys = _ops.SeqSet.dense(Y)
+    pairs = _ops.dense_pairs(X.shape[0], Y.shape[0], dev)
+    lev = _ops.signature_levels(Xs, Ys, pairs, self.n_levels, self.order,
+                                self.difference, kind, param)
+    return lev.t().reshape(self.n_levels + 1, X.shape[0], Y.shape[0])
+
+  def _compute_kernel(self, X, Y=None, diag: bool = False) -> torch.Tensor:
+    """ksig/kernels.py:147-164: levels if ``normalize`` else their sum."""
+    lev = self._levels(X, Y, diag=diag)
+    return lev if self.normalize else lev.sum(0)
+
+  def _Kdiag(self, X) -> torch.Tensor:
+    """ksig/kernels.py:166-178."""
+    if self.normalize:
+      return torch.full((X.shape[0],), 1., device=X.device, dtype=torch.float64)
+    return self._compute_kernel(X, diag=True).to(X.dtype)
+
+  def _K(self, X, Y=None) -> torch.Tensor:
+    """ksig/kernels.py:180-214: normalise each level by the diagonal and
+    average over levels (including the constant level 0)."""
+    K = self._compute_kernel(X, Y)
+    if self.normalize:
+      if Y is None:
+        Kd = utils.matrix_diag(K)
+        K = _normalise(K, Kd, Kd)
+      else:
+        K = _normalise(K, self._compute_kernel(X, diag=True),
+                       self._compute_kernel(Y, diag=True))
+    if K.ndim == 3:
+      K = torch.mean(K, dim=0)
+    return K.to(X.dtype)
+
+
+class SignaturePDEKernel(SignatureKernel):
+  """Signature-PDE kernel (ksig/kernels.py:218-256).
+
+  ``dyadic_order`` has no reference counterpart: it refines every ΔΔ cell
+  into 2^λ x 2^λ cells carrying M / 4^λ; the default 0 is the reference."""
+
+  def __init__(self, difference: bool = True, normalize: bool = True,
+               n_features: Optional[int] = None,
+               static_kernel: Optional[StaticKernel] = None,
+               dyadic_order: int = 0):
+    self.difference = difference
+    self.normalize = normalize
+    self.n_features = (utils.check_positive_value(n_features, 'n_features')
+                       if n_features is not None else None)
+    self.static_kernel = static_kernel or static.kernels.RBFKernel()
+    if dyadic_order < 0:
+      raise ValueError('The parameter \'dyadic_order\' should be non-negative.')
+    self.dyadic_order = dyadic_order
+
+  def _compute_kernel(self, X, Y=None, diag: bool = False) -> torch.Tensor:
+    """ksig/kernels.py:242-256 (float64 values)."""
+    code = _ops.static_code(self.static_kernel)
+    if code is None:
+      M = self._compute_embedding(X, Y, diag=diag)
+      return algorithms.signature_kern_pde(
+          M, difference=self.difference,
+          dyadic_order=self.dyadic_order).to(torch.float64)
+    kind, param = code
+    dev = X.device
+    Xs = _ops.SeqSet.dense(X)
+    if diag:
+      return _ops.signature_pde(Xs, Xs, _ops.diag_pairs(X.shape[0], dev),
+                                self.difference, kind, param, self.dyadic_order)
+    if Y is None:
+      n = X.shape[0]
+      pairs = _ops.triu_pairs(n, dev)
+      v = _ops.signature_pde(Xs, Xs, pairs, self.difference, kind, param,
+                             self.dyadic_order)
+      K = torch.empty((n, n), device=dev, dtype=torch.float64)
+      i, j = pairs[:, 0].long(), pairs[:, 1].long()
+      K[i, j] = v
+      K[j, i] = v
+      return K
+    Ys = _ops.SeqSet.dense(Y)
+    pairs = _ops.dense_pairs(X.shape[0], Y.shape[0], dev)
+    v = _ops.signature_pde(Xs, Ys, pairs, self.difference, kind, param,
+                           self.dyadic_order)
+    return v.reshape(X.shape[0], Y.shape[0])

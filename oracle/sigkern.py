@@ -1,0 +1,414 @@
+"""FP64 CPU oracle for the signature-kernel hot path.
+
+TEST INFRASTRUCTURE ONLY.  This module is the checker used by ``tests/``,
+``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py``.
+The product path (``ksig_amd``) never imports it; it runs the HIP library.
+
+It restates, in NumPy float64 (or any dtype given), the reference algorithms
+of tgcsaba/KSig v1.0.0 with the same dataflow (materialise the static Gram,
+difference it, per-level exclusive 2-D cumulative sums):
+
+* static kernels ............ ksig/static/kernels.py:145-155 (Linear),
+                              :248-260 (RBF), ksig/utils.py:125-157
+* exclusive multi-cumsum .... ksig/utils.py:63-90
+* DP signature kernel ....... ksig/algorithms.py:22-44 (dispatch),
+                              :47-85 (first order), :88-140 (higher order)
+* signature-PDE kernel ...... ksig/algorithms.py:344-399 (Goursat update),
+                              :402-446 (launcher), written as the plain 2-D
+                              recurrence the antidiagonal sweep evaluates
+* normalisation / level mean  ksig/kernels.py:166-214, ksig/utils.py:16,93-102,173-182
+* class-level embedding ..... ksig/kernels.py:124-164,242-256
+
+Additions that have no reference counterpart (build extensions):
+``dyadic_order`` for the PDE (each ΔΔ cell refined into 2^λ×2^λ sub-cells
+carrying M/4^λ), and the closed forms used to pin the restatement
+(Chen signatures, iterated sums, chain enumeration, Bessel I0).
+
+Parity pins (all in tests/test_oracle.py): DP vs Chen-signature inner
+products (Linear kernel, full order), DP order 1 vs truncated iterated-sum
+inner products, DP (any order, RBF) vs brute-force chain enumeration,
+PDE one-cell closed form, PDE dyadic refinement vs I0(2*sqrt(ab)),
+stuttering / translation invariance, symmetry and the normalised diagonal.
+"""
+
+from __future__ import annotations
+
+import itertools
+import math
+from typing import Optional
+
+import numpy as np
+
+EPS = 1e-12  # ksig/utils.py:16 (_EPS)
+
+
+# -----------------------------------------------------------------------------
+# Static kernels (ksig/static/kernels.py, ksig/utils.py)
+# -----------------------------------------------------------------------------
+
+def squared_norm(X: np.ndarray, axis: int = -1) -> np.ndarray:
+  """ksig/utils.py:125-135."""
+  return np.sum(np.square(X), axis=axis)
+
+
+def squared_euclid_dist(X: np.ndarray, Y: Optional[np.ndarray] = None
+                        ) -> np.ndarray:
+  """ksig/utils.py:138-157: ||x||^2 + ||y||^2 - 2<x, y>, no clamp."""
+  Xn2 = squared_norm(X)
+  if Y is None:
+    return Xn2[..., :, None] + Xn2[..., None, :] - 2 * X @ np.swapaxes(X, -1, -2)
+  Yn2 = squared_norm(Y)
+  return Xn2[..., :, None] + Yn2[..., None, :] - 2 * X @ np.swapaxes(Y, -1, -2)
+
+
+def rbf_gram(X: np.ndarray, Y: Optional[np.ndarray] = None,
+             bandwidth: float = 1.0) -> np.ndarray:
+  """ksig/static/kernels.py:248-260: exp(-D2 / max(2 sigma^2, EPS))."""
+  return np.exp(-squared_euclid_dist(X, Y) / max(2 * bandwidth**2, EPS))
+
+
+def linear_gram(X: np.ndarray, Y: Optional[np.ndarray] = None,
+                scale: float = 1.0) -> np.ndarray:
+  """ksig/static/kernels.py:145-155: scale * X Y^T."""
+  Y = X if Y is None else Y
+  return scale * (X @ np.swapaxes(Y, -1, -2))
+
+
+def static_gram(X, Y, kind: str, param: float):
+  if kind == 'rbf':
+    return rbf_gram(X, Y, param)
+  if kind == 'linear':
+    return linear_gram(X, Y, param)
+  raise ValueError(kind)
+
+
+def pairwise_gram(X: np.ndarray, Y: Optional[np.ndarray], kind: str,
+                  param: float, diag: bool = False) -> np.ndarray:
+  """Embedding of ksig/kernels.py:124-145.
+
+  Returns M of shape [nX, lX, lX] if ``diag`` else [nX, nY, lX, lY]."""
+  if diag:
+    return static_gram(X, X, kind, param)
+  Y = X if Y is None else Y
+  return static_gram(X[:, None], Y[None, :], kind, param)
+
+
+# -----------------------------------------------------------------------------
+# Exclusive cumulative sums (ksig/utils.py:63-90)
+# -----------------------------------------------------------------------------
+
+def multi_cumsum(M: np.ndarray, exclusive: bool = False, axis=-1) -> np.ndarray:
+  """Cumulative sum over one or several axes; ``exclusive`` drops the last
+  entry along each axis and pre-pads with a zero (ksig/utils.py:63-90)."""
+  ndim = M.ndim
+  axes = [axis] if np.isscalar(axis) else list(axis)
+  axes = [ndim + a if a < 0 else a for a in axes]
+  if exclusive:
+    M = M[tuple(slice(-1) if a in axes else slice(None) for a in range(ndim))]
+  for a in axes:
+    M = np.cumsum(M, axis=a)
+  if exclusive:
+    M = np.pad(M, [(1, 0) if a in axes else (0, 0) for a in range(ndim)])
+  return M
+
+
+def double_diff(M: np.ndarray) -> np.ndarray:
+  """ΔΔ of ksig/algorithms.py:64 (diff over axis -2, then -1)."""
+  return np.diff(np.diff(M, axis=-2), axis=-1)
+
+
+# -----------------------------------------------------------------------------
+# DP signature kernel (ksig/algorithms.py:22-140)
+# -----------------------------------------------------------------------------
+
+def normalize_order(n_levels: int, order: int) -> int:
+  """ksig/algorithms.py:37 and ksig/kernels.py:49-50."""
+  return n_levels if order <= 0 or order >= n_levels else order
+
+
+def signature_kern(M: np.ndarray, n_levels: int, order: int = -1,
+                   difference: bool = True, return_levels: bool = False
+                   ) -> np.ndarray:
+  """ksig/algorithms.py:22-44."""
+  order = normalize_order(n_levels, order)
+  if order == 1:
+    return signature_kern_first_order(M, n_levels, difference, return_levels)
+  return signature_kern_higher_order(M, n_levels, order, difference,
+                                     return_levels)
+
+
+def signature_kern_first_order(M, n_levels, difference=True,
+                               return_levels=False):
+  """ksig/algorithms.py:47-85.
+
+  K_0 = 1, R_1 = M, R_m = M * E(R_{m-1}), K_m = sum(R_m), with E the
+  exclusive 2-D cumulative sum over the two time axes."""
+  if difference:
+    M = double_diff(M)
+  lead = M.shape[:-2]
+  levels = [np.ones(lead, dtype=M.dtype), np.sum(M, axis=(-2, -1))]
+  R = M.copy()
+  for _ in range(1, n_levels):
+    R = M * multi_cumsum(R, exclusive=True, axis=(-2, -1))
+    levels.append(np.sum(R, axis=(-2, -1)))
+  if return_levels:
+    return np.stack(levels, axis=0)
+  return sum(levels[1:], levels[0].copy())
+
+
+def signature_kern_higher_order(M, n_levels, order, difference=True,
+                                return_levels=False):
+  """ksig/algorithms.py:88-140.
+
+  State R[r, s] with r (s) the current repeat count of the X (Y) time index;
+  repeats carry 1/(r+1) weights, i.e. 1/l! per run of length l."""
+  if difference:
+    M = double_diff(M)
+  lead = M.shape[:-2]
+  levels = [np.ones(lead, dtype=M.dtype), np.sum(M, axis=(-2, -1))]
+  R = M.copy()[None, None]
+  for i in range(1, n_levels):
+    d = min(i + 1, order)
+    Rn = np.empty((d, d) + M.shape, dtype=M.dtype)
+    Rn[0, 0] = M * multi_cumsum(np.sum(R, axis=(0, 1)), exclusive=True,
+                                axis=(-2, -1))
+    for r in range(1, d):
+      Rn[0, r] = 1. / (r + 1) * M * multi_cumsum(
+        np.sum(R[:, r - 1], axis=0), exclusive=True, axis=-2)
+      Rn[r, 0] = 1. / (r + 1) * M * multi_cumsum(
+        np.sum(R[r - 1, :], axis=0), exclusive=True, axis=-1)
+      for s in range(1, d):
+        Rn[r, s] = 1. / ((r + 1) * (s + 1)) * M * R[r - 1, s - 1]
+    R = Rn
+    levels.append(np.sum(R, axis=(0, 1, -2, -1)))
+  if return_levels:
+    return np.stack(levels, axis=0)
+  return sum(levels[1:], levels[0].copy())
+
+
+# -----------------------------------------------------------------------------
+# Signature-PDE kernel (ksig/algorithms.py:344-446)
+# -----------------------------------------------------------------------------
+
+def refine_dyadic(M: np.ndarray, dyadic_order: int) -> np.ndarray:
+  """Build extension (no reference counterpart): each cell of the ΔΔ grid
+  becomes a 2^λ × 2^λ block of cells carrying M / 4^λ."""
+  if dyadic_order == 0:
+    return M
+  f = 1 << dyadic_order
+  Mr = np.repeat(np.repeat(M, f, axis=-2), f, axis=-1)
+  return Mr / float(f * f)
+
+
+def goursat_solve(M: np.ndarray) -> np.ndarray:
+  """Explicit Goursat scheme of ksig/algorithms.py:380-385 as a 2-D recurrence:
+  U(i,j) = (U(i-1,j) + U(i,j-1)) (1 + M/2 + M^2/12) - U(i-1,j-1) (1 - M^2/12)
+  with U = 1 on the boundary (i = -1 or j = -1); returns U(lX-1, lY-1).
+
+  Vectorised over the leading axes; loops over rows, sequential over columns
+  inside each row (the antidiagonal order of the reference evaluates the same
+  expressions on the same operands)."""
+  lead = M.shape[:-2]
+  lx, ly = M.shape[-2:]
+  A = 1. + 0.5 * M + M * M / 12.
+  B = 1. - M * M / 12.
+  prev = np.ones(lead + (ly + 1,), dtype=M.dtype)  # row i-1, index 0 = j=-1
+  for i in range(lx):
+    cur = np.ones(lead + (ly + 1,), dtype=M.dtype)
+    for j in range(ly):
+      cur[..., j + 1] = ((prev[..., j + 1] + cur[..., j]) * A[..., i, j]
+                         - prev[..., j] * B[..., i, j])
+    prev = cur
+  return prev[..., ly]
+
+
+def signature_kern_pde(M: np.ndarray, difference: bool = True,
+                       dyadic_order: int = 0) -> np.ndarray:
+  """ksig/algorithms.py:402-446 (plus the dyadic extension)."""
+  if M.ndim not in (3, 4):
+    raise ValueError('The `M` matrix must have `.ndim==3` or `.ndim==4`.')
+  if difference:
+    M = double_diff(M)
+  if M.shape[-1] == 0 or M.shape[-2] == 0:
+    return np.ones(M.shape[:-2], dtype=M.dtype)
+  return goursat_solve(refine_dyadic(M, dyadic_order))
+
+
+# -----------------------------------------------------------------------------
+# Class-level semantics (ksig/kernels.py:124-256)
+# -----------------------------------------------------------------------------
+
+def _normalise(K, Kxd, Kyd):
+  """ksig/kernels.py:191-207 with ksig/utils.py:173-182."""
+  sx = np.maximum(np.sqrt(np.maximum(Kxd, EPS)), EPS)
+  sy = np.maximum(np.sqrt(np.maximum(Kyd, EPS)), EPS)
+  return K / (sx[..., :, None] * sy[..., None, :])
+
+
+def signature_kernel(X, Y=None, diag=False, n_levels=5, order=1,
+                     difference=True, normalize=True, kind='rbf', param=1.0):
+  """SignatureKernel(...)(X, Y, diag) of ksig/kernels.py:97-214."""
+  order = normalize_order(n_levels, order)
+  if diag:
+    if normalize:
+      return np.full((X.shape[0],), 1.)  # ksig/kernels.py:175-176
+    M = pairwise_gram(X, None, kind, param, diag=True)
+    return signature_kern(M, n_levels, order, difference, False)
+  M = pairwise_gram(X, Y, kind, param)
+  K = signature_kern(M, n_levels, order, difference, return_levels=normalize)
+  if normalize:
+    if Y is None:
+      Kd = np.einsum('...ii->...i', K)
+      K = _normalise(K, Kd, Kd)
+    else:
+      Kxd = signature_kern(pairwise_gram(X, None, kind, param, diag=True),
+                           n_levels, order, difference, True)
+      Kyd = signature_kern(pairwise_gram(Y, None, kind, param, diag=True),
+                           n_levels, order, difference, True)
+      K = _normalise(K, Kxd, Kyd)
+    K = np.mean(K, axis=0)
+  return K
+
+
+def signature_pde_kernel(X, Y=None, diag=False, difference=True,
+                         normalize=True, kind='rbf', param=1.0,
+                         dyadic_order=0):
+  """SignaturePDEKernel(...)(X, Y, diag) of ksig/kernels.py:218-256."""
+  if diag:
+    if normalize:
+      return np.full((X.shape[0],), 1.)
+    M = pairwise_gram(X, None, kind, param, diag=True)
+    return signature_kern_pde(M, difference, dyadic_order)
+  M = pairwise_gram(X, Y, kind, param)
+  K = signature_kern_pde(M, difference, dyadic_order)
+  if normalize:
+    if Y is None:
+      Kd = np.einsum('...ii->...i', K)
+      K = _normalise(K, Kd, Kd)
+    else:
+      Kxd = signature_kern_pde(pairwise_gram(X, None, kind, param, True),
+                               difference, dyadic_order)
+      Kyd = signature_kern_pde(pairwise_gram(Y, None, kind, param, True),
+                               difference, dyadic_order)
+      K = _normalise(K, Kxd, Kyd)
+  return K
+
+
+# -----------------------------------------------------------------------------
+# Per-pair helpers (ragged batches: each pair at its native lengths)
+# -----------------------------------------------------------------------------
+
+def pair_levels(x, y, n_levels, order=1, difference=True, kind='rbf',
+                param=1.0):
+  """Level values K_0..K_n of one pair (x [lx, d], y [ly, d])."""
+  M = static_gram(x, y, kind, param)
+  return signature_kern(M, n_levels, order, difference, return_levels=True)
+
+
+def pair_pde(x, y, difference=True, kind='rbf', param=1.0, dyadic_order=0):
+  M = static_gram(x, y, kind, param)
+  return signature_kern_pde(M[None], difference, dyadic_order)[0]
+
+
+# -----------------------------------------------------------------------------
+# MMD^2 (build extension; unbiased estimator)
+# -----------------------------------------------------------------------------
+
+def mmd2_unbiased(Kxx, Kyy, Kxy):
+  """mean_{i!=j} Kxx + mean_{i!=j} Kyy - 2 mean Kxy."""
+  n, m = Kxx.shape[0], Kyy.shape[0]
+  sxx = (np.sum(Kxx) - np.trace(Kxx)) / (n * (n - 1))
+  syy = (np.sum(Kyy) - np.trace(Kyy)) / (m * (m - 1))
+  return sxx + syy - 2. * np.mean(Kxy)
+
+
+# -----------------------------------------------------------------------------
+# Closed forms and brute force used to pin the restatement
+# -----------------------------------------------------------------------------
+
+def _tensor_exp(v: np.ndarray, depth: int):
+  """Truncated tensor exponential of a vector: [1, v, v⊗v/2!, ...]."""
+  out = [np.ones(())]
+  for m in range(1, depth + 1):
+    out.append(np.multiply.outer(out[-1], v) / m)
+  return out
+
+
+def _tensor_mul(a, b, depth):
+  out = []
+  for m in range(depth + 1):
+    acc = np.zeros(a[m].shape if m else ())
+    for k in range(m + 1):
+      acc = acc + np.multiply.outer(a[k], b[m - k])
+    out.append(acc)
+  return out
+
+
+def chen_signature(x: np.ndarray, depth: int):
+  """Signature of the piecewise-linear path through the rows of x, truncated
+  at ``depth``, via Chen's identity (product of segment exponentials)."""
+  d = x.shape[1]
+  sig = [np.ones(())] + [np.zeros((d,) * m) for m in range(1, depth + 1)]
+  for inc in np.diff(x, axis=0):
+    sig = _tensor_mul(sig, _tensor_exp(inc, depth), depth)
+  return sig
+
+
+def iterated_sums(x: np.ndarray, depth: int):
+  """Discrete iterated sums S_m = sum_{i1<...<im} dx_i1 ⊗ ... ⊗ dx_im."""
+  dx = np.diff(x, axis=0)
+  d = x.shape[1]
+  S = [np.ones(())] + [np.zeros((d,) * m) for m in range(1, depth + 1)]
+  for inc in dx:
+    for m in range(depth, 0, -1):
+      S[m] = S[m] + np.multiply.outer(S[m - 1], inc)
+  return S
+
+
+def brute_force_levels(M: np.ndarray, n_levels: int, order: int):
+  """K_m by explicit enumeration of index chains (tiny grids only).
+
+  K_m = sum over non-decreasing i_1<=...<=i_m and j_1<=...<=j_m whose runs of
+  equal indices have length <= order, of prod_k M[i_k, j_k] times
+  prod over runs of 1/(run length)! on each side.  Independent of the DP."""
+  lx, ly = M.shape
+  order = normalize_order(n_levels, order)
+
+  def sequences(n, m):
+    out = []
+    for seq in itertools.combinations_with_replacement(range(n), m):
+      runs = [len(list(g)) for _, g in itertools.groupby(seq)]
+      if max(runs) <= order:
+        w = 1.
+        for r in runs:
+          w /= math.factorial(r)
+        out.append((seq, w))
+    return out
+
+  K = [1.]
+  for m in range(1, n_levels + 1):
+    tot = 0.
+    sx, sy = sequences(lx, m), sequences(ly, m)
+    for (ix, wx) in sx:
+      for (jy, wy) in sy:
+        tot += wx * wy * np.prod(M[list(ix), list(jy)])
+    K.append(tot)
+  return np.array(K)
+
+
+def bessel_i0(z: float, terms: int = 60) -> float:
+  """I_0(z) = sum_k (z/2)^{2k} / (k!)^2."""
+  return float(sum((z / 2.) ** (2 * k) / math.factorial(k) ** 2
+                   for k in range(terms)))
+
+
+# -----------------------------------------------------------------------------
+# Synthetic inputs shared by tests and the benchmark
+# -----------------------------------------------------------------------------
+
+def brownian(rng: np.random.Generator, n: int, L: int, d: int,
+             dtype=np.float64) -> np.ndarray:
+  """n paths: cumsum over L steps of i.i.d. N(0, 1/L) increments per channel
+  (the first point equals the first increment)."""
+  inc = rng.standard_normal((n, L, d)) / math.sqrt(L)
+  return np.cumsum(inc, axis=1).astype(dtype)
