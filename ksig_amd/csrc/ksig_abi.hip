@@ -166,7 +166,7 @@ static int pde_common(int dtype, ksig::PdeFwdArgs& a, int cols, int dmax, void* 
   long pc = (long)(nc + 1) << a.dyadic;
   if (pc < ksig::kWave) pc = ksig::kWave;
   const size_t ycols = a.kind == ksig::kGram ? 0 : (size_t)cols;
-  const size_t lds = (ycols * dmax + ycols + (size_t)pc * 3) * esz;
+  const size_t lds = (ycols * dmax + ycols + (size_t)pc * 4) * esz;
   if (lds > max_lds_bytes())
     return fail(KSIG_ELDS, "per-pair LDS state of %zu bytes exceeds the device limit", lds);
   hipError_t e = ksig::pde_forward_launch(a, dtype, lds, (hipStream_t)stream);

@@ -10,6 +10,14 @@
 // from it through DPP wave_shr:1 (lane 0 from an LDS FIFO per column), so a
 // step costs no barrier at all.
 //
+// Precision: the update is evaluated in increment form.  With
+// V(i,j) = U(i,j) - U(i,j-1), the scheme reads
+//   V(i,j) = V(i-1,j) + δ(i,j),   U(i,j) = U(i,j-1) + V(i,j),
+//   δ = (U(i-1,j) + U(i,j-1)) (M/2 + M^2/12) + U(i-1,j-1) M^2/12,
+// which is algebraically identical but never subtracts two O(U) terms, so
+// float32 arithmetic keeps ~1e-7 relative accuracy where the direct form
+// loses ~1e-4 (V flows down the column like the DP's column sums).
+//
 // Dyadic refinement (no reference counterpart; dyadic_order = 0 is the
 // reference scheme): each ΔΔ cell becomes a 2^λ × 2^λ block of cells carrying
 // M / 4^λ.  The first refined row of each original row ("leader" lane)
@@ -26,7 +34,7 @@ namespace ksig {
 template <typename T, int DMAX>
 __global__ void __launch_bounds__(kWave)
 pde_forward_kernel(const PdeFwdArgs a) {
-  constexpr int NF = 3;  // flow: [G(r, c+1), G(r+1, c+1), U]
+  constexpr int NF = 4;  // flow: [G(r, c+1), G(r+1, c+1), U, V]
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* smem = reinterpret_cast<T*>(smem_raw);
 
@@ -129,6 +137,7 @@ pde_forward_kernel(const PdeFwdArgs a) {
       fifo[s * NF + 0] = T(0);
       fifo[s * NF + 1] = g;
       fifo[s * NF + 2] = T(1);
+      fifo[s * NF + 3] = T(0);
     }
   }
   __syncthreads();
@@ -143,7 +152,7 @@ pde_forward_kernel(const PdeFwdArgs a) {
   T xv2[DMAX], xnrm2;            // next band's row, prefetched
   load_x(xv2, xnrm2, ((rr + kWave) >> lam) + rshift);
 
-  T fl[NF] = {T(0), T(0), T(1)};
+  T fl[NF] = {T(0), T(0), T(1), T(0)};
   T gt_old = T(0), gb_old = T(0), gt = T(0), gb = T(0);
   T M = T(0);
   T uleft = T(1), udiag = T(1);
@@ -152,16 +161,18 @@ pde_forward_kernel(const PdeFwdArgs a) {
   const int total = nb * Pc + kWave - 1;
   for (int step = 0; step < total; ++step) {
     const bool active = cc >= -F && band < nb;
-    T fin0 = T(0), fin1 = T(0), fin2 = T(1);
+    T fin0 = T(0), fin1 = T(0), fin2 = T(1), fin3 = T(0);
     if (lane == 0 && active) {
       const int s = cc + F;
       fin0 = fifo[s * NF + 0];
       fin1 = fifo[s * NF + 1];
       fin2 = fifo[s * NF + 2];
+      fin3 = fifo[s * NF + 3];
     }
     const T in_gt = shift_down(fl[0], fin0);
     const T in_gb = shift_down(fl[1], fin1);
     const T in_u = shift_down(fl[2], fin2);
+    const T in_v = shift_down(fl[3], fin3);
 
     const bool newcol = (cc & (F - 1)) == 0;  // first refined step of a column
     const int ocol = cc >> lam;               // original column (-1 = priming)
@@ -180,11 +191,13 @@ pde_forward_kernel(const PdeFwdArgs a) {
     gb = ngb;
 
     const bool cell = active && cc >= 0 && cc < Ncr && rr < Nrr;
-    const T M2 = M * M;
-    const T A = T(1) + T(0.5) * M + M2 * T(1.0 / 12.0);
-    const T B = T(1) - M2 * T(1.0 / 12.0);
-    T u = (in_u + uleft) * A - udiag * B;
+    const T b = M * M * T(1.0 / 12.0);
+    const T aa = T(0.5) * M + b;
+    const T delta = (in_u + uleft) * aa + udiag * b;
+    T v = in_v + delta;
+    T u = uleft + v;
     u = cell ? u : T(1);  // priming / padding columns carry the boundary value
+    v = cell ? v : in_v;
     if (cell && rr == Nrr - 1 && cc == Ncr - 1) a.out[pr] = (double)u;
     uleft = u;
     udiag = in_u;
@@ -192,11 +205,13 @@ pde_forward_kernel(const PdeFwdArgs a) {
     fl[0] = gt;
     fl[1] = gb;
     fl[2] = u;
+    fl[3] = v;
     if (lane == kWave - 1 && active) {
       const int s = cc + F;
       fifo[s * NF + 0] = gt;
       fifo[s * NF + 1] = gb;
       fifo[s * NF + 2] = u;
+      fifo[s * NF + 3] = v;
     }
 
     ++cc;
