@@ -68,6 +68,43 @@ int ksig_signature_levels(int dtype, const void* x, const int64_t* x_off,
                           void* stream);
 
 /*
+ * Training forward: ksig_signature_levels for order 1 with difference, which
+ * also writes the boundary state the adjoint needs.  For pair p, save holds
+ * (n_levels-1) * (len_x + len_y - 2) values of dtype at element offset
+ * save_off[p] (row-end prefixes of the recursion and its final column sums).
+ * Same reference interface as ksig_signature_levels; the state itself has no
+ * reference counterpart (the reference has no gradients).
+ */
+int ksig_signature_levels_fwd_save(int dtype, const void* x, const int64_t* x_off,
+                                   const int32_t* x_len, int max_len_x, const void* y,
+                                   const int64_t* y_off, const int32_t* y_len,
+                                   int max_len_y, int d, const int32_t* pairs,
+                                   int64_t n_pairs, int static_kind, double static_param,
+                                   int n_levels, double* out, void* save,
+                                   const int64_t* save_off, void* stream);
+
+/*
+ * Adjoint of ksig_signature_levels_fwd_save (no reference counterpart: the
+ * reference, ksig/algorithms.py:47-85, is forward-only).  Given
+ * grad_levels[p, m] = dL/dK_m (m = 0..n_levels), writes for every pair
+ *   omega[omega_off[p] + a*len_y + b] = dL/dG(a, b) * dG/dz
+ * over the static-kernel Gram G(a, b) = k(x_a, y_b) (times G for RBF, plain
+ * dL/dG for linear), and grad_param[p] = sum omega * D'(a,b) (RBF,
+ * D' = log2(e) |x_a - y_b|^2 / max(2 sigma^2, 1e-12)) or sum omega * G (linear).
+ * The caller contracts omega with the inputs to obtain dL/dx, dL/dy and the
+ * bandwidth / scale gradient.
+ */
+int ksig_signature_levels_bwd(int dtype, const void* x, const int64_t* x_off,
+                              const int32_t* x_len, int max_len_x, const void* y,
+                              const int64_t* y_off, const int32_t* y_len,
+                              int max_len_y, int d, const int32_t* pairs,
+                              int64_t n_pairs, int static_kind, double static_param,
+                              int n_levels, const double* grad_levels, const void* save,
+                              const int64_t* save_off, void* omega,
+                              const int64_t* omega_off, double* grad_param,
+                              void* stream);
+
+/*
  * Truncated signature kernel on caller-supplied static-kernel matrices.
  *
  * Replaces signature_kern(M, n_levels, order, difference, return_levels=True)

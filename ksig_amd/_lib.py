@@ -28,6 +28,12 @@ SIGNATURES = {
     "ksig_signature_levels": (_c_int, [
         _c_int, _vp, _vp, _vp, _c_int, _vp, _vp, _vp, _c_int, _c_int, _vp,
         _c_i64, _c_int, _c_dbl, _c_int, _c_int, _c_int, _vp, _vp]),
+    "ksig_signature_levels_fwd_save": (_c_int, [
+        _c_int, _vp, _vp, _vp, _c_int, _vp, _vp, _vp, _c_int, _c_int, _vp,
+        _c_i64, _c_int, _c_dbl, _c_int, _vp, _vp, _vp, _vp]),
+    "ksig_signature_levels_bwd": (_c_int, [
+        _c_int, _vp, _vp, _vp, _c_int, _vp, _vp, _vp, _c_int, _c_int, _vp,
+        _c_i64, _c_int, _c_dbl, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "ksig_signature_levels_gram": (_c_int, [
         _c_int, _vp, _c_i64, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp]),
     "ksig_signature_pde": (_c_int, [

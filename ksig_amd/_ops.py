@@ -52,13 +52,17 @@ class SeqSet:
     return SeqSet(data, off, lens, max(lens_list) if lens_list else 0, len(seqs))
 
 
-def static_code(static_kernel) -> tuple:
-  """(kind, param) of a fused static kernel, or None for a generic one."""
+def static_code(static_kernel, raw: bool = False) -> tuple:
+  """(kind, param) of a fused static kernel, or None for a generic one.
+  With ``raw`` the parameter is returned as stored (it may be a tensor that
+  requires grad)."""
   from .static.kernels import LinearKernel, RBFKernel
   if type(static_kernel) is RBFKernel:
-    return _lib.STATIC_RBF, float(static_kernel.bandwidth)
+    p = static_kernel.bandwidth
+    return _lib.STATIC_RBF, (p if raw else float(p))
   if type(static_kernel) is LinearKernel:
-    return _lib.STATIC_LINEAR, float(static_kernel.scale)
+    p = static_kernel.scale
+    return _lib.STATIC_LINEAR, (p if raw else float(p))
   return None
 
 

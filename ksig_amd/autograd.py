@@ -1,0 +1,181 @@
+"""Differentiable signature-kernel levels and MMD^2 (no reference counterpart:
+ksig is forward-only).
+
+``signature_levels`` returns the per-pair level values K_0..K_n of the
+first-order DP signature kernel (RBF or linear static kernel, difference=True)
+as a differentiable function of the sequences and of the bandwidth / scale.
+The forward is the fused HIP sweep (it also leaves the O((Lx+Ly)·n) boundary
+state per pair); the backward is the HIP adjoint sweep, which writes
+omega = dL/dG * dG/dz per static-kernel entry, followed by the contraction of
+omega with the inputs (batched GEMMs over each pair's L x d blocks):
+
+  RBF,    G = exp(-s |x-y|^2), s = 1/max(2 sigma^2, 1e-12):
+    dL/dx_a = -2s (x_a sum_b omega_ab - sum_b omega_ab y_b)
+    dL/dy_b =  2s (sum_a omega_ab x_a - y_b sum_a omega_ab)
+    dL/dsigma = sum omega_ab |x_a - y_b|^2 / sigma^3
+  linear, G = c <x, y>:
+    dL/dx_a = c sum_b omega_ab y_b,  dL/dy_b = c sum_a omega_ab x_a,
+    dL/dc = sum omega_ab G_ab / c
+"""
+
+from __future__ import annotations
+
+import math
+from typing import Optional
+
+import torch
+
+from . import _lib, _ops
+
+LOG2E = 1.4426950408889634
+_OMEGA_BYTES = 1 << 30  # omega scratch per chunk of pairs
+
+
+def _as_param(p, device) -> torch.Tensor:
+  if isinstance(p, torch.Tensor):
+    return p.to(device=device, dtype=torch.float64).reshape(())
+  return torch.tensor(float(p), device=device, dtype=torch.float64)
+
+
+class _SigLevels(torch.autograd.Function):
+
+  @staticmethod
+  def forward(ctx, xdata, ydata, param, xs_meta, ys_meta, pairs, n_levels, kind):
+    lib = _lib.load()
+    Xs = _ops.SeqSet(xdata.detach().contiguous(), *xs_meta)
+    Ys = _ops.SeqSet(ydata.detach().contiguous(), *ys_meta)
+    pval = float(param)
+    n_pairs = pairs.shape[0]
+    dev = xdata.device
+    out = torch.empty((n_pairs, n_levels + 1), device=dev, dtype=torch.float64)
+    la = Xs.len.to(torch.int64)[pairs[:, 0].long()]
+    lb = Ys.len.to(torch.int64)[pairs[:, 1].long()]
+    per = (n_levels - 1) * torch.clamp(la + lb - 2, min=0)
+    save_off = torch.zeros(n_pairs, device=dev, dtype=torch.int64)
+    if n_pairs > 1:
+      save_off[1:] = torch.cumsum(per, 0)[:-1]
+    total = int(per.sum().item()) if n_pairs else 0
+    save = torch.empty(max(total, 1), device=dev, dtype=Xs.data.dtype)
+    if n_pairs:
+      _lib.check(lib.ksig_signature_levels_fwd_save(
+          _lib.dtype_code(Xs.data.dtype), Xs.data.data_ptr(), Xs.off.data_ptr(),
+          Xs.len.data_ptr(), Xs.max_len, Ys.data.data_ptr(), Ys.off.data_ptr(),
+          Ys.len.data_ptr(), Ys.max_len, Xs.d, pairs.data_ptr(), n_pairs, kind, pval,
+          n_levels, out.data_ptr(), save.data_ptr(), save_off.data_ptr(),
+          _lib.stream_handle(dev)))
+    ctx.save_for_backward(xdata, ydata, pairs, save, save_off, la, lb)
+    ctx.meta = (xs_meta, ys_meta, n_levels, kind, pval)
+    return out
+
+  @staticmethod
+  def backward(ctx, gout):
+    lib = _lib.load()
+    xdata, ydata, pairs, save, save_off, la, lb = ctx.saved_tensors
+    xs_meta, ys_meta, n_levels, kind, pval = ctx.meta
+    Xs = _ops.SeqSet(xdata.detach().contiguous(), *xs_meta)
+    Ys = _ops.SeqSet(ydata.detach().contiguous(), *ys_meta)
+    dev = xdata.device
+    dt = Xs.data.dtype
+    gout = gout.to(torch.float64).contiguous()
+    n_pairs = pairs.shape[0]
+    gx = torch.zeros_like(Xs.data) if ctx.needs_input_grad[0] else None
+    gy = torch.zeros_like(Ys.data) if ctx.needs_input_grad[1] else None
+    gparam_total = torch.zeros((), device=dev, dtype=torch.float64)
+    if n_pairs == 0:
+      return gx, gy, gparam_total, None, None, None, None, None
+    sizes = (la * lb)
+    esz = Xs.data.element_size()
+    # chunk pairs so that omega stays within the scratch budget
+    budget = max(_OMEGA_BYTES // esz, int(sizes.max().item()))
+    chunk_id = ((torch.cumsum(sizes, 0) - sizes) // budget).cpu()
+    change = torch.nonzero(chunk_id[1:] != chunk_id[:-1]).flatten() + 1
+    bounds = [0] + change.tolist() + [n_pairs]
+    s_rbf = 1.0 / max(2.0 * pval * pval, 1e-12)
+    for c0, c1 in zip(bounds[:-1], bounds[1:]):
+      pc = pairs[c0:c1].contiguous()
+      sz = sizes[c0:c1]
+      w_off = torch.zeros(c1 - c0, device=dev, dtype=torch.int64)
+      if c1 - c0 > 1:
+        w_off[1:] = torch.cumsum(sz, 0)[:-1]
+      omega = torch.empty(int(sz.sum().item()), device=dev, dtype=dt)
+      gp = torch.empty(c1 - c0, device=dev, dtype=torch.float64)
+      _lib.check(lib.ksig_signature_levels_bwd(
+          _lib.dtype_code(dt), Xs.data.data_ptr(), Xs.off.data_ptr(), Xs.len.data_ptr(),
+          Xs.max_len, Ys.data.data_ptr(), Ys.off.data_ptr(), Ys.len.data_ptr(),
+          Ys.max_len, Xs.d, pc.data_ptr(), c1 - c0, kind, pval, n_levels,
+          gout[c0:c1].data_ptr(), save.data_ptr(), save_off[c0:c1].data_ptr(),
+          omega.data_ptr(), w_off.data_ptr(), gp.data_ptr(), _lib.stream_handle(dev)))
+      gparam_total = gparam_total + gp.sum()
+      if gx is None and gy is None:
+        continue
+      _contract(omega, w_off, pc, Xs, Ys, la[c0:c1], lb[c0:c1], kind, pval, s_rbf, gx, gy)
+    if kind == _lib.STATIC_RBF:
+      den = 2.0 * pval * pval
+      if den > 1e-12:
+        q = LOG2E / den
+        gparam_total = gparam_total / (q * pval ** 3)
+      else:
+        gparam_total = gparam_total * 0.0
+    else:
+      gparam_total = gparam_total / pval
+    return gx, gy, gparam_total, None, None, None, None, None
+
+
+def _contract(omega, w_off, pairs, Xs, Ys, la, lb, kind, pval, s_rbf, gx, gy):
+  """dL/dx, dL/dy from omega (per pair [La, Lb] blocks)."""
+  ia = pairs[:, 0].long()
+  ib = pairs[:, 1].long()
+  if bool((la == la[0]).all()) and bool((lb == lb[0]).all()):
+    A, B = int(la[0]), int(lb[0])
+    _contract_block(omega.view(-1, A, B), ia, ib, A, B, Xs, Ys, kind, pval, s_rbf, gx, gy)
+    return
+  for p in range(pairs.shape[0]):  # ragged pairs: one block at a time
+    A, B = int(la[p]), int(lb[p])
+    o = int(w_off[p])
+    _contract_block(omega[o:o + A * B].view(1, A, B), ia[p:p + 1], ib[p:p + 1], A, B,
+                    Xs, Ys, kind, pval, s_rbf, gx, gy)
+
+
+def _contract_block(W, ia, ib, A, B, Xs, Ys, kind, pval, s_rbf, gx, gy):
+  d = Xs.d
+  dev = W.device
+  rows_a = Xs.off[ia][:, None] + torch.arange(A, device=dev)[None]
+  rows_b = Ys.off[ib][:, None] + torch.arange(B, device=dev)[None]
+  xa = Xs.data[rows_a]  # [P, A, d]
+  yb = Ys.data[rows_b]  # [P, B, d]
+  Wt = W.transpose(1, 2).contiguous()
+  if kind == _lib.STATIC_RBF:
+    if gx is not None:
+      part = -2.0 * s_rbf * (xa * W.sum(-1, keepdim=True) - torch.bmm(W, yb))
+      gx.index_add_(0, rows_a.reshape(-1), part.reshape(-1, d))
+    if gy is not None:
+      part = 2.0 * s_rbf * (torch.bmm(Wt, xa) - yb * W.sum(-2)[..., None])
+      gy.index_add_(0, rows_b.reshape(-1), part.reshape(-1, d))
+  else:
+    if gx is not None:
+      gx.index_add_(0, rows_a.reshape(-1), (pval * torch.bmm(W, yb)).reshape(-1, d))
+    if gy is not None:
+      gy.index_add_(0, rows_b.reshape(-1), (pval * torch.bmm(Wt, xa)).reshape(-1, d))
+
+
+def signature_levels(X: _ops.SeqSet, Y: _ops.SeqSet, pairs: torch.Tensor, n_levels: int,
+                     kind: int, param, xdata: Optional[torch.Tensor] = None,
+                     ydata: Optional[torch.Tensor] = None) -> torch.Tensor:
+  """Differentiable level values [n_pairs, n_levels+1] (float64) of the
+  first-order DP kernel with difference.  ``xdata`` / ``ydata`` are the packed
+  [rows, d] tensors the gradient flows to (default: X.data / Y.data);
+  ``param`` may be a tensor requiring grad."""
+  xdata = X.data if xdata is None else xdata
+  ydata = Y.data if ydata is None else ydata
+  p = _as_param(param, xdata.device)
+  xs_meta = (X.off, X.len, X.max_len, X.n)
+  ys_meta = (Y.off, Y.len, Y.max_len, Y.n)
+  return _SigLevels.apply(xdata, ydata, p, xs_meta, ys_meta, pairs, n_levels, kind)
+
+
+def mmd2_unbiased(Kxx: torch.Tensor, Kyy: torch.Tensor, Kxy: torch.Tensor) -> torch.Tensor:
+  """Unbiased MMD^2: mean_{i!=j} Kxx + mean_{i!=j} Kyy - 2 mean Kxy."""
+  n, m = Kxx.shape[0], Kyy.shape[0]
+  sxx = (Kxx.sum() - torch.diagonal(Kxx).sum()) / (n * (n - 1))
+  syy = (Kyy.sum() - torch.diagonal(Kyy).sum()) / (m * (m - 1))
+  return sxx + syy - 2.0 * Kxy.mean()

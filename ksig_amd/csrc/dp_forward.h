@@ -34,6 +34,28 @@ struct DpFwdArgs {
   const void* gram;      // kind == kGram: [n_pairs, gram_lx, gram_ly]
   int gram_lx, gram_ly;
   double* out;           // [n_pairs, n_levels + 1]
+  // Optional boundary state for the adjoint (order 1, difference only):
+  // per pair at save_off[p], [n_levels-1][Nr] row-end prefixes P_a(i, Nc)
+  // then [n_levels-1][Nc] final column sums C_a(Nr, j), in the kernel's
+  // row/column orientation (rows = the longer sequence).
+  void* save;
+  const int64_t* save_off;
+};
+
+struct DpBwdArgs {
+  SeqSet x, y;
+  int d;
+  const int32_t* pairs;
+  int64_t n_pairs;
+  int kind;
+  double xscale, yscale;
+  int n_levels;
+  const double* gout;      // [n_pairs, n_levels + 1] d loss / d K_m
+  const void* save;        // boundary state written by the forward
+  const int64_t* save_off;
+  void* omega;             // per pair [len_x, len_y] at omega_off[p]: dL/dG * dG/d(arg)
+  const int64_t* omega_off;
+  double* gparam;          // [n_pairs]: sum omega * D' (RBF) or omega * G (linear)
 };
 
 struct PdeFwdArgs {
@@ -52,6 +74,8 @@ struct PdeFwdArgs {
 int dp_flow_count(int nlev, int order);
 bool dp_supported(int n_levels, int order);
 hipError_t dp_forward_launch(const DpFwdArgs& a, int dtype, size_t lds, hipStream_t s);
+hipError_t dp_backward_launch(const DpBwdArgs& a, int dtype, size_t lds, hipStream_t s);
+bool dp_backward_supported(int n_levels);
 hipError_t pde_forward_launch(const PdeFwdArgs& a, int dtype, size_t lds, hipStream_t s);
 
 }  // namespace ksig

@@ -260,6 +260,8 @@ dp_forward_kernel(const DpFwdArgs a) {
   };
   T gpre = T(0);
   if (GRAM) gpre = gram_at(row, c);
+  T* save = (!GRAM && a.save != nullptr)
+                ? reinterpret_cast<T*>(a.save) + a.save_off[pr] : nullptr;
 
   const int total = nb * Pc + kWave - 1;
   int next_stage = Pc + kWave;  // step at which band b+2 rows replace band b's
@@ -310,6 +312,11 @@ dp_forward_kernel(const DpFwdArgs a) {
     dp_cell<T, NLEV, ORDER>(M, in, of, PS, HB, c == -1);
 #pragma unroll
     for (int k = 0; k < NF; ++k) fl[k] = of[k];
+    if (ORDER == 1 && NLEV > 1 && save != nullptr && active && c == Nc - 1 && row < Nr) {
+      // PS now holds the row-end prefixes P_a(row, Nc) (a = 1..NLEV-1)
+#pragma unroll
+      for (int m = 0; m < NPS; ++m) save[m * Nr + row] = PS[m];
+    }
 
     if (lane == kWave - 1 && active) {
 #pragma unroll
@@ -336,6 +343,14 @@ dp_forward_kernel(const DpFwdArgs a) {
   }
 #pragma unroll
   for (int m = 0; m < NLEV; ++m) acc[m] = wave_sum(acc[m]);
+  if (ORDER == 1 && NLEV > 1 && save != nullptr) {
+    // final column sums C_a(Nr, j), a = 1..NLEV-1
+    T* cs = save + (NLEV - 1) * Nr;
+    for (int s = lane + 1; s <= Nc; s += kWave) {
+#pragma unroll
+      for (int m = 0; m < NLEV - 1; ++m) cs[m * Nc + s - 1] = fifo[s * NF + 1 + m];
+    }
+  }
   if (lane == 0) out[0] = 1.0;
 #pragma unroll
   for (int m = 0; m < NLEV; ++m)

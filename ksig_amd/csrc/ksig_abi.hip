@@ -133,6 +133,88 @@ int ksig_signature_levels(int dtype, const void* x, const int64_t* x_off,
   return dp_levels_common(dtype, a, cols < 1 ? 1 : cols, dmax_for(d), stream);
 }
 
+int ksig_signature_levels_fwd_save(int dtype, const void* x, const int64_t* x_off,
+                                   const int32_t* x_len, int max_len_x, const void* y,
+                                   const int64_t* y_off, const int32_t* y_len,
+                                   int max_len_y, int d, const int32_t* pairs,
+                                   int64_t n_pairs, int static_kind, double static_param,
+                                   int n_levels, double* out, void* save,
+                                   const int64_t* save_off, void* stream) {
+  int rc = check_common(dtype, x, y, d, pairs, n_pairs, out);
+  if (rc) return rc;
+  if (n_levels < 1) return fail(KSIG_EINVAL, "n_levels must be positive");
+  if (!ksig::dp_supported(n_levels, 1) || !ksig::dp_backward_supported(n_levels))
+    return fail(KSIG_EUNSUPPORTED, "n_levels=%d has no adjoint built", n_levels);
+  ksig::DpFwdArgs a{};
+  rc = static_scales(static_kind, static_param, &a.xscale, &a.yscale);
+  if (rc) return rc;
+  if (n_pairs == 0) return 0;
+  if (n_levels > 1 && (!save || !save_off)) return fail(KSIG_EINVAL, "null save buffer");
+  a.x = {x, x_off, x_len};
+  a.y = {y, y_off, y_len};
+  a.d = d;
+  a.pairs = pairs;
+  a.n_pairs = n_pairs;
+  a.kind = static_kind;
+  a.n_levels = n_levels;
+  a.order = 1;
+  a.difference = 1;
+  a.out = out;
+  a.save = n_levels > 1 ? save : nullptr;
+  a.save_off = save_off;
+  const int cols = max_len_x < max_len_y ? max_len_x : max_len_y;
+  return dp_levels_common(dtype, a, cols < 1 ? 1 : cols, dmax_for(d), stream);
+}
+
+int ksig_signature_levels_bwd(int dtype, const void* x, const int64_t* x_off,
+                              const int32_t* x_len, int max_len_x, const void* y,
+                              const int64_t* y_off, const int32_t* y_len,
+                              int max_len_y, int d, const int32_t* pairs,
+                              int64_t n_pairs, int static_kind, double static_param,
+                              int n_levels, const double* grad_levels, const void* save,
+                              const int64_t* save_off, void* omega,
+                              const int64_t* omega_off, double* grad_param,
+                              void* stream) {
+  int rc = check_common(dtype, x, y, d, pairs, n_pairs, grad_param);
+  if (rc) return rc;
+  if (n_levels < 1) return fail(KSIG_EINVAL, "n_levels must be positive");
+  if (!ksig::dp_backward_supported(n_levels))
+    return fail(KSIG_EUNSUPPORTED, "n_levels=%d has no adjoint built", n_levels);
+  ksig::DpBwdArgs a{};
+  rc = static_scales(static_kind, static_param, &a.xscale, &a.yscale);
+  if (rc) return rc;
+  if (n_pairs == 0) return 0;
+  if (!grad_levels || !omega || !omega_off || !save_off || (n_levels > 1 && !save))
+    return fail(KSIG_EINVAL, "null device pointer");
+  a.x = {x, x_off, x_len};
+  a.y = {y, y_off, y_len};
+  a.d = d;
+  a.pairs = pairs;
+  a.n_pairs = n_pairs;
+  a.kind = static_kind;
+  a.n_levels = n_levels;
+  a.gout = grad_levels;
+  a.save = save;
+  a.save_off = save_off;
+  a.omega = omega;
+  a.omega_off = omega_off;
+  a.gparam = grad_param;
+  int cols = max_len_x < max_len_y ? max_len_x : max_len_y;
+  if (cols < 1) cols = 1;
+  const size_t esz = dtype == KSIG_F32 ? 4 : 8;
+  const int dmax = dmax_for(d);
+  const int nl1 = n_levels > 1 ? n_levels - 1 : 1;
+  const int nf = 3 + 2 * (n_levels - 1);
+  const int pc = cols + 1 > ksig::kWave ? cols + 1 : ksig::kWave;
+  const size_t lds = (2 * (size_t)ksig::kWave * (dmax + 1 + nl1) + (size_t)cols * (dmax + 1) +
+                      (size_t)pc * nf) * esz;
+  if (lds > max_lds_bytes())
+    return fail(KSIG_ELDS, "adjoint LDS state of %zu bytes exceeds the device limit", lds);
+  hipError_t e = ksig::dp_backward_launch(a, dtype, lds, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "dp_backward launch");
+  return 0;
+}
+
 int ksig_signature_levels_gram(int dtype, const void* M, int64_t n_mats, int lx,
                                int ly, int n_levels, int order, int difference,
                                double* out, void* stream) {

@@ -71,6 +71,18 @@ def _normalise(K: torch.Tensor, Kxd: torch.Tensor, Kyd: torch.Tensor) -> torch.T
   return K / (sx[..., :, None] * sy[..., None, :])
 
 
+def _mirror(vals: torch.Tensor, pairs: torch.Tensor, n: int) -> torch.Tensor:
+  """Symmetric [..., n, n] matrix from upper-triangle pair values [..., P]
+  (out-of-place index_add, so autograd sums both mirror images)."""
+  i, j = pairs[:, 0].long(), pairs[:, 1].long()
+  off = i != j
+  lead = vals.shape[:-1]
+  K = torch.zeros(lead + (n * n,), device=vals.device, dtype=vals.dtype)
+  K = K.index_add(-1, i * n + j, vals)
+  K = K.index_add(-1, (j * n + i)[off], vals[..., off])
+  return K.reshape(lead + (n, n))
+
+
 class SignatureKernel(SignatureBase):
   """Full-rank signature kernel (ksig/kernels.py:97-214)."""
 
@@ -105,26 +117,41 @@ class SignatureKernel(SignatureBase):
     kind, param = code
     dev = X.device
     Xs = _ops.SeqSet.dense(X)
+    Ys = Xs if (Y is None or diag) else _ops.SeqSet.dense(Y)
+    grad = self._wants_grad(X, Y, self.static_kernel)
+
+    def levels(pairs):
+      if grad:
+        from . import autograd
+        return autograd.signature_levels(
+            Xs, Ys, pairs, self.n_levels, kind, _ops.static_code(self.static_kernel, True)[1],
+            xdata=X.reshape(-1, X.shape[-1]),
+            ydata=(X if (Y is None or diag) else Y).reshape(-1, X.shape[-1]))
+      return _ops.signature_levels(Xs, Ys, pairs, self.n_levels, self.order,
+                                   self.difference, kind, param)
+
     if diag:
-      pairs = _ops.diag_pairs(X.shape[0], dev)
-      lev = _ops.signature_levels(Xs, Xs, pairs, self.n_levels, self.order,
-                                  self.difference, kind, param)
-      return lev.t()
+      return levels(_ops.diag_pairs(X.shape[0], dev)).t()
     if Y is None:
       n = X.shape[0]
       pairs = _ops.triu_pairs(n, dev)
-      lev = _ops.signature_levels(Xs, Xs, pairs, self.n_levels, self.order,
-                                  self.difference, kind, param)
-      K = torch.empty((self.n_levels + 1, n, n), device=dev, dtype=torch.float64)
-      i, j = pairs[:, 0].long(), pairs[:, 1].long()
-      K[:, i, j] = lev.t()
-      K[:, j, i] = lev.t()
-      return K
-    Ys = _ops.SeqSet.dense(Y)
+      lev = levels(pairs).t()
+      return _mirror(lev, pairs, n)
     pairs = _ops.dense_pairs(X.shape[0], Y.shape[0], dev)
-    lev = _ops.signature_levels(Xs, Ys, pairs, self.n_levels, self.order,
-                                self.difference, kind, param)
-    return lev.t().reshape(self.n_levels + 1, X.shape[0], Y.shape[0])
+    return levels(pairs).t().reshape(self.n_levels + 1, X.shape[0], Y.shape[0])
+
+  def _wants_grad(self, X, Y, static_kernel) -> bool:
+    """Use the differentiable path when autograd needs gradients of the
+    sequences or of the static-kernel parameter (first order, difference)."""
+    if not torch.is_grad_enabled():
+      return False
+    p = _ops.static_code(static_kernel, True)[1]
+    need = (X.requires_grad or (Y is not None and Y.requires_grad)
+            or (isinstance(p, torch.Tensor) and p.requires_grad))
+    if need and (self.order != 1 or not self.difference):
+      raise NotImplementedError(
+          'gradients are implemented for order=1, difference=True')
+    return need
 
   def _compute_kernel(self, X, Y=None, diag: bool = False) -> torch.Tensor:
     """ksig/kernels.py:147-164: levels if ``normalize`` else their sum."""
@@ -191,11 +218,7 @@ class SignaturePDEKernel(SignatureKernel):
       pairs = _ops.triu_pairs(n, dev)
       v = _ops.signature_pde(Xs, Xs, pairs, self.difference, kind, param,
                              self.dyadic_order)
-      K = torch.empty((n, n), device=dev, dtype=torch.float64)
-      i, j = pairs[:, 0].long(), pairs[:, 1].long()
-      K[i, j] = v
-      K[j, i] = v
-      return K
+      return _mirror(v, pairs, n)
     Ys = _ops.SeqSet.dense(Y)
     pairs = _ops.dense_pairs(X.shape[0], Y.shape[0], dev)
     v = _ops.signature_pde(Xs, Ys, pairs, self.difference, kind, param,
