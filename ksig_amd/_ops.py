@@ -52,6 +52,10 @@ class SeqSet:
     return SeqSet(data, off, lens, max(lens_list) if lens_list else 0, len(seqs))
 
 
+def _scalar(p) -> float:
+  return float(p.detach()) if isinstance(p, torch.Tensor) else float(p)
+
+
 def static_code(static_kernel, raw: bool = False) -> tuple:
   """(kind, param) of a fused static kernel, or None for a generic one.
   With ``raw`` the parameter is returned as stored (it may be a tensor that
@@ -59,10 +63,10 @@ def static_code(static_kernel, raw: bool = False) -> tuple:
   from .static.kernels import LinearKernel, RBFKernel
   if type(static_kernel) is RBFKernel:
     p = static_kernel.bandwidth
-    return _lib.STATIC_RBF, (p if raw else float(p))
+    return _lib.STATIC_RBF, (p if raw else _scalar(p))
   if type(static_kernel) is LinearKernel:
     p = static_kernel.scale
-    return _lib.STATIC_LINEAR, (p if raw else float(p))
+    return _lib.STATIC_LINEAR, (p if raw else _scalar(p))
   return None
 
 

@@ -44,7 +44,7 @@ class _SigLevels(torch.autograd.Function):
     lib = _lib.load()
     Xs = _ops.SeqSet(xdata.detach().contiguous(), *xs_meta)
     Ys = _ops.SeqSet(ydata.detach().contiguous(), *ys_meta)
-    pval = float(param)
+    pval = float(param.detach())
     n_pairs = pairs.shape[0]
     dev = xdata.device
     out = torch.empty((n_pairs, n_levels + 1), device=dev, dtype=torch.float64)
@@ -179,3 +179,138 @@ def mmd2_unbiased(Kxx: torch.Tensor, Kyy: torch.Tensor, Kxy: torch.Tensor) -> to
   sxx = (Kxx.sum() - torch.diagonal(Kxx).sum()) / (n * (n - 1))
   syy = (Kyy.sum() - torch.diagonal(Kyy).sum()) / (m * (m - 1))
   return sxx + syy - 2.0 * Kxy.mean()
+
+
+# -----------------------------------------------------------------------------
+# Signature-PDE kernel
+# -----------------------------------------------------------------------------
+
+_DUMP_BYTES = 1 << 30  # U dump per chunk of pairs
+
+
+class _SigPDE(torch.autograd.Function):
+
+  @staticmethod
+  def forward(ctx, xdata, ydata, param, xs_meta, ys_meta, pairs, kind, dyadic, difference):
+    Xs = _ops.SeqSet(xdata.detach().contiguous(), *xs_meta)
+    Ys = _ops.SeqSet(ydata.detach().contiguous(), *ys_meta)
+    pval = float(param.detach())
+    out = _ops.signature_pde(Xs, Ys, pairs, difference, kind, pval, dyadic)
+    ctx.save_for_backward(xdata, ydata, pairs)
+    ctx.meta = (xs_meta, ys_meta, kind, pval, dyadic, difference)
+    return out
+
+  @staticmethod
+  def backward(ctx, gout):
+    lib = _lib.load()
+    xdata, ydata, pairs = ctx.saved_tensors
+    xs_meta, ys_meta, kind, pval, dyadic, difference = ctx.meta
+    Xs = _ops.SeqSet(xdata.detach().contiguous(), *xs_meta)
+    Ys = _ops.SeqSet(ydata.detach().contiguous(), *ys_meta)
+    dev, dt = xdata.device, Xs.data.dtype
+    gout = gout.to(torch.float64).contiguous()
+    n_pairs = pairs.shape[0]
+    gx = torch.zeros_like(Xs.data) if ctx.needs_input_grad[0] else None
+    gy = torch.zeros_like(Ys.data) if ctx.needs_input_grad[1] else None
+    gparam = torch.zeros((), device=dev, dtype=torch.float64)
+    if n_pairs == 0:
+      return gx, gy, gparam, None, None, None, None, None, None
+    la = Xs.len.to(torch.int64)[pairs[:, 0].long()]
+    lb = Ys.len.to(torch.int64)[pairs[:, 1].long()]
+    sh = 1 if difference else 0
+    nr = torch.clamp(torch.maximum(la, lb) - sh, min=0)   # kernel orientation
+    nc = torch.clamp(torch.minimum(la, lb) - sh, min=0)
+    usz = (nr * nc) << (2 * dyadic)
+    msz = nr * nc
+    budget = max(_DUMP_BYTES // Xs.data.element_size(), int(usz.max().item()))
+    chunk_id = ((torch.cumsum(usz, 0) - usz) // budget).cpu()
+    change = torch.nonzero(chunk_id[1:] != chunk_id[:-1]).flatten() + 1
+    bounds = [0] + change.tolist() + [n_pairs]
+    s_rbf = 1.0 / max(2.0 * pval * pval, 1e-12)
+    stream = _lib.stream_handle(dev)
+    for c0, c1 in zip(bounds[:-1], bounds[1:]):
+      pc = pairs[c0:c1].contiguous()
+      P = c1 - c0
+      u_off = torch.zeros(P, device=dev, dtype=torch.int64)
+      m_off = torch.zeros(P, device=dev, dtype=torch.int64)
+      if P > 1:
+        u_off[1:] = torch.cumsum(usz[c0:c1], 0)[:-1]
+        m_off[1:] = torch.cumsum(msz[c0:c1], 0)[:-1]
+      u_all = torch.empty(max(int(usz[c0:c1].sum()), 1), device=dev, dtype=dt)
+      m_all = torch.empty(max(int(msz[c0:c1].sum()), 1), device=dev, dtype=dt)
+      gm = torch.zeros_like(m_all)
+      tmp = torch.empty(P, device=dev, dtype=torch.float64)
+      _lib.check(lib.ksig_signature_pde_fwd_dump(
+          _lib.dtype_code(dt), Xs.data.data_ptr(), Xs.off.data_ptr(), Xs.len.data_ptr(),
+          Xs.max_len, Ys.data.data_ptr(), Ys.off.data_ptr(), Ys.len.data_ptr(), Ys.max_len,
+          Xs.d, pc.data_ptr(), P, kind, pval, dyadic, int(difference), tmp.data_ptr(),
+          u_all.data_ptr(), u_off.data_ptr(), m_all.data_ptr(), m_off.data_ptr(), stream))
+      _lib.check(lib.ksig_signature_pde_bwd(
+          _lib.dtype_code(dt), Xs.len.data_ptr(), Xs.max_len, Ys.len.data_ptr(), Ys.max_len,
+          pc.data_ptr(), P, dyadic, int(difference), gout[c0:c1].data_ptr(),
+          u_all.data_ptr(), u_off.data_ptr(), m_all.data_ptr(), m_off.data_ptr(),
+          gm.data_ptr(), stream))
+      del u_all
+      _pde_chain(gm, m_off, pc, Xs, Ys, la[c0:c1], lb[c0:c1], kind, pval, s_rbf,
+                 difference, gx, gy, gparam)
+    return gx, gy, gparam, None, None, None, None, None, None
+
+
+def _pde_chain(gm, m_off, pairs, Xs, Ys, la, lb, kind, pval, s_rbf, difference,
+               gx, gy, gparam):
+  """dL/dM per pair -> dL/dG (ΔΔ^T) -> static-kernel chain rule."""
+  ia = pairs[:, 0].long()
+  ib = pairs[:, 1].long()
+  sh = 1 if difference else 0
+  uniform = bool((la == la[0]).all()) and bool((lb == lb[0]).all())
+  groups = ([(torch.arange(pairs.shape[0], device=gm.device), int(la[0]), int(lb[0]))]
+            if uniform else
+            [(torch.tensor([p], device=gm.device), int(la[p]), int(lb[p]))
+             for p in range(pairs.shape[0])])
+  for idx, A, B in groups:
+    swapped = B > A
+    nr, nc = (max(A, B) - sh, min(A, B) - sh)
+    if nr <= 0 or nc <= 0:
+      continue
+    if uniform:
+      gM = gm[:idx.numel() * nr * nc].view(-1, nr, nc)
+    else:
+      o = int(m_off[idx[0]])
+      gM = gm[o:o + nr * nc].view(1, nr, nc)
+    if swapped:
+      gM = gM.transpose(1, 2)
+    if difference:
+      Pd = torch.nn.functional.pad(gM, (1, 1, 1, 1))
+      gG = Pd[:, :-1, :-1] - Pd[:, :-1, 1:] - Pd[:, 1:, :-1] + Pd[:, 1:, 1:]
+    else:
+      gG = gM
+    gG = gG.contiguous()
+    rows_a = Xs.off[ia[idx]][:, None] + torch.arange(A, device=gm.device)[None]
+    rows_b = Ys.off[ib[idx]][:, None] + torch.arange(B, device=gm.device)[None]
+    xa = Xs.data[rows_a]
+    yb = Ys.data[rows_b]
+    dot = torch.bmm(xa, yb.transpose(1, 2).contiguous())
+    if kind == _lib.STATIC_RBF:
+      D2 = (xa * xa).sum(-1)[:, :, None] + (yb * yb).sum(-1)[:, None, :] - 2 * dot
+      G = torch.exp(-D2 * s_rbf)
+      W = gG * G
+      if 2.0 * pval * pval > 1e-12:
+        gparam += (W.double() * D2.double()).sum() / pval ** 3
+    else:
+      W = gG
+      gparam += (W.double() * dot.double()).sum()
+    _contract_block(W, ia[idx], ib[idx], A, B, Xs, Ys, kind, pval, s_rbf, gx, gy)
+
+
+def signature_pde(X: _ops.SeqSet, Y: _ops.SeqSet, pairs: torch.Tensor, kind: int, param,
+                  dyadic_order: int = 0, difference: bool = True,
+                  xdata: Optional[torch.Tensor] = None,
+                  ydata: Optional[torch.Tensor] = None) -> torch.Tensor:
+  """Differentiable signature-PDE values [n_pairs] (float64)."""
+  xdata = X.data if xdata is None else xdata
+  ydata = Y.data if ydata is None else ydata
+  p = _as_param(param, xdata.device)
+  xs_meta = (X.off, X.len, X.max_len, X.n)
+  ys_meta = (Y.off, Y.len, Y.max_len, Y.n)
+  return _SigPDE.apply(xdata, ydata, p, xs_meta, ys_meta, pairs, kind, int(dyadic_order),
+                       bool(difference))

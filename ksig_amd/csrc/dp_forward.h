@@ -69,6 +69,27 @@ struct PdeFwdArgs {
   const void* gram;
   int gram_lx, gram_ly;
   double* out;           // [n_pairs]
+  // Optional dumps for the adjoint (kernel orientation, rows = longer side):
+  // U of every refined cell, [Nr << λ][Nc << λ] at u_off[p], and the refined
+  // cell value M / 4^λ of every original cell, [Nr][Nc] at m_off[p].
+  void* dump_u;
+  const int64_t* u_off;
+  void* dump_m;
+  const int64_t* m_off;
+};
+
+struct PdeBwdArgs {
+  const int32_t* x_len;
+  const int32_t* y_len;
+  const int32_t* pairs;
+  int64_t n_pairs;
+  int dyadic, difference;
+  const double* gout;    // [n_pairs] dL/dK
+  const void* u_all;
+  const int64_t* u_off;
+  const void* m_all;
+  const int64_t* m_off;
+  void* gm;              // [Nr][Nc] at m_off[p]: dL/dM of every original cell (accumulated)
 };
 
 int dp_flow_count(int nlev, int order);
@@ -77,5 +98,6 @@ hipError_t dp_forward_launch(const DpFwdArgs& a, int dtype, size_t lds, hipStrea
 hipError_t dp_backward_launch(const DpBwdArgs& a, int dtype, size_t lds, hipStream_t s);
 bool dp_backward_supported(int n_levels);
 hipError_t pde_forward_launch(const PdeFwdArgs& a, int dtype, size_t lds, hipStream_t s);
+hipError_t pde_backward_launch(const PdeBwdArgs& a, int dtype, size_t lds, hipStream_t s);
 
 }  // namespace ksig

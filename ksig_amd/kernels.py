@@ -210,17 +210,27 @@ class SignaturePDEKernel(SignatureKernel):
     kind, param = code
     dev = X.device
     Xs = _ops.SeqSet.dense(X)
+    Ys = Xs if (Y is None or diag) else _ops.SeqSet.dense(Y)
+    p_raw = _ops.static_code(self.static_kernel, True)[1]
+    grad = torch.is_grad_enabled() and (
+        X.requires_grad or (Y is not None and Y.requires_grad)
+        or (isinstance(p_raw, torch.Tensor) and p_raw.requires_grad))
+
+    def values(pairs):
+      if grad:
+        from . import autograd
+        return autograd.signature_pde(
+            Xs, Ys, pairs, kind, p_raw, self.dyadic_order, self.difference,
+            xdata=X.reshape(-1, X.shape[-1]),
+            ydata=(X if (Y is None or diag) else Y).reshape(-1, X.shape[-1]))
+      return _ops.signature_pde(Xs, Ys, pairs, self.difference, kind, param,
+                                self.dyadic_order)
+
     if diag:
-      return _ops.signature_pde(Xs, Xs, _ops.diag_pairs(X.shape[0], dev),
-                                self.difference, kind, param, self.dyadic_order)
+      return values(_ops.diag_pairs(X.shape[0], dev))
     if Y is None:
       n = X.shape[0]
       pairs = _ops.triu_pairs(n, dev)
-      v = _ops.signature_pde(Xs, Xs, pairs, self.difference, kind, param,
-                             self.dyadic_order)
-      return _mirror(v, pairs, n)
-    Ys = _ops.SeqSet.dense(Y)
+      return _mirror(values(pairs), pairs, n)
     pairs = _ops.dense_pairs(X.shape[0], Y.shape[0], dev)
-    v = _ops.signature_pde(Xs, Ys, pairs, self.difference, kind, param,
-                           self.dyadic_order)
-    return v.reshape(X.shape[0], Y.shape[0])
+    return values(pairs).reshape(X.shape[0], Y.shape[0])
