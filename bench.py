@@ -98,6 +98,9 @@ def main():
   ap.add_argument("--cpu-seconds", type=float, default=12.0)
   ap.add_argument("--no-cpu", action="store_true")
   ap.add_argument("--no-plain", action="store_true")
+  ap.add_argument("--gram", default="fp32", choices=["valu", "fp32", "bf16x3"],
+                  help="static-kernel Gram evaluation inside the sweep")
+  ap.add_argument("--no-variants", action="store_true")
   args = ap.parse_args()
   if args.gpus != 1:
     print(f"bench.py measures one GPU only (the signature-kernel path shards by "
@@ -121,8 +124,10 @@ def main():
   cells = n_pairs * (L - 1) ** 2
   cell_levels = cells * n_levels
 
-  def step():
-    return _ops.signature_levels(Xs, Xs, pairs, n_levels, 1, True, _lib.STATIC_RBF, sigma)
+  mode = _lib.GRAM_MODES[args.gram]
+
+  def step(m=mode):
+    return _ops.signature_levels(Xs, Xs, pairs, n_levels, 1, True, _lib.STATIC_RBF, sigma, m)
 
   for _ in range(args.warmup):
     step()
@@ -150,6 +155,27 @@ def main():
                   for a, b in pr])
   err = float(np.max(np.abs(got - ref).max(1) / np.abs(ref).max(1)))
 
+  def time_mode(m, reps=5):
+    step(m)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+      o = step(m)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    gotm = o.cpu().numpy()[idx]
+    errm = float(np.max(np.abs(gotm - ref).max(1) / np.abs(ref).max(1)))
+    return e0.elapsed_time(e1) / reps, errm
+
+  variants = {}
+  if not args.no_variants:
+    for name in ("valu", "fp32", "bf16x3"):
+      ms, e = time_mode(_lib.GRAM_MODES[name])
+      variants[name] = {"kernel_ms": ms, "cell_levels_per_s": cell_levels / (ms * 1e-3),
+                        "max_rel_err_vs_fp64": e}
+
   flops_cell = 4 * n_levels + 2 * d   # recursion + static Gram
   achieved = cells * flops_cell / (kern_ms * 1e-3) / 1e12
   res = {
@@ -167,9 +193,12 @@ def main():
                  "full_gram_equivalent_cell_levels_per_s":
                      n * n * (L - 1) ** 2 * n_levels / (ms_per_step * 1e-3)},
       "max_rel_err_vs_fp64": err,
+      "gram": args.gram,
+      "gram_variants": variants,
       "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                    "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
-                   "traffic": None, "kernel": "dp_forward_kernel<float,5,1,8,false>",
+                   "traffic": None, "kernel": ("dp_mfma_kernel<5,1,8,%d>" % (0 if args.gram == "fp32" else 1)
+                              if args.gram != "valu" else "dp_forward_kernel<float,5,1,8,false>"),
                    "kernel_ms": kern_ms,
                    "flops_per_cell": flops_cell},
   }

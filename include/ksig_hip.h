@@ -30,6 +30,12 @@ extern "C" {
 #define KSIG_STATIC_RBF 0    /* ksig/static/kernels.py:245-260, param = bandwidth */
 #define KSIG_STATIC_LINEAR 1 /* ksig/static/kernels.py:134-166, param = scale */
 
+/* static-kernel Gram evaluation inside the DP sweep (float32 inputs) */
+#define KSIG_GRAM_AUTO -1         /* MFMA float32 when available, else VALU */
+#define KSIG_GRAM_VALU 0          /* one float32 (or float64) VALU dot product per cell */
+#define KSIG_GRAM_MFMA_F32 1      /* v_mfma_f32_16x16x4f32 tiles */
+#define KSIG_GRAM_MFMA_BF16X3 2   /* bf16 hi/lo split, 3 products, v_mfma_f32_16x16x32_bf16 */
+
 /* error codes (besides hipError_t values) */
 #define KSIG_EINVAL 1001      /* bad argument (shape, dtype, kind, null pointer) */
 #define KSIG_EUNSUPPORTED 1002 /* (n_levels, order) or d outside the built set */
@@ -57,15 +63,17 @@ int ksig_dp_supported(int n_levels, int order);
  *   with K_0 = 1, as float64.
  *
  * max_len_x / max_len_y bound the sequence lengths (they size LDS).
- * dtype selects float32 or float64 inputs and arithmetic.
+ * dtype selects float32 or float64 inputs and arithmetic; gram_mode selects
+ * how the static-kernel values are produced (KSIG_GRAM_*; float64 and
+ * difference=0 always use KSIG_GRAM_VALU).
  */
 int ksig_signature_levels(int dtype, const void* x, const int64_t* x_off,
                           const int32_t* x_len, int max_len_x, const void* y,
                           const int64_t* y_off, const int32_t* y_len,
                           int max_len_y, int d, const int32_t* pairs,
                           int64_t n_pairs, int static_kind, double static_param,
-                          int n_levels, int order, int difference, double* out,
-                          void* stream);
+                          int n_levels, int order, int difference, int gram_mode,
+                          double* out, void* stream);
 
 /*
  * Training forward: ksig_signature_levels for order 1 with difference, which

@@ -17,6 +17,9 @@ _lib = None
 
 F32, F64 = 0, 1
 STATIC_RBF, STATIC_LINEAR = 0, 1
+GRAM_AUTO, GRAM_VALU, GRAM_MFMA_F32, GRAM_MFMA_BF16X3 = -1, 0, 1, 2
+GRAM_MODES = {'auto': GRAM_AUTO, 'valu': GRAM_VALU, 'fp32': GRAM_MFMA_F32,
+              'bf16x3': GRAM_MFMA_BF16X3}
 
 _c_int, _c_i64, _c_dbl, _vp = ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
 
@@ -27,7 +30,7 @@ SIGNATURES = {
     "ksig_dp_supported": (_c_int, [_c_int, _c_int]),
     "ksig_signature_levels": (_c_int, [
         _c_int, _vp, _vp, _vp, _c_int, _vp, _vp, _vp, _c_int, _c_int, _vp,
-        _c_i64, _c_int, _c_dbl, _c_int, _c_int, _c_int, _vp, _vp]),
+        _c_i64, _c_int, _c_dbl, _c_int, _c_int, _c_int, _c_int, _vp, _vp]),
     "ksig_signature_levels_fwd_save": (_c_int, [
         _c_int, _vp, _vp, _vp, _c_int, _vp, _vp, _vp, _c_int, _c_int, _vp,
         _c_i64, _c_int, _c_dbl, _c_int, _vp, _vp, _vp, _vp]),

@@ -95,8 +95,8 @@ def _check_sets(X: SeqSet, Y: SeqSet):
 
 
 def signature_levels(X: SeqSet, Y: SeqSet, pairs: torch.Tensor, n_levels: int,
-                     order: int, difference: bool, kind: int, param: float
-                     ) -> torch.Tensor:
+                     order: int, difference: bool, kind: int, param: float,
+                     gram_mode: int = -1) -> torch.Tensor:
   """Level values [n_pairs, n_levels+1] (float64) of the fused DP kernel."""
   lib = _lib.load()
   _check_sets(X, Y)
@@ -108,8 +108,8 @@ def signature_levels(X: SeqSet, Y: SeqSet, pairs: torch.Tensor, n_levels: int,
       _lib.dtype_code(X.data.dtype), X.data.data_ptr(), X.off.data_ptr(),
       X.len.data_ptr(), X.max_len, Y.data.data_ptr(), Y.off.data_ptr(),
       Y.len.data_ptr(), Y.max_len, X.d, pairs.data_ptr(), n_pairs, kind,
-      float(param), n_levels, order, int(bool(difference)), out.data_ptr(),
-      _lib.stream_handle(X.data.device)))
+      float(param), n_levels, order, int(bool(difference)), int(gram_mode),
+      out.data_ptr(), _lib.stream_handle(X.data.device)))
   return out
 
 

@@ -93,10 +93,16 @@ struct PdeBwdArgs {
 };
 
 int dp_flow_count(int nlev, int order);
+// fused VALU sweep for difference=1 (dp_sweep.h): LDS bytes for a column side of `cols`
+size_t dp_sweep_lds(int dtype, int d, int cols, int n_levels, int order);
+hipError_t dp_sweep_launch(const DpFwdArgs& a, int dtype, size_t lds, hipStream_t s);
 bool dp_supported(int n_levels, int order);
 hipError_t dp_forward_launch(const DpFwdArgs& a, int dtype, size_t lds, hipStream_t s);
 hipError_t dp_backward_launch(const DpBwdArgs& a, int dtype, size_t lds, hipStream_t s);
 bool dp_backward_supported(int n_levels);
+bool dp_mfma_supported(int n_levels, int order);
+// mode 1: float32 MFMA Gram, mode 2: bf16 hi/lo split MFMA Gram
+hipError_t dp_mfma_launch(const DpFwdArgs& a, int mode, size_t lds, hipStream_t s);
 hipError_t pde_forward_launch(const PdeFwdArgs& a, int dtype, size_t lds, hipStream_t s);
 hipError_t pde_backward_launch(const PdeBwdArgs& a, int dtype, size_t lds, hipStream_t s);
 

@@ -106,8 +106,8 @@ int ksig_signature_levels(int dtype, const void* x, const int64_t* x_off,
                           const int64_t* y_off, const int32_t* y_len,
                           int max_len_y, int d, const int32_t* pairs,
                           int64_t n_pairs, int static_kind, double static_param,
-                          int n_levels, int order, int difference, double* out,
-                          void* stream) {
+                          int n_levels, int order, int difference, int gram_mode,
+                          double* out, void* stream) {
   int rc = check_common(dtype, x, y, d, pairs, n_pairs, out);
   if (rc) return rc;
   if (n_levels < 1) return fail(KSIG_EINVAL, "n_levels must be positive");
@@ -129,8 +129,37 @@ int ksig_signature_levels(int dtype, const void* x, const int64_t* x_off,
   a.order = order;
   a.difference = difference ? 1 : 0;
   a.out = out;
-  const int cols = max_len_x < max_len_y ? max_len_x : max_len_y;
-  return dp_levels_common(dtype, a, cols < 1 ? 1 : cols, dmax_for(d), stream);
+  int cols = max_len_x < max_len_y ? max_len_x : max_len_y;
+  if (cols < 1) cols = 1;
+  if (gram_mode < KSIG_GRAM_AUTO || gram_mode > KSIG_GRAM_MFMA_BF16X3)
+    return fail(KSIG_EINVAL, "bad gram_mode %d", gram_mode);
+  int mode = gram_mode;
+  const bool mfma_ok = dtype == KSIG_F32 && a.difference && ksig::dp_mfma_supported(n_levels, order);
+  if (mode == KSIG_GRAM_AUTO) mode = mfma_ok ? KSIG_GRAM_MFMA_F32 : KSIG_GRAM_VALU;
+  if (mode != KSIG_GRAM_VALU) {
+    if (!mfma_ok)
+      return fail(KSIG_EUNSUPPORTED, "MFMA Gram needs float32, difference=1 and a built "
+                  "(n_levels=%d, order=%d)", n_levels, order);
+    const int nf = ksig::dp_flow_count(n_levels, order);
+    int pc = cols > 80 ? cols : 80;
+    pc = (pc + 15) & ~15;
+    const int dmax = dmax_for(d);
+    const size_t lds = ((size_t)80 * 65 + (size_t)ksig::kWave * (dmax + 1) + (size_t)pc * nf) * 4;
+    if (lds > max_lds_bytes())
+      return fail(KSIG_ELDS, "per-pair LDS state of %zu bytes exceeds the device limit", lds);
+    hipError_t e = ksig::dp_mfma_launch(a, mode, lds, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "dp_mfma launch");
+    return 0;
+  }
+  if (a.difference) {
+    const size_t lds = ksig::dp_sweep_lds(dtype, d, cols, n_levels, order);
+    if (lds > max_lds_bytes())
+      return fail(KSIG_ELDS, "per-pair LDS state of %zu bytes exceeds the device limit", lds);
+    hipError_t e = ksig::dp_sweep_launch(a, dtype, lds, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "dp_sweep launch");
+    return 0;
+  }
+  return dp_levels_common(dtype, a, cols, dmax_for(d), stream);
 }
 
 int ksig_signature_levels_fwd_save(int dtype, const void* x, const int64_t* x_off,

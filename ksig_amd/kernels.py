@@ -17,7 +17,7 @@ from typing import Optional
 
 import torch
 
-from . import _ops, algorithms, static, utils
+from . import _lib, _ops, algorithms, static, utils
 from .static.kernels import Kernel, StaticKernel
 from .utils import _EPS
 
@@ -88,10 +88,19 @@ class SignatureKernel(SignatureBase):
 
   def __init__(self, n_levels: int = 5, order: int = 1, difference: bool = True,
                normalize: bool = True, n_features: Optional[int] = None,
-               static_kernel: Optional[StaticKernel] = None):
+               static_kernel: Optional[StaticKernel] = None,
+               gram_precision: str = 'auto'):
+    """``gram_precision`` (no reference counterpart) selects how float32
+    static-kernel values are produced inside the sweep: 'auto' (matrix-core
+    float32 when available), 'valu' (one float32 dot product per cell on the
+    vector units), 'fp32' (float32 MFMA tiles), 'bf16x3' (bf16 hi/lo split,
+    three MFMA products).  float64 inputs always use exact float64 arithmetic."""
     super().__init__(n_levels=n_levels, order=order, difference=difference,
                      normalize=normalize, n_features=n_features)
     self.static_kernel = static_kernel or static.kernels.RBFKernel()
+    if gram_precision not in _lib.GRAM_MODES:
+      raise ValueError(f'gram_precision must be one of {sorted(_lib.GRAM_MODES)}')
+    self.gram_precision = gram_precision
 
   # -- embedding (generic static kernels only) --------------------------------
   def _compute_embedding(self, X, Y=None, diag: bool = False) -> torch.Tensor:
@@ -128,7 +137,8 @@ class SignatureKernel(SignatureBase):
             xdata=X.reshape(-1, X.shape[-1]),
             ydata=(X if (Y is None or diag) else Y).reshape(-1, X.shape[-1]))
       return _ops.signature_levels(Xs, Ys, pairs, self.n_levels, self.order,
-                                   self.difference, kind, param)
+                                   self.difference, kind, param,
+                                   _lib.GRAM_MODES[getattr(self, 'gram_precision', 'auto')])
 
     if diag:
       return levels(_ops.diag_pairs(X.shape[0], dev)).t()
