@@ -76,6 +76,39 @@ int ksig_signature_levels(int dtype, const void* x, const int64_t* x_off,
                           double* out, void* stream);
 
 /*
+ * Resident-wave capacity of the row-stream kernel behind
+ * ksig_signature_levels_batched for a problem shape on the current device:
+ * *slots = waves that run concurrently (0 if no row-stream kernel or LDS
+ * layout fits), *max_pairs_per_wave = the largest slice one wave accepts.
+ * A scheduling query with no reference counterpart (the reference launches
+ * one CuPy kernel sequence per Gram, ksig/kernels.py:147-164).
+ */
+int ksig_stream_slots(int d, int static_kind, int n_levels, int order, int max_len_x,
+                      int max_len_y, int* slots, int* max_pairs_per_wave);
+
+/*
+ * ksig_signature_levels for float32 inputs with difference=1 and a caller
+ * schedule, in one persistent launch: wave w evaluates the pairs
+ * sched_order[wave_beg[w] .. wave_beg[w+1]) (sched_order null: the identity),
+ * at most *max_pairs_per_wave of ksig_stream_slots per wave, writing out[q]
+ * for each pair index q it takes.  A wave stacks its pairs' rows and sweeps
+ * them with one column count (the longest shorter side among them), so
+ * slices of similar lengths waste least.  wave_beg null: equal slices sized
+ * from the device's resident-wave count.  Replaces the same reference
+ * composition as ksig_signature_levels (ksig/kernels.py:147-164 over
+ * ksig/algorithms.py:22-140) for pair lists the reference cannot take
+ * (uneven lengths; the reference needs one dense [n, L, d] array,
+ * ksig/kernels.py:87).
+ */
+int ksig_signature_levels_batched(const void* x, const int64_t* x_off, const int32_t* x_len,
+                                  int max_len_x, const void* y, const int64_t* y_off,
+                                  const int32_t* y_len, int max_len_y, int d,
+                                  const int32_t* pairs, int64_t n_pairs, int static_kind,
+                                  double static_param, int n_levels, int order,
+                                  const int32_t* sched_order, const int32_t* wave_beg,
+                                  int n_waves, double* out, void* stream);
+
+/*
  * Training forward: ksig_signature_levels for order 1 with difference, which
  * also writes the boundary state the adjoint needs.  For pair p, save holds
  * (n_levels-1) * (len_x + len_y - 2) values of dtype at element offset

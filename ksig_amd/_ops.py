@@ -7,9 +7,11 @@ is computed on the host.
 
 from __future__ import annotations
 
-from dataclasses import dataclass
+import ctypes
+from dataclasses import dataclass, field
 from typing import Optional, Sequence
 
+import numpy as np
 import torch
 
 from . import _lib
@@ -23,6 +25,7 @@ class SeqSet:
   len: torch.Tensor      # int32 [n]
   max_len: int
   n: int
+  host_len: Optional[np.ndarray] = field(default=None, repr=False)  # ragged sets only
 
   @property
   def d(self) -> int:
@@ -49,7 +52,18 @@ class SeqSet:
       offs.append(offs[-1] + L)
     off = torch.tensor(offs, device=device, dtype=torch.int64)
     lens = torch.tensor(lens_list, device=device, dtype=torch.int32)
-    return SeqSet(data, off, lens, max(lens_list) if lens_list else 0, len(seqs))
+    return SeqSet(data, off, lens, max(lens_list) if lens_list else 0, len(seqs),
+                  np.asarray(lens_list, dtype=np.int64))
+
+  def lengths(self) -> np.ndarray:
+    """Sequence lengths on the host."""
+    if self.host_len is not None:
+      return self.host_len
+    return np.full(self.n, self.max_len, dtype=np.int64)
+
+  @property
+  def uniform(self) -> bool:
+    return self.host_len is None or bool(np.all(self.host_len == self.max_len))
 
 
 def _scalar(p) -> float:
@@ -101,6 +115,15 @@ def signature_levels(X: SeqSet, Y: SeqSet, pairs: torch.Tensor, n_levels: int,
   lib = _lib.load()
   _check_sets(X, Y)
   n_pairs = pairs.shape[0]
+  if (n_pairs > 0 and X.data.dtype == torch.float32 and difference and gram_mode in (-1, 0)
+      and kind in (_lib.STATIC_RBF, _lib.STATIC_LINEAR) and not (X.uniform and Y.uniform)):
+    # uneven lengths: length-bucketed schedule for the row stream
+    try:
+      sched = make_schedule(X, Y, pairs, kind, n_levels, order)
+    except NotImplementedError:
+      sched = None
+    if sched is not None:
+      return signature_levels_batched(X, Y, pairs, n_levels, order, kind, param, sched)
   out = torch.empty((n_pairs, n_levels + 1), device=X.data.device, dtype=torch.float64)
   if n_pairs == 0:
     return out
@@ -111,6 +134,98 @@ def signature_levels(X: SeqSet, Y: SeqSet, pairs: torch.Tensor, n_levels: int,
       float(param), n_levels, order, int(bool(difference)), int(gram_mode),
       out.data_ptr(), _lib.stream_handle(X.data.device)))
   return out
+
+
+def stream_slots(X: SeqSet, Y: SeqSet, kind: int, n_levels: int, order: int):
+  """(resident waves, max pairs per wave) of the row-stream kernel for this
+  problem shape; resident waves is 0 when no row-stream kernel fits."""
+  lib = _lib.load()
+  s, m = ctypes.c_int(0), ctypes.c_int(0)
+  _lib.check(lib.ksig_stream_slots(X.d, kind, n_levels, order, X.max_len, Y.max_len,
+                                   ctypes.byref(s), ctypes.byref(m)))
+  return s.value, m.value
+
+
+def stream_schedule(lx: np.ndarray, ly: np.ndarray, slots: int, max_per_wave: int,
+                    min_cols: int = 68):
+  """Length-bucketed schedule of a pair list for the row stream (host side).
+
+  Pairs are ordered by their shorter side, longest first (a wave sweeps all
+  its pairs with one column count), then cut into contiguous slices of about
+  equal work (rows x columns), one slice per resident wave and round, each
+  slice at most ``max_per_wave`` pairs.  Returns (order, wave_beg) as int32
+  numpy arrays."""
+  lx = np.asarray(lx, dtype=np.int64)
+  ly = np.asarray(ly, dtype=np.int64)
+  n = lx.shape[0]
+  cols = np.minimum(lx, ly)
+  rows = np.maximum(lx, ly)
+  order = np.argsort(-cols, kind="stable")
+  live = cols[order] >= 2
+  cost = np.where(live, rows[order] * np.maximum(cols[order], min_cols), 0).astype(np.float64)
+  slots = max(int(slots), 1)
+  rounds = max(1, -(-n // (slots * max_per_wave)))
+  n_target = min(n, slots * rounds)
+  cum = np.cumsum(cost)
+  total = cum[-1] if n else 0.0
+  if total > 0 and n_target > 1:
+    targets = total * np.arange(1, n_target) / n_target
+    b = np.searchsorted(cum, targets, side="left") + 1
+  else:
+    b = np.zeros(0, dtype=np.int64)
+  b = np.unique(np.concatenate([[0], np.clip(b, 0, n), [n]]))
+  sizes = np.diff(b)
+  parts = np.maximum(1, -(-sizes // max_per_wave))
+  base = np.repeat(b[:-1], parts)
+  size_r = np.repeat(sizes, parts)
+  part_r = np.repeat(parts, parts)
+  t = np.arange(parts.sum()) - np.repeat(np.cumsum(parts) - parts, parts)
+  starts = base + (t * size_r) // part_r
+  wave_beg = np.concatenate([starts, [n]]).astype(np.int32)
+  return order.astype(np.int32), wave_beg
+
+
+def signature_levels_batched(X: SeqSet, Y: SeqSet, pairs: torch.Tensor, n_levels: int,
+                             order: int, kind: int, param: float,
+                             schedule: Optional[tuple] = None) -> torch.Tensor:
+  """Level values [n_pairs, n_levels+1] of float32 pairs of uneven lengths in
+  one persistent row-stream launch.  ``schedule`` = (order, wave_beg) device
+  int32 tensors, or None for the length-bucketed schedule of
+  ``stream_schedule``."""
+  lib = _lib.load()
+  _check_sets(X, Y)
+  if X.data.dtype != torch.float32:
+    raise TypeError("the batched row stream takes float32 inputs")
+  n_pairs = pairs.shape[0]
+  dev = X.data.device
+  out = torch.empty((n_pairs, n_levels + 1), device=dev, dtype=torch.float64)
+  if n_pairs == 0:
+    return out
+  if schedule is None:
+    schedule = make_schedule(X, Y, pairs, kind, n_levels, order)
+  so, wb = schedule
+  _lib.check(lib.ksig_signature_levels_batched(
+      X.data.data_ptr(), X.off.data_ptr(), X.len.data_ptr(), X.max_len,
+      Y.data.data_ptr(), Y.off.data_ptr(), Y.len.data_ptr(), Y.max_len, X.d,
+      pairs.data_ptr(), n_pairs, kind, float(param), n_levels, order,
+      so.data_ptr() if so is not None else None, wb.data_ptr(), wb.shape[0] - 1,
+      out.data_ptr(), _lib.stream_handle(dev)))
+  return out
+
+
+def make_schedule(X: SeqSet, Y: SeqSet, pairs: torch.Tensor, kind: int, n_levels: int,
+                  order: int):
+  """Device (order, wave_beg) tensors of the length-bucketed schedule."""
+  norm_order = n_levels if (order <= 0 or order >= n_levels) else order
+  slots, maxp = stream_slots(X, Y, kind, n_levels, norm_order)
+  if slots <= 0:
+    raise NotImplementedError("no row-stream kernel fits this problem shape")
+  pr = pairs.detach().cpu().numpy()
+  lx = X.lengths()[pr[:, 0]]
+  ly = Y.lengths()[pr[:, 1]]
+  so, wb = stream_schedule(lx, ly, slots, maxp)
+  dev = X.data.device
+  return (torch.as_tensor(so, device=dev), torch.as_tensor(wb, device=dev))
 
 
 def signature_levels_gram(M: torch.Tensor, n_levels: int, order: int,

@@ -25,7 +25,7 @@ size_t dp_sweep_lds(int dtype, int d, int cols, int n_levels, int order) {
   const int quad = (int)(16 / esz);
   const int rs = ((dmax / quad) % 2 == 0) ? dmax + quad : dmax + 2 * quad;
   const int pc = cols > kWave ? cols : kWave + 1;
-  const int rows_per_band = kWave * (dtype == KSIG_F32 ? 2 : 1);
+  const int rows_per_band = kWave * (dtype == KSIG_F32 ? 4 : 1);
   return ((size_t)rows_per_band * rs + (size_t)cols * rs +
           (size_t)pc * dp_flow_count(n_levels, order)) * esz;
 }

@@ -42,6 +42,29 @@ struct DpFwdArgs {
   const int64_t* save_off;
 };
 
+// Row-stream forward (dp_stream.h): each wave takes a contiguous slice of the
+// processing order, [wave_beg[w], wave_beg[w+1]) or [w*per_wave, ...) when
+// wave_beg is null, at most kStreamMaxPairs pairs.
+struct DpStreamArgs {
+  SeqSet x, y;
+  int d;
+  int vec;                 // 1: d == DMAX and both data pointers 16-byte aligned
+  const int32_t* pairs;    // [n_pairs, 2]
+  int64_t n_pairs;
+  const int32_t* order;    // processing order of the pair list (null: identity)
+  const int32_t* wave_beg; // [n_waves + 1] or null
+  int n_waves;
+  int per_wave;
+  float gdot, gn;          // static-kernel factors folded into the row side
+  double* out;             // [n_pairs, n_levels + 1]
+};
+
+// A compiled row-stream kernel: entry point and rows per lane.
+struct StreamEntry {
+  const void* fn;
+  int r;
+};
+
 struct DpBwdArgs {
   SeqSet x, y;
   int d;
@@ -103,6 +126,10 @@ bool dp_backward_supported(int n_levels);
 bool dp_mfma_supported(int n_levels, int order);
 // mode 1: float32 MFMA Gram, mode 2: bf16 hi/lo split MFMA Gram
 hipError_t dp_mfma_launch(const DpFwdArgs& a, int mode, size_t lds, hipStream_t s);
+StreamEntry dp_stream_entry(int kind, int dmax, int n_levels, int order);
+hipError_t dp_stream_launch(const StreamEntry& e, const DpStreamArgs& a, size_t lds,
+                            hipStream_t s);
+int dp_stream_occupancy(const StreamEntry& e, size_t lds);
 hipError_t pde_forward_launch(const PdeFwdArgs& a, int dtype, size_t lds, hipStream_t s);
 hipError_t pde_backward_launch(const PdeBwdArgs& a, int dtype, size_t lds, hipStream_t s);
 

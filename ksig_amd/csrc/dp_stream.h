@@ -1,0 +1,424 @@
+// Truncated signature kernel by dynamic programming, forward pass, float32,
+// fused RBF / linear static kernel on the vector ALUs, difference=True:
+// the row-stream sweep (gfx950).  Template implementation; instantiated by
+// dp_stream_*.hip.  This is the production path for float32 inputs.
+//
+// Recursion: ksig/algorithms.py:47-85 (first order) and :88-140 (higher
+// order) on M = ΔΔ k(x_i, y_j) (ksig/algorithms.py:64), k from
+// ksig/static/kernels.py:248-260 (RBF) or :145-155 (linear).
+//
+// One wavefront owns a list of pairs and lays their rows end to end:
+//   pair k occupies stream rows [start_k, start_k + rows_k), where row 0 of a
+//   pair is its priming row (the first sequence point, the grid row above the
+//   ΔΔ grid) and rows 1..Nr are the ΔΔ rows; rows_k is padded up to a multiple
+//   of R with copies of the last row (their ΔΔ is exactly zero).
+// Lane i owns R consecutive stream rows of every 64R-row band and walks the
+// band's columns one step behind lane i-1 (the skewed systolic wave of
+// dp_sweep.h); the column prefix sums of the recursion flow from lane to
+// lane by DPP and from lane 63 to lane 0 of the next band through an LDS FIFO.
+// A lane whose rows start a pair ignores what flows in (the previous pair's
+// column sums); a lane whose rows end a pair adds the column sums leaving it
+// into per-lane FP64 accumulators and writes the pair's level sums when its
+// band pass ends.  Consequences:
+//   * no lane idles between pairs: a wave pays the 63-step pipeline fill once
+//     per pair list, not once per pair, and rows are padded to R, not to 64;
+//   * a band pass has one column count Pc for the whole wave (the longest
+//     column side in the list; shorter pairs replicate their last column,
+//     whose ΔΔ is zero), so pair lists are best made of similar lengths;
+//   * the column sequence is read from global memory (L1/L2 resident: the 64
+//     lanes of a wave touch the same 64-row window), two steps ahead in
+//     registers, so LDS holds only the pair table, each lane's next-band x rows
+//     and the FIFO, and occupancy does not depend on the sequence length
+//     beyond the FIFO.
+//
+// Static kernel with all scale factors folded into the row side:
+//   RBF:    G = exp2(<gdot x, y> - gn |x|^2 - gn |y|^2),  gdot = 2 gn,
+//           gn = log2(e) / max(2 sigma^2, 1e-12)
+//   linear: G = <gdot x, y>,                              gdot = scale
+#pragma once
+
+#include "ksig_common.h"
+#include "ksig_hip.h"
+#include "dp_forward.h"
+#include "dp_forward_impl.h"
+#include "dp_sweep.h"
+
+namespace ksig {
+
+// Row-stream pair table entry (LDS): rows = the longer sequence.
+struct StreamPair {
+  int32_t q;      // index into the caller's pair list (output row)
+  int32_t lr1;    // row sequence length - 1 (= Nr)
+  int32_t lc1;    // column sequence length - 1 (= Nc)
+  int32_t swap;   // 1: rows come from the y set
+  int64_t roff;   // element offset of the row sequence in its set
+  int64_t coff;   // element offset of the column sequence in its set
+};
+
+template <int DMAX>
+struct StreamLayout {
+  // Staged x row (lane-private): DMAX features then the norm term, padded to
+  // an odd number of 16-byte quads.
+  static constexpr int XS = ((DMAX / 4) % 2 == 0) ? DMAX + 4 : DMAX + 8;
+};
+
+constexpr int kStreamMaxPairs = 128;  // pair-table entries per wave
+constexpr int kStreamMinCols = 68;    // smallest band width Pc (FIFO hazard + staging)
+
+__host__ __device__ inline size_t stream_lds_bytes(int dmax, int r, int pc, int nf) {
+  const int xs = ((dmax / 4) % 2 == 0) ? dmax + 4 : dmax + 8;
+  return sizeof(StreamPair) * kStreamMaxPairs + 4 * (kStreamMaxPairs + 4) +
+         (size_t)4 * kWave * r * xs + (size_t)4 * pc * nf;
+}
+
+// Load one sequence point (d features) into v[0..DMAX): 16-byte vector loads
+// when vec (d == DMAX and 16-byte aligned data), predicated dword loads
+// otherwise.
+template <int DMAX>
+__device__ __forceinline__ void stream_load_row(float* v, const float* p, int d, bool vec) {
+  if (vec) {
+#pragma unroll
+    for (int f = 0; f < DMAX; f += 4) {
+      const float4 t = *reinterpret_cast<const float4*>(p + f);
+      v[f] = t.x; v[f + 1] = t.y; v[f + 2] = t.z; v[f + 3] = t.w;
+    }
+  } else {
+#pragma unroll
+    for (int f = 0; f < DMAX; ++f) v[f] = f < d ? p[f] : 0.0f;
+  }
+}
+
+template <int NLEV, int ORDER, int DMAX, int R, int KIND>
+__global__ void __launch_bounds__(kWave)
+dp_stream_kernel(const DpStreamArgs a) {
+  using T = float;
+  constexpr int NSIDE = dp_num_side(NLEV, ORDER);
+  constexpr int NF = 1 + NLEV + NSIDE;
+  constexpr int NPS = NLEV > 1 ? NLEV - 1 : 1;
+  constexpr int NHB = NSIDE > 0 ? NSIDE : 1;
+  constexpr int XS = StreamLayout<DMAX>::XS;
+  constexpr int RB = kWave * R;
+  constexpr bool RBF = KIND == kRBF;
+
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  StreamPair* tab = reinterpret_cast<StreamPair*>(smem_raw);
+  int32_t* tstart = reinterpret_cast<int32_t*>(tab + kStreamMaxPairs);
+  T* xst = reinterpret_cast<T*>(tstart + kStreamMaxPairs + 4);
+  T* fifo = xst + RB * XS;
+
+  const int lane = threadIdx.x;
+  const int w = blockIdx.x;
+  const int d = a.d;
+  const bool vec = a.vec != 0;
+  int kb, ke;
+  if (a.wave_beg) {
+    kb = a.wave_beg[w];
+    ke = a.wave_beg[w + 1];
+  } else {
+    kb = w * a.per_wave;
+    ke = kb + a.per_wave;
+    ke = ke < a.n_pairs ? ke : (int)a.n_pairs;
+  }
+  int nk = ke - kb;
+  if (nk <= 0) return;
+  nk = nk < kStreamMaxPairs ? nk : kStreamMaxPairs;  // the host keeps slices within the table
+
+  // ---- pair table: orientation, padded row counts, exclusive row starts ----
+  int pcl = 0;
+  int carry = 0;
+  for (int k0 = 0; k0 < nk; k0 += kWave) {
+    const int k = k0 + lane;
+    int rows = 0;
+    if (k < nk) {
+      const int q = a.order ? a.order[kb + k] : kb + k;
+      const int ia = a.pairs[2 * q], ib = a.pairs[2 * q + 1];
+      const int lx = a.x.len[ia], ly = a.y.len[ib];
+      const bool sw = ly > lx;
+      const int lr = sw ? ly : lx, lc = sw ? lx : ly;
+      const int64_t xo = a.x.off[ia] * d, yo = a.y.off[ib] * d;
+      StreamPair e;
+      e.q = q;
+      e.lr1 = lr - 1;
+      e.lc1 = lc - 1;
+      e.swap = sw ? 1 : 0;
+      e.roff = sw ? yo : xo;
+      e.coff = sw ? xo : yo;
+      tab[k] = e;
+      if (lr >= 2 && lc >= 2) {
+        rows = (lr + R - 1) / R * R;
+        pcl = pcl > lc ? pcl : lc;
+      } else {  // empty ΔΔ grid: K = (1, 0, ..., 0)
+        double* o = a.out + (int64_t)q * (NLEV + 1);
+        o[0] = 1.0;
+        for (int m = 1; m <= NLEV; ++m) o[m] = 0.0;
+      }
+    }
+    int incl = rows;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+      const int t = __shfl_up(incl, o, kWave);
+      if (lane >= o) incl += t;
+    }
+    if (k < nk) tstart[k] = carry + incl - rows;
+    carry += __shfl(incl, kWave - 1, kWave);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const int t = __shfl_xor(pcl, o, kWave);
+    pcl = pcl > t ? pcl : t;
+  }
+  const int total_rows = carry;
+  if (lane == 0) tstart[nk] = total_rows;
+  __syncthreads();
+  if (total_rows == 0) return;
+  const int Pc = pcl > kStreamMinCols ? pcl : kStreamMinCols;
+  const int nb = (total_rows + RB - 1) / RB;
+
+  const T gdot = a.gdot, gn = a.gn;
+  const T* xdata = reinterpret_cast<const T*>(a.x.data);
+  const T* ydata = reinterpret_cast<const T*>(a.y.data);
+
+  // ---- per-lane band state ----
+  // current band: column sequence, its last column, output row, flags
+  const T* yptr;
+  int ccl, q;
+  bool prim, last;
+  // next band (written by stage(), taken over at the lane's band change)
+  const T* pyptr = nullptr;
+  int pccl = 0, pq = 0;
+  bool pprim = true, plast = false;
+
+  // Stage band `band` for this lane: find the pair of its rows, write its x
+  // rows (scaled, with the norm term) to the lane's private LDS slot and its
+  // metadata to the pending registers.
+  auto stage = [&](int band) {
+    const int g0 = band * RB + R * lane;
+    const bool live = g0 < total_rows;
+    const int gs = live ? g0 : total_rows - 1;
+    int lo = 0, hi = nk - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (tstart[mid] <= gs) lo = mid; else hi = mid - 1;
+    }
+    const StreamPair e = tab[lo];
+    const int r0 = g0 - tstart[lo];
+    const int rows_k = tstart[lo + 1] - tstart[lo];
+    pprim = !live || r0 == 0;
+    plast = live && r0 + R >= rows_k;
+    pq = e.q;
+    pccl = e.lc1;
+    pyptr = (e.swap ? xdata : ydata) + e.coff;
+    const T* rb = (e.swap ? ydata : xdata) + e.roff;
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      int r = r0 + j;
+      r = r < e.lr1 ? r : e.lr1;
+      T v[DMAX];
+      stream_load_row<DMAX>(v, rb + (int64_t)r * d, d, vec);
+      T n2 = T(0);
+      T* dst = xst + (R * lane + j) * XS;
+#pragma unroll
+      for (int f = 0; f < DMAX; ++f) {
+        n2 += v[f] * v[f];
+        dst[f] = gdot * v[f];
+      }
+      dst[DMAX] = RBF ? gn * n2 : T(0);
+    }
+  };
+
+  T xr[R][DMAX];
+  T xn[R];
+  auto enter_band = [&]() {
+    yptr = pyptr;
+    ccl = pccl;
+    q = pq;
+    prim = pprim;
+    last = plast;
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      const T* src = xst + (R * lane + j) * XS;
+#pragma unroll
+      for (int f = 0; f < DMAX; ++f) xr[j][f] = src[f];
+      xn[j] = src[DMAX];
+    }
+  };
+
+  stage(0);
+  enter_band();
+
+  T fl[NF];
+#pragma unroll
+  for (int k = 0; k < NF; ++k) fl[k] = T(0);
+  T PS[R][NPS];
+  T HB[R][NHB];
+  T gleft[R];
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+#pragma unroll
+    for (int m = 0; m < NPS; ++m) PS[j][m] = T(0);
+#pragma unroll
+    for (int m = 0; m < NHB; ++m) HB[j][m] = T(0);
+    gleft[j] = T(0);
+  }
+  T gdiag = T(0);
+  T accf[NLEV];
+  double acc[NLEV];
+#pragma unroll
+  for (int m = 0; m < NLEV; ++m) {
+    accf[m] = T(0);
+    acc[m] = 0.0;
+  }
+
+  int cp = -lane;             // this lane's column in its band (< 0: not started)
+  const int total = nb * Pc + kWave - 1;
+  int s0 = 0;                 // lane 0's FIFO slot (= its column)
+  int s63 = Pc - (kWave - 1); // lane 63's FIFO slot
+  int next_stage = kWave - 1; // band b+1 is staged at step b*Pc + 63
+  int stage_band = 1;
+
+  // y row of column c2 of this lane (c2 >= Pc: the next band's sequence)
+  auto yload = [&](T* dst, int c2) {
+    const bool nx = c2 >= Pc;
+    const T* p = nx ? pyptr : yptr;
+    int cc = nx ? c2 - Pc : c2;
+    cc = cc > 0 ? cc : 0;
+    const int cl = nx ? pccl : ccl;
+    cc = cc < cl ? cc : cl;
+    stream_load_row<DMAX>(dst, p + (int64_t)cc * d, d, vec);
+  };
+
+  T y0[DMAX], y1[DMAX], y2[DMAX], fA[NF], fB[NF];
+  auto step = [&](int T_, const T* yv, T* ynext, const T* fv, T* fnext) {
+    if (T_ == next_stage) {  // wave-uniform
+      if (stage_band < nb) stage(stage_band);
+      ++stage_band;
+      next_stage += Pc;
+    }
+    yload(ynext, cp + 2);
+    {
+      int sn = s0 + 1;
+      sn = sn == Pc ? 0 : sn;
+#pragma unroll
+      for (int k = 0; k < NF; ++k) fnext[k] = fifo[sn * NF + k];
+    }
+    T in[NF];
+    in[0] = shift_down(fl[0], fv[0]);
+#pragma unroll
+    for (int k = 1; k < NF; ++k) {
+      const T t = shift_down(fl[k], fv[k]);
+      in[k] = prim ? T(0) : t;
+    }
+    T yn = T(0);
+    if (RBF) {
+#pragma unroll
+      for (int f = 0; f < DMAX; ++f) yn += yv[f] * yv[f];
+      yn *= gn;
+    }
+    T g[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      if (RBF) {
+        T s = -(xn[j] + yn);
+#pragma unroll
+        for (int f = 0; f < DMAX; ++f) s += xr[j][f] * yv[f];
+        g[j] = fast_exp2(s);
+      } else {
+        T s = T(0);
+#pragma unroll
+        for (int f = 0; f < DMAX; ++f) s += xr[j][f] * yv[f];
+        g[j] = s;
+      }
+    }
+    const bool keep = cp > 0;
+    const T keepf = keep ? T(1) : T(0);
+    T Mk[R];
+    {
+      T dprev = in[0] - gdiag;
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        const T dj = g[j] - gleft[j];
+        const T m = dj - dprev;
+        const bool kj = j == 0 ? (keep && !prim) : keep;
+        Mk[j] = kj ? m : T(0);
+        dprev = dj;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < R; ++j) gleft[j] = g[j];
+    gdiag = in[0];
+
+    T cur[NF];
+#pragma unroll
+    for (int k = 0; k < NF; ++k) cur[k] = in[k];
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      T of[NF];
+      of[0] = g[j];
+      dp_cell_keep<T, NLEV, ORDER>(Mk[j], cur, of, PS[j], HB[j], keepf);
+#pragma unroll
+      for (int k = 0; k < NF; ++k) cur[k] = of[k];
+    }
+#pragma unroll
+    for (int k = 0; k < NF; ++k) fl[k] = cur[k];
+#pragma unroll
+    for (int m = 0; m < NLEV; ++m) accf[m] += cur[1 + m];
+    if (lane == kWave - 1 && T_ >= kWave - 1) {
+#pragma unroll
+      for (int k = 0; k < NF; ++k) fifo[s63 * NF + k] = cur[k];
+    }
+    if ((T_ & 15) == 15) {  // wave-uniform flush of the FP32 partial sums
+#pragma unroll
+      for (int m = 0; m < NLEV; ++m) {
+        acc[m] += (double)accf[m];
+        accf[m] = T(0);
+      }
+    }
+    ++cp;
+    if (cp == Pc) {  // this lane's band pass ends
+      if (last) {
+        double* o = a.out + (int64_t)q * (NLEV + 1);
+        o[0] = 1.0;
+#pragma unroll
+        for (int m = 0; m < NLEV; ++m) o[m + 1] = acc[m] + (double)accf[m];
+      }
+#pragma unroll
+      for (int m = 0; m < NLEV; ++m) {
+        acc[m] = 0.0;
+        accf[m] = T(0);
+      }
+      cp = 0;
+      enter_band();
+    }
+    ++s0;
+    s0 = s0 == Pc ? 0 : s0;
+    ++s63;
+    s63 = s63 == Pc ? 0 : s63;
+  };
+
+  yload(y0, cp);
+  yload(y1, cp + 1);
+#pragma unroll
+  for (int k = 0; k < NF; ++k) fA[k] = fifo[k];
+  int T_ = 0;
+  for (; T_ + 3 <= total; T_ += 3) {
+    step(T_, y0, y2, fA, fB);
+    step(T_ + 1, y1, y0, fB, fA);
+    step(T_ + 2, y2, y1, fA, fB);
+    // rotate: after three steps the roles return with fA/fB swapped
+#pragma unroll
+    for (int k = 0; k < NF; ++k) {
+      const T t = fA[k];
+      fA[k] = fB[k];
+      fB[k] = t;
+    }
+  }
+  if (T_ < total) step(T_, y0, y2, fA, fB);
+  if (T_ + 1 < total) step(T_ + 1, y1, y0, fB, fA);
+}
+
+template <int NLEV, int ORDER, int DMAX, int R, int KIND>
+StreamEntry stream_entry() {
+  return StreamEntry{(const void*)dp_stream_kernel<NLEV, ORDER, DMAX, R, KIND>, R};
+}
+
+}  // namespace ksig

@@ -345,7 +345,7 @@ dp_sweep_kernel(const DpFwdArgs a) {
 
 template <typename T, int NLEV, int ORDER, int DMAX>
 hipError_t launch_dp_sweep(const DpFwdArgs& a, size_t lds, hipStream_t s) {
-  constexpr int R = sizeof(T) == 4 ? 2 : 1;
+  constexpr int R = sizeof(T) == 4 ? 4 : 1;
   auto k = dp_sweep_kernel<T, NLEV, ORDER, DMAX, R>;
   if (lds > 64 * 1024) {
     hipError_t e = hipFuncSetAttribute((const void*)k,

@@ -6,9 +6,12 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <mutex>
+
 #include "ksig_common.h"
 #include "ksig_hip.h"
 #include "dp_forward.h"
+#include "dp_stream.h"
 
 namespace {
 
@@ -36,6 +39,28 @@ size_t max_lds_bytes() {
 }
 
 int dmax_for(int d) { return d <= 8 ? 8 : (d <= 16 ? 16 : 32); }
+
+int device_cus() {
+  int dev = 0, v = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 1;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return 1;
+  return v > 0 ? v : 1;
+}
+
+// Resident row-stream waves per CU for a (kernel, LDS size), cached.
+int stream_occupancy(const ksig::StreamEntry& e, size_t lds) {
+  struct Slot { const void* fn; size_t lds; int occ; };
+  static Slot cache[64];
+  static int used = 0;
+  static std::mutex mu;
+  std::lock_guard<std::mutex> g(mu);
+  for (int i = 0; i < used; ++i)
+    if (cache[i].fn == e.fn && cache[i].lds == lds) return cache[i].occ;
+  const int occ = ksig::dp_stream_occupancy(e, lds);
+  if (used < 64) cache[used++] = Slot{e.fn, lds, occ};
+  return occ;
+}
 
 // Scale factors folded into the inputs: RBF uses exp2(-|x'-y'|^2) with
 // x' = sqrt(log2(e) / max(2 sigma^2, 1e-12)) x (ksig/static/kernels.py:258-260,
@@ -101,6 +126,80 @@ static int dp_levels_common(int dtype, ksig::DpFwdArgs& a, int cols, int dmax,
   return 0;
 }
 
+// Row-stream launch (float32, difference=1): one persistent pass, every wave a
+// contiguous slice of the pair list.  stream_levels returns -1 when no
+// compiled row-stream kernel or no LDS layout fits, so the caller can fall
+// back.
+struct StreamPlan {
+  ksig::StreamEntry e;
+  size_t lds;
+  int64_t slots;  // resident waves on the device
+};
+
+// Kernel and LDS size of the row stream for a problem; false if none fits.
+static bool stream_plan(int d, int kind, int n_levels, int order, int max_len_x,
+                        int max_len_y, StreamPlan* p) {
+  const int dmax = dmax_for(d);
+  p->e = ksig::dp_stream_entry(kind, dmax, n_levels, order);
+  if (!p->e.fn) return false;
+  const int nf = ksig::dp_flow_count(n_levels, order);
+  int pc = max_len_x < max_len_y ? max_len_x : max_len_y;
+  if (pc < ksig::kStreamMinCols) pc = ksig::kStreamMinCols;
+  p->lds = ksig::stream_lds_bytes(dmax, p->e.r, pc, nf);
+  if (p->lds > max_lds_bytes()) return false;
+  const int occ = stream_occupancy(p->e, p->lds);
+  p->slots = (int64_t)device_cus() * (occ > 0 ? occ : 1);
+  return true;
+}
+
+static int stream_launch(const StreamPlan& pl, const ksig::DpFwdArgs& f, double static_param,
+                         const int32_t* sched_order, const int32_t* wave_beg, int n_waves,
+                         void* stream) {
+  const int64_t n = f.n_pairs;
+  int64_t per = 0, waves = n_waves;
+  if (!wave_beg) {  // equal slices, one round of resident waves when they fit the table
+    const int64_t cap = pl.slots * ksig::kStreamMaxPairs;
+    const int64_t rounds = (n + cap - 1) / cap;
+    per = (n + pl.slots * rounds - 1) / (pl.slots * rounds);
+    if (per < 1) per = 1;
+    waves = (n + per - 1) / per;
+  }
+  const int dmax = dmax_for(f.d);
+  ksig::DpStreamArgs a{};
+  a.x = f.x;
+  a.y = f.y;
+  a.d = f.d;
+  a.vec = (f.d == dmax && ((uintptr_t)f.x.data & 15) == 0 && ((uintptr_t)f.y.data & 15) == 0)
+              ? 1 : 0;
+  a.pairs = f.pairs;
+  a.n_pairs = n;
+  a.order = sched_order;
+  a.wave_beg = wave_beg;
+  a.n_waves = (int)waves;
+  a.per_wave = (int)per;
+  if (f.kind == ksig::kRBF) {
+    double den = 2.0 * static_param * static_param;
+    if (den < 1e-12) den = 1e-12;
+    const double gn = 1.4426950408889634 / den;
+    a.gn = (float)gn;
+    a.gdot = (float)(2.0 * gn);
+  } else {
+    a.gn = 0.0f;
+    a.gdot = (float)static_param;
+  }
+  a.out = f.out;
+  hipError_t err = ksig::dp_stream_launch(pl.e, a, pl.lds, (hipStream_t)stream);
+  if (err != hipSuccess) return hip_fail(err, "dp_stream launch");
+  return 0;
+}
+
+static int stream_levels(const ksig::DpFwdArgs& f, int max_len_x, int max_len_y,
+                         double static_param, void* stream) {
+  StreamPlan pl;
+  if (!stream_plan(f.d, f.kind, f.n_levels, f.order, max_len_x, max_len_y, &pl)) return -1;
+  return stream_launch(pl, f, static_param, nullptr, nullptr, 0, stream);
+}
+
 int ksig_signature_levels(int dtype, const void* x, const int64_t* x_off,
                           const int32_t* x_len, int max_len_x, const void* y,
                           const int64_t* y_off, const int32_t* y_len,
@@ -135,7 +234,7 @@ int ksig_signature_levels(int dtype, const void* x, const int64_t* x_off,
     return fail(KSIG_EINVAL, "bad gram_mode %d", gram_mode);
   int mode = gram_mode;
   const bool mfma_ok = dtype == KSIG_F32 && a.difference && ksig::dp_mfma_supported(n_levels, order);
-  if (mode == KSIG_GRAM_AUTO) mode = mfma_ok ? KSIG_GRAM_MFMA_F32 : KSIG_GRAM_VALU;
+  if (mode == KSIG_GRAM_AUTO) mode = KSIG_GRAM_VALU;
   if (mode != KSIG_GRAM_VALU) {
     if (!mfma_ok)
       return fail(KSIG_EUNSUPPORTED, "MFMA Gram needs float32, difference=1 and a built "
@@ -151,6 +250,10 @@ int ksig_signature_levels(int dtype, const void* x, const int64_t* x_off,
     if (e != hipSuccess) return hip_fail(e, "dp_mfma launch");
     return 0;
   }
+  if (a.difference && dtype == KSIG_F32) {
+    rc = stream_levels(a, max_len_x, max_len_y, static_param, stream);
+    if (rc != -1) return rc;  // -1: no row-stream kernel fits, use the per-pair sweep
+  }
   if (a.difference) {
     const size_t lds = ksig::dp_sweep_lds(dtype, d, cols, n_levels, order);
     if (lds > max_lds_bytes())
@@ -160,6 +263,59 @@ int ksig_signature_levels(int dtype, const void* x, const int64_t* x_off,
     return 0;
   }
   return dp_levels_common(dtype, a, cols, dmax_for(d), stream);
+}
+
+int ksig_stream_slots(int d, int static_kind, int n_levels, int order, int max_len_x,
+                      int max_len_y, int* slots, int* max_pairs_per_wave) {
+  if (!slots || !max_pairs_per_wave) return fail(KSIG_EINVAL, "null output pointer");
+  *slots = 0;
+  *max_pairs_per_wave = ksig::kStreamMaxPairs;
+  if (d < 1 || d > 32) return 0;
+  if (static_kind != ksig::kRBF && static_kind != ksig::kLinear) return 0;
+  if (n_levels < 1) return fail(KSIG_EINVAL, "n_levels must be positive");
+  if (order <= 0 || order >= n_levels) order = n_levels;
+  StreamPlan pl;
+  if (!stream_plan(d, static_kind, n_levels, order, max_len_x, max_len_y, &pl)) return 0;
+  *slots = (int)pl.slots;
+  return 0;
+}
+
+int ksig_signature_levels_batched(const void* x, const int64_t* x_off, const int32_t* x_len,
+                                  int max_len_x, const void* y, const int64_t* y_off,
+                                  const int32_t* y_len, int max_len_y, int d,
+                                  const int32_t* pairs, int64_t n_pairs, int static_kind,
+                                  double static_param, int n_levels, int order,
+                                  const int32_t* sched_order, const int32_t* wave_beg,
+                                  int n_waves, double* out, void* stream) {
+  int rc = check_common(KSIG_F32, x, y, d, pairs, n_pairs, out);
+  if (rc) return rc;
+  if (n_levels < 1) return fail(KSIG_EINVAL, "n_levels must be positive");
+  if (order <= 0 || order >= n_levels) order = n_levels;
+  if (static_kind != ksig::kRBF && static_kind != ksig::kLinear)
+    return fail(KSIG_EINVAL, "unknown static kernel kind %d", static_kind);
+  double xs, ys;
+  rc = static_scales(static_kind, static_param, &xs, &ys);
+  if (rc) return rc;
+  if (n_pairs == 0) return 0;
+  if (max_len_x < 0 || max_len_y < 0) return fail(KSIG_EINVAL, "negative max length");
+  if (wave_beg && n_waves < 1) return fail(KSIG_EINVAL, "n_waves must be positive");
+  StreamPlan pl;
+  if (!stream_plan(d, static_kind, n_levels, order, max_len_x, max_len_y, &pl))
+    return fail(KSIG_EUNSUPPORTED, "no row-stream kernel for d=%d, (n_levels=%d, order=%d) "
+                "and column length %d", d, n_levels, order,
+                max_len_x < max_len_y ? max_len_x : max_len_y);
+  ksig::DpFwdArgs f{};
+  f.x = {x, x_off, x_len};
+  f.y = {y, y_off, y_len};
+  f.d = d;
+  f.pairs = pairs;
+  f.n_pairs = n_pairs;
+  f.kind = static_kind;
+  f.n_levels = n_levels;
+  f.order = order;
+  f.difference = 1;
+  f.out = out;
+  return stream_launch(pl, f, static_param, sched_order, wave_beg, n_waves, stream);
 }
 
 int ksig_signature_levels_fwd_save(int dtype, const void* x, const int64_t* x_off,

@@ -1,0 +1,124 @@
+"""GPU parity of the row-stream DP kernel (float32, difference=True): many
+pairs stacked in one wave, caller schedules, uneven lengths, every feature
+bucket, against the FP64 oracle on the float32-rounded inputs.
+
+Tolerance: 1e-5 normwise relative error over each pair's level vector (the
+float32 bar of the DP kernels)."""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import ksig_amd as ksig
+from ksig_amd import _lib, _ops
+from oracle import sigkern as O
+from tests.gpu_util import dev, level_err, to_dev
+
+pytestmark = pytest.mark.gpu
+TOL32 = 1e-5
+
+
+def _ragged(rng, lens, d):
+  seqs = [O.brownian(rng, 1, max(L, 1), d)[0].astype(np.float32) for L in lens]
+  return seqs, _ops.SeqSet.ragged([to_dev(s) for s in seqs])
+
+
+def _oracle(xs, ys, pairs, n_levels, order, kind, param):
+  out = []
+  for a, b in pairs:
+    xa = xs[a].astype(np.float64)
+    yb = ys[b].astype(np.float64)
+    out.append(O.pair_levels(xa, yb, n_levels, order, True, kind, param))
+  return np.stack(out)
+
+
+def _check(got, ref):
+  got = np.asarray(got)
+  # K_0 = 1 exactly; empty grids give exact zeros
+  np.testing.assert_array_equal(got[:, 0], 1.0)
+  assert level_err(got.T, ref.T) < TOL32
+
+
+@pytest.mark.parametrize("kind,param", [("rbf", 1.5), ("linear", 0.7)])
+@pytest.mark.parametrize("nlev,order", [(5, 1), (4, 2)])
+def test_schedules_with_many_pairs_per_wave(kind, param, nlev, order):
+  rng = np.random.default_rng(21)
+  lx = rng.integers(1, 41, 40)
+  ly = rng.integers(1, 41, 30)
+  lx[:3] = [1, 2, 3]
+  ly[:2] = [1, 2]
+  xs, X = _ragged(rng, lx, 3)
+  ys, Y = _ragged(rng, ly, 3)
+  n_pairs = 600
+  pr = np.stack([rng.integers(0, 40, n_pairs), rng.integers(0, 30, n_pairs)], -1).astype(np.int32)
+  pairs = to_dev(pr)
+  code = _lib.STATIC_RBF if kind == "rbf" else _lib.STATIC_LINEAR
+  ref = _oracle(xs, ys, pr, nlev, order, kind, param)
+  # 1) length-bucketed schedule from the capacity query
+  got = _ops.signature_levels_batched(X, Y, pairs, nlev, order, code, param)
+  _check(got.cpu().numpy(), ref)
+  # 2) a random processing order in full 128-pair slices
+  perm = rng.permutation(n_pairs).astype(np.int32)
+  wb = np.arange(0, n_pairs + 128, 128).clip(max=n_pairs).astype(np.int32)
+  got = _ops.signature_levels_batched(X, Y, pairs, nlev, order, code, param,
+                                      (to_dev(perm), to_dev(wb)))
+  _check(got.cpu().numpy(), ref)
+  # 3) identity order, ragged slice sizes including single pairs
+  wb = np.unique(np.concatenate([[0, 1, 2, 130, 131], rng.integers(0, n_pairs, 12),
+                                 np.arange(0, n_pairs, 100), [n_pairs]])).astype(np.int32)
+  assert np.diff(wb).max() <= 128
+  got = _ops.signature_levels_batched(X, Y, pairs, nlev, order, code, param,
+                                      (None, to_dev(wb)))
+  _check(got.cpu().numpy(), ref)
+
+
+@pytest.mark.parametrize("d", [1, 5, 8, 13, 16, 24, 32])
+def test_uneven_lengths_across_bands(d):
+  """Lengths that cross the 68-column minimum band, 64R-row bands and R-row
+  padding, stacked several to a wave."""
+  rng = np.random.default_rng(300 + d)
+  lens = [2, 3, 67, 68, 69, 70, 129, 130, 150, 257, 5, 40]
+  xs, X = _ragged(rng, lens, d)
+  n = len(lens)
+  pr = np.array([[a, b] for a in range(n) for b in range(n) if (a + 2 * b) % 3 == 0],
+                dtype=np.int32)
+  sig = math.sqrt(d)
+  ref = _oracle(xs, xs, pr, 5, 1, "rbf", sig)
+  for wb in (None, np.array([0, 7, 8, 40, len(pr)], dtype=np.int32)):
+    sched = None if wb is None else (None, to_dev(wb))
+    got = _ops.signature_levels_batched(X, X, to_dev(pr), 5, 1, _lib.STATIC_RBF, sig, sched)
+    _check(got.cpu().numpy(), ref)
+
+
+@pytest.mark.parametrize("d,L,n", [(8, 24, 70), (16, 20, 60), (32, 18, 55)])
+def test_dense_gram_more_pairs_than_resident_waves(d, L, n):
+  """Full Gram through SignatureKernel: more pairs than resident waves, so
+  every wave stacks several pairs (equal slices chosen by the library)."""
+  rng = np.random.default_rng(40 + d)
+  X = O.brownian(rng, n, L, d).astype(np.float32)
+  sig = math.sqrt(d)
+  su = ksig.kernels.SignatureKernel(5, normalize=False,
+                                    static_kernel=ksig.static.kernels.RBFKernel(sig))
+  lev = su._levels(to_dev(X), to_dev(X)).cpu().numpy()
+  ref = O.signature_kern(O.pairwise_gram(X.astype(np.float64), None, "rbf", sig), 5, 1,
+                         True, True)
+  assert level_err(lev, ref) < TOL32
+  slots, _ = _ops.stream_slots(_ops.SeqSet.dense(to_dev(X)), _ops.SeqSet.dense(to_dev(X)),
+                               _lib.STATIC_RBF, 5, 1)
+  assert 0 < slots < n * n
+
+
+def test_ragged_signature_kernel_uses_schedule():
+  """Uneven float32 sets through the host op pick the batched row stream and
+  agree with the per-pair oracle."""
+  rng = np.random.default_rng(77)
+  lens_x = [33, 90, 7, 64, 120]
+  lens_y = [50, 2, 75]
+  xs, X = _ragged(rng, lens_x, 4)
+  ys, Y = _ragged(rng, lens_y, 4)
+  pairs = _ops.dense_pairs(len(lens_x), len(lens_y), dev())
+  got = _ops.signature_levels(X, Y, pairs, 6, 2, True, _lib.STATIC_RBF, 1.3)
+  ref = _oracle(xs, ys, pairs.cpu().numpy(), 6, 2, "rbf", 1.3)
+  _check(got.cpu().numpy(), ref)

@@ -1,0 +1,56 @@
+"""Host-side length-bucketed schedule of the row-stream kernel (no GPU)."""
+
+import numpy as np
+import pytest
+
+from ksig_amd._ops import stream_schedule
+
+
+def _check(lx, ly, slots, maxp):
+  order, wb = stream_schedule(lx, ly, slots, maxp)
+  n = len(lx)
+  assert order.dtype == np.int32 and wb.dtype == np.int32
+  np.testing.assert_array_equal(np.sort(order), np.arange(n))
+  assert wb[0] == 0 and wb[-1] == n
+  sizes = np.diff(wb)
+  assert sizes.min() >= 1 and sizes.max() <= maxp
+  cols = np.minimum(lx, ly)[order]
+  assert np.all(np.diff(cols) <= 0)  # shorter side, longest first
+  return order, wb
+
+
+def test_uniform_lengths_one_round_of_equal_slices():
+  n = 131328
+  order, wb = _check(np.full(n, 200), np.full(n, 200), 1792, 128)
+  sizes = np.diff(wb)
+  assert len(sizes) == 1792
+  assert sizes.max() - sizes.min() <= 1
+
+
+def test_more_pairs_than_table_capacity_uses_balanced_rounds():
+  n = 300000
+  order, wb = _check(np.full(n, 20), np.full(n, 20), 1024, 128)
+  sizes = np.diff(wb)
+  assert len(sizes) % 1024 == 0
+  assert sizes.max() - sizes.min() <= 1
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_uneven_lengths_balance_work(seed):
+  rng = np.random.default_rng(seed)
+  n = 4096
+  lx = rng.integers(32, 1025, n)
+  ly = rng.integers(32, 1025, n)
+  order, wb = _check(lx, ly, 512, 128)
+  cost = (np.maximum(lx, ly) * np.maximum(np.minimum(lx, ly), 68))[order].astype(float)
+  per = np.add.reduceat(cost, wb[:-1])
+  # slices hold whole pairs: each is within one pair of the mean work
+  assert per.max() <= per.mean() + cost.max()
+  assert len(per) <= 512
+
+
+def test_degenerate_and_tiny_lists():
+  _check(np.array([1]), np.array([5]), 64, 128)
+  _check(np.array([3, 1, 2]), np.array([2, 2, 9]), 1, 128)
+  order, wb = stream_schedule(np.zeros(0, np.int64), np.zeros(0, np.int64), 8, 128)
+  assert len(order) == 0 and list(wb) == [0]
