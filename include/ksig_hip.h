@@ -77,24 +77,26 @@ int ksig_signature_levels(int dtype, const void* x, const int64_t* x_off,
 
 /*
  * Resident-wave capacity of the row-stream kernel behind
- * ksig_signature_levels_batched for a problem shape on the current device:
+ * ksig_signature_levels_batched for a problem shape on the current device,
+ * with slices of at most max_slice pairs (<= 0: the largest accepted):
  * *slots = waves that run concurrently (0 if no row-stream kernel or LDS
  * layout fits), *max_pairs_per_wave = the largest slice one wave accepts.
  * A scheduling query with no reference counterpart (the reference launches
  * one CuPy kernel sequence per Gram, ksig/kernels.py:147-164).
  */
 int ksig_stream_slots(int d, int static_kind, int n_levels, int order, int max_len_x,
-                      int max_len_y, int* slots, int* max_pairs_per_wave);
+                      int max_len_y, int max_slice, int* slots, int* max_pairs_per_wave);
 
 /*
  * ksig_signature_levels for float32 inputs with difference=1 and a caller
  * schedule, in one persistent launch: wave w evaluates the pairs
  * sched_order[wave_beg[w] .. wave_beg[w+1]) (sched_order null: the identity),
- * at most *max_pairs_per_wave of ksig_stream_slots per wave, writing out[q]
- * for each pair index q it takes.  A wave stacks its pairs' rows and sweeps
- * them with one column count (the longest shorter side among them), so
- * slices of similar lengths waste least.  wave_beg null: equal slices sized
- * from the device's resident-wave count.  Replaces the same reference
+ * n_waves slices of at most max_slice pairs each (1 <= max_slice <= the
+ * *max_pairs_per_wave of ksig_stream_slots), writing out[q] for each pair
+ * index q it takes.  A wave stacks its pairs' rows and sweeps them with one
+ * column count (the longest shorter side among them), so slices of similar
+ * lengths waste least.  wave_beg null: slices of max_slice consecutive pairs
+ * of the order.  Replaces the same reference
  * composition as ksig_signature_levels (ksig/kernels.py:147-164 over
  * ksig/algorithms.py:22-140) for pair lists the reference cannot take
  * (uneven lengths; the reference needs one dense [n, L, d] array,
@@ -106,7 +108,7 @@ int ksig_signature_levels_batched(const void* x, const int64_t* x_off, const int
                                   const int32_t* pairs, int64_t n_pairs, int static_kind,
                                   double static_param, int n_levels, int order,
                                   const int32_t* sched_order, const int32_t* wave_beg,
-                                  int n_waves, double* out, void* stream);
+                                  int n_waves, int max_slice, double* out, void* stream);
 
 /*
  * Training forward: ksig_signature_levels for order 1 with difference, which

@@ -32,10 +32,10 @@ SIGNATURES = {
         _c_int, _vp, _vp, _vp, _c_int, _vp, _vp, _vp, _c_int, _c_int, _vp,
         _c_i64, _c_int, _c_dbl, _c_int, _c_int, _c_int, _c_int, _vp, _vp]),
     "ksig_stream_slots": (_c_int, [
-        _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp]),
+        _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp]),
     "ksig_signature_levels_batched": (_c_int, [
         _vp, _vp, _vp, _c_int, _vp, _vp, _vp, _c_int, _c_int, _vp, _c_i64, _c_int,
-        _c_dbl, _c_int, _c_int, _vp, _vp, _c_int, _vp, _vp]),
+        _c_dbl, _c_int, _c_int, _vp, _vp, _c_int, _c_int, _vp, _vp]),
     "ksig_signature_levels_fwd_save": (_c_int, [
         _c_int, _vp, _vp, _vp, _c_int, _vp, _vp, _vp, _c_int, _c_int, _vp,
         _c_i64, _c_int, _c_dbl, _c_int, _vp, _vp, _vp, _vp]),

@@ -136,13 +136,15 @@ def signature_levels(X: SeqSet, Y: SeqSet, pairs: torch.Tensor, n_levels: int,
   return out
 
 
-def stream_slots(X: SeqSet, Y: SeqSet, kind: int, n_levels: int, order: int):
+def stream_slots(X: SeqSet, Y: SeqSet, kind: int, n_levels: int, order: int,
+                 max_slice: int = 0):
   """(resident waves, max pairs per wave) of the row-stream kernel for this
-  problem shape; resident waves is 0 when no row-stream kernel fits."""
+  problem shape and slice bound; resident waves is 0 when no row-stream
+  kernel fits."""
   lib = _lib.load()
   s, m = ctypes.c_int(0), ctypes.c_int(0)
   _lib.check(lib.ksig_stream_slots(X.d, kind, n_levels, order, X.max_len, Y.max_len,
-                                   ctypes.byref(s), ctypes.byref(m)))
+                                   int(max_slice), ctypes.byref(s), ctypes.byref(m)))
   return s.value, m.value
 
 
@@ -189,9 +191,9 @@ def signature_levels_batched(X: SeqSet, Y: SeqSet, pairs: torch.Tensor, n_levels
                              order: int, kind: int, param: float,
                              schedule: Optional[tuple] = None) -> torch.Tensor:
   """Level values [n_pairs, n_levels+1] of float32 pairs of uneven lengths in
-  one persistent row-stream launch.  ``schedule`` = (order, wave_beg) device
-  int32 tensors, or None for the length-bucketed schedule of
-  ``stream_schedule``."""
+  one persistent row-stream launch.  ``schedule`` = (order, wave_beg[,
+  max_slice]) with device int32 tensors (order may be None), or None for the
+  length-bucketed schedule of ``stream_schedule``."""
   lib = _lib.load()
   _check_sets(X, Y)
   if X.data.dtype != torch.float32:
@@ -203,19 +205,21 @@ def signature_levels_batched(X: SeqSet, Y: SeqSet, pairs: torch.Tensor, n_levels
     return out
   if schedule is None:
     schedule = make_schedule(X, Y, pairs, kind, n_levels, order)
-  so, wb = schedule
+  so, wb = schedule[0], schedule[1]
+  max_slice = schedule[2] if len(schedule) > 2 else 128
   _lib.check(lib.ksig_signature_levels_batched(
       X.data.data_ptr(), X.off.data_ptr(), X.len.data_ptr(), X.max_len,
       Y.data.data_ptr(), Y.off.data_ptr(), Y.len.data_ptr(), Y.max_len, X.d,
       pairs.data_ptr(), n_pairs, kind, float(param), n_levels, order,
       so.data_ptr() if so is not None else None, wb.data_ptr(), wb.shape[0] - 1,
-      out.data_ptr(), _lib.stream_handle(dev)))
+      int(max_slice), out.data_ptr(), _lib.stream_handle(dev)))
   return out
 
 
 def make_schedule(X: SeqSet, Y: SeqSet, pairs: torch.Tensor, kind: int, n_levels: int,
                   order: int):
-  """Device (order, wave_beg) tensors of the length-bucketed schedule."""
+  """(order, wave_beg, max_slice) of the length-bucketed schedule (device
+  int32 tensors and the largest slice)."""
   norm_order = n_levels if (order <= 0 or order >= n_levels) else order
   slots, maxp = stream_slots(X, Y, kind, n_levels, norm_order)
   if slots <= 0:
@@ -225,7 +229,8 @@ def make_schedule(X: SeqSet, Y: SeqSet, pairs: torch.Tensor, kind: int, n_levels
   ly = Y.lengths()[pr[:, 1]]
   so, wb = stream_schedule(lx, ly, slots, maxp)
   dev = X.data.device
-  return (torch.as_tensor(so, device=dev), torch.as_tensor(wb, device=dev))
+  return (torch.as_tensor(so, device=dev), torch.as_tensor(wb, device=dev),
+          int(np.diff(wb).max()) if len(wb) > 1 else 1)
 
 
 def signature_levels_gram(M: torch.Tensor, n_levels: int, order: int,

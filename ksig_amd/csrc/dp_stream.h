@@ -65,18 +65,22 @@ struct StreamLayout {
 constexpr int kStreamMaxPairs = 128;  // pair-table entries per wave
 constexpr int kStreamMinCols = 68;    // smallest band width Pc (FIFO hazard + staging)
 
-__host__ __device__ inline size_t stream_lds_bytes(int dmax, int r, int pc, int nf) {
+// Dynamic LDS of the row stream: pair table of `cap` entries (cap % 4 == 0),
+// row starts, lane-private x staging, FIFO.
+__host__ __device__ inline size_t stream_lds_bytes(int dmax, int r, int pc, int nf, int cap) {
   const int xs = ((dmax / 4) % 2 == 0) ? dmax + 4 : dmax + 8;
-  return sizeof(StreamPair) * kStreamMaxPairs + 4 * (kStreamMaxPairs + 4) +
-         (size_t)4 * kWave * r * xs + (size_t)4 * pc * nf;
+  return sizeof(StreamPair) * cap + 4 * ((size_t)cap + 4) + (size_t)4 * kWave * r * xs +
+         (size_t)4 * pc * nf;
 }
 
-// Load one sequence point (d features) into v[0..DMAX): 16-byte vector loads
-// when vec (d == DMAX and 16-byte aligned data), predicated dword loads
-// otherwise.
-template <int DMAX>
-__device__ __forceinline__ void stream_load_row(float* v, const float* p, int d, bool vec) {
-  if (vec) {
+// Load one sequence point (d features) into v[0..DMAX).  VEC: 16-byte vector
+// loads (d == DMAX, 16-byte aligned data).  Otherwise one dword load per
+// feature slot with the address clamped into the row and the slots past d
+// zeroed, so both forms are straight-line code (the compiler then tracks the
+// outstanding loads exactly and the two-step prefetch stays in flight).
+template <int DMAX, bool VEC>
+__device__ __forceinline__ void stream_load_row(float* v, const float* p, int d) {
+  if (VEC) {
 #pragma unroll
     for (int f = 0; f < DMAX; f += 4) {
       const float4 t = *reinterpret_cast<const float4*>(p + f);
@@ -84,11 +88,14 @@ __device__ __forceinline__ void stream_load_row(float* v, const float* p, int d,
     }
   } else {
 #pragma unroll
-    for (int f = 0; f < DMAX; ++f) v[f] = f < d ? p[f] : 0.0f;
+    for (int f = 0; f < DMAX; ++f) {
+      const float t = p[f < d ? f : d - 1];
+      v[f] = f < d ? t : 0.0f;
+    }
   }
 }
 
-template <int NLEV, int ORDER, int DMAX, int R, int KIND>
+template <int NLEV, int ORDER, int DMAX, int R, int KIND, bool VEC>
 __global__ void __launch_bounds__(kWave)
 dp_stream_kernel(const DpStreamArgs a) {
   using T = float;
@@ -101,15 +108,15 @@ dp_stream_kernel(const DpStreamArgs a) {
   constexpr bool RBF = KIND == kRBF;
 
   extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int cap = a.tab_cap;  // pair-table entries (a multiple of 4)
   StreamPair* tab = reinterpret_cast<StreamPair*>(smem_raw);
-  int32_t* tstart = reinterpret_cast<int32_t*>(tab + kStreamMaxPairs);
-  T* xst = reinterpret_cast<T*>(tstart + kStreamMaxPairs + 4);
+  int32_t* tstart = reinterpret_cast<int32_t*>(tab + cap);
+  T* xst = reinterpret_cast<T*>(tstart + cap + 4);
   T* fifo = xst + RB * XS;
 
   const int lane = threadIdx.x;
   const int w = blockIdx.x;
   const int d = a.d;
-  const bool vec = a.vec != 0;
   int kb, ke;
   if (a.wave_beg) {
     kb = a.wave_beg[w];
@@ -121,7 +128,7 @@ dp_stream_kernel(const DpStreamArgs a) {
   }
   int nk = ke - kb;
   if (nk <= 0) return;
-  nk = nk < kStreamMaxPairs ? nk : kStreamMaxPairs;  // the host keeps slices within the table
+  nk = nk < cap ? nk : cap;  // the host keeps slices within the table
 
   // ---- pair table: orientation, padded row counts, exclusive row starts ----
   int pcl = 0;
@@ -214,7 +221,7 @@ dp_stream_kernel(const DpStreamArgs a) {
       int r = r0 + j;
       r = r < e.lr1 ? r : e.lr1;
       T v[DMAX];
-      stream_load_row<DMAX>(v, rb + (int64_t)r * d, d, vec);
+      stream_load_row<DMAX, VEC>(v, rb + (int64_t)r * d, d);
       T n2 = T(0);
       T* dst = xst + (R * lane + j) * XS;
 #pragma unroll
@@ -284,7 +291,7 @@ dp_stream_kernel(const DpStreamArgs a) {
     cc = cc > 0 ? cc : 0;
     const int cl = nx ? pccl : ccl;
     cc = cc < cl ? cc : cl;
-    stream_load_row<DMAX>(dst, p + (int64_t)cc * d, d, vec);
+    stream_load_row<DMAX, VEC>(dst, p + (int64_t)cc * d, d);
   };
 
   T y0[DMAX], y1[DMAX], y2[DMAX], fA[NF], fB[NF];
@@ -416,9 +423,9 @@ dp_stream_kernel(const DpStreamArgs a) {
   if (T_ + 1 < total) step(T_ + 1, y1, y0, fB, fA);
 }
 
-template <int NLEV, int ORDER, int DMAX, int R, int KIND>
+template <int NLEV, int ORDER, int DMAX, int R, int KIND, bool VEC>
 StreamEntry stream_entry() {
-  return StreamEntry{(const void*)dp_stream_kernel<NLEV, ORDER, DMAX, R, KIND>, R};
+  return StreamEntry{(const void*)dp_stream_kernel<NLEV, ORDER, DMAX, R, KIND, VEC>, R};
 }
 
 }  // namespace ksig

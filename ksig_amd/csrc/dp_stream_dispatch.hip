@@ -3,23 +3,32 @@
 
 namespace ksig {
 
-StreamEntry dp_stream_rbf_d8(int n_levels, int order);
-StreamEntry dp_stream_rbf_d16(int n_levels, int order);
-StreamEntry dp_stream_rbf_d32(int n_levels, int order);
-StreamEntry dp_stream_lin_d8(int n_levels, int order);
-StreamEntry dp_stream_lin_d16(int n_levels, int order);
-StreamEntry dp_stream_lin_d32(int n_levels, int order);
+StreamEntry dp_stream_rbf_d8v(int n_levels, int order);
+StreamEntry dp_stream_rbf_d8g(int n_levels, int order);
+StreamEntry dp_stream_rbf_d16v(int n_levels, int order);
+StreamEntry dp_stream_rbf_d16g(int n_levels, int order);
+StreamEntry dp_stream_rbf_d32v(int n_levels, int order);
+StreamEntry dp_stream_rbf_d32g(int n_levels, int order);
+StreamEntry dp_stream_lin_d8v(int n_levels, int order);
+StreamEntry dp_stream_lin_d8g(int n_levels, int order);
+StreamEntry dp_stream_lin_d16v(int n_levels, int order);
+StreamEntry dp_stream_lin_d16g(int n_levels, int order);
+StreamEntry dp_stream_lin_d32v(int n_levels, int order);
+StreamEntry dp_stream_lin_d32g(int n_levels, int order);
 
-StreamEntry dp_stream_entry(int kind, int dmax, int n_levels, int order) {
-  if (kind == kRBF) {
-    if (dmax == 8) return dp_stream_rbf_d8(n_levels, order);
-    if (dmax == 16) return dp_stream_rbf_d16(n_levels, order);
-    if (dmax == 32) return dp_stream_rbf_d32(n_levels, order);
-  } else if (kind == kLinear) {
-    if (dmax == 8) return dp_stream_lin_d8(n_levels, order);
-    if (dmax == 16) return dp_stream_lin_d16(n_levels, order);
-    if (dmax == 32) return dp_stream_lin_d32(n_levels, order);
+StreamEntry dp_stream_entry(int kind, int dmax, bool vec, int n_levels, int order) {
+#define KSIG_PICK(K, KN)                                                           \
+  if (kind == KN) {                                                                \
+    if (dmax == 8) return vec ? dp_stream_##K##_d8v(n_levels, order)               \
+                              : dp_stream_##K##_d8g(n_levels, order);              \
+    if (dmax == 16) return vec ? dp_stream_##K##_d16v(n_levels, order)             \
+                               : dp_stream_##K##_d16g(n_levels, order);            \
+    if (dmax == 32) return vec ? dp_stream_##K##_d32v(n_levels, order)             \
+                               : dp_stream_##K##_d32g(n_levels, order);            \
   }
+  KSIG_PICK(rbf, kRBF)
+  KSIG_PICK(lin, kLinear)
+#undef KSIG_PICK
   return StreamEntry{nullptr, 0};
 }
 

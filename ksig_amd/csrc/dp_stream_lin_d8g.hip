@@ -4,9 +4,9 @@
 namespace ksig {
 
 #define KSIG_CASE(L, O) \
-  if (n_levels == L && order == O) return stream_entry<L, O, 8, 4, kLinear>();
+  if (n_levels == L && order == O) return stream_entry<L, O, 8, 4, kLinear, false>();
 
-StreamEntry dp_stream_lin_d8(int n_levels, int order) {
+StreamEntry dp_stream_lin_d8g(int n_levels, int order) {
   KSIG_DP_LEVEL_ORDER_TABLE(KSIG_CASE)
   return StreamEntry{nullptr, 0};
 }

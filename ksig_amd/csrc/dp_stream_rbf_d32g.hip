@@ -4,9 +4,9 @@
 namespace ksig {
 
 #define KSIG_CASE(L, O) \
-  if (n_levels == L && order == O) return stream_entry<L, O, 32, 2, kRBF>();
+  if (n_levels == L && order == O) return stream_entry<L, O, 32, 2, kRBF, false>();
 
-StreamEntry dp_stream_rbf_d32(int n_levels, int order) {
+StreamEntry dp_stream_rbf_d32g(int n_levels, int order) {
   KSIG_DP_LEVEL_ORDER_TABLE(KSIG_CASE)
   return StreamEntry{nullptr, 0};
 }

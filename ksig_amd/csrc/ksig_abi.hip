@@ -51,14 +51,14 @@ int device_cus() {
 // Resident row-stream waves per CU for a (kernel, LDS size), cached.
 int stream_occupancy(const ksig::StreamEntry& e, size_t lds) {
   struct Slot { const void* fn; size_t lds; int occ; };
-  static Slot cache[64];
+  static Slot cache[256];
   static int used = 0;
   static std::mutex mu;
   std::lock_guard<std::mutex> g(mu);
   for (int i = 0; i < used; ++i)
     if (cache[i].fn == e.fn && cache[i].lds == lds) return cache[i].occ;
   const int occ = ksig::dp_stream_occupancy(e, lds);
-  if (used < 64) cache[used++] = Slot{e.fn, lds, occ};
+  if (used < 256) cache[used++] = Slot{e.fn, lds, occ};
   return occ;
 }
 
@@ -133,50 +133,46 @@ static int dp_levels_common(int dtype, ksig::DpFwdArgs& a, int cols, int dmax,
 struct StreamPlan {
   ksig::StreamEntry e;
   size_t lds;
+  int cap;        // pair-table entries
   int64_t slots;  // resident waves on the device
 };
 
-// Kernel and LDS size of the row stream for a problem; false if none fits.
+// Kernel, LDS size and resident waves of the row stream for a problem with a
+// pair table of `cap` entries; false if no kernel or LDS layout fits.
 static bool stream_plan(int d, int kind, int n_levels, int order, int max_len_x,
-                        int max_len_y, StreamPlan* p) {
+                        int max_len_y, bool vec, int cap, StreamPlan* p) {
   const int dmax = dmax_for(d);
-  p->e = ksig::dp_stream_entry(kind, dmax, n_levels, order);
+  p->e = ksig::dp_stream_entry(kind, dmax, vec, n_levels, order);
   if (!p->e.fn) return false;
   const int nf = ksig::dp_flow_count(n_levels, order);
   int pc = max_len_x < max_len_y ? max_len_x : max_len_y;
   if (pc < ksig::kStreamMinCols) pc = ksig::kStreamMinCols;
-  p->lds = ksig::stream_lds_bytes(dmax, p->e.r, pc, nf);
+  p->cap = (cap + 3) & ~3;
+  p->lds = ksig::stream_lds_bytes(dmax, p->e.r, pc, nf, p->cap);
   if (p->lds > max_lds_bytes()) return false;
   const int occ = stream_occupancy(p->e, p->lds);
   p->slots = (int64_t)device_cus() * (occ > 0 ? occ : 1);
   return true;
 }
 
+static bool stream_vec(int d, const void* x, const void* y) {
+  return d == dmax_for(d) && ((uintptr_t)x & 15) == 0 && ((uintptr_t)y & 15) == 0;
+}
+
 static int stream_launch(const StreamPlan& pl, const ksig::DpFwdArgs& f, double static_param,
                          const int32_t* sched_order, const int32_t* wave_beg, int n_waves,
-                         void* stream) {
-  const int64_t n = f.n_pairs;
-  int64_t per = 0, waves = n_waves;
-  if (!wave_beg) {  // equal slices, one round of resident waves when they fit the table
-    const int64_t cap = pl.slots * ksig::kStreamMaxPairs;
-    const int64_t rounds = (n + cap - 1) / cap;
-    per = (n + pl.slots * rounds - 1) / (pl.slots * rounds);
-    if (per < 1) per = 1;
-    waves = (n + per - 1) / per;
-  }
-  const int dmax = dmax_for(f.d);
+                         int64_t per_wave, void* stream) {
   ksig::DpStreamArgs a{};
   a.x = f.x;
   a.y = f.y;
   a.d = f.d;
-  a.vec = (f.d == dmax && ((uintptr_t)f.x.data & 15) == 0 && ((uintptr_t)f.y.data & 15) == 0)
-              ? 1 : 0;
+  a.tab_cap = pl.cap;
   a.pairs = f.pairs;
-  a.n_pairs = n;
+  a.n_pairs = f.n_pairs;
   a.order = sched_order;
   a.wave_beg = wave_beg;
-  a.n_waves = (int)waves;
-  a.per_wave = (int)per;
+  a.n_waves = n_waves;
+  a.per_wave = (int)per_wave;
   if (f.kind == ksig::kRBF) {
     double den = 2.0 * static_param * static_param;
     if (den < 1e-12) den = 1e-12;
@@ -193,11 +189,28 @@ static int stream_launch(const StreamPlan& pl, const ksig::DpFwdArgs& f, double 
   return 0;
 }
 
+// Equal slices of the pair list: the smallest pair table (in steps of 4) whose
+// resident waves take every pair in one round; past 128 pairs per wave, equal
+// rounds of full tables.  Returns -1 when no row-stream kernel fits.
 static int stream_levels(const ksig::DpFwdArgs& f, int max_len_x, int max_len_y,
                          double static_param, void* stream) {
+  const bool vec = stream_vec(f.d, f.x.data, f.y.data);
+  const int64_t n = f.n_pairs;
   StreamPlan pl;
-  if (!stream_plan(f.d, f.kind, f.n_levels, f.order, max_len_x, max_len_y, &pl)) return -1;
-  return stream_launch(pl, f, static_param, nullptr, nullptr, 0, stream);
+  int64_t per = 0;
+  for (int cap = 4; cap <= ksig::kStreamMaxPairs; cap += 4) {
+    if (!stream_plan(f.d, f.kind, f.n_levels, f.order, max_len_x, max_len_y, vec, cap, &pl))
+      return -1;
+    per = (n + pl.slots - 1) / pl.slots;
+    if (per <= cap) break;
+  }
+  if (per > pl.cap) {
+    const int64_t rounds = (n + pl.slots * pl.cap - 1) / (pl.slots * pl.cap);
+    per = (n + pl.slots * rounds - 1) / (pl.slots * rounds);
+  }
+  if (per < 1) per = 1;
+  const int64_t waves = (n + per - 1) / per;
+  return stream_launch(pl, f, static_param, nullptr, nullptr, (int)waves, per, stream);
 }
 
 int ksig_signature_levels(int dtype, const void* x, const int64_t* x_off,
@@ -266,7 +279,7 @@ int ksig_signature_levels(int dtype, const void* x, const int64_t* x_off,
 }
 
 int ksig_stream_slots(int d, int static_kind, int n_levels, int order, int max_len_x,
-                      int max_len_y, int* slots, int* max_pairs_per_wave) {
+                      int max_len_y, int max_slice, int* slots, int* max_pairs_per_wave) {
   if (!slots || !max_pairs_per_wave) return fail(KSIG_EINVAL, "null output pointer");
   *slots = 0;
   *max_pairs_per_wave = ksig::kStreamMaxPairs;
@@ -274,8 +287,11 @@ int ksig_stream_slots(int d, int static_kind, int n_levels, int order, int max_l
   if (static_kind != ksig::kRBF && static_kind != ksig::kLinear) return 0;
   if (n_levels < 1) return fail(KSIG_EINVAL, "n_levels must be positive");
   if (order <= 0 || order >= n_levels) order = n_levels;
+  if (max_slice < 1 || max_slice > ksig::kStreamMaxPairs) max_slice = ksig::kStreamMaxPairs;
   StreamPlan pl;
-  if (!stream_plan(d, static_kind, n_levels, order, max_len_x, max_len_y, &pl)) return 0;
+  if (!stream_plan(d, static_kind, n_levels, order, max_len_x, max_len_y, d == dmax_for(d),
+                   max_slice, &pl))
+    return 0;
   *slots = (int)pl.slots;
   return 0;
 }
@@ -286,7 +302,7 @@ int ksig_signature_levels_batched(const void* x, const int64_t* x_off, const int
                                   const int32_t* pairs, int64_t n_pairs, int static_kind,
                                   double static_param, int n_levels, int order,
                                   const int32_t* sched_order, const int32_t* wave_beg,
-                                  int n_waves, double* out, void* stream) {
+                                  int n_waves, int max_slice, double* out, void* stream) {
   int rc = check_common(KSIG_F32, x, y, d, pairs, n_pairs, out);
   if (rc) return rc;
   if (n_levels < 1) return fail(KSIG_EINVAL, "n_levels must be positive");
@@ -299,8 +315,12 @@ int ksig_signature_levels_batched(const void* x, const int64_t* x_off, const int
   if (n_pairs == 0) return 0;
   if (max_len_x < 0 || max_len_y < 0) return fail(KSIG_EINVAL, "negative max length");
   if (wave_beg && n_waves < 1) return fail(KSIG_EINVAL, "n_waves must be positive");
+  if (max_slice < 1 || max_slice > ksig::kStreamMaxPairs)
+    return fail(KSIG_EINVAL, "max_slice must be in [1, %d] (got %d)", ksig::kStreamMaxPairs,
+                max_slice);
   StreamPlan pl;
-  if (!stream_plan(d, static_kind, n_levels, order, max_len_x, max_len_y, &pl))
+  if (!stream_plan(d, static_kind, n_levels, order, max_len_x, max_len_y,
+                   stream_vec(d, x, y), max_slice, &pl))
     return fail(KSIG_EUNSUPPORTED, "no row-stream kernel for d=%d, (n_levels=%d, order=%d) "
                 "and column length %d", d, n_levels, order,
                 max_len_x < max_len_y ? max_len_x : max_len_y);
@@ -315,7 +335,9 @@ int ksig_signature_levels_batched(const void* x, const int64_t* x_off, const int
   f.order = order;
   f.difference = 1;
   f.out = out;
-  return stream_launch(pl, f, static_param, sched_order, wave_beg, n_waves, stream);
+  if (wave_beg) return stream_launch(pl, f, static_param, sched_order, wave_beg, n_waves, 0, stream);
+  const int64_t waves = (n_pairs + max_slice - 1) / max_slice;
+  return stream_launch(pl, f, static_param, sched_order, nullptr, (int)waves, max_slice, stream);
 }
 
 int ksig_signature_levels_fwd_save(int dtype, const void* x, const int64_t* x_off,
