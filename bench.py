@@ -98,15 +98,30 @@ def main():
   ap.add_argument("--cpu-seconds", type=float, default=12.0)
   ap.add_argument("--no-cpu", action="store_true")
   ap.add_argument("--no-plain", action="store_true")
-  ap.add_argument("--gram", default="fp32", choices=["valu", "fp32", "bf16x3"],
+  ap.add_argument("--gram", default="valu", choices=["valu", "fp32", "bf16x3"],
                   help="static-kernel Gram evaluation inside the sweep")
   ap.add_argument("--no-variants", action="store_true")
+  ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                  help="only run a device-to-device copy of MIB MiB (5 times): the "
+                       "reference point for scaling HBM byte counters")
   args = ap.parse_args()
   if args.gpus != 1:
     print(f"bench.py measures one GPU only (the signature-kernel path shards by "
           f"pairs with no collective in the timed loop); --gpus {args.gpus} is not "
           f"supported in this build", file=sys.stderr)
     sys.exit(2)
+
+  if args.copy_probe:
+    dev = torch.device("cuda", 0)
+    n = args.copy_probe * (1 << 20) // 4
+    src = torch.ones(n, device=dev, dtype=torch.float32)
+    dst = torch.empty_like(src)
+    for _ in range(5):
+      dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4,
+                      "calls": 5}), flush=True)
+    return
 
   import ksig_amd
   from ksig_amd import _lib, _ops
@@ -198,7 +213,7 @@ def main():
       "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                    "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
                    "traffic": None, "kernel": ("dp_mfma_kernel<5,1,8,%d>" % (0 if args.gram == "fp32" else 1)
-                              if args.gram != "valu" else "dp_forward_kernel<float,5,1,8,false>"),
+                              if args.gram != "valu" else "dp_stream_kernel<5,1,8,4,0,true>"),
                    "kernel_ms": kern_ms,
                    "flops_per_cell": flops_cell},
   }

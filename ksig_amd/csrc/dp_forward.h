@@ -49,6 +49,7 @@ struct DpStreamArgs {
   SeqSet x, y;
   int d;
   int tab_cap;             // pair-table entries in LDS (>= every slice, multiple of 4)
+  int fifo_cols;           // FIFO slots in LDS (>= every wave's column count)
   const int32_t* pairs;    // [n_pairs, 2]
   int64_t n_pairs;
   const int32_t* order;    // processing order of the pair list (null: identity)

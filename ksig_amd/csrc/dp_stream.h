@@ -65,12 +65,12 @@ struct StreamLayout {
 constexpr int kStreamMaxPairs = 128;  // pair-table entries per wave
 constexpr int kStreamMinCols = 68;    // smallest band width Pc (FIFO hazard + staging)
 
-// Dynamic LDS of the row stream: pair table of `cap` entries (cap % 4 == 0),
-// row starts, lane-private x staging, FIFO.
+// Dynamic LDS of the row stream: lane-private x staging, FIFO of `pc` slots,
+// row starts and pair table of `cap` entries (cap % 4 == 0).
 __host__ __device__ inline size_t stream_lds_bytes(int dmax, int r, int pc, int nf, int cap) {
   const int xs = ((dmax / 4) % 2 == 0) ? dmax + 4 : dmax + 8;
-  return sizeof(StreamPair) * cap + 4 * ((size_t)cap + 4) + (size_t)4 * kWave * r * xs +
-         (size_t)4 * pc * nf;
+  return (size_t)4 * kWave * r * xs + (size_t)4 * ((pc * nf + 1) & ~1) +
+         4 * ((size_t)cap + 4) + sizeof(StreamPair) * cap;
 }
 
 // Load one sequence point (d features) into v[0..DMAX).  VEC: 16-byte vector
@@ -104,15 +104,17 @@ dp_stream_kernel(const DpStreamArgs a) {
   constexpr int NPS = NLEV > 1 ? NLEV - 1 : 1;
   constexpr int NHB = NSIDE > 0 ? NSIDE : 1;
   constexpr int XS = StreamLayout<DMAX>::XS;
+  constexpr int NFP = NF;  // FIFO slot stride (unpadded: LDS size sets occupancy)
   constexpr int RB = kWave * R;
   constexpr bool RBF = KIND == kRBF;
 
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int cap = a.tab_cap;  // pair-table entries (a multiple of 4)
-  StreamPair* tab = reinterpret_cast<StreamPair*>(smem_raw);
-  int32_t* tstart = reinterpret_cast<int32_t*>(tab + cap);
-  T* xst = reinterpret_cast<T*>(tstart + cap + 4);
-  T* fifo = xst + RB * XS;
+  T* xst = reinterpret_cast<T*>(smem_raw);            // [RB][XS], 16-byte rows
+  T* fifo = xst + RB * XS;                            // [Pc][NFP]
+  // FIFO rounded to an even word count and cap % 4 == 0: the table is 8-byte aligned
+  int32_t* tstart = reinterpret_cast<int32_t*>(fifo + ((a.fifo_cols * NFP + 1) & ~1));
+  StreamPair* tab = reinterpret_cast<StreamPair*>(tstart + cap + 4);
 
   const int lane = threadIdx.x;
   const int w = blockIdx.x;
@@ -174,11 +176,12 @@ dp_stream_kernel(const DpStreamArgs a) {
     const int t = __shfl_xor(pcl, o, kWave);
     pcl = pcl > t ? pcl : t;
   }
-  const int total_rows = carry;
+  // wave-uniform values live in scalar registers
+  const int total_rows = __builtin_amdgcn_readfirstlane(carry);
   if (lane == 0) tstart[nk] = total_rows;
   __syncthreads();
   if (total_rows == 0) return;
-  const int Pc = pcl > kStreamMinCols ? pcl : kStreamMinCols;
+  const int Pc = __builtin_amdgcn_readfirstlane(pcl > kStreamMinCols ? pcl : kStreamMinCols);
   const int nb = (total_rows + RB - 1) / RB;
 
   const T gdot = a.gdot, gn = a.gn;
@@ -283,15 +286,22 @@ dp_stream_kernel(const DpStreamArgs a) {
   int next_stage = kWave - 1; // band b+1 is staged at step b*Pc + 63
   int stage_band = 1;
 
-  // y row of column c2 of this lane (c2 >= Pc: the next band's sequence)
-  auto yload = [&](T* dst, int c2) {
-    const bool nx = c2 >= Pc;
-    const T* p = nx ? pyptr : yptr;
-    int cc = nx ? c2 - Pc : c2;
-    cc = cc > 0 ? cc : 0;
-    const int cl = nx ? pccl : ccl;
-    cc = cc < cl ? cc : cl;
-    stream_load_row<DMAX, VEC>(dst, p + (int64_t)cc * d, d);
+  // Prefetch cursor, two columns ahead of cp: column a_c of the sequence at
+  // a_ptr (last column a_cl); it moves to the next band's sequence when a_c
+  // wraps (the pending metadata is staged by then).
+  int a_c = cp;
+  const T* a_ptr = yptr;
+  int a_cl = ccl;
+  auto yload = [&](T* dst) {
+    int cc = a_c > 0 ? a_c : 0;
+    cc = cc < a_cl ? cc : a_cl;
+    stream_load_row<DMAX, VEC>(dst, a_ptr + (uint32_t)cc * (uint32_t)(VEC ? DMAX : d), d);
+    ++a_c;
+    if (a_c == Pc) {
+      a_c = 0;
+      a_ptr = pyptr;
+      a_cl = pccl;
+    }
   };
 
   T y0[DMAX], y1[DMAX], y2[DMAX], fA[NF], fB[NF];
@@ -301,12 +311,12 @@ dp_stream_kernel(const DpStreamArgs a) {
       ++stage_band;
       next_stage += Pc;
     }
-    yload(ynext, cp + 2);
+    yload(ynext);
     {
       int sn = s0 + 1;
       sn = sn == Pc ? 0 : sn;
 #pragma unroll
-      for (int k = 0; k < NF; ++k) fnext[k] = fifo[sn * NF + k];
+      for (int k = 0; k < NF; ++k) fnext[k] = fifo[sn * NFP + k];
     }
     T in[NF];
     in[0] = shift_down(fl[0], fv[0]);
@@ -371,7 +381,7 @@ dp_stream_kernel(const DpStreamArgs a) {
     for (int m = 0; m < NLEV; ++m) accf[m] += cur[1 + m];
     if (lane == kWave - 1 && T_ >= kWave - 1) {
 #pragma unroll
-      for (int k = 0; k < NF; ++k) fifo[s63 * NF + k] = cur[k];
+      for (int k = 0; k < NF; ++k) fifo[s63 * NFP + k] = cur[k];
     }
     if ((T_ & 15) == 15) {  // wave-uniform flush of the FP32 partial sums
 #pragma unroll
@@ -388,13 +398,15 @@ dp_stream_kernel(const DpStreamArgs a) {
 #pragma unroll
         for (int m = 0; m < NLEV; ++m) o[m + 1] = acc[m] + (double)accf[m];
       }
-#pragma unroll
-      for (int m = 0; m < NLEV; ++m) {
-        acc[m] = 0.0;
-        accf[m] = T(0);
-      }
       cp = 0;
       enter_band();
+      if (last) {  // only the accumulators of a pair-ending pass are read
+#pragma unroll
+        for (int m = 0; m < NLEV; ++m) {
+          acc[m] = 0.0;
+          accf[m] = T(0);
+        }
+      }
     }
     ++s0;
     s0 = s0 == Pc ? 0 : s0;
@@ -402,8 +414,8 @@ dp_stream_kernel(const DpStreamArgs a) {
     s63 = s63 == Pc ? 0 : s63;
   };
 
-  yload(y0, cp);
-  yload(y1, cp + 1);
+  yload(y0);
+  yload(y1);
 #pragma unroll
   for (int k = 0; k < NF; ++k) fA[k] = fifo[k];
   int T_ = 0;

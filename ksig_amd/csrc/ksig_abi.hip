@@ -133,6 +133,7 @@ static int dp_levels_common(int dtype, ksig::DpFwdArgs& a, int cols, int dmax,
 struct StreamPlan {
   ksig::StreamEntry e;
   size_t lds;
+  int pc;         // FIFO slots
   int cap;        // pair-table entries
   int64_t slots;  // resident waves on the device
 };
@@ -148,6 +149,7 @@ static bool stream_plan(int d, int kind, int n_levels, int order, int max_len_x,
   int pc = max_len_x < max_len_y ? max_len_x : max_len_y;
   if (pc < ksig::kStreamMinCols) pc = ksig::kStreamMinCols;
   p->cap = (cap + 3) & ~3;
+  p->pc = pc;
   p->lds = ksig::stream_lds_bytes(dmax, p->e.r, pc, nf, p->cap);
   if (p->lds > max_lds_bytes()) return false;
   const int occ = stream_occupancy(p->e, p->lds);
@@ -167,6 +169,7 @@ static int stream_launch(const StreamPlan& pl, const ksig::DpFwdArgs& f, double 
   a.y = f.y;
   a.d = f.d;
   a.tab_cap = pl.cap;
+  a.fifo_cols = pl.pc;
   a.pairs = f.pairs;
   a.n_pairs = f.n_pairs;
   a.order = sched_order;
