@@ -78,14 +78,19 @@ int ksig_signature_levels(int dtype, const void* x, const int64_t* x_off,
 /*
  * Resident-wave capacity of the row-stream kernel behind
  * ksig_signature_levels_batched for a problem shape on the current device,
- * with slices of at most max_slice pairs (<= 0: the largest accepted):
- * *slots = waves that run concurrently (0 if no row-stream kernel or LDS
- * layout fits), *max_pairs_per_wave = the largest slice one wave accepts.
- * A scheduling query with no reference counterpart (the reference launches
- * one CuPy kernel sequence per Gram, ksig/kernels.py:147-164).
+ * with slices of at most max_slice pairs (<= 0: the largest accepted) and the
+ * band hand-over FIFO in LDS (global_fifo = 0) or in a global-memory
+ * workspace (global_fifo = 1; long sequences, where an LDS FIFO would cut
+ * occupancy): *slots = waves that run concurrently (0 if no row-stream kernel
+ * or LDS layout fits), *max_pairs_per_wave = the largest slice one wave
+ * accepts, *workspace_per_wave = workspace bytes per launched wave (0 for the
+ * LDS FIFO).  A scheduling query with no reference counterpart (the
+ * reference launches one CuPy kernel sequence per Gram,
+ * ksig/kernels.py:147-164).
  */
 int ksig_stream_slots(int d, int static_kind, int n_levels, int order, int max_len_x,
-                      int max_len_y, int max_slice, int* slots, int* max_pairs_per_wave);
+                      int max_len_y, int max_slice, int global_fifo, int* slots,
+                      int* max_pairs_per_wave, size_t* workspace_per_wave);
 
 /*
  * ksig_signature_levels for float32 inputs with difference=1 and a caller
@@ -96,7 +101,9 @@ int ksig_stream_slots(int d, int static_kind, int n_levels, int order, int max_l
  * index q it takes.  A wave stacks its pairs' rows and sweeps them with one
  * column count (the longest shorter side among them), so slices of similar
  * lengths waste least.  wave_beg null: slices of max_slice consecutive pairs
- * of the order.  Replaces the same reference
+ * of the order.  workspace non-null selects the global-memory FIFO and must
+ * hold n_waves * *workspace_per_wave bytes (16-byte aligned; contents are
+ * scratch); null keeps the FIFO in LDS.  Replaces the same reference
  * composition as ksig_signature_levels (ksig/kernels.py:147-164 over
  * ksig/algorithms.py:22-140) for pair lists the reference cannot take
  * (uneven lengths; the reference needs one dense [n, L, d] array,
@@ -108,7 +115,8 @@ int ksig_signature_levels_batched(const void* x, const int64_t* x_off, const int
                                   const int32_t* pairs, int64_t n_pairs, int static_kind,
                                   double static_param, int n_levels, int order,
                                   const int32_t* sched_order, const int32_t* wave_beg,
-                                  int n_waves, int max_slice, double* out, void* stream);
+                                  int n_waves, int max_slice, void* workspace,
+                                  size_t workspace_bytes, double* out, void* stream);
 
 /*
  * Training forward: ksig_signature_levels for order 1 with difference, which

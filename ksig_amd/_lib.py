@@ -22,6 +22,7 @@ GRAM_MODES = {'auto': GRAM_AUTO, 'valu': GRAM_VALU, 'fp32': GRAM_MFMA_F32,
               'bf16x3': GRAM_MFMA_BF16X3}
 
 _c_int, _c_i64, _c_dbl, _vp = ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
+_c_size = ctypes.c_size_t
 
 # name -> (restype, argtypes); mirrors include/ksig_hip.h
 SIGNATURES = {
@@ -32,10 +33,10 @@ SIGNATURES = {
         _c_int, _vp, _vp, _vp, _c_int, _vp, _vp, _vp, _c_int, _c_int, _vp,
         _c_i64, _c_int, _c_dbl, _c_int, _c_int, _c_int, _c_int, _vp, _vp]),
     "ksig_stream_slots": (_c_int, [
-        _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp]),
+        _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp]),
     "ksig_signature_levels_batched": (_c_int, [
         _vp, _vp, _vp, _c_int, _vp, _vp, _vp, _c_int, _c_int, _vp, _c_i64, _c_int,
-        _c_dbl, _c_int, _c_int, _vp, _vp, _c_int, _c_int, _vp, _vp]),
+        _c_dbl, _c_int, _c_int, _vp, _vp, _c_int, _c_int, _vp, _c_size, _vp, _vp]),
     "ksig_signature_levels_fwd_save": (_c_int, [
         _c_int, _vp, _vp, _vp, _c_int, _vp, _vp, _vp, _c_int, _c_int, _vp,
         _c_i64, _c_int, _c_dbl, _c_int, _vp, _vp, _vp, _vp]),

@@ -137,15 +137,16 @@ def signature_levels(X: SeqSet, Y: SeqSet, pairs: torch.Tensor, n_levels: int,
 
 
 def stream_slots(X: SeqSet, Y: SeqSet, kind: int, n_levels: int, order: int,
-                 max_slice: int = 0):
-  """(resident waves, max pairs per wave) of the row-stream kernel for this
-  problem shape and slice bound; resident waves is 0 when no row-stream
-  kernel fits."""
+                 max_slice: int = 0, global_fifo: bool = False):
+  """(resident waves, max pairs per wave, workspace bytes per wave) of the
+  row-stream kernel for this problem shape, slice bound and FIFO placement;
+  resident waves is 0 when no row-stream kernel fits."""
   lib = _lib.load()
-  s, m = ctypes.c_int(0), ctypes.c_int(0)
+  s, m, w = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_size_t(0)
   _lib.check(lib.ksig_stream_slots(X.d, kind, n_levels, order, X.max_len, Y.max_len,
-                                   int(max_slice), ctypes.byref(s), ctypes.byref(m)))
-  return s.value, m.value
+                                   int(max_slice), int(bool(global_fifo)), ctypes.byref(s),
+                                   ctypes.byref(m), ctypes.byref(w)))
+  return s.value, m.value, w.value
 
 
 def stream_schedule(lx: np.ndarray, ly: np.ndarray, slots: int, max_per_wave: int,
@@ -192,8 +193,8 @@ def signature_levels_batched(X: SeqSet, Y: SeqSet, pairs: torch.Tensor, n_levels
                              schedule: Optional[tuple] = None) -> torch.Tensor:
   """Level values [n_pairs, n_levels+1] of float32 pairs of uneven lengths in
   one persistent row-stream launch.  ``schedule`` = (order, wave_beg[,
-  max_slice]) with device int32 tensors (order may be None), or None for the
-  length-bucketed schedule of ``stream_schedule``."""
+  max_slice[, global_fifo]]) with device int32 tensors (order may be None),
+  or None for the length-bucketed schedule of ``make_schedule``."""
   lib = _lib.load()
   _check_sets(X, Y)
   if X.data.dtype != torch.float32:
@@ -207,30 +208,44 @@ def signature_levels_batched(X: SeqSet, Y: SeqSet, pairs: torch.Tensor, n_levels
     schedule = make_schedule(X, Y, pairs, kind, n_levels, order)
   so, wb = schedule[0], schedule[1]
   max_slice = schedule[2] if len(schedule) > 2 else 128
+  global_fifo = bool(schedule[3]) if len(schedule) > 3 else False
+  n_waves = wb.shape[0] - 1
+  ws, ws_bytes = None, 0
+  if global_fifo:
+    norm_order = n_levels if (order <= 0 or order >= n_levels) else order
+    _, _, per_wave = stream_slots(X, Y, kind, n_levels, norm_order, max_slice, True)
+    ws_bytes = per_wave * n_waves
+    ws = torch.empty(max(ws_bytes, 16), device=dev, dtype=torch.uint8)
   _lib.check(lib.ksig_signature_levels_batched(
       X.data.data_ptr(), X.off.data_ptr(), X.len.data_ptr(), X.max_len,
       Y.data.data_ptr(), Y.off.data_ptr(), Y.len.data_ptr(), Y.max_len, X.d,
       pairs.data_ptr(), n_pairs, kind, float(param), n_levels, order,
-      so.data_ptr() if so is not None else None, wb.data_ptr(), wb.shape[0] - 1,
-      int(max_slice), out.data_ptr(), _lib.stream_handle(dev)))
+      so.data_ptr() if so is not None else None, wb.data_ptr(), n_waves,
+      int(max_slice), ws.data_ptr() if ws is not None else None, ws_bytes,
+      out.data_ptr(), _lib.stream_handle(dev)))
   return out
 
 
 def make_schedule(X: SeqSet, Y: SeqSet, pairs: torch.Tensor, kind: int, n_levels: int,
                   order: int):
-  """(order, wave_beg, max_slice) of the length-bucketed schedule (device
-  int32 tensors and the largest slice)."""
+  """(order, wave_beg, max_slice, global_fifo) of the length-bucketed schedule
+  (device int32 tensors, the largest slice and the FIFO placement)."""
   norm_order = n_levels if (order <= 0 or order >= n_levels) else order
-  slots, maxp = stream_slots(X, Y, kind, n_levels, norm_order)
+  slots, maxp, _ = stream_slots(X, Y, kind, n_levels, norm_order)
+  g_slots, _, _ = stream_slots(X, Y, kind, n_levels, norm_order, 0, True)
+  # the FIFO goes to global memory when an LDS FIFO would cost resident waves
+  global_fifo = g_slots > slots
+  slots = max(slots, g_slots)
   if slots <= 0:
     raise NotImplementedError("no row-stream kernel fits this problem shape")
   pr = pairs.detach().cpu().numpy()
   lx = X.lengths()[pr[:, 0]]
   ly = Y.lengths()[pr[:, 1]]
-  so, wb = stream_schedule(lx, ly, slots, maxp)
+  so, wb = stream_schedule(lx, ly, slots, maxp,
+                           min_cols=96 if global_fifo else 68)
   dev = X.data.device
   return (torch.as_tensor(so, device=dev), torch.as_tensor(wb, device=dev),
-          int(np.diff(wb).max()) if len(wb) > 1 else 1)
+          int(np.diff(wb).max()) if len(wb) > 1 else 1, global_fifo)
 
 
 def signature_levels_gram(M: torch.Tensor, n_levels: int, order: int,

@@ -49,7 +49,8 @@ struct DpStreamArgs {
   SeqSet x, y;
   int d;
   int tab_cap;             // pair-table entries in LDS (>= every slice, multiple of 4)
-  int fifo_cols;           // FIFO slots in LDS (>= every wave's column count)
+  int fifo_cols;           // FIFO slots (>= every wave's column count)
+  void* gfifo;             // global-memory FIFO, [n_waves][fifo_cols][4-rounded flows] floats
   const int32_t* pairs;    // [n_pairs, 2]
   int64_t n_pairs;
   const int32_t* order;    // processing order of the pair list (null: identity)
@@ -127,8 +128,10 @@ bool dp_backward_supported(int n_levels);
 bool dp_mfma_supported(int n_levels, int order);
 // mode 1: float32 MFMA Gram, mode 2: bf16 hi/lo split MFMA Gram
 hipError_t dp_mfma_launch(const DpFwdArgs& a, int mode, size_t lds, hipStream_t s);
-// vec: d equals the bucket width and both data pointers are 16-byte aligned
-StreamEntry dp_stream_entry(int kind, int dmax, bool vec, int n_levels, int order);
+// vec: d equals the bucket width and both data pointers are 16-byte aligned;
+// global_fifo: band hand-over through a global-memory workspace (vec only)
+StreamEntry dp_stream_entry(int kind, int dmax, bool vec, bool global_fifo, int n_levels,
+                            int order);
 hipError_t dp_stream_launch(const StreamEntry& e, const DpStreamArgs& a, size_t lds,
                             hipStream_t s);
 int dp_stream_occupancy(const StreamEntry& e, size_t lds);

@@ -62,15 +62,28 @@ struct StreamLayout {
   static constexpr int XS = ((DMAX / 4) % 2 == 0) ? DMAX + 4 : DMAX + 8;
 };
 
-constexpr int kStreamMaxPairs = 128;  // pair-table entries per wave
-constexpr int kStreamMinCols = 68;    // smallest band width Pc (FIFO hazard + staging)
+constexpr int kStreamMaxPairs = 128;       // pair-table entries per wave
+constexpr int kStreamMinCols = 68;         // smallest band width Pc (FIFO hazard + staging)
+constexpr int kStreamMinColsGlobal = 96;   // same with the FIFO in global memory
+
+// Rows per lane of an instantiation: 4 when the per-row registers (x row,
+// norm, row prefixes, side sums, left value) stay small, else 2.
+__host__ __device__ constexpr int stream_rows(int nlev, int order, int dmax) {
+  return (dmax + 3 + (nlev - 1) + dp_num_side(nlev, order)) * 4 <= 100 ? 4 : 2;
+}
 
 // Dynamic LDS of the row stream: lane-private x staging, FIFO of `pc` slots,
 // row starts and pair table of `cap` entries (cap % 4 == 0).
-__host__ __device__ inline size_t stream_lds_bytes(int dmax, int r, int pc, int nf, int cap) {
+__host__ __device__ inline size_t stream_lds_bytes(int dmax, int r, int pc, int nf, int cap,
+                                                   bool global_fifo) {
   const int xs = ((dmax / 4) % 2 == 0) ? dmax + 4 : dmax + 8;
-  return (size_t)4 * kWave * r * xs + (size_t)4 * ((pc * nf + 1) & ~1) +
+  return (size_t)4 * kWave * r * xs + (global_fifo ? 0 : (size_t)4 * ((pc * nf + 1) & ~1)) +
          4 * ((size_t)cap + 4) + sizeof(StreamPair) * cap;
+}
+
+// Global FIFO workspace bytes: every wave owns [pc][nf rounded up to 4] floats.
+__host__ __device__ inline size_t stream_gfifo_bytes(int pc, int nf, int64_t n_waves) {
+  return (size_t)4 * pc * ((nf + 3) & ~3) * (size_t)n_waves;
 }
 
 // Load one sequence point (d features) into v[0..DMAX).  VEC: 16-byte vector
@@ -95,7 +108,7 @@ __device__ __forceinline__ void stream_load_row(float* v, const float* p, int d)
   }
 }
 
-template <int NLEV, int ORDER, int DMAX, int R, int KIND, bool VEC>
+template <int NLEV, int ORDER, int DMAX, int R, int KIND, bool VEC, bool GF>
 __global__ void __launch_bounds__(kWave)
 dp_stream_kernel(const DpStreamArgs a) {
   using T = float;
@@ -104,16 +117,18 @@ dp_stream_kernel(const DpStreamArgs a) {
   constexpr int NPS = NLEV > 1 ? NLEV - 1 : 1;
   constexpr int NHB = NSIDE > 0 ? NSIDE : 1;
   constexpr int XS = StreamLayout<DMAX>::XS;
-  constexpr int NFP = NF;  // FIFO slot stride (unpadded: LDS size sets occupancy)
+  constexpr int NFP = NF;                // LDS FIFO slot stride (unpadded: LDS sets occupancy)
+  constexpr int NFG = (NF + 3) & ~3;     // global FIFO slot stride (16-byte quads)
   constexpr int RB = kWave * R;
   constexpr bool RBF = KIND == kRBF;
 
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int cap = a.tab_cap;  // pair-table entries (a multiple of 4)
   T* xst = reinterpret_cast<T*>(smem_raw);            // [RB][XS], 16-byte rows
-  T* fifo = xst + RB * XS;                            // [Pc][NFP]
+  T* fifo = xst + RB * XS;                            // [Pc][NFP] (LDS FIFO)
   // FIFO rounded to an even word count and cap % 4 == 0: the table is 8-byte aligned
-  int32_t* tstart = reinterpret_cast<int32_t*>(fifo + ((a.fifo_cols * NFP + 1) & ~1));
+  const int lds_fifo_words = GF ? 0 : ((a.fifo_cols * NFP + 1) & ~1);
+  int32_t* tstart = reinterpret_cast<int32_t*>(fifo + lds_fifo_words);
   StreamPair* tab = reinterpret_cast<StreamPair*>(tstart + cap + 4);
 
   const int lane = threadIdx.x;
@@ -181,7 +196,10 @@ dp_stream_kernel(const DpStreamArgs a) {
   if (lane == 0) tstart[nk] = total_rows;
   __syncthreads();
   if (total_rows == 0) return;
-  const int Pc = __builtin_amdgcn_readfirstlane(pcl > kStreamMinCols ? pcl : kStreamMinCols);
+  constexpr int kMinCols = GF ? kStreamMinColsGlobal : kStreamMinCols;
+  const int Pc = __builtin_amdgcn_readfirstlane(pcl > kMinCols ? pcl : kMinCols);
+  // global FIFO: this wave's [fifo_cols][NFG] region of the workspace
+  T* gfifo = GF ? reinterpret_cast<T*>(a.gfifo) + (size_t)w * a.fifo_cols * NFG : nullptr;
   const int nb = (total_rows + RB - 1) / RB;
 
   const T gdot = a.gdot, gn = a.gn;
@@ -304,7 +322,27 @@ dp_stream_kernel(const DpStreamArgs a) {
     }
   };
 
-  T y0[DMAX], y1[DMAX], y2[DMAX], fA[NF], fB[NF];
+  // FIFO slot of lane 0 two steps ahead (prefetched like the y rows)
+  int s2 = 2;
+  auto fload = [&](T* dst) {
+    if (GF) {
+#pragma unroll
+      for (int k = 0; k < NFG; k += 4) {
+        const float4 t = *reinterpret_cast<const float4*>(gfifo + s2 * NFG + k);
+        dst[k] = t.x;
+        if (k + 1 < NF) dst[k + 1] = t.y;
+        if (k + 2 < NF) dst[k + 2] = t.z;
+        if (k + 3 < NF) dst[k + 3] = t.w;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < NF; ++k) dst[k] = fifo[s2 * NFP + k];
+    }
+    ++s2;
+    s2 = s2 == Pc ? 0 : s2;
+  };
+
+  T y0[DMAX], y1[DMAX], y2[DMAX], f0[NFG], f1[NFG], f2[NFG];
   auto step = [&](int T_, const T* yv, T* ynext, const T* fv, T* fnext) {
     if (T_ == next_stage) {  // wave-uniform
       if (stage_band < nb) stage(stage_band);
@@ -312,12 +350,7 @@ dp_stream_kernel(const DpStreamArgs a) {
       next_stage += Pc;
     }
     yload(ynext);
-    {
-      int sn = s0 + 1;
-      sn = sn == Pc ? 0 : sn;
-#pragma unroll
-      for (int k = 0; k < NF; ++k) fnext[k] = fifo[sn * NFP + k];
-    }
+    fload(fnext);
     T in[NF];
     in[0] = shift_down(fl[0], fv[0]);
 #pragma unroll
@@ -380,8 +413,22 @@ dp_stream_kernel(const DpStreamArgs a) {
 #pragma unroll
     for (int m = 0; m < NLEV; ++m) accf[m] += cur[1 + m];
     if (lane == kWave - 1 && T_ >= kWave - 1) {
+      if (GF) {
+        // read back by lane 0 at least Pc - 65 >= 4 steps later: by then the
+        // in-order vector memory counter has retired this store
 #pragma unroll
-      for (int k = 0; k < NF; ++k) fifo[s63 * NFP + k] = cur[k];
+        for (int k = 0; k < NFG; k += 4) {
+          float4 t;
+          t.x = cur[k];
+          t.y = k + 1 < NF ? cur[k + 1] : T(0);
+          t.z = k + 2 < NF ? cur[k + 2] : T(0);
+          t.w = k + 3 < NF ? cur[k + 3] : T(0);
+          *reinterpret_cast<float4*>(gfifo + s63 * NFG + k) = t;
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < NF; ++k) fifo[s63 * NFP + k] = cur[k];
+      }
     }
     if ((T_ & 15) == 15) {  // wave-uniform flush of the FP32 partial sums
 #pragma unroll
@@ -416,28 +463,24 @@ dp_stream_kernel(const DpStreamArgs a) {
 
   yload(y0);
   yload(y1);
-#pragma unroll
-  for (int k = 0; k < NF; ++k) fA[k] = fifo[k];
+  // lane 0 starts band 0 with a priming row: slots 0 and 1 are read but unused
+  s2 = 0;
+  fload(f0);
+  fload(f1);
   int T_ = 0;
   for (; T_ + 3 <= total; T_ += 3) {
-    step(T_, y0, y2, fA, fB);
-    step(T_ + 1, y1, y0, fB, fA);
-    step(T_ + 2, y2, y1, fA, fB);
-    // rotate: after three steps the roles return with fA/fB swapped
-#pragma unroll
-    for (int k = 0; k < NF; ++k) {
-      const T t = fA[k];
-      fA[k] = fB[k];
-      fB[k] = t;
-    }
+    step(T_, y0, y2, f0, f2);
+    step(T_ + 1, y1, y0, f1, f0);
+    step(T_ + 2, y2, y1, f2, f1);
   }
-  if (T_ < total) step(T_, y0, y2, fA, fB);
-  if (T_ + 1 < total) step(T_ + 1, y1, y0, fB, fA);
+  if (T_ < total) step(T_, y0, y2, f0, f2);
+  if (T_ + 1 < total) step(T_ + 1, y1, y0, f1, f0);
 }
 
-template <int NLEV, int ORDER, int DMAX, int R, int KIND, bool VEC>
+template <int NLEV, int ORDER, int DMAX, int KIND, bool VEC, bool GF>
 StreamEntry stream_entry() {
-  return StreamEntry{(const void*)dp_stream_kernel<NLEV, ORDER, DMAX, R, KIND, VEC>, R};
+  constexpr int R = stream_rows(NLEV, ORDER, DMAX);
+  return StreamEntry{(const void*)dp_stream_kernel<NLEV, ORDER, DMAX, R, KIND, VEC, GF>, R};
 }
 
 }  // namespace ksig

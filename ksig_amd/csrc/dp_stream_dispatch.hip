@@ -5,30 +5,41 @@ namespace ksig {
 
 StreamEntry dp_stream_rbf_d8v(int n_levels, int order);
 StreamEntry dp_stream_rbf_d8g(int n_levels, int order);
+StreamEntry dp_stream_rbf_d8w(int n_levels, int order);
 StreamEntry dp_stream_rbf_d16v(int n_levels, int order);
 StreamEntry dp_stream_rbf_d16g(int n_levels, int order);
+StreamEntry dp_stream_rbf_d16w(int n_levels, int order);
 StreamEntry dp_stream_rbf_d32v(int n_levels, int order);
 StreamEntry dp_stream_rbf_d32g(int n_levels, int order);
+StreamEntry dp_stream_rbf_d32w(int n_levels, int order);
 StreamEntry dp_stream_lin_d8v(int n_levels, int order);
 StreamEntry dp_stream_lin_d8g(int n_levels, int order);
+StreamEntry dp_stream_lin_d8w(int n_levels, int order);
 StreamEntry dp_stream_lin_d16v(int n_levels, int order);
 StreamEntry dp_stream_lin_d16g(int n_levels, int order);
+StreamEntry dp_stream_lin_d16w(int n_levels, int order);
 StreamEntry dp_stream_lin_d32v(int n_levels, int order);
 StreamEntry dp_stream_lin_d32g(int n_levels, int order);
+StreamEntry dp_stream_lin_d32w(int n_levels, int order);
 
-StreamEntry dp_stream_entry(int kind, int dmax, bool vec, int n_levels, int order) {
+StreamEntry dp_stream_entry(int kind, int dmax, bool vec, bool global_fifo, int n_levels,
+                            int order) {
+  if (global_fifo && !vec) return StreamEntry{nullptr, 0};
+#define KSIG_PICK_D(K, D)                                                          \
+  if (dmax == D)                                                                   \
+    return global_fifo ? dp_stream_##K##_d##D##w(n_levels, order)                  \
+                       : (vec ? dp_stream_##K##_d##D##v(n_levels, order)           \
+                              : dp_stream_##K##_d##D##g(n_levels, order));
 #define KSIG_PICK(K, KN)                                                           \
   if (kind == KN) {                                                                \
-    if (dmax == 8) return vec ? dp_stream_##K##_d8v(n_levels, order)               \
-                              : dp_stream_##K##_d8g(n_levels, order);              \
-    if (dmax == 16) return vec ? dp_stream_##K##_d16v(n_levels, order)             \
-                               : dp_stream_##K##_d16g(n_levels, order);            \
-    if (dmax == 32) return vec ? dp_stream_##K##_d32v(n_levels, order)             \
-                               : dp_stream_##K##_d32g(n_levels, order);            \
+    KSIG_PICK_D(K, 8)                                                              \
+    KSIG_PICK_D(K, 16)                                                             \
+    KSIG_PICK_D(K, 32)                                                             \
   }
   KSIG_PICK(rbf, kRBF)
   KSIG_PICK(lin, kLinear)
 #undef KSIG_PICK
+#undef KSIG_PICK_D
   return StreamEntry{nullptr, 0};
 }
 

@@ -1,10 +1,11 @@
-// Row-stream DP forward, float32, linear static kernel, d <= 8, 4 rows per lane.
+// Row-stream DP forward, float32, linear static kernel, d <= 8, LDS FIFO.
 #include "dp_stream.h"
 
 namespace ksig {
 
 #define KSIG_CASE(L, O) \
-  if (n_levels == L && order == O) return stream_entry<L, O, 8, 4, kLinear, false>();
+  if (n_levels == L && order == O) \
+    return stream_entry<L, O, 8, kLinear, false, false>();
 
 StreamEntry dp_stream_lin_d8g(int n_levels, int order) {
   KSIG_DP_LEVEL_ORDER_TABLE(KSIG_CASE)

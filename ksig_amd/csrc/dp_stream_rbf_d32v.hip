@@ -1,10 +1,11 @@
-// Row-stream DP forward, float32, RBF static kernel, d == 32 with 16-byte vector loads, 2 rows per lane.
+// Row-stream DP forward, float32, RBF static kernel, d == 32, 16-byte vector loads, LDS FIFO.
 #include "dp_stream.h"
 
 namespace ksig {
 
 #define KSIG_CASE(L, O) \
-  if (n_levels == L && order == O) return stream_entry<L, O, 32, 2, kRBF, true>();
+  if (n_levels == L && order == O) \
+    return stream_entry<L, O, 32, kRBF, true, false>();
 
 StreamEntry dp_stream_rbf_d32v(int n_levels, int order) {
   KSIG_DP_LEVEL_ORDER_TABLE(KSIG_CASE)

@@ -1,0 +1,15 @@
+// Row-stream DP forward, float32, RBF static kernel, d == 8, 16-byte vector loads, global-memory FIFO.
+#include "dp_stream.h"
+
+namespace ksig {
+
+#define KSIG_CASE(L, O) \
+  if (n_levels == L && order == O) \
+    return stream_entry<L, O, 8, kRBF, true, true>();
+
+StreamEntry dp_stream_rbf_d8w(int n_levels, int order) {
+  KSIG_DP_LEVEL_ORDER_TABLE(KSIG_CASE)
+  return StreamEntry{nullptr, 0};
+}
+
+}  // namespace ksig

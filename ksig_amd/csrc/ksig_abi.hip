@@ -133,6 +133,7 @@ static int dp_levels_common(int dtype, ksig::DpFwdArgs& a, int cols, int dmax,
 struct StreamPlan {
   ksig::StreamEntry e;
   size_t lds;
+  bool gf;        // FIFO in global memory
   int pc;         // FIFO slots
   int cap;        // pair-table entries
   int64_t slots;  // resident waves on the device
@@ -141,16 +142,18 @@ struct StreamPlan {
 // Kernel, LDS size and resident waves of the row stream for a problem with a
 // pair table of `cap` entries; false if no kernel or LDS layout fits.
 static bool stream_plan(int d, int kind, int n_levels, int order, int max_len_x,
-                        int max_len_y, bool vec, int cap, StreamPlan* p) {
+                        int max_len_y, bool vec, bool gf, int cap, StreamPlan* p) {
   const int dmax = dmax_for(d);
-  p->e = ksig::dp_stream_entry(kind, dmax, vec, n_levels, order);
+  p->e = ksig::dp_stream_entry(kind, dmax, vec, gf, n_levels, order);
   if (!p->e.fn) return false;
   const int nf = ksig::dp_flow_count(n_levels, order);
   int pc = max_len_x < max_len_y ? max_len_x : max_len_y;
-  if (pc < ksig::kStreamMinCols) pc = ksig::kStreamMinCols;
+  const int min_cols = gf ? ksig::kStreamMinColsGlobal : ksig::kStreamMinCols;
+  if (pc < min_cols) pc = min_cols;
+  p->gf = gf;
   p->cap = (cap + 3) & ~3;
   p->pc = pc;
-  p->lds = ksig::stream_lds_bytes(dmax, p->e.r, pc, nf, p->cap);
+  p->lds = ksig::stream_lds_bytes(dmax, p->e.r, pc, nf, p->cap, gf);
   if (p->lds > max_lds_bytes()) return false;
   const int occ = stream_occupancy(p->e, p->lds);
   p->slots = (int64_t)device_cus() * (occ > 0 ? occ : 1);
@@ -163,8 +166,9 @@ static bool stream_vec(int d, const void* x, const void* y) {
 
 static int stream_launch(const StreamPlan& pl, const ksig::DpFwdArgs& f, double static_param,
                          const int32_t* sched_order, const int32_t* wave_beg, int n_waves,
-                         int64_t per_wave, void* stream) {
+                         int64_t per_wave, void* workspace, void* stream) {
   ksig::DpStreamArgs a{};
+  a.gfifo = pl.gf ? workspace : nullptr;
   a.x = f.x;
   a.y = f.y;
   a.d = f.d;
@@ -202,7 +206,8 @@ static int stream_levels(const ksig::DpFwdArgs& f, int max_len_x, int max_len_y,
   StreamPlan pl;
   int64_t per = 0;
   for (int cap = 4; cap <= ksig::kStreamMaxPairs; cap += 4) {
-    if (!stream_plan(f.d, f.kind, f.n_levels, f.order, max_len_x, max_len_y, vec, cap, &pl))
+    if (!stream_plan(f.d, f.kind, f.n_levels, f.order, max_len_x, max_len_y, vec, false, cap,
+                     &pl))
       return -1;
     per = (n + pl.slots - 1) / pl.slots;
     if (per <= cap) break;
@@ -213,7 +218,7 @@ static int stream_levels(const ksig::DpFwdArgs& f, int max_len_x, int max_len_y,
   }
   if (per < 1) per = 1;
   const int64_t waves = (n + per - 1) / per;
-  return stream_launch(pl, f, static_param, nullptr, nullptr, (int)waves, per, stream);
+  return stream_launch(pl, f, static_param, nullptr, nullptr, (int)waves, per, nullptr, stream);
 }
 
 int ksig_signature_levels(int dtype, const void* x, const int64_t* x_off,
@@ -282,10 +287,13 @@ int ksig_signature_levels(int dtype, const void* x, const int64_t* x_off,
 }
 
 int ksig_stream_slots(int d, int static_kind, int n_levels, int order, int max_len_x,
-                      int max_len_y, int max_slice, int* slots, int* max_pairs_per_wave) {
-  if (!slots || !max_pairs_per_wave) return fail(KSIG_EINVAL, "null output pointer");
+                      int max_len_y, int max_slice, int global_fifo, int* slots,
+                      int* max_pairs_per_wave, size_t* workspace_per_wave) {
+  if (!slots || !max_pairs_per_wave || !workspace_per_wave)
+    return fail(KSIG_EINVAL, "null output pointer");
   *slots = 0;
   *max_pairs_per_wave = ksig::kStreamMaxPairs;
+  *workspace_per_wave = 0;
   if (d < 1 || d > 32) return 0;
   if (static_kind != ksig::kRBF && static_kind != ksig::kLinear) return 0;
   if (n_levels < 1) return fail(KSIG_EINVAL, "n_levels must be positive");
@@ -293,9 +301,11 @@ int ksig_stream_slots(int d, int static_kind, int n_levels, int order, int max_l
   if (max_slice < 1 || max_slice > ksig::kStreamMaxPairs) max_slice = ksig::kStreamMaxPairs;
   StreamPlan pl;
   if (!stream_plan(d, static_kind, n_levels, order, max_len_x, max_len_y, d == dmax_for(d),
-                   max_slice, &pl))
+                   global_fifo != 0, max_slice, &pl))
     return 0;
   *slots = (int)pl.slots;
+  if (pl.gf)
+    *workspace_per_wave = ksig::stream_gfifo_bytes(pl.pc, ksig::dp_flow_count(n_levels, order), 1);
   return 0;
 }
 
@@ -305,7 +315,8 @@ int ksig_signature_levels_batched(const void* x, const int64_t* x_off, const int
                                   const int32_t* pairs, int64_t n_pairs, int static_kind,
                                   double static_param, int n_levels, int order,
                                   const int32_t* sched_order, const int32_t* wave_beg,
-                                  int n_waves, int max_slice, double* out, void* stream) {
+                                  int n_waves, int max_slice, void* workspace,
+                                  size_t workspace_bytes, double* out, void* stream) {
   int rc = check_common(KSIG_F32, x, y, d, pairs, n_pairs, out);
   if (rc) return rc;
   if (n_levels < 1) return fail(KSIG_EINVAL, "n_levels must be positive");
@@ -321,9 +332,13 @@ int ksig_signature_levels_batched(const void* x, const int64_t* x_off, const int
   if (max_slice < 1 || max_slice > ksig::kStreamMaxPairs)
     return fail(KSIG_EINVAL, "max_slice must be in [1, %d] (got %d)", ksig::kStreamMaxPairs,
                 max_slice);
+  const bool gf = workspace != nullptr;
+  if (gf && !stream_vec(d, x, y))
+    return fail(KSIG_EUNSUPPORTED, "the global-memory FIFO needs d in {8, 16, 32} and 16-byte "
+                "aligned inputs");
   StreamPlan pl;
   if (!stream_plan(d, static_kind, n_levels, order, max_len_x, max_len_y,
-                   stream_vec(d, x, y), max_slice, &pl))
+                   stream_vec(d, x, y), gf, max_slice, &pl))
     return fail(KSIG_EUNSUPPORTED, "no row-stream kernel for d=%d, (n_levels=%d, order=%d) "
                 "and column length %d", d, n_levels, order,
                 max_len_x < max_len_y ? max_len_x : max_len_y);
@@ -338,9 +353,16 @@ int ksig_signature_levels_batched(const void* x, const int64_t* x_off, const int
   f.order = order;
   f.difference = 1;
   f.out = out;
-  if (wave_beg) return stream_launch(pl, f, static_param, sched_order, wave_beg, n_waves, 0, stream);
-  const int64_t waves = (n_pairs + max_slice - 1) / max_slice;
-  return stream_launch(pl, f, static_param, sched_order, nullptr, (int)waves, max_slice, stream);
+  const int64_t waves = wave_beg ? n_waves : (n_pairs + max_slice - 1) / max_slice;
+  if (gf) {
+    const size_t need =
+        ksig::stream_gfifo_bytes(pl.pc, ksig::dp_flow_count(n_levels, order), waves);
+    if (workspace_bytes < need || ((uintptr_t)workspace & 15) != 0)
+      return fail(KSIG_EINVAL, "workspace of %zu bytes (16-byte aligned) needed, got %zu", need,
+                  workspace_bytes);
+  }
+  return stream_launch(pl, f, static_param, sched_order, wave_beg, (int)waves,
+                       wave_beg ? 0 : max_slice, workspace, stream);
 }
 
 int ksig_signature_levels_fwd_save(int dtype, const void* x, const int64_t* x_off,

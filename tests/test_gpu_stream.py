@@ -105,7 +105,7 @@ def test_dense_gram_more_pairs_than_resident_waves(d, L, n):
   ref = O.signature_kern(O.pairwise_gram(X.astype(np.float64), None, "rbf", sig), 5, 1,
                          True, True)
   assert level_err(lev, ref) < TOL32
-  slots, _ = _ops.stream_slots(_ops.SeqSet.dense(to_dev(X)), _ops.SeqSet.dense(to_dev(X)),
+  slots, _, _ = _ops.stream_slots(_ops.SeqSet.dense(to_dev(X)), _ops.SeqSet.dense(to_dev(X)),
                                _lib.STATIC_RBF, 5, 1)
   assert 0 < slots < n * n
 
@@ -122,3 +122,51 @@ def test_ragged_signature_kernel_uses_schedule():
   got = _ops.signature_levels(X, Y, pairs, 6, 2, True, _lib.STATIC_RBF, 1.3)
   ref = _oracle(xs, ys, pairs.cpu().numpy(), 6, 2, "rbf", 1.3)
   _check(got.cpu().numpy(), ref)
+
+
+@pytest.mark.parametrize("d,nlev,order", [(8, 5, 1), (16, 8, 2), (32, 6, 1), (16, 4, 4)])
+def test_global_memory_fifo(d, nlev, order):
+  """Band hand-over through the global-memory workspace (the long-sequence
+  layout): lengths below and above the 96-column minimum band, several pairs
+  per wave, random order, against the oracle and the LDS-FIFO layout."""
+  rng = np.random.default_rng(500 + d + nlev)
+  lens = [2, 5, 40, 95, 96, 97, 150, 200, 260, 31]
+  xs, X = _ragged(rng, lens, d)
+  n = len(lens)
+  pr = np.array([[a, b] for a in range(n) for b in range(n)], dtype=np.int32)
+  perm = rng.permutation(len(pr)).astype(np.int32)
+  sig = math.sqrt(d)
+  ref = _oracle(xs, xs, pr, nlev, order, "rbf", sig)
+  pairs = to_dev(pr)
+  wb = np.array([0, 1, 9, 30, 64, 99, len(pr)], dtype=np.int32)
+  sched_g = (to_dev(perm), to_dev(wb), int(np.diff(wb).max()), True)
+  got_g = _ops.signature_levels_batched(X, X, pairs, nlev, order, _lib.STATIC_RBF, sig, sched_g)
+  _check(got_g.cpu().numpy(), ref)
+  sched_l = (to_dev(perm), to_dev(wb), int(np.diff(wb).max()), False)
+  got_l = _ops.signature_levels_batched(X, X, pairs, nlev, order, _lib.STATIC_RBF, sig, sched_l)
+  _check(got_l.cpu().numpy(), ref)
+  assert level_err(got_g.cpu().numpy().T, got_l.cpu().numpy().T) < 1e-6
+
+
+def test_c3_shape_sample_uses_workspace_fifo():
+  """A 48-pair sample of the uneven-length workload (lengths in [32, 1024],
+  d = 16, 8 levels, order 2, sigma = 4) through the length-bucketed schedule;
+  long columns select the global-memory FIFO."""
+  rng = np.random.default_rng(3)
+  n = 48
+  lx = rng.integers(32, 1025, n)
+  ly = rng.integers(32, 1025, n)
+  xs = [(O.brownian(rng, 1, L, 16)[0] + 0.1 * rng.standard_normal(16)).astype(np.float32)
+        for L in lx]
+  ys = [(O.brownian(rng, 1, L, 16)[0] + 0.1 * rng.standard_normal(16)).astype(np.float32)
+        for L in ly]
+  X = _ops.SeqSet.ragged([to_dev(s) for s in xs])
+  Y = _ops.SeqSet.ragged([to_dev(s) for s in ys])
+  pr = np.stack([np.arange(n), np.arange(n)], -1).astype(np.int32)
+  sched = _ops.make_schedule(X, Y, to_dev(pr), _lib.STATIC_RBF, 8, 2)
+  assert sched[3] is True
+  got = _ops.signature_levels_batched(X, Y, to_dev(pr), 8, 2, _lib.STATIC_RBF, 4.0, sched)
+  idx = rng.choice(n, 6, replace=False)
+  ref = _oracle([xs[i] for i in idx], [ys[i] for i in idx],
+                [(k, k) for k in range(len(idx))], 8, 2, "rbf", 4.0)
+  _check(got.cpu().numpy()[idx], ref)
