@@ -59,6 +59,65 @@ def plain_fp32_levels(X, Y, n_levels, sigma):
   return torch.stack(K, -1)
 
 
+def recursion_flops(n_levels: int, order: int) -> int:
+  """Multiply/add count of one cell of the level recursion (the reference's
+  R-matrix updates, ksig/algorithms.py:88-140, of which order 1 is the
+  special case :47-85), excluding the static kernel and ΔΔ."""
+  f = 0
+  for m in range(1, n_levels):
+    dn = min(m + 1, order)
+    f += 1 + 4 * (dn - 1) + 2 * (dn - 1) ** 2      # new R-matrix entries
+  for m in range(1, n_levels + 1):
+    dm = min(m, order)
+    f += dm * dm                                    # level sum + column flow
+    if m < n_levels:
+      dn = min(m + 1, order)
+      f += 1 + 2 * (dn - 1) * dm                    # prefix and side sums
+  return f
+
+
+def excl_cumsum(M: torch.Tensor, dims) -> torch.Tensor:
+  """Exclusive cumulative sum over `dims` (ksig/utils.py:63-90)."""
+  sl = [slice(None)] * M.dim()
+  pad = [0, 0] * M.dim()
+  for a in dims:
+    a = a % M.dim()
+    sl[a] = slice(0, -1)
+  R = M[tuple(sl)]
+  for a in dims:
+    R = torch.cumsum(R, a)
+  for a in dims:
+    a = a % M.dim()
+    pad[2 * (M.dim() - 1 - a)] = 1
+  return torch.nn.functional.pad(R, pad)
+
+
+def plain_fp32_levels_ho(X, Y, n_levels, order, sigma):
+  """Paired batches X [B, Lx, d], Y [B, Ly, d]: the reference dataflow of the
+  higher-order recursion (ksig/algorithms.py:88-140) on the materialised
+  ΔΔ RBF Gram (ksig/static/kernels.py:248-260, ksig/algorithms.py:106) in
+  FP32 torch ops."""
+  Xn = (X * X).sum(-1)
+  Yn = (Y * Y).sum(-1)
+  dot = torch.bmm(X, Y.transpose(1, 2).contiguous())
+  M = torch.exp(-(Xn[:, :, None] + Yn[:, None, :] - 2 * dot) / max(2 * sigma**2, 1e-12))
+  M = torch.diff(torch.diff(M, dim=-2), dim=-1)
+  levels = [torch.ones(M.shape[0], device=M.device, dtype=M.dtype), M.sum((-2, -1))]
+  R = {(0, 0): M}
+  for i in range(1, n_levels):
+    dd = min(i + 1, order)
+    Rn = {(0, 0): M * excl_cumsum(sum(R.values()), (-2, -1))}
+    for r in range(1, dd):
+      Rn[(0, r)] = M * excl_cumsum(sum(v for (a, b), v in R.items() if b == r - 1), (-2,)) / (r + 1)
+      Rn[(r, 0)] = M * excl_cumsum(sum(v for (a, b), v in R.items() if a == r - 1), (-1,)) / (r + 1)
+      for s2 in range(1, dd):
+        Rn[(r, s2)] = M * R[(r - 1, s2 - 1)] / ((r + 1) * (s2 + 1)) if (r - 1, s2 - 1) in R \
+            else torch.zeros_like(M)
+    R = Rn
+    levels.append(sum(v.sum((-2, -1)) for v in R.values()))
+  return torch.stack(levels, -1)
+
+
 def cpu_baseline(X64: np.ndarray, n_levels: int, sigma: float, seconds: float):
   """FP64 NumPy oracle (the reference algorithm and dataflow) on one host
   core, over seeded pairs of the same workload until `seconds` elapse."""
@@ -92,6 +151,9 @@ def cpu_baseline(X64: np.ndarray, n_levels: int, sigma: float, seconds: float):
 
 def main():
   ap = argparse.ArgumentParser()
+  ap.add_argument("--workload", default="c1", choices=["c1", "c3"],
+                  help="c1: 512 x L=200 x d=8 symmetric Gram (the headline line); "
+                       "c3: 4096 uneven-length pairs, L in [32, 1024], d=16, 8 levels, order 2")
   ap.add_argument("--gpus", type=int, default=1)
   ap.add_argument("--steps", type=int, default=20)
   ap.add_argument("--warmup", type=int, default=3)
@@ -122,6 +184,9 @@ def main():
     print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4,
                       "calls": 5}), flush=True)
     return
+
+  if args.workload == "c3":
+    return bench_c3(args)
 
   import ksig_amd
   from ksig_amd import _lib, _ops
@@ -236,6 +301,143 @@ def main():
     del plain_levels
   if not args.no_cpu:
     res["cpu_baseline"] = cpu_baseline(X64, n_levels, sigma, args.cpu_seconds)
+  print(json.dumps(res), flush=True)
+
+
+def ragged_walks(rng: np.random.Generator, lens, d: int, offset_scale: float, dev):
+  """Random walks of the given lengths (cumsum of N(0, 1/L) increments plus a
+  per-sequence per-channel offset), packed as a ragged float32 SeqSet."""
+  from ksig_amd import _ops
+  lens = np.asarray(lens, dtype=np.int64)
+  total = int(lens.sum())
+  seg = np.repeat(np.arange(len(lens)), lens)
+  inc = rng.standard_normal((total, d)) / np.sqrt(lens[seg])[:, None]
+  csum = np.cumsum(inc, 0)
+  starts = np.concatenate([[0], np.cumsum(lens)[:-1]])
+  base = np.where((starts > 0)[:, None], csum[np.maximum(starts - 1, 0)], 0.0)
+  walk = csum - base[seg] + offset_scale * rng.standard_normal((len(lens), d))[seg]
+  data = torch.as_tensor(walk.astype(np.float32), device=dev)
+  off = torch.as_tensor(starts, device=dev, dtype=torch.int64)
+  ln = torch.as_tensor(lens, device=dev, dtype=torch.int32)
+  return _ops.SeqSet(data, off, ln, int(lens.max()), len(lens), lens), walk
+
+
+def bench_c3(args):
+  """4096 pairs of uneven lengths in one persistent row-stream launch."""
+  from ksig_amd import _lib, _ops
+  from oracle import sigkern as O
+  dev = torch.device("cuda", 0)
+  torch.cuda.set_device(dev)
+  n, d, n_levels, order, sigma = 4096, 16, 8, 2, 4.0
+  rng = np.random.default_rng(2024)
+  lx = rng.integers(32, 1025, n)
+  ly = rng.integers(32, 1025, n)
+  X, xw = ragged_walks(rng, lx, d, 0.1, dev)
+  Y, yw = ragged_walks(rng, ly, d, 0.1, dev)
+  pairs = torch.stack([torch.arange(n, device=dev, dtype=torch.int32)] * 2, -1).contiguous()
+  cells = float(np.sum((lx - 1) * (ly - 1)))
+  cell_levels = cells * n_levels
+  t0 = time.perf_counter()
+  sched = _ops.make_schedule(X, Y, pairs, _lib.STATIC_RBF, n_levels, order)
+  plan_ms = (time.perf_counter() - t0) * 1e3
+
+  def step():
+    return _ops.signature_levels_batched(X, Y, pairs, n_levels, order, _lib.STATIC_RBF, sigma,
+                                         sched)
+  for _ in range(args.warmup):
+    step()
+  torch.cuda.synchronize()
+  stream = torch.cuda.current_stream(dev)
+  e0 = torch.cuda.Event(enable_timing=True)
+  e1 = torch.cuda.Event(enable_timing=True)
+  t1 = time.perf_counter()
+  e0.record(stream)
+  for _ in range(args.steps):
+    out = step()
+  e1.record(stream)
+  torch.cuda.synchronize()
+  wall = time.perf_counter() - t1
+  kern_ms = e0.elapsed_time(e1) / args.steps
+  ms_per_step = wall * 1e3 / args.steps
+
+  # per-pair error vs the FP64 oracle on a sample (inputs rounded to float32)
+  starts_x = np.concatenate([[0], np.cumsum(lx)[:-1]])
+  starts_y = np.concatenate([[0], np.cumsum(ly)[:-1]])
+  xw32 = xw.astype(np.float32).astype(np.float64)
+  yw32 = yw.astype(np.float32).astype(np.float64)
+  sample = np.random.default_rng(5).choice(n, 16, replace=False)
+  got = out.cpu().numpy()
+  errs = []
+  for p in sample:
+    xa = xw32[starts_x[p]:starts_x[p] + lx[p]]
+    yb = yw32[starts_y[p]:starts_y[p] + ly[p]]
+    ref = O.pair_levels(xa, yb, n_levels, order, True, "rbf", sigma)
+    errs.append(float(np.max(np.abs(got[p] - ref)) / np.max(np.abs(ref))))
+
+  res = {
+      "metric": "DP-cells/sec (pairs x (Lx-1)(Ly-1) x levels), fused fwd, fp32, uneven lengths",
+      "value": cell_levels / (ms_per_step * 1e-3),
+      "unit": "DP cell-levels/s",
+      "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+      "ms_per_step": ms_per_step,
+      "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+      "dtype": "f32", "data": "synthetic (seeded random walks with channel offsets)",
+      "config": {"workload": "batched uneven DP kernel: 4096 pairs, L ~ U{32..1024} per "
+                             "sequence, d=16, 8 levels, order 2, RBF sigma=4",
+                 "cells": cells, "n_waves": int(sched[1].shape[0] - 1),
+                 "global_fifo": bool(sched[3]), "plan_ms": plan_ms},
+      "max_rel_err_vs_fp64": max(errs),
+      "err_sample": "16 random pairs, normwise over each pair's level vector",
+  }
+  flops_cell = recursion_flops(n_levels, order) + 3 + 2 * d + 3
+  achieved = cells * flops_cell / (kern_ms * 1e-3) / 1e12
+  res["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "kernel": "dp_stream_kernel<8,2,16,2,0,true,%s>" %
+                               ("true" if sched[3] else "false"),
+                     "kernel_ms": kern_ms, "flops_per_cell": flops_cell}
+  if not args.no_plain:
+    # plain FP32: the reference dataflow over length buckets (pairs padded to
+    # the bucket's lengths by repeating their last point: ΔΔ of the padding
+    # is exactly zero), timed on a sample of pairs, counted in true cells
+    prng = np.random.default_rng(11)
+    sub = prng.choice(n, 512, replace=False)
+    bl = 256
+    keyx = -(-lx[sub] // bl) * bl
+    keyy = -(-ly[sub] // bl) * bl
+    groups = {}
+    for p, kx, ky in zip(sub, keyx, keyy):
+      groups.setdefault((int(kx), int(ky)), []).append(int(p))
+
+    def padded(walk, starts, lens, idx, L):
+      out = np.empty((len(idx), L, d), dtype=np.float32)
+      for k, p in enumerate(idx):
+        seq = walk[starts[p]:starts[p] + lens[p]]
+        out[k, :lens[p]] = seq
+        out[k, lens[p]:] = seq[-1]
+      return torch.as_tensor(out, device=dev)
+    batches = [(padded(xw, starts_x, lx, idx, kx), padded(yw, starts_y, ly, idx, ky), idx)
+               for (kx, ky), idx in groups.items()]
+    plain_out = {}
+    for bx, by, idx in batches:   # warm-up + results
+      lev = plain_fp32_levels_ho(bx, by, n_levels, order, sigma)
+      for k, p in enumerate(idx):
+        plain_out[p] = lev[k]
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    for bx, by, idx in batches:
+      plain_fp32_levels_ho(bx, by, n_levels, order, sigma)
+    torch.cuda.synchronize()
+    tp = time.perf_counter() - t2
+    sub_cells = float(np.sum((lx[sub] - 1) * (ly[sub] - 1))) * n_levels
+    pv = sub_cells / tp
+    perr = max(float(torch.max(torch.abs(plain_out[p].cpu() - out[p].cpu())) /
+                     torch.max(torch.abs(out[p].cpu()))) for p in sub[:8])
+    res["plain_fp32"] = {"value": pv, "unit": "DP cell-levels/s",
+                         "speedup_fused_over_plain": res["value"] / pv,
+                         "sample": "512 random pairs batched in length buckets of 256 "
+                                   "(about 32 pairs per batch), torch FP32 ops",
+                         "max_rel_diff_vs_fused": perr}
   print(json.dumps(res), flush=True)
 
 

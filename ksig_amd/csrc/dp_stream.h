@@ -342,15 +342,15 @@ dp_stream_kernel(const DpStreamArgs a) {
     s2 = s2 == Pc ? 0 : s2;
   };
 
-  T y0[DMAX], y1[DMAX], y2[DMAX], f0[NFG], f1[NFG], f2[NFG];
-  auto step = [&](int T_, const T* yv, T* ynext, const T* fv, T* fnext) {
+  // Two register sets per prefetched stream: a step consumes its set, then
+  // refills it with the data of the step after next.
+  T y0[DMAX], y1[DMAX], f0[NFG], f1[NFG];
+  auto step = [&](int T_, T* yv, T* fv) {
     if (T_ == next_stage) {  // wave-uniform
       if (stage_band < nb) stage(stage_band);
       ++stage_band;
       next_stage += Pc;
     }
-    yload(ynext);
-    fload(fnext);
     T in[NF];
     in[0] = shift_down(fl[0], fv[0]);
 #pragma unroll
@@ -358,6 +358,7 @@ dp_stream_kernel(const DpStreamArgs a) {
       const T t = shift_down(fl[k], fv[k]);
       in[k] = prim ? T(0) : t;
     }
+    fload(fv);
     T yn = T(0);
     if (RBF) {
 #pragma unroll
@@ -379,6 +380,7 @@ dp_stream_kernel(const DpStreamArgs a) {
         g[j] = s;
       }
     }
+    yload(yv);
     const bool keep = cp > 0;
     const T keepf = keep ? T(1) : T(0);
     T Mk[R];
@@ -468,13 +470,11 @@ dp_stream_kernel(const DpStreamArgs a) {
   fload(f0);
   fload(f1);
   int T_ = 0;
-  for (; T_ + 3 <= total; T_ += 3) {
-    step(T_, y0, y2, f0, f2);
-    step(T_ + 1, y1, y0, f1, f0);
-    step(T_ + 2, y2, y1, f2, f1);
+  for (; T_ + 2 <= total; T_ += 2) {
+    step(T_, y0, f0);
+    step(T_ + 1, y1, f1);
   }
-  if (T_ < total) step(T_, y0, y2, f0, f2);
-  if (T_ + 1 < total) step(T_ + 1, y1, y0, f1, f0);
+  if (T_ < total) step(T_, y0, f0);
 }
 
 template <int NLEV, int ORDER, int DMAX, int KIND, bool VEC, bool GF>
