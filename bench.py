@@ -151,9 +151,10 @@ def cpu_baseline(X64: np.ndarray, n_levels: int, sigma: float, seconds: float):
 
 def main():
   ap = argparse.ArgumentParser()
-  ap.add_argument("--workload", default="c1", choices=["c1", "c3"],
+  ap.add_argument("--workload", default="c1", choices=["c1", "c3", "c4"],
                   help="c1: 512 x L=200 x d=8 symmetric Gram (the headline line); "
-                       "c3: 4096 uneven-length pairs, L in [32, 1024], d=16, 8 levels, order 2")
+                       "c3: 4096 uneven-length pairs, L in [32, 1024], d=16, 8 levels, order 2; "
+                       "c4: MMD^2 training step, 1024 x 1024 paths, L=128, d=32, 6 levels")
   ap.add_argument("--gpus", type=int, default=1)
   ap.add_argument("--steps", type=int, default=20)
   ap.add_argument("--warmup", type=int, default=3)
@@ -187,6 +188,8 @@ def main():
 
   if args.workload == "c3":
     return bench_c3(args)
+  if args.workload == "c4":
+    return bench_c4(args)
 
   import ksig_amd
   from ksig_amd import _lib, _ops
@@ -438,6 +441,216 @@ def bench_c3(args):
                          "sample": "512 random pairs batched in length buckets of 256 "
                                    "(about 32 pairs per batch), torch FP32 ops",
                          "max_rel_diff_vs_fused": perr}
+  print(json.dumps(res), flush=True)
+
+
+def plain_levels_fo(X, Y, n_levels, sigma):
+  """All pairs of X [n, L, d] and Y [m, L, d]: the reference dataflow of the
+  first-order recursion (ksig/algorithms.py:47-85) on the materialised ΔΔ RBF
+  Gram, in the inputs' dtype, differentiable by torch autograd."""
+  Xn = (X * X).sum(-1)
+  Yn = (Y * Y).sum(-1)
+  dot = torch.einsum("ail,bjl->abij", X, Y)
+  M = torch.exp(-(Xn[:, None, :, None] + Yn[None, :, None, :] - 2 * dot) /
+                max(2 * sigma**2, 1e-12))
+  M = torch.diff(torch.diff(M, dim=-2), dim=-1)
+  K = [torch.ones(M.shape[:2], device=M.device, dtype=M.dtype), M.sum((-2, -1))]
+  R = M
+  for _ in range(1, n_levels):
+    R = M * excl_cumsum(R, (-2, -1))
+    K.append(R.sum((-2, -1)))
+  return torch.stack(K, -1)
+
+
+def robust_sqrt(v, eps=1e-12):
+  """ksig/utils.py:173-182 with the level normalisation of ksig/kernels.py:180-214."""
+  return torch.clamp(torch.sqrt(torch.clamp(v, min=eps)), min=eps)
+
+
+def flat_sum(t: torch.Tensor) -> torch.Tensor:
+  """Sum in two per-row stages (rows of 1024), so no reduction spans several
+  blocks: such reductions record memset nodes, which misbehave in replayed
+  HIP graphs."""
+  v = t.reshape(-1)
+  pad = (-v.numel()) % 1024
+  if pad:
+    v = torch.nn.functional.pad(v, (0, pad))
+  return v.view(-1, 1024).sum(1).sum()
+
+
+def bench_c4(args):
+  """MMD^2 training step: generated paths with a learnable per-channel scale
+  against fixed targets; forward + adjoint, eager and HIP-graph captured."""
+  from ksig_amd import _lib, _ops, autograd
+  dev = torch.device("cuda", 0)
+  torch.cuda.set_device(dev)
+  n, L, d, n_levels = 1024, 128, 32, 6
+  sigma = math.sqrt(d)
+  gen = torch.Generator(device=dev).manual_seed(44)
+  Z = torch.randn((n, L, d), generator=gen, device=dev)
+  Y = brownian(gen, n, L, d, dev)
+  scale = torch.full((d,), 0.5, device=dev, dtype=torch.float32, requires_grad=True)
+  Ys = _ops.SeqSet.dense(Y)
+  triu = _ops.triu_pairs(n, dev)
+  full = _ops.dense_pairs(n, n, dev)
+  ti, tj = triu[:, 0].long(), triu[:, 1].long()
+  fi, fj = full[:, 0].long(), full[:, 1].long()
+  diag_pos = torch.nonzero(ti == tj).flatten()
+  off_x = torch.arange(n, device=dev, dtype=torch.int64) * L
+  len_x = torch.full((n,), L, device=dev, dtype=torch.int32)
+  with torch.no_grad():
+    lev_yy = _ops.signature_levels(Ys, Ys, triu, n_levels, 1, True, _lib.STATIC_RBF, sigma)
+    dy = lev_yy[diag_pos]
+    kyy = (lev_yy / (robust_sqrt(dy[ti]) * robust_sqrt(dy[tj]))).mean(-1)
+    syy = 2.0 * kyy[ti != tj].sum() / (n * (n - 1))
+  offmask = (ti != tj).to(torch.float64)  # masks, not boolean indexing: capturable
+  rdy_f = robust_sqrt(dy)[fj]
+
+  def loss_fn(fwd_only=False):
+    Xg = torch.cumsum(scale * Z, 1)
+    xd = Xg.reshape(-1, d)
+    Xs = _ops.SeqSet(xd, off_x, len_x, L, n)
+    if fwd_only:
+      lev_xx = _ops.signature_levels(Xs, Xs, triu, n_levels, 1, True, _lib.STATIC_RBF, sigma)
+      lev_xy = _ops.signature_levels(Xs, Ys, full, n_levels, 1, True, _lib.STATIC_RBF, sigma)
+    else:
+      lev_xx = autograd.signature_levels(Xs, Xs, triu, n_levels, _lib.STATIC_RBF, sigma,
+                                         xdata=xd, ydata=xd)
+      lev_xy = autograd.signature_levels(Xs, Ys, full, n_levels, _lib.STATIC_RBF, sigma,
+                                         xdata=xd, ydata=Ys.data)
+    # index_select (backward: atomic index_add) rather than advanced indexing
+    rdx = robust_sqrt(torch.index_select(lev_xx, 0, diag_pos))
+    kxx = (lev_xx / (torch.index_select(rdx, 0, ti) * torch.index_select(rdx, 0, tj))).mean(-1)
+    kxy = (lev_xy / (torch.index_select(rdx, 0, fi) * rdy_f)).mean(-1)
+    sxx = 2.0 * flat_sum(kxx * offmask) / (n * (n - 1))
+    return sxx + syy - 2.0 * flat_sum(kxy) / kxy.numel()
+
+  def train_step():
+    scale.grad = None
+    loss = loss_fn()
+    loss.backward()
+    return loss
+
+  cells = (triu.shape[0] + full.shape[0]) * (L - 1) ** 2
+  cell_levels = cells * n_levels
+  stream = torch.cuda.current_stream(dev)
+
+  def timed(fn, reps):
+    fn()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(reps):
+      fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / reps, e0.elapsed_time(e1) / reps
+
+  with torch.no_grad():
+    fwd_wall, fwd_ms = timed(lambda: loss_fn(True), max(2, args.steps // 2))
+  for _ in range(args.warmup):
+    train_step()
+  eager_wall, eager_ms = timed(train_step, args.steps)
+  loss_eager = float(train_step())
+  grad_eager = scale.grad.detach().clone()
+
+  # HIP graph of the whole step (forward, adjoint kernels, contractions)
+  side = torch.cuda.Stream(dev)
+  side.wait_stream(stream)
+  with torch.cuda.stream(side):
+    for _ in range(2):
+      train_step()
+  stream.wait_stream(side)
+  graph = torch.cuda.CUDAGraph()
+  scale.grad = None
+  with torch.cuda.graph(graph):
+    static_loss = loss_fn()
+    static_loss.backward()
+  static_grad = scale.grad
+  replay_err = 0.0
+  for _ in range(3):   # replays against the eager step, eager work in between
+    graph.replay()
+    torch.cuda.synchronize()
+    replay_err = max(replay_err, abs(float(static_loss) - loss_eager) / abs(loss_eager),
+                     float((static_grad - grad_eager).abs().max() / grad_eager.abs().max()))
+    _ = torch.cumsum(Z[:4], 1).sum().item()
+  graph_wall, graph_ms = timed(graph.replay, args.steps)
+
+  # gradients vs autograd through the FP64 transcription, 16 x 16 subset
+  ns = 16
+  sc64 = scale.detach().double().clone().requires_grad_(True)
+  Xg64 = torch.cumsum(sc64 * Z[:ns].double(), 1)
+  kxx64 = plain_levels_fo(Xg64, Xg64, n_levels, sigma)
+  kyy64 = plain_levels_fo(Y[:ns].double(), Y[:ns].double(), n_levels, sigma)
+  kxy64 = plain_levels_fo(Xg64, Y[:ns].double(), n_levels, sigma)
+
+  def mmd_sub(kxx, kyy, kxy):
+    dxs = torch.diagonal(kxx, 0, 0, 1).t()
+    dys = torch.diagonal(kyy, 0, 0, 1).t()
+    nx = (kxx / (robust_sqrt(dxs[:, None]) * robust_sqrt(dxs[None]))).mean(-1)
+    ny = (kyy / (robust_sqrt(dys[:, None]) * robust_sqrt(dys[None]))).mean(-1)
+    nxy = (kxy / (robust_sqrt(dxs[:, None]) * robust_sqrt(dys[None]))).mean(-1)
+    return autograd.mmd2_unbiased(nx, ny, nxy)
+  l64 = mmd_sub(kxx64, kyy64, kxy64)
+  l64.backward()
+  sc32 = scale.detach().clone().requires_grad_(True)
+  Xg32 = torch.cumsum(sc32 * Z[:ns], 1)
+  Xs32 = _ops.SeqSet.dense(Xg32.detach())
+  Ys32 = _ops.SeqSet.dense(Y[:ns].contiguous())
+  pp = _ops.dense_pairs(ns, ns, dev)
+  xd32 = Xg32.reshape(-1, d)
+  k_xx = autograd.signature_levels(Xs32, Xs32, pp, n_levels, _lib.STATIC_RBF, sigma,
+                                   xdata=xd32, ydata=xd32).view(ns, ns, -1)
+  k_yy = _ops.signature_levels(Ys32, Ys32, pp, n_levels, 1, True, _lib.STATIC_RBF,
+                               sigma).view(ns, ns, -1)
+  k_xy = autograd.signature_levels(Xs32, Ys32, pp, n_levels, _lib.STATIC_RBF, sigma,
+                                   xdata=xd32, ydata=Ys32.data).view(ns, ns, -1)
+  l32 = mmd_sub(k_xx, k_yy, k_xy)
+  l32.backward()
+  grad_err = float((sc32.grad.double() - sc64.grad).abs().max() / sc64.grad.abs().max())
+  loss_err = abs(float(l32) - float(l64)) / abs(float(l64))
+
+  res = {
+      "metric": "DP-cells/sec (pairs x (L-1)^2 x levels), MMD^2 training step fwd+bwd, fp32",
+      "value": cell_levels / (graph_wall * 1e-3),
+      "unit": "DP cell-levels/s",
+      "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+      "ms_per_step": graph_wall,
+      "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+      "dtype": "f32", "data": "synthetic (seeded Gaussian increments, learnable scale 0.5)",
+      "config": {"workload": "MMD^2 training step: 1024 generated (L=128, d=32) vs 1024 "
+                             "Brownian targets, DP kernel 6 levels, order 1, RBF sigma=sqrt(32), "
+                             "normalised levels; K_XX on the upper triangle, K_YY constant",
+                 "pairs": int(triu.shape[0] + full.shape[0]), "cells": cells},
+      "fwd_ms": fwd_wall, "fwd_bwd_eager_ms": eager_wall, "fwd_bwd_graph_ms": graph_wall,
+      "bwd_over_fwd": graph_wall / fwd_wall,
+      "graph_replay_max_rel_diff_vs_eager": replay_err,
+      "grad_scale_max_rel_err_vs_fp64": grad_err, "loss_rel_err_vs_fp64": loss_err,
+      "grad_check": "16 x 16 subset, autograd through the FP64 torch transcription",
+  }
+  if not args.no_plain:
+    # plain FP32 fallback: the reference dataflow with torch autograd, on an
+    # 8 x 1024 slice of K_XY
+    sc = scale.detach().clone().requires_grad_(True)
+
+    def plain_step():
+      sc.grad = None
+      Xg = torch.cumsum(sc * Z[:8], 1)
+      lv = plain_levels_fo(Xg, Y, n_levels, sigma)
+      lv.mean().backward()
+    plain_step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(2):
+      plain_step()
+    torch.cuda.synchronize()
+    tp = (time.perf_counter() - t0) / 2
+    pv = 8 * n * (L - 1) ** 2 * n_levels / tp
+    res["plain_fp32"] = {"value": pv, "unit": "DP cell-levels/s (fwd+bwd)",
+                         "speedup_fused_over_plain": res["value"] / pv,
+                         "sample": "8 x 1024 pairs of K_XY, torch FP32 ops + autograd"}
   print(json.dumps(res), flush=True)
 
 

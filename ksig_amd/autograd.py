@@ -37,24 +37,44 @@ def _as_param(p, device) -> torch.Tensor:
   return torch.tensor(float(p), device=device, dtype=torch.float64)
 
 
+def _uniform_lengths(Xs: _ops.SeqSet, Ys: _ops.SeqSet):
+  """(len_x, len_y) when every sequence of each set has its set's maximum
+  length (known on the host: no device synchronisation), else None."""
+  if Xs.uniform and Ys.uniform:
+    return Xs.max_len, Ys.max_len
+  return None
+
+
 class _SigLevels(torch.autograd.Function):
+  """For sets of uniform length the forward and the adjoint size every buffer
+  on the host and never synchronise with the device, so a whole training step
+  can be captured in a graph."""
 
   @staticmethod
-  def forward(ctx, xdata, ydata, param, xs_meta, ys_meta, pairs, n_levels, kind):
+  def forward(ctx, xdata, ydata, param, pval, xs_meta, ys_meta, pairs, n_levels, kind):
     lib = _lib.load()
     Xs = _ops.SeqSet(xdata.detach().contiguous(), *xs_meta)
     Ys = _ops.SeqSet(ydata.detach().contiguous(), *ys_meta)
-    pval = float(param.detach())
+    if pval is None:
+      pval = float(param.detach())
     n_pairs = pairs.shape[0]
     dev = xdata.device
     out = torch.empty((n_pairs, n_levels + 1), device=dev, dtype=torch.float64)
-    la = Xs.len.to(torch.int64)[pairs[:, 0].long()]
-    lb = Ys.len.to(torch.int64)[pairs[:, 1].long()]
-    per = (n_levels - 1) * torch.clamp(la + lb - 2, min=0)
-    save_off = torch.zeros(n_pairs, device=dev, dtype=torch.int64)
-    if n_pairs > 1:
-      save_off[1:] = torch.cumsum(per, 0)[:-1]
-    total = int(per.sum().item()) if n_pairs else 0
+    uni = _uniform_lengths(Xs, Ys)
+    if uni is not None:
+      A, B = uni
+      per_pair = (n_levels - 1) * max(A + B - 2, 0)
+      save_off = torch.arange(n_pairs, device=dev, dtype=torch.int64) * per_pair
+      total = n_pairs * per_pair
+      la = lb = torch.empty(0, device=dev, dtype=torch.int64)
+    else:
+      la = Xs.len.to(torch.int64)[pairs[:, 0].long()]
+      lb = Ys.len.to(torch.int64)[pairs[:, 1].long()]
+      per = (n_levels - 1) * torch.clamp(la + lb - 2, min=0)
+      save_off = torch.zeros(n_pairs, device=dev, dtype=torch.int64)
+      if n_pairs > 1:
+        save_off[1:] = torch.cumsum(per, 0)[:-1]
+      total = int(per.sum().item()) if n_pairs else 0
     save = torch.empty(max(total, 1), device=dev, dtype=Xs.data.dtype)
     if n_pairs:
       _lib.check(lib.ksig_signature_levels_fwd_save(
@@ -64,14 +84,14 @@ class _SigLevels(torch.autograd.Function):
           n_levels, out.data_ptr(), save.data_ptr(), save_off.data_ptr(),
           _lib.stream_handle(dev)))
     ctx.save_for_backward(xdata, ydata, pairs, save, save_off, la, lb)
-    ctx.meta = (xs_meta, ys_meta, n_levels, kind, pval)
+    ctx.meta = (xs_meta, ys_meta, n_levels, kind, pval, uni)
     return out
 
   @staticmethod
   def backward(ctx, gout):
     lib = _lib.load()
     xdata, ydata, pairs, save, save_off, la, lb = ctx.saved_tensors
-    xs_meta, ys_meta, n_levels, kind, pval = ctx.meta
+    xs_meta, ys_meta, n_levels, kind, pval, uni = ctx.meta
     Xs = _ops.SeqSet(xdata.detach().contiguous(), *xs_meta)
     Ys = _ops.SeqSet(ydata.detach().contiguous(), *ys_meta)
     dev = xdata.device
@@ -82,22 +102,31 @@ class _SigLevels(torch.autograd.Function):
     gy = torch.zeros_like(Ys.data) if ctx.needs_input_grad[1] else None
     gparam_total = torch.zeros((), device=dev, dtype=torch.float64)
     if n_pairs == 0:
-      return gx, gy, gparam_total, None, None, None, None, None
-    sizes = (la * lb)
+      return gx, gy, gparam_total, None, None, None, None, None, None
     esz = Xs.data.element_size()
-    # chunk pairs so that omega stays within the scratch budget
-    budget = max(_OMEGA_BYTES // esz, int(sizes.max().item()))
-    chunk_id = ((torch.cumsum(sizes, 0) - sizes) // budget).cpu()
-    change = torch.nonzero(chunk_id[1:] != chunk_id[:-1]).flatten() + 1
-    bounds = [0] + change.tolist() + [n_pairs]
     s_rbf = 1.0 / max(2.0 * pval * pval, 1e-12)
+    # chunk pairs so that omega stays within the scratch budget
+    if uni is not None:
+      A, B = uni
+      step = max(1, _OMEGA_BYTES // (esz * max(A * B, 1)))
+      bounds = list(range(0, n_pairs, step)) + [n_pairs]
+    else:
+      sizes = (la * lb)
+      budget = max(_OMEGA_BYTES // esz, int(sizes.max().item()))
+      chunk_id = ((torch.cumsum(sizes, 0) - sizes) // budget).cpu()
+      change = torch.nonzero(chunk_id[1:] != chunk_id[:-1]).flatten() + 1
+      bounds = [0] + change.tolist() + [n_pairs]
     for c0, c1 in zip(bounds[:-1], bounds[1:]):
       pc = pairs[c0:c1].contiguous()
-      sz = sizes[c0:c1]
-      w_off = torch.zeros(c1 - c0, device=dev, dtype=torch.int64)
-      if c1 - c0 > 1:
-        w_off[1:] = torch.cumsum(sz, 0)[:-1]
-      omega = torch.empty(int(sz.sum().item()), device=dev, dtype=dt)
+      if uni is not None:
+        w_off = torch.arange(c1 - c0, device=dev, dtype=torch.int64) * (A * B)
+        omega = torch.empty(max((c1 - c0) * A * B, 1), device=dev, dtype=dt)
+      else:
+        sz = sizes[c0:c1]
+        w_off = torch.zeros(c1 - c0, device=dev, dtype=torch.int64)
+        if c1 - c0 > 1:
+          w_off[1:] = torch.cumsum(sz, 0)[:-1]
+        omega = torch.empty(int(sz.sum().item()), device=dev, dtype=dt)
       gp = torch.empty(c1 - c0, device=dev, dtype=torch.float64)
       _lib.check(lib.ksig_signature_levels_bwd(
           _lib.dtype_code(dt), Xs.data.data_ptr(), Xs.off.data_ptr(), Xs.len.data_ptr(),
@@ -105,10 +134,16 @@ class _SigLevels(torch.autograd.Function):
           Ys.max_len, Xs.d, pc.data_ptr(), c1 - c0, kind, pval, n_levels,
           gout[c0:c1].data_ptr(), save.data_ptr(), save_off[c0:c1].data_ptr(),
           omega.data_ptr(), w_off.data_ptr(), gp.data_ptr(), _lib.stream_handle(dev)))
-      gparam_total = gparam_total + gp.sum()
+      if ctx.needs_input_grad[2]:
+        gparam_total = gparam_total + gp.sum()
       if gx is None and gy is None:
         continue
-      _contract(omega, w_off, pc, Xs, Ys, la[c0:c1], lb[c0:c1], kind, pval, s_rbf, gx, gy)
+      if uni is not None:
+        A, B = uni
+        _contract_block(omega[:(c1 - c0) * A * B].view(-1, A, B), pc[:, 0].long(),
+                        pc[:, 1].long(), A, B, Xs, Ys, kind, pval, s_rbf, gx, gy)
+      else:
+        _contract(omega, w_off, pc, Xs, Ys, la[c0:c1], lb[c0:c1], kind, pval, s_rbf, gx, gy)
     if kind == _lib.STATIC_RBF:
       den = 2.0 * pval * pval
       if den > 1e-12:
@@ -118,7 +153,9 @@ class _SigLevels(torch.autograd.Function):
         gparam_total = gparam_total * 0.0
     else:
       gparam_total = gparam_total / pval
-    return gx, gy, gparam_total, None, None, None, None, None
+    if not ctx.needs_input_grad[2]:
+      gparam_total = None
+    return gx, gy, gparam_total, None, None, None, None, None, None
 
 
 def _contract(omega, w_off, pairs, Xs, Ys, la, lb, kind, pval, s_rbf, gx, gy):
@@ -167,10 +204,13 @@ def signature_levels(X: _ops.SeqSet, Y: _ops.SeqSet, pairs: torch.Tensor, n_leve
   ``param`` may be a tensor requiring grad."""
   xdata = X.data if xdata is None else xdata
   ydata = Y.data if ydata is None else ydata
-  p = _as_param(param, xdata.device)
-  xs_meta = (X.off, X.len, X.max_len, X.n)
-  ys_meta = (Y.off, Y.len, Y.max_len, Y.n)
-  return _SigLevels.apply(xdata, ydata, p, xs_meta, ys_meta, pairs, n_levels, kind)
+  if isinstance(param, torch.Tensor):
+    p, pval = _as_param(param, xdata.device), None
+  else:  # a fixed parameter: its value stays on the host
+    p, pval = None, float(param)
+  xs_meta = (X.off, X.len, X.max_len, X.n, X.host_len)
+  ys_meta = (Y.off, Y.len, Y.max_len, Y.n, Y.host_len)
+  return _SigLevels.apply(xdata, ydata, p, pval, xs_meta, ys_meta, pairs, n_levels, kind)
 
 
 def mmd2_unbiased(Kxx: torch.Tensor, Kyy: torch.Tensor, Kxy: torch.Tensor) -> torch.Tensor:
