@@ -59,6 +59,9 @@ struct DpStreamArgs {
   int per_wave;
   float gdot, gn;          // static-kernel factors folded into the row side
   double* out;             // [n_pairs, n_levels + 1]
+  // training forward (order 1): the adjoint's boundary state, as in DpFwdArgs
+  void* save;
+  const int64_t* save_off;
 };
 
 // A compiled row-stream kernel: entry point and rows per lane.
@@ -132,6 +135,9 @@ hipError_t dp_mfma_launch(const DpFwdArgs& a, int mode, size_t lds, hipStream_t 
 // global_fifo: band hand-over through a global-memory workspace (vec only)
 StreamEntry dp_stream_entry(int kind, int dmax, bool vec, bool global_fifo, int n_levels,
                             int order);
+// training forward (order 1, vector loads, LDS FIFO) that also writes the
+// adjoint's boundary state
+StreamEntry dp_stream_save_entry(int kind, int dmax, int n_levels);
 hipError_t dp_stream_launch(const StreamEntry& e, const DpStreamArgs& a, size_t lds,
                             hipStream_t s);
 int dp_stream_occupancy(const StreamEntry& e, size_t lds);

@@ -108,7 +108,7 @@ __device__ __forceinline__ void stream_load_row(float* v, const float* p, int d)
   }
 }
 
-template <int NLEV, int ORDER, int DMAX, int R, int KIND, bool VEC, bool GF>
+template <int NLEV, int ORDER, int DMAX, int R, int KIND, bool VEC, bool GF, bool SAVE>
 __global__ void __launch_bounds__(kWave)
 dp_stream_kernel(const DpStreamArgs a) {
   using T = float;
@@ -121,6 +121,8 @@ dp_stream_kernel(const DpStreamArgs a) {
   constexpr int NFG = (NF + 3) & ~3;     // global FIFO slot stride (16-byte quads)
   constexpr int RB = kWave * R;
   constexpr bool RBF = KIND == kRBF;
+  constexpr int NC = NLEV - 1;  // saved levels (SAVE: order 1 only)
+  static_assert(!SAVE || ORDER == 1, "the adjoint boundary state exists for order 1");
 
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int cap = a.tab_cap;  // pair-table entries (a multiple of 4)
@@ -215,6 +217,11 @@ dp_stream_kernel(const DpStreamArgs a) {
   const T* pyptr = nullptr;
   int pccl = 0, pq = 0;
   bool pprim = true, plast = false;
+  // SAVE: the pair's boundary-state block, its local first row, Nr
+  T* sv = nullptr;
+  int sr0 = 0, snr = 0;
+  T* psv = nullptr;
+  int psr0 = 0, psnr = 0;
 
   // Stage band `band` for this lane: find the pair of its rows, write its x
   // rows (scaled, with the norm term) to the lane's private LDS slot and its
@@ -236,6 +243,11 @@ dp_stream_kernel(const DpStreamArgs a) {
     pq = e.q;
     pccl = e.lc1;
     pyptr = (e.swap ? xdata : ydata) + e.coff;
+    if (SAVE) {
+      psv = reinterpret_cast<T*>(a.save) + a.save_off[e.q];
+      psr0 = r0;
+      psnr = e.lr1;
+    }
     const T* rb = (e.swap ? ydata : xdata) + e.roff;
 #pragma unroll
     for (int j = 0; j < R; ++j) {
@@ -256,12 +268,22 @@ dp_stream_kernel(const DpStreamArgs a) {
 
   T xr[R][DMAX];
   T xn[R];
+  // After its last band a lane keeps stepping (at most 62 steps, fewer than a
+  // band) on a stale copy of that band's metadata: it must not count as the
+  // end of a pair there.
+  int lband = -1;
   auto enter_band = [&]() {
+    ++lband;
     yptr = pyptr;
     ccl = pccl;
     q = pq;
     prim = pprim;
-    last = plast;
+    last = plast && lband < nb;
+    if (SAVE) {
+      sv = psv;
+      sr0 = psr0;
+      snr = psnr;
+    }
 #pragma unroll
     for (int j = 0; j < R; ++j) {
       const T* src = xst + (R * lane + j) * XS;
@@ -414,6 +436,12 @@ dp_stream_kernel(const DpStreamArgs a) {
     for (int k = 0; k < NF; ++k) fl[k] = cur[k];
 #pragma unroll
     for (int m = 0; m < NLEV; ++m) accf[m] += cur[1 + m];
+    if (SAVE && NC > 0 && last && cp >= 1 && cp <= ccl) {
+      // final column sums C_a(Nr, j), a = 1..NLEV-1, j = cp - 1
+      T* cs = sv + NC * snr;
+#pragma unroll
+      for (int m = 0; m < NC; ++m) cs[m * ccl + cp - 1] = cur[1 + m];
+    }
     if (lane == kWave - 1 && T_ >= kWave - 1) {
       if (GF) {
         // read back by lane 0 at least Pc - 65 >= 4 steps later: by then the
@@ -441,6 +469,17 @@ dp_stream_kernel(const DpStreamArgs a) {
     }
     ++cp;
     if (cp == Pc) {  // this lane's band pass ends
+      if (SAVE && NC > 0) {
+        // row-end prefixes P_a(i, Nc) of the lane's rows (ΔΔ row i = local row - 1)
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+          const int r = sr0 + j;
+          if (r >= 1 && r <= snr) {
+#pragma unroll
+            for (int m = 0; m < NC; ++m) sv[m * snr + r - 1] = PS[j][m];
+          }
+        }
+      }
       if (last) {
         double* o = a.out + (int64_t)q * (NLEV + 1);
         o[0] = 1.0;
@@ -477,10 +516,10 @@ dp_stream_kernel(const DpStreamArgs a) {
   if (T_ < total) step(T_, y0, f0);
 }
 
-template <int NLEV, int ORDER, int DMAX, int KIND, bool VEC, bool GF>
+template <int NLEV, int ORDER, int DMAX, int KIND, bool VEC, bool GF, bool SAVE = false>
 StreamEntry stream_entry() {
   constexpr int R = stream_rows(NLEV, ORDER, DMAX);
-  return StreamEntry{(const void*)dp_stream_kernel<NLEV, ORDER, DMAX, R, KIND, VEC, GF>, R};
+  return StreamEntry{(const void*)dp_stream_kernel<NLEV, ORDER, DMAX, R, KIND, VEC, GF, SAVE>, R};
 }
 
 }  // namespace ksig

@@ -22,6 +22,26 @@ StreamEntry dp_stream_lin_d32v(int n_levels, int order);
 StreamEntry dp_stream_lin_d32g(int n_levels, int order);
 StreamEntry dp_stream_lin_d32w(int n_levels, int order);
 
+StreamEntry dp_stream_rbf_d8t(int n_levels);
+StreamEntry dp_stream_rbf_d16t(int n_levels);
+StreamEntry dp_stream_rbf_d32t(int n_levels);
+StreamEntry dp_stream_lin_d8t(int n_levels);
+StreamEntry dp_stream_lin_d16t(int n_levels);
+StreamEntry dp_stream_lin_d32t(int n_levels);
+
+StreamEntry dp_stream_save_entry(int kind, int dmax, int n_levels) {
+  if (kind == kRBF) {
+    if (dmax == 8) return dp_stream_rbf_d8t(n_levels);
+    if (dmax == 16) return dp_stream_rbf_d16t(n_levels);
+    if (dmax == 32) return dp_stream_rbf_d32t(n_levels);
+  } else if (kind == kLinear) {
+    if (dmax == 8) return dp_stream_lin_d8t(n_levels);
+    if (dmax == 16) return dp_stream_lin_d16t(n_levels);
+    if (dmax == 32) return dp_stream_lin_d32t(n_levels);
+  }
+  return StreamEntry{nullptr, 0};
+}
+
 StreamEntry dp_stream_entry(int kind, int dmax, bool vec, bool global_fifo, int n_levels,
                             int order) {
   if (global_fifo && !vec) return StreamEntry{nullptr, 0};

@@ -142,9 +142,12 @@ struct StreamPlan {
 // Kernel, LDS size and resident waves of the row stream for a problem with a
 // pair table of `cap` entries; false if no kernel or LDS layout fits.
 static bool stream_plan(int d, int kind, int n_levels, int order, int max_len_x,
-                        int max_len_y, bool vec, bool gf, int cap, StreamPlan* p) {
+                        int max_len_y, bool vec, bool gf, int cap, StreamPlan* p,
+                        bool save = false) {
   const int dmax = dmax_for(d);
-  p->e = ksig::dp_stream_entry(kind, dmax, vec, gf, n_levels, order);
+  if (save && (!vec || gf || order != 1)) return false;
+  p->e = save ? ksig::dp_stream_save_entry(kind, dmax, n_levels)
+              : ksig::dp_stream_entry(kind, dmax, vec, gf, n_levels, order);
   if (!p->e.fn) return false;
   const int nf = ksig::dp_flow_count(n_levels, order);
   int pc = max_len_x < max_len_y ? max_len_x : max_len_y;
@@ -168,6 +171,8 @@ static int stream_launch(const StreamPlan& pl, const ksig::DpFwdArgs& f, double 
                          const int32_t* sched_order, const int32_t* wave_beg, int n_waves,
                          int64_t per_wave, void* workspace, void* stream) {
   ksig::DpStreamArgs a{};
+  a.save = f.save;
+  a.save_off = f.save_off;
   a.gfifo = pl.gf ? workspace : nullptr;
   a.x = f.x;
   a.y = f.y;
@@ -202,12 +207,13 @@ static int stream_launch(const StreamPlan& pl, const ksig::DpFwdArgs& f, double 
 static int stream_levels(const ksig::DpFwdArgs& f, int max_len_x, int max_len_y,
                          double static_param, void* stream) {
   const bool vec = stream_vec(f.d, f.x.data, f.y.data);
+  const bool save = f.save != nullptr;
   const int64_t n = f.n_pairs;
   StreamPlan pl;
   int64_t per = 0;
   for (int cap = 4; cap <= ksig::kStreamMaxPairs; cap += 4) {
     if (!stream_plan(f.d, f.kind, f.n_levels, f.order, max_len_x, max_len_y, vec, false, cap,
-                     &pl))
+                     &pl, save))
       return -1;
     per = (n + pl.slots - 1) / pl.slots;
     if (per <= cap) break;
@@ -394,6 +400,10 @@ int ksig_signature_levels_fwd_save(int dtype, const void* x, const int64_t* x_of
   a.out = out;
   a.save = n_levels > 1 ? save : nullptr;
   a.save_off = save_off;
+  if (dtype == KSIG_F32 && a.save != nullptr) {
+    rc = stream_levels(a, max_len_x, max_len_y, static_param, stream);
+    if (rc != -1) return rc;  // -1: no row-stream training kernel fits, use the per-pair one
+  }
   const int cols = max_len_x < max_len_y ? max_len_x : max_len_y;
   return dp_levels_common(dtype, a, cols < 1 ? 1 : cols, dmax_for(d), stream);
 }

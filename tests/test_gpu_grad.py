@@ -13,7 +13,7 @@ import ksig_amd as ksig
 from ksig_amd import _lib, _ops, autograd
 from oracle import sigkern as O
 from tests import torch_reference as TR
-from tests.gpu_util import dev
+from tests.gpu_util import dev, to_dev
 
 pytestmark = pytest.mark.gpu
 
@@ -132,3 +132,49 @@ def test_mmd_gradients_float32_vs_float64():
   assert abs(grads[torch.float64][0] - float(ref)) < 1e-10
   assert _rel(grads[torch.float64][1], scale.grad) < 1e-8
   assert _rel(grads[torch.float32][1], scale.grad) < 2e-4
+
+
+@pytest.mark.parametrize("d", [8, 16, 32])
+def test_training_forward_boundary_state_layout(d):
+  """The float32 training forward (row stream for d in {8, 16, 32}) writes the
+  same adjoint boundary state -- row-end prefixes then final column sums, in
+  the rows-are-the-longer-sequence orientation -- as the float64 per-pair
+  kernel, for uneven lengths (both orientations, several bands per pair)."""
+  import ctypes
+  from ksig_amd import _ops
+  rng = np.random.default_rng(70 + d)
+  lens_x = [3, 70, 140, 33, 300]
+  lens_y = [90, 5, 140, 261]
+  xs = [O.brownian(rng, 1, L, d)[0] for L in lens_x]
+  ys = [O.brownian(rng, 1, L, d)[0] for L in lens_y]
+  n_levels = 5
+  pr = np.array([[a, b] for a in range(len(lens_x)) for b in range(len(lens_y))], np.int32)
+  la = np.array([lens_x[a] for a, _ in pr])
+  lb = np.array([lens_y[b] for _, b in pr])
+  per = (n_levels - 1) * np.maximum(la + lb - 2, 0)
+  off = np.concatenate([[0], np.cumsum(per)[:-1]]).astype(np.int64)
+  lib = _lib.load()
+  saves, outs = {}, {}
+  for dt in (torch.float64, torch.float32):
+    X = _ops.SeqSet.ragged([torch.tensor(s, device=dev(), dtype=dt) for s in xs])
+    Y = _ops.SeqSet.ragged([torch.tensor(s, device=dev(), dtype=dt) for s in ys])
+    pairs = to_dev(pr)
+    save = torch.full((int(per.sum()),), float("nan"), device=dev(), dtype=dt)
+    so = to_dev(off)
+    out = torch.empty((len(pr), n_levels + 1), device=dev(), dtype=torch.float64)
+    _lib.check(lib.ksig_signature_levels_fwd_save(
+        _lib.dtype_code(dt), X.data.data_ptr(), X.off.data_ptr(), X.len.data_ptr(), X.max_len,
+        Y.data.data_ptr(), Y.off.data_ptr(), Y.len.data_ptr(), Y.max_len, d,
+        pairs.data_ptr(), len(pr), _lib.STATIC_RBF, math.sqrt(d), n_levels, out.data_ptr(),
+        save.data_ptr(), so.data_ptr(), _lib.stream_handle(dev())))
+    saves[dt] = save.double().cpu().numpy()
+    outs[dt] = out.cpu().numpy()
+  assert not np.isnan(saves[torch.float32]).any()
+  for p in range(len(pr)):
+    a, b = off[p], off[p] + per[p]
+    ref = saves[torch.float64][a:b]
+    got = saves[torch.float32][a:b]
+    if b > a:
+      assert np.max(np.abs(got - ref)) <= 2e-5 * max(np.max(np.abs(ref)), 1e-30), p
+  assert np.max(np.abs(outs[torch.float32] - outs[torch.float64])) <= 1e-5 * np.max(
+      np.abs(outs[torch.float64]))
