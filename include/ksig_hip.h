@@ -215,6 +215,39 @@ int ksig_signature_pde_bwd(int dtype, const int32_t* x_len, int max_len_x,
                            void* stream);
 
 /*
+ * Training forward of the signature-PDE kernel: ksig_signature_pde, which also
+ * writes, per pair at element offset e_off[p] (dtype, kernel orientation:
+ * rows = the longer sequence), U of the last refined row (Ncr values) then U
+ * of the last refined column (Nrr values), Nrr = (len_rows - difference) << dyadic_order,
+ * Ncr likewise.  Same reference interface as ksig_signature_pde; the edges
+ * have no reference counterpart (the reference has no gradients).
+ */
+int ksig_signature_pde_fwd_edges(int dtype, const void* x, const int64_t* x_off,
+                                 const int32_t* x_len, int max_len_x, const void* y,
+                                 const int64_t* y_off, const int32_t* y_len,
+                                 int max_len_y, int d, const int32_t* pairs,
+                                 int64_t n_pairs, int static_kind, double static_param,
+                                 int dyadic_order, int difference, double* out,
+                                 void* edges, const int64_t* e_off, void* stream);
+
+/*
+ * Adjoint of ksig_signature_pde_fwd_edges (no reference counterpart: the
+ * scheme of ksig/algorithms.py:380-385 is forward-only).  Given grad_out[p] =
+ * dL/dK, writes dL/dM for every original ΔΔ cell, [Nr][Nc] in kernel
+ * orientation at element offset m_off[p] of grad_m (dtype).  The solution U is
+ * rebuilt in the reverse sweep from the edges; the static kernel is
+ * recomputed.
+ */
+int ksig_signature_pde_bwd_edges(int dtype, const void* x, const int64_t* x_off,
+                                 const int32_t* x_len, int max_len_x, const void* y,
+                                 const int64_t* y_off, const int32_t* y_len,
+                                 int max_len_y, int d, const int32_t* pairs,
+                                 int64_t n_pairs, int static_kind, double static_param,
+                                 int dyadic_order, int difference, const double* grad_out,
+                                 const void* edges, const int64_t* e_off, void* grad_m,
+                                 const int64_t* m_off, void* stream);
+
+/*
  * Signature-PDE kernel on caller-supplied matrices M [n_mats, lx, ly].
  * Replaces signature_kern_pde(M, difference) of ksig/algorithms.py:402-446.
  */

@@ -229,21 +229,46 @@ _DUMP_BYTES = 1 << 30  # U dump per chunk of pairs
 
 
 class _SigPDE(torch.autograd.Function):
+  """The forward also leaves U of the last refined row and column of every
+  pair (O(Lx + Ly) values); the adjoint kernel rebuilds the rest of U from
+  them in its reverse sweep and writes dL/dM per original cell, which the
+  ΔΔ-transpose and static-kernel chain rule below turn into input gradients."""
 
   @staticmethod
   def forward(ctx, xdata, ydata, param, xs_meta, ys_meta, pairs, kind, dyadic, difference):
+    lib = _lib.load()
     Xs = _ops.SeqSet(xdata.detach().contiguous(), *xs_meta)
     Ys = _ops.SeqSet(ydata.detach().contiguous(), *ys_meta)
     pval = float(param.detach())
-    out = _ops.signature_pde(Xs, Ys, pairs, difference, kind, pval, dyadic)
-    ctx.save_for_backward(xdata, ydata, pairs)
+    n_pairs = pairs.shape[0]
+    dev = xdata.device
+    out = torch.empty((n_pairs,), device=dev, dtype=torch.float64)
+    sh = 1 if difference else 0
+    la = Xs.len.to(torch.int64)[pairs[:, 0].long()]
+    lb = Ys.len.to(torch.int64)[pairs[:, 1].long()]
+    nrr = torch.clamp(torch.maximum(la, lb) - sh, min=0) << dyadic
+    ncr = torch.clamp(torch.minimum(la, lb) - sh, min=0) << dyadic
+    esz = nrr + ncr
+    e_off = torch.zeros(n_pairs, device=dev, dtype=torch.int64)
+    if n_pairs > 1:
+      e_off[1:] = torch.cumsum(esz, 0)[:-1]
+    total = int(esz.sum().item()) if n_pairs else 0
+    edges = torch.empty(max(total, 1), device=dev, dtype=Xs.data.dtype)
+    if n_pairs:
+      _lib.check(lib.ksig_signature_pde_fwd_edges(
+          _lib.dtype_code(Xs.data.dtype), Xs.data.data_ptr(), Xs.off.data_ptr(),
+          Xs.len.data_ptr(), Xs.max_len, Ys.data.data_ptr(), Ys.off.data_ptr(),
+          Ys.len.data_ptr(), Ys.max_len, Xs.d, pairs.data_ptr(), n_pairs, kind, pval,
+          int(dyadic), int(difference), out.data_ptr(), edges.data_ptr(), e_off.data_ptr(),
+          _lib.stream_handle(dev)))
+    ctx.save_for_backward(xdata, ydata, pairs, edges, e_off, la, lb)
     ctx.meta = (xs_meta, ys_meta, kind, pval, dyadic, difference)
     return out
 
   @staticmethod
   def backward(ctx, gout):
     lib = _lib.load()
-    xdata, ydata, pairs = ctx.saved_tensors
+    xdata, ydata, pairs, edges, e_off, la, lb = ctx.saved_tensors
     xs_meta, ys_meta, kind, pval, dyadic, difference = ctx.meta
     Xs = _ops.SeqSet(xdata.detach().contiguous(), *xs_meta)
     Ys = _ops.SeqSet(ydata.detach().contiguous(), *ys_meta)
@@ -255,15 +280,12 @@ class _SigPDE(torch.autograd.Function):
     gparam = torch.zeros((), device=dev, dtype=torch.float64)
     if n_pairs == 0:
       return gx, gy, gparam, None, None, None, None, None, None
-    la = Xs.len.to(torch.int64)[pairs[:, 0].long()]
-    lb = Ys.len.to(torch.int64)[pairs[:, 1].long()]
     sh = 1 if difference else 0
     nr = torch.clamp(torch.maximum(la, lb) - sh, min=0)   # kernel orientation
     nc = torch.clamp(torch.minimum(la, lb) - sh, min=0)
-    usz = (nr * nc) << (2 * dyadic)
     msz = nr * nc
-    budget = max(_DUMP_BYTES // Xs.data.element_size(), int(usz.max().item()))
-    chunk_id = ((torch.cumsum(usz, 0) - usz) // budget).cpu()
+    budget = max(_DUMP_BYTES // Xs.data.element_size(), int(msz.max().item()))
+    chunk_id = ((torch.cumsum(msz, 0) - msz) // budget).cpu()
     change = torch.nonzero(chunk_id[1:] != chunk_id[:-1]).flatten() + 1
     bounds = [0] + change.tolist() + [n_pairs]
     s_rbf = 1.0 / max(2.0 * pval * pval, 1e-12)
@@ -271,26 +293,16 @@ class _SigPDE(torch.autograd.Function):
     for c0, c1 in zip(bounds[:-1], bounds[1:]):
       pc = pairs[c0:c1].contiguous()
       P = c1 - c0
-      u_off = torch.zeros(P, device=dev, dtype=torch.int64)
       m_off = torch.zeros(P, device=dev, dtype=torch.int64)
       if P > 1:
-        u_off[1:] = torch.cumsum(usz[c0:c1], 0)[:-1]
         m_off[1:] = torch.cumsum(msz[c0:c1], 0)[:-1]
-      u_all = torch.empty(max(int(usz[c0:c1].sum()), 1), device=dev, dtype=dt)
-      m_all = torch.empty(max(int(msz[c0:c1].sum()), 1), device=dev, dtype=dt)
-      gm = torch.zeros_like(m_all)
-      tmp = torch.empty(P, device=dev, dtype=torch.float64)
-      _lib.check(lib.ksig_signature_pde_fwd_dump(
+      gm = torch.zeros(max(int(msz[c0:c1].sum()), 1), device=dev, dtype=dt)
+      _lib.check(lib.ksig_signature_pde_bwd_edges(
           _lib.dtype_code(dt), Xs.data.data_ptr(), Xs.off.data_ptr(), Xs.len.data_ptr(),
           Xs.max_len, Ys.data.data_ptr(), Ys.off.data_ptr(), Ys.len.data_ptr(), Ys.max_len,
-          Xs.d, pc.data_ptr(), P, kind, pval, dyadic, int(difference), tmp.data_ptr(),
-          u_all.data_ptr(), u_off.data_ptr(), m_all.data_ptr(), m_off.data_ptr(), stream))
-      _lib.check(lib.ksig_signature_pde_bwd(
-          _lib.dtype_code(dt), Xs.len.data_ptr(), Xs.max_len, Ys.len.data_ptr(), Ys.max_len,
-          pc.data_ptr(), P, dyadic, int(difference), gout[c0:c1].data_ptr(),
-          u_all.data_ptr(), u_off.data_ptr(), m_all.data_ptr(), m_off.data_ptr(),
-          gm.data_ptr(), stream))
-      del u_all
+          Xs.d, pc.data_ptr(), P, kind, pval, int(dyadic), int(difference),
+          gout[c0:c1].data_ptr(), edges.data_ptr(), e_off[c0:c1].data_ptr(), gm.data_ptr(),
+          m_off.data_ptr(), stream))
       _pde_chain(gm, m_off, pc, Xs, Ys, la[c0:c1], lb[c0:c1], kind, pval, s_rbf,
                  difference, gx, gy, gparam)
     return gx, gy, gparam, None, None, None, None, None, None

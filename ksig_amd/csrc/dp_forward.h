@@ -104,6 +104,26 @@ struct PdeFwdArgs {
   const int64_t* u_off;
   void* dump_m;
   const int64_t* m_off;
+  // Optional edges for the rebuilding adjoint (kernel orientation): U of the
+  // last refined row [Ncr] then of the last refined column [Nrr] at e_off[p].
+  void* edges;
+  const int64_t* e_off;
+};
+
+// Adjoint of the PDE sweep that rebuilds U from the saved edges.
+struct PdeBwdEdgeArgs {
+  SeqSet x, y;
+  int d;
+  const int32_t* pairs;
+  int64_t n_pairs;
+  int kind;
+  double xscale, yscale;
+  int dyadic, difference;
+  const double* gout;    // [n_pairs] dL/dK
+  const void* edges;     // as PdeFwdArgs::edges
+  const int64_t* e_off;
+  void* gm;              // [Nr][Nc] at m_off[p]: dL/dM of every original cell
+  const int64_t* m_off;
 };
 
 struct PdeBwdArgs {
@@ -143,5 +163,7 @@ hipError_t dp_stream_launch(const StreamEntry& e, const DpStreamArgs& a, size_t 
 int dp_stream_occupancy(const StreamEntry& e, size_t lds);
 hipError_t pde_forward_launch(const PdeFwdArgs& a, int dtype, size_t lds, hipStream_t s);
 hipError_t pde_backward_launch(const PdeBwdArgs& a, int dtype, size_t lds, hipStream_t s);
+size_t pde_bwd_edges_lds(int dtype, int d, int ly, int pc);
+hipError_t pde_bwd_edges_launch(const PdeBwdEdgeArgs& a, int dtype, size_t lds, hipStream_t s);
 
 }  // namespace ksig

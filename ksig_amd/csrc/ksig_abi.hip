@@ -594,6 +594,83 @@ int ksig_signature_pde_bwd(int dtype, const int32_t* x_len, int max_len_x,
   return 0;
 }
 
+int ksig_signature_pde_fwd_edges(int dtype, const void* x, const int64_t* x_off,
+                                 const int32_t* x_len, int max_len_x, const void* y,
+                                 const int64_t* y_off, const int32_t* y_len,
+                                 int max_len_y, int d, const int32_t* pairs,
+                                 int64_t n_pairs, int static_kind, double static_param,
+                                 int dyadic_order, int difference, double* out,
+                                 void* edges, const int64_t* e_off, void* stream) {
+  int rc = check_common(dtype, x, y, d, pairs, n_pairs, out);
+  if (rc) return rc;
+  ksig::PdeFwdArgs a{};
+  rc = static_scales(static_kind, static_param, &a.xscale, &a.yscale);
+  if (rc) return rc;
+  if (n_pairs == 0) return 0;
+  if (!edges || !e_off) return fail(KSIG_EINVAL, "null edge buffer");
+  if (static_kind != ksig::kRBF && static_kind != ksig::kLinear)
+    return fail(KSIG_EINVAL, "unknown static kernel kind %d", static_kind);
+  a.x = {x, x_off, x_len};
+  a.y = {y, y_off, y_len};
+  a.d = d;
+  a.pairs = pairs;
+  a.n_pairs = n_pairs;
+  a.kind = static_kind;
+  a.dyadic = dyadic_order;
+  a.difference = difference ? 1 : 0;
+  a.out = out;
+  a.edges = edges;
+  a.e_off = e_off;
+  const int cols = max_len_x < max_len_y ? max_len_x : max_len_y;
+  return pde_common(dtype, a, cols < 1 ? 1 : cols, dmax_for(d), stream);
+}
+
+int ksig_signature_pde_bwd_edges(int dtype, const void* x, const int64_t* x_off,
+                                 const int32_t* x_len, int max_len_x, const void* y,
+                                 const int64_t* y_off, const int32_t* y_len,
+                                 int max_len_y, int d, const int32_t* pairs,
+                                 int64_t n_pairs, int static_kind, double static_param,
+                                 int dyadic_order, int difference, const double* grad_out,
+                                 const void* edges, const int64_t* e_off, void* grad_m,
+                                 const int64_t* m_off, void* stream) {
+  int rc = check_common(dtype, x, y, d, pairs, n_pairs, (double*)grad_m);
+  if (rc) return rc;
+  ksig::PdeBwdEdgeArgs a{};
+  rc = static_scales(static_kind, static_param, &a.xscale, &a.yscale);
+  if (rc) return rc;
+  if (dyadic_order < 0 || dyadic_order > 6)
+    return fail(KSIG_EINVAL, "dyadic_order must be in [0, 6]");
+  if (static_kind != ksig::kRBF && static_kind != ksig::kLinear)
+    return fail(KSIG_EINVAL, "unknown static kernel kind %d", static_kind);
+  if (n_pairs == 0) return 0;
+  if (!grad_out || !edges || !e_off || !grad_m || !m_off)
+    return fail(KSIG_EINVAL, "null device pointer");
+  a.x = {x, x_off, x_len};
+  a.y = {y, y_off, y_len};
+  a.d = d;
+  a.pairs = pairs;
+  a.n_pairs = n_pairs;
+  a.kind = static_kind;
+  a.dyadic = dyadic_order;
+  a.difference = difference ? 1 : 0;
+  a.gout = grad_out;
+  a.edges = edges;
+  a.e_off = e_off;
+  a.gm = grad_m;
+  a.m_off = m_off;
+  const int ly = max_len_x < max_len_y ? max_len_x : max_len_y;  // the shorter side
+  int nc = difference ? ly - 1 : ly;
+  if (nc < 1) nc = 1;
+  long pc = (long)nc << dyadic_order;
+  if (pc < ksig::kWave) pc = ksig::kWave;
+  const size_t lds = ksig::pde_bwd_edges_lds(dtype, d, ly < 1 ? 1 : ly, (int)pc);
+  if (lds > max_lds_bytes())
+    return fail(KSIG_ELDS, "adjoint LDS state of %zu bytes exceeds the device limit", lds);
+  hipError_t e = ksig::pde_bwd_edges_launch(a, dtype, lds, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "pde_bwd_edges launch");
+  return 0;
+}
+
 int ksig_signature_pde_gram(int dtype, const void* M, int64_t n_mats, int lx,
                             int ly, int dyadic_order, int difference,
                             double* out, void* stream) {

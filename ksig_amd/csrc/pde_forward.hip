@@ -154,6 +154,7 @@ pde_forward_kernel(const PdeFwdArgs a) {
 
   T* dump_u = a.dump_u ? reinterpret_cast<T*>(a.dump_u) + a.u_off[pr] : nullptr;
   T* dump_m = a.dump_m ? reinterpret_cast<T*>(a.dump_m) + a.m_off[pr] : nullptr;
+  T* edges = a.edges ? reinterpret_cast<T*>(a.edges) + a.e_off[pr] : nullptr;
 
   T fl[NF] = {T(0), T(0), T(1), T(0)};
   T gt_old = T(0), gb_old = T(0), gt = T(0), gb = T(0);
@@ -204,6 +205,10 @@ pde_forward_kernel(const PdeFwdArgs a) {
     if (cell && rr == Nrr - 1 && cc == Ncr - 1) a.out[pr] = (double)u;
     if (dump_u != nullptr && cell) dump_u[(int64_t)rr * Ncr + cc] = u;
     if (dump_m != nullptr && cell && leader && newcol) dump_m[(int64_t)orow * Nc + ocol] = M;
+    if (edges != nullptr && cell) {  // last refined row, then last refined column
+      if (rr == Nrr - 1) edges[cc] = u;
+      if (cc == Ncr - 1) edges[Ncr + rr] = u;
+    }
     uleft = u;
     udiag = in_u;
 
