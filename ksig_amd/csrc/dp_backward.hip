@@ -34,12 +34,52 @@
 
 namespace ksig {
 
+// LDS of the adjoint: lane-private staging of the next band's x row and row-end
+// prefixes, the y norms, the FIFO.  The y rows themselves stream from L2.
 template <typename T, int NLEV, int DMAX>
+struct BwdLayout {
+  static constexpr int NL1 = NLEV > 1 ? NLEV - 1 : 1;
+  static constexpr int NF = 3 + 2 * (NLEV - 1);
+  static constexpr int XS = DMAX + 1 + NL1;  // x row, norm, row-end prefixes
+};
+
+// One sequence point into registers: 16-byte vector loads when VEC (d == DMAX,
+// aligned data), else one load per slot with the address clamped into the row
+// and the slots past d zeroed.  Both forms are straight-line code, which keeps
+// the compiler's wait counts exact so the two-step prefetch stays in flight.
+template <typename T, int DMAX, bool VEC>
+__device__ __forceinline__ void bwd_load_row(T* v, const T* p, int d) {
+  if (VEC) {
+    if constexpr (sizeof(T) == 4) {
+#pragma unroll
+      for (int f = 0; f < DMAX; f += 4) {
+        const float4 t = *reinterpret_cast<const float4*>(p + f);
+        v[f] = t.x; v[f + 1] = t.y; v[f + 2] = t.z; v[f + 3] = t.w;
+      }
+    } else {
+#pragma unroll
+      for (int f = 0; f < DMAX; f += 2) {
+        const double2 t = *reinterpret_cast<const double2*>(p + f);
+        v[f] = t.x; v[f + 1] = t.y;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int f = 0; f < DMAX; ++f) {
+      const T t = p[f < d ? f : d - 1];
+      v[f] = f < d ? t : T(0);
+    }
+  }
+}
+
+template <typename T, int NLEV, int DMAX, bool VEC>
 __global__ void __launch_bounds__(kWave)
 dp_backward_kernel(const DpBwdArgs a) {
-  constexpr int NL1 = NLEV > 1 ? NLEV - 1 : 1;
+  using L = BwdLayout<T, NLEV, DMAX>;
+  constexpr int NL1 = L::NL1;
   constexpr int NC = NLEV - 1;                  // prefix levels 1..NLEV-1
-  constexpr int NF = 3 + 2 * NC;                // [G, D', gM, C_1.., Dq_1..]
+  constexpr int NF = L::NF;                     // [G, D', gM, C_1.., Dq_1..]
+  constexpr int XS = L::XS;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* smem = reinterpret_cast<T*>(smem_raw);
 
@@ -73,62 +113,52 @@ dp_backward_kernel(const DpBwdArgs a) {
   const T* colsum = rowend + NC * Nr;                                       // [NC][Nc]
   T* omega = reinterpret_cast<T*>(a.omega) + a.omega_off[pr];
 
+  // Scale factors are folded into the x side: staged x rows carry
+  // xscale * yscale (so y rows are used raw), x norms xscale^2 |x|^2 and the y
+  // norms yscale^2 |y|^2.
   const T xscale = T(a.xscale), yscale = T(a.yscale);
-  // LDS: x rows [2][64][DMAX], x norms [2][64], row-end prefixes [2][64][NL1],
-  // y rows [Ly][DMAX], y norms [Ly], FIFO [Pc][NF].
-  T* xl = smem;
-  T* xnl = xl + 2 * kWave * DMAX;
-  T* rel = xnl + 2 * kWave;
-  T* yl = rel + 2 * kWave * NL1;
-  T* ynl = yl + Ly * DMAX;
-  T* fifo = ynl + Ly;
+  const T xyscale = xscale * yscale, ys2 = yscale * yscale;
+  T* xst = smem;                      // [64][XS], lane-private slots
+  T* ynl = xst + kWave * XS;          // [Ly] scaled y norms
+  T* fifo = ynl + Ly;                 // [Pc][NF]
 
-  // Stage flipped row fr (M row i = Nr-1-fr, whose top edge G row is i) into buffer b.
-  auto stage = [&](int b, int fr) {
+  // Stage flipped row fr (M row i = Nr-1-fr, whose top edge G row is i) into
+  // this lane's slot.
+  auto stage = [&](int fr) {
     const int i = Nr - 1 - fr;
     const bool ok = i >= 0 && fr < nrows;
+    T* dst = xst + lane * XS;
     T n2 = T(0);
 #pragma unroll
     for (int k = 0; k < DMAX; ++k) {
-      const T v = (ok && k < d) ? xs[(int64_t)i * d + k] * xscale : T(0);
-      xl[(b * kWave + lane) * DMAX + k] = v;
+      const T v = (ok && k < d) ? xs[(int64_t)i * d + k] : T(0);
+      dst[k] = v * xyscale;
       n2 += v * v;
     }
-    xnl[b * kWave + lane] = n2;
+    dst[DMAX] = n2 * xscale * xscale;
 #pragma unroll
-    for (int m = 0; m < NC; ++m)
-      rel[(b * kWave + lane) * NL1 + m] = ok ? rowend[m * Nr + i] : T(0);
+    for (int m = 0; m < NC; ++m) dst[DMAX + 1 + m] = ok ? rowend[m * Nr + i] : T(0);
   };
+  T xr[DMAX], xn, rel[NL1];
+  auto take = [&]() {
+    const T* src = xst + lane * XS;
+#pragma unroll
+    for (int k = 0; k < DMAX; ++k) xr[k] = src[k];
+    xn = src[DMAX];
+#pragma unroll
+    for (int m = 0; m < NL1; ++m) rel[m] = src[DMAX + 1 + m];
+  };
+  // raw y row yc into registers
+  auto yrow = [&](T* dst, int yc) { bwd_load_row<T, DMAX, VEC>(dst, ys + (int64_t)yc * d, d); };
 
   for (int j = lane; j < Ly; j += kWave) {
+    T yv[DMAX];
+    yrow(yv, j);
     T n2 = T(0);
 #pragma unroll
-    for (int k = 0; k < DMAX; ++k) {
-      const T v = k < d ? ys[(int64_t)j * d + k] * yscale : T(0);
-      yl[j * DMAX + k] = v;
-      n2 += v * v;
-    }
-    ynl[j] = n2;
+    for (int k = 0; k < DMAX; ++k) n2 += yv[k] * yv[k];
+    ynl[j] = n2 * ys2;
   }
-  stage(0, lane);
-  stage(1, kWave + lane);
-  __syncthreads();
-
-  // G value and its "distance" companion (D' for RBF, G for linear) for x row
-  // held in buffer b and y row yc.
-  auto gpoint = [&](int b, int yc, T& dv) -> T {
-    const T* xr = xl + (b * kWave + lane) * DMAX;
-    const T* yr = yl + yc * DMAX;
-    T dot = T(0);
-#pragma unroll
-    for (int k = 0; k < DMAX; ++k) dot += xr[k] * yr[k];
-    if (kind == kRBF) {
-      dv = xnl[b * kWave + lane] + ynl[yc] - T(2) * dot;
-      return fast_exp2(-dv);
-    }
-    dv = dot;
-    return dot;
-  };
 
   // FIFO: the row below the grid (G row Nr = Lx-1) with the forward's final
   // column sums; slot s covers column j = Nc - s.
@@ -137,19 +167,26 @@ dp_backward_kernel(const DpBwdArgs a) {
     T xbn = T(0);
 #pragma unroll
     for (int k = 0; k < DMAX; ++k) {
-      xb[k] = k < d ? xs[(int64_t)Nr * d + k] * xscale : T(0);
-      xbn += xb[k] * xb[k];
+      const T v = k < d ? xs[(int64_t)Nr * d + k] : T(0);
+      xb[k] = v * xyscale;
+      xbn += v * v;
     }
+    xbn *= xscale * xscale;
     for (int s = lane; s < Pc; s += kWave) {
       const int j = Nc - s;
       T g = T(0), dv = T(0);
       if (j >= 0) {
-        const T* yr = yl + j * DMAX;
-        T dot = T(0);
+        T yv[DMAX];
+        yrow(yv, j);
+        T dot = T(0), yn = T(0);
 #pragma unroll
-        for (int k = 0; k < DMAX; ++k) dot += xb[k] * yr[k];
+        for (int k = 0; k < DMAX; ++k) {
+          dot += xb[k] * yv[k];
+          yn += yv[k] * yv[k];
+        }
+        yn *= ys2;
         if (kind == kRBF) {
-          dv = xbn + ynl[j] - T(2) * dot;
+          dv = xbn + yn - T(2) * dot;
           g = fast_exp2(-dv);
         } else {
           dv = g = dot;
@@ -165,7 +202,9 @@ dp_backward_kernel(const DpBwdArgs a) {
       }
     }
   }
+  stage(lane);
   __syncthreads();
+  take();
 
   T fl[NF];
 #pragma unroll
@@ -181,13 +220,30 @@ dp_backward_kernel(const DpBwdArgs a) {
   int band = 0;
   int fr = lane;  // flipped row
   const int total = nb * Pc + kWave - 1;
-  int next_stage = Pc + kWave;
-  int stage_band = 2;
-  for (int step = 0; step < total; ++step) {
-    if (step == next_stage) {
-      stage(stage_band & 1, stage_band * kWave + lane);
-      next_stage += Pc;
+  // band b+1 is staged at step b*Pc + 63: every lane has taken band b by then
+  int next_stage = kWave - 1;
+  int stage_band = 1;
+
+  // y rows prefetched two steps ahead (column of step t+2 of this lane)
+  int a_s = s;
+  auto ycol = [&](int st) {
+    int j = Nc - (st < 0 ? 0 : st);
+    return j < 0 ? 0 : (j > Ly - 1 ? Ly - 1 : j);
+  };
+  auto yload = [&](T* dst) {
+    yrow(dst, ycol(a_s));
+    ++a_s;
+    if (a_s == Pc) a_s = 0;
+  };
+  T y0[DMAX], y1[DMAX];
+  yload(y0);
+  yload(y1);
+
+  auto step_fn = [&](int step, T* yv) {
+    if (step == next_stage) {  // wave-uniform
+      if (stage_band < nb) stage(stage_band * kWave + lane);
       ++stage_band;
+      next_stage += Pc;
     }
     const bool active = s >= 0 && band < nb;
     T fin[NF];
@@ -203,10 +259,20 @@ dp_backward_kernel(const DpBwdArgs a) {
 
     const int j = Nc - s;
     const int i = Nr - 1 - fr;
-    const int b = band & 1;
     const int yc = j < 0 ? 0 : (j > Ly - 1 ? Ly - 1 : j);
-    T dv;
-    T g = gpoint(b, yc, dv);
+    T dv, g;
+    {
+      T dot = T(0);
+#pragma unroll
+      for (int k = 0; k < DMAX; ++k) dot += xr[k] * yv[k];
+      if (kind == kRBF) {
+        dv = xn + ynl[yc] - T(2) * dot;
+        g = fast_exp2(-dv);
+      } else {
+        dv = g = dot;
+      }
+    }
+    yload(yv);
     const bool gvalid = active && i >= 0 && j >= 0;
     g = gvalid ? g : T(0);
     dv = gvalid ? dv : T(0);
@@ -258,7 +324,7 @@ dp_backward_kernel(const DpBwdArgs a) {
     gmup_right = in[2];
 #pragma unroll
     for (int m = 0; m < NC; ++m) {
-      P[m] = prim ? rel[(b * kWave + lane) * NL1 + m] : Pn[m];
+      P[m] = prim ? rel[m] : Pn[m];
       S[m] = prim ? T(0) : S[m] + in[3 + NC + m];
     }
     T of[NF];
@@ -284,16 +350,24 @@ dp_backward_kernel(const DpBwdArgs a) {
       fr += kWave;
       gsig += (double)gsig_row;
       gsig_row = T(0);
+      take();
     }
+  };
+
+  int step = 0;
+  for (; step + 2 <= total; step += 2) {
+    step_fn(step, y0);
+    step_fn(step + 1, y1);
   }
+  if (step < total) step_fn(step, y0);
   gsig += (double)gsig_row;
   gsig = wave_sum(gsig);
   if (lane == 0) a.gparam[pr] = gsig;
 }
 
-template <typename T, int NLEV, int DMAX>
+template <typename T, int NLEV, int DMAX, bool VEC>
 static hipError_t launch_bwd(const DpBwdArgs& a, size_t lds, hipStream_t s) {
-  auto k = dp_backward_kernel<T, NLEV, DMAX>;
+  auto k = dp_backward_kernel<T, NLEV, DMAX, VEC>;
   if (lds > 64 * 1024) {
     hipError_t e = hipFuncSetAttribute((const void*)k,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -307,7 +381,11 @@ static hipError_t launch_bwd(const DpBwdArgs& a, size_t lds, hipStream_t s) {
 
 template <typename T, int DMAX>
 static hipError_t dispatch_bwd_l(const DpBwdArgs& a, size_t lds, hipStream_t s) {
-#define KSIG_CASE(L) if (a.n_levels == L) return launch_bwd<T, L, DMAX>(a, lds, s);
+  const bool vec = a.d == DMAX && ((uintptr_t)a.x.data & 15) == 0 &&
+                   ((uintptr_t)a.y.data & 15) == 0;
+#define KSIG_CASE(L)                                                              \
+  if (a.n_levels == L)                                                            \
+    return vec ? launch_bwd<T, L, DMAX, true>(a, lds, s) : launch_bwd<T, L, DMAX, false>(a, lds, s);
   KSIG_BWD_LEVELS(KSIG_CASE)
 #undef KSIG_CASE
   return hipErrorInvalidValue;
@@ -325,6 +403,15 @@ bool dp_backward_supported(int n_levels) {
   KSIG_BWD_LEVELS(KSIG_SUP)
 #undef KSIG_SUP
   return false;
+}
+
+size_t dp_backward_lds(int dtype, int d, int cols, int n_levels) {
+  const size_t esz = dtype == KSIG_F32 ? 4 : 8;
+  const int dmax = d <= 8 ? 8 : (d <= 16 ? 16 : 32);
+  const int nl1 = n_levels > 1 ? n_levels - 1 : 1;
+  const int nf = 3 + 2 * (n_levels - 1);
+  const int pc = cols + 1 > kWave ? cols + 1 : kWave;
+  return ((size_t)kWave * (dmax + 1 + nl1) + (size_t)cols + (size_t)pc * nf) * esz;
 }
 
 hipError_t dp_backward_launch(const DpBwdArgs& a, int dtype, size_t lds, hipStream_t s) {

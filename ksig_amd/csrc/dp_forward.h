@@ -146,6 +146,7 @@ size_t dp_sweep_lds(int dtype, int d, int cols, int n_levels, int order);
 hipError_t dp_sweep_launch(const DpFwdArgs& a, int dtype, size_t lds, hipStream_t s);
 bool dp_supported(int n_levels, int order);
 hipError_t dp_forward_launch(const DpFwdArgs& a, int dtype, size_t lds, hipStream_t s);
+size_t dp_backward_lds(int dtype, int d, int cols, int n_levels);
 hipError_t dp_backward_launch(const DpBwdArgs& a, int dtype, size_t lds, hipStream_t s);
 bool dp_backward_supported(int n_levels);
 bool dp_mfma_supported(int n_levels, int order);

@@ -443,13 +443,7 @@ int ksig_signature_levels_bwd(int dtype, const void* x, const int64_t* x_off,
   a.gparam = grad_param;
   int cols = max_len_x < max_len_y ? max_len_x : max_len_y;
   if (cols < 1) cols = 1;
-  const size_t esz = dtype == KSIG_F32 ? 4 : 8;
-  const int dmax = dmax_for(d);
-  const int nl1 = n_levels > 1 ? n_levels - 1 : 1;
-  const int nf = 3 + 2 * (n_levels - 1);
-  const int pc = cols + 1 > ksig::kWave ? cols + 1 : ksig::kWave;
-  const size_t lds = (2 * (size_t)ksig::kWave * (dmax + 1 + nl1) + (size_t)cols * (dmax + 1) +
-                      (size_t)pc * nf) * esz;
+  const size_t lds = ksig::dp_backward_lds(dtype, d, cols, n_levels);
   if (lds > max_lds_bytes())
     return fail(KSIG_ELDS, "adjoint LDS state of %zu bytes exceeds the device limit", lds);
   hipError_t e = ksig::dp_backward_launch(a, dtype, lds, (hipStream_t)stream);
