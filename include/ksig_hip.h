@@ -156,6 +156,28 @@ int ksig_signature_levels_bwd(int dtype, const void* x, const int64_t* x_off,
                               void* stream);
 
 /*
+ * Chain rule through the static kernel (no reference counterpart: the
+ * reference's static kernels, ksig/static/kernels.py:134-166 and 245-260, have
+ * no gradients).  For every pair p = (i, j) with omega_p = omega + omega_off[p]
+ * of shape [len_x[i], len_y[j]] (row-major), adds
+ *   grad_x[x_off[i] + a] += c_cross * sum_b omega_p[a, b] y_b
+ *                         + c_self * (sum_b omega_p[a, b]) x_a
+ *   grad_y[y_off[j] + b] += c_cross * sum_a omega_p[a, b] x_a
+ *                         + c_self * (sum_a omega_p[a, b]) y_b
+ * (d-vectors, dtype of x).  RBF: c_cross = -c_self = 1 / sigma^2,
+ * omega = dL/dG * G; linear: c_cross = scale, c_self = 0, omega = dL/dG.
+ * Either gradient may be NULL; both may alias (x and y the same set).
+ * Accumulates with device atomics: the result is order-dependent in its last
+ * bits.  1 <= d <= 32.
+ */
+int ksig_static_grad_contract(int dtype, const void* x, const int64_t* x_off,
+                              const int32_t* x_len, int max_len_x, const void* y,
+                              const int64_t* y_off, const int32_t* y_len, int max_len_y,
+                              int d, const int32_t* pairs, int64_t n_pairs,
+                              const void* omega, const int64_t* omega_off, double c_cross,
+                              double c_self, void* grad_x, void* grad_y, void* stream);
+
+/*
  * Truncated signature kernel on caller-supplied static-kernel matrices.
  *
  * Replaces signature_kern(M, n_levels, order, difference, return_levels=True)

@@ -6,8 +6,8 @@ first-order DP signature kernel (RBF or linear static kernel, difference=True)
 as a differentiable function of the sequences and of the bandwidth / scale.
 The forward is the fused HIP sweep (it also leaves the O((Lx+Ly)·n) boundary
 state per pair); the backward is the HIP adjoint sweep, which writes
-omega = dL/dG * dG/dz per static-kernel entry, followed by the contraction of
-omega with the inputs (batched GEMMs over each pair's L x d blocks):
+omega = dL/dG * dG/dz per static-kernel entry, followed by the HIP contraction
+of omega with the inputs (ksig_static_grad_contract, MFMA tiles per pair):
 
   RBF,    G = exp(-s |x-y|^2), s = 1/max(2 sigma^2, 1e-12):
     dL/dx_a = -2s (x_a sum_b omega_ab - sum_b omega_ab y_b)
@@ -136,14 +136,7 @@ class _SigLevels(torch.autograd.Function):
           omega.data_ptr(), w_off.data_ptr(), gp.data_ptr(), _lib.stream_handle(dev)))
       if ctx.needs_input_grad[2]:
         gparam_total = gparam_total + gp.sum()
-      if gx is None and gy is None:
-        continue
-      if uni is not None:
-        A, B = uni
-        _contract_block(omega[:(c1 - c0) * A * B].view(-1, A, B), pc[:, 0].long(),
-                        pc[:, 1].long(), A, B, Xs, Ys, kind, pval, s_rbf, gx, gy)
-      else:
-        _contract(omega, w_off, pc, Xs, Ys, la[c0:c1], lb[c0:c1], kind, pval, s_rbf, gx, gy)
+      _contract_hip(omega, w_off, pc, Xs, Ys, kind, pval, s_rbf, gx, gy)
     if kind == _lib.STATIC_RBF:
       den = 2.0 * pval * pval
       if den > 1e-12:
@@ -158,41 +151,22 @@ class _SigLevels(torch.autograd.Function):
     return gx, gy, gparam_total, None, None, None, None, None, None
 
 
-def _contract(omega, w_off, pairs, Xs, Ys, la, lb, kind, pval, s_rbf, gx, gy):
-  """dL/dx, dL/dy from omega (per pair [La, Lb] blocks)."""
-  ia = pairs[:, 0].long()
-  ib = pairs[:, 1].long()
-  if bool((la == la[0]).all()) and bool((lb == lb[0]).all()):
-    A, B = int(la[0]), int(lb[0])
-    _contract_block(omega.view(-1, A, B), ia, ib, A, B, Xs, Ys, kind, pval, s_rbf, gx, gy)
+def _contract_hip(omega, w_off, pairs, Xs, Ys, kind, pval, s_rbf, gx, gy):
+  """gx / gy += the chain rule of omega (per pair [len_x, len_y] at w_off)
+  through the static kernel, on the device (ksig_static_grad_contract)."""
+  if gx is None and gy is None:
     return
-  for p in range(pairs.shape[0]):  # ragged pairs: one block at a time
-    A, B = int(la[p]), int(lb[p])
-    o = int(w_off[p])
-    _contract_block(omega[o:o + A * B].view(1, A, B), ia[p:p + 1], ib[p:p + 1], A, B,
-                    Xs, Ys, kind, pval, s_rbf, gx, gy)
-
-
-def _contract_block(W, ia, ib, A, B, Xs, Ys, kind, pval, s_rbf, gx, gy):
-  d = Xs.d
-  dev = W.device
-  rows_a = Xs.off[ia][:, None] + torch.arange(A, device=dev)[None]
-  rows_b = Ys.off[ib][:, None] + torch.arange(B, device=dev)[None]
-  xa = Xs.data[rows_a]  # [P, A, d]
-  yb = Ys.data[rows_b]  # [P, B, d]
-  Wt = W.transpose(1, 2).contiguous()
   if kind == _lib.STATIC_RBF:
-    if gx is not None:
-      part = -2.0 * s_rbf * (xa * W.sum(-1, keepdim=True) - torch.bmm(W, yb))
-      gx.index_add_(0, rows_a.reshape(-1), part.reshape(-1, d))
-    if gy is not None:
-      part = 2.0 * s_rbf * (torch.bmm(Wt, xa) - yb * W.sum(-2)[..., None])
-      gy.index_add_(0, rows_b.reshape(-1), part.reshape(-1, d))
+    c_cross, c_self = 2.0 * s_rbf, -2.0 * s_rbf
   else:
-    if gx is not None:
-      gx.index_add_(0, rows_a.reshape(-1), (pval * torch.bmm(W, yb)).reshape(-1, d))
-    if gy is not None:
-      gy.index_add_(0, rows_b.reshape(-1), (pval * torch.bmm(Wt, xa)).reshape(-1, d))
+    c_cross, c_self = float(pval), 0.0
+  lib = _lib.load()
+  _lib.check(lib.ksig_static_grad_contract(
+      _lib.dtype_code(Xs.data.dtype), Xs.data.data_ptr(), Xs.off.data_ptr(),
+      Xs.len.data_ptr(), Xs.max_len, Ys.data.data_ptr(), Ys.off.data_ptr(), Ys.len.data_ptr(),
+      Ys.max_len, Xs.d, pairs.data_ptr(), pairs.shape[0], omega.data_ptr(), w_off.data_ptr(),
+      c_cross, c_self, None if gx is None else gx.data_ptr(),
+      None if gy is None else gy.data_ptr(), _lib.stream_handle(omega.device)))
 
 
 def signature_levels(X: _ops.SeqSet, Y: _ops.SeqSet, pairs: torch.Tensor, n_levels: int,
@@ -351,7 +325,9 @@ def _pde_chain(gm, m_off, pairs, Xs, Ys, la, lb, kind, pval, s_rbf, difference,
     else:
       W = gG
       gparam += (W.double() * dot.double()).sum()
-    _contract_block(W, ia[idx], ib[idx], A, B, Xs, Ys, kind, pval, s_rbf, gx, gy)
+    W = W.contiguous()
+    w_off = torch.arange(idx.numel(), device=gm.device, dtype=torch.int64) * (A * B)
+    _contract_hip(W, w_off, pairs[idx].contiguous(), Xs, Ys, kind, pval, s_rbf, gx, gy)
 
 
 def signature_pde(X: _ops.SeqSet, Y: _ops.SeqSet, pairs: torch.Tensor, kind: int, param,

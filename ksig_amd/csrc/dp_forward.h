@@ -110,6 +110,21 @@ struct PdeFwdArgs {
   const int64_t* e_off;
 };
 
+// Contraction of a per-pair Gram adjoint omega with the inputs (the chain
+// rule through the static kernel); gx / gy may be null.
+struct ContractArgs {
+  SeqSet x, y;
+  int d;
+  const int32_t* pairs;
+  int64_t n_pairs;
+  const void* omega;          // per pair [len_x, len_y] at omega_off[p]
+  const int64_t* omega_off;
+  double c_cross, c_self;
+  void* gx;
+  void* gy;
+  int group;                  // consecutive pairs per workgroup (set by the launcher)
+};
+
 // Adjoint of the PDE sweep that rebuilds U from the saved edges.
 struct PdeBwdEdgeArgs {
   SeqSet x, y;
@@ -166,5 +181,7 @@ hipError_t pde_forward_launch(const PdeFwdArgs& a, int dtype, size_t lds, hipStr
 hipError_t pde_backward_launch(const PdeBwdArgs& a, int dtype, size_t lds, hipStream_t s);
 size_t pde_bwd_edges_lds(int dtype, int d, int ly, int pc);
 hipError_t pde_bwd_edges_launch(const PdeBwdEdgeArgs& a, int dtype, size_t lds, hipStream_t s);
+hipError_t grad_contract_launch(const ContractArgs& a, bool f64, int max_len_x, int max_len_y,
+                                int n_cus, hipStream_t s);
 
 }  // namespace ksig

@@ -451,6 +451,36 @@ int ksig_signature_levels_bwd(int dtype, const void* x, const int64_t* x_off,
   return 0;
 }
 
+int ksig_static_grad_contract(int dtype, const void* x, const int64_t* x_off,
+                              const int32_t* x_len, int max_len_x, const void* y,
+                              const int64_t* y_off, const int32_t* y_len, int max_len_y,
+                              int d, const int32_t* pairs, int64_t n_pairs,
+                              const void* omega, const int64_t* omega_off, double c_cross,
+                              double c_self, void* grad_x, void* grad_y, void* stream) {
+  if (dtype != KSIG_F32 && dtype != KSIG_F64) return fail(KSIG_EINVAL, "bad dtype %d", dtype);
+  if (n_pairs < 0 || n_pairs > 0x7fffffffLL) return fail(KSIG_EINVAL, "bad n_pairs");
+  if (d < 1 || d > 32) return fail(KSIG_EUNSUPPORTED, "d = %d outside [1, 32]", d);
+  if (n_pairs == 0 || (!grad_x && !grad_y)) return 0;
+  if (!x || !x_off || !x_len || !y || !y_off || !y_len || !pairs || !omega || !omega_off)
+    return fail(KSIG_EINVAL, "null device pointer");
+  ksig::ContractArgs a{};
+  a.x = {x, x_off, x_len};
+  a.y = {y, y_off, y_len};
+  a.d = d;
+  a.pairs = pairs;
+  a.n_pairs = n_pairs;
+  a.omega = omega;
+  a.omega_off = omega_off;
+  a.c_cross = c_cross;
+  a.c_self = c_self;
+  a.gx = grad_x;
+  a.gy = grad_y;
+  hipError_t e = ksig::grad_contract_launch(a, dtype == KSIG_F64, max_len_x, max_len_y,
+                                            device_cus(), (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "grad_contract launch");
+  return 0;
+}
+
 int ksig_signature_levels_gram(int dtype, const void* M, int64_t n_mats, int lx,
                                int ly, int n_levels, int order, int difference,
                                double* out, void* stream) {
