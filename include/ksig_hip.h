@@ -206,35 +206,6 @@ int ksig_signature_pde(int dtype, const void* x, const int64_t* x_off,
                        int dyadic_order, int difference, double* out,
                        void* stream);
 
-/*
- * ksig_signature_pde that also dumps what its adjoint reads (no reference
- * counterpart).  In the kernel's orientation (rows = the longer sequence of
- * the pair, Nr/Nc = lengths - 1 with difference, else the lengths):
- *   dump_u[u_off[p] + r * (Nc << λ) + c] = U at refined cell (r, c),
- *   dump_m[m_off[p] + i * Nc + j]       = M(i, j) / 4^λ.
- */
-int ksig_signature_pde_fwd_dump(int dtype, const void* x, const int64_t* x_off,
-                                const int32_t* x_len, int max_len_x, const void* y,
-                                const int64_t* y_off, const int32_t* y_len,
-                                int max_len_y, int d, const int32_t* pairs,
-                                int64_t n_pairs, int static_kind, double static_param,
-                                int dyadic_order, int difference, double* out,
-                                void* dump_u, const int64_t* u_off, void* dump_m,
-                                const int64_t* m_off, void* stream);
-
-/*
- * Adjoint of the signature-PDE sweep (no reference counterpart:
- * ksig/algorithms.py:344-446 is forward-only).  Given grad_out[p] = dL/dK_p
- * and the dumps of ksig_signature_pde_fwd_dump for the same pairs, ADDS
- * dL/dM(i, j) of every (differenced) static-kernel cell into
- * grad_m[m_off[p] + i * Nc + j] (kernel orientation; zero it first).
- */
-int ksig_signature_pde_bwd(int dtype, const int32_t* x_len, int max_len_x,
-                           const int32_t* y_len, int max_len_y, const int32_t* pairs,
-                           int64_t n_pairs, int dyadic_order, int difference,
-                           const double* grad_out, const void* u_all, const int64_t* u_off,
-                           const void* m_all, const int64_t* m_off, void* grad_m,
-                           void* stream);
 
 /*
  * Training forward of the signature-PDE kernel: ksig_signature_pde, which also

@@ -51,12 +51,6 @@ SIGNATURES = {
     "ksig_signature_pde": (_c_int, [
         _c_int, _vp, _vp, _vp, _c_int, _vp, _vp, _vp, _c_int, _c_int, _vp,
         _c_i64, _c_int, _c_dbl, _c_int, _c_int, _vp, _vp]),
-    "ksig_signature_pde_fwd_dump": (_c_int, [
-        _c_int, _vp, _vp, _vp, _c_int, _vp, _vp, _vp, _c_int, _c_int, _vp,
-        _c_i64, _c_int, _c_dbl, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
-    "ksig_signature_pde_bwd": (_c_int, [
-        _c_int, _vp, _c_int, _vp, _c_int, _vp, _c_i64, _c_int, _c_int, _vp, _vp, _vp,
-        _vp, _vp, _vp, _vp]),
     "ksig_signature_pde_fwd_edges": (_c_int, [
         _c_int, _vp, _vp, _vp, _c_int, _vp, _vp, _vp, _c_int, _c_int, _vp,
         _c_i64, _c_int, _c_dbl, _c_int, _c_int, _vp, _vp, _vp, _vp]),

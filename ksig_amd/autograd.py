@@ -199,7 +199,7 @@ def mmd2_unbiased(Kxx: torch.Tensor, Kyy: torch.Tensor, Kxy: torch.Tensor) -> to
 # Signature-PDE kernel
 # -----------------------------------------------------------------------------
 
-_DUMP_BYTES = 1 << 30  # U dump per chunk of pairs
+_GM_BYTES = 1 << 30  # dL/dM scratch per chunk of pairs
 
 
 class _SigPDE(torch.autograd.Function):
@@ -258,7 +258,7 @@ class _SigPDE(torch.autograd.Function):
     nr = torch.clamp(torch.maximum(la, lb) - sh, min=0)   # kernel orientation
     nc = torch.clamp(torch.minimum(la, lb) - sh, min=0)
     msz = nr * nc
-    budget = max(_DUMP_BYTES // Xs.data.element_size(), int(msz.max().item()))
+    budget = max(_GM_BYTES // Xs.data.element_size(), int(msz.max().item()))
     chunk_id = ((torch.cumsum(msz, 0) - msz) // budget).cpu()
     change = torch.nonzero(chunk_id[1:] != chunk_id[:-1]).flatten() + 1
     bounds = [0] + change.tolist() + [n_pairs]

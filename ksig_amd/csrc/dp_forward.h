@@ -97,13 +97,6 @@ struct PdeFwdArgs {
   const void* gram;
   int gram_lx, gram_ly;
   double* out;           // [n_pairs]
-  // Optional dumps for the adjoint (kernel orientation, rows = longer side):
-  // U of every refined cell, [Nr << λ][Nc << λ] at u_off[p], and the refined
-  // cell value M / 4^λ of every original cell, [Nr][Nc] at m_off[p].
-  void* dump_u;
-  const int64_t* u_off;
-  void* dump_m;
-  const int64_t* m_off;
   // Optional edges for the rebuilding adjoint (kernel orientation): U of the
   // last refined row [Ncr] then of the last refined column [Nrr] at e_off[p].
   void* edges;
@@ -141,20 +134,6 @@ struct PdeBwdEdgeArgs {
   const int64_t* m_off;
 };
 
-struct PdeBwdArgs {
-  const int32_t* x_len;
-  const int32_t* y_len;
-  const int32_t* pairs;
-  int64_t n_pairs;
-  int dyadic, difference;
-  const double* gout;    // [n_pairs] dL/dK
-  const void* u_all;
-  const int64_t* u_off;
-  const void* m_all;
-  const int64_t* m_off;
-  void* gm;              // [Nr][Nc] at m_off[p]: dL/dM of every original cell (accumulated)
-};
-
 int dp_flow_count(int nlev, int order);
 // fused VALU sweep for difference=1 (dp_sweep.h): LDS bytes for a column side of `cols`
 size_t dp_sweep_lds(int dtype, int d, int cols, int n_levels, int order);
@@ -178,7 +157,6 @@ hipError_t dp_stream_launch(const StreamEntry& e, const DpStreamArgs& a, size_t 
                             hipStream_t s);
 int dp_stream_occupancy(const StreamEntry& e, size_t lds);
 hipError_t pde_forward_launch(const PdeFwdArgs& a, int dtype, size_t lds, hipStream_t s);
-hipError_t pde_backward_launch(const PdeBwdArgs& a, int dtype, size_t lds, hipStream_t s);
 size_t pde_bwd_edges_lds(int dtype, int d, int ly, int pc);
 hipError_t pde_bwd_edges_launch(const PdeBwdEdgeArgs& a, int dtype, size_t lds, hipStream_t s);
 hipError_t grad_contract_launch(const ContractArgs& a, bool f64, int max_len_x, int max_len_y,

@@ -548,76 +548,6 @@ int ksig_signature_pde(int dtype, const void* x, const int64_t* x_off,
   return pde_common(dtype, a, cols < 1 ? 1 : cols, dmax_for(d), stream);
 }
 
-int ksig_signature_pde_fwd_dump(int dtype, const void* x, const int64_t* x_off,
-                                const int32_t* x_len, int max_len_x, const void* y,
-                                const int64_t* y_off, const int32_t* y_len,
-                                int max_len_y, int d, const int32_t* pairs,
-                                int64_t n_pairs, int static_kind, double static_param,
-                                int dyadic_order, int difference, double* out,
-                                void* dump_u, const int64_t* u_off, void* dump_m,
-                                const int64_t* m_off, void* stream) {
-  int rc = check_common(dtype, x, y, d, pairs, n_pairs, out);
-  if (rc) return rc;
-  ksig::PdeFwdArgs a{};
-  rc = static_scales(static_kind, static_param, &a.xscale, &a.yscale);
-  if (rc) return rc;
-  if (n_pairs == 0) return 0;
-  if (!dump_u || !u_off || !dump_m || !m_off) return fail(KSIG_EINVAL, "null dump buffer");
-  a.x = {x, x_off, x_len};
-  a.y = {y, y_off, y_len};
-  a.d = d;
-  a.pairs = pairs;
-  a.n_pairs = n_pairs;
-  a.kind = static_kind;
-  a.dyadic = dyadic_order;
-  a.difference = difference ? 1 : 0;
-  a.out = out;
-  a.dump_u = dump_u;
-  a.u_off = u_off;
-  a.dump_m = dump_m;
-  a.m_off = m_off;
-  const int cols = max_len_x < max_len_y ? max_len_x : max_len_y;
-  return pde_common(dtype, a, cols < 1 ? 1 : cols, dmax_for(d), stream);
-}
-
-int ksig_signature_pde_bwd(int dtype, const int32_t* x_len, int max_len_x,
-                           const int32_t* y_len, int max_len_y, const int32_t* pairs,
-                           int64_t n_pairs, int dyadic_order, int difference,
-                           const double* grad_out, const void* u_all, const int64_t* u_off,
-                           const void* m_all, const int64_t* m_off, void* grad_m,
-                           void* stream) {
-  if (dtype != KSIG_F32 && dtype != KSIG_F64) return fail(KSIG_EINVAL, "bad dtype %d", dtype);
-  if (n_pairs < 0 || n_pairs > 0x7fffffffLL) return fail(KSIG_EINVAL, "bad n_pairs");
-  if (dyadic_order < 0 || dyadic_order > 6) return fail(KSIG_EINVAL, "dyadic_order must be in [0, 6]");
-  if (n_pairs == 0) return 0;
-  if (!x_len || !y_len || !pairs || !grad_out || !u_all || !u_off || !m_all || !m_off || !grad_m)
-    return fail(KSIG_EINVAL, "null device pointer");
-  ksig::PdeBwdArgs a{};
-  a.x_len = x_len;
-  a.y_len = y_len;
-  a.pairs = pairs;
-  a.n_pairs = n_pairs;
-  a.dyadic = dyadic_order;
-  a.difference = difference ? 1 : 0;
-  a.gout = grad_out;
-  a.u_all = u_all;
-  a.u_off = u_off;
-  a.m_all = m_all;
-  a.m_off = m_off;
-  a.gm = grad_m;
-  int cols = max_len_x < max_len_y ? max_len_x : max_len_y;
-  cols = difference ? cols - 1 : cols;
-  if (cols < 1) cols = 1;
-  long pc = (long)cols << dyadic_order;
-  if (pc < ksig::kWave) pc = ksig::kWave;
-  const size_t lds = (size_t)pc * 4 * (dtype == KSIG_F32 ? 4 : 8);
-  if (lds > max_lds_bytes())
-    return fail(KSIG_ELDS, "adjoint LDS state of %zu bytes exceeds the device limit", lds);
-  hipError_t e = ksig::pde_backward_launch(a, dtype, lds, (hipStream_t)stream);
-  if (e != hipSuccess) return hip_fail(e, "pde_backward launch");
-  return 0;
-}
-
 int ksig_signature_pde_fwd_edges(int dtype, const void* x, const int64_t* x_off,
                                  const int32_t* x_len, int max_len_x, const void* y,
                                  const int64_t* y_off, const int32_t* y_len,

@@ -15,7 +15,7 @@
 //   U(r-1,c-1) = ((U(r-1,c) + U(r,c-1)) A(r,c) - U(r,c)) / B(r,c),
 // so the same reverse sweep rebuilds every U from the forward's last refined
 // row and last refined column (O(Lx + Ly) values per pair, written by
-// ksig_signature_pde_fwd_edges) instead of reading a dump of all cells.  The
+// ksig_signature_pde_fwd_edges), with no per-cell storage.  The
 // rebuild runs in float64 (full rate on the vector ALUs of gfx950): it is a
 // 2-D difference recursion whose rounding errors add up along the sweep.
 //

@@ -152,8 +152,6 @@ pde_forward_kernel(const PdeFwdArgs a) {
   T xv2[DMAX], xnrm2;            // next band's row, prefetched
   load_x(xv2, xnrm2, ((rr + kWave) >> lam) + rshift);
 
-  T* dump_u = a.dump_u ? reinterpret_cast<T*>(a.dump_u) + a.u_off[pr] : nullptr;
-  T* dump_m = a.dump_m ? reinterpret_cast<T*>(a.dump_m) + a.m_off[pr] : nullptr;
   T* edges = a.edges ? reinterpret_cast<T*>(a.edges) + a.e_off[pr] : nullptr;
 
   T fl[NF] = {T(0), T(0), T(1), T(0)};
@@ -203,8 +201,6 @@ pde_forward_kernel(const PdeFwdArgs a) {
     u = cell ? u : T(1);  // priming / padding columns carry the boundary value
     v = cell ? v : in_v;
     if (cell && rr == Nrr - 1 && cc == Ncr - 1) a.out[pr] = (double)u;
-    if (dump_u != nullptr && cell) dump_u[(int64_t)rr * Ncr + cc] = u;
-    if (dump_m != nullptr && cell && leader && newcol) dump_m[(int64_t)orow * Nc + ocol] = M;
     if (edges != nullptr && cell) {  // last refined row, then last refined column
       if (rr == Nrr - 1) edges[cc] = u;
       if (cc == Ncr - 1) edges[Ncr + rr] = u;
