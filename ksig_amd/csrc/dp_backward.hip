@@ -31,6 +31,7 @@
 #include "ksig_common.h"
 #include "ksig_hip.h"
 #include "dp_forward.h"
+#include "dp_backward_mfma.h"
 
 namespace ksig {
 
@@ -412,6 +413,21 @@ size_t dp_backward_lds(int dtype, int d, int cols, int n_levels) {
   const int nf = 3 + 2 * (n_levels - 1);
   const int pc = cols + 1 > kWave ? cols + 1 : kWave;
   return ((size_t)kWave * (dmax + 1 + nl1) + (size_t)cols + (size_t)pc * nf) * esz;
+}
+
+hipError_t dp_backward_mfma_d32(const DpBwdArgs& a, size_t lds, hipStream_t s);
+
+size_t dp_backward_mfma_lds(int d, int cols, int n_levels) {
+  (void)d;
+  const int nl1 = n_levels > 1 ? n_levels - 1 : 1;
+  const int nf = 3 + 2 * (n_levels - 1);
+  const int pc = bwd_mfma_pc(cols - 1);  // Nc = shorter side - 1
+  return ((size_t)kBwdRingW * kBwdRingS + kWave + (size_t)kWave * nl1 + (size_t)pc * nf) * 4;
+}
+
+hipError_t dp_backward_mfma_launch(const DpBwdArgs& a, size_t lds, hipStream_t s) {
+  if (a.d <= 16) return hipErrorInvalidValue;  // built for the 17..32 bucket only
+  return dp_backward_mfma_d32(a, lds, s);
 }
 
 hipError_t dp_backward_launch(const DpBwdArgs& a, int dtype, size_t lds, hipStream_t s) {

@@ -1,0 +1,439 @@
+// Adjoint of the first-order DP signature kernel, float32, with the static
+// Gram on the matrix cores (gfx950).  Template implementation; instantiated
+// by dp_backward_mfma_d*.hip.
+//
+// Same reverse sweep as dp_backward.hip (the adjoint of the recursion of
+// ksig/algorithms.py:47-85 on M = ΔΔ k(x_i, y_j), ksig/algorithms.py:64,
+// ksig/static/kernels.py:145-155, 248-260): lane L of band b owns flipped row
+// 64b + L (M row i = Nr-1-(64b+L)) and walks the columns right to left, one
+// step behind lane L-1; the column flows move by DPP and lane 63 hands over
+// to the next band through an LDS FIFO.
+//
+// What changes is where the static-kernel values come from.  The lane no
+// longer evaluates <x_i, y_j> itself (which needs the whole y row of every
+// cell in registers: 4d bytes per cell through the vector-memory path).
+// Every 16 steps the wave computes a 64-row x 16-column tile of dot products
+// on v_mfma_f32_16x16x4f32 and writes the argument of the static kernel
+// (scaled squared distance for RBF, scaled dot product for linear) into an
+// LDS ring.  The ring is skewed: ring column g = band * Pc + s of row r lives
+// in slot (g + r) mod W, so at step T every lane reads slot T mod W at its
+// own row, with no address arithmetic.  Pc is a multiple of 16, so a 16-column
+// chunk never straddles a band, and W = 80 keeps chunk k until lane 63 has read
+// it (chunk k + 5 overwrites it at step 16k + 80; lane 63 reads its last column
+// at step 16k + 78).
+//
+// MFMA operand layout: lane (c16 = lane & 15, grp = lane >> 4) of instruction
+// kb holds feature grp * KB + kb (KB = DMAX / 4) of x row 16t + c16 (A, four
+// row tiles t) and of y column c16 (B).  Every feature is covered once, and
+// each lane loads KB consecutive features of a row (vector loads when d = DMAX).
+#pragma once
+
+#include "ksig_common.h"
+#include "ksig_hip.h"
+#include "dp_forward.h"
+
+namespace ksig {
+
+typedef float bmf4 __attribute__((ext_vector_type(4)));
+
+constexpr int kBwdRingW = 80;  // ring columns: 64-lane skew + one 16-column chunk
+constexpr int kBwdRingS = 65;  // ring column stride (floats)
+
+// Features grp*KB .. grp*KB+KB-1 of one row into v (KB = DMAX / 4): vector
+// loads when VEC (d == DMAX, 16-byte aligned rows), else clamped scalar loads
+// with the features past d zeroed.  Straight-line either way.
+template <int DMAX, bool VEC>
+__device__ __forceinline__ void bwd_mfma_feats(float* v, const float* row, int d, int grp) {
+  constexpr int KB = DMAX / 4;
+  const int f0 = grp * KB;
+  if constexpr (VEC && KB % 4 == 0) {
+#pragma unroll
+    for (int e = 0; e < KB; e += 4) {
+      const float4 t = *reinterpret_cast<const float4*>(row + f0 + e);
+      v[e] = t.x; v[e + 1] = t.y; v[e + 2] = t.z; v[e + 3] = t.w;
+    }
+  } else if constexpr (VEC && KB == 2) {
+    const float2 t = *reinterpret_cast<const float2*>(row + f0);
+    v[0] = t.x; v[1] = t.y;
+  } else {
+#pragma unroll
+    for (int e = 0; e < KB; ++e) {
+      const int f = f0 + e;
+      const float t = row[f < d ? f : d - 1];
+      v[e] = f < d ? t : 0.f;
+    }
+  }
+}
+
+__host__ __device__ constexpr int bwd_mfma_pc(int nc) {
+  // priming column, Nc columns, column -1; whole chunks; at least 80 so that
+  // lane 63 wrote a FIFO slot 16 steps before lane 0 prefetches it
+  return (((nc + 2 > 80 ? nc + 2 : 80) + 15) / 16) * 16;
+}
+
+template <int NLEV, int DMAX, int KIND, bool VEC>
+__global__ void __launch_bounds__(kWave)
+dp_backward_mfma_kernel(const DpBwdArgs a) {
+  using T = float;
+  constexpr int NL1 = NLEV > 1 ? NLEV - 1 : 1;
+  constexpr int NC = NLEV - 1;              // prefix levels 1..NLEV-1
+  constexpr int NF = 3 + 2 * (NLEV - 1);    // [arg, D', gM, C_1.., Dq_1..]
+  constexpr int KB = DMAX / 4;
+  constexpr int W = kBwdRingW, RS = kBwdRingS;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* ring = reinterpret_cast<T*>(smem_raw);  // [W][RS] static-kernel arguments
+  T* xnrm = ring + W * RS;                    // [64] x norms of the producing band
+  T* relst = xnrm + kWave;                    // [64][NL1] lane-private row-end prefixes
+  T* fifo = relst + kWave * NL1;              // [Pc][NF]
+
+  const int lane = threadIdx.x;
+  const int c16 = lane & 15, grp = lane >> 4;
+  const int64_t pr = blockIdx.x;
+  const int d = a.d;
+  constexpr bool rbf = KIND == kRBF;
+  const int32_t ia = a.pairs[2 * pr];
+  const int32_t ib = a.pairs[2 * pr + 1];
+  const T* xs = reinterpret_cast<const T*>(a.x.data) + a.x.off[ia] * d;
+  const T* ys = reinterpret_cast<const T*>(a.y.data) + a.y.off[ib] * d;
+  int Lx = a.x.len[ia], Ly = a.y.len[ib];
+  const bool swapped = Ly > Lx;  // same orientation rule as the forward
+  if (swapped) {
+    const T* tp = xs; xs = ys; ys = tp;
+    int tl = Lx; Lx = Ly; Ly = tl;
+  }
+  const int Nr = Lx - 1, Nc = Ly - 1;
+  if (Nr <= 0 || Nc <= 0) {
+    if (lane == 0) a.gparam[pr] = 0.0;
+    return;
+  }
+  const int nrows = Nr + 1;  // + virtual row above the grid
+  const int Pc = bwd_mfma_pc(Nc);
+  const int nb = (nrows + kWave - 1) / kWave;
+  const int ncols = nb * Pc;  // ring columns in total
+
+  T w[NLEV + 1];
+#pragma unroll
+  for (int m = 1; m <= NLEV; ++m) w[m] = T(a.gout[pr * (NLEV + 1) + m]);
+  const T* rowend = reinterpret_cast<const T*>(a.save) + a.save_off[pr];  // [NC][Nr]
+  const T* colsum = rowend + NC * Nr;                                       // [NC][Nc]
+  T* omega = reinterpret_cast<T*>(a.omega) + a.omega_off[pr];
+
+  const T xscale = T(a.xscale), yscale = T(a.yscale);
+  const T xyscale = xscale * yscale, xs2 = xscale * xscale, ys2 = yscale * yscale;
+
+  // ---- Gram producer -------------------------------------------------------
+  T A[4][KB];  // raw x rows of the producing band (row tile t, feature block grp)
+  T B[KB];     // raw y rows of the next chunk's 16 columns
+  auto load_A = [&](int b) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      int i = Nr - 1 - (b * kWave + 16 * t + c16);
+      i = i < 0 ? 0 : i;  // virtual row and rows past the grid: masked in the sweep
+      bwd_mfma_feats<DMAX, VEC>(A[t], xs + (int64_t)i * d, d, grp);
+    }
+  };
+  auto load_B = [&](int k) {  // chunk k: ring columns 16k .. 16k+15, one band
+    const int g0 = 16 * k;
+    const int s = g0 - (g0 / Pc) * Pc + c16;
+    int j = Nc - s;
+    j = j < 0 ? 0 : j;  // columns past the grid: masked in the sweep
+    bwd_mfma_feats<DMAX, VEC>(B, ys + (int64_t)j * d, d, grp);
+  };
+  auto x_norms = [&]() {  // after load_A: scaled squared norms of the band's rows
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      T s2 = T(0);
+#pragma unroll
+      for (int e = 0; e < KB; ++e) s2 += A[t][e] * A[t][e];
+      s2 += __shfl_xor(s2, 16, kWave);
+      s2 += __shfl_xor(s2, 32, kWave);
+      if (grp == 0) xnrm[16 * t + c16] = s2 * xs2;
+    }
+  };
+  auto produce = [&](int k) {
+    bmf4 acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] = bmf4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[t][kb], B[kb], acc[t], 0, 0, 0);
+    T yn = T(0);
+#pragma unroll
+    for (int e = 0; e < KB; ++e) yn += B[e] * B[e];
+    yn += __shfl_xor(yn, 16, kWave);
+    yn += __shfl_xor(yn, 32, kWave);
+    yn *= ys2;
+    const int base = (16 * k) % W + c16 + 4 * grp;  // < W + 16 + 12
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int row = 16 * t + 4 * grp + q;
+        const T dot = acc[t][q] * xyscale;
+        const T arg = rbf ? xnrm[row] + yn - T(2) * dot : dot;
+        int slot = base + 16 * t + q;  // (16k + c16 + row) mod W
+        slot = slot >= W ? slot - W : slot;
+        slot = slot >= W ? slot - W : slot;
+        ring[slot * RS + row] = arg;
+      }
+  };
+
+  // Stage the row-end prefixes of flipped row fr into this lane's slot.
+  auto stage = [&](int fr) {
+    const int i = Nr - 1 - fr;
+    const bool ok = i >= 0 && fr < nrows;
+#pragma unroll
+    for (int m = 0; m < NC; ++m) relst[lane * NL1 + m] = ok ? rowend[m * Nr + i] : T(0);
+  };
+  T rel[NL1];
+  auto take = [&]() {
+#pragma unroll
+    for (int m = 0; m < NL1; ++m) rel[m] = relst[lane * NL1 + m];
+  };
+
+  // FIFO: the row below the grid (G row Nr = Lx-1) with the forward's final
+  // column sums; slot s covers column j = Nc - s.
+  {
+    T xb[DMAX];
+    T xbn = T(0);
+#pragma unroll
+    for (int k = 0; k < DMAX; ++k) {
+      const T v = k < d ? xs[(int64_t)Nr * d + k] : T(0);
+      xb[k] = v * xyscale;
+      xbn += v * v;
+    }
+    xbn *= xs2;
+    for (int s = lane; s < Pc; s += kWave) {
+      const int j = Nc - s;
+      T g = T(0), dv = T(0);
+      if (j >= 0) {
+        T dot = T(0), yn = T(0);
+#pragma unroll
+        for (int k = 0; k < DMAX; ++k) {
+          const T v = k < d ? ys[(int64_t)j * d + k] : T(0);
+          dot += xb[k] * v;
+          yn += v * v;
+        }
+        yn *= ys2;
+        if (rbf) {
+          dv = xbn + yn - T(2) * dot;
+          g = fast_exp2(-dv);
+        } else {
+          dv = g = dot;
+        }
+      }
+      fifo[s * NF + 0] = g;
+      fifo[s * NF + 1] = dv;
+      fifo[s * NF + 2] = T(0);
+#pragma unroll
+      for (int m = 0; m < NC; ++m) {
+        fifo[s * NF + 3 + m] = (j >= 0 && j < Nc) ? colsum[m * Nc + j] : T(0);
+        fifo[s * NF + 3 + NC + m] = T(0);
+      }
+    }
+  }
+  stage(lane);
+  load_A(0);
+  load_B(0);
+  __syncthreads();
+  take();
+  x_norms();
+
+  T fl[NF];
+#pragma unroll
+  for (int k = 0; k < NF; ++k) fl[k] = T(0);
+  T P[NL1], S[NL1];
+#pragma unroll
+  for (int m = 0; m < NL1; ++m) P[m] = S[m] = T(0);
+  T g_right = T(0), gup_right = T(0), dup_right = T(0), gm_right = T(0), gmup_right = T(0);
+  T gsig_row = T(0);
+  double gsig = 0.0;
+
+  int s = -lane;  // step within the band: column j = Nc - s
+  int band = 0;
+  int fr = lane;  // flipped row
+  const int total = ncols + kWave - 1;
+  // band b+1 is staged at step b*Pc + 63: every lane has taken band b by then
+  int next_stage = kWave - 1;
+  int stage_band = 1;
+  int rslot = 0;  // ring slot of the prefetched step: (step + 1) mod W
+  bool norms_due = false;
+  // omega index of G point (p, q) = (i+1, j+1) in the caller's orientation:
+  // p * Ly + q, or q * Lx + p when swapped; q falls by one per step.
+  const int cst = swapped ? Lx : 1;
+  const int rst = swapped ? 1 : Ly;
+  int oi_row = (Nr - fr) * rst;        // p = i + 1 = Nr - fr
+  int oi_col = (Nc + 1 - s) * cst;     // q = j + 1 = Nc + 1 - s
+
+  // One step ahead: the ring value and lane 0's FIFO entry of step t+1 are
+  // read at the top of step t (chunk k is produced at step 16k - 1 for that).
+  auto produce_chunk = [&](int k) {  // wave-uniform
+    if (16 * k >= ncols) return;
+    if (norms_due) {  // first chunk of a band: A was loaded 16 steps ago
+      x_norms();
+      norms_due = false;
+    }
+    produce(k);
+    const int gn = 16 * k + 16;
+    if (gn < ncols) {
+      load_B(k + 1);
+      if (gn % Pc == 0) {
+        load_A(gn / Pc);
+        norms_due = true;
+      }
+    }
+  };
+  // Inputs of step t (the ring value of this lane, lane 0's FIFO entry) are
+  // read during step t-1 into one of two register sets that alternate over a
+  // two-step unroll.  Every lane reads lane 0's FIFO slot (a broadcast; only
+  // lane 0 keeps it), so the read needs no lane mask.
+  auto prefetch = [&](int st, T& arg_o, T* fin_o) {  // inputs of step st (uniform)
+    arg_o = ring[rslot * RS + lane];
+    const int f0 = st - (st / Pc) * Pc;  // lane 0's column slot at step st
+#pragma unroll
+    for (int k = 0; k < NF; ++k) fin_o[k] = fifo[f0 * NF + k];
+  };
+  auto step_fn = [&](int step, const T arg, const T* fin, T& arg_o, T* fin_o) {
+    if (((step + 1) & 15) == 0) produce_chunk((step + 1) >> 4);
+    rslot = rslot + 1 == W ? 0 : rslot + 1;
+    prefetch(step + 1, arg_o, fin_o);
+    if (step == next_stage) {  // wave-uniform
+      if (stage_band < nb) stage(stage_band * kWave + lane);
+      ++stage_band;
+      next_stage += Pc;
+    }
+    const bool active = s >= 0 && band < nb;
+    T in[NF];
+#pragma unroll
+    for (int k = 0; k < NF; ++k) in[k] = shift_down(fl[k], fin[k]);
+
+    const int j = Nc - s;
+    const int i = Nr - 1 - fr;
+    const bool gvalid = active && i >= 0 && j >= 0;
+    T g = rbf ? fast_exp2(-arg) : arg;
+    g = gvalid ? g : T(0);
+    const T dv = gvalid ? arg : T(0);
+    const bool prim = s == 0;
+    const bool cell = active && i >= 0 && j >= 0 && j < Nc;
+    const T M = cell ? (gup_right - in[0]) - (g_right - g) : T(0);
+
+    // forward reconstruction: C_a(i,j), P_a(i,j)
+    T Pn[NL1], Cout[NL1];
+    {
+      T R = M;
+#pragma unroll
+      for (int m = 0; m < NC; ++m) {
+        const T C = in[3 + m] - R;
+        Pn[m] = P[m] - C;
+        Cout[m] = C;
+        R = M * Pn[m];
+      }
+    }
+    // adjoint
+    T lam[NLEV + 1];
+    lam[NLEV] = w[NLEV];
+#pragma unroll
+    for (int m = NLEV - 1; m >= 1; --m) lam[m] = w[m] + S[m - 1];
+    T gM = lam[1];
+#pragma unroll
+    for (int m = 2; m <= NLEV; ++m) gM += lam[m] * Pn[m - 2];
+    gM = cell ? gM : T(0);
+    T Dout[NL1];
+#pragma unroll
+    for (int m = 0; m < NC; ++m) Dout[m] = in[3 + NC + m] + M * lam[m + 2];
+
+    // finalise dL/dG at G point (i+1, j+1)
+    const bool fin_ok = active && !prim && i >= -1 && j >= -1;
+    const T gG = (gM - gm_right) - (in[2] - gmup_right);
+    const T om = rbf ? gG * gup_right : gG;
+    if (fin_ok) {
+      omega[oi_row + oi_col] = om;
+      gsig_row += om * dup_right;
+    }
+
+    // state
+    g_right = g;
+    gup_right = in[0];
+    dup_right = in[1];
+    gm_right = gM;
+    gmup_right = in[2];
+#pragma unroll
+    for (int m = 0; m < NC; ++m) {
+      P[m] = prim ? rel[m] : Pn[m];
+      S[m] = prim ? T(0) : S[m] + in[3 + NC + m];
+    }
+    fl[0] = g;
+    fl[1] = dv;
+    fl[2] = gM;
+#pragma unroll
+    for (int m = 0; m < NC; ++m) {
+      fl[3 + m] = Cout[m];
+      fl[3 + NC + m] = Dout[m];
+    }
+    if (lane == kWave - 1 && active) {
+#pragma unroll
+      for (int k = 0; k < NF; ++k) fifo[s * NF + k] = fl[k];
+    }
+
+    ++s;
+    oi_col -= cst;
+    if (s == Pc) {
+      s = 0;
+      ++band;
+      fr += kWave;
+      oi_row = (Nr - fr) * rst;
+      oi_col = (Nc + 1) * cst;
+      gsig += (double)gsig_row;
+      gsig_row = T(0);
+      take();
+    }
+  };
+
+  T arg0, arg1, fin0[NF], fin1[NF];
+  produce_chunk(0);
+  prefetch(0, arg0, fin0);
+  int step = 0;
+  for (; step + 2 <= total; step += 2) {
+    step_fn(step, arg0, fin0, arg1, fin1);
+    step_fn(step + 1, arg1, fin1, arg0, fin0);
+  }
+  if (step < total) step_fn(step, arg0, fin0, arg1, fin1);
+  gsig += (double)gsig_row;
+  gsig = wave_sum(gsig);
+  if (lane == 0) a.gparam[pr] = gsig;
+}
+
+template <int NLEV, int DMAX, int KIND, bool VEC>
+hipError_t launch_bwd_mfma(const DpBwdArgs& a, size_t lds, hipStream_t s) {
+  auto k = dp_backward_mfma_kernel<NLEV, DMAX, KIND, VEC>;
+  if (lds > 64 * 1024) {
+    hipError_t e = hipFuncSetAttribute((const void*)k,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k, dim3((unsigned)a.n_pairs), dim3(kWave), lds, s, a);
+  return hipGetLastError();
+}
+
+#define KSIG_BWD_MFMA_LEVELS(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(12)
+
+template <int DMAX>
+hipError_t dispatch_bwd_mfma(const DpBwdArgs& a, size_t lds, hipStream_t s) {
+  const bool vec = a.d == DMAX && ((uintptr_t)a.x.data & 15) == 0 &&
+                   ((uintptr_t)a.y.data & 15) == 0;
+#define KSIG_CASE(L)                                                             \
+  if (a.n_levels == L) {                                                         \
+    if (a.kind == kRBF)                                                          \
+      return vec ? launch_bwd_mfma<L, DMAX, kRBF, true>(a, lds, s)               \
+                 : launch_bwd_mfma<L, DMAX, kRBF, false>(a, lds, s);             \
+    return vec ? launch_bwd_mfma<L, DMAX, kLinear, true>(a, lds, s)              \
+               : launch_bwd_mfma<L, DMAX, kLinear, false>(a, lds, s);            \
+  }
+  KSIG_BWD_MFMA_LEVELS(KSIG_CASE)
+#undef KSIG_CASE
+  return hipErrorInvalidValue;
+}
+
+}  // namespace ksig

@@ -143,6 +143,9 @@ hipError_t dp_forward_launch(const DpFwdArgs& a, int dtype, size_t lds, hipStrea
 size_t dp_backward_lds(int dtype, int d, int cols, int n_levels);
 hipError_t dp_backward_launch(const DpBwdArgs& a, int dtype, size_t lds, hipStream_t s);
 bool dp_backward_supported(int n_levels);
+// float32 adjoint with the static Gram on MFMA (dp_backward_mfma.h)
+size_t dp_backward_mfma_lds(int d, int cols, int n_levels);
+hipError_t dp_backward_mfma_launch(const DpBwdArgs& a, size_t lds, hipStream_t s);
 bool dp_mfma_supported(int n_levels, int order);
 // mode 1: float32 MFMA Gram, mode 2: bf16 hi/lo split MFMA Gram
 hipError_t dp_mfma_launch(const DpFwdArgs& a, int mode, size_t lds, hipStream_t s);

@@ -443,10 +443,17 @@ int ksig_signature_levels_bwd(int dtype, const void* x, const int64_t* x_off,
   a.gparam = grad_param;
   int cols = max_len_x < max_len_y ? max_len_x : max_len_y;
   if (cols < 1) cols = 1;
-  const size_t lds = ksig::dp_backward_lds(dtype, d, cols, n_levels);
+  if ((int64_t)max_len_x * max_len_y >= (int64_t(1) << 31))
+    return fail(KSIG_EINVAL, "a pair's omega block must stay below 2^31 entries");
+  // float32 with d > 16: static Gram on MFMA (the per-lane y rows of the VALU
+  // adjoint make it bound by the vector-memory path there); else VALU.
+  const bool mfma = dtype == KSIG_F32 && d > 16;
+  const size_t lds = mfma ? ksig::dp_backward_mfma_lds(d, cols, n_levels)
+                          : ksig::dp_backward_lds(dtype, d, cols, n_levels);
   if (lds > max_lds_bytes())
     return fail(KSIG_ELDS, "adjoint LDS state of %zu bytes exceeds the device limit", lds);
-  hipError_t e = ksig::dp_backward_launch(a, dtype, lds, (hipStream_t)stream);
+  hipError_t e = mfma ? ksig::dp_backward_mfma_launch(a, lds, (hipStream_t)stream)
+                      : ksig::dp_backward_launch(a, dtype, lds, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "dp_backward launch");
   return 0;
 }
