@@ -143,7 +143,9 @@ int ksig_signature_levels_fwd_save(int dtype, const void* x, const int64_t* x_of
  * dL/dG for linear), and grad_param[p] = sum omega * D'(a,b) (RBF,
  * D' = log2(e) |x_a - y_b|^2 / max(2 sigma^2, 1e-12)) or sum omega * G (linear).
  * The caller contracts omega with the inputs to obtain dL/dx, dL/dy and the
- * bandwidth / scale gradient.
+ * bandwidth / scale gradient (ksig_static_grad_contract).  workspace: device
+ * scratch of at least ksig_signature_levels_bwd_workspace(...) bytes, 16-byte
+ * aligned (may be NULL when that is 0).
  */
 int ksig_signature_levels_bwd(int dtype, const void* x, const int64_t* x_off,
                               const int32_t* x_len, int max_len_x, const void* y,
@@ -153,7 +155,17 @@ int ksig_signature_levels_bwd(int dtype, const void* x, const int64_t* x_off,
                               int n_levels, const double* grad_levels, const void* save,
                               const int64_t* save_off, void* omega,
                               const int64_t* omega_off, double* grad_param,
-                              void* stream);
+                              void* workspace, size_t workspace_bytes, void* stream);
+
+/*
+ * Workspace bytes ksig_signature_levels_bwd needs for n_pairs pairs of at most
+ * max_len_x x max_len_y points (0 when the chosen adjoint keeps all of its
+ * state on chip: float64, and float32 with d <= 16).  The workspace holds the
+ * adjoint's band hand-over state, one region per pair; contents are scratch.
+ * No reference counterpart (a sizing query of this library).
+ */
+size_t ksig_signature_levels_bwd_workspace(int dtype, int d, int n_levels, int max_len_x,
+                                           int max_len_y, int64_t n_pairs);
 
 /*
  * Chain rule through the static kernel (no reference counterpart: the

@@ -128,12 +128,16 @@ class _SigLevels(torch.autograd.Function):
           w_off[1:] = torch.cumsum(sz, 0)[:-1]
         omega = torch.empty(int(sz.sum().item()), device=dev, dtype=dt)
       gp = torch.empty(c1 - c0, device=dev, dtype=torch.float64)
+      ws_bytes = lib.ksig_signature_levels_bwd_workspace(
+          _lib.dtype_code(dt), Xs.d, n_levels, Xs.max_len, Ys.max_len, c1 - c0)
+      ws = torch.empty(max(ws_bytes // 4, 1), device=dev, dtype=torch.float32)
       _lib.check(lib.ksig_signature_levels_bwd(
           _lib.dtype_code(dt), Xs.data.data_ptr(), Xs.off.data_ptr(), Xs.len.data_ptr(),
           Xs.max_len, Ys.data.data_ptr(), Ys.off.data_ptr(), Ys.len.data_ptr(),
           Ys.max_len, Xs.d, pc.data_ptr(), c1 - c0, kind, pval, n_levels,
           gout[c0:c1].data_ptr(), save.data_ptr(), save_off[c0:c1].data_ptr(),
-          omega.data_ptr(), w_off.data_ptr(), gp.data_ptr(), _lib.stream_handle(dev)))
+          omega.data_ptr(), w_off.data_ptr(), gp.data_ptr(), ws.data_ptr(), ws_bytes,
+          _lib.stream_handle(dev)))
       if ctx.needs_input_grad[2]:
         gparam_total = gparam_total + gp.sum()
       _contract_hip(omega, w_off, pc, Xs, Ys, kind, pval, s_rbf, gx, gy)

@@ -419,10 +419,16 @@ hipError_t dp_backward_mfma_d32(const DpBwdArgs& a, size_t lds, hipStream_t s);
 
 size_t dp_backward_mfma_lds(int d, int cols, int n_levels) {
   (void)d;
+  (void)cols;
   const int nl1 = n_levels > 1 ? n_levels - 1 : 1;
+  return ((size_t)kBwdRingW * kBwdRingS + kWave + (size_t)kWave * nl1) * 4;
+}
+
+int dp_backward_mfma_fifo_cols(int cols) { return bwd_mfma_pc(cols - 1); }  // Nc = cols - 1
+
+size_t dp_backward_mfma_fifo_bytes(int cols, int n_levels) {
   const int nf = 3 + 2 * (n_levels - 1);
-  const int pc = bwd_mfma_pc(cols - 1);  // Nc = shorter side - 1
-  return ((size_t)kBwdRingW * kBwdRingS + kWave + (size_t)kWave * nl1 + (size_t)pc * nf) * 4;
+  return (size_t)dp_backward_mfma_fifo_cols(cols) * ((nf + 3) & ~3) * 4;
 }
 
 hipError_t dp_backward_mfma_launch(const DpBwdArgs& a, size_t lds, hipStream_t s) {

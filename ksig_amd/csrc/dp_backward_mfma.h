@@ -7,7 +7,8 @@
 // ksig/static/kernels.py:145-155, 248-260): lane L of band b owns flipped row
 // 64b + L (M row i = Nr-1-(64b+L)) and walks the columns right to left, one
 // step behind lane L-1; the column flows move by DPP and lane 63 hands over
-// to the next band through an LDS FIFO.
+// to the next band through a FIFO in a global-memory workspace (one region
+// per pair), which keeps LDS to the ring below.
 //
 // What changes is where the static-kernel values come from.  The lane no
 // longer evaluates <x_i, y_j> itself (which needs the whole y row of every
@@ -19,8 +20,9 @@
 // in slot (g + r) mod W, so at step T every lane reads slot T mod W at its
 // own row, with no address arithmetic.  Pc is a multiple of 16, so a 16-column
 // chunk never straddles a band, and W = 80 keeps chunk k until lane 63 has read
-// it (chunk k + 5 overwrites it at step 16k + 80; lane 63 reads its last column
-// at step 16k + 78).
+// it.  Chunk k is produced at step 16k - 2 and every ring read is issued two
+// steps before its use: chunk k + 5 overwrites chunk k at step 16k + 78, after
+// lane 63's read of its last column (issued at step 16k + 76).
 //
 // MFMA operand layout: lane (c16 = lane & 15, grp = lane >> 4) of instruction
 // kb holds feature grp * KB + kb (KB = DMAX / 4) of x row 16t + c16 (A, four
@@ -78,13 +80,13 @@ dp_backward_mfma_kernel(const DpBwdArgs a) {
   constexpr int NL1 = NLEV > 1 ? NLEV - 1 : 1;
   constexpr int NC = NLEV - 1;              // prefix levels 1..NLEV-1
   constexpr int NF = 3 + 2 * (NLEV - 1);    // [arg, D', gM, C_1.., Dq_1..]
+  constexpr int NFG = (NF + 3) & ~3;        // FIFO slot stride (16-byte quads)
   constexpr int KB = DMAX / 4;
   constexpr int W = kBwdRingW, RS = kBwdRingS;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* ring = reinterpret_cast<T*>(smem_raw);  // [W][RS] static-kernel arguments
   T* xnrm = ring + W * RS;                    // [64] x norms of the producing band
   T* relst = xnrm + kWave;                    // [64][NL1] lane-private row-end prefixes
-  T* fifo = relst + kWave * NL1;              // [Pc][NF]
 
   const int lane = threadIdx.x;
   const int c16 = lane & 15, grp = lane >> 4;
@@ -110,6 +112,8 @@ dp_backward_mfma_kernel(const DpBwdArgs a) {
   const int Pc = bwd_mfma_pc(Nc);
   const int nb = (nrows + kWave - 1) / kWave;
   const int ncols = nb * Pc;  // ring columns in total
+  // band hand-over FIFO in this pair's [fifo_cols][NFG] region of the workspace
+  T* fifo = reinterpret_cast<T*>(a.fifo) + (size_t)pr * a.fifo_cols * NFG;
 
   T w[NLEV + 1];
 #pragma unroll
@@ -124,7 +128,7 @@ dp_backward_mfma_kernel(const DpBwdArgs a) {
   // ---- Gram producer -------------------------------------------------------
   T A[4][KB];  // raw x rows of the producing band (row tile t, feature block grp)
   T B[KB];     // raw y rows of the next chunk's 16 columns
-  auto load_A = [&](int b) {
+  auto load_A = [&](int b) __attribute__((always_inline)) {
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       int i = Nr - 1 - (b * kWave + 16 * t + c16);
@@ -132,14 +136,14 @@ dp_backward_mfma_kernel(const DpBwdArgs a) {
       bwd_mfma_feats<DMAX, VEC>(A[t], xs + (int64_t)i * d, d, grp);
     }
   };
-  auto load_B = [&](int k) {  // chunk k: ring columns 16k .. 16k+15, one band
+  auto load_B = [&](int k) __attribute__((always_inline)) {  // chunk k: ring columns 16k .. 16k+15, one band
     const int g0 = 16 * k;
     const int s = g0 - (g0 / Pc) * Pc + c16;
     int j = Nc - s;
     j = j < 0 ? 0 : j;  // columns past the grid: masked in the sweep
     bwd_mfma_feats<DMAX, VEC>(B, ys + (int64_t)j * d, d, grp);
   };
-  auto x_norms = [&]() {  // after load_A: scaled squared norms of the band's rows
+  auto x_norms = [&]() __attribute__((always_inline)) {  // after load_A: scaled squared norms of the band's rows
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       T s2 = T(0);
@@ -150,7 +154,7 @@ dp_backward_mfma_kernel(const DpBwdArgs a) {
       if (grp == 0) xnrm[16 * t + c16] = s2 * xs2;
     }
   };
-  auto produce = [&](int k) {
+  auto produce = [&](int k) __attribute__((always_inline)) {
     bmf4 acc[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) acc[t] = bmf4{0.f, 0.f, 0.f, 0.f};
@@ -181,14 +185,14 @@ dp_backward_mfma_kernel(const DpBwdArgs a) {
   };
 
   // Stage the row-end prefixes of flipped row fr into this lane's slot.
-  auto stage = [&](int fr) {
+  auto stage = [&](int fr) __attribute__((always_inline)) {
     const int i = Nr - 1 - fr;
     const bool ok = i >= 0 && fr < nrows;
 #pragma unroll
     for (int m = 0; m < NC; ++m) relst[lane * NL1 + m] = ok ? rowend[m * Nr + i] : T(0);
   };
   T rel[NL1];
-  auto take = [&]() {
+  auto take = [&]() __attribute__((always_inline)) {
 #pragma unroll
     for (int m = 0; m < NL1; ++m) rel[m] = relst[lane * NL1 + m];
   };
@@ -224,14 +228,20 @@ dp_backward_mfma_kernel(const DpBwdArgs a) {
           dv = g = dot;
         }
       }
-      fifo[s * NF + 0] = g;
-      fifo[s * NF + 1] = dv;
-      fifo[s * NF + 2] = T(0);
+      T v[NFG];
+      v[0] = g;
+      v[1] = dv;
+      v[2] = T(0);
 #pragma unroll
       for (int m = 0; m < NC; ++m) {
-        fifo[s * NF + 3 + m] = (j >= 0 && j < Nc) ? colsum[m * Nc + j] : T(0);
-        fifo[s * NF + 3 + NC + m] = T(0);
+        v[3 + m] = (j >= 0 && j < Nc) ? colsum[m * Nc + j] : T(0);
+        v[3 + NC + m] = T(0);
       }
+#pragma unroll
+      for (int k = NF; k < NFG; ++k) v[k] = T(0);
+#pragma unroll
+      for (int k = 0; k < NFG; k += 4)
+        *reinterpret_cast<float4*>(fifo + s * NFG + k) = float4{v[k], v[k + 1], v[k + 2], v[k + 3]};
     }
   }
   stage(lane);
@@ -258,7 +268,7 @@ dp_backward_mfma_kernel(const DpBwdArgs a) {
   // band b+1 is staged at step b*Pc + 63: every lane has taken band b by then
   int next_stage = kWave - 1;
   int stage_band = 1;
-  int rslot = 0;  // ring slot of the prefetched step: (step + 1) mod W
+  int rslot = 0;  // ring slot of the next prefetched step: (step + 2) mod W
   bool norms_due = false;
   // omega index of G point (p, q) = (i+1, j+1) in the caller's orientation:
   // p * Ly + q, or q * Lx + p when swapped; q falls by one per step.
@@ -269,7 +279,7 @@ dp_backward_mfma_kernel(const DpBwdArgs a) {
 
   // One step ahead: the ring value and lane 0's FIFO entry of step t+1 are
   // read at the top of step t (chunk k is produced at step 16k - 1 for that).
-  auto produce_chunk = [&](int k) {  // wave-uniform
+  auto produce_chunk = [&](int k) __attribute__((always_inline)) {  // wave-uniform
     if (16 * k >= ncols) return;
     if (norms_due) {  // first chunk of a band: A was loaded 16 steps ago
       x_norms();
@@ -286,19 +296,30 @@ dp_backward_mfma_kernel(const DpBwdArgs a) {
     }
   };
   // Inputs of step t (the ring value of this lane, lane 0's FIFO entry) are
-  // read during step t-1 into one of two register sets that alternate over a
-  // two-step unroll.  Every lane reads lane 0's FIFO slot (a broadcast; only
-  // lane 0 keeps it), so the read needs no lane mask.
-  auto prefetch = [&](int st, T& arg_o, T* fin_o) {  // inputs of step st (uniform)
+  // read during step t-2 into one of two register sets: a step consumes its
+  // set, then refills it for the step after next.  Every lane reads lane 0's
+  // FIFO slot (a broadcast; only lane 0 keeps it), so the read needs no lane
+  // mask.  Lane 63 wrote that slot Pc - 65 >= 15 steps earlier.
+  auto prefetch = [&](int st, T& arg_o, T* fin_o) __attribute__((always_inline)) {  // inputs of step st (uniform)
     arg_o = ring[rslot * RS + lane];
+    rslot = rslot + 1 == W ? 0 : rslot + 1;
     const int f0 = st - (st / Pc) * Pc;  // lane 0's column slot at step st
 #pragma unroll
-    for (int k = 0; k < NF; ++k) fin_o[k] = fifo[f0 * NF + k];
+    for (int k = 0; k < NFG; k += 4) {
+      const float4 t = *reinterpret_cast<const float4*>(fifo + f0 * NFG + k);
+      fin_o[k] = t.x;
+      if (k + 1 < NF) fin_o[k + 1] = t.y;
+      if (k + 2 < NF) fin_o[k + 2] = t.z;
+      if (k + 3 < NF) fin_o[k + 3] = t.w;
+    }
   };
-  auto step_fn = [&](int step, const T arg, const T* fin, T& arg_o, T* fin_o) {
-    if (((step + 1) & 15) == 0) produce_chunk((step + 1) >> 4);
-    rslot = rslot + 1 == W ? 0 : rslot + 1;
-    prefetch(step + 1, arg_o, fin_o);
+  auto step_fn = [&](int step, T& arg_io, T* fin_io) __attribute__((always_inline)) {
+    const T arg = arg_io;
+    T fin[NF];
+#pragma unroll
+    for (int k = 0; k < NF; ++k) fin[k] = fin_io[k];
+    if (((step + 2) & 15) == 0) produce_chunk((step + 2) >> 4);
+    prefetch(step + 2, arg_io, fin_io);
     if (step == next_stage) {  // wave-uniform
       if (stage_band < nb) stage(stage_band * kWave + lane);
       ++stage_band;
@@ -374,7 +395,14 @@ dp_backward_mfma_kernel(const DpBwdArgs a) {
     }
     if (lane == kWave - 1 && active) {
 #pragma unroll
-      for (int k = 0; k < NF; ++k) fifo[s * NF + k] = fl[k];
+      for (int k = 0; k < NFG; k += 4) {
+        float4 t;
+        t.x = fl[k];
+        t.y = k + 1 < NF ? fl[k + 1] : T(0);
+        t.z = k + 2 < NF ? fl[k + 2] : T(0);
+        t.w = k + 3 < NF ? fl[k + 3] : T(0);
+        *reinterpret_cast<float4*>(fifo + s * NFG + k) = t;
+      }
     }
 
     ++s;
@@ -391,15 +419,16 @@ dp_backward_mfma_kernel(const DpBwdArgs a) {
     }
   };
 
-  T arg0, arg1, fin0[NF], fin1[NF];
+  T arg0, arg1, fin0[NFG], fin1[NFG];
   produce_chunk(0);
   prefetch(0, arg0, fin0);
+  prefetch(1, arg1, fin1);
   int step = 0;
   for (; step + 2 <= total; step += 2) {
-    step_fn(step, arg0, fin0, arg1, fin1);
-    step_fn(step + 1, arg1, fin1, arg0, fin0);
+    step_fn(step, arg0, fin0);
+    step_fn(step + 1, arg1, fin1);
   }
-  if (step < total) step_fn(step, arg0, fin0, arg1, fin1);
+  if (step < total) step_fn(step, arg0, fin0);
   gsig += (double)gsig_row;
   gsig = wave_sum(gsig);
   if (lane == 0) a.gparam[pr] = gsig;

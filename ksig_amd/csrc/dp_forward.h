@@ -84,6 +84,8 @@ struct DpBwdArgs {
   void* omega;             // per pair [len_x, len_y] at omega_off[p]: dL/dG * dG/d(arg)
   const int64_t* omega_off;
   double* gparam;          // [n_pairs]: sum omega * D' (RBF) or omega * G (linear)
+  void* fifo;              // MFMA variant: band FIFO workspace, [n_pairs][fifo_cols][4-rounded flows]
+  int fifo_cols;
 };
 
 struct PdeFwdArgs {
@@ -145,6 +147,9 @@ hipError_t dp_backward_launch(const DpBwdArgs& a, int dtype, size_t lds, hipStre
 bool dp_backward_supported(int n_levels);
 // float32 adjoint with the static Gram on MFMA (dp_backward_mfma.h)
 size_t dp_backward_mfma_lds(int d, int cols, int n_levels);
+// FIFO slots per pair and workspace bytes per pair of the MFMA variant
+int dp_backward_mfma_fifo_cols(int cols);
+size_t dp_backward_mfma_fifo_bytes(int cols, int n_levels);
 hipError_t dp_backward_mfma_launch(const DpBwdArgs& a, size_t lds, hipStream_t s);
 bool dp_mfma_supported(int n_levels, int order);
 // mode 1: float32 MFMA Gram, mode 2: bf16 hi/lo split MFMA Gram

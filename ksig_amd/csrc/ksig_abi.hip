@@ -408,6 +408,18 @@ int ksig_signature_levels_fwd_save(int dtype, const void* x, const int64_t* x_of
   return dp_levels_common(dtype, a, cols < 1 ? 1 : cols, dmax_for(d), stream);
 }
 
+// float32 with d > 16: the adjoint with the static Gram on MFMA (the per-lane
+// y rows of the VALU adjoint make it bound by the vector-memory path there).
+static bool bwd_uses_mfma(int dtype, int d) { return dtype == KSIG_F32 && d > 16; }
+
+size_t ksig_signature_levels_bwd_workspace(int dtype, int d, int n_levels, int max_len_x,
+                                           int max_len_y, int64_t n_pairs) {
+  if (!bwd_uses_mfma(dtype, d) || n_pairs <= 0 || n_levels < 1) return 0;
+  int cols = max_len_x < max_len_y ? max_len_x : max_len_y;
+  if (cols < 1) cols = 1;
+  return (size_t)n_pairs * ksig::dp_backward_mfma_fifo_bytes(cols, n_levels);
+}
+
 int ksig_signature_levels_bwd(int dtype, const void* x, const int64_t* x_off,
                               const int32_t* x_len, int max_len_x, const void* y,
                               const int64_t* y_off, const int32_t* y_len,
@@ -416,7 +428,7 @@ int ksig_signature_levels_bwd(int dtype, const void* x, const int64_t* x_off,
                               int n_levels, const double* grad_levels, const void* save,
                               const int64_t* save_off, void* omega,
                               const int64_t* omega_off, double* grad_param,
-                              void* stream) {
+                              void* workspace, size_t workspace_bytes, void* stream) {
   int rc = check_common(dtype, x, y, d, pairs, n_pairs, grad_param);
   if (rc) return rc;
   if (n_levels < 1) return fail(KSIG_EINVAL, "n_levels must be positive");
@@ -445,9 +457,16 @@ int ksig_signature_levels_bwd(int dtype, const void* x, const int64_t* x_off,
   if (cols < 1) cols = 1;
   if ((int64_t)max_len_x * max_len_y >= (int64_t(1) << 31))
     return fail(KSIG_EINVAL, "a pair's omega block must stay below 2^31 entries");
-  // float32 with d > 16: static Gram on MFMA (the per-lane y rows of the VALU
-  // adjoint make it bound by the vector-memory path there); else VALU.
-  const bool mfma = dtype == KSIG_F32 && d > 16;
+  const bool mfma = bwd_uses_mfma(dtype, d);
+  if (mfma) {
+    const size_t need = ksig_signature_levels_bwd_workspace(dtype, d, n_levels, max_len_x,
+                                                            max_len_y, n_pairs);
+    if (!workspace || workspace_bytes < need || ((uintptr_t)workspace & 15))
+      return fail(KSIG_EINVAL, "workspace of %zu bytes (16-byte aligned) required, got %zu",
+                  need, workspace ? workspace_bytes : (size_t)0);
+    a.fifo = workspace;
+    a.fifo_cols = ksig::dp_backward_mfma_fifo_cols(cols);
+  }
   const size_t lds = mfma ? ksig::dp_backward_mfma_lds(d, cols, n_levels)
                           : ksig::dp_backward_lds(dtype, d, cols, n_levels);
   if (lds > max_lds_bytes())
