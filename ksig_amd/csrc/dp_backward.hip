@@ -1,4 +1,5 @@
-// Adjoint of the first-order DP signature kernel (gfx950).
+// Adjoint of the first-order DP signature kernel (gfx950), float64 (the
+// float32 adjoint, with the static Gram on MFMA, is dp_backward_mfma.h).
 //
 // The reference has no gradients (its recursion is CuPy-only); this is the
 // reverse-mode derivative of the recursion of ksig/algorithms.py:47-85 on
@@ -415,6 +416,8 @@ size_t dp_backward_lds(int dtype, int d, int cols, int n_levels) {
   return ((size_t)kWave * (dmax + 1 + nl1) + (size_t)cols + (size_t)pc * nf) * esz;
 }
 
+hipError_t dp_backward_mfma_d8(const DpBwdArgs& a, size_t lds, hipStream_t s);
+hipError_t dp_backward_mfma_d16(const DpBwdArgs& a, size_t lds, hipStream_t s);
 hipError_t dp_backward_mfma_d32(const DpBwdArgs& a, size_t lds, hipStream_t s);
 
 size_t dp_backward_mfma_lds(int d, int cols, int n_levels) {
@@ -432,12 +435,14 @@ size_t dp_backward_mfma_fifo_bytes(int cols, int n_levels) {
 }
 
 hipError_t dp_backward_mfma_launch(const DpBwdArgs& a, size_t lds, hipStream_t s) {
-  if (a.d <= 16) return hipErrorInvalidValue;  // built for the 17..32 bucket only
+  if (a.d <= 8) return dp_backward_mfma_d8(a, lds, s);
+  if (a.d <= 16) return dp_backward_mfma_d16(a, lds, s);
   return dp_backward_mfma_d32(a, lds, s);
 }
 
 hipError_t dp_backward_launch(const DpBwdArgs& a, int dtype, size_t lds, hipStream_t s) {
-  if (dtype == KSIG_F32) return dispatch_bwd<float>(a, lds, s);
+  // float32 runs the MFMA variant (dp_backward_mfma.h)
+  if (dtype == KSIG_F32) return hipErrorInvalidValue;
   return dispatch_bwd<double>(a, lds, s);
 }
 

@@ -408,9 +408,9 @@ int ksig_signature_levels_fwd_save(int dtype, const void* x, const int64_t* x_of
   return dp_levels_common(dtype, a, cols < 1 ? 1 : cols, dmax_for(d), stream);
 }
 
-// float32 with d > 16: the adjoint with the static Gram on MFMA (the per-lane
-// y rows of the VALU adjoint make it bound by the vector-memory path there).
-static bool bwd_uses_mfma(int dtype, int d) { return dtype == KSIG_F32 && d > 16; }
+// float32: the adjoint with the static Gram on MFMA; float64: the VALU adjoint
+// (no float64 MFMA Gram: the float64 path is the verification dtype).
+static bool bwd_uses_mfma(int dtype, int d) { return dtype == KSIG_F32 && d > 0; }
 
 size_t ksig_signature_levels_bwd_workspace(int dtype, int d, int n_levels, int max_len_x,
                                            int max_len_y, int64_t n_pairs) {
