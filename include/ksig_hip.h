@@ -168,6 +168,31 @@ size_t ksig_signature_levels_bwd_workspace(int dtype, int d, int n_levels, int m
                                            int max_len_y, int64_t n_pairs);
 
 /*
+ * Chain rule of the signature-PDE adjoint through ΔΔ and the static kernel
+ * (no reference counterpart: ksig/algorithms.py:402-446 and the static kernels
+ * of ksig/static/kernels.py:134-166, 245-260 are forward-only).  For every
+ * pair p = (i, j), grad_m + m_off[p] holds dL/dM of
+ * ksig_signature_pde_bwd_edges (dtype, [Nr][Nc] in the kernel's orientation:
+ * rows = the longer sequence, Nr/Nc = lengths - difference).  With
+ * dL/dG = ΔΔ^T(dL/dM) (difference=1) or dL/dM (difference=0) over the static
+ * Gram G(a, b) = k(x_a, y_b), this ADDS
+ *   grad_x[x_off[i] + a] += sum_b dL/dG(a,b) dG(a,b)/dx_a,
+ *   grad_y[y_off[j] + b] += sum_a dL/dG(a,b) dG(a,b)/dy_b,
+ *   *grad_param += sum over all pairs of dL/dG dG/dparam
+ * (RBF: param = sigma; linear: param = scale).  Any of grad_x, grad_y,
+ * grad_param may be NULL; grad_x and grad_y may alias.  Device atomics: the
+ * result is order-dependent in its last bits.  1 <= d <= 32.
+ */
+int ksig_signature_pde_grad_contract(int dtype, const void* x, const int64_t* x_off,
+                                     const int32_t* x_len, int max_len_x, const void* y,
+                                     const int64_t* y_off, const int32_t* y_len,
+                                     int max_len_y, int d, const int32_t* pairs,
+                                     int64_t n_pairs, int static_kind, double static_param,
+                                     int difference, const void* grad_m,
+                                     const int64_t* m_off, void* grad_x, void* grad_y,
+                                     double* grad_param, void* stream);
+
+/*
  * Chain rule through the static kernel (no reference counterpart: the
  * reference's static kernels, ksig/static/kernels.py:134-166 and 245-260, have
  * no gradients).  For every pair p = (i, j) with omega_p = omega + omega_off[p]

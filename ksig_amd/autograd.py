@@ -209,8 +209,9 @@ _GM_BYTES = 1 << 30  # dL/dM scratch per chunk of pairs
 class _SigPDE(torch.autograd.Function):
   """The forward also leaves U of the last refined row and column of every
   pair (O(Lx + Ly) values); the adjoint kernel rebuilds the rest of U from
-  them in its reverse sweep and writes dL/dM per original cell, which the
-  ΔΔ-transpose and static-kernel chain rule below turn into input gradients."""
+  them in its reverse sweep and writes dL/dM per original cell, which
+  ksig_signature_pde_grad_contract turns into input and parameter gradients
+  (ΔΔ-transpose and static-kernel chain rule, fused with the contraction)."""
 
   @staticmethod
   def forward(ctx, xdata, ydata, param, xs_meta, ys_meta, pairs, kind, dyadic, difference):
@@ -266,7 +267,6 @@ class _SigPDE(torch.autograd.Function):
     chunk_id = ((torch.cumsum(msz, 0) - msz) // budget).cpu()
     change = torch.nonzero(chunk_id[1:] != chunk_id[:-1]).flatten() + 1
     bounds = [0] + change.tolist() + [n_pairs]
-    s_rbf = 1.0 / max(2.0 * pval * pval, 1e-12)
     stream = _lib.stream_handle(dev)
     for c0, c1 in zip(bounds[:-1], bounds[1:]):
       pc = pairs[c0:c1].contiguous()
@@ -281,57 +281,14 @@ class _SigPDE(torch.autograd.Function):
           Xs.d, pc.data_ptr(), P, kind, pval, int(dyadic), int(difference),
           gout[c0:c1].data_ptr(), edges.data_ptr(), e_off[c0:c1].data_ptr(), gm.data_ptr(),
           m_off.data_ptr(), stream))
-      _pde_chain(gm, m_off, pc, Xs, Ys, la[c0:c1], lb[c0:c1], kind, pval, s_rbf,
-                 difference, gx, gy, gparam)
+      _lib.check(lib.ksig_signature_pde_grad_contract(
+          _lib.dtype_code(dt), Xs.data.data_ptr(), Xs.off.data_ptr(), Xs.len.data_ptr(),
+          Xs.max_len, Ys.data.data_ptr(), Ys.off.data_ptr(), Ys.len.data_ptr(), Ys.max_len,
+          Xs.d, pc.data_ptr(), P, kind, pval, int(difference), gm.data_ptr(),
+          m_off.data_ptr(), None if gx is None else gx.data_ptr(),
+          None if gy is None else gy.data_ptr(),
+          gparam.data_ptr() if ctx.needs_input_grad[2] else None, stream))
     return gx, gy, gparam, None, None, None, None, None, None
-
-
-def _pde_chain(gm, m_off, pairs, Xs, Ys, la, lb, kind, pval, s_rbf, difference,
-               gx, gy, gparam):
-  """dL/dM per pair -> dL/dG (ΔΔ^T) -> static-kernel chain rule."""
-  ia = pairs[:, 0].long()
-  ib = pairs[:, 1].long()
-  sh = 1 if difference else 0
-  uniform = bool((la == la[0]).all()) and bool((lb == lb[0]).all())
-  groups = ([(torch.arange(pairs.shape[0], device=gm.device), int(la[0]), int(lb[0]))]
-            if uniform else
-            [(torch.tensor([p], device=gm.device), int(la[p]), int(lb[p]))
-             for p in range(pairs.shape[0])])
-  for idx, A, B in groups:
-    swapped = B > A
-    nr, nc = (max(A, B) - sh, min(A, B) - sh)
-    if nr <= 0 or nc <= 0:
-      continue
-    if uniform:
-      gM = gm[:idx.numel() * nr * nc].view(-1, nr, nc)
-    else:
-      o = int(m_off[idx[0]])
-      gM = gm[o:o + nr * nc].view(1, nr, nc)
-    if swapped:
-      gM = gM.transpose(1, 2)
-    if difference:
-      Pd = torch.nn.functional.pad(gM, (1, 1, 1, 1))
-      gG = Pd[:, :-1, :-1] - Pd[:, :-1, 1:] - Pd[:, 1:, :-1] + Pd[:, 1:, 1:]
-    else:
-      gG = gM
-    gG = gG.contiguous()
-    rows_a = Xs.off[ia[idx]][:, None] + torch.arange(A, device=gm.device)[None]
-    rows_b = Ys.off[ib[idx]][:, None] + torch.arange(B, device=gm.device)[None]
-    xa = Xs.data[rows_a]
-    yb = Ys.data[rows_b]
-    dot = torch.bmm(xa, yb.transpose(1, 2).contiguous())
-    if kind == _lib.STATIC_RBF:
-      D2 = (xa * xa).sum(-1)[:, :, None] + (yb * yb).sum(-1)[:, None, :] - 2 * dot
-      G = torch.exp(-D2 * s_rbf)
-      W = gG * G
-      if 2.0 * pval * pval > 1e-12:
-        gparam += (W.double() * D2.double()).sum() / pval ** 3
-    else:
-      W = gG
-      gparam += (W.double() * dot.double()).sum()
-    W = W.contiguous()
-    w_off = torch.arange(idx.numel(), device=gm.device, dtype=torch.int64) * (A * B)
-    _contract_hip(W, w_off, pairs[idx].contiguous(), Xs, Ys, kind, pval, s_rbf, gx, gy)
 
 
 def signature_pde(X: _ops.SeqSet, Y: _ops.SeqSet, pairs: torch.Tensor, kind: int, param,

@@ -118,6 +118,13 @@ struct ContractArgs {
   void* gx;
   void* gy;
   int group;                  // consecutive pairs per workgroup (set by the launcher)
+  // signature-PDE chain rule (mode 1: with difference, 2: without): omega is
+  // dL/dM per pair in the adjoint's orientation and omega is built from it
+  int mode;
+  int kind;
+  double gsc;                 // RBF: log2(e) / max(2 sigma^2, 1e-12)
+  double gp_scale;            // factor of the accumulated dL/dparam
+  double* gparam;             // may be null
 };
 
 // Adjoint of the PDE sweep that rebuilds U from the saved edges.

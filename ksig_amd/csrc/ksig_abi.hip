@@ -477,6 +477,59 @@ int ksig_signature_levels_bwd(int dtype, const void* x, const int64_t* x_off,
   return 0;
 }
 
+int ksig_signature_pde_grad_contract(int dtype, const void* x, const int64_t* x_off,
+                                     const int32_t* x_len, int max_len_x, const void* y,
+                                     const int64_t* y_off, const int32_t* y_len,
+                                     int max_len_y, int d, const int32_t* pairs,
+                                     int64_t n_pairs, int static_kind, double static_param,
+                                     int difference, const void* grad_m,
+                                     const int64_t* m_off, void* grad_x, void* grad_y,
+                                     double* grad_param, void* stream) {
+  if (dtype != KSIG_F32 && dtype != KSIG_F64) return fail(KSIG_EINVAL, "bad dtype %d", dtype);
+  if (n_pairs < 0 || n_pairs > 0x7fffffffLL) return fail(KSIG_EINVAL, "bad n_pairs");
+  if (d < 1 || d > 32) return fail(KSIG_EUNSUPPORTED, "d = %d outside [1, 32]", d);
+  if (!(static_param > 0.0))
+    return fail(KSIG_EINVAL, "static kernel parameter must be positive (got %g)", static_param);
+  if (static_kind != ksig::kRBF && static_kind != ksig::kLinear)
+    return fail(KSIG_EINVAL, "unknown static kernel kind %d", static_kind);
+  if (n_pairs == 0 || (!grad_x && !grad_y && !grad_param)) return 0;
+  if (!x || !x_off || !x_len || !y || !y_off || !y_len || !pairs || !grad_m || !m_off)
+    return fail(KSIG_EINVAL, "null device pointer");
+  if ((int64_t)max_len_x * max_len_y >= (int64_t(1) << 31))
+    return fail(KSIG_EINVAL, "a pair's dL/dM block must stay below 2^31 entries");
+  ksig::ContractArgs a{};
+  a.x = {x, x_off, x_len};
+  a.y = {y, y_off, y_len};
+  a.d = d;
+  a.pairs = pairs;
+  a.n_pairs = n_pairs;
+  a.omega = grad_m;
+  a.omega_off = m_off;
+  a.gx = grad_x;
+  a.gy = grad_y;
+  a.mode = difference ? 1 : 2;
+  a.kind = static_kind;
+  a.gparam = grad_param;
+  if (static_kind == ksig::kRBF) {
+    const double den = 2.0 * static_param * static_param;
+    const double s = 1.0 / (den > 1e-12 ? den : 1e-12);
+    a.c_cross = 2.0 * s;
+    a.c_self = -2.0 * s;
+    a.gsc = 1.4426950408889634 * s;
+    // d/dsigma of exp(-|x-y|^2 / (2 sigma^2)) = G |x-y|^2 / sigma^3 (0 once clamped)
+    a.gp_scale = den > 1e-12 ? 1.0 / (static_param * static_param * static_param) : 0.0;
+  } else {
+    a.c_cross = static_param;
+    a.c_self = 0.0;
+    a.gsc = 0.0;
+    a.gp_scale = 1.0;
+  }
+  hipError_t e = ksig::grad_contract_launch(a, dtype == KSIG_F64, max_len_x, max_len_y,
+                                            device_cus(), (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "pde grad_contract launch");
+  return 0;
+}
+
 int ksig_static_grad_contract(int dtype, const void* x, const int64_t* x_off,
                               const int32_t* x_len, int max_len_x, const void* y,
                               const int64_t* y_off, const int32_t* y_len, int max_len_y,
