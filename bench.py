@@ -507,7 +507,6 @@ def bench_c4(args):
   triu = _ops.triu_pairs(n, dev)
   full = _ops.dense_pairs(n, n, dev)
   ti, tj = triu[:, 0].long(), triu[:, 1].long()
-  fi, fj = full[:, 0].long(), full[:, 1].long()
   diag_pos = torch.nonzero(ti == tj).flatten()
   off_x = torch.arange(n, device=dev, dtype=torch.int64) * L
   len_x = torch.full((n,), L, device=dev, dtype=torch.int32)
@@ -517,7 +516,14 @@ def bench_c4(args):
     kyy = (lev_yy / (robust_sqrt(dy[ti]) * robust_sqrt(dy[tj]))).mean(-1)
     syy = 2.0 * kyy[ti != tj].sum() / (n * (n - 1))
   offmask = (ti != tj).to(torch.float64)  # masks, not boolean indexing: capturable
-  rdy_f = robust_sqrt(dy)[fj]
+  # The normalised MMD^2 terms as per-level bilinear forms u_m^T K_m v_m over
+  # dense level matrices (u = 1 / sqrt(diag K_XX), v = 1 / sqrt(diag K_YY)):
+  # their backward is a gather and batched products, where per-pair
+  # normalisation through index_select would need an atomic index_add_ of
+  # every pair into n diagonal entries.
+  tri_flat = ti * n + tj
+  v_y = (1.0 / robust_sqrt(dy)).t().contiguous().unsqueeze(-1)  # [levels+1, n, 1]
+  nl1 = n_levels + 1
 
   def loss_fn(fwd_only=False):
     Xg = torch.cumsum(scale * Z, 1)
@@ -531,12 +537,15 @@ def bench_c4(args):
                                          xdata=xd, ydata=xd)
       lev_xy = autograd.signature_levels(Xs, Ys, full, n_levels, _lib.STATIC_RBF, sigma,
                                          xdata=xd, ydata=Ys.data)
-    # index_select (backward: atomic index_add) rather than advanced indexing
-    rdx = robust_sqrt(torch.index_select(lev_xx, 0, diag_pos))
-    kxx = (lev_xx / (torch.index_select(rdx, 0, ti) * torch.index_select(rdx, 0, tj))).mean(-1)
-    kxy = (lev_xy / (torch.index_select(rdx, 0, fi) * rdy_f)).mean(-1)
-    sxx = 2.0 * flat_sum(kxx * offmask) / (n * (n - 1))
-    return sxx + syy - 2.0 * flat_sum(kxy) / kxy.numel()
+    u = (1.0 / robust_sqrt(torch.index_select(lev_xx, 0, diag_pos))).t().contiguous()
+    dxx = torch.zeros((n * n, nl1), device=dev, dtype=torch.float64).index_put(
+        (tri_flat,), lev_xx * offmask[:, None])
+    kxx_m = dxx.view(n, n, nl1).permute(2, 0, 1).contiguous()     # [levels+1, n, n]
+    kxy_m = lev_xy.view(n, n, nl1).permute(2, 0, 1).contiguous()
+    ut = u.unsqueeze(1)                                              # [levels+1, 1, n]
+    sxx = torch.bmm(torch.bmm(ut, kxx_m), u.unsqueeze(-1)).sum() * (2.0 / (n * (n - 1) * nl1))
+    sxy = torch.bmm(torch.bmm(ut, kxy_m), v_y).sum() * (1.0 / (n * n * nl1))
+    return sxx + syy - 2.0 * sxy
 
   def train_step():
     scale.grad = None
