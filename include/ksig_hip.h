@@ -266,7 +266,8 @@ int ksig_signature_pde_fwd_edges(int dtype, const void* x, const int64_t* x_off,
  * dL/dK, writes dL/dM for every original ΔΔ cell, [Nr][Nc] in kernel
  * orientation at element offset m_off[p] of grad_m (dtype).  The solution U is
  * rebuilt in the reverse sweep from the edges; the static kernel is
- * recomputed.
+ * recomputed.  workspace: device scratch of at least
+ * ksig_signature_pde_bwd_workspace(...) bytes, 16-byte aligned.
  */
 int ksig_signature_pde_bwd_edges(int dtype, const void* x, const int64_t* x_off,
                                  const int32_t* x_len, int max_len_x, const void* y,
@@ -275,7 +276,16 @@ int ksig_signature_pde_bwd_edges(int dtype, const void* x, const int64_t* x_off,
                                  int64_t n_pairs, int static_kind, double static_param,
                                  int dyadic_order, int difference, const double* grad_out,
                                  const void* edges, const int64_t* e_off, void* grad_m,
-                                 const int64_t* m_off, void* stream);
+                                 const int64_t* m_off, void* workspace,
+                                 size_t workspace_bytes, void* stream);
+
+/*
+ * Workspace bytes of ksig_signature_pde_bwd_edges for n_pairs pairs of at most
+ * max_len_x x max_len_y points: one region per pair for the adjoint's band
+ * hand-over state (contents are scratch).  No reference counterpart.
+ */
+size_t ksig_signature_pde_bwd_workspace(int dtype, int max_len_x, int max_len_y,
+                                        int dyadic_order, int difference, int64_t n_pairs);
 
 /*
  * Signature-PDE kernel on caller-supplied matrices M [n_mats, lx, ly].

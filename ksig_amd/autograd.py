@@ -275,12 +275,15 @@ class _SigPDE(torch.autograd.Function):
       if P > 1:
         m_off[1:] = torch.cumsum(msz[c0:c1], 0)[:-1]
       gm = torch.zeros(max(int(msz[c0:c1].sum()), 1), device=dev, dtype=dt)
+      ws_bytes = lib.ksig_signature_pde_bwd_workspace(
+          _lib.dtype_code(dt), Xs.max_len, Ys.max_len, int(dyadic), int(difference), P)
+      ws = torch.empty(max(ws_bytes // 4, 1), device=dev, dtype=torch.float32)
       _lib.check(lib.ksig_signature_pde_bwd_edges(
           _lib.dtype_code(dt), Xs.data.data_ptr(), Xs.off.data_ptr(), Xs.len.data_ptr(),
           Xs.max_len, Ys.data.data_ptr(), Ys.off.data_ptr(), Ys.len.data_ptr(), Ys.max_len,
           Xs.d, pc.data_ptr(), P, kind, pval, int(dyadic), int(difference),
           gout[c0:c1].data_ptr(), edges.data_ptr(), e_off[c0:c1].data_ptr(), gm.data_ptr(),
-          m_off.data_ptr(), stream))
+          m_off.data_ptr(), ws.data_ptr(), ws_bytes, stream))
       _lib.check(lib.ksig_signature_pde_grad_contract(
           _lib.dtype_code(dt), Xs.data.data_ptr(), Xs.off.data_ptr(), Xs.len.data_ptr(),
           Xs.max_len, Ys.data.data_ptr(), Ys.off.data_ptr(), Ys.len.data_ptr(), Ys.max_len,

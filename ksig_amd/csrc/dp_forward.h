@@ -141,6 +141,8 @@ struct PdeBwdEdgeArgs {
   const int64_t* e_off;
   void* gm;              // [Nr][Nc] at m_off[p]: dL/dM of every original cell
   const int64_t* m_off;
+  void* fifo;            // band FIFO workspace: [n_pairs][fifo_cols] 16-byte-quad slots
+  int fifo_cols;
 };
 
 int dp_flow_count(int nlev, int order);
@@ -172,7 +174,9 @@ hipError_t dp_stream_launch(const StreamEntry& e, const DpStreamArgs& a, size_t 
                             hipStream_t s);
 int dp_stream_occupancy(const StreamEntry& e, size_t lds);
 hipError_t pde_forward_launch(const PdeFwdArgs& a, int dtype, size_t lds, hipStream_t s);
-size_t pde_bwd_edges_lds(int dtype, int d, int ly, int pc);
+size_t pde_bwd_edges_lds(int dtype, int d, int ly, int ncr, int nrr);
+int pde_bwd_fifo_cols(int ncr);
+size_t pde_bwd_fifo_bytes(int dtype, int ncr);
 hipError_t pde_bwd_edges_launch(const PdeBwdEdgeArgs& a, int dtype, size_t lds, hipStream_t s);
 hipError_t grad_contract_launch(const ContractArgs& a, bool f64, int max_len_x, int max_len_y,
                                 int n_cus, hipStream_t s);

@@ -658,6 +658,26 @@ int ksig_signature_pde_fwd_edges(int dtype, const void* x, const int64_t* x_off,
   return pde_common(dtype, a, cols < 1 ? 1 : cols, dmax_for(d), stream);
 }
 
+// Refined columns (shorter side) and rows (longer side) of the PDE grid.
+static void pde_refined(int max_len_x, int max_len_y, int dyadic_order, int difference,
+                        long* ncr, long* nrr) {
+  const int lo = max_len_x < max_len_y ? max_len_x : max_len_y;
+  const int hi = max_len_x < max_len_y ? max_len_y : max_len_x;
+  long nc = difference ? lo - 1 : lo, nr = difference ? hi - 1 : hi;
+  if (nc < 1) nc = 1;
+  if (nr < 1) nr = 1;
+  *ncr = nc << dyadic_order;
+  *nrr = nr << dyadic_order;
+}
+
+size_t ksig_signature_pde_bwd_workspace(int dtype, int max_len_x, int max_len_y,
+                                        int dyadic_order, int difference, int64_t n_pairs) {
+  if (n_pairs <= 0 || dyadic_order < 0 || dyadic_order > 6) return 0;
+  long ncr, nrr;
+  pde_refined(max_len_x, max_len_y, dyadic_order, difference, &ncr, &nrr);
+  return (size_t)n_pairs * ksig::pde_bwd_fifo_bytes(dtype, (int)ncr);
+}
+
 int ksig_signature_pde_bwd_edges(int dtype, const void* x, const int64_t* x_off,
                                  const int32_t* x_len, int max_len_x, const void* y,
                                  const int64_t* y_off, const int32_t* y_len,
@@ -665,7 +685,8 @@ int ksig_signature_pde_bwd_edges(int dtype, const void* x, const int64_t* x_off,
                                  int64_t n_pairs, int static_kind, double static_param,
                                  int dyadic_order, int difference, const double* grad_out,
                                  const void* edges, const int64_t* e_off, void* grad_m,
-                                 const int64_t* m_off, void* stream) {
+                                 const int64_t* m_off, void* workspace,
+                                 size_t workspace_bytes, void* stream) {
   int rc = check_common(dtype, x, y, d, pairs, n_pairs, (double*)grad_m);
   if (rc) return rc;
   ksig::PdeBwdEdgeArgs a{};
@@ -692,13 +713,18 @@ int ksig_signature_pde_bwd_edges(int dtype, const void* x, const int64_t* x_off,
   a.gm = grad_m;
   a.m_off = m_off;
   const int ly = max_len_x < max_len_y ? max_len_x : max_len_y;  // the shorter side
-  int nc = difference ? ly - 1 : ly;
-  if (nc < 1) nc = 1;
-  long pc = (long)nc << dyadic_order;
-  if (pc < ksig::kWave) pc = ksig::kWave;
-  const size_t lds = ksig::pde_bwd_edges_lds(dtype, d, ly < 1 ? 1 : ly, (int)pc);
+  long ncr, nrr;
+  pde_refined(max_len_x, max_len_y, dyadic_order, difference, &ncr, &nrr);
+  const size_t lds = ksig::pde_bwd_edges_lds(dtype, d, ly < 1 ? 1 : ly, (int)ncr, (int)nrr);
   if (lds > max_lds_bytes())
     return fail(KSIG_ELDS, "adjoint LDS state of %zu bytes exceeds the device limit", lds);
+  const size_t need = ksig_signature_pde_bwd_workspace(dtype, max_len_x, max_len_y,
+                                                       dyadic_order, difference, n_pairs);
+  if (!workspace || workspace_bytes < need || ((uintptr_t)workspace & 15))
+    return fail(KSIG_EINVAL, "workspace of %zu bytes (16-byte aligned) required, got %zu", need,
+                workspace ? workspace_bytes : (size_t)0);
+  a.fifo = workspace;
+  a.fifo_cols = ksig::pde_bwd_fifo_cols((int)ncr);
   hipError_t e = ksig::pde_bwd_edges_launch(a, dtype, lds, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "pde_bwd_edges launch");
   return 0;

@@ -23,7 +23,9 @@
 // r = Nrr-1-(64b+k) and walks the refined columns right-to-left one step behind
 // lane k-1; the lane of row r passes λ, μ, aλ, bλ, M'(r,c) and U(r,c-1) up to
 // the lane of row r-1 (DPP wave_shr:1; lane 63 -> lane 0 of the next band
-// through an LDS FIFO).  The lane of row r-1 completes cell (r, c): it knows
+// through a FIFO in a global-memory workspace, read two steps ahead).  The
+// saved edges and the next band's x rows are staged in LDS, so no lane waits
+// on global memory at a band change.  The lane of row r-1 completes cell (r, c): it knows
 // λ(r,c), U(r,c-1) from below and U(r-1,c), U(r-1,c-1) from its own rebuild.
 // A virtual row r = -1 (U = 1) completes row 0.  The 2^λ x 2^λ refined cells
 // of an original cell are summed along each lane's 2^λ columns and across the
@@ -37,16 +39,24 @@
 
 namespace ksig {
 
+// Band hand-over slot in the global workspace: λ, μ, aλ, bλ, M', block
+// partial and U(r, c-1), padded to whole 16-byte quads.
 template <typename T>
 struct PdeSlot {
-  T v[6];    // λ, μ, aλ, bλ, M', block partial
-  double u;  // U(r, c-1)
+  T v[6];
+  double u;
 };
+template <typename T>
+constexpr int pde_slot_quads() { return (int)((sizeof(PdeSlot<T>) + 15) / 16); }
+
+__host__ __device__ constexpr int pde_bwd_pc(int ncr) { return ncr > 80 ? ncr : 80; }
 
 template <typename T, int DMAX>
 __global__ void __launch_bounds__(kWave)
 pde_bwd_edges_kernel(const PdeBwdEdgeArgs a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
+  constexpr int XS = 2 * DMAX + 2;  // staged x rows of the next band: top, bottom, norms
+  constexpr int SQ = pde_slot_quads<T>();
 
   const int lane = threadIdx.x;
   const int64_t pr = blockIdx.x;
@@ -69,7 +79,7 @@ pde_bwd_edges_kernel(const PdeBwdEdgeArgs a) {
   const int lam = a.dyadic;
   const int F = 1 << lam;
   const int Nrr = Nr << lam, Ncr = Nc << lam;
-  const int Pc = Ncr > kWave ? Ncr : kWave;
+  const int Pc = pde_bwd_pc(Ncr);
   const int nrows = Nrr + 1;  // + virtual row above the grid
   const int nb = (nrows + kWave - 1) / kWave;
   const T inv4 = T(1) / T(F * F);
@@ -77,12 +87,15 @@ pde_bwd_edges_kernel(const PdeBwdEdgeArgs a) {
   const T* edge_row = reinterpret_cast<const T*>(a.edges) + a.e_off[pr];  // [Ncr]
   const T* edge_col = edge_row + Ncr;                                       // [Nrr]
   T* gm = reinterpret_cast<T*>(a.gm) + a.m_off[pr];                         // [Nr][Nc]
+  float4* fifo = reinterpret_cast<float4*>(a.fifo) + (size_t)pr * a.fifo_cols * SQ;
 
-  // LDS: y rows [Ly][DMAX] (scaled), y norms [Ly], FIFO [Pc] slots
+  // LDS: y rows [Ly][DMAX] (scaled), y norms [Ly], the saved last refined row
+  // [Ncr] and column [Nrr], lane-private staging of the next band's x rows
   T* ylds = reinterpret_cast<T*>(smem_raw);
   T* ynrm = ylds + Ly * DMAX;
-  PdeSlot<T>* fifo = reinterpret_cast<PdeSlot<T>*>(
-      smem_raw + (((size_t)Ly * (DMAX + 1) * sizeof(T) + 15) & ~(size_t)15));
+  T* erow = ynrm + Ly;
+  T* ecol = erow + Ncr;
+  T* xst = ecol + Nrr;
   const T xscale = T(a.xscale), yscale = T(a.yscale);
   for (int j = lane; j < Ly; j += kWave) {
     T n2 = T(0);
@@ -94,44 +107,76 @@ pde_bwd_edges_kernel(const PdeBwdEdgeArgs a) {
     }
     ynrm[j] = n2;
   }
-  for (int s = lane; s < Pc; s += kWave) {
-    PdeSlot<T> z;
-#pragma unroll
-    for (int k = 0; k < 6; ++k) z.v[k] = T(0);
-    z.u = 1.0;
-    fifo[s] = z;
-  }
-  __syncthreads();
+  for (int c = lane; c < Ncr; c += kWave) erow[c] = edge_row[c];
+  for (int c = lane; c < Nrr; c += kWave) ecol[c] = edge_col[c];
 
-  auto gval = [&](const T* xv, T xn, int yc) -> T {
+  auto gval = [&](const T* xv, T xn, int yc) __attribute__((always_inline)) -> T {
     const T* yr = ylds + yc * DMAX;
     T dot = T(0);
 #pragma unroll
     for (int k = 0; k < DMAX; ++k) dot += xv[k] * yr[k];
     return kind == kRBF ? fast_exp2(-(xn + ynrm[yc] - T(2) * dot)) : dot;
   };
-  auto load_x = [&](T* xv, T& xn, int p) {
-    T n2 = T(0);
+  // stage the x rows of refined row rr into this lane's slot
+  auto stage = [&](int rr) __attribute__((always_inline)) {
+    T* dst = xst + lane * XS;
+    if (rr >= 0) {
+      const int i = rr >> lam;
+      const int i2 = diff ? i + 1 : i;
+      T n0 = T(0), n1 = T(0);
 #pragma unroll
-    for (int k = 0; k < DMAX; ++k) {
-      const T v = k < d ? xs[(int64_t)p * d + k] * xscale : T(0);
-      xv[k] = v;
-      n2 += v * v;
+      for (int k = 0; k < DMAX; ++k) {
+        const T v0 = k < d ? xs[(int64_t)i * d + k] * xscale : T(0);
+        const T v1 = k < d ? xs[(int64_t)i2 * d + k] * xscale : T(0);
+        dst[k] = v0;
+        dst[DMAX + k] = v1;
+        n0 += v0 * v0;
+        n1 += v1 * v1;
+      }
+      dst[2 * DMAX] = n0;
+      dst[2 * DMAX + 1] = n1;
     }
-    xn = n2;
   };
-
   int band = 0;
   int r = Nrr - 1 - lane;  // refined row of this lane (flipped order)
   T xt[DMAX], xb[DMAX], xtn = T(0), xbn = T(0);
-  auto enter_row = [&]() {
-    if (r >= 0) {
-      const int i = r >> lam;
-      load_x(xt, xtn, i);
-      load_x(xb, xbn, diff ? i + 1 : i);
+  auto take = [&]() __attribute__((always_inline)) {
+    const T* src = xst + lane * XS;
+#pragma unroll
+    for (int k = 0; k < DMAX; ++k) {
+      xt[k] = src[k];
+      xb[k] = src[DMAX + k];
     }
+    xtn = src[2 * DMAX];
+    xbn = src[2 * DMAX + 1];
   };
-  enter_row();
+  stage(r);
+  __syncthreads();
+  take();
+
+  // Lane 0's input of step st, read two steps ahead by every lane (a
+  // broadcast; only lane 0 keeps it): in bands 1 .. nb-1 the slot lane 63
+  // wrote Pc - 65 >= 15 steps earlier; otherwise (band 0: the row below the
+  // grid; steps where lane 0 is idle) a zero adjoint with U = 1, so that no
+  // unwritten workspace is read.
+  auto fread = [&](int st, T* v, double& u) __attribute__((always_inline)) {
+    const int b0 = st / Pc;
+    const int f0 = st - b0 * Pc;
+    const bool live = b0 >= 1 && b0 < nb && f0 < Ncr;
+    const int fs = live ? f0 : 0;
+    float4 q[SQ];
+#pragma unroll
+    for (int k = 0; k < SQ; ++k) q[k] = fifo[fs * SQ + k];
+    PdeSlot<T> sl;
+    __builtin_memcpy(&sl, q, sizeof(PdeSlot<T>));
+#pragma unroll
+    for (int k = 0; k < 6; ++k) v[k] = live ? sl.v[k] : T(0);
+    u = live ? sl.u : 1.0;
+  };
+  T fv0[6], fv1[6];
+  double fu0, fu1;
+  fread(0, fv0, fu0);
+  fread(1, fv1, fu1);
 
   T fl[6];
 #pragma unroll
@@ -142,23 +187,32 @@ pde_bwd_edges_kernel(const PdeBwdEdgeArgs a) {
   double u_self = 1.0, u_bp = 1.0;
   int s = -lane;
   const int total = nb * Pc + kWave - 1;
-  for (int step = 0; step < total; ++step) {
-    const bool active = s >= 0 && s < Ncr && band < nb && r >= -1;
-    PdeSlot<T> fin;
+  // band b+1 is staged at step b*Pc + 63: every lane has taken band b by then
+  int next_stage = kWave - 1;
+  int stage_band = 1;
+
+  auto step_fn = [&](int step, T* fv, double& fu) __attribute__((always_inline)) {
+    T fin[6];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) fin.v[k] = T(0);
-    fin.u = 1.0;
-    if (lane == 0 && active) fin = fifo[s];
+    for (int k = 0; k < 6; ++k) fin[k] = fv[k];
+    const double fin_u = fu;
+    fread(step + 2, fv, fu);
+    if (step == next_stage) {  // wave-uniform
+      if (stage_band < nb) stage(r - kWave);
+      ++stage_band;
+      next_stage += Pc;
+    }
+    const bool active = s >= 0 && s < Ncr && band < nb && r >= -1;
     T in[6];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) in[k] = shift_down(fl[k], fin.v[k]);
-    const double in_u = shift_down(flu, fin.u);
+    for (int k = 0; k < 6; ++k) in[k] = shift_down(fl[k], fin[k]);
+    const double in_u = shift_down(flu, fin_u);
 
     const int c = Ncr - 1 - s;
     const bool cell = active && r >= 0;
     if (active && s == 0) {  // right edge: saved last refined column
-      u_self = r >= 0 ? (double)edge_col[r] : 1.0;
-      u_bp = (r + 1 <= Nrr - 1) ? (double)edge_col[r + 1] : 1.0;
+      u_self = r >= 0 ? (double)ecol[r] : 1.0;
+      u_bp = (r + 1 <= Nrr - 1) ? (double)ecol[r + 1] : 1.0;
     }
     // this lane's cell value M'(r, c), recomputed once per original column
     if (cell && (c & (F - 1)) == F - 1) {
@@ -194,7 +248,7 @@ pde_bwd_edges_kernel(const PdeBwdEdgeArgs a) {
     double u_new = 1.0;
     if (active && c >= 1) {
       if (r == Nrr - 1) {
-        u_new = (double)edge_row[c - 1];
+        u_new = (double)erow[c - 1];
       } else if (r >= 0) {
         const double mb = (double)in[4];
         const double bq = mb * mb * (1.0 / 12.0);
@@ -217,16 +271,23 @@ pde_bwd_edges_kernel(const PdeBwdEdgeArgs a) {
     if (below && (c & (F - 1)) == 0 && (rb & (F - 1)) == 0)
       gm[(int64_t)(rb >> lam) * Nc + (c >> lam)] = part * inv4;
 
-    T of[6] = {lv, mu, cell ? aa * lv : T(0), cell ? bb * lv : T(0), cell ? M : T(0), part};
-#pragma unroll
-    for (int k = 0; k < 6; ++k) fl[k] = of[k];
+    fl[0] = lv;
+    fl[1] = mu;
+    fl[2] = cell ? aa * lv : T(0);
+    fl[3] = cell ? bb * lv : T(0);
+    fl[4] = cell ? M : T(0);
+    fl[5] = part;
     flu = u_new;
     if (lane == kWave - 1 && active) {
       PdeSlot<T> o;
 #pragma unroll
-      for (int k = 0; k < 6; ++k) o.v[k] = of[k];
+      for (int k = 0; k < 6; ++k) o.v[k] = fl[k];
       o.u = u_new;
-      fifo[s] = o;
+      float4 q[SQ];
+      __builtin_memset(q, 0, sizeof(q));
+      __builtin_memcpy(q, &o, sizeof(PdeSlot<T>));
+#pragma unroll
+      for (int k = 0; k < SQ; ++k) fifo[s * SQ + k] = q[k];
     }
     lam_r = lv;
     a_r = cell ? aa : T(0);
@@ -239,11 +300,18 @@ pde_bwd_edges_kernel(const PdeBwdEdgeArgs a) {
       s = 0;
       ++band;
       r -= kWave;
-      enter_row();
+      take();
       lam_r = a_r = beta_prev = gacc = M = T(0);
       u_self = u_bp = 1.0;
     }
+  };
+
+  int step = 0;
+  for (; step + 2 <= total; step += 2) {
+    step_fn(step, fv0, fu0);
+    step_fn(step + 1, fv1, fu1);
   }
+  if (step < total) step_fn(step, fv0, fu0);
 }
 
 template <typename T, int DMAX>
@@ -265,12 +333,18 @@ static hipError_t dispatch_bwd_edges(const PdeBwdEdgeArgs& a, size_t lds, hipStr
   return launch_bwd_edges<T, 32>(a, lds, s);
 }
 
-size_t pde_bwd_edges_lds(int dtype, int d, int ly, int pc) {
+size_t pde_bwd_edges_lds(int dtype, int d, int ly, int ncr, int nrr) {
   const size_t esz = dtype == KSIG_F32 ? 4 : 8;
   const int dmax = d <= 8 ? 8 : (d <= 16 ? 16 : 32);
-  const size_t head = (((size_t)ly * (dmax + 1) * esz) + 15) & ~(size_t)15;
-  const size_t slot = dtype == KSIG_F32 ? sizeof(PdeSlot<float>) : sizeof(PdeSlot<double>);
-  return head + (size_t)pc * slot;
+  return ((size_t)ly * (dmax + 1) + (size_t)ncr + (size_t)nrr +
+          (size_t)kWave * (2 * dmax + 2)) * esz;
+}
+
+int pde_bwd_fifo_cols(int ncr) { return pde_bwd_pc(ncr); }
+
+size_t pde_bwd_fifo_bytes(int dtype, int ncr) {
+  const int quads = dtype == KSIG_F32 ? pde_slot_quads<float>() : pde_slot_quads<double>();
+  return (size_t)pde_bwd_pc(ncr) * quads * 16;
 }
 
 hipError_t pde_bwd_edges_launch(const PdeBwdEdgeArgs& a, int dtype, size_t lds, hipStream_t s) {
