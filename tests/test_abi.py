@@ -53,3 +53,34 @@ def test_host_only_entry_points():
   assert rc == 1001 and b"dtype" in lib.ksig_last_error()
   rc = lib.ksig_signature_levels_gram(0, None, 0, 4, 4, 40, 1, 1, None, None)
   assert rc == 1002
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libksig_hip.so not built")
+def test_workspace_queries_and_adjoint_argument_checks():
+  """Host-side sizing of the adjoint workspaces and the validation that runs
+  before any device call (dummy non-null pointers are never dereferenced)."""
+  from ksig_amd import _lib
+  lib = ctypes.CDLL(LIB)
+  for name, (res, args) in _lib.SIGNATURES.items():
+    fn = getattr(lib, name)
+    fn.restype, fn.argtypes = res, args
+  # DP adjoint, float32: per pair a [Pc][16] float FIFO, Pc = 128 + 1 -> 144
+  assert lib.ksig_signature_levels_bwd_workspace(_lib.F32, 8, 6, 128, 128, 10) == 10 * 144 * 16 * 4
+  assert lib.ksig_signature_levels_bwd_workspace(_lib.F32, 8, 6, 20, 30, 1) == 80 * 16 * 4
+  assert lib.ksig_signature_levels_bwd_workspace(_lib.F64, 8, 6, 128, 128, 10) == 0
+  # PDE adjoint: per pair (refined columns, at least 80) 32-byte (float) slots
+  assert lib.ksig_signature_pde_bwd_workspace(_lib.F32, 256, 256, 1, 1, 3) == 3 * 510 * 32
+  assert lib.ksig_signature_pde_bwd_workspace(_lib.F64, 40, 10, 0, 0, 2) == 2 * 80 * 64
+  p = ctypes.c_void_p(16)  # never dereferenced: validation fails first
+  rc = lib.ksig_signature_levels_bwd(
+      _lib.F32, p, p, p, 128, p, p, p, 128, 8, p, 4, 0, 1.0, 6, p, p, p, p, p, p, None, 0, None)
+  assert rc == 1001 and b"workspace" in lib.ksig_last_error()
+  rc = lib.ksig_static_grad_contract(_lib.F32, p, p, p, 4, p, p, p, 4, 40, p, 1, p, p, 1.0, 0.0,
+                                     p, p, None)
+  assert rc == 1002
+  rc = lib.ksig_signature_pde_grad_contract(_lib.F32, p, p, p, 4, p, p, p, 4, 3, p, 1, 7, 1.0,
+                                            1, p, p, p, p, None, None)
+  assert rc == 1001
+  rc = lib.ksig_signature_pde_grad_contract(_lib.F32, p, p, p, 4, p, p, p, 4, 3, p, 1, 0, -1.0,
+                                            1, p, p, p, p, None, None)
+  assert rc == 1001
