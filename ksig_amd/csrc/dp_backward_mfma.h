@@ -300,10 +300,12 @@ dp_backward_mfma_kernel(const DpBwdArgs a) {
   // set, then refills it for the step after next.  Every lane reads lane 0's
   // FIFO slot (a broadcast; only lane 0 keeps it), so the read needs no lane
   // mask.  Lane 63 wrote that slot Pc - 65 >= 15 steps earlier.
-  auto prefetch = [&](int st, T& arg_o, T* fin_o) __attribute__((always_inline)) {  // inputs of step st (uniform)
+  int fslot = 0;  // lane 0's FIFO slot of the next prefetched step: (step + 2) mod Pc
+  auto prefetch = [&](T& arg_o, T* fin_o) __attribute__((always_inline)) {  // uniform
     arg_o = ring[rslot * RS + lane];
     rslot = rslot + 1 == W ? 0 : rslot + 1;
-    const int f0 = st - (st / Pc) * Pc;  // lane 0's column slot at step st
+    const int f0 = fslot;
+    fslot = fslot + 1 == Pc ? 0 : fslot + 1;
 #pragma unroll
     for (int k = 0; k < NFG; k += 4) {
       const float4 t = *reinterpret_cast<const float4*>(fifo + f0 * NFG + k);
@@ -319,7 +321,7 @@ dp_backward_mfma_kernel(const DpBwdArgs a) {
 #pragma unroll
     for (int k = 0; k < NF; ++k) fin[k] = fin_io[k];
     if (((step + 2) & 15) == 0) produce_chunk((step + 2) >> 4);
-    prefetch(step + 2, arg_io, fin_io);
+    prefetch(arg_io, fin_io);
     if (step == next_stage) {  // wave-uniform
       if (stage_band < nb) stage(stage_band * kWave + lane);
       ++stage_band;
@@ -421,8 +423,8 @@ dp_backward_mfma_kernel(const DpBwdArgs a) {
 
   T arg0, arg1, fin0[NFG], fin1[NFG];
   produce_chunk(0);
-  prefetch(0, arg0, fin0);
-  prefetch(1, arg1, fin1);
+  prefetch(arg0, fin0);
+  prefetch(arg1, fin1);
   int step = 0;
   for (; step + 2 <= total; step += 2) {
     step_fn(step, arg0, fin0);

@@ -159,9 +159,13 @@ pde_bwd_edges_kernel(const PdeBwdEdgeArgs a) {
   // wrote Pc - 65 >= 15 steps earlier; otherwise (band 0: the row below the
   // grid; steps where lane 0 is idle) a zero adjoint with U = 1, so that no
   // unwritten workspace is read.
-  auto fread = [&](int st, T* v, double& u) __attribute__((always_inline)) {
-    const int b0 = st / Pc;
-    const int f0 = st - b0 * Pc;
+  int fslot = 0, fband = 0;  // lane 0's slot and band at the next prefetched step
+  auto fread = [&](T* v, double& u) __attribute__((always_inline)) {
+    const int b0 = fband, f0 = fslot;
+    if (++fslot == Pc) {
+      fslot = 0;
+      ++fband;
+    }
     const bool live = b0 >= 1 && b0 < nb && f0 < Ncr;
     const int fs = live ? f0 : 0;
     float4 q[SQ];
@@ -175,8 +179,8 @@ pde_bwd_edges_kernel(const PdeBwdEdgeArgs a) {
   };
   T fv0[6], fv1[6];
   double fu0, fu1;
-  fread(0, fv0, fu0);
-  fread(1, fv1, fu1);
+  fread(fv0, fu0);
+  fread(fv1, fu1);
 
   T fl[6];
 #pragma unroll
@@ -196,7 +200,7 @@ pde_bwd_edges_kernel(const PdeBwdEdgeArgs a) {
 #pragma unroll
     for (int k = 0; k < 6; ++k) fin[k] = fv[k];
     const double fin_u = fu;
-    fread(step + 2, fv, fu);
+    fread(fv, fu);
     if (step == next_stage) {  // wave-uniform
       if (stage_band < nb) stage(r - kWave);
       ++stage_band;
