@@ -316,12 +316,7 @@ dp_backward_mfma_kernel(const DpBwdArgs a) {
     }
   };
   auto step_fn = [&](int step, T& arg_io, T* fin_io) __attribute__((always_inline)) {
-    const T arg = arg_io;
-    T fin[NF];
-#pragma unroll
-    for (int k = 0; k < NF; ++k) fin[k] = fin_io[k];
     if (((step + 2) & 15) == 0) produce_chunk((step + 2) >> 4);
-    prefetch(arg_io, fin_io);
     if (step == next_stage) {  // wave-uniform
       if (stage_band < nb) stage(stage_band * kWave + lane);
       ++stage_band;
@@ -330,12 +325,15 @@ dp_backward_mfma_kernel(const DpBwdArgs a) {
     const bool active = s >= 0 && band < nb;
     T in[NF];
 #pragma unroll
-    for (int k = 0; k < NF; ++k) in[k] = shift_down(fl[k], fin[k]);
+    for (int k = 0; k < NF; ++k) in[k] = shift_down(fl[k], fin_io[k]);
+    const T arg = arg_io;
+    T g = rbf ? fast_exp2(-arg) : arg;
+    // this step's inputs are consumed: refill the set for the step after next
+    prefetch(arg_io, fin_io);
 
     const int j = Nc - s;
     const int i = Nr - 1 - fr;
     const bool gvalid = active && i >= 0 && j >= 0;
-    T g = rbf ? fast_exp2(-arg) : arg;
     g = gvalid ? g : T(0);
     const T dv = gvalid ? arg : T(0);
     const bool prim = s == 0;

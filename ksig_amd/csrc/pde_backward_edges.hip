@@ -196,11 +196,6 @@ pde_bwd_edges_kernel(const PdeBwdEdgeArgs a) {
   int stage_band = 1;
 
   auto step_fn = [&](int step, T* fv, double& fu) __attribute__((always_inline)) {
-    T fin[6];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) fin[k] = fv[k];
-    const double fin_u = fu;
-    fread(fv, fu);
     if (step == next_stage) {  // wave-uniform
       if (stage_band < nb) stage(r - kWave);
       ++stage_band;
@@ -209,8 +204,10 @@ pde_bwd_edges_kernel(const PdeBwdEdgeArgs a) {
     const bool active = s >= 0 && s < Ncr && band < nb && r >= -1;
     T in[6];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) in[k] = shift_down(fl[k], fin[k]);
-    const double in_u = shift_down(flu, fin_u);
+    for (int k = 0; k < 6; ++k) in[k] = shift_down(fl[k], fv[k]);
+    const double in_u = shift_down(flu, fu);
+    // this step's inputs are consumed: refill the set for the step after next
+    fread(fv, fu);
 
     const int c = Ncr - 1 - s;
     const bool cell = active && r >= 0;
