@@ -123,8 +123,9 @@ int ksig_signature_levels_batched(const void* x, const int64_t* x_off, const int
  * also writes the boundary state the adjoint needs.  For pair p, save holds
  * (n_levels-1) * (len_x + len_y - 2) values of dtype at element offset
  * save_off[p] (row-end prefixes of the recursion and its final column sums).
- * Same reference interface as ksig_signature_levels; the state itself has no
- * reference counterpart (the reference has no gradients).
+ * save may be NULL: levels only.  Same reference interface as
+ * ksig_signature_levels; the state itself has no reference counterpart (the
+ * reference has no gradients).
  */
 int ksig_signature_levels_fwd_save(int dtype, const void* x, const int64_t* x_off,
                                    const int32_t* x_len, int max_len_x, const void* y,
@@ -132,7 +133,19 @@ int ksig_signature_levels_fwd_save(int dtype, const void* x, const int64_t* x_of
                                    int max_len_y, int d, const int32_t* pairs,
                                    int64_t n_pairs, int static_kind, double static_param,
                                    int n_levels, double* out, void* save,
-                                   const int64_t* save_off, void* stream);
+                                   const int64_t* save_off, void* workspace,
+                                   size_t workspace_bytes, void* stream);
+
+/*
+ * Workspace bytes with which ksig_signature_levels_fwd_save takes its fastest
+ * kernel for the shape (the static Gram on MFMA: float32, 8 < d <= 32); 0
+ * when no workspace is used.  The workspace is optional: NULL or smaller
+ * selects a kernel without one.  Contents are scratch.  No reference
+ * counterpart.
+ */
+size_t ksig_signature_levels_fwd_save_workspace(int dtype, int d, int n_levels,
+                                                int max_len_x, int max_len_y,
+                                                int64_t n_pairs);
 
 /*
  * Adjoint of ksig_signature_levels_fwd_save (no reference counterpart: the

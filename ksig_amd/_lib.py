@@ -39,7 +39,9 @@ SIGNATURES = {
         _c_dbl, _c_int, _c_int, _vp, _vp, _c_int, _c_int, _vp, _c_size, _vp, _vp]),
     "ksig_signature_levels_fwd_save": (_c_int, [
         _c_int, _vp, _vp, _vp, _c_int, _vp, _vp, _vp, _c_int, _c_int, _vp,
-        _c_i64, _c_int, _c_dbl, _c_int, _vp, _vp, _vp, _vp]),
+        _c_i64, _c_int, _c_dbl, _c_int, _vp, _vp, _vp, _vp, _c_size, _vp]),
+    "ksig_signature_levels_fwd_save_workspace": (_c_size, [
+        _c_int, _c_int, _c_int, _c_int, _c_int, _c_i64]),
     "ksig_signature_levels_bwd": (_c_int, [
         _c_int, _vp, _vp, _vp, _c_int, _vp, _vp, _vp, _c_int, _c_int, _vp,
         _c_i64, _c_int, _c_dbl, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_size, _vp]),

@@ -127,6 +127,21 @@ def signature_levels(X: SeqSet, Y: SeqSet, pairs: torch.Tensor, n_levels: int,
   out = torch.empty((n_pairs, n_levels + 1), device=X.data.device, dtype=torch.float64)
   if n_pairs == 0:
     return out
+  eff_order = n_levels if (order <= 0 or order >= n_levels) else order
+  if (X.data.dtype == torch.float32 and difference and gram_mode in (-1, 0) and eff_order == 1
+      and kind in (_lib.STATIC_RBF, _lib.STATIC_LINEAR)):
+    # first order, float32: the training forward's entry without the state
+    # picks the MFMA-Gram forward when it is faster (d > 16) given a workspace
+    ws_bytes = lib.ksig_signature_levels_fwd_save_workspace(
+        _lib.F32, X.d, n_levels, X.max_len, Y.max_len, n_pairs)
+    if ws_bytes > 0:
+      ws = torch.empty(ws_bytes // 4, device=X.data.device, dtype=torch.float32)
+      _lib.check(lib.ksig_signature_levels_fwd_save(
+          _lib.F32, X.data.data_ptr(), X.off.data_ptr(), X.len.data_ptr(), X.max_len,
+          Y.data.data_ptr(), Y.off.data_ptr(), Y.len.data_ptr(), Y.max_len, X.d,
+          pairs.data_ptr(), n_pairs, kind, float(param), n_levels, out.data_ptr(), None, None,
+          ws.data_ptr(), ws_bytes, _lib.stream_handle(X.data.device)))
+      return out
   _lib.check(lib.ksig_signature_levels(
       _lib.dtype_code(X.data.dtype), X.data.data_ptr(), X.off.data_ptr(),
       X.len.data_ptr(), X.max_len, Y.data.data_ptr(), Y.off.data_ptr(),

@@ -77,12 +77,15 @@ class _SigLevels(torch.autograd.Function):
       total = int(per.sum().item()) if n_pairs else 0
     save = torch.empty(max(total, 1), device=dev, dtype=Xs.data.dtype)
     if n_pairs:
+      ws_bytes = lib.ksig_signature_levels_fwd_save_workspace(
+          _lib.dtype_code(Xs.data.dtype), Xs.d, n_levels, Xs.max_len, Ys.max_len, n_pairs)
+      ws = torch.empty(max(ws_bytes // 4, 1), device=dev, dtype=torch.float32)
       _lib.check(lib.ksig_signature_levels_fwd_save(
           _lib.dtype_code(Xs.data.dtype), Xs.data.data_ptr(), Xs.off.data_ptr(),
           Xs.len.data_ptr(), Xs.max_len, Ys.data.data_ptr(), Ys.off.data_ptr(),
           Ys.len.data_ptr(), Ys.max_len, Xs.d, pairs.data_ptr(), n_pairs, kind, pval,
           n_levels, out.data_ptr(), save.data_ptr(), save_off.data_ptr(),
-          _lib.stream_handle(dev)))
+          ws.data_ptr(), ws_bytes, _lib.stream_handle(dev)))
     ctx.save_for_backward(xdata, ydata, pairs, save, save_off, la, lb)
     ctx.meta = (xs_meta, ys_meta, n_levels, kind, pval, uni)
     return out

@@ -40,6 +40,10 @@ struct DpFwdArgs {
   // row/column orientation (rows = the longer sequence).
   void* save;
   const int64_t* save_off;
+  // MFMA-Gram forward (dp_fwd_mfma.h): band FIFO workspace, one region of
+  // fifo_cols 16-byte-quad slots per pair
+  void* fifo;
+  int fifo_cols;
 };
 
 // Row-stream forward (dp_stream.h): each wave takes a contiguous slice of the
@@ -156,6 +160,12 @@ hipError_t dp_backward_launch(const DpBwdArgs& a, int dtype, size_t lds, hipStre
 bool dp_backward_supported(int n_levels);
 // float32 adjoint with the static Gram on MFMA (dp_backward_mfma.h)
 size_t dp_backward_mfma_lds(int d, int cols, int n_levels);
+// float32 first-order forward with the static Gram on MFMA (dp_fwd_mfma.h)
+size_t dp_fwd_mfma_lds();
+int dp_fwd_mfma_fifo_cols(int cols);
+size_t dp_fwd_mfma_fifo_bytes(int cols, int n_levels);
+bool dp_fwd_mfma_supported(int d, int n_levels);
+hipError_t dp_fwd_mfma_launch(const DpFwdArgs& a, bool save, size_t lds, hipStream_t s);
 // FIFO slots per pair and workspace bytes per pair of the MFMA variant
 int dp_backward_mfma_fifo_cols(int cols);
 size_t dp_backward_mfma_fifo_bytes(int cols, int n_levels);

@@ -371,13 +371,23 @@ int ksig_signature_levels_batched(const void* x, const int64_t* x_off, const int
                        wave_beg ? 0 : max_slice, workspace, stream);
 }
 
+size_t ksig_signature_levels_fwd_save_workspace(int dtype, int d, int n_levels,
+                                                int max_len_x, int max_len_y,
+                                                int64_t n_pairs) {
+  if (dtype != KSIG_F32 || n_pairs <= 0 || !ksig::dp_fwd_mfma_supported(d, n_levels)) return 0;
+  int cols = max_len_x < max_len_y ? max_len_x : max_len_y;
+  if (cols < 1) cols = 1;
+  return (size_t)n_pairs * ksig::dp_fwd_mfma_fifo_bytes(cols, n_levels);
+}
+
 int ksig_signature_levels_fwd_save(int dtype, const void* x, const int64_t* x_off,
                                    const int32_t* x_len, int max_len_x, const void* y,
                                    const int64_t* y_off, const int32_t* y_len,
                                    int max_len_y, int d, const int32_t* pairs,
                                    int64_t n_pairs, int static_kind, double static_param,
                                    int n_levels, double* out, void* save,
-                                   const int64_t* save_off, void* stream) {
+                                   const int64_t* save_off, void* workspace,
+                                   size_t workspace_bytes, void* stream) {
   int rc = check_common(dtype, x, y, d, pairs, n_pairs, out);
   if (rc) return rc;
   if (n_levels < 1) return fail(KSIG_EINVAL, "n_levels must be positive");
@@ -387,7 +397,7 @@ int ksig_signature_levels_fwd_save(int dtype, const void* x, const int64_t* x_of
   rc = static_scales(static_kind, static_param, &a.xscale, &a.yscale);
   if (rc) return rc;
   if (n_pairs == 0) return 0;
-  if (n_levels > 1 && (!save || !save_off)) return fail(KSIG_EINVAL, "null save buffer");
+  if (n_levels > 1 && save && !save_off) return fail(KSIG_EINVAL, "null save offsets");
   a.x = {x, x_off, x_len};
   a.y = {y, y_off, y_len};
   a.d = d;
@@ -400,11 +410,24 @@ int ksig_signature_levels_fwd_save(int dtype, const void* x, const int64_t* x_of
   a.out = out;
   a.save = n_levels > 1 ? save : nullptr;
   a.save_off = save_off;
-  if (dtype == KSIG_F32 && a.save != nullptr) {
-    rc = stream_levels(a, max_len_x, max_len_y, static_param, stream);
-    if (rc != -1) return rc;  // -1: no row-stream training kernel fits, use the per-pair one
-  }
   const int cols = max_len_x < max_len_y ? max_len_x : max_len_y;
+  // float32, 8 < d <= 32, with the workspace: the MFMA-Gram forward (the row
+  // stream's per-lane y rows bound it by the vector-memory path there)
+  const size_t need = ksig_signature_levels_fwd_save_workspace(dtype, d, n_levels, max_len_x,
+                                                               max_len_y, n_pairs);
+  if (need > 0 && workspace && workspace_bytes >= need && ((uintptr_t)workspace & 15) == 0 &&
+      (int64_t)max_len_x * max_len_y < (int64_t(1) << 31)) {
+    a.fifo = workspace;
+    a.fifo_cols = ksig::dp_fwd_mfma_fifo_cols(cols < 1 ? 1 : cols);
+    const size_t lds = ksig::dp_fwd_mfma_lds();
+    hipError_t e = ksig::dp_fwd_mfma_launch(a, a.save != nullptr, lds, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "dp_fwd_mfma launch");
+    return 0;
+  }
+  if (dtype == KSIG_F32) {
+    rc = stream_levels(a, max_len_x, max_len_y, static_param, stream);
+    if (rc != -1) return rc;  // -1: no row-stream kernel fits, use the per-pair one
+  }
   return dp_levels_common(dtype, a, cols < 1 ? 1 : cols, dmax_for(d), stream);
 }
 

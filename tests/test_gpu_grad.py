@@ -134,7 +134,7 @@ def test_mmd_gradients_float32_vs_float64():
   assert _rel(grads[torch.float32][1], scale.grad) < 2e-4
 
 
-@pytest.mark.parametrize("d", [8, 16, 32])
+@pytest.mark.parametrize("d", [8, 13, 16, 32])
 def test_training_forward_boundary_state_layout(d):
   """The float32 training forward (row stream for d in {8, 16, 32}) writes the
   same adjoint boundary state -- row-end prefixes then final column sums, in
@@ -155,26 +155,33 @@ def test_training_forward_boundary_state_layout(d):
   off = np.concatenate([[0], np.cumsum(per)[:-1]]).astype(np.int64)
   lib = _lib.load()
   saves, outs = {}, {}
-  for dt in (torch.float64, torch.float32):
+  # float64 per-pair kernel; float32 without a workspace (row stream) and, for
+  # 8 < d <= 32, with one (the MFMA-Gram forward)
+  for key in ("f64", "f32", "f32_ws"):
+    dt = torch.float64 if key == "f64" else torch.float32
     X = _ops.SeqSet.ragged([torch.tensor(s, device=dev(), dtype=dt) for s in xs])
     Y = _ops.SeqSet.ragged([torch.tensor(s, device=dev(), dtype=dt) for s in ys])
     pairs = to_dev(pr)
     save = torch.full((int(per.sum()),), float("nan"), device=dev(), dtype=dt)
     so = to_dev(off)
     out = torch.empty((len(pr), n_levels + 1), device=dev(), dtype=torch.float64)
+    wsb = (lib.ksig_signature_levels_fwd_save_workspace(
+        _lib.dtype_code(dt), d, n_levels, X.max_len, Y.max_len, len(pr)) if key == "f32_ws" else 0)
+    ws = torch.empty(max(wsb // 4, 1), device=dev(), dtype=torch.float32)
     _lib.check(lib.ksig_signature_levels_fwd_save(
         _lib.dtype_code(dt), X.data.data_ptr(), X.off.data_ptr(), X.len.data_ptr(), X.max_len,
         Y.data.data_ptr(), Y.off.data_ptr(), Y.len.data_ptr(), Y.max_len, d,
         pairs.data_ptr(), len(pr), _lib.STATIC_RBF, math.sqrt(d), n_levels, out.data_ptr(),
-        save.data_ptr(), so.data_ptr(), _lib.stream_handle(dev())))
-    saves[dt] = save.double().cpu().numpy()
-    outs[dt] = out.cpu().numpy()
-  assert not np.isnan(saves[torch.float32]).any()
-  for p in range(len(pr)):
-    a, b = off[p], off[p] + per[p]
-    ref = saves[torch.float64][a:b]
-    got = saves[torch.float32][a:b]
-    if b > a:
-      assert np.max(np.abs(got - ref)) <= 2e-5 * max(np.max(np.abs(ref)), 1e-30), p
-  assert np.max(np.abs(outs[torch.float32] - outs[torch.float64])) <= 1e-5 * np.max(
-      np.abs(outs[torch.float64]))
+        save.data_ptr(), so.data_ptr(), ws.data_ptr() if wsb else None, wsb,
+        _lib.stream_handle(dev())))
+    saves[key] = save.double().cpu().numpy()
+    outs[key] = out.cpu().numpy()
+  for key in ("f32", "f32_ws"):
+    assert not np.isnan(saves[key]).any()
+    for p in range(len(pr)):
+      a, b = off[p], off[p] + per[p]
+      ref = saves["f64"][a:b]
+      got = saves[key][a:b]
+      if b > a:
+        assert np.max(np.abs(got - ref)) <= 2e-5 * max(np.max(np.abs(ref)), 1e-30), (key, p)
+    assert np.max(np.abs(outs[key] - outs["f64"])) <= 1e-5 * np.max(np.abs(outs["f64"]))
