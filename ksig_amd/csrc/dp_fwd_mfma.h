@@ -19,7 +19,8 @@
 // Chunk k is produced at step 16k - 2 and every ring / FIFO input is read two
 // steps before its use into one of two alternating register sets; lane 0's
 // FIFO slot is read by every lane (a broadcast) and was written by lane 63
-// Pc - 65 >= 15 steps earlier.  LDS holds only the ring and 64 x norms.
+// Pc - 65 >= 15 steps earlier.  LDS holds only the ring (20 KB: 8 waves per
+// CU); the x norms of the producing band stay in registers.
 //
 // Beyond the sequence ends the inputs are replicated (rows clamped to Lx-1,
 // columns to Ly-1), which makes ΔΔ exactly zero there, so the sweep needs no
@@ -38,6 +39,8 @@
 
 namespace ksig {
 
+constexpr int kFwdRingS = 64;  // ring slot stride (floats), XOR-swizzled rows
+
 __host__ __device__ constexpr int fwd_mfma_pc(int nc) {
   // priming column + Nc columns, whole chunks, at least 80
   return (((nc + 1 > 80 ? nc + 1 : 80) + 15) / 16) * 16;
@@ -51,11 +54,13 @@ dp_fwd_mfma_kernel(const DpFwdArgs a) {
   constexpr int NFG = (NF + 3) & ~3;       // FIFO slot stride (16-byte quads)
   constexpr int NPS = NLEV > 1 ? NLEV - 1 : 1;
   constexpr int KB = DMAX / 4;
-  constexpr int W = kBwdRingW, RS = kBwdRingS;
+  constexpr int W = kBwdRingW, RS = kFwdRingS;
   constexpr bool rbf = KIND == kRBF;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* ring = reinterpret_cast<T*>(smem_raw);  // [W][RS] static-kernel values
-  T* xnrm = ring + W * RS;                    // [64] x norms of the producing band
+  // [W][64] static-kernel values; row r of slot s sits at position r ^ (s & 31)
+  // of the slot, so both the producer's column writes and the sweep's row
+  // reads spread over the 32 banks
+  T* ring = reinterpret_cast<T*>(smem_raw);
 
   const int lane = threadIdx.x;
   const int c16 = lane & 15, grp = lane >> 4;
@@ -107,6 +112,7 @@ dp_fwd_mfma_kernel(const DpFwdArgs a) {
     yc = yc < Ly - 1 ? yc : Ly - 1;
     bwd_mfma_feats<DMAX, VEC>(B, ys + (int64_t)yc * d, d, grp);
   };
+  T xnq[4][4];  // scaled squared norms of rows 16t + 4grp + q of the band
   auto x_norms = [&]() __attribute__((always_inline)) {
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
@@ -114,8 +120,9 @@ dp_fwd_mfma_kernel(const DpFwdArgs a) {
 #pragma unroll
       for (int e = 0; e < KB; ++e) s2 += A[t][e] * A[t][e];
       s2 += __shfl_xor(s2, 16, kWave);
-      s2 += __shfl_xor(s2, 32, kWave);
-      if (grp == 0) xnrm[16 * t + c16] = s2 * xs2;
+      s2 += __shfl_xor(s2, 32, kWave);  // every lane: the norm of row 16t + c16
+#pragma unroll
+      for (int q = 0; q < 4; ++q) xnq[t][q] = __shfl(s2, 4 * grp + q, kWave) * xs2;
     }
   };
   auto produce = [&](int k) __attribute__((always_inline)) {
@@ -140,11 +147,11 @@ dp_fwd_mfma_kernel(const DpFwdArgs a) {
       for (int q = 0; q < 4; ++q) {
         const int row = 16 * t + 4 * grp + q;
         const T dot = acc[t][q] * xyscale;
-        const T G = rbf ? fast_exp2(-(xnrm[row] + yn - T(2) * dot)) : dot;
+        const T G = rbf ? fast_exp2(-(xnq[t][q] + yn - T(2) * dot)) : dot;
         int slot = base + 16 * t + q;  // (16k + c16 + row) mod W
         slot = slot >= W ? slot - W : slot;
         slot = slot >= W ? slot - W : slot;
-        ring[slot * RS + row] = G;
+        ring[slot * RS + (row ^ (slot & 31))] = G;
       }
   };
   bool norms_due = false;
@@ -199,7 +206,7 @@ dp_fwd_mfma_kernel(const DpFwdArgs a) {
   load_A(0);
   load_B(0);
   x_norms();
-  __syncthreads();  // FIFO stores retired, x norms visible
+  __syncthreads();  // FIFO stores retired
 
   T fl[NF];
 #pragma unroll
@@ -220,7 +227,7 @@ dp_fwd_mfma_kernel(const DpFwdArgs a) {
   int fslot = 0;   // lane 0's FIFO slot of the next prefetched step
   int s63 = Pc - (kWave - 1);  // lane 63's FIFO slot ((t - 63) mod Pc)
   auto prefetch = [&](T& g_o, T* fin_o) __attribute__((always_inline)) {  // uniform
-    g_o = ring[rslot * RS + lane];
+    g_o = ring[rslot * RS + (lane ^ (rslot & 31))];
     rslot = rslot + 1 == W ? 0 : rslot + 1;
 #pragma unroll
     for (int k = 0; k < NFG / 4; ++k) {
