@@ -423,8 +423,8 @@ hipError_t dp_backward_mfma_d32(const DpBwdArgs& a, size_t lds, hipStream_t s);
 size_t dp_backward_mfma_lds(int d, int cols, int n_levels) {
   (void)d;
   (void)cols;
-  const int nl1 = n_levels > 1 ? n_levels - 1 : 1;
-  return ((size_t)kBwdRingW * kBwdRingS + kWave + (size_t)kWave * nl1) * 4;
+  (void)n_levels;
+  return (size_t)kBwdRingW * kRingS * 4;  // the ring only: 20 KB, 8 waves per CU
 }
 
 int dp_backward_mfma_fifo_cols(int cols) { return bwd_mfma_pc(cols - 1); }  // Nc = cols - 1

@@ -39,7 +39,12 @@ namespace ksig {
 typedef float bmf4 __attribute__((ext_vector_type(4)));
 
 constexpr int kBwdRingW = 80;  // ring columns: 64-lane skew + one 16-column chunk
-constexpr int kBwdRingS = 65;  // ring column stride (floats)
+constexpr int kRingS = 64;     // ring slot stride (floats)
+
+// Position of row r within ring slot `slot`: rows are XOR-swizzled by the slot
+// so that the producer's writes (16 slots x 4 rows per instruction) and the
+// sweep's reads (one slot, 64 rows) both spread over the LDS banks.
+__device__ __forceinline__ int ring_pos(int slot, int r) { return slot * kRingS + (r ^ (slot & 31)); }
 
 // Features grp*KB .. grp*KB+KB-1 of one row into v (KB = DMAX / 4): vector
 // loads when VEC (d == DMAX, 16-byte aligned rows), else clamped scalar loads
@@ -82,11 +87,9 @@ dp_backward_mfma_kernel(const DpBwdArgs a) {
   constexpr int NF = 3 + 2 * (NLEV - 1);    // [arg, D', gM, C_1.., Dq_1..]
   constexpr int NFG = (NF + 3) & ~3;        // FIFO slot stride (16-byte quads)
   constexpr int KB = DMAX / 4;
-  constexpr int W = kBwdRingW, RS = kBwdRingS;
+  constexpr int W = kBwdRingW;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* ring = reinterpret_cast<T*>(smem_raw);  // [W][RS] static-kernel arguments
-  T* xnrm = ring + W * RS;                    // [64] x norms of the producing band
-  T* relst = xnrm + kWave;                    // [64][NL1] lane-private row-end prefixes
+  T* ring = reinterpret_cast<T*>(smem_raw);  // [W][64] static-kernel arguments (20 KB)
 
   const int lane = threadIdx.x;
   const int c16 = lane & 15, grp = lane >> 4;
@@ -143,15 +146,17 @@ dp_backward_mfma_kernel(const DpBwdArgs a) {
     j = j < 0 ? 0 : j;  // columns past the grid: masked in the sweep
     bwd_mfma_feats<DMAX, VEC>(B, ys + (int64_t)j * d, d, grp);
   };
-  auto x_norms = [&]() __attribute__((always_inline)) {  // after load_A: scaled squared norms of the band's rows
+  T xnq[4][4];  // scaled squared norms of the band's rows 16t + 4grp + q
+  auto x_norms = [&]() __attribute__((always_inline)) {  // after load_A
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       T s2 = T(0);
 #pragma unroll
       for (int e = 0; e < KB; ++e) s2 += A[t][e] * A[t][e];
       s2 += __shfl_xor(s2, 16, kWave);
-      s2 += __shfl_xor(s2, 32, kWave);
-      if (grp == 0) xnrm[16 * t + c16] = s2 * xs2;
+      s2 += __shfl_xor(s2, 32, kWave);  // every lane: the norm of row 16t + c16
+#pragma unroll
+      for (int q = 0; q < 4; ++q) xnq[t][q] = __shfl(s2, 4 * grp + q, kWave) * xs2;
     }
   };
   auto produce = [&](int k) __attribute__((always_inline)) {
@@ -176,25 +181,29 @@ dp_backward_mfma_kernel(const DpBwdArgs a) {
       for (int q = 0; q < 4; ++q) {
         const int row = 16 * t + 4 * grp + q;
         const T dot = acc[t][q] * xyscale;
-        const T arg = rbf ? xnrm[row] + yn - T(2) * dot : dot;
+        const T arg = rbf ? xnq[t][q] + yn - T(2) * dot : dot;
         int slot = base + 16 * t + q;  // (16k + c16 + row) mod W
         slot = slot >= W ? slot - W : slot;
         slot = slot >= W ? slot - W : slot;
-        ring[slot * RS + row] = arg;
+        ring[ring_pos(slot, row)] = arg;
       }
   };
 
-  // Stage the row-end prefixes of flipped row fr into this lane's slot.
+  // Row-end prefixes of flipped row fr, loaded a band ahead into relnext.
+  T rel[NL1], relnext[NL1];
   auto stage = [&](int fr) __attribute__((always_inline)) {
     const int i = Nr - 1 - fr;
     const bool ok = i >= 0 && fr < nrows;
+    const int ic = ok ? i : 0;
 #pragma unroll
-    for (int m = 0; m < NC; ++m) relst[lane * NL1 + m] = ok ? rowend[m * Nr + i] : T(0);
+    for (int m = 0; m < NL1; ++m) {
+      const T v = m < NC ? rowend[m * Nr + ic] : T(0);
+      relnext[m] = ok ? v : T(0);
+    }
   };
-  T rel[NL1];
   auto take = [&]() __attribute__((always_inline)) {
 #pragma unroll
-    for (int m = 0; m < NL1; ++m) rel[m] = relst[lane * NL1 + m];
+    for (int m = 0; m < NL1; ++m) rel[m] = relnext[m];
   };
 
   // FIFO: the row below the grid (G row Nr = Lx-1) with the forward's final
@@ -302,7 +311,7 @@ dp_backward_mfma_kernel(const DpBwdArgs a) {
   // mask.  Lane 63 wrote that slot Pc - 65 >= 15 steps earlier.
   int fslot = 0;  // lane 0's FIFO slot of the next prefetched step: (step + 2) mod Pc
   auto prefetch = [&](T& arg_o, T* fin_o) __attribute__((always_inline)) {  // uniform
-    arg_o = ring[rslot * RS + lane];
+    arg_o = ring[ring_pos(rslot, lane)];
     rslot = rslot + 1 == W ? 0 : rslot + 1;
     const int f0 = fslot;
     fslot = fslot + 1 == Pc ? 0 : fslot + 1;

@@ -39,8 +39,6 @@
 
 namespace ksig {
 
-constexpr int kFwdRingS = 64;  // ring slot stride (floats), XOR-swizzled rows
-
 __host__ __device__ constexpr int fwd_mfma_pc(int nc) {
   // priming column + Nc columns, whole chunks, at least 80
   return (((nc + 1 > 80 ? nc + 1 : 80) + 15) / 16) * 16;
@@ -54,13 +52,10 @@ dp_fwd_mfma_kernel(const DpFwdArgs a) {
   constexpr int NFG = (NF + 3) & ~3;       // FIFO slot stride (16-byte quads)
   constexpr int NPS = NLEV > 1 ? NLEV - 1 : 1;
   constexpr int KB = DMAX / 4;
-  constexpr int W = kBwdRingW, RS = kFwdRingS;
+  constexpr int W = kBwdRingW;
   constexpr bool rbf = KIND == kRBF;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  // [W][64] static-kernel values; row r of slot s sits at position r ^ (s & 31)
-  // of the slot, so both the producer's column writes and the sweep's row
-  // reads spread over the 32 banks
-  T* ring = reinterpret_cast<T*>(smem_raw);
+  T* ring = reinterpret_cast<T*>(smem_raw);  // [W][64] static-kernel values (ring_pos)
 
   const int lane = threadIdx.x;
   const int c16 = lane & 15, grp = lane >> 4;
@@ -151,12 +146,12 @@ dp_fwd_mfma_kernel(const DpFwdArgs a) {
         int slot = base + 16 * t + q;  // (16k + c16 + row) mod W
         slot = slot >= W ? slot - W : slot;
         slot = slot >= W ? slot - W : slot;
-        ring[slot * RS + (row ^ (slot & 31))] = G;
+        ring[ring_pos(slot, row)] = G;
       }
   };
   bool norms_due = false;
   // the ring starts at zero: lanes read it before their first column
-  for (int e = lane; e < W * RS; e += kWave) ring[e] = T(0);
+  for (int e = lane; e < W * kRingS; e += kWave) ring[e] = T(0);
   auto produce_chunk = [&](int k) __attribute__((always_inline)) {  // wave-uniform
     if (16 * k >= ncols) return;
     if (norms_due) {  // first chunk of a band: A was loaded 16 steps ago
@@ -227,7 +222,7 @@ dp_fwd_mfma_kernel(const DpFwdArgs a) {
   int fslot = 0;   // lane 0's FIFO slot of the next prefetched step
   int s63 = Pc - (kWave - 1);  // lane 63's FIFO slot ((t - 63) mod Pc)
   auto prefetch = [&](T& g_o, T* fin_o) __attribute__((always_inline)) {  // uniform
-    g_o = ring[rslot * RS + (lane ^ (rslot & 31))];
+    g_o = ring[ring_pos(rslot, lane)];
     rslot = rslot + 1 == W ? 0 : rslot + 1;
 #pragma unroll
     for (int k = 0; k < NFG / 4; ++k) {

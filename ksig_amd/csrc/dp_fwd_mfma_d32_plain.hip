@@ -14,7 +14,7 @@ namespace ksig {
 
 hipError_t dp_fwd_mfma_d32_save(const DpFwdArgs& a, size_t lds, hipStream_t s);
 
-size_t dp_fwd_mfma_lds() { return (size_t)kBwdRingW * kFwdRingS * sizeof(float); }
+size_t dp_fwd_mfma_lds() { return (size_t)kBwdRingW * kRingS * sizeof(float); }
 
 int dp_fwd_mfma_fifo_cols(int cols) { return fwd_mfma_pc(cols - 1); }  // Nc = cols - 1
 
