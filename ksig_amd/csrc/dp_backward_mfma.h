@@ -123,7 +123,9 @@ dp_backward_mfma_kernel(const DpBwdArgs a) {
   for (int m = 1; m <= NLEV; ++m) w[m] = T(a.gout[pr * (NLEV + 1) + m]);
   const T* rowend = reinterpret_cast<const T*>(a.save) + a.save_off[pr];  // [NC][Nr]
   const T* colsum = rowend + NC * Nr;                                       // [NC][Nc]
-  T* omega = reinterpret_cast<T*>(a.omega) + a.omega_off[pr];
+  // this pair's omega block [Lx][Ly] (or [Ly][Lx]); below 2^31 bytes
+  const __amdgpu_buffer_rsrc_t omega_rs =
+      raw_rsrc(reinterpret_cast<T*>(a.omega) + a.omega_off[pr], unsigned(Lx * Ly) * 4u);
 
   const T xscale = T(a.xscale), yscale = T(a.yscale);
   const T xyscale = xscale * yscale, xs2 = xscale * xscale, ys2 = yscale * yscale;
@@ -190,7 +192,7 @@ dp_backward_mfma_kernel(const DpBwdArgs a) {
   };
 
   // Row-end prefixes of flipped row fr, loaded a band ahead into relnext.
-  T rel[NL1], relnext[NL1];
+  T relnext[NL1];
   auto stage = [&](int fr) __attribute__((always_inline)) {
     const int i = Nr - 1 - fr;
     const bool ok = i >= 0 && fr < nrows;
@@ -200,10 +202,6 @@ dp_backward_mfma_kernel(const DpBwdArgs a) {
       const T v = m < NC ? rowend[m * Nr + ic] : T(0);
       relnext[m] = ok ? v : T(0);
     }
-  };
-  auto take = [&]() __attribute__((always_inline)) {
-#pragma unroll
-    for (int m = 0; m < NL1; ++m) rel[m] = relnext[m];
   };
 
   // FIFO: the row below the grid (G row Nr = Lx-1) with the forward's final
@@ -257,15 +255,20 @@ dp_backward_mfma_kernel(const DpBwdArgs a) {
   load_A(0);
   load_B(0);
   __syncthreads();
-  take();
   x_norms();
 
   T fl[NF];
 #pragma unroll
   for (int k = 0; k < NF; ++k) fl[k] = T(0);
+  // Row state.  The column flows C and D are zero in the priming column (and
+  // before it), so the priming step leaves P and S as they are: a lane enters
+  // a band with P = its row-end prefixes and S = 0, set at the band change.
   T P[NL1], S[NL1];
 #pragma unroll
-  for (int m = 0; m < NL1; ++m) P[m] = S[m] = T(0);
+  for (int m = 0; m < NL1; ++m) {
+    P[m] = relnext[m];
+    S[m] = T(0);
+  }
   T g_right = T(0), gup_right = T(0), dup_right = T(0), gm_right = T(0), gmup_right = T(0);
   T gsig_row = T(0);
   double gsig = 0.0;
@@ -309,19 +312,23 @@ dp_backward_mfma_kernel(const DpBwdArgs a) {
   // set, then refills it for the step after next.  Every lane reads lane 0's
   // FIFO slot (a broadcast; only lane 0 keeps it), so the read needs no lane
   // mask.  Lane 63 wrote that slot Pc - 65 >= 15 steps earlier.
-  int fslot = 0;  // lane 0's FIFO slot of the next prefetched step: (step + 2) mod Pc
+  // The FIFO through a buffer descriptor: lane 0's slot is a scalar offset,
+  // lane 63's store is the only one in range (the other lanes' are dropped).
+  const __amdgpu_buffer_rsrc_t fifo_rs = raw_rsrc(fifo, unsigned(Pc * NFG) * 4u);
+  constexpr int kSlotB = NFG * 4;  // FIFO slot bytes
+  int fbyte = 0;  // lane 0's FIFO slot of the next prefetched step: ((step + 2) mod Pc) * kSlotB
   auto prefetch = [&](T& arg_o, T* fin_o) __attribute__((always_inline)) {  // uniform
     arg_o = ring[ring_pos(rslot, lane)];
     rslot = rslot + 1 == W ? 0 : rslot + 1;
-    const int f0 = fslot;
-    fslot = fslot + 1 == Pc ? 0 : fslot + 1;
+    const int f0 = fbyte;
+    fbyte = fbyte + kSlotB == Pc * kSlotB ? 0 : fbyte + kSlotB;
 #pragma unroll
     for (int k = 0; k < NFG; k += 4) {
-      const float4 t = *reinterpret_cast<const float4*>(fifo + f0 * NFG + k);
-      fin_o[k] = t.x;
-      if (k + 1 < NF) fin_o[k + 1] = t.y;
-      if (k + 2 < NF) fin_o[k + 2] = t.z;
-      if (k + 3 < NF) fin_o[k + 3] = t.w;
+      const u32x4 t = __builtin_amdgcn_raw_buffer_load_b128(fifo_rs, 0, f0 + 4 * k, 0);
+      fin_o[k] = __uint_as_float(t.x);
+      if (k + 1 < NF) fin_o[k + 1] = __uint_as_float(t.y);
+      if (k + 2 < NF) fin_o[k + 2] = __uint_as_float(t.z);
+      if (k + 3 < NF) fin_o[k + 3] = __uint_as_float(t.w);
     }
   };
   auto step_fn = [&](int step, T& arg_io, T* fin_io) __attribute__((always_inline)) {
@@ -334,11 +341,9 @@ dp_backward_mfma_kernel(const DpBwdArgs a) {
     const bool active = s >= 0 && band < nb;
     T in[NF];
 #pragma unroll
-    for (int k = 0; k < NF; ++k) in[k] = shift_down(fl[k], fin_io[k]);
+    for (int k = 0; k < NF; ++k) in[k] = shift_in(fl[k], fin_io[k], lane == 0);
     const T arg = arg_io;
     T g = rbf ? fast_exp2(-arg) : arg;
-    // this step's inputs are consumed: refill the set for the step after next
-    prefetch(arg_io, fin_io);
 
     const int j = Nc - s;
     const int i = Nr - 1 - fr;
@@ -378,10 +383,9 @@ dp_backward_mfma_kernel(const DpBwdArgs a) {
     const bool fin_ok = active && !prim && i >= -1 && j >= -1;
     const T gG = (gM - gm_right) - (in[2] - gmup_right);
     const T om = rbf ? gG * gup_right : gG;
-    if (fin_ok) {
-      omega[oi_row + oi_col] = om;
-      gsig_row += om * dup_right;
-    }
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(om), omega_rs,
+                                          fin_ok ? (oi_row + oi_col) * 4 : int(kBufOob), 0, 0);
+    gsig_row += (fin_ok ? om : T(0)) * dup_right;
 
     // state
     g_right = g;
@@ -391,8 +395,8 @@ dp_backward_mfma_kernel(const DpBwdArgs a) {
     gmup_right = in[2];
 #pragma unroll
     for (int m = 0; m < NC; ++m) {
-      P[m] = prim ? rel[m] : Pn[m];
-      S[m] = prim ? T(0) : S[m] + in[3 + NC + m];
+      P[m] = Pn[m];
+      S[m] += in[3 + NC + m];
     }
     fl[0] = g;
     fl[1] = dv;
@@ -402,15 +406,16 @@ dp_backward_mfma_kernel(const DpBwdArgs a) {
       fl[3 + m] = Cout[m];
       fl[3 + NC + m] = Dout[m];
     }
-    if (lane == kWave - 1 && active) {
+    {
+      const int fo = lane == kWave - 1 && active ? s * kSlotB : int(kBufOob);
 #pragma unroll
       for (int k = 0; k < NFG; k += 4) {
-        float4 t;
-        t.x = fl[k];
-        t.y = k + 1 < NF ? fl[k + 1] : T(0);
-        t.z = k + 2 < NF ? fl[k + 2] : T(0);
-        t.w = k + 3 < NF ? fl[k + 3] : T(0);
-        *reinterpret_cast<float4*>(fifo + s * NFG + k) = t;
+        u32x4 t;
+        t.x = __float_as_uint(fl[k]);
+        t.y = k + 1 < NF ? __float_as_uint(fl[k + 1]) : 0u;
+        t.z = k + 2 < NF ? __float_as_uint(fl[k + 2]) : 0u;
+        t.w = k + 3 < NF ? __float_as_uint(fl[k + 3]) : 0u;
+        __builtin_amdgcn_raw_buffer_store_b128(t, fifo_rs, fo + 4 * k, 0, 0);
       }
     }
 
@@ -424,8 +429,17 @@ dp_backward_mfma_kernel(const DpBwdArgs a) {
       oi_col = (Nc + 1) * cst;
       gsig += (double)gsig_row;
       gsig_row = T(0);
-      take();
+#pragma unroll
+      for (int m = 0; m < NC; ++m) {
+        P[m] = relnext[m];
+        S[m] = T(0);
+      }
     }
+    // This step's inputs are consumed: refill the set for the step after
+    // next.  Issued last, so that the loads reuse the registers of this
+    // step's inputs instead of copies of them.
+    __builtin_amdgcn_sched_barrier(0);
+    prefetch(arg_io, fin_io);
   };
 
   T arg0, arg1, fin0[NFG], fin1[NFG];

@@ -238,10 +238,8 @@ dp_fwd_mfma_kernel(const DpFwdArgs a) {
     if (((step + 2) & 15) == 0) produce_chunk((step + 2) >> 4);
     T in[NF];
 #pragma unroll
-    for (int k = 0; k < NF; ++k) in[k] = shift_down(fl[k], fin_io[k]);
+    for (int k = 0; k < NF; ++k) in[k] = shift_in(fl[k], fin_io[k], lane == 0);
     const T gnew = g_io;
-    // this step's inputs are consumed: refill the set for the step after next
-    prefetch(g_io, fin_io);
 
     const bool keep = cp != 0;
     T M = (gnew - gleft) - (in[0] - gdiag);
@@ -250,16 +248,11 @@ dp_fwd_mfma_kernel(const DpFwdArgs a) {
     gdiag = in[0];
     T of[NF];
     of[0] = gnew;
-    dp_cell<T, NLEV, 1>(M, in, of, PS, HB, !keep);
+    // The column sums are zero in the priming column (c' = 0), so that step
+    // leaves the row sums as they are: they are cleared when c' wraps.
+    dp_cell<T, NLEV, 1>(M, in, of, PS, HB, false);
 #pragma unroll
     for (int k = 0; k < NF; ++k) fl[k] = of[k];
-    if constexpr (SAVE && NLEV > 1) {
-      // PS holds the row-end prefixes P_a(row, Nc) after column c' = Nc
-      if (cp == Nc && row < Nr && brow < nb && step >= lane) {
-#pragma unroll
-        for (int m = 0; m < NPS; ++m) sv[m * Nr + row] = PS[m];
-      }
-    }
     if (lane == kWave - 1 && step >= kWave - 1) {
 #pragma unroll
       for (int k = 0; k < NFG / 4; ++k) {
@@ -275,11 +268,26 @@ dp_fwd_mfma_kernel(const DpFwdArgs a) {
     if (cp == Pc) {
       cp = 0;
       if (step >= lane) {
+        if constexpr (SAVE && NLEV > 1) {
+          // Row-end prefixes P_a(row, Nc): the padding columns past Nc have
+          // M = 0 and zero column sums, so PS still holds them here.
+          if (row < Nr && brow < nb) {
+#pragma unroll
+            for (int m = 0; m < NPS; ++m) sv[m * Nr + row] = PS[m];
+          }
+        }
         row += kWave;
         ++brow;
       }
+#pragma unroll
+      for (int k = 0; k < NPS; ++k) PS[k] = T(0);
     }
     s63 = s63 + 1 == Pc ? 0 : s63 + 1;
+    // This step's inputs are consumed: refill the set for the step after
+    // next.  Issued last, so that the loads reuse the registers of this
+    // step's inputs instead of copies of them.
+    __builtin_amdgcn_sched_barrier(0);
+    prefetch(g_io, fin_io);
   };
 
   T g0, g1, f0[NFG], f1[NFG];

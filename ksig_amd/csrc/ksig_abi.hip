@@ -481,6 +481,8 @@ int ksig_signature_levels_bwd(int dtype, const void* x, const int64_t* x_off,
   if ((int64_t)max_len_x * max_len_y >= (int64_t(1) << 31))
     return fail(KSIG_EINVAL, "a pair's omega block must stay below 2^31 entries");
   const bool mfma = bwd_uses_mfma(dtype, d);
+  if (mfma && (int64_t)max_len_x * max_len_y >= (int64_t(1) << 29))
+    return fail(KSIG_EINVAL, "a pair's float32 omega block must stay below 2^31 bytes");
   if (mfma) {
     const size_t need = ksig_signature_levels_bwd_workspace(dtype, d, n_levels, max_len_x,
                                                             max_len_y, n_pairs);

@@ -29,6 +29,14 @@ __device__ __forceinline__ float shift_down(float v, float fill) {
       __float_as_int(fill), __float_as_int(v), 0x138, 0xf, 0xf, false));
 }
 
+// Same shift with lane 0's value selected rather than carried in the DPP's
+// `old` operand: `fill` is only read, so a register that a load refills every
+// step (a prefetched FIFO entry) is free again once the step has read it.
+__device__ __forceinline__ float shift_in(float v, float fill, bool lane0) {
+  const float up = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xf, 0xf, true));
+  return lane0 ? fill : up;
+}
+
 __device__ __forceinline__ double shift_down(double v, double fill) {
   const int2 a = *reinterpret_cast<const int2*>(&v);
   const int2 f = *reinterpret_cast<const int2*>(&fill);
@@ -37,6 +45,16 @@ __device__ __forceinline__ double shift_down(double v, double fill) {
   r.y = __builtin_amdgcn_update_dpp(f.y, a.y, 0x138, 0xf, 0xf, false);
   return *reinterpret_cast<const double*>(&r);
 }
+
+// Raw buffer descriptor (word 3 = 0x00020000: 32-bit data, no swizzle or
+// stride) over `bytes` bytes at p.  A lane whose offset is at or past `bytes`
+// reads zero and its store is dropped, so an offset of kBufOob masks a lane's
+// store without a branch.  `p` must be wave-uniform.
+constexpr unsigned kBufOob = 0x80000000u;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t raw_rsrc(const void* p, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
 // Full-wave sum in double (used once per pair for the level sums).
 __device__ __forceinline__ double wave_sum(double v) {
