@@ -446,7 +446,13 @@ dp_backward_mfma_kernel(const DpBwdArgs a) {
   produce_chunk(0);
   prefetch(arg0, fin0);
   prefetch(arg1, fin1);
-  int step = 0;
+  // The first two steps run outside the loop, so that the loop is entered
+  // with the memory operations of a whole step issued after each prefetch,
+  // as on every later iteration; the compiler's wait counts then only wait
+  // for the prefetched loads, not for the stores issued after them.
+  step_fn(0, arg0, fin0);
+  step_fn(1, arg1, fin1);
+  int step = 2;
   for (; step + 2 <= total; step += 2) {
     step_fn(step, arg0, fin0);
     step_fn(step + 1, arg1, fin1);

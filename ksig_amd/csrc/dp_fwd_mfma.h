@@ -219,20 +219,24 @@ dp_fwd_mfma_kernel(const DpFwdArgs a) {
 
   const int total = ncols + kWave - 1;
   int rslot = 0;   // ring slot of the next prefetched step
-  int fslot = 0;   // lane 0's FIFO slot of the next prefetched step
-  int s63 = Pc - (kWave - 1);  // lane 63's FIFO slot ((t - 63) mod Pc)
+  // The FIFO through a buffer descriptor: slots are scalar offsets, and lane
+  // 63's store is the only one in range (the other lanes' are dropped).
+  const __amdgpu_buffer_rsrc_t fifo_rs = raw_rsrc(fifo, unsigned(Pc * NFG) * 4u);
+  constexpr int kSlotB = NFG * 4;  // FIFO slot bytes
+  int fbyte = 0;  // lane 0's FIFO slot of the next prefetched step (bytes)
+  int s63b = (Pc - (kWave - 1)) * kSlotB;  // lane 63's FIFO slot ((t - 63) mod Pc, bytes)
   auto prefetch = [&](T& g_o, T* fin_o) __attribute__((always_inline)) {  // uniform
     g_o = ring[ring_pos(rslot, lane)];
     rslot = rslot + 1 == W ? 0 : rslot + 1;
 #pragma unroll
     for (int k = 0; k < NFG / 4; ++k) {
-      const float4 q = fifo[fslot * (NFG / 4) + k];
-      fin_o[4 * k] = q.x;
-      if (4 * k + 1 < NF) fin_o[4 * k + 1] = q.y;
-      if (4 * k + 2 < NF) fin_o[4 * k + 2] = q.z;
-      if (4 * k + 3 < NF) fin_o[4 * k + 3] = q.w;
+      const u32x4 q = __builtin_amdgcn_raw_buffer_load_b128(fifo_rs, 0, fbyte + 16 * k, 0);
+      fin_o[4 * k] = __uint_as_float(q.x);
+      if (4 * k + 1 < NF) fin_o[4 * k + 1] = __uint_as_float(q.y);
+      if (4 * k + 2 < NF) fin_o[4 * k + 2] = __uint_as_float(q.z);
+      if (4 * k + 3 < NF) fin_o[4 * k + 3] = __uint_as_float(q.w);
     }
-    fslot = fslot + 1 == Pc ? 0 : fslot + 1;
+    fbyte = fbyte + kSlotB == Pc * kSlotB ? 0 : fbyte + kSlotB;
   };
   auto step_fn = [&](int step, T& g_io, T* fin_io) __attribute__((always_inline)) {
     if (((step + 2) & 15) == 0) produce_chunk((step + 2) >> 4);
@@ -253,15 +257,16 @@ dp_fwd_mfma_kernel(const DpFwdArgs a) {
     dp_cell<T, NLEV, 1>(M, in, of, PS, HB, false);
 #pragma unroll
     for (int k = 0; k < NF; ++k) fl[k] = of[k];
-    if (lane == kWave - 1 && step >= kWave - 1) {
+    {
+      const int vo = lane == kWave - 1 && step >= kWave - 1 ? 0 : int(kBufOob);
 #pragma unroll
       for (int k = 0; k < NFG / 4; ++k) {
-        float4 q;
-        q.x = of[4 * k];
-        q.y = 4 * k + 1 < NF ? of[4 * k + 1] : T(0);
-        q.z = 4 * k + 2 < NF ? of[4 * k + 2] : T(0);
-        q.w = 4 * k + 3 < NF ? of[4 * k + 3] : T(0);
-        fifo[s63 * (NFG / 4) + k] = q;
+        u32x4 q;
+        q.x = __float_as_uint(of[4 * k]);
+        q.y = 4 * k + 1 < NF ? __float_as_uint(of[4 * k + 1]) : 0u;
+        q.z = 4 * k + 2 < NF ? __float_as_uint(of[4 * k + 2]) : 0u;
+        q.w = 4 * k + 3 < NF ? __float_as_uint(of[4 * k + 3]) : 0u;
+        __builtin_amdgcn_raw_buffer_store_b128(q, fifo_rs, vo + 16 * k, s63b, 0);
       }
     }
     ++cp;
@@ -282,7 +287,7 @@ dp_fwd_mfma_kernel(const DpFwdArgs a) {
 #pragma unroll
       for (int k = 0; k < NPS; ++k) PS[k] = T(0);
     }
-    s63 = s63 + 1 == Pc ? 0 : s63 + 1;
+    s63b = s63b + kSlotB == Pc * kSlotB ? 0 : s63b + kSlotB;
     // This step's inputs are consumed: refill the set for the step after
     // next.  Issued last, so that the loads reuse the registers of this
     // step's inputs instead of copies of them.
@@ -294,7 +299,13 @@ dp_fwd_mfma_kernel(const DpFwdArgs a) {
   produce_chunk(0);
   prefetch(g0, f0);
   prefetch(g1, f1);
-  int step = 0;
+  // The first two steps run outside the loop, so that the loop is entered
+  // with the memory operations of a whole step issued after each prefetch,
+  // as on every later iteration; the compiler's wait counts then only wait
+  // for the prefetched loads, not for the stores issued after them.
+  step_fn(0, g0, f0);
+  step_fn(1, g1, f1);
+  int step = 2;
   for (; step + 2 <= total; step += 2) {
     step_fn(step, g0, f0);
     step_fn(step + 1, g1, f1);
