@@ -312,11 +312,13 @@ dp_backward_mfma_kernel(const DpBwdArgs a) {
   // set, then refills it for the step after next.  Every lane reads lane 0's
   // FIFO slot (a broadcast; only lane 0 keeps it), so the read needs no lane
   // mask.  Lane 63 wrote that slot Pc - 65 >= 15 steps earlier.
-  // The FIFO through a buffer descriptor: lane 0's slot is a scalar offset,
-  // lane 63's store is the only one in range (the other lanes' are dropped).
+  // The FIFO through a buffer descriptor: lane 0's slot is a scalar offset;
+  // only lane 0's load and lane 63's store are in range (the other lanes
+  // read zero, which the lane shift adds, and their stores are dropped).
   const __amdgpu_buffer_rsrc_t fifo_rs = raw_rsrc(fifo, unsigned(Pc * NFG) * 4u);
   constexpr int kSlotB = NFG * 4;  // FIFO slot bytes
   int fbyte = 0;  // lane 0's FIFO slot of the next prefetched step: ((step + 2) mod Pc) * kSlotB
+  const int fin_vo = lane == 0 ? 0 : int(kBufOob);
   auto prefetch = [&](T& arg_o, T* fin_o) __attribute__((always_inline)) {  // uniform
     arg_o = ring[ring_pos(rslot, lane)];
     rslot = rslot + 1 == W ? 0 : rslot + 1;
@@ -324,7 +326,7 @@ dp_backward_mfma_kernel(const DpBwdArgs a) {
     fbyte = fbyte + kSlotB == Pc * kSlotB ? 0 : fbyte + kSlotB;
 #pragma unroll
     for (int k = 0; k < NFG; k += 4) {
-      const u32x4 t = __builtin_amdgcn_raw_buffer_load_b128(fifo_rs, 0, f0 + 4 * k, 0);
+      const u32x4 t = __builtin_amdgcn_raw_buffer_load_b128(fifo_rs, fin_vo, f0 + 4 * k, 0);
       fin_o[k] = __uint_as_float(t.x);
       if (k + 1 < NF) fin_o[k + 1] = __uint_as_float(t.y);
       if (k + 2 < NF) fin_o[k + 2] = __uint_as_float(t.z);
@@ -341,7 +343,7 @@ dp_backward_mfma_kernel(const DpBwdArgs a) {
     const bool active = s >= 0 && band < nb;
     T in[NF];
 #pragma unroll
-    for (int k = 0; k < NF; ++k) in[k] = shift_in(fl[k], fin_io[k], lane == 0);
+    for (int k = 0; k < NF; ++k) in[k] = shift_add(fl[k], fin_io[k]);
     const T arg = arg_io;
     T g = rbf ? fast_exp2(-arg) : arg;
 

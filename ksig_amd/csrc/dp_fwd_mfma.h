@@ -219,18 +219,20 @@ dp_fwd_mfma_kernel(const DpFwdArgs a) {
 
   const int total = ncols + kWave - 1;
   int rslot = 0;   // ring slot of the next prefetched step
-  // The FIFO through a buffer descriptor: slots are scalar offsets, and lane
-  // 63's store is the only one in range (the other lanes' are dropped).
+  // The FIFO through a buffer descriptor: slots are scalar offsets; only lane
+  // 0's load and lane 63's store are in range (the other lanes read zero,
+  // which the lane shift adds, and their stores are dropped).
   const __amdgpu_buffer_rsrc_t fifo_rs = raw_rsrc(fifo, unsigned(Pc * NFG) * 4u);
   constexpr int kSlotB = NFG * 4;  // FIFO slot bytes
   int fbyte = 0;  // lane 0's FIFO slot of the next prefetched step (bytes)
+  const int fin_vo = lane == 0 ? 0 : int(kBufOob);
   int s63b = (Pc - (kWave - 1)) * kSlotB;  // lane 63's FIFO slot ((t - 63) mod Pc, bytes)
   auto prefetch = [&](T& g_o, T* fin_o) __attribute__((always_inline)) {  // uniform
     g_o = ring[ring_pos(rslot, lane)];
     rslot = rslot + 1 == W ? 0 : rslot + 1;
 #pragma unroll
     for (int k = 0; k < NFG / 4; ++k) {
-      const u32x4 q = __builtin_amdgcn_raw_buffer_load_b128(fifo_rs, 0, fbyte + 16 * k, 0);
+      const u32x4 q = __builtin_amdgcn_raw_buffer_load_b128(fifo_rs, fin_vo, fbyte + 16 * k, 0);
       fin_o[4 * k] = __uint_as_float(q.x);
       if (4 * k + 1 < NF) fin_o[4 * k + 1] = __uint_as_float(q.y);
       if (4 * k + 2 < NF) fin_o[4 * k + 2] = __uint_as_float(q.z);
@@ -242,7 +244,7 @@ dp_fwd_mfma_kernel(const DpFwdArgs a) {
     if (((step + 2) & 15) == 0) produce_chunk((step + 2) >> 4);
     T in[NF];
 #pragma unroll
-    for (int k = 0; k < NF; ++k) in[k] = shift_in(fl[k], fin_io[k], lane == 0);
+    for (int k = 0; k < NF; ++k) in[k] = shift_add(fl[k], fin_io[k]);
     const T gnew = g_io;
 
     const bool keep = cp != 0;

@@ -29,12 +29,12 @@ __device__ __forceinline__ float shift_down(float v, float fill) {
       __float_as_int(fill), __float_as_int(v), 0x138, 0xf, 0xf, false));
 }
 
-// Same shift with lane 0's value selected rather than carried in the DPP's
-// `old` operand: `fill` is only read, so a register that a load refills every
-// step (a prefetched FIFO entry) is free again once the step has read it.
-__device__ __forceinline__ float shift_in(float v, float fill, bool lane0) {
-  const float up = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xf, 0xf, true));
-  return lane0 ? fill : up;
+// The same shift when `fill` is zero on every lane but lane 0 (a load that
+// only lane 0 keeps in range): lane 0 gets 0 + fill, lane i > 0 gets v of
+// lane i-1 + 0.  The compiler folds the DPP move into the add (one
+// v_add_f32_dpp per value); the adds are exact.
+__device__ __forceinline__ float shift_add(float v, float fill0) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xf, 0xf, true)) + fill0;
 }
 
 __device__ __forceinline__ double shift_down(double v, double fill) {
