@@ -216,6 +216,10 @@ dp_fwd_mfma_kernel(const DpFwdArgs a) {
   int row = lane;             // grid row of this lane's current band
   int brow = 0;               // band of this lane (counts the wraps of cp)
   T* sv = SAVE ? reinterpret_cast<T*>(a.save) + a.save_off[pr] : nullptr;
+  // row-end prefixes [NPS][Nr] through a buffer descriptor: one VGPR offset
+  // per lane, the level as a scalar offset
+  const __amdgpu_buffer_rsrc_t sv_rs =
+      raw_rsrc(sv, SAVE ? unsigned(NPS * Nr) * 4u : 0u);
 
   const int total = ncols + kWave - 1;
   int rslot = 0;   // ring slot of the next prefetched step
@@ -278,10 +282,10 @@ dp_fwd_mfma_kernel(const DpFwdArgs a) {
         if constexpr (SAVE && NLEV > 1) {
           // Row-end prefixes P_a(row, Nc): the padding columns past Nc have
           // M = 0 and zero column sums, so PS still holds them here.
-          if (row < Nr && brow < nb) {
+          const int vo = row < Nr && brow < nb ? row * 4 : int(kBufOob);
 #pragma unroll
-            for (int m = 0; m < NPS; ++m) sv[m * Nr + row] = PS[m];
-          }
+          for (int m = 0; m < NPS; ++m)
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(PS[m]), sv_rs, vo, m * Nr * 4, 0);
         }
         row += kWave;
         ++brow;
