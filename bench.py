@@ -174,6 +174,10 @@ def main():
   ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                   help="only run an elementwise copy of MIB MiB (5 times): the "
                        "reference point for scaling HBM byte counters")
+  ap.add_argument("--kernel-probe", default=None, choices=["fwd_save", "adjoint", "contract"],
+                  help="only time one kernel of the C4 training step on one 16384-pair "
+                       "chunk (128 x 128 pairs, L=128, d=32, 6 levels), --steps launches: "
+                       "the unit that counter collection and the per-chunk costs refer to")
   args = ap.parse_args()
   if args.gpus != 1:
     print(f"bench.py measures one GPU only (the signature-kernel path shards by "
@@ -193,6 +197,8 @@ def main():
                       "calls": 5}), flush=True)
     return
 
+  if args.kernel_probe:
+    return kernel_probe(args)
   if args.workload == "c3":
     return bench_c3(args)
   if args.workload == "c4":
@@ -316,6 +322,72 @@ def main():
   if not args.no_cpu:
     res["cpu_baseline"] = cpu_baseline(X64, n_levels, sigma, args.cpu_seconds)
   print(json.dumps(res), flush=True)
+
+
+def kernel_probe(args):
+  """One C4 kernel on one chunk of the training step: 128 Brownian paths
+  (L=128, d=32) against themselves, 16384 pairs, 6 levels, RBF sigma=sqrt(32);
+  the training forward (boundary state), the adjoint (omega) or the
+  contraction of omega with the inputs, each launched --steps times after
+  one untimed launch, timed with HIP events on the library's stream."""
+  from ksig_amd import _lib, _ops
+  dev = torch.device("cuda", 0)
+  lib = _lib.load()
+  st = _lib.stream_handle(dev)
+  n, L, d, nl = 128, 128, 32, 6
+  g = torch.Generator(device=dev).manual_seed(1)
+  X = brownian(g, n, L, d, dev).reshape(-1, d).contiguous()
+  S = _ops.SeqSet.dense(X.reshape(n, L, d))
+  pairs = _ops.dense_pairs(n, n, dev)
+  P = pairs.shape[0]
+  per = (nl - 1) * (2 * L - 2)
+  save_off = torch.arange(P, device=dev, dtype=torch.int64) * per
+  save = torch.empty(P * per, device=dev)
+  out = torch.empty((P, nl + 1), device=dev, dtype=torch.float64)
+  sig = math.sqrt(d)
+  xa = (S.data.data_ptr(), S.off.data_ptr(), S.len.data_ptr(), L)
+  fwsb = lib.ksig_signature_levels_fwd_save_workspace(0, d, nl, L, L, P)
+  fws = torch.empty(max(fwsb // 4, 1), device=dev)
+  gout = torch.randn((P, nl + 1), device=dev, dtype=torch.float64, generator=g)
+  omega = torch.empty(P * L * L, device=dev)
+  w_off = torch.arange(P, device=dev, dtype=torch.int64) * (L * L)
+  gp = torch.empty(P, device=dev, dtype=torch.float64)
+  wsb = lib.ksig_signature_levels_bwd_workspace(0, d, nl, L, L, P)
+  ws = torch.empty(max(wsb // 4, 1), device=dev)
+  gx = torch.zeros_like(X)
+
+  def fwd_save():
+    _lib.check(lib.ksig_signature_levels_fwd_save(
+        0, *xa, *xa, d, pairs.data_ptr(), P, _lib.STATIC_RBF, sig, nl, out.data_ptr(),
+        save.data_ptr(), save_off.data_ptr(), fws.data_ptr(), fwsb, st))
+
+  def adjoint():
+    _lib.check(lib.ksig_signature_levels_bwd(
+        0, *xa, *xa, d, pairs.data_ptr(), P, _lib.STATIC_RBF, sig, nl, gout.data_ptr(),
+        save.data_ptr(), save_off.data_ptr(), omega.data_ptr(), w_off.data_ptr(), gp.data_ptr(),
+        ws.data_ptr(), wsb, st))
+
+  def contract():
+    c = 1.0 / (sig * sig)
+    _lib.check(lib.ksig_static_grad_contract(
+        0, *xa, *xa, d, pairs.data_ptr(), P, omega.data_ptr(), w_off.data_ptr(), c, -c,
+        gx.data_ptr(), gx.data_ptr(), st))
+
+  fwd_save()
+  if args.kernel_probe != "fwd_save":
+    adjoint()
+  f = {"fwd_save": fwd_save, "adjoint": adjoint, "contract": contract}[args.kernel_probe]
+  f()
+  torch.cuda.synchronize()
+  e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+  e0.record()
+  for _ in range(args.steps):
+    f()
+  e1.record()
+  torch.cuda.synchronize()
+  ms = e0.elapsed_time(e1) / args.steps
+  print(json.dumps({"kernel_probe": args.kernel_probe, "pairs": P, "L": L, "d": d,
+                    "n_levels": nl, "launches": args.steps, "ms_per_launch": ms}), flush=True)
 
 
 def ragged_walks(rng: np.random.Generator, lens, d: int, offset_scale: float, dev):
