@@ -168,6 +168,8 @@ def main():
   ap.add_argument("--cpu-seconds", type=float, default=12.0)
   ap.add_argument("--no-cpu", action="store_true")
   ap.add_argument("--no-plain", action="store_true")
+  ap.add_argument("--no-tune", action="store_true",
+                  help="c3: skip the autotuner and time the default schedule")
   ap.add_argument("--gram", default="valu", choices=["valu", "fp32", "bf16x3"],
                   help="static-kernel Gram evaluation inside the sweep")
   ap.add_argument("--no-variants", action="store_true")
@@ -426,6 +428,21 @@ def bench_c3(args):
   t0 = time.perf_counter()
   sched = _ops.make_schedule(X, Y, pairs, _lib.STATIC_RBF, n_levels, order)
   plan_ms = (time.perf_counter() - t0) * 1e3
+  tuned = None
+  if not args.no_tune:
+    # launch choices (FIFO placement, slice bound, slices per resident wave)
+    # timed on this pair list by the autotuner, before the timed region
+    from ksig_amd import tune
+    t0 = time.perf_counter()
+    best, times = tune.autotune(X, Y, pairs, n_levels, order, _lib.STATIC_RBF, sigma, reps=3)
+    tune_s = time.perf_counter() - t0
+    default_cfg = tune.StreamConfig(bool(sched[3]), 128, 1)
+    tuned = {"best": {"global_fifo": best.global_fifo, "max_slice": best.max_slice,
+                      "oversub": best.oversub},
+             "candidates_ms": {f"{'gfifo' if c.global_fifo else 'lds'}/s{c.max_slice}/x{c.oversub}":
+                               round(t, 4) for c, t in times.items()},
+             "default_ms": times.get(default_cfg), "tune_s": tune_s}
+    sched = tune.tuned_schedule(X, Y, pairs, _lib.STATIC_RBF, n_levels, order)
 
   def step():
     return _ops.signature_levels_batched(X, Y, pairs, n_levels, order, _lib.STATIC_RBF, sigma,
@@ -471,7 +488,7 @@ def bench_c3(args):
       "config": {"workload": "batched uneven DP kernel: 4096 pairs, L ~ U{32..1024} per "
                              "sequence, d=16, 8 levels, order 2, RBF sigma=4",
                  "cells": cells, "n_waves": int(sched[1].shape[0] - 1),
-                 "global_fifo": bool(sched[3]), "plan_ms": plan_ms},
+                 "global_fifo": bool(sched[3]), "plan_ms": plan_ms, "autotune": tuned},
       "max_rel_err_vs_fp64": max(errs),
       "err_sample": "16 random pairs, normwise over each pair's level vector",
   }

@@ -12,5 +12,6 @@ from . import utils
 from . import static
 from . import algorithms
 from . import kernels
+from . import tune
 
 __version__ = "0.1.0"

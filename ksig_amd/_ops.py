@@ -118,8 +118,9 @@ def signature_levels(X: SeqSet, Y: SeqSet, pairs: torch.Tensor, n_levels: int,
   if (n_pairs > 0 and X.data.dtype == torch.float32 and difference and gram_mode in (-1, 0)
       and kind in (_lib.STATIC_RBF, _lib.STATIC_LINEAR) and not (X.uniform and Y.uniform)):
     # uneven lengths: length-bucketed schedule for the row stream
+    from . import tune
     try:
-      sched = make_schedule(X, Y, pairs, kind, n_levels, order)
+      sched = tune.tuned_schedule(X, Y, pairs, kind, n_levels, order)
     except NotImplementedError:
       sched = None
     if sched is not None:
@@ -242,25 +243,38 @@ def signature_levels_batched(X: SeqSet, Y: SeqSet, pairs: torch.Tensor, n_levels
 
 
 def make_schedule(X: SeqSet, Y: SeqSet, pairs: torch.Tensor, kind: int, n_levels: int,
-                  order: int):
+                  order: int, global_fifo: Optional[bool] = None, max_slice: int = 0,
+                  oversub: int = 1):
   """(order, wave_beg, max_slice, global_fifo) of the length-bucketed schedule
-  (device int32 tensors, the largest slice and the FIFO placement)."""
+  (device int32 tensors, the largest slice and the FIFO placement).
+
+  ``global_fifo`` None picks the FIFO placement with more resident waves;
+  ``max_slice`` (0 = the kernel's 128) bounds the pairs of one wave, which
+  sizes its LDS pair table; ``oversub`` cuts that many slices per resident
+  wave (the hardware hands the later, lighter slices to waves that finish
+  early).  The autotuner (ksig_amd.tune) searches these three."""
   norm_order = n_levels if (order <= 0 or order >= n_levels) else order
-  slots, maxp, _ = stream_slots(X, Y, kind, n_levels, norm_order)
-  g_slots, _, _ = stream_slots(X, Y, kind, n_levels, norm_order, 0, True)
-  # the FIFO goes to global memory when an LDS FIFO would cost resident waves
-  global_fifo = g_slots > slots
-  slots = max(slots, g_slots)
+  if global_fifo is None:
+    slots, maxp, _ = stream_slots(X, Y, kind, n_levels, norm_order, max_slice)
+    g_slots, _, _ = stream_slots(X, Y, kind, n_levels, norm_order, max_slice, True)
+    # the FIFO goes to global memory when an LDS FIFO would cost resident waves
+    global_fifo = g_slots > slots
+    slots = max(slots, g_slots)
+  else:
+    slots, maxp, _ = stream_slots(X, Y, kind, n_levels, norm_order, max_slice,
+                                  bool(global_fifo))
   if slots <= 0:
     raise NotImplementedError("no row-stream kernel fits this problem shape")
+  if max_slice > 0:
+    maxp = min(maxp, int(max_slice))
   pr = pairs.detach().cpu().numpy()
   lx = X.lengths()[pr[:, 0]]
   ly = Y.lengths()[pr[:, 1]]
-  so, wb = stream_schedule(lx, ly, slots, maxp,
+  so, wb = stream_schedule(lx, ly, slots * max(int(oversub), 1), maxp,
                            min_cols=96 if global_fifo else 68)
   dev = X.data.device
   return (torch.as_tensor(so, device=dev), torch.as_tensor(wb, device=dev),
-          int(np.diff(wb).max()) if len(wb) > 1 else 1, global_fifo)
+          int(np.diff(wb).max()) if len(wb) > 1 else 1, bool(global_fifo))
 
 
 def signature_levels_gram(M: torch.Tensor, n_levels: int, order: int,

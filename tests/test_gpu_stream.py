@@ -170,3 +170,39 @@ def test_c3_shape_sample_uses_workspace_fifo():
   ref = _oracle([xs[i] for i in idx], [ys[i] for i in idx],
                 [(k, k) for k in range(len(idx))], 8, 2, "rbf", 4.0)
   _check(got.cpu().numpy()[idx], ref)
+
+
+def test_autotuned_launch_choices_agree_with_oracle():
+  """Every candidate the autotuner times (FIFO placement, slice bound,
+  slices per resident wave) computes the same levels, and the kept choice
+  is the one signature_levels uses afterwards."""
+  from ksig_amd import tune
+  rng = np.random.default_rng(8)
+  n = 96
+  lx = rng.integers(2, 300, n)
+  ly = rng.integers(2, 300, n)
+  xs, X = _ragged(rng, lx, 16)
+  ys, Y = _ragged(rng, ly, 16)
+  pr = np.stack([np.arange(n), rng.permutation(n)], -1).astype(np.int32)
+  pairs = to_dev(pr)
+  cands = tune.candidates(X, Y, _lib.STATIC_RBF, 4, 2)
+  assert len(cands) >= 3
+  results = []
+  for cfg in cands:
+    sched = _ops.make_schedule(X, Y, pairs, _lib.STATIC_RBF, 4, 2, cfg.global_fifo,
+                               cfg.max_slice, cfg.oversub)
+    assert sched[3] == cfg.global_fifo and sched[2] <= cfg.max_slice
+    results.append(_ops.signature_levels_batched(X, Y, pairs, 4, 2, _lib.STATIC_RBF, 2.0,
+                                                 sched).cpu().numpy())
+  idx = rng.choice(n, 8, replace=False)
+  ref = _oracle(xs, ys, pr[idx], 4, 2, "rbf", 2.0)
+  for got in results:
+    _check(got[idx], ref)
+    assert level_err(got.T, results[0].T) < 1e-6
+  tune.clear()
+  best, times = tune.autotune(X, Y, pairs, 4, 2, _lib.STATIC_RBF, 2.0, reps=1)
+  assert best in times and set(times) == set(cands)
+  assert tune.lookup(X, Y, pairs, _lib.STATIC_RBF, 4, 2) == best
+  got = _ops.signature_levels(X, Y, pairs, 4, 2, True, _lib.STATIC_RBF, 2.0).cpu().numpy()
+  _check(got[idx], ref)
+  tune.clear()
