@@ -157,7 +157,7 @@ def cpu_baseline(X64: np.ndarray, n_levels: int, sigma: float, seconds: float):
 
 def main():
   ap = argparse.ArgumentParser()
-  ap.add_argument("--workload", default="c1", choices=["c1", "c2", "c3", "c4"],
+  ap.add_argument("--workload", default="c1", choices=["c1", "c2", "c3", "c4", "gak"],
                   help="c1: 512 x L=200 x d=8 symmetric Gram (the headline line); "
                        "c3: 4096 uneven-length pairs, L in [32, 1024], d=16, 8 levels, order 2; "
                        "c4: MMD^2 training step, 1024 x 1024 paths, L=128, d=32, 6 levels; "
@@ -207,6 +207,8 @@ def main():
     return bench_c4(args)
   if args.workload == "c2":
     return bench_c2(args)
+  if args.workload == "gak":
+    return bench_gak(args)
 
   import ksig_amd
   from ksig_amd import _lib, _ops
@@ -323,6 +325,139 @@ def main():
     del plain_levels
   if not args.no_cpu:
     res["cpu_baseline"] = cpu_baseline(X64, n_levels, sigma, args.cpu_seconds)
+  print(json.dumps(res), flush=True)
+
+
+# per cell: static kernel (d multiply-adds, norm terms, exp), the transform
+# log(M / (2 - M)) (sub, log, sub, max), the log-sum-exp of three (2 max,
+# 3 sub, 3 exp, 2 add, log) and the two final adds
+GAK_FLOPS_PER_CELL_BASE = 4 + 4 + 11 + 2
+
+
+def plain_gak_log(X, Y, sigma):
+  """The reference dataflow of ksig/algorithms.py:514-559 in torch FP32 ops on
+  the GPU: materialise M [b, Lx, Ly] for pairs (X[k], Y[k]), transform and
+  take logs, then sweep the antidiagonals with one batched log-sum-exp per
+  antidiagonal over every pair and cell on it."""
+  b, Lx, _ = X.shape
+  Ly = Y.shape[1]
+  D2 = (X * X).sum(-1)[:, :, None] + (Y * Y).sum(-1)[:, None, :] - 2 * X @ Y.transpose(1, 2)
+  M = torch.exp(-D2 / max(2 * sigma ** 2, 1e-12))
+  lg = torch.log(torch.clamp(M / (2 - M), min=1e-12))
+  K = torch.full((b, Lx + 1, Ly + 1), float("-inf"), device=X.device, dtype=X.dtype)
+  K[:, 0, 0] = 0.0
+  for t in range(Lx + Ly - 1):
+    i = torch.arange(max(0, t - Ly + 1), min(t, Lx - 1) + 1, device=X.device)
+    j = t - i
+    up, left, dg = K[:, i, j + 1], K[:, i + 1, j], K[:, i, j]
+    m = torch.maximum(torch.maximum(up, left), dg)
+    K[:, i + 1, j + 1] = lg[:, i, j] + m + torch.log(
+        torch.exp(up - m) + torch.exp(left - m) + torch.exp(dg - m))
+  return K[:, Lx, Ly]
+
+
+def bench_gak(args):
+  """Global alignment kernel (log space), fused HIP sweep: 512 Brownian paths,
+  L=200, d=8, RBF sigma=sqrt(8), symmetric Gram as 131,328 upper-triangle
+  pairs (the shape of the DP signature-kernel workload)."""
+  from ksig_amd import _lib, _ops
+  from oracle import sigkern as O
+  dev = torch.device("cuda", 0)
+  torch.cuda.set_device(dev)
+  n, L, d = 512, 200, 8
+  sigma = math.sqrt(d)
+  g = torch.Generator(device=dev).manual_seed(0)
+  X = brownian(g, n, L, d, dev)
+  S = _ops.SeqSet.dense(X)
+  pairs = _ops.triu_pairs(n, dev)
+  n_pairs = pairs.shape[0]
+  cells = float(n_pairs) * L * L
+
+  def step():
+    return _ops.global_align_log(S, S, pairs, _lib.STATIC_RBF, sigma)
+  for _ in range(args.warmup):
+    step()
+  torch.cuda.synchronize()
+  stream = torch.cuda.current_stream(dev)
+  e0 = torch.cuda.Event(enable_timing=True)
+  e1 = torch.cuda.Event(enable_timing=True)
+  t0 = time.perf_counter()
+  e0.record(stream)
+  for _ in range(args.steps):
+    out = step()
+  e1.record(stream)
+  torch.cuda.synchronize()
+  wall = time.perf_counter() - t0
+  kern_ms = e0.elapsed_time(e1) / args.steps
+  ms_per_step = wall * 1e3 / args.steps
+
+  X64 = X.double().cpu().numpy()
+  idx = np.random.default_rng(3).choice(n_pairs, 16, replace=False)
+  pr = pairs.cpu().numpy()[idx]
+  got = out.cpu().numpy()[idx]
+  ref = np.array([O.pair_gak_log(X64[a], X64[b], "rbf", sigma) for a, b in pr])
+  err = float(np.max(np.abs(got - ref) / np.maximum(1.0, np.abs(ref))))
+
+  flops_cell = GAK_FLOPS_PER_CELL_BASE + 2 * d
+  achieved = cells * flops_cell / (kern_ms * 1e-3) / 1e12
+  res = {
+      "metric": "GA-cells/sec (pairs x Lx x Ly), fused log-space fwd, fp32 inputs",
+      "value": cells / (ms_per_step * 1e-3),
+      "unit": "GA cells/s",
+      "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+      "ms_per_step": ms_per_step,
+      "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+      "dtype": "f32 inputs, f64 log accumulation", "data": "synthetic (seeded Brownian paths)",
+      "config": {"workload": "global alignment kernel, 512 x L=200 x d=8 Brownian, RBF "
+                             "sigma=sqrt(8), symmetric Gram (131328 upper-triangle pairs)",
+                 "pairs_evaluated": n_pairs, "L": L, "d": d},
+      "max_abs_log_err_vs_fp64": err,
+      "err_sample": "16 random pairs, |log GA - log GA_64| / max(1, |log GA_64|)",
+      "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+                   "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                   "kernel": "gak_forward_kernel<float,8>", "kernel_ms": kern_ms,
+                   "flops_per_cell": flops_cell},
+  }
+  if not args.no_plain:
+    nb = 256
+    a = torch.arange(nb, device=dev) % n
+    b = (torch.arange(nb, device=dev) * 7 + 3) % n
+    Xa, Xb = X[a].contiguous(), X[b].contiguous()
+    pl = plain_gak_log(Xa, Xb, sigma)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    plain_gak_log(Xa, Xb, sigma)
+    torch.cuda.synchronize()
+    tp = time.perf_counter() - t1
+    pv = nb * L * L / tp
+    chk = _ops.global_align_log(S, S, torch.stack([a, b], -1).to(torch.int32).contiguous(),
+                                _lib.STATIC_RBF, sigma)
+    pdiff = float(torch.max(torch.abs(pl.double() - chk) / torch.clamp(chk.abs(), min=1.0)))
+    res["plain_fp32"] = {"value": pv, "unit": "GA cells/s",
+                         "speedup_fused_over_plain": res["value"] / pv,
+                         "sample": f"{nb} pairs per call: M materialised, one batched "
+                                   "log-sum-exp per antidiagonal, torch FP32 ops",
+                         "max_abs_log_diff_vs_fused": pdiff}
+  if not args.no_cpu:
+    try:
+      from threadpoolctl import threadpool_limits
+      lim = threadpool_limits(1)
+    except Exception:  # pragma: no cover
+      lim = None
+    rng = np.random.default_rng(7)
+    batch, done = 64, 0
+    t2 = time.perf_counter()
+    while time.perf_counter() - t2 < args.cpu_seconds:
+      ia = rng.integers(0, n, batch)
+      ib = rng.integers(0, n, batch)
+      O.global_align_kern_log(O.rbf_gram(X64[ia], X64[ib], sigma))
+      done += batch
+    dt = time.perf_counter() - t2
+    res["cpu_baseline"] = {"value": done * L * L / dt, "unit": "GA cells/s", "cores": 1,
+                           "kind": "port",
+                           "sample": f"{done} random pairs of the workload, FP64 NumPy "
+                                     f"oracle (vectorised over {batch} pairs), {dt:.1f} s "
+                                     "on 1 host core"}
   print(json.dumps(res), flush=True)
 
 

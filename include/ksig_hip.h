@@ -308,6 +308,31 @@ int ksig_signature_pde_gram(int dtype, const void* M, int64_t n_mats, int lx,
                             int ly, int dyadic_order, int difference,
                             double* out, void* stream);
 
+/*
+ * Global alignment kernel in log space for a list of sequence pairs, with the
+ * static kernel fused (RBF or linear): out[p] = log GA(x_a, y_b) of
+ * pairs[p] = (a, b), float64.  Replaces the device recursion
+ * _global_align_kern_log of ksig/algorithms.py:453-511 behind
+ * global_align_kern_log(M) (:514-559), as called by
+ * GlobalAlignmentKernel._compute_kernel (ksig/kernels.py:395-406), including
+ * the transform M / (2 - M) and the clamp at 1e-12 (:532-534).  One wave per
+ * pair; the shorter sequence of a pair (at most min(max_len_x, max_len_y)
+ * points) is staged in LDS: KSIG_ELDS when it does not fit.
+ */
+int ksig_global_align_log(int dtype, const void* x, const int64_t* x_off,
+                          const int32_t* x_len, int max_len_x, const void* y,
+                          const int64_t* y_off, const int32_t* y_len, int max_len_y,
+                          int d, const int32_t* pairs, int64_t n_pairs, int static_kind,
+                          double static_param, double* out, void* stream);
+
+/*
+ * The same on caller-supplied static-kernel matrices M [n_mats, lx, ly]
+ * (untransformed values, dtype): global_align_kern_log(M) of
+ * ksig/algorithms.py:514-559 for M of shape [n, lx, ly].
+ */
+int ksig_global_align_log_gram(int dtype, const void* M, int64_t n_mats, int lx, int ly,
+                               double* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

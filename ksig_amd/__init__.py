@@ -13,5 +13,6 @@ from . import static
 from . import algorithms
 from . import kernels
 from . import tune
+from . import models
 
 __version__ = "0.1.0"

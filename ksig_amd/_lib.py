@@ -68,6 +68,11 @@ SIGNATURES = {
         _c_int, _c_int, _c_int, _c_int, _c_int, _c_i64]),
     "ksig_signature_pde_gram": (_c_int, [
         _c_int, _vp, _c_i64, _c_int, _c_int, _c_int, _c_int, _vp, _vp]),
+    "ksig_global_align_log": (_c_int, [
+        _c_int, _vp, _vp, _vp, _c_int, _vp, _vp, _vp, _c_int, _c_int, _vp,
+        _c_i64, _c_int, _c_dbl, _vp, _vp]),
+    "ksig_global_align_log_gram": (_c_int, [
+        _c_int, _vp, _c_i64, _c_int, _c_int, _vp, _vp]),
 }
 
 

@@ -324,3 +324,35 @@ def signature_pde_gram(M: torch.Tensor, difference: bool, dyadic_order: int
       _lib.dtype_code(Mc.dtype), Mc.data_ptr(), n, lx, ly, int(dyadic_order),
       int(bool(difference)), out.data_ptr(), _lib.stream_handle(M.device)))
   return out
+
+
+def global_align_log(X: SeqSet, Y: SeqSet, pairs: torch.Tensor, kind: int,
+                     param: float) -> torch.Tensor:
+  """log GA values [n_pairs] (float64) with the static kernel fused."""
+  lib = _lib.load()
+  _check_sets(X, Y)
+  n_pairs = pairs.shape[0]
+  out = torch.empty((n_pairs,), device=X.data.device, dtype=torch.float64)
+  if n_pairs == 0:
+    return out
+  _lib.check(lib.ksig_global_align_log(
+      _lib.dtype_code(X.data.dtype), X.data.data_ptr(), X.off.data_ptr(),
+      X.len.data_ptr(), X.max_len, Y.data.data_ptr(), Y.off.data_ptr(),
+      Y.len.data_ptr(), Y.max_len, X.d, pairs.data_ptr(), n_pairs, kind,
+      float(param), out.data_ptr(), _lib.stream_handle(X.data.device)))
+  return out
+
+
+def global_align_log_gram(M: torch.Tensor) -> torch.Tensor:
+  """log GA values [prod(lead)] (float64) for matrices M [..., lx, ly]."""
+  lib = _lib.load()
+  lx, ly = M.shape[-2:]
+  Mc = M.reshape(-1, lx, ly).contiguous()
+  n = Mc.shape[0]
+  out = torch.empty((n,), device=M.device, dtype=torch.float64)
+  if n == 0:
+    return out
+  _lib.check(lib.ksig_global_align_log_gram(
+      _lib.dtype_code(Mc.dtype), Mc.data_ptr(), n, lx, ly, out.data_ptr(),
+      _lib.stream_handle(M.device)))
+  return out

@@ -55,3 +55,26 @@ def signature_kern_pde(M, difference: bool = True, dyadic_order: int = 0
   if is_diag:
     K = K[:, 0]
   return K
+
+
+def global_align_kern_log(M) -> torch.Tensor:
+  """Logarithm of the global alignment kernel (ksig/algorithms.py:514-559):
+  M' = M / (2 - M), log-space alignment recursion over max(M', 1e-12).
+
+  Args:
+    M: static-kernel evaluations of shape [n_X, n_Y, l_X, l_Y] or [n, l_X, l_Y].
+
+  Returns:
+    log GA of shape [n_X, n_Y] or [n], dtype of M, on the GPU.
+  """
+  M = utils.as_device(M)
+  is_diag = M.ndim == 3
+  if is_diag:
+    M = M[:, None, :, :]
+  if M.ndim != 4:
+    raise ValueError('The `M` matrix must have `.ndim==3` or `.ndim==4`.')
+  n_X, n_Y = M.shape[:2]
+  K = _ops.global_align_log_gram(M).reshape(n_X, n_Y).to(M.dtype)
+  if is_diag:
+    K = K[:, 0]
+  return K

@@ -12,6 +12,7 @@
 #include "ksig_hip.h"
 #include "dp_forward.h"
 #include "dp_stream.h"
+#include "gak.h"
 
 namespace {
 
@@ -774,6 +775,57 @@ int ksig_signature_pde_gram(int dtype, const void* M, int64_t n_mats, int lx,
   a.gram_ly = ly;
   a.out = out;
   return pde_common(dtype, a, ly < 1 ? 1 : ly, 8, stream);
+}
+
+static int gak_common(int dtype, ksig::GakArgs& a, int cols, void* stream) {
+  const size_t lds = ksig::gak_lds_bytes(dtype, a.kind, a.d, cols);
+  if (lds > max_lds_bytes())
+    return fail(KSIG_ELDS, "per-pair LDS state of %zu bytes exceeds the device limit", lds);
+  hipError_t e = ksig::gak_forward_launch(a, dtype, lds, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "gak_forward launch");
+  return 0;
+}
+
+int ksig_global_align_log(int dtype, const void* x, const int64_t* x_off,
+                          const int32_t* x_len, int max_len_x, const void* y,
+                          const int64_t* y_off, const int32_t* y_len, int max_len_y,
+                          int d, const int32_t* pairs, int64_t n_pairs, int static_kind,
+                          double static_param, double* out, void* stream) {
+  int rc = check_common(dtype, x, y, d, pairs, n_pairs, out);
+  if (rc) return rc;
+  ksig::GakArgs a{};
+  rc = static_scales(static_kind, static_param, &a.xscale, &a.yscale);
+  if (rc) return rc;
+  if (max_len_x < 0 || max_len_y < 0) return fail(KSIG_EINVAL, "negative max length");
+  if (n_pairs == 0) return 0;
+  a.x = {x, x_off, x_len};
+  a.y = {y, y_off, y_len};
+  a.d = d;
+  a.pairs = pairs;
+  a.n_pairs = n_pairs;
+  a.kind = static_kind;
+  a.out = out;
+  // the shorter side of a pair (the LDS side) is at most the smaller maximum
+  const int cols = max_len_x < max_len_y ? max_len_x : max_len_y;
+  return gak_common(dtype, a, cols < 1 ? 1 : cols, stream);
+}
+
+int ksig_global_align_log_gram(int dtype, const void* M, int64_t n_mats, int lx, int ly,
+                               double* out, void* stream) {
+  if (dtype != KSIG_F32 && dtype != KSIG_F64) return fail(KSIG_EINVAL, "bad dtype %d", dtype);
+  if (n_mats < 0 || n_mats > 0x7fffffffLL) return fail(KSIG_EINVAL, "bad n_mats");
+  if (lx < 0 || ly < 0) return fail(KSIG_EINVAL, "negative matrix shape");
+  if (n_mats == 0) return 0;
+  if (!M || !out) return fail(KSIG_EINVAL, "null device pointer");
+  ksig::GakArgs a{};
+  a.kind = ksig::kGram;
+  a.d = 1;
+  a.n_pairs = n_mats;
+  a.gram = M;
+  a.gram_lx = lx;
+  a.gram_ly = ly;
+  a.out = out;
+  return gak_common(dtype, a, ly < 1 ? 1 : ly, stream);
 }
 
 }  // extern "C"

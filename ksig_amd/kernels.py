@@ -244,3 +244,57 @@ class SignaturePDEKernel(SignatureKernel):
       return _mirror(values(pairs), pairs, n)
     pairs = _ops.dense_pairs(X.shape[0], Y.shape[0], dev)
     return values(pairs).reshape(X.shape[0], Y.shape[0])
+
+
+class GlobalAlignmentKernel(SignatureKernel):
+  """Normalised global alignment kernel (ksig/kernels.py:366-409).
+
+  As in the reference it takes only ``n_features`` and ``static_kernel``
+  (it does not call the signature-kernel initializer), is always normalised
+  and works in log space: K(x, y) = exp(log GA(x, y) - (log GA(x, x) +
+  log GA(y, y)) / 2) (ksig/kernels.py:191-209)."""
+
+  def __init__(self, n_features: Optional[int] = None,
+               static_kernel: Optional[StaticKernel] = None):
+    self.n_features = (utils.check_positive_value(n_features, 'n_features')
+                       if n_features is not None else None)
+    self.static_kernel = static_kernel or static.kernels.RBFKernel()
+
+  @property
+  def normalize(self):
+    return True
+
+  @property
+  def is_log_space(self):
+    return True
+
+  def _compute_kernel(self, X, Y=None, diag: bool = False) -> torch.Tensor:
+    """ksig/kernels.py:395-406: log GA(X, Y) or its diagonal (float64)."""
+    code = _ops.static_code(self.static_kernel)
+    if code is None:
+      M = self._compute_embedding(X, Y, diag=diag)
+      return algorithms.global_align_kern_log(M).to(torch.float64)
+    kind, param = code
+    dev = X.device
+    Xs = _ops.SeqSet.dense(X)
+    Ys = Xs if (Y is None or diag) else _ops.SeqSet.dense(Y)
+    if diag:
+      return _ops.global_align_log(Xs, Xs, _ops.diag_pairs(X.shape[0], dev), kind, param)
+    if Y is None:
+      n = X.shape[0]
+      pairs = _ops.triu_pairs(n, dev)
+      return _mirror(_ops.global_align_log(Xs, Xs, pairs, kind, param), pairs, n)
+    pairs = _ops.dense_pairs(X.shape[0], Y.shape[0], dev)
+    return _ops.global_align_log(Xs, Ys, pairs, kind, param).reshape(X.shape[0], Y.shape[0])
+
+  def _K(self, X, Y=None) -> torch.Tensor:
+    """ksig/kernels.py:180-214, log-space branch."""
+    K = self._compute_kernel(X, Y)
+    if Y is None:
+      Kd = utils.matrix_diag(K)
+      K = K - 0.5 * (Kd[..., :, None] + Kd[..., None, :])
+    else:
+      Kxd = self._compute_kernel(X, diag=True)
+      Kyd = self._compute_kernel(Y, diag=True)
+      K = K - 0.5 * (Kxd[..., :, None] + Kyd[..., None, :])
+    return torch.exp(K).to(X.dtype)

@@ -18,6 +18,9 @@ difference it, per-level exclusive 2-D cumulative sums):
                               recurrence the antidiagonal sweep evaluates
 * normalisation / level mean  ksig/kernels.py:166-214, ksig/utils.py:16,93-102,173-182
 * class-level embedding ..... ksig/kernels.py:124-164,242-256
+* global alignment kernel ... ksig/algorithms.py:453-559 (log-space
+                              recursion), ksig/kernels.py:366-409 (class,
+                              log-space normalisation :191-209)
 
 Additions that have no reference counterpart (build extensions):
 ``dyadic_order`` for the PDE (each ΔΔ cell refined into 2^λ×2^λ sub-cells
@@ -28,7 +31,9 @@ Parity pins (all in tests/test_oracle.py): DP vs Chen-signature inner
 products (Linear kernel, full order), DP order 1 vs truncated iterated-sum
 inner products, DP (any order, RBF) vs brute-force chain enumeration,
 PDE one-cell closed form, PDE dyadic refinement vs I0(2*sqrt(ab)),
-stuttering / translation invariance, symmetry and the normalised diagonal.
+stuttering / translation invariance, symmetry and the normalised diagonal;
+GA vs explicit enumeration of alignment paths in exp space and vs Delannoy
+numbers (M' = 1).
 """
 
 from __future__ import annotations
@@ -308,6 +313,97 @@ def pair_levels(x, y, n_levels, order=1, difference=True, kind='rbf',
 def pair_pde(x, y, difference=True, kind='rbf', param=1.0, dyadic_order=0):
   M = static_gram(x, y, kind, param)
   return signature_kern_pde(M[None], difference, dyadic_order)[0]
+
+
+# -----------------------------------------------------------------------------
+# Global alignment kernel (ksig/algorithms.py:453-559, ksig/kernels.py:366-409)
+# -----------------------------------------------------------------------------
+
+def global_align_kern_log(M: np.ndarray) -> np.ndarray:
+  """log GA of ksig/algorithms.py:514-559 for M [..., lX, lY]: M' = M/(2-M)
+  (:532), l = log(max(M', EPS)) (:534), then the log-space alignment
+  recursion of the device kernel (:453-511), written as the plain 2-D
+  recurrence the antidiagonal sweep evaluates (each cell reads only its
+  three predecessors, so the traversal order does not change any value):
+    logK(i,j) = l(i,j) + LSE(logK(i-1,j), logK(i,j-1), logK(i-1,j-1)),
+  logK(-1,-1) = 0, other out-of-grid predecessors -inf, LSE with the
+  max-shift of :494-497.  Returns logK(lX-1, lY-1) per leading index."""
+  M = np.asarray(M)
+  Mp = M / (2. - M)
+  with np.errstate(divide='ignore'):
+    lg = np.log(np.maximum(Mp, EPS))
+  lX, lY = lg.shape[-2:]
+  lead = lg.shape[:-2]
+  ninf = np.full(lead, -np.inf, dtype=lg.dtype)
+  prev = [ninf] * lY  # row i-1
+  for i in range(lX):
+    cur = []
+    for j in range(lY):
+      k01 = prev[j]
+      k10 = cur[j - 1] if j > 0 else ninf
+      if i == 0 and j == 0:
+        k00 = np.zeros(lead, dtype=lg.dtype)
+      elif i == 0 or j == 0:
+        k00 = ninf
+      else:
+        k00 = prev[j - 1]
+      m = np.maximum(np.maximum(k01, k10), k00)
+      with np.errstate(invalid='ignore'):
+        s = np.exp(k01 - m) + np.exp(k10 - m) + np.exp(k00 - m)
+      cur.append(lg[..., i, j] + m + np.log(s))
+    prev = cur
+  return prev[lY - 1]
+
+
+def global_alignment_kernel(X, Y=None, diag=False, kind='rbf', param=1.0):
+  """GlobalAlignmentKernel(...)(X, Y, diag) of ksig/kernels.py:366-409 with
+  the log-space normalisation of ksig/kernels.py:191-209."""
+  if diag:
+    return np.full((X.shape[0],), 1.)
+  logK = global_align_kern_log(pairwise_gram(X, Y, kind, param))
+  if Y is None:
+    d = np.diagonal(logK)
+    logK = logK - .5 * (d[:, None] + d[None, :])
+  else:
+    dx = global_align_kern_log(pairwise_gram(X, None, kind, param, diag=True))
+    dy = global_align_kern_log(pairwise_gram(Y, None, kind, param, diag=True))
+    logK = logK - .5 * (dx[:, None] + dy[None, :])
+  return np.exp(logK)
+
+
+def pair_gak_log(x, y, kind='rbf', param=1.0):
+  """log GA of one pair (x [lx, d], y [ly, d])."""
+  return global_align_kern_log(static_gram(x, y, kind, param)[None])[0]
+
+
+def delannoy(m: int, n: int) -> int:
+  """Number of monotone lattice paths (0,0) -> (m,n) with steps (1,0), (0,1),
+  (1,1): sum_k C(m,k) C(n,k) 2^k."""
+  return sum(math.comb(m, k) * math.comb(n, k) * 2 ** k for k in range(min(m, n) + 1))
+
+
+def gak_path_sum(Mp: np.ndarray) -> float:
+  """GA by explicit enumeration of alignments (tiny grids): the sum over
+  monotone paths (0,0) -> (lX-1,lY-1) with steps (1,0), (0,1), (1,1) of the
+  product of the transformed values M' on the path, in exp space."""
+  lX, lY = Mp.shape
+  total = 0.0
+
+  def walk(i, j, prod):
+    nonlocal total
+    prod = prod * Mp[i, j]
+    if i == lX - 1 and j == lY - 1:
+      total += prod
+      return
+    if i + 1 < lX:
+      walk(i + 1, j, prod)
+    if j + 1 < lY:
+      walk(i, j + 1, prod)
+    if i + 1 < lX and j + 1 < lY:
+      walk(i + 1, j + 1, prod)
+
+  walk(0, 0, 1.0)
+  return total
 
 
 # -----------------------------------------------------------------------------

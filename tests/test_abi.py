@@ -84,3 +84,31 @@ def test_workspace_queries_and_adjoint_argument_checks():
   rc = lib.ksig_signature_pde_grad_contract(_lib.F32, p, p, p, 4, p, p, p, 4, 3, p, 1, 0, -1.0,
                                             1, p, p, p, p, None, None)
   assert rc == 1001
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libksig_hip.so not built")
+def test_global_alignment_argument_checks():
+  """Validation of the global alignment entry points runs before any device
+  call (empty inputs return 0, bad arguments an error code)."""
+  lib = ctypes.CDLL(LIB)
+  lib.ksig_last_error.restype = ctypes.c_char_p
+  f = lib.ksig_global_align_log_gram
+  f.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
+                ctypes.c_void_p, ctypes.c_void_p]
+  assert f(3, None, 0, 4, 4, None, None) == 1001
+  assert f(0, None, -1, 4, 4, None, None) == 1001
+  assert f(0, None, 0, 4, 4, None, None) == 0
+  assert f(0, None, 2, 4, 4, None, None) == 1001  # null pointers
+  g = lib.ksig_global_align_log
+  g.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_double,
+                ctypes.c_void_p, ctypes.c_void_p]
+  args = [0, None, None, None, 4, None, None, None, 4, 3, None, 0, 0, 1.0, None, None]
+  assert g(*args) == 0
+  bad = list(args); bad[12] = 5  # unknown static kind
+  assert g(*bad) == 1001
+  bad = list(args); bad[13] = -1.0  # non-positive bandwidth
+  assert g(*bad) == 1001 and b"positive" in lib.ksig_last_error()
+  bad = list(args); bad[9] = 40  # d beyond the built buckets
+  assert g(*bad) == 1002

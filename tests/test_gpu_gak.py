@@ -1,0 +1,121 @@
+"""GPU parity of the global alignment kernel (log space) against the FP64
+oracle: fused RBF / linear static kernels, caller-supplied matrices, band
+boundaries, uneven lengths, the Delannoy closed form and the class API.
+
+Tolerances: log values within 1e-11 (absolute, relative above 1) for float64
+inputs and 1e-5 for float32 inputs; the normalised float32 Gram within 1e-5."""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import ksig_amd as ksig
+from ksig_amd import _lib, _ops
+from oracle import sigkern as O
+from tests.gpu_util import dev, normwise, to_dev
+
+pytestmark = pytest.mark.gpu
+
+
+def _logerr(got, ref):
+  got = np.asarray(got, dtype=np.float64)
+  ref = np.asarray(ref, dtype=np.float64)
+  return float(np.max(np.abs(got - ref) / np.maximum(1.0, np.abs(ref))))
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-11), (np.float32, 1e-5)])
+@pytest.mark.parametrize("kind,param", [("rbf", 1.3), ("linear", 0.3)])
+@pytest.mark.parametrize("lx,ly,d", [(12, 9, 3), (1, 5, 2), (70, 64, 8), (150, 131, 16),
+                                     (200, 3, 32), (64, 65, 5)])
+def test_fused_log_values(dtype, tol, kind, param, lx, ly, d):
+  rng = np.random.default_rng(lx * 7 + ly + d)
+  X = O.brownian(rng, 3, lx, d).astype(dtype)
+  Y = O.brownian(rng, 2, ly, d).astype(dtype)
+  code = _lib.STATIC_RBF if kind == "rbf" else _lib.STATIC_LINEAR
+  if kind == "linear":
+    param = param / d  # keeps M = scale <x, y> well below the pole of M / (2 - M)
+  got = _ops.global_align_log(_ops.SeqSet.dense(to_dev(X)), _ops.SeqSet.dense(to_dev(Y)),
+                              _ops.dense_pairs(3, 2, dev()), code, param).cpu().numpy()
+  ref = O.global_align_kern_log(O.pairwise_gram(X.astype(np.float64), Y.astype(np.float64),
+                                                kind, param)).reshape(-1)
+  assert _logerr(got, ref) < tol
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-11), (np.float32, 1e-5)])
+def test_gram_matrices(dtype, tol):
+  rng = np.random.default_rng(2)
+  M = rng.uniform(-0.5, 1.0, (3, 2, 70, 40)).astype(dtype)
+  got = ksig.algorithms.global_align_kern_log(to_dev(M)).cpu().numpy()
+  ref = O.global_align_kern_log(M.astype(np.float64))
+  assert got.shape == (3, 2) and _logerr(got, ref) < tol
+  got3 = ksig.algorithms.global_align_kern_log(to_dev(M[0])).cpu().numpy()
+  assert got3.shape == (2,) and _logerr(got3, ref[0]) < tol
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-12), (np.float32, 2e-7)])
+def test_delannoy_closed_form(dtype, tol):
+  X = np.ones((1, 300, 4), dtype=dtype) * 0.25
+  Y = np.ones((1, 130, 4), dtype=dtype) * 0.25
+  got = _ops.global_align_log(_ops.SeqSet.dense(to_dev(X)), _ops.SeqSet.dense(to_dev(Y)),
+                              _ops.dense_pairs(1, 1, dev()), _lib.STATIC_RBF, 1.0).item()
+  ref = math.log(O.delannoy(299, 129))
+  assert abs(got - ref) < tol * ref
+
+
+def test_ragged_pairs_match_per_pair_oracle():
+  rng = np.random.default_rng(9)
+  lens_x = [1, 2, 40, 64, 65, 129, 200]
+  lens_y = [1, 3, 63, 64, 100]
+  xs = [O.brownian(rng, 1, L, 6)[0] for L in lens_x]
+  ys = [O.brownian(rng, 1, L, 6)[0] for L in lens_y]
+  X = _ops.SeqSet.ragged([to_dev(s) for s in xs])
+  Y = _ops.SeqSet.ragged([to_dev(s) for s in ys])
+  pr = np.array([(a, b) for a in range(len(xs)) for b in range(len(ys))], dtype=np.int32)
+  got = _ops.global_align_log(X, Y, to_dev(pr), _lib.STATIC_RBF, 0.8).cpu().numpy()
+  ref = np.array([O.pair_gak_log(xs[a], ys[b], 'rbf', 0.8) for a, b in pr])
+  assert _logerr(got, ref) < 1e-11
+
+
+@pytest.mark.parametrize("static", ["rbf", "linear"])
+def test_class_api(static):
+  rng = np.random.default_rng(10)
+  X = O.brownian(rng, 7, 40, 4).astype(np.float32)
+  Y = O.brownian(rng, 5, 33, 4).astype(np.float32)
+  if static == "rbf":
+    sk, kind, param = ksig.static.kernels.RBFKernel(math.sqrt(4)), 'rbf', 2.0
+  else:
+    sk, kind, param = ksig.static.kernels.LinearKernel(0.2), 'linear', 0.2
+  gak = ksig.kernels.GlobalAlignmentKernel(static_kernel=sk)
+  assert gak.normalize and gak.is_log_space
+  K = gak(X)
+  ref = O.global_alignment_kernel(X.astype(np.float64), kind=kind, param=param)
+  assert K.shape == (7, 7) and K.dtype == np.float32 and normwise(K, ref) < 1e-5
+  Kxy = gak(X, Y)
+  refxy = O.global_alignment_kernel(X.astype(np.float64), Y.astype(np.float64),
+                                    kind=kind, param=param)
+  assert Kxy.shape == (7, 5) and normwise(Kxy, refxy) < 1e-5
+  assert np.all(gak(X, diag=True) == 1.0)
+  Kg = gak(to_dev(X), return_on_gpu=True)
+  assert isinstance(Kg, torch.Tensor) and Kg.is_cuda
+  assert gak.get_params()["static_kernel"] is sk
+
+
+def test_generic_static_kernel_goes_through_matrices():
+  """A static kernel the sweep does not fuse: its Gram is evaluated and fed
+  to the matrix entry point."""
+
+  class HalfRBF(ksig.static.kernels.RBFKernel):
+    def _K(self, X, Y=None):
+      return 0.5 * super()._K(X, Y)
+
+  rng = np.random.default_rng(11)
+  X = O.brownian(rng, 4, 20, 3)
+  gak = ksig.kernels.GlobalAlignmentKernel(static_kernel=HalfRBF(1.0))
+  K = gak(X)
+  M = 0.5 * O.pairwise_gram(X, None, 'rbf', 1.0)
+  logK = O.global_align_kern_log(M)
+  dg = np.diagonal(logK)
+  ref = np.exp(logK - .5 * (dg[:, None] + dg[None, :]))
+  assert normwise(K, ref) < 1e-11
