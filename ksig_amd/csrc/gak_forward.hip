@@ -1,4 +1,4 @@
-// Global alignment kernel in log space, forward pass (gfx950).
+// Global alignment kernel (log of the value), forward pass (gfx950).
 //
 // Computes log GA(x, y) of ksig/algorithms.py:514-559 (launcher) and
 // :453-511 (device recursion): with the transformed static kernel
@@ -10,24 +10,29 @@
 //
 // The reference sweeps antidiagonals with two block barriers per step and a
 // three-slot LDS ring per antidiagonal element.  Here the same recurrence is
-// a skewed systolic wave (as the signature-PDE forward): lane i owns row
-// 64*band + i, walks the columns one step behind lane i-1 and receives
-// logK(r-1, c) from it through DPP wave_shr:1; lane 63's values go to an LDS
-// FIFO (one slot per column) that lane 0 reads in the next band.  No barrier
+// a skewed systolic wave (as the DP row stream): lane i owns R consecutive
+// rows of each 64R-row band, walks the columns one step behind lane i-1 and
+// receives K(r-1, c) of the row above its first row from it through DPP
+// wave_shr:1; the R cells of a step share the column's y row and the step's
+// control work.  Lane 63's last row goes to an LDS FIFO (one slot per
+// column) that lane 0 reads in the next band.  No barrier
 // inside the sweep.  The longer sequence is the row side (GA is symmetric
 // under transposition: the alignment paths and M' transpose together).
 //
-// Precision: the log values grow like the path length, so they are carried
-// in float64 (full rate on the vector ALUs of gfx950); for float32 inputs the
-// transcendentals (exp / log of the shifted LSE terms, of the static kernel)
-// run on the float32 hardware units (v_exp_f32 / v_log_f32), whose results
-// are O(1) and enter the float64 sum with absolute error ~1e-7 per cell.
-// float64 inputs use float64 exp / log throughout.
-//
-// log(0) is represented by kGakNeg = -1e300: every real cell has at least one
-// finite predecessor, and exp(kGakNeg - m) underflows to exactly 0 for any
-// finite m, while cells outside the grid (all three inputs kGakNeg) stay
-// finite instead of producing NaN.
+// Representation: the sum over alignments is carried in exp space as a
+// float64 mantissa and an int32 power-of-two exponent, K = v * 2^e, so the
+// cell update is the exp-space form of the same recurrence,
+//   K(i,j) = M'(i,j) (K(i-1,j) + K(i,j-1) + K(i-1,j-1)),
+// with the three terms aligned to the largest exponent by ldexp (exact) and
+// the product renormalised by frexp (exact): one float64 rounding per add and
+// per multiply, no exp / log per cell (the log-sum-exp form needs three exp
+// and a log), and no overflow or underflow at any length because the
+// exponent absorbs the scale.  A term whose exponent lies more than 1074
+// below the largest one flushes to zero, exactly as exp(a - m) underflows in
+// the log-sum-exp.  The result is log v + e ln 2 at the last cell.  The
+// static-kernel value and M' = max(M / (2 - M), 1e-12) are computed in the
+// input precision (float32 inputs: v_exp_f32 and v_rcp_f32, ~1 ulp each).
+// An empty sum (log 0) is v = 0 with exponent kGakZeroExp.
 
 #include <math.h>
 
@@ -37,60 +42,46 @@
 
 namespace ksig {
 
-constexpr double kGakNeg = -1e300;
+constexpr int kGakZeroExp = -(1 << 29);
 constexpr double kLn2 = 0.69314718055994530942;
-constexpr double kLog2e = 1.44269504088896340736;
 
-// log(max(M/(2-M), 1e-12)) from the static-kernel value M (linear, Gram).
-__device__ __forceinline__ double gak_logm_direct(float m) {
-  const float mp = m / (2.0f - m);
-  return (double)(__builtin_amdgcn_logf(fmaxf(mp, 1e-12f)) * (float)kLn2);
+// M' = max(M / (2 - M), 1e-12) from the static-kernel value M.
+__device__ __forceinline__ double gak_mprime(float m) {
+  const float mp = m * __builtin_amdgcn_rcpf(2.0f - m);
+  return (double)fmaxf(mp, 1e-12f);
 }
-__device__ __forceinline__ double gak_logm_direct(double m) {
-  return log(fmax(m / (2.0 - m), 1e-12));
-}
-
-// The same for RBF from q = |x' - y'|^2 with M = 2^-q (scales folded in):
-// log M' = -q ln 2 - log(2 - M), clamped at log(1e-12).
-__device__ __forceinline__ double gak_logm_rbf(float q) {
-  const float m = __builtin_amdgcn_exp2f(-q);
-  const float lg = -q * (float)kLn2 - __builtin_amdgcn_logf(2.0f - m) * (float)kLn2;
-  return (double)fmaxf(lg, -27.631021115928547f);
-}
-__device__ __forceinline__ double gak_logm_rbf(double q) {
-  const double m = exp2(-q);
-  return fmax(-q * kLn2 - log(2.0 - m), -27.631021115928547);
+__device__ __forceinline__ double gak_mprime(double m) {
+  return fmax(m / (2.0 - m), 1e-12);
 }
 
-// log(e^a + e^b + e^c) with m = max(a, b, c) shifted out.
-template <typename T>
-__device__ __forceinline__ double gak_lse3(double a, double b, double c) {
-  const double m = fmax(fmax(a, b), c);
-  if constexpr (sizeof(T) == 4) {
-    const float s = __builtin_amdgcn_exp2f((float)((a - m) * kLog2e)) +
-                    __builtin_amdgcn_exp2f((float)((b - m) * kLog2e)) +
-                    __builtin_amdgcn_exp2f((float)((c - m) * kLog2e));
-    return m + (double)(__builtin_amdgcn_logf(s) * (float)kLn2);
-  } else {
-    return m + log(exp(a - m) + exp(b - m) + exp(c - m));
-  }
+// One cell: M' * (a + b + c) for a = va 2^ea etc., renormalised to a
+// mantissa in [0.5, 1) (or 0) and its exponent.
+__device__ __forceinline__ void gak_cell(double mp, double va, int ea, double vb, int eb,
+                                         double vc, int ec, double& v, int& e) {
+  const int m = max(max(ea, eb), ec);
+  const double s = __builtin_amdgcn_ldexp(va, ea - m) + __builtin_amdgcn_ldexp(vb, eb - m) +
+                   __builtin_amdgcn_ldexp(vc, ec - m);
+  const double p = s * mp;
+  v = __builtin_amdgcn_frexp_mant(p);
+  e = m + __builtin_amdgcn_frexp_exp(p);
 }
 
-template <typename T, int DMAX>
+template <typename T, int DMAX, int KIND, int R>
 __global__ void __launch_bounds__(kWave)
 gak_forward_kernel(const GakArgs a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
+  constexpr bool kGramIn = KIND == kGram;
+  constexpr int BR = kWave * R;  // rows per band
 
   const int lane = threadIdx.x;
   const int64_t pr = blockIdx.x;
-  const int kind = a.kind;
   const int d = a.d;
 
   const T* xs = nullptr;
   const T* ys = nullptr;
   const T* gm = nullptr;
   int Lx, Ly;
-  if (kind == kGram) {
+  if constexpr (kGramIn) {
     Lx = a.gram_lx;
     Ly = a.gram_ly;
     gm = reinterpret_cast<const T*>(a.gram) + pr * (int64_t)Lx * Ly;
@@ -110,15 +101,19 @@ gak_forward_kernel(const GakArgs a) {
     if (lane == 0) a.out[pr] = -INFINITY;  // empty sum of alignments
     return;
   }
-  const int Pc = Ly < kWave ? kWave : Ly;
-  const int nb = (Lx + kWave - 1) / kWave;
+  // Columns per band pass; more than 64 so that lane 0's FIFO read, issued
+  // one step ahead, comes at least a step after lane 63's write of the slot.
+  const int Pc = Ly <= kWave ? kWave + 1 : Ly;
+  const int nb = (Lx + BR - 1) / BR;
 
-  // LDS: FIFO [Pc] doubles, then y rows [Ly][DMAX] (scaled) and norms [Ly].
-  double* fifo = reinterpret_cast<double*>(smem_raw);
-  T* ylds = reinterpret_cast<T*>(fifo + Pc);
-  T* ynrm = ylds + (kind == kGram ? 0 : Ly * DMAX);
+  // LDS: FIFO mantissas [Pc] (double) and exponents [Pc] (int), then y rows
+  // [Ly][DMAX] (scaled) and norms [Ly].
+  double* fifo_v = reinterpret_cast<double*>(smem_raw);
+  int* fifo_e = reinterpret_cast<int*>(fifo_v + Pc);
+  T* ylds = reinterpret_cast<T*>(fifo_e + ((Pc + 3) & ~3));
+  T* ynrm = ylds + (kGramIn ? 0 : Ly * DMAX);
   const T xscale = T(a.xscale), yscale = T(a.yscale);
-  if (kind != kGram) {
+  if constexpr (!kGramIn) {
     for (int j = lane; j < Ly; j += kWave) {
       T n2 = T(0);
 #pragma unroll
@@ -132,77 +127,144 @@ gak_forward_kernel(const GakArgs a) {
   }
   __syncthreads();
 
-  auto load_x = [&](T* xv, T& xnrm, int xr) {
-    const bool ok = xr < Lx && kind != kGram;
-    T n2 = T(0);
+  // this lane's R rows of band `b`: x rows scaled, with their norms
+  T xv[R][DMAX], xn[R];
+  auto load_rows = [&](int b) __attribute__((always_inline)) {
 #pragma unroll
-    for (int k = 0; k < DMAX; ++k) {
-      const T v = (ok && k < d) ? xs[(int64_t)xr * d + k] * xscale : T(0);
-      xv[k] = v;
-      n2 += v * v;
+    for (int q = 0; q < R; ++q) {
+      const int xr = b * BR + lane * R + q;
+      const bool ok = xr < Lx && !kGramIn;
+      T n2 = T(0);
+#pragma unroll
+      for (int k = 0; k < DMAX; ++k) {
+        const T v = (ok && k < d) ? xs[(int64_t)xr * d + k] * xscale : T(0);
+        xv[q][k] = v;
+        n2 += v * v;
+      }
+      xn[q] = n2;
     }
-    xnrm = n2;
   };
-  // log M'(r, c) of this lane's row r (x row in registers) and column c.
-  auto logm = [&](const T* xv, T xnrm, int r, int c) -> double {
-    if (kind == kGram) return gak_logm_direct(gm[(int64_t)r * Ly + c]);
-    const T* yr = ylds + c * DMAX;
-    T dot = T(0);
+
+  // Inputs of one step, read a step ahead: the column's y row and norm (or
+  // the caller's matrix values of the lane's rows), and lane 0's FIFO entry.
+  struct StepIn {
+    T y[kGramIn ? R : DMAX];
+    T yn;
+    double fv;
+    int fe;
+  };
+  auto fetch = [&](StepIn& in, int c, int b) __attribute__((always_inline)) {
+    const int cy = c < 0 ? 0 : (c >= Ly ? Ly - 1 : c);
+    if constexpr (kGramIn) {
 #pragma unroll
-    for (int k = 0; k < DMAX; ++k) dot += xv[k] * yr[k];
-    if (kind == kRBF) return gak_logm_rbf(xnrm + ynrm[c] - T(2) * dot);
-    return gak_logm_direct(dot);
+      for (int q = 0; q < R; ++q) {
+        int rr = b * BR + lane * R + q;
+        rr = rr < Lx ? rr : Lx - 1;
+        in.y[q] = gm[(int64_t)rr * Ly + cy];
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < DMAX; ++k) in.y[k] = ylds[cy * DMAX + k];
+      in.yn = ynrm[cy];
+    }
+    in.fv = 0.0;
+    in.fe = kGakZeroExp;
+    if (lane == 0 && b > 0 && b < nb && c >= 0) {
+      in.fv = fifo_v[c];
+      in.fe = fifo_e[c];
+    }
+  };
+  // M' of row q of this lane and the fetched column.
+  auto mprime = [&](int q, const StepIn& in) __attribute__((always_inline)) -> double {
+    if constexpr (kGramIn) {
+      return gak_mprime(in.y[q]);
+    } else {
+      T dot = T(0);
+#pragma unroll
+      for (int k = 0; k < DMAX; ++k) dot += xv[q][k] * in.y[k];
+      if constexpr (KIND == kRBF) return gak_mprime(fast_exp2(-(xn[q] + in.yn - T(2) * dot)));
+      else return gak_mprime(dot);
+    }
   };
 
   int band = 0;
-  int r = lane;
-  T xv[DMAX], xnrm;
-  load_x(xv, xnrm, r);
-  T xv2[DMAX], xnrm2;  // next band's row, prefetched
-  load_x(xv2, xnrm2, r + kWave);
-
-  double up = kGakNeg;     // logK(r, c) flowing down to row r+1
-  double left = kGakNeg;   // logK(r, c-1)
-  double diag = kGakNeg;   // logK(r-1, c-1)
+  load_rows(0);
+  // K of this lane's rows at column c-1, and K of the row above at c-1
+  double lv[R];
+  int le[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    lv[q] = 0.0;
+    le[q] = kGakZeroExp;
+  }
+  double pin_v = 0.0, up_v = 0.0;
+  int pin_e = kGakZeroExp, up_e = kGakZeroExp;
   int cc = -lane;
 
+  StepIn cur;
+  fetch(cur, cc, band);
   const int total = nb * Pc + kWave - 1;
   for (int step = 0; step < total; ++step) {
     const bool active = cc >= 0 && band < nb;
-    double fin = kGakNeg;  // row above the band: row -1 (band 0) or the FIFO
-    if (lane == 0 && active && band > 0) fin = fifo[cc];
-    const double in_up = shift_down(up, fin);
-
-    const bool cell = active && cc < Ly && r < Lx;
-    double val = kGakNeg;
-    if (cell) {
-      const double dg = cc == 0 ? (r == 0 ? 0.0 : kGakNeg) : diag;
-      const double lf = cc == 0 ? kGakNeg : left;
-      val = logm(xv, xnrm, r, cc) + gak_lse3<T>(in_up, lf, dg);
-      if (r == Lx - 1 && cc == Ly - 1) a.out[pr] = val;
+    const bool colok = active && cc < Ly;
+    const bool c0 = cc == 0;
+    // row above the lane's first row: lane i-1's last row, or the FIFO
+    // (row -1 in band 0: empty sums)
+    const double in_v = shift_down(up_v, cur.fv);
+    const int in_e = __builtin_amdgcn_update_dpp(cur.fe, up_e, 0x138, 0xf, 0xf, false);
+    double mp[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) mp[q] = mprime(q, cur);
+    // next step's inputs, in flight while this step computes
+    {
+      const bool wrap = cc + 1 == Pc;
+      fetch(cur, wrap ? 0 : cc + 1, band + (wrap ? 1 : 0));
     }
-    if (lane == kWave - 1 && active) fifo[cc] = val;
-    up = val;
-    left = val;
-    diag = in_up;
+
+    const int row0 = band * BR + lane * R;
+    double uv = in_v, dv = pin_v;   // K(row-1, c), K(row-1, c-1)
+    int ue = in_e, de = pin_e;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int row = row0 + q;
+      // column -1: empty sums, except K(-1, -1) = 1 above-left of (0, 0)
+      const double dq = c0 ? (row == 0 ? 1.0 : 0.0) : dv;
+      const int dqe = c0 ? (row == 0 ? 0 : kGakZeroExp) : de;
+      double v;
+      int e;
+      gak_cell(mp[q], uv, ue, c0 ? 0.0 : lv[q], c0 ? kGakZeroExp : le[q], dq, dqe, v, e);
+      const bool cell = colok && row < Lx;
+      v = cell ? v : 0.0;
+      e = cell ? e : kGakZeroExp;
+      if (cell && row == Lx - 1 && cc == Ly - 1) a.out[pr] = log(v) + (double)e * kLn2;
+      dv = lv[q];
+      de = le[q];
+      lv[q] = v;
+      le[q] = e;
+      uv = v;
+      ue = e;
+    }
+    if (lane == kWave - 1 && active) {
+      fifo_v[cc] = uv;
+      fifo_e[cc] = ue;
+    }
+    up_v = uv;
+    up_e = ue;
+    pin_v = in_v;
+    pin_e = in_e;
 
     ++cc;
     if (cc == Pc) {  // wrap to the next band
       cc = 0;
       ++band;
-      r += kWave;
-#pragma unroll
-      for (int k = 0; k < DMAX; ++k) xv[k] = xv2[k];
-      xnrm = xnrm2;
-      load_x(xv2, xnrm2, r + kWave);
-      left = kGakNeg;
+      if (band < nb) load_rows(band);
     }
   }
 }
 
-template <typename T, int DMAX>
+template <typename T, int DMAX, int KIND, int R>
 static hipError_t launch_gak(const GakArgs& a, size_t lds, hipStream_t s) {
-  auto k = gak_forward_kernel<T, DMAX>;
+  auto k = gak_forward_kernel<T, DMAX, KIND, R>;
   if (lds > 64 * 1024) {
     hipError_t e = hipFuncSetAttribute((const void*)k,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -212,19 +274,29 @@ static hipError_t launch_gak(const GakArgs& a, size_t lds, hipStream_t s) {
   return hipGetLastError();
 }
 
+// Rows per lane from register pressure: float32 4 (d <= 16) or 2, float64 2
+// (d <= 8) or 1.
+template <typename T, int KIND>
+static hipError_t dispatch_gak_d(const GakArgs& a, size_t lds, hipStream_t s) {
+  constexpr bool f32 = sizeof(T) == 4;
+  if (a.d <= 8) return launch_gak<T, 8, KIND, f32 ? 4 : 2>(a, lds, s);
+  if (a.d <= 16) return launch_gak<T, 16, KIND, f32 ? 4 : 1>(a, lds, s);
+  return launch_gak<T, 32, KIND, f32 ? 2 : 1>(a, lds, s);
+}
+
 template <typename T>
 static hipError_t dispatch_gak(const GakArgs& a, size_t lds, hipStream_t s) {
-  if (a.kind == kGram || a.d <= 8) return launch_gak<T, 8>(a, lds, s);
-  if (a.d <= 16) return launch_gak<T, 16>(a, lds, s);
-  return launch_gak<T, 32>(a, lds, s);
+  if (a.kind == kGram) return launch_gak<T, 8, kGram, sizeof(T) == 4 ? 4 : 2>(a, lds, s);
+  if (a.kind == kRBF) return dispatch_gak_d<T, kRBF>(a, lds, s);
+  return dispatch_gak_d<T, kLinear>(a, lds, s);
 }
 
 size_t gak_lds_bytes(int dtype, int kind, int d, int cols) {
   const size_t esz = dtype == KSIG_F32 ? 4 : 8;
   const int dmax = d <= 8 ? 8 : (d <= 16 ? 16 : 32);
-  const size_t pc = cols < kWave ? kWave : cols;
+  const size_t pc = cols <= kWave ? kWave + 1 : cols;
   const size_t ycols = kind == kGram ? 0 : (size_t)cols;
-  return pc * 8 + (ycols * dmax + ycols) * esz;
+  return pc * 8 + ((pc + 3) & ~(size_t)3) * 4 + (ycols * dmax + ycols) * esz;
 }
 
 hipError_t gak_forward_launch(const GakArgs& a, int dtype, size_t lds, hipStream_t s) {
