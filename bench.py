@@ -33,11 +33,13 @@ sys.path.insert(0, ROOT)
 PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 (vector and matrix)
 PEAK_HBM_GBS = 8000.0
 
-# HBM bytes per launch of the row-stream kernel at the default workload sizes,
+# HBM bytes per launch of the dominant kernel at the default workload sizes,
 # from two rocprofv3 --pmc passes (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the
 # factors an elementwise copy of 1 GiB reads on the same counters); recorded
-# with the kernel version they were measured on.
-MEASURED_TRAFFIC_BYTES = {"c1": 38658 * 1024, "c3": (663993 * 2 + 580160) * 1024}
+# with the kernel version and schedule they were measured on (c3: the
+# autotuned schedule, 2738 waves).
+MEASURED_TRAFFIC_BYTES = {"c1": 38658 * 1024, "c3": (663693 * 2 + 581215) * 1024,
+                          "gak": (21130 * 2 + 4104) * 1024}
 
 
 def brownian(gen: torch.Generator, n, L, d, device, dtype=torch.float32):
@@ -414,8 +416,10 @@ def bench_gak(args):
       "max_abs_log_err_vs_fp64": err,
       "err_sample": "16 random pairs, |log GA - log GA_64| / max(1, |log GA_64|)",
       "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
-                   "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                   "kernel": "gak_forward_kernel<float,8>", "kernel_ms": kern_ms,
+                   "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+                   "traffic": MEASURED_TRAFFIC_BYTES["gak"],
+                   "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE)",
+                   "kernel": "gak_forward_kernel<float,8,RBF,4>", "kernel_ms": kern_ms,
                    "flops_per_cell": flops_cell},
   }
   if not args.no_plain:
