@@ -333,6 +333,34 @@ int ksig_global_align_log(int dtype, const void* x, const int64_t* x_off,
 int ksig_global_align_log_gram(int dtype, const void* M, int64_t n_mats, int lx, int ly,
                                double* out, void* stream);
 
+/*
+ * Random Warping Series: the DTW value of every sequence a against every
+ * warping series b, out[a * n_warps + b] (float64), with the squared
+ * Euclidean distances computed in the kernel.  Replaces the device kernel
+ * _random_warping_series_dtw behind random_warping_series(D, warp_lens)
+ * (ksig/algorithms.py:565-659) as called by
+ * RandomWarpingSeries._compute_features (ksig/kernels.py:459-477), including
+ * the distance tensor D of ksig/kernels.py:466.  warp: the series' points
+ * [warp_rows, d] (dtype), warp_off: [n_warps + 1] row offsets (non-decreasing,
+ * warp_off[0] = 0, warp_off[n_warps] = warp_rows), groups: [n_groups + 1]
+ * non-decreasing split points of the series from 0 to n_warps; one wave runs
+ * one sequence against one group of consecutive series.  All device arrays.
+ */
+int ksig_random_warping_series(int dtype, const void* x, const int64_t* x_off,
+                               const int32_t* x_len, int max_len_x, int64_t n_seq, int d,
+                               const void* warp, const int32_t* warp_off, int n_warps,
+                               int warp_rows, const int32_t* groups, int n_groups,
+                               double* out, void* stream);
+
+/*
+ * The same on caller-supplied distances D [n_seq, lx, warp_rows] (dtype):
+ * random_warping_series(D, warp_lens) of ksig/algorithms.py:630-659.
+ */
+int ksig_random_warping_series_gram(int dtype, const void* D, int64_t n_seq, int lx,
+                                    int warp_rows, const int32_t* warp_off, int n_warps,
+                                    const int32_t* groups, int n_groups, double* out,
+                                    void* stream);
+
 #ifdef __cplusplus
 }
 #endif

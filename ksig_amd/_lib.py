@@ -73,6 +73,11 @@ SIGNATURES = {
         _c_i64, _c_int, _c_dbl, _vp, _vp]),
     "ksig_global_align_log_gram": (_c_int, [
         _c_int, _vp, _c_i64, _c_int, _c_int, _vp, _vp]),
+    "ksig_random_warping_series": (_c_int, [
+        _c_int, _vp, _vp, _vp, _c_int, _c_i64, _c_int, _vp, _vp, _c_int, _c_int, _vp,
+        _c_int, _vp, _vp]),
+    "ksig_random_warping_series_gram": (_c_int, [
+        _c_int, _vp, _c_i64, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _vp]),
 }
 
 

@@ -356,3 +356,69 @@ def global_align_log_gram(M: torch.Tensor) -> torch.Tensor:
       _lib.dtype_code(Mc.dtype), Mc.data_ptr(), n, lx, ly, out.data_ptr(),
       _lib.stream_handle(M.device)))
   return out
+
+
+def rws_groups(warp_lens: np.ndarray, n_seq: int, target_waves: int = 4096) -> np.ndarray:
+  """Split points [n_groups + 1] of the warping series into groups of about
+  equal rows: enough groups that n_seq x n_groups waves fill the chip, but
+  at least one 64-row band of rows per group where there are enough rows."""
+  lens = np.asarray(warp_lens, dtype=np.int64)
+  n_w = lens.shape[0]
+  total = int(lens.sum())
+  if n_w == 0:
+    return np.zeros(1, dtype=np.int32)
+  g = -(-int(target_waves) // max(int(n_seq), 1))
+  g = max(1, min(g, n_w, max(1, total // 64)))
+  cum = np.cumsum(lens)
+  cuts = np.searchsorted(cum, total * np.arange(1, g) / g, side="left") + 1
+  b = np.unique(np.concatenate([[0], np.clip(cuts, 1, n_w - 1) if g > 1 else [], [n_w]]))
+  return b.astype(np.int32)
+
+
+def random_warping_series(X: SeqSet, warp: torch.Tensor, warp_lens: np.ndarray
+                          ) -> torch.Tensor:
+  """DTW values [n_seq, n_warps] (float64) of every sequence against every
+  warping series (points ``warp`` [sum of lengths, d], lengths ``warp_lens``)."""
+  lib = _lib.load()
+  if warp.dtype != X.data.dtype or warp.device != X.data.device:
+    raise ValueError("warping series must share the sequences' dtype and device")
+  if warp.shape[-1] != X.d:
+    raise ValueError("warping series must have the sequences' number of features")
+  lens = np.asarray(warp_lens, dtype=np.int64)
+  n_w = lens.shape[0]
+  dev = X.data.device
+  out = torch.empty((X.n, n_w), device=dev, dtype=torch.float64)
+  if X.n == 0 or n_w == 0:
+    return out
+  off = torch.as_tensor(np.concatenate([[0], np.cumsum(lens)]).astype(np.int32), device=dev)
+  groups = rws_groups(lens, X.n)
+  g = torch.as_tensor(groups, device=dev)
+  w = warp.contiguous()
+  _lib.check(lib.ksig_random_warping_series(
+      _lib.dtype_code(X.data.dtype), X.data.data_ptr(), X.off.data_ptr(), X.len.data_ptr(),
+      X.max_len, X.n, X.d, w.data_ptr(), off.data_ptr(), n_w, int(lens.sum()),
+      g.data_ptr(), len(groups) - 1, out.data_ptr(), _lib.stream_handle(dev)))
+  return out
+
+
+def random_warping_series_gram(D: torch.Tensor, warp_lens: np.ndarray) -> torch.Tensor:
+  """DTW values [n_seq, n_warps] (float64) from distances D [n_seq, lx, sum of
+  lengths]."""
+  lib = _lib.load()
+  lens = np.asarray(warp_lens, dtype=np.int64)
+  n_w = lens.shape[0]
+  n, lx, W = D.shape
+  if W != int(lens.sum()):
+    raise ValueError("the last axis of D must be the total length of the warping series")
+  dev = D.device
+  out = torch.empty((n, n_w), device=dev, dtype=torch.float64)
+  if n == 0 or n_w == 0:
+    return out
+  Dc = D.contiguous()
+  off = torch.as_tensor(np.concatenate([[0], np.cumsum(lens)]).astype(np.int32), device=dev)
+  groups = rws_groups(lens, n)
+  g = torch.as_tensor(groups, device=dev)
+  _lib.check(lib.ksig_random_warping_series_gram(
+      _lib.dtype_code(Dc.dtype), Dc.data_ptr(), n, lx, W, off.data_ptr(), n_w,
+      g.data_ptr(), len(groups) - 1, out.data_ptr(), _lib.stream_handle(dev)))
+  return out

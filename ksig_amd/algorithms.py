@@ -78,3 +78,20 @@ def global_align_kern_log(M) -> torch.Tensor:
   if is_diag:
     K = K[:, 0]
   return K
+
+
+def random_warping_series(D, warp_lens) -> torch.Tensor:
+  """Random Warping Series DTW values (ksig/algorithms.py:630-659).
+
+  Args:
+    D: squared distances of shape [n_X, l_X, sum of warp lengths].
+    warp_lens: lengths of the warping series, shape [n_Y].
+
+  Returns:
+    [n_X, n_Y], dtype of D, on the GPU.
+  """
+  D = utils.as_device(D)
+  if D.ndim != 3:
+    raise ValueError('`D` distances array must have `.ndim==3`.')
+  lens = warp_lens.detach().cpu().numpy() if isinstance(warp_lens, torch.Tensor) else warp_lens
+  return _ops.random_warping_series_gram(D, lens).to(D.dtype)

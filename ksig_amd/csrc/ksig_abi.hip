@@ -13,6 +13,7 @@
 #include "dp_forward.h"
 #include "dp_stream.h"
 #include "gak.h"
+#include "rws.h"
 
 namespace {
 
@@ -826,6 +827,67 @@ int ksig_global_align_log_gram(int dtype, const void* M, int64_t n_mats, int lx,
   a.gram_ly = ly;
   a.out = out;
   return gak_common(dtype, a, ly < 1 ? 1 : ly, stream);
+}
+
+static int rws_common(int dtype, ksig::RwsArgs& a, int lx, void* stream) {
+  if (a.n_seq < 0 || a.n_seq * (int64_t)(a.n_groups > 0 ? a.n_groups : 1) > 0x7fffffffLL)
+    return fail(KSIG_EINVAL, "bad n_seq");
+  if (a.n_warps < 0 || a.warp_rows < 0) return fail(KSIG_EINVAL, "negative warp counts");
+  if (a.n_seq == 0 || a.n_warps == 0) return 0;
+  if (a.n_groups < 1) return fail(KSIG_EINVAL, "n_groups must be positive");
+  if (!a.warp_off || !a.groups || !a.out) return fail(KSIG_EINVAL, "null device pointer");
+  const size_t lds = ksig::rws_lds_bytes(dtype, a.gram != nullptr, a.d, lx < 1 ? 1 : lx);
+  if (lds > max_lds_bytes())
+    return fail(KSIG_ELDS, "per-wave LDS state of %zu bytes exceeds the device limit", lds);
+  hipError_t e = ksig::rws_forward_launch(a, dtype, lds, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "rws_forward launch");
+  return 0;
+}
+
+int ksig_random_warping_series(int dtype, const void* x, const int64_t* x_off,
+                               const int32_t* x_len, int max_len_x, int64_t n_seq, int d,
+                               const void* warp, const int32_t* warp_off, int n_warps,
+                               int warp_rows, const int32_t* groups, int n_groups,
+                               double* out, void* stream) {
+  if (dtype != KSIG_F32 && dtype != KSIG_F64) return fail(KSIG_EINVAL, "bad dtype %d", dtype);
+  if (d < 1) return fail(KSIG_EINVAL, "d must be >= 1 (got %d)", d);
+  if (d > 32) return fail(KSIG_EUNSUPPORTED, "d = %d > 32 is not built into the fused kernels", d);
+  if (max_len_x < 0) return fail(KSIG_EINVAL, "negative max length");
+  if (n_seq > 0 && n_warps > 0 && (!x || !x_off || !x_len || !warp))
+    return fail(KSIG_EINVAL, "null device pointer");
+  ksig::RwsArgs a{};
+  a.x = {x, x_off, x_len};
+  a.n_seq = n_seq;
+  a.d = d;
+  a.warp = warp;
+  a.warp_off = warp_off;
+  a.n_warps = n_warps;
+  a.warp_rows = warp_rows;
+  a.groups = groups;
+  a.n_groups = n_groups;
+  a.out = out;
+  return rws_common(dtype, a, max_len_x, stream);
+}
+
+int ksig_random_warping_series_gram(int dtype, const void* D, int64_t n_seq, int lx,
+                                    int warp_rows, const int32_t* warp_off, int n_warps,
+                                    const int32_t* groups, int n_groups, double* out,
+                                    void* stream) {
+  if (dtype != KSIG_F32 && dtype != KSIG_F64) return fail(KSIG_EINVAL, "bad dtype %d", dtype);
+  if (lx < 0) return fail(KSIG_EINVAL, "negative matrix shape");
+  if (n_seq > 0 && n_warps > 0 && !D) return fail(KSIG_EINVAL, "null device pointer");
+  ksig::RwsArgs a{};
+  a.n_seq = n_seq;
+  a.d = 1;
+  a.warp_off = warp_off;
+  a.n_warps = n_warps;
+  a.warp_rows = warp_rows;
+  a.groups = groups;
+  a.n_groups = n_groups;
+  a.gram = D;
+  a.gram_lx = lx;
+  a.out = out;
+  return rws_common(dtype, a, lx, stream);
 }
 
 }  // extern "C"

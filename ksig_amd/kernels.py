@@ -18,6 +18,7 @@ from typing import Optional
 import torch
 
 from . import _lib, _ops, algorithms, static, utils
+import numpy as np
 from .static.kernels import Kernel, StaticKernel
 from .utils import _EPS
 
@@ -298,3 +299,47 @@ class GlobalAlignmentKernel(SignatureKernel):
       Kyd = self._compute_kernel(Y, diag=True)
       K = K - 0.5 * (Kxd[..., :, None] + Kyd[..., None, :])
     return torch.exp(K).to(X.dtype)
+
+
+class RandomWarpingSeries(static.features.KernelFeatures):
+  """Normalised Random Warping Series features (ksig/kernels.py:413-477):
+  the DTW value of a sequence against each of ``n_components`` random
+  warping series (Gaussian points around the data's per-channel mean and
+  spread, lengths uniform in 1..max_warp), normalised to unit norm.  The
+  random series are drawn on the host with NumPy (the reference draws them
+  with CuPy, so equal seeds give different series); the DTW values for given
+  series follow ksig/algorithms.py:565-659 exactly."""
+
+  def __init__(self, n_components: int = 100, stdev: float = 1.,
+               max_warp: int = 32, normalize: bool = True,
+               n_features: Optional[int] = None, random_state=None):
+    self.n_components = utils.check_positive_value(n_components, 'n_components')
+    self.stdev = utils.check_positive_value(stdev, 'stdev')
+    self.max_warp = utils.check_positive_value(max_warp, 'max_warp')
+    self.normalize = normalize
+    self.n_features = (utils.check_positive_value(n_features, 'n_features')
+                       if n_features is not None else None)
+    self.random_state = static.features.check_random_state(random_state)
+
+  def _make_feature_components(self, X):
+    """ksig/kernels.py:443-457."""
+    Xf = X.reshape(-1, X.shape[-1])
+    self.data_means_ = torch.mean(Xf, 0)
+    self.data_stdevs_ = torch.std(Xf, 0, unbiased=False)
+    self.warp_lens_ = self.random_state.randint(1, self.max_warp + 1, size=[self.n_components])
+    z = self.random_state.normal(size=[int(np.sum(self.warp_lens_)), self.n_features_])
+    self.warp_series_ = self.data_means_[None, :] + self.stdev * (
+        self.data_stdevs_[None, :] * torch.as_tensor(z, device=Xf.device, dtype=Xf.dtype))
+
+  def _compute_features(self, X) -> torch.Tensor:
+    """ksig/kernels.py:459-477: DTW against every warping series (distances
+    computed inside the sweep), then unit-norm rows."""
+    if X.ndim != 3:
+      raise ValueError('RandomWarpingSeries takes sequence arrays of ndim==3.')
+    W = self.warp_series_.to(device=X.device, dtype=X.dtype)
+    P = _ops.random_warping_series(_ops.SeqSet.dense(X.contiguous()), W,
+                                   self.warp_lens_).to(X.dtype)
+    if self.normalize:
+      nrm = utils.robust_sqrt(utils.squared_norm(P, axis=-1))
+      P = P / torch.clamp(nrm, min=_EPS)[..., None]
+    return P

@@ -1,1 +1,2 @@
 from . import kernels
+from . import features

@@ -1,0 +1,82 @@
+"""Featurized kernels: the ``KernelFeatures`` base of
+ksig/static/features.py:21-177 (fit / transform / kernel through the
+Euclidean inner product of the features), used by RandomWarpingSeries."""
+
+from __future__ import annotations
+
+from abc import ABCMeta, abstractmethod
+from typing import Optional
+
+import numpy as np
+import torch
+from sklearn.base import TransformerMixin
+from sklearn.utils.validation import check_is_fitted
+
+from .. import utils
+from .kernels import Kernel
+
+
+def check_random_state(random_state=None) -> np.random.RandomState:
+  """ksig/utils.py:38-56 with a NumPy random state in place of CuPy's (the
+  host draws the random components; the streams differ from CuPy's)."""
+  if random_state is None or isinstance(random_state, (int, np.integer)):
+    return np.random.RandomState(random_state)
+  if isinstance(random_state, np.random.RandomState):
+    return random_state
+  raise ValueError(f'{random_state} cannot seed a `numpy.random.RandomState` instance')
+
+
+class KernelFeatures(Kernel, TransformerMixin, metaclass=ABCMeta):
+  """Base class for featurized kernels (ksig/static/features.py:21-177)."""
+
+  def __init__(self, n_components: int = 100, random_state=None):
+    self.n_components = utils.check_positive_value(n_components, 'n_components')
+    self.random_state = check_random_state(random_state)
+
+  def _validate_format(self, X, reset: bool = False):
+    """ksig/static/features.py:43-59."""
+    n_features = X.shape[-1]
+    if reset or not hasattr(self, 'n_features_') or self.n_features_ is None:
+      self.n_features_ = n_features
+    elif n_features != self.n_features_:
+      raise ValueError(
+          'Data `X` has a different number of features than during fit.',
+          f'({n_features} != {self.n_features_})')
+
+  def _validate_data(self, X, reset: bool = False):
+    self._validate_format(X, reset=reset)
+    return X
+
+  @abstractmethod
+  def _make_feature_components(self, X):
+    """Initializes internal variables, called by ``fit``."""
+
+  def fit(self, X, y=None) -> 'KernelFeatures':
+    X = self._validate_data(utils.as_device(X), reset=True)
+    self._make_feature_components(X)
+    return self
+
+  @abstractmethod
+  def _compute_features(self, X: torch.Tensor) -> torch.Tensor:
+    """The feature map, called by ``transform``."""
+
+  def transform(self, X, return_on_gpu: bool = False):
+    """ksig/static/features.py:108-129."""
+    check_is_fitted(self)
+    X = utils.as_device(self._validate_data(X))
+    F = self._compute_features(X)
+    return F if return_on_gpu else utils.to_host(F)
+
+  def _K(self, X, Y=None):
+    Fx = self._compute_features(X)
+    Fy = self._compute_features(Y) if Y is not None else None
+    return utils.matrix_mult(Fx, Fy, transpose_Y=True)
+
+  def _Kdiag(self, X):
+    return utils.squared_norm(self._compute_features(X), axis=-1)
+
+  def __call__(self, X, Y: Optional[object] = None, diag: bool = False,
+               return_on_gpu: bool = False):
+    """ksig/static/features.py:157-177."""
+    check_is_fitted(self)
+    return super().__call__(X, Y=Y, diag=diag, return_on_gpu=return_on_gpu)

@@ -21,6 +21,8 @@ difference it, per-level exclusive 2-D cumulative sums):
 * global alignment kernel ... ksig/algorithms.py:453-559 (log-space
                               recursion), ksig/kernels.py:366-409 (class,
                               log-space normalisation :191-209)
+* random warping series ..... ksig/algorithms.py:565-659 (DTW recursion),
+                              ksig/kernels.py:459-477 (features)
 
 Additions that have no reference counterpart (build extensions):
 ``dyadic_order`` for the PDE (each ΔΔ cell refined into 2^λ×2^λ sub-cells
@@ -33,7 +35,8 @@ inner products, DP (any order, RBF) vs brute-force chain enumeration,
 PDE one-cell closed form, PDE dyadic refinement vs I0(2*sqrt(ab)),
 stuttering / translation invariance, symmetry and the normalised diagonal;
 GA vs explicit enumeration of alignment paths in exp space and vs Delannoy
-numbers (M' = 1).
+numbers (M' = 1); DTW vs explicit enumeration of warping paths and the
+one-point closed forms.
 """
 
 from __future__ import annotations
@@ -404,6 +407,81 @@ def gak_path_sum(Mp: np.ndarray) -> float:
 
   walk(0, 0, 1.0)
   return total
+
+
+# -----------------------------------------------------------------------------
+# Random Warping Series (ksig/algorithms.py:565-659, ksig/kernels.py:413-477)
+# -----------------------------------------------------------------------------
+
+def random_warping_series(D: np.ndarray, warp_lens) -> np.ndarray:
+  """P [n_X, n_Y] of ksig/algorithms.py:630-659 from distances D [n_X, l_X,
+  sum of warp lengths]: for each warping series b (columns warp_off[b] ..
+  warp_off[b+1]-1 of D) the dynamic time warping recursion of the device
+  kernel (:565-627),
+    P(i,j) = D(i,j) + min(P(i-1,j), P(i,j-1), P(i-1,j-1)),
+  P(-1,-1) = 0, other out-of-grid predecessors +inf, written as the plain 2-D
+  recurrence the antidiagonal sweep evaluates."""
+  D = np.asarray(D)
+  lens = np.asarray(warp_lens, dtype=np.int64)
+  n_X, l_X = D.shape[:2]
+  off = np.concatenate([[0], np.cumsum(lens)])
+  P = np.empty((n_X, len(lens)), dtype=D.dtype)
+  inf = np.full((n_X,), np.inf, dtype=D.dtype)
+  for b in range(len(lens)):
+    Db = D[:, :, off[b]:off[b + 1]]
+    l_Y = Db.shape[2]
+    prev = [inf] * l_Y
+    for i in range(l_X):
+      cur = []
+      for j in range(l_Y):
+        p01 = prev[j]
+        p10 = cur[j - 1] if j > 0 else inf
+        if i == 0 and j == 0:
+          p00 = np.zeros_like(inf)
+        elif i == 0 or j == 0:
+          p00 = inf
+        else:
+          p00 = prev[j - 1]
+        cur.append(Db[:, i, j] + np.minimum(np.minimum(p01, p10), p00))
+      prev = cur
+    P[:, b] = prev[l_Y - 1]
+  return P
+
+
+def rws_features(X, warp_series, warp_lens, normalize=True):
+  """RandomWarpingSeries._compute_features of ksig/kernels.py:459-477 for
+  given warping series: squared distances, DTW, then unit-norm rows
+  (ksig/utils.py:125-135,173-182)."""
+  D = squared_euclid_dist(X, warp_series[None])
+  P = random_warping_series(D, warp_lens)
+  if normalize:
+    nrm = np.sqrt(np.maximum(np.sum(P * P, -1), EPS))
+    P = P / np.maximum(nrm, EPS)[..., None]
+  return P
+
+
+def dtw_path_min(D: np.ndarray) -> float:
+  """DTW by explicit enumeration (tiny grids): the minimum over monotone
+  paths (0,0) -> (lX-1,lY-1) with steps (1,0), (0,1), (1,1) of the sum of D
+  on the path."""
+  lX, lY = D.shape
+  best = np.inf
+
+  def walk(i, j, acc):
+    nonlocal best
+    acc = acc + D[i, j]
+    if i == lX - 1 and j == lY - 1:
+      best = min(best, acc)
+      return
+    if i + 1 < lX:
+      walk(i + 1, j, acc)
+    if j + 1 < lY:
+      walk(i, j + 1, acc)
+    if i + 1 < lX and j + 1 < lY:
+      walk(i + 1, j + 1, acc)
+
+  walk(0, 0, 0.0)
+  return best
 
 
 # -----------------------------------------------------------------------------
