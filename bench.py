@@ -159,7 +159,7 @@ def cpu_baseline(X64: np.ndarray, n_levels: int, sigma: float, seconds: float):
 
 def main():
   ap = argparse.ArgumentParser()
-  ap.add_argument("--workload", default="c1", choices=["c1", "c2", "c3", "c4", "gak"],
+  ap.add_argument("--workload", default="c1", choices=["c1", "c2", "c3", "c4", "gak", "rws"],
                   help="c1: 512 x L=200 x d=8 symmetric Gram (the headline line); "
                        "c3: 4096 uneven-length pairs, L in [32, 1024], d=16, 8 levels, order 2; "
                        "c4: MMD^2 training step, 1024 x 1024 paths, L=128, d=32, 6 levels; "
@@ -211,6 +211,8 @@ def main():
     return bench_c2(args)
   if args.workload == "gak":
     return bench_gak(args)
+  if args.workload == "rws":
+    return bench_rws(args)
 
   import ksig_amd
   from ksig_amd import _lib, _ops
@@ -462,6 +464,131 @@ def bench_gak(args):
                            "sample": f"{done} random pairs of the workload, FP64 NumPy "
                                      f"oracle (vectorised over {batch} pairs), {dt:.1f} s "
                                      "on 1 host core"}
+  print(json.dumps(res), flush=True)
+
+
+def plain_rws(X, W, lens):
+  """The reference dataflow of ksig/algorithms.py:630-659 and
+  ksig/kernels.py:466 in torch FP32 ops on the GPU: materialise the distances
+  D [n, L, sum of lengths], then per warping series sweep the antidiagonals
+  with one batched min over every sequence and cell on it."""
+  n, L, _ = X.shape
+  D = (X * X).sum(-1)[:, :, None] + (W * W).sum(-1)[None, None, :] - 2 * X @ W.t()
+  out = []
+  off = 0
+  for lb in lens:
+    Db = D[:, :, off:off + lb]
+    off += lb
+    P = torch.full((n, L + 1, lb + 1), float("inf"), device=X.device, dtype=X.dtype)
+    P[:, 0, 0] = 0.0
+    for t in range(L + lb - 1):
+      i = torch.arange(max(0, t - lb + 1), min(t, L - 1) + 1, device=X.device)
+      j = t - i
+      P[:, i + 1, j + 1] = Db[:, i, j] + torch.minimum(
+          torch.minimum(P[:, i, j + 1], P[:, i + 1, j]), P[:, i, j])
+    out.append(P[:, L, lb])
+  return torch.stack(out, -1)
+
+
+def bench_rws(args):
+  """Random Warping Series features: 4096 Brownian paths, L=200, d=8,
+  against 100 warping series of lengths U{1..32} (the class defaults),
+  DTW values of every (sequence, series) pair in one launch."""
+  from ksig_amd import _ops
+  from oracle import sigkern as O
+  dev = torch.device("cuda", 0)
+  torch.cuda.set_device(dev)
+  n, L, d, n_w, max_warp = 4096, 200, 8, 100, 32
+  g = torch.Generator(device=dev).manual_seed(0)
+  X = brownian(g, n, L, d, dev)
+  rng = np.random.RandomState(0)
+  lens = rng.randint(1, max_warp + 1, size=n_w)
+  W = torch.as_tensor(rng.standard_normal((int(lens.sum()), d)) * 0.5, device=dev,
+                      dtype=torch.float32)
+  S = _ops.SeqSet.dense(X)
+  cells = float(n) * L * float(lens.sum())
+
+  def step():
+    return _ops.random_warping_series(S, W, lens)
+  for _ in range(args.warmup):
+    step()
+  torch.cuda.synchronize()
+  stream = torch.cuda.current_stream(dev)
+  e0 = torch.cuda.Event(enable_timing=True)
+  e1 = torch.cuda.Event(enable_timing=True)
+  t0 = time.perf_counter()
+  e0.record(stream)
+  for _ in range(args.steps):
+    out = step()
+  e1.record(stream)
+  torch.cuda.synchronize()
+  wall = time.perf_counter() - t0
+  kern_ms = e0.elapsed_time(e1) / args.steps
+  ms_per_step = wall * 1e3 / args.steps
+
+  X64 = X.double().cpu().numpy()
+  W64 = W.double().cpu().numpy()
+  idx = np.random.default_rng(3).choice(n, 4, replace=False)
+  ref = O.random_warping_series(O.squared_euclid_dist(X64[idx], W64[None]), lens)
+  got = out.cpu().numpy()[idx]
+  err = float(np.max(np.abs(got - ref).max(-1) / np.abs(ref).max(-1)))
+  flops_cell = 3 * d + 4   # distance (d mul-adds, norm terms) + min of three + add
+  achieved = cells * flops_cell / (kern_ms * 1e-3) / 1e12
+  res = {
+      "metric": "DTW-cells/sec (sequences x L x total warping-series length), fused fwd, fp32",
+      "value": cells / (ms_per_step * 1e-3),
+      "unit": "DTW cells/s",
+      "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+      "ms_per_step": ms_per_step,
+      "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+      "dtype": "f32", "data": "synthetic (seeded Brownian paths, Gaussian warping series)",
+      "config": {"workload": "random warping series, 4096 x L=200 x d=8 Brownian against 100 "
+                             "series of lengths U{1..32}",
+                 "n_seq": n, "L": L, "d": d, "n_warps": n_w, "warp_rows": int(lens.sum())},
+      "max_rel_err_vs_fp64": err,
+      "err_sample": "4 random sequences, normwise over their 100 DTW values",
+      "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+                   "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                   "kernel": "rws_forward_kernel<float,8,false,4>", "kernel_ms": kern_ms,
+                   "flops_per_cell": flops_cell},
+  }
+  if not args.no_plain:
+    nb, nwb = 256, 25
+    Xs = X[:nb].contiguous()
+    Ws = W[:int(lens[:nwb].sum())].contiguous()
+    pl = plain_rws(Xs, Ws, lens[:nwb])
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    plain_rws(Xs, Ws, lens[:nwb])
+    torch.cuda.synchronize()
+    tp = time.perf_counter() - t1
+    pv = nb * L * float(lens[:nwb].sum()) / tp
+    pdiff = float(torch.max(torch.abs(pl.double() - out[:nb, :nwb]) / out[:nb, :nwb].abs()))
+    res["plain_fp32"] = {"value": pv, "unit": "DTW cells/s",
+                         "speedup_fused_over_plain": res["value"] / pv,
+                         "sample": f"{nb} sequences x {nwb} series per call: distances "
+                                   "materialised, one batched min per antidiagonal, torch "
+                                   "FP32 ops",
+                         "max_rel_diff_vs_fused": pdiff}
+  if not args.no_cpu:
+    try:
+      from threadpoolctl import threadpool_limits
+      lim = threadpool_limits(1)
+    except Exception:  # pragma: no cover
+      lim = None
+    rng2 = np.random.default_rng(7)
+    batch, done = 64, 0
+    t2 = time.perf_counter()
+    while time.perf_counter() - t2 < args.cpu_seconds:
+      ia = rng2.integers(0, n, batch)
+      O.random_warping_series(O.squared_euclid_dist(X64[ia], W64[None]), lens[:10])
+      done += batch
+    dt = time.perf_counter() - t2
+    res["cpu_baseline"] = {"value": done * L * float(lens[:10].sum()) / dt,
+                           "unit": "DTW cells/s", "cores": 1, "kind": "port",
+                           "sample": f"{done} random sequences against the first 10 series, "
+                                     f"FP64 NumPy oracle (vectorised over {batch} sequences), "
+                                     f"{dt:.1f} s on 1 host core"}
   print(json.dumps(res), flush=True)
 
 

@@ -13,10 +13,11 @@
 // Here the warping series are short (at most max_warp points), so one wave
 // takes one sequence and a group of consecutive warping series stacked end
 // to end along the rows (the row-stream layout of the DP kernel): lane i owns
-// row 64*band + i of the stack, the sequence's points are the columns (staged
-// in LDS with their norms), and the lane walks them one step behind lane
-// i-1, receiving P of the row above through DPP wave_shr:1 (lane 0 from an
-// LDS FIFO written by lane 63 in the previous band).  A lane whose row is the
+// R consecutive rows of each 64R-row band of the stack, the sequence's points
+// are the columns (staged in LDS with their norms, shared by the R rows of a
+// step), and the lane walks them one step behind lane i-1, receiving P of the
+// row above its first row through DPP wave_shr:1 (lane 0 from an LDS FIFO
+// written by lane 63 in the previous band).  A lane whose row is the
 // first of its warping series ignores what flows in (the previous series'
 // values) and starts from the boundary; the lane of a series' last row writes
 // P at the last column.  D is computed in registers: no distance tensor.
@@ -30,11 +31,12 @@
 
 namespace ksig {
 
-template <typename T, int DMAX, bool GRAMIN>
+template <typename T, int DMAX, bool GRAMIN, int R>
 __global__ void __launch_bounds__(kWave)
 rws_forward_kernel(const RwsArgs a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const T kInf = T(INFINITY);
+  constexpr int BR = kWave * R;  // rows per band
 
   const int lane = threadIdx.x;
   const int wave = blockIdx.x;
@@ -66,7 +68,7 @@ rws_forward_kernel(const RwsArgs a) {
   // one step ahead, comes at least a step after lane 63's write of the slot.
   const int Pc = Lx <= kWave ? kWave + 1 : Lx;
   const int nrows = row_end - row_beg;
-  const int nb = (nrows + kWave - 1) / kWave;
+  const int nb = (nrows + BR - 1) / BR;
 
   // LDS: FIFO [Pc], then the sequence's points [Lx][DMAX] and norms [Lx].
   T* fifo = reinterpret_cast<T*>(smem_raw);
@@ -87,44 +89,52 @@ rws_forward_kernel(const RwsArgs a) {
   __syncthreads();
 
   const T* wdata = reinterpret_cast<const T*>(a.warp);
-  // this lane's row of band b: its warp point, norm and series
-  T wv[DMAX], wn = T(0);
-  int row = 0, wid = 0;
-  bool valid = false, first = false, last = false;
-  auto load_row = [&](int b) __attribute__((always_inline)) {
-    row = row_beg + b * kWave + lane;
-    valid = row < row_end;
-    // series of the row: the last w in [w_beg, w_end) with warp_off[w] <= row
-    int lo = w_beg, hi = w_end - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (a.warp_off[mid] <= row) lo = mid; else hi = mid - 1;
-    }
-    wid = lo;
-    first = valid && row == a.warp_off[wid];
-    last = valid && row == a.warp_off[wid + 1] - 1;
-    T n2 = T(0);
+  // this lane's R rows of band b: warp points, norms and series
+  T wv[R][DMAX], wn[R];
+  int rowq[R], wid[R];
+  bool first[R], last[R];
+  auto load_rows = [&](int b) __attribute__((always_inline)) {
 #pragma unroll
-    for (int k = 0; k < DMAX; ++k) {
-      const T v = (!GRAMIN && valid && k < d) ? wdata[(int64_t)row * d + k] : T(0);
-      wv[k] = v;
-      n2 += v * v;
+    for (int q = 0; q < R; ++q) {
+      const int row = row_beg + b * BR + lane * R + q;
+      const bool valid = row < row_end;
+      rowq[q] = valid ? row : row_end;  // row_end marks padding
+      // series of the row: the last w in [w_beg, w_end) with warp_off[w] <= row
+      int lo = w_beg, hi = w_end - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (a.warp_off[mid] <= row) lo = mid; else hi = mid - 1;
+      }
+      wid[q] = lo;
+      first[q] = valid && row == a.warp_off[lo];
+      last[q] = valid && row == a.warp_off[lo + 1] - 1;
+      T n2 = T(0);
+#pragma unroll
+      for (int k = 0; k < DMAX; ++k) {
+        const T v = (!GRAMIN && valid && k < d) ? wdata[(int64_t)row * d + k] : T(0);
+        wv[q][k] = v;
+        n2 += v * v;
+      }
+      wn[q] = n2;
     }
-    wn = n2;
   };
 
   // Inputs of one step, read a step ahead: the column's point and norm (or
-  // the caller's distance), and lane 0's FIFO entry.
+  // the caller's distances of the lane's rows), and lane 0's FIFO entry.
   struct StepIn {
-    T x[GRAMIN ? 1 : DMAX];
+    T x[GRAMIN ? R : DMAX];
     T xn;
     T f;
   };
-  auto fetch = [&](StepIn& in, int c, int b, int r) __attribute__((always_inline)) {
+  auto fetch = [&](StepIn& in, int c, int b) __attribute__((always_inline)) {
     const int cx = c < 0 ? 0 : (c >= Lx ? Lx - 1 : c);
     if constexpr (GRAMIN) {
-      const int rr = r < row_end ? r : row_end - 1;
-      in.xn = gm[(int64_t)cx * a.warp_rows + rr];
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        int rr = row_beg + b * BR + lane * R + q;
+        rr = rr < row_end ? rr : row_end - 1;
+        in.x[q] = gm[(int64_t)cx * a.warp_rows + rr];
+      }
     } else {
 #pragma unroll
       for (int k = 0; k < DMAX; ++k) in.x[k] = xl[cx * DMAX + k];
@@ -135,55 +145,68 @@ rws_forward_kernel(const RwsArgs a) {
   };
 
   int band = 0;
-  load_row(0);
-  T up = kInf, left = kInf, pin = kInf;
+  load_rows(0);
+  T left[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q) left[q] = kInf;
+  T up = kInf, pin = kInf;
   int cc = -lane;
   StepIn cur;
-  fetch(cur, cc, band, row);
+  fetch(cur, cc, band);
   const int total = nb * Pc + kWave - 1;
   for (int step = 0; step < total; ++step) {
     const bool active = cc >= 0 && band < nb;
+    const bool colok = active && cc < Lx;
+    const bool c0 = cc == 0;
     const T in = shift_down(up, cur.f);
-    T dist;
-    if constexpr (GRAMIN) {
-      dist = cur.xn;
-    } else {
-      T dot = T(0);
+    T dist[R];
 #pragma unroll
-      for (int k = 0; k < DMAX; ++k) dot += wv[k] * cur.x[k];
-      dist = wn + cur.xn - T(2) * dot;
+    for (int q = 0; q < R; ++q) {
+      if constexpr (GRAMIN) {
+        dist[q] = cur.x[q];
+      } else {
+        T dot = T(0);
+#pragma unroll
+        for (int k = 0; k < DMAX; ++k) dot += wv[q][k] * cur.x[k];
+        dist[q] = wn[q] + cur.xn - T(2) * dot;
+      }
     }
     {
       const bool wrap = cc + 1 == Pc;
-      fetch(cur, wrap ? 0 : cc + 1, band + (wrap ? 1 : 0), wrap ? row + kWave : row);
+      fetch(cur, wrap ? 0 : cc + 1, band + (wrap ? 1 : 0));
     }
-    const bool c0 = cc == 0;
-    // the first row of a series starts from the boundary: P(-1, c) = inf,
-    // P(-1, -1) = 0; column -1 is +inf
-    const T u = first ? kInf : in;
-    const T dg = c0 ? (first ? T(0) : kInf) : (first ? kInf : pin);
-    const T lf = c0 ? kInf : left;
-    T val = dist + fminf_t(fminf_t(u, lf), dg);
-    const bool cell = active && cc < Lx && valid;
-    val = cell ? val : kInf;
-    if (cell && last && cc == Lx - 1) out[wid] = (double)val;
-    if (lane == kWave - 1 && active) fifo[cc] = val;
-    up = val;
-    left = val;
+    T u = in, dg = pin;  // P(row-1, c), P(row-1, c-1)
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      // the first row of a series starts from the boundary: P(-1, c) = inf,
+      // P(-1, -1) = 0; column -1 is +inf
+      const T uq = first[q] ? kInf : u;
+      const T dq = c0 ? (first[q] ? T(0) : kInf) : (first[q] ? kInf : dg);
+      const T lq = c0 ? kInf : left[q];
+      T val = dist[q] + fminf_t(fminf_t(uq, lq), dq);
+      const bool cell = colok && rowq[q] < row_end;
+      val = cell ? val : kInf;
+      if (cell && last[q] && cc == Lx - 1) out[wid[q]] = (double)val;
+      dg = left[q];
+      left[q] = val;
+      u = val;
+    }
+    if (lane == kWave - 1 && active) fifo[cc] = u;
+    up = u;
     pin = in;
 
     ++cc;
     if (cc == Pc) {  // wrap to the next band
       cc = 0;
       ++band;
-      if (band < nb) load_row(band);
+      if (band < nb) load_rows(band);
     }
   }
 }
 
-template <typename T, int DMAX, bool GRAMIN>
+template <typename T, int DMAX, bool GRAMIN, int R>
 static hipError_t launch_rws(const RwsArgs& a, size_t lds, hipStream_t s) {
-  auto k = rws_forward_kernel<T, DMAX, GRAMIN>;
+  auto k = rws_forward_kernel<T, DMAX, GRAMIN, R>;
   if (lds > 64 * 1024) {
     hipError_t e = hipFuncSetAttribute((const void*)k,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -194,12 +217,15 @@ static hipError_t launch_rws(const RwsArgs& a, size_t lds, hipStream_t s) {
   return hipGetLastError();
 }
 
+// Rows per lane from register pressure: 4 (float32, d <= 16; float64, d <= 8),
+// 2 otherwise.
 template <typename T>
 static hipError_t dispatch_rws(const RwsArgs& a, size_t lds, hipStream_t s) {
-  if (a.gram) return launch_rws<T, 8, true>(a, lds, s);
-  if (a.d <= 8) return launch_rws<T, 8, false>(a, lds, s);
-  if (a.d <= 16) return launch_rws<T, 16, false>(a, lds, s);
-  return launch_rws<T, 32, false>(a, lds, s);
+  constexpr bool f32 = sizeof(T) == 4;
+  if (a.gram) return launch_rws<T, 8, true, 4>(a, lds, s);
+  if (a.d <= 8) return launch_rws<T, 8, false, 4>(a, lds, s);
+  if (a.d <= 16) return launch_rws<T, 16, false, f32 ? 4 : 2>(a, lds, s);
+  return launch_rws<T, 32, false, 2>(a, lds, s);
 }
 
 size_t rws_lds_bytes(int dtype, bool gram, int d, int lx) {
