@@ -39,7 +39,7 @@ PEAK_HBM_GBS = 8000.0
 # with the kernel version and schedule they were measured on (c3: the
 # autotuned schedule, 2738 waves).
 MEASURED_TRAFFIC_BYTES = {"c1": 38658 * 1024, "c3": (663693 * 2 + 581215) * 1024,
-                          "gak": (21130 * 2 + 4104) * 1024}
+                          "gak": (21130 * 2 + 4104) * 1024, "rws": (13451 * 2 + 3200) * 1024}
 
 
 def brownian(gen: torch.Generator, n, L, d, device, dtype=torch.float32):
@@ -548,7 +548,9 @@ def bench_rws(args):
       "max_rel_err_vs_fp64": err,
       "err_sample": "4 random sequences, normwise over their 100 DTW values",
       "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
-                   "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                   "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+                   "traffic": MEASURED_TRAFFIC_BYTES["rws"],
+                   "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE)",
                    "kernel": "rws_forward_kernel<float,8,false,4>", "kernel_ms": kern_ms,
                    "flops_per_cell": flops_cell},
   }
