@@ -618,7 +618,7 @@ static int pde_common(int dtype, ksig::PdeFwdArgs& a, int cols, int dmax, void* 
   if (a.dyadic < 0 || a.dyadic > 6) return fail(KSIG_EINVAL, "dyadic_order must be in [0, 6]");
   const int nc = a.difference ? cols - 1 : cols;
   long pc = (long)(nc + 1) << a.dyadic;
-  if (pc < ksig::kWave) pc = ksig::kWave;
+  if (pc <= ksig::kWave) pc = ksig::kWave + 1;  // as the kernel's band pass
   const size_t ycols = a.kind == ksig::kGram ? 0 : (size_t)cols;
   const size_t lds = (ycols * dmax + ycols + (size_t)pc * 4) * esz;
   if (lds > max_lds_bytes())
@@ -651,7 +651,7 @@ int ksig_signature_pde(int dtype, const void* x, const int64_t* x_off,
   a.difference = difference ? 1 : 0;
   a.out = out;
   const int cols = max_len_x < max_len_y ? max_len_x : max_len_y;
-  return pde_common(dtype, a, cols < 1 ? 1 : cols, dmax_for(d), stream);
+  return pde_common(dtype, a, cols < 1 ? 1 : cols, d <= 4 ? 4 : dmax_for(d), stream);
 }
 
 int ksig_signature_pde_fwd_edges(int dtype, const void* x, const int64_t* x_off,
@@ -682,7 +682,7 @@ int ksig_signature_pde_fwd_edges(int dtype, const void* x, const int64_t* x_off,
   a.edges = edges;
   a.e_off = e_off;
   const int cols = max_len_x < max_len_y ? max_len_x : max_len_y;
-  return pde_common(dtype, a, cols < 1 ? 1 : cols, dmax_for(d), stream);
+  return pde_common(dtype, a, cols < 1 ? 1 : cols, d <= 4 ? 4 : dmax_for(d), stream);
 }
 
 // Refined columns (shorter side) and rows (longer side) of the PDE grid.

@@ -5,10 +5,13 @@
 // with U = 1 on the row/column before the grid, returning U at the last cell
 // (ksig/algorithms.py:344-399, host launcher :402-446).  The reference sweeps
 // antidiagonals with two block barriers per step; here the same recurrence is
-// evaluated as a skewed systolic wave: lane i owns refined row r' = 64*band+i,
-// walks the refined columns one step behind lane i-1 and receives U(r'-1, c')
-// from it through DPP wave_shr:1 (lane 0 from an LDS FIFO per column), so a
-// step costs no barrier at all.
+// evaluated as a skewed systolic wave: lane i owns R consecutive refined rows
+// of each 64R-row band (R = 2 where registers allow: at dyadic order 1 one
+// original row, whose static-kernel value the two rows share), walks the
+// refined columns one step behind lane i-1 and receives U of the row above
+// its first row from it through DPP wave_shr:1 (lane 0 from an LDS FIFO per
+// column), so a step costs no barrier at all.  A step's inputs (the y row of
+// the column, lane 0's FIFO entry) are read a step ahead.
 //
 // Precision: the update is evaluated in increment form.  With
 // V(i,j) = U(i,j) - U(i,j-1), the scheme reads
@@ -31,27 +34,27 @@
 
 namespace ksig {
 
-template <typename T, int DMAX>
+template <typename T, int DMAX, int KIND, int R>
 __global__ void __launch_bounds__(kWave)
 pde_forward_kernel(const PdeFwdArgs a) {
   constexpr int NF = 4;  // flow: [G(r, c+1), G(r+1, c+1), U, V]
+  constexpr bool kGramIn = KIND == kGram;
+  constexpr int BR = kWave * R;  // refined rows per band
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* smem = reinterpret_cast<T*>(smem_raw);
 
   const int lane = threadIdx.x;
   const int64_t pr = blockIdx.x;
-  const int kind = a.kind;
   const int d = a.d;
 
-  const T* xs;
-  const T* ys;
+  const T* xs = nullptr;
+  const T* ys = nullptr;
   int Lx, Ly;
   const T* gm = nullptr;
-  if (kind == kGram) {
+  if constexpr (kGramIn) {
     Lx = a.gram_lx;
     Ly = a.gram_ly;
     gm = reinterpret_cast<const T*>(a.gram) + pr * (int64_t)Lx * Ly;
-    xs = ys = nullptr;
   } else {
     const int32_t ia = a.pairs[2 * pr];
     const int32_t ib = a.pairs[2 * pr + 1];
@@ -74,18 +77,20 @@ pde_forward_kernel(const PdeFwdArgs a) {
   const int lam = a.dyadic;
   const int F = 1 << lam;
   const int Nrr = Nr << lam, Ncr = Nc << lam;
-  int Pc = (Nc + 1) << lam;  // priming column (F steps) + refined columns
-  if (Pc < kWave) Pc = kWave;
-  const int nb = (Nrr + kWave - 1) / kWave;
+  // priming column (F steps) + refined columns; more than 64 so that lane 0's
+  // FIFO read, issued a step ahead, follows lane 63's write of the slot
+  int Pc = (Nc + 1) << lam;
+  if (Pc <= kWave) Pc = kWave + 1;
+  const int nb = (Nrr + BR - 1) / BR;
   const T inv4 = T(1) / T(F * F);
   const int rshift = diff ? 1 : 0;
 
   const T xscale = T(a.xscale), yscale = T(a.yscale);
   T* ylds = smem;
-  T* ynrm = ylds + (kind == kGram ? 0 : Ly * DMAX);
-  T* fifo = ynrm + (kind == kGram ? 0 : Ly);
+  T* ynrm = ylds + (kGramIn ? 0 : Ly * DMAX);
+  T* fifo = ynrm + (kGramIn ? 0 : Ly);
 
-  if (kind != kGram) {
+  if constexpr (!kGramIn) {
     for (int j = lane; j < Ly; j += kWave) {
       T n2 = T(0);
 #pragma unroll
@@ -99,21 +104,8 @@ pde_forward_kernel(const PdeFwdArgs a) {
   }
   __syncthreads();
 
-  // Static-kernel value at (x row xr, y row yc), given x row registers.
-  auto gval = [&](const T* xv, T xnrm, int xr, int yc) -> T {
-    if (kind == kGram) {
-      if (xr < 0 || xr >= Lx || yc < 0 || yc >= Ly) return T(0);
-      return gm[(int64_t)xr * Ly + yc];
-    }
-    yc = yc < 0 ? 0 : (yc >= Ly ? Ly - 1 : yc);
-    const T* yr = ylds + yc * DMAX;
-    T dot = T(0);
-#pragma unroll
-    for (int k = 0; k < DMAX; ++k) dot += xv[k] * yr[k];
-    return kind == kRBF ? fast_exp2(-(xnrm + ynrm[yc] - T(2) * dot)) : dot;
-  };
   auto load_x = [&](T* xv, T& xnrm, int xr) {
-    const bool ok = xr >= 0 && xr < Lx && kind != kGram;
+    const bool ok = xr >= 0 && xr < Lx && !kGramIn;
     T n2 = T(0);
 #pragma unroll
     for (int k = 0; k < DMAX; ++k) {
@@ -133,7 +125,17 @@ pde_forward_kernel(const PdeFwdArgs a) {
     for (int s = lane; s < Pc; s += kWave) {
       const int yc = (s >> lam);
       T g = T(0);
-      if (diff && yc < Ly) g = gval(x0, x0n, 0, yc);
+      if (diff && yc < Ly) {
+        if constexpr (kGramIn) {
+          g = gm[yc];
+        } else {
+          const T* yr = ylds + yc * DMAX;
+          T dot = T(0);
+#pragma unroll
+          for (int k = 0; k < DMAX; ++k) dot += x0[k] * yr[k];
+          g = KIND == kRBF ? fast_exp2(-(x0n + ynrm[yc] - T(2) * dot)) : dot;
+        }
+      }
       fifo[s * NF + 0] = T(0);
       fifo[s * NF + 1] = g;
       fifo[s * NF + 2] = T(1);
@@ -142,104 +144,167 @@ pde_forward_kernel(const PdeFwdArgs a) {
   }
   __syncthreads();
 
-  // Per-lane state.
+  // Per-lane state: R consecutive refined rows of the band.
   int band = 0;
-  int rr = lane;                 // refined row
-  int orow = rr >> lam;          // original grid row
-  bool leader = (rr & (F - 1)) == 0;
-  T xv[DMAX], xnrm;
-  load_x(xv, xnrm, orow + rshift);
-  T xv2[DMAX], xnrm2;            // next band's row, prefetched
-  load_x(xv2, xnrm2, ((rr + kWave) >> lam) + rshift);
+  int rr0 = lane * R;            // first refined row of the lane
+  T xv[R][DMAX], xnrm[R];
+  T xv2[R][DMAX], xnrm2[R];      // next band's rows, prefetched
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    load_x(xv[q], xnrm[q], ((rr0 + q) >> lam) + rshift);
+    load_x(xv2[q], xnrm2[q], ((rr0 + q + BR) >> lam) + rshift);
+  }
 
   T* edges = a.edges ? reinterpret_cast<T*>(a.edges) + a.e_off[pr] : nullptr;
 
   T fl[NF] = {T(0), T(0), T(1), T(0)};
-  T gt_old = T(0), gb_old = T(0), gt = T(0), gb = T(0);
-  T M = T(0);
-  T uleft = T(1), udiag = T(1);
+  T gt_old[R], gb_old[R], gt[R], gb[R], M[R], uleft[R], udiag[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    gt_old[q] = gb_old[q] = gt[q] = gb[q] = M[q] = T(0);
+    uleft[q] = udiag[q] = T(1);
+  }
   int cc = -F - lane;            // refined column (priming columns are -F..-1)
 
+  // Inputs of one step, read a step ahead: the y row and norm of the
+  // original column whose right edge the step's leader rows evaluate (or the
+  // caller's values), and lane 0's FIFO entry.
+  struct StepIn {
+    T y[kGramIn ? R : DMAX];
+    T yn;
+    T f[NF];
+  };
+  auto fetch = [&](StepIn& in, int c, int b, int r0) __attribute__((always_inline)) {
+    int yc = (c >> lam) + rshift;
+    if constexpr (kGramIn) {
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const int xr = ((r0 + q) >> lam) + rshift;
+        in.y[q] = (xr < Lx && yc >= 0 && yc < Ly) ? gm[(int64_t)xr * Ly + yc] : T(0);
+      }
+    } else {
+      yc = yc < 0 ? 0 : (yc >= Ly ? Ly - 1 : yc);
+#pragma unroll
+      for (int k = 0; k < DMAX; ++k) in.y[k] = ylds[yc * DMAX + k];
+      in.yn = ynrm[yc];
+    }
+    in.f[0] = T(0);
+    in.f[1] = T(0);
+    in.f[2] = T(1);
+    in.f[3] = T(0);
+    if (lane == 0 && c >= -F && b < nb) {
+      const int s = c + F;
+#pragma unroll
+      for (int k = 0; k < NF; ++k) in.f[k] = fifo[s * NF + k];
+    }
+  };
+
+  StepIn cur;
+  fetch(cur, cc, band, rr0);
   const int total = nb * Pc + kWave - 1;
   for (int step = 0; step < total; ++step) {
     const bool active = cc >= -F && band < nb;
-    T fin0 = T(0), fin1 = T(0), fin2 = T(1), fin3 = T(0);
-    if (lane == 0 && active) {
-      const int s = cc + F;
-      fin0 = fifo[s * NF + 0];
-      fin1 = fifo[s * NF + 1];
-      fin2 = fifo[s * NF + 2];
-      fin3 = fifo[s * NF + 3];
+    const T in_gt = shift_down(fl[0], cur.f[0]);
+    const T in_gb = shift_down(fl[1], cur.f[1]);
+    const T in_u = shift_down(fl[2], cur.f[2]);
+    const T in_v = shift_down(fl[3], cur.f[3]);
+    // static-kernel values of the leader rows at the right edge of the
+    // original column (used at the column's first refined step)
+    T gnew[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      if constexpr (kGramIn) {
+        gnew[q] = cur.y[q];
+      } else {
+        T dot = T(0);
+#pragma unroll
+        for (int k = 0; k < DMAX; ++k) dot += xv[q][k] * cur.y[k];
+        gnew[q] = KIND == kRBF ? fast_exp2(-(xnrm[q] + cur.yn - T(2) * dot)) : dot;
+      }
     }
-    const T in_gt = shift_down(fl[0], fin0);
-    const T in_gb = shift_down(fl[1], fin1);
-    const T in_u = shift_down(fl[2], fin2);
-    const T in_v = shift_down(fl[3], fin3);
+    {
+      const bool wrap = cc + 1 == Pc - F;
+      fetch(cur, wrap ? -F : cc + 1, band + (wrap ? 1 : 0), wrap ? rr0 + BR : rr0);
+    }
 
     const bool newcol = (cc & (F - 1)) == 0;  // first refined step of a column
-    const int ocol = cc >> lam;               // original column (-1 = priming)
-    // Gram values of this lane's original cell (right edge, column ocol+1).
-    T ngt = leader ? in_gb : in_gt;
-    T ngb = in_gb;
-    if (leader && newcol) ngb = gval(xv, xnrm, orow + rshift, ocol + rshift);
-    else if (leader) ngb = gb;
-    if (newcol) {
-      gt_old = gt;
-      gb_old = gb;
-      const T m = diff ? ((ngb - gb_old) - (ngt - gt_old)) : ngb;
-      M = m * inv4;
-    }
-    gt = ngt;
-    gb = ngb;
+    const bool colok = active && cc >= 0 && cc < Ncr;
+    T igt = in_gt, igb = in_gb, iu = in_u, iv = in_v;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int rr = rr0 + q;
+      const bool leader = (rr & (F - 1)) == 0;
+      // Gram values of this row's original cell (right edge, column ocol+1)
+      const T ngt = leader ? igb : igt;
+      const T ngb = leader ? (newcol ? gnew[q] : gb[q]) : igb;
+      if (newcol) {
+        gt_old[q] = gt[q];
+        gb_old[q] = gb[q];
+        const T m = diff ? ((ngb - gb_old[q]) - (ngt - gt_old[q])) : ngb;
+        M[q] = m * inv4;
+      }
+      gt[q] = ngt;
+      gb[q] = ngb;
 
-    const bool cell = active && cc >= 0 && cc < Ncr && rr < Nrr;
-    const T b = M * M * T(1.0 / 12.0);
-    const T aa = T(0.5) * M + b;
-    const T delta = (in_u + uleft) * aa + udiag * b;
-    T v = in_v + delta;
-    T u = uleft + v;
-    u = cell ? u : T(1);  // priming / padding columns carry the boundary value
-    v = cell ? v : in_v;
-    if (cell && rr == Nrr - 1 && cc == Ncr - 1) a.out[pr] = (double)u;
-    if (edges != nullptr && cell) {  // last refined row, then last refined column
-      if (rr == Nrr - 1) edges[cc] = u;
-      if (cc == Ncr - 1) edges[Ncr + rr] = u;
+      const bool cell = colok && rr < Nrr;
+      const T b = M[q] * M[q] * T(1.0 / 12.0);
+      const T aa = T(0.5) * M[q] + b;
+      const T delta = (iu + uleft[q]) * aa + udiag[q] * b;
+      T v = iv + delta;
+      T u = uleft[q] + v;
+      u = cell ? u : T(1);  // priming / padding columns carry the boundary value
+      v = cell ? v : iv;
+      if (cell && rr == Nrr - 1 && cc == Ncr - 1) a.out[pr] = (double)u;
+      if (edges != nullptr && cell) {  // last refined row, then last refined column
+        if (rr == Nrr - 1) edges[cc] = u;
+        if (cc == Ncr - 1) edges[Ncr + rr] = u;
+      }
+      udiag[q] = iu;
+      uleft[q] = u;
+      igt = ngt;
+      igb = ngb;
+      iu = u;
+      iv = v;
     }
-    uleft = u;
-    udiag = in_u;
 
-    fl[0] = gt;
-    fl[1] = gb;
-    fl[2] = u;
-    fl[3] = v;
+    fl[0] = igt;
+    fl[1] = igb;
+    fl[2] = iu;
+    fl[3] = iv;
     if (lane == kWave - 1 && active) {
       const int s = cc + F;
-      fifo[s * NF + 0] = gt;
-      fifo[s * NF + 1] = gb;
-      fifo[s * NF + 2] = u;
-      fifo[s * NF + 3] = v;
+      fifo[s * NF + 0] = igt;
+      fifo[s * NF + 1] = igb;
+      fifo[s * NF + 2] = iu;
+      fifo[s * NF + 3] = iv;
     }
 
     ++cc;
     if (cc == Pc - F) {  // wrap to the next band
       cc = -F;
       ++band;
-      rr += kWave;
-      orow = rr >> lam;
-      leader = (rr & (F - 1)) == 0;
+      rr0 += BR;
 #pragma unroll
-      for (int k = 0; k < DMAX; ++k) xv[k] = xv2[k];
-      xnrm = xnrm2;
-      load_x(xv2, xnrm2, ((rr + kWave) >> lam) + rshift);
-      uleft = T(1);
-      gt = gb = T(0);
+      for (int q = 0; q < R; ++q) {
+#pragma unroll
+        for (int k = 0; k < DMAX; ++k) xv[q][k] = xv2[q][k];
+        xnrm[q] = xnrm2[q];
+        load_x(xv2[q], xnrm2[q], ((rr0 + q + BR) >> lam) + rshift);
+        uleft[q] = T(1);
+        gt[q] = gb[q] = T(0);
+      }
     }
   }
 }
 
+// Refined rows per lane from register pressure (x rows of this and the next
+// band stay in registers): 2 up to 8 float64 / 16 float32 features, else 1.
 template <typename T, int DMAX>
+constexpr int pde_rows() { return DMAX * (int)sizeof(T) <= 64 ? 2 : 1; }
+
+template <typename T, int DMAX, int KIND>
 static hipError_t launch_pde(const PdeFwdArgs& a, size_t lds, hipStream_t s) {
-  auto k = pde_forward_kernel<T, DMAX>;
+  auto k = pde_forward_kernel<T, DMAX, KIND, pde_rows<T, DMAX>()>;
   if (lds > 64 * 1024) {
     hipError_t e = hipFuncSetAttribute((const void*)k,
                                        hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -250,11 +315,19 @@ static hipError_t launch_pde(const PdeFwdArgs& a, size_t lds, hipStream_t s) {
   return hipGetLastError();
 }
 
+template <typename T, int KIND>
+static hipError_t dispatch_pde_d(const PdeFwdArgs& a, size_t lds, hipStream_t s) {
+  if (a.d <= 4) return launch_pde<T, 4, KIND>(a, lds, s);
+  if (a.d <= 8) return launch_pde<T, 8, KIND>(a, lds, s);
+  if (a.d <= 16) return launch_pde<T, 16, KIND>(a, lds, s);
+  return launch_pde<T, 32, KIND>(a, lds, s);
+}
+
 template <typename T>
 static hipError_t dispatch_pde(const PdeFwdArgs& a, size_t lds, hipStream_t s) {
-  if (a.kind == kGram || a.d <= 8) return launch_pde<T, 8>(a, lds, s);
-  if (a.d <= 16) return launch_pde<T, 16>(a, lds, s);
-  return launch_pde<T, 32>(a, lds, s);
+  if (a.kind == kGram) return launch_pde<T, 8, kGram>(a, lds, s);
+  if (a.kind == kRBF) return dispatch_pde_d<T, kRBF>(a, lds, s);
+  return dispatch_pde_d<T, kLinear>(a, lds, s);
 }
 
 hipError_t pde_forward_launch(const PdeFwdArgs& a, int dtype, size_t lds, hipStream_t s) {
