@@ -51,12 +51,13 @@ constexpr int pde_slot_quads() { return (int)((sizeof(PdeSlot<T>) + 15) / 16); }
 
 __host__ __device__ constexpr int pde_bwd_pc(int ncr) { return ncr > 80 ? ncr : 80; }
 
-template <typename T, int DMAX>
+template <typename T, int DMAX, int R>
 __global__ void __launch_bounds__(kWave)
 pde_bwd_edges_kernel(const PdeBwdEdgeArgs a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int XS = 2 * DMAX + 2;  // staged x rows of the next band: top, bottom, norms
   constexpr int SQ = pde_slot_quads<T>();
+  constexpr int BR = kWave * R;     // refined rows per band
 
   const int lane = threadIdx.x;
   const int64_t pr = blockIdx.x;
@@ -78,10 +79,13 @@ pde_bwd_edges_kernel(const PdeBwdEdgeArgs a) {
   if (Nr <= 0 || Nc <= 0) return;
   const int lam = a.dyadic;
   const int F = 1 << lam;
+  // with two rows per lane and dyadic order >= 1 the lane's rows (odd r0 and
+  // r0 - 1) lie in one original row and share its static-kernel values
+  const bool shared = R == 2 && lam >= 1;
   const int Nrr = Nr << lam, Ncr = Nc << lam;
   const int Pc = pde_bwd_pc(Ncr);
   const int nrows = Nrr + 1;  // + virtual row above the grid
-  const int nb = (nrows + kWave - 1) / kWave;
+  const int nb = (nrows + BR - 1) / BR;
   const T inv4 = T(1) / T(F * F);
   const T w = T(a.gout[pr]);
   const T* edge_row = reinterpret_cast<const T*>(a.edges) + a.e_off[pr];  // [Ncr]
@@ -117,40 +121,47 @@ pde_bwd_edges_kernel(const PdeBwdEdgeArgs a) {
     for (int k = 0; k < DMAX; ++k) dot += xv[k] * yr[k];
     return kind == kRBF ? fast_exp2(-(xn + ynrm[yc] - T(2) * dot)) : dot;
   };
-  // stage the x rows of refined row rr into this lane's slot
-  auto stage = [&](int rr) __attribute__((always_inline)) {
-    T* dst = xst + lane * XS;
-    if (rr >= 0) {
-      const int i = rr >> lam;
-      const int i2 = diff ? i + 1 : i;
-      T n0 = T(0), n1 = T(0);
+  // stage the x rows of the lane's refined rows r0, r0-1, ... into its slots
+  auto stage = [&](int r0) __attribute__((always_inline)) {
 #pragma unroll
-      for (int k = 0; k < DMAX; ++k) {
-        const T v0 = k < d ? xs[(int64_t)i * d + k] * xscale : T(0);
-        const T v1 = k < d ? xs[(int64_t)i2 * d + k] * xscale : T(0);
-        dst[k] = v0;
-        dst[DMAX + k] = v1;
-        n0 += v0 * v0;
-        n1 += v1 * v1;
+    for (int q = 0; q < R; ++q) {
+      T* dst = xst + (lane * R + q) * XS;
+      const int rr = r0 - q;
+      if (rr >= 0) {
+        const int i = rr >> lam;
+        const int i2 = diff ? i + 1 : i;
+        T n0 = T(0), n1 = T(0);
+#pragma unroll
+        for (int k = 0; k < DMAX; ++k) {
+          const T v0 = k < d ? xs[(int64_t)i * d + k] * xscale : T(0);
+          const T v1 = k < d ? xs[(int64_t)i2 * d + k] * xscale : T(0);
+          dst[k] = v0;
+          dst[DMAX + k] = v1;
+          n0 += v0 * v0;
+          n1 += v1 * v1;
+        }
+        dst[2 * DMAX] = n0;
+        dst[2 * DMAX + 1] = n1;
       }
-      dst[2 * DMAX] = n0;
-      dst[2 * DMAX + 1] = n1;
     }
   };
   int band = 0;
-  int r = Nrr - 1 - lane;  // refined row of this lane (flipped order)
-  T xt[DMAX], xb[DMAX], xtn = T(0), xbn = T(0);
+  int r0 = Nrr - 1 - lane * R;  // the lane's first refined row (flipped order)
+  T xt[R][DMAX], xb[R][DMAX], xtn[R], xbn[R];
   auto take = [&]() __attribute__((always_inline)) {
-    const T* src = xst + lane * XS;
 #pragma unroll
-    for (int k = 0; k < DMAX; ++k) {
-      xt[k] = src[k];
-      xb[k] = src[DMAX + k];
+    for (int q = 0; q < R; ++q) {
+      const T* src = xst + (lane * R + q) * XS;
+#pragma unroll
+      for (int k = 0; k < DMAX; ++k) {
+        xt[q][k] = src[k];
+        xb[q][k] = src[DMAX + k];
+      }
+      xtn[q] = src[2 * DMAX];
+      xbn[q] = src[2 * DMAX + 1];
     }
-    xtn = src[2 * DMAX];
-    xbn = src[2 * DMAX + 1];
   };
-  stage(r);
+  stage(r0);
   __syncthreads();
   take();
 
@@ -186,9 +197,14 @@ pde_bwd_edges_kernel(const PdeBwdEdgeArgs a) {
 #pragma unroll
   for (int k = 0; k < 6; ++k) fl[k] = T(0);
   double flu = 1.0;
-  T lam_r = T(0), a_r = T(0), beta_prev = T(0), gacc = T(0), M = T(0);
-  T gtL = T(0), gbL = T(0);
-  double u_self = 1.0, u_bp = 1.0;
+  // per-row state
+  T lam_r[R], a_r[R], beta_prev[R], gacc[R], M[R], gtL[R], gbL[R];
+  double u_self[R], u_bp[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    lam_r[q] = a_r[q] = beta_prev[q] = gacc[q] = M[q] = gtL[q] = gbL[q] = T(0);
+    u_self[q] = u_bp[q] = 1.0;
+  }
   int s = -lane;
   const int total = nb * Pc + kWave - 1;
   // band b+1 is staged at step b*Pc + 63: every lane has taken band b by then
@@ -197,113 +213,134 @@ pde_bwd_edges_kernel(const PdeBwdEdgeArgs a) {
 
   auto step_fn = [&](int step, T* fv, double& fu) __attribute__((always_inline)) {
     if (step == next_stage) {  // wave-uniform
-      if (stage_band < nb) stage(r - kWave);
+      if (stage_band < nb) stage(r0 - BR);
       ++stage_band;
       next_stage += Pc;
     }
-    const bool active = s >= 0 && s < Ncr && band < nb && r >= -1;
+    const bool colok = s >= 0 && s < Ncr && band < nb;
     T in[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) in[k] = shift_down(fl[k], fv[k]);
-    const double in_u = shift_down(flu, fu);
+    double in_u = shift_down(flu, fu);
     // this step's inputs are consumed: refill the set for the step after next
     fread(fv, fu);
 
     const int c = Ncr - 1 - s;
-    const bool cell = active && r >= 0;
-    if (active && s == 0) {  // right edge: saved last refined column
-      u_self = r >= 0 ? (double)ecol[r] : 1.0;
-      u_bp = (r + 1 <= Nrr - 1) ? (double)ecol[r + 1] : 1.0;
-    }
-    // this lane's cell value M'(r, c), recomputed once per original column
-    if (cell && (c & (F - 1)) == F - 1) {
-      const int j = c >> lam;
-      if (diff) {
-        T gtR, gbR;
-        if (s == 0) {
-          gtR = gval(xt, xtn, j + 1);
-          gbR = gval(xb, xbn, j + 1);
+    const bool newblk = (c & (F - 1)) == F - 1;  // first refined column of an original column
+    const int j = c >> lam;
+    bool last_active = false;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int r = r0 - q;
+      const bool active = colok && r >= -1;
+      const bool cell = active && r >= 0;
+      last_active = active;
+      if (active && s == 0) {  // right edge: saved last refined column
+        u_self[q] = r >= 0 ? (double)ecol[r] : 1.0;
+        u_bp[q] = (r + 1 <= Nrr - 1) ? (double)ecol[r + 1] : 1.0;
+      }
+      // this row's cell value M'(r, c), recomputed once per original column
+      if (cell && newblk) {
+        if (shared && q > 0) {
+          M[q] = M[0];
+        } else if (diff) {
+          T gtR, gbR;
+          if (s == 0) {
+            gtR = gval(xt[q], xtn[q], j + 1);
+            gbR = gval(xb[q], xbn[q], j + 1);
+          } else {
+            gtR = gtL[q];
+            gbR = gbL[q];
+          }
+          gtL[q] = gval(xt[q], xtn[q], j);
+          gbL[q] = gval(xb[q], xbn[q], j);
+          M[q] = ((gbR - gbL[q]) - (gtR - gtL[q])) * inv4;
         } else {
-          gtR = gtL;
-          gbR = gbL;
+          M[q] = gval(xt[q], xtn[q], j) * inv4;
         }
-        gtL = gval(xt, xtn, j);
-        gbL = gval(xb, xbn, j);
-        M = ((gbR - gbL) - (gtR - gtL)) * inv4;
-      } else {
-        M = gval(xt, xtn, j) * inv4;
       }
-    }
 
-    // adjoint λ of this lane's cell (r, c)
-    const T bb = M * M * T(1.0 / 12.0);
-    const T aa = T(0.5) * M + bb;
-    const T eps = in[2] + a_r * lam_r + beta_prev;
-    const T seed = (cell && r == Nrr - 1 && c == Ncr - 1) ? w : T(0);
-    T mu = in[1] + eps + seed;
-    T lv = lam_r + mu;
-    mu = cell ? mu : in[1];
-    lv = cell ? lv : T(0);
+      // adjoint λ of this row's cell (r, c)
+      const T Mq = M[q];
+      const T bb = Mq * Mq * T(1.0 / 12.0);
+      const T aa = T(0.5) * Mq + bb;
+      const T eps = in[2] + a_r[q] * lam_r[q] + beta_prev[q];
+      const T seed = (cell && r == Nrr - 1 && c == Ncr - 1) ? w : T(0);
+      T mu = in[1] + eps + seed;
+      T lv = lam_r[q] + mu;
+      mu = cell ? mu : in[1];
+      lv = cell ? lv : T(0);
 
-    // rebuild U(r, c-1)
-    double u_new = 1.0;
-    if (active && c >= 1) {
-      if (r == Nrr - 1) {
-        u_new = (double)erow[c - 1];
-      } else if (r >= 0) {
+      // rebuild U(r, c-1)
+      double u_new = 1.0;
+      if (active && c >= 1) {
+        if (r == Nrr - 1) {
+          u_new = (double)erow[c - 1];
+        } else if (r >= 0) {
+          const double mb = (double)in[4];
+          const double bq = mb * mb * (1.0 / 12.0);
+          const double A = 1.0 + 0.5 * mb + bq;
+          const double B = 1.0 - bq;
+          u_new = ((u_self[q] + in_u) * A - u_bp[q]) / B;
+        }
+      }
+
+      // cell (r+1, c), completed here: λ and M' from below, U from both rows
+      const int rb = r + 1;
+      const bool below = active && rb <= Nrr - 1;
+      T g = T(0);
+      if (below) {
         const double mb = (double)in[4];
-        const double bq = mb * mb * (1.0 / 12.0);
-        const double A = 1.0 + 0.5 * mb + bq;
-        const double B = 1.0 - bq;
-        u_new = ((u_self + in_u) * A - u_bp) / B;
+        g = in[0] * (T)((u_self[q] + in_u) * (0.5 + mb * (1.0 / 6.0)) +
+                        u_new * mb * (1.0 / 6.0));
       }
+      gacc[q] = (newblk ? T(0) : gacc[q]) + g;
+      const T part = (((rb & (F - 1)) == F - 1) ? T(0) : in[5]) + gacc[q];
+      if (below && (c & (F - 1)) == 0 && (rb & (F - 1)) == 0)
+        gm[(int64_t)(rb >> lam) * Nc + (c >> lam)] = part * inv4;
+
+      // this row's outputs are the next row's inputs
+      const T o3 = in[3];
+      in[0] = lv;
+      in[1] = mu;
+      in[2] = cell ? aa * lv : T(0);
+      in[3] = cell ? bb * lv : T(0);
+      in[4] = cell ? Mq : T(0);
+      in[5] = part;
+      lam_r[q] = lv;
+      a_r[q] = cell ? aa : T(0);
+      beta_prev[q] = o3;
+      u_bp[q] = in_u;
+      u_self[q] = u_new;
+      in_u = u_new;
     }
 
-    // cell (r+1, c), completed here: λ and M' from below, U from both rows
-    const int rb = r + 1;
-    const bool below = active && rb <= Nrr - 1;
-    T g = T(0);
-    if (below) {
-      const double mb = (double)in[4];
-      g = in[0] * (T)((u_self + in_u) * (0.5 + mb * (1.0 / 6.0)) + u_new * mb * (1.0 / 6.0));
-    }
-    gacc = ((c & (F - 1)) == F - 1 ? T(0) : gacc) + g;
-    const T part = (((rb & (F - 1)) == F - 1) ? T(0) : in[5]) + gacc;
-    if (below && (c & (F - 1)) == 0 && (rb & (F - 1)) == 0)
-      gm[(int64_t)(rb >> lam) * Nc + (c >> lam)] = part * inv4;
-
-    fl[0] = lv;
-    fl[1] = mu;
-    fl[2] = cell ? aa * lv : T(0);
-    fl[3] = cell ? bb * lv : T(0);
-    fl[4] = cell ? M : T(0);
-    fl[5] = part;
-    flu = u_new;
-    if (lane == kWave - 1 && active) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) fl[k] = in[k];
+    flu = in_u;
+    if (lane == kWave - 1 && last_active) {
       PdeSlot<T> o;
 #pragma unroll
       for (int k = 0; k < 6; ++k) o.v[k] = fl[k];
-      o.u = u_new;
-      float4 q[SQ];
-      __builtin_memset(q, 0, sizeof(q));
-      __builtin_memcpy(q, &o, sizeof(PdeSlot<T>));
+      o.u = flu;
+      float4 q4[SQ];
+      __builtin_memset(q4, 0, sizeof(q4));
+      __builtin_memcpy(q4, &o, sizeof(PdeSlot<T>));
 #pragma unroll
-      for (int k = 0; k < SQ; ++k) fifo[s * SQ + k] = q[k];
+      for (int k = 0; k < SQ; ++k) fifo[s * SQ + k] = q4[k];
     }
-    lam_r = lv;
-    a_r = cell ? aa : T(0);
-    beta_prev = in[3];
-    u_bp = in_u;
-    u_self = u_new;
 
     ++s;
     if (s == Pc) {
       s = 0;
       ++band;
-      r -= kWave;
+      r0 -= BR;
       take();
-      lam_r = a_r = beta_prev = gacc = M = T(0);
-      u_self = u_bp = 1.0;
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        lam_r[q] = a_r[q] = beta_prev[q] = gacc[q] = M[q] = T(0);
+        u_self[q] = u_bp[q] = 1.0;
+      }
     }
   };
 
@@ -315,9 +352,14 @@ pde_bwd_edges_kernel(const PdeBwdEdgeArgs a) {
   if (step < total) step_fn(step, fv0, fu0);
 }
 
+// Refined rows per lane: 2 where the staged and register x rows fit (up to
+// 16 float32 / 8 float64 features), else 1.
+template <typename T, int DMAX>
+constexpr int pde_bwd_rows() { return DMAX * (int)sizeof(T) <= 64 ? 2 : 1; }
+
 template <typename T, int DMAX>
 static hipError_t launch_bwd_edges(const PdeBwdEdgeArgs& a, size_t lds, hipStream_t s) {
-  auto k = pde_bwd_edges_kernel<T, DMAX>;
+  auto k = pde_bwd_edges_kernel<T, DMAX, pde_bwd_rows<T, DMAX>()>;
   if (lds > 64 * 1024) {
     hipError_t e = hipFuncSetAttribute((const void*)k,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -329,6 +371,7 @@ static hipError_t launch_bwd_edges(const PdeBwdEdgeArgs& a, size_t lds, hipStrea
 
 template <typename T>
 static hipError_t dispatch_bwd_edges(const PdeBwdEdgeArgs& a, size_t lds, hipStream_t s) {
+  if (a.d <= 4) return launch_bwd_edges<T, 4>(a, lds, s);
   if (a.d <= 8) return launch_bwd_edges<T, 8>(a, lds, s);
   if (a.d <= 16) return launch_bwd_edges<T, 16>(a, lds, s);
   return launch_bwd_edges<T, 32>(a, lds, s);
@@ -336,9 +379,10 @@ static hipError_t dispatch_bwd_edges(const PdeBwdEdgeArgs& a, size_t lds, hipStr
 
 size_t pde_bwd_edges_lds(int dtype, int d, int ly, int ncr, int nrr) {
   const size_t esz = dtype == KSIG_F32 ? 4 : 8;
-  const int dmax = d <= 8 ? 8 : (d <= 16 ? 16 : 32);
+  const int dmax = d <= 4 ? 4 : (d <= 8 ? 8 : (d <= 16 ? 16 : 32));
+  const int rows = dmax * (int)esz <= 64 ? 2 : 1;  // as pde_bwd_rows
   return ((size_t)ly * (dmax + 1) + (size_t)ncr + (size_t)nrr +
-          (size_t)kWave * (2 * dmax + 2)) * esz;
+          (size_t)kWave * rows * (2 * dmax + 2)) * esz;
 }
 
 int pde_bwd_fifo_cols(int ncr) { return pde_bwd_pc(ncr); }
