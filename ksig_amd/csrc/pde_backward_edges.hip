@@ -79,9 +79,10 @@ pde_bwd_edges_kernel(const PdeBwdEdgeArgs a) {
   if (Nr <= 0 || Nc <= 0) return;
   const int lam = a.dyadic;
   const int F = 1 << lam;
-  // with two rows per lane and dyadic order >= 1 the lane's rows (odd r0 and
-  // r0 - 1) lie in one original row and share its static-kernel values
-  const bool shared = R == 2 && lam >= 1;
+  // with an even number of rows per lane and dyadic order >= 1, each odd row
+  // r0 - 2k and the row r0 - 2k - 1 after it lie in one original row and
+  // share its static-kernel values (r0 = Nrr - 1 - R * (...) is odd)
+  const bool shared = (R % 2 == 0) && lam >= 1;
   const int Nrr = Nr << lam, Ncr = Nc << lam;
   const int Pc = pde_bwd_pc(Ncr);
   const int nrows = Nrr + 1;  // + virtual row above the grid
@@ -241,8 +242,8 @@ pde_bwd_edges_kernel(const PdeBwdEdgeArgs a) {
       }
       // this row's cell value M'(r, c), recomputed once per original column
       if (cell && newblk) {
-        if (shared && q > 0) {
-          M[q] = M[0];
+        if (shared && (q & 1)) {
+          M[q] = M[q - 1];
         } else if (diff) {
           T gtR, gbR;
           if (s == 0) {
@@ -353,9 +354,13 @@ pde_bwd_edges_kernel(const PdeBwdEdgeArgs a) {
 }
 
 // Refined rows per lane: 2 where the staged and register x rows fit (up to
-// 16 float32 / 8 float64 features), else 1.
+// 16 float32 / 8 float64 features), else 1 (4 rows measured no faster at
+// C2: the adjoint's float64 rebuild keeps it issue bound per row).
+__host__ __device__ constexpr int pde_bwd_rows_for(int dmax, int esz) {
+  return dmax * esz <= 64 ? 2 : 1;
+}
 template <typename T, int DMAX>
-constexpr int pde_bwd_rows() { return DMAX * (int)sizeof(T) <= 64 ? 2 : 1; }
+constexpr int pde_bwd_rows() { return pde_bwd_rows_for(DMAX, (int)sizeof(T)); }
 
 template <typename T, int DMAX>
 static hipError_t launch_bwd_edges(const PdeBwdEdgeArgs& a, size_t lds, hipStream_t s) {
@@ -380,7 +385,7 @@ static hipError_t dispatch_bwd_edges(const PdeBwdEdgeArgs& a, size_t lds, hipStr
 size_t pde_bwd_edges_lds(int dtype, int d, int ly, int ncr, int nrr) {
   const size_t esz = dtype == KSIG_F32 ? 4 : 8;
   const int dmax = d <= 4 ? 4 : (d <= 8 ? 8 : (d <= 16 ? 16 : 32));
-  const int rows = dmax * (int)esz <= 64 ? 2 : 1;  // as pde_bwd_rows
+  const int rows = pde_bwd_rows_for(dmax, (int)esz);
   return ((size_t)ly * (dmax + 1) + (size_t)ncr + (size_t)nrr +
           (size_t)kWave * rows * (2 * dmax + 2)) * esz;
 }

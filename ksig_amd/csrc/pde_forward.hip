@@ -6,8 +6,9 @@
 // (ksig/algorithms.py:344-399, host launcher :402-446).  The reference sweeps
 // antidiagonals with two block barriers per step; here the same recurrence is
 // evaluated as a skewed systolic wave: lane i owns R consecutive refined rows
-// of each 64R-row band (R = 2 where registers allow: at dyadic order 1 one
-// original row, whose static-kernel value the two rows share), walks the
+// of each 64R-row band (R = 2 or 4 where registers allow; at dyadic order 1
+// each pair of rows is one original row, whose static-kernel value the two
+// rows share), walks the
 // refined columns one step behind lane i-1 and receives U of the row above
 // its first row from it through DPP wave_shr:1 (lane 0 from an LDS FIFO per
 // column), so a step costs no barrier at all.  A step's inputs (the y row of
@@ -298,9 +299,10 @@ pde_forward_kernel(const PdeFwdArgs a) {
 }
 
 // Refined rows per lane from register pressure (x rows of this and the next
-// band stay in registers): 2 up to 8 float64 / 16 float32 features, else 1.
+// band stay in registers): 4 up to 4 float32 / 2 float64 features, 2 up to
+// 8 float64 / 16 float32 features, else 1.
 template <typename T, int DMAX>
-constexpr int pde_rows() { return DMAX * (int)sizeof(T) <= 64 ? 2 : 1; }
+constexpr int pde_rows() { return DMAX * (int)sizeof(T) <= 16 ? 4 : (DMAX * (int)sizeof(T) <= 64 ? 2 : 1); }
 
 template <typename T, int DMAX, int KIND>
 static hipError_t launch_pde(const PdeFwdArgs& a, size_t lds, hipStream_t s) {
