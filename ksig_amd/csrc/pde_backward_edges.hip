@@ -122,6 +122,25 @@ pde_bwd_edges_kernel(const PdeBwdEdgeArgs a) {
     for (int k = 0; k < DMAX; ++k) dot += xv[k] * yr[k];
     return kind == kRBF ? fast_exp2(-(xn + ynrm[yc] - T(2) * dot)) : dot;
   };
+  // The y row (and norm) of the original column a step needs, read a step
+  // ahead: column c of step s is Ncr - 1 - s, its original column c >> lam.
+  struct YRow {
+    T y[DMAX];
+    T yn;
+  };
+  auto yfetch = [&](YRow& yr, int s_at) __attribute__((always_inline)) {
+    int j = (Ncr - 1 - s_at) >> lam;
+    j = j < 0 ? 0 : (j >= Ly ? Ly - 1 : j);
+#pragma unroll
+    for (int k = 0; k < DMAX; ++k) yr.y[k] = ylds[j * DMAX + k];
+    yr.yn = ynrm[j];
+  };
+  auto gval_y = [&](const T* xv, T xn, const YRow& yr) __attribute__((always_inline)) -> T {
+    T dot = T(0);
+#pragma unroll
+    for (int k = 0; k < DMAX; ++k) dot += xv[k] * yr.y[k];
+    return kind == kRBF ? fast_exp2(-(xn + yr.yn - T(2) * dot)) : dot;
+  };
   // stage the x rows of the lane's refined rows r0, r0-1, ... into its slots
   auto stage = [&](int r0) __attribute__((always_inline)) {
 #pragma unroll
@@ -207,6 +226,8 @@ pde_bwd_edges_kernel(const PdeBwdEdgeArgs a) {
     u_self[q] = u_bp[q] = 1.0;
   }
   int s = -lane;
+  YRow ycur;
+  yfetch(ycur, s);
   const int total = nb * Pc + kWave - 1;
   // band b+1 is staged at step b*Pc + 63: every lane has taken band b by then
   int next_stage = kWave - 1;
@@ -253,11 +274,11 @@ pde_bwd_edges_kernel(const PdeBwdEdgeArgs a) {
             gtR = gtL[q];
             gbR = gbL[q];
           }
-          gtL[q] = gval(xt[q], xtn[q], j);
-          gbL[q] = gval(xb[q], xbn[q], j);
+          gtL[q] = gval_y(xt[q], xtn[q], ycur);
+          gbL[q] = gval_y(xb[q], xbn[q], ycur);
           M[q] = ((gbR - gbL[q]) - (gtR - gtL[q])) * inv4;
         } else {
-          M[q] = gval(xt[q], xtn[q], j) * inv4;
+          M[q] = gval_y(xt[q], xtn[q], ycur) * inv4;
         }
       }
 
@@ -316,6 +337,8 @@ pde_bwd_edges_kernel(const PdeBwdEdgeArgs a) {
       in_u = u_new;
     }
 
+    // the next step's y row, in flight while the flows are handed over
+    yfetch(ycur, s + 1 == Pc ? 0 : s + 1);
 #pragma unroll
     for (int k = 0; k < 6; ++k) fl[k] = in[k];
     flu = in_u;
