@@ -112,3 +112,29 @@ def test_global_alignment_argument_checks():
   assert g(*bad) == 1001 and b"positive" in lib.ksig_last_error()
   bad = list(args); bad[9] = 40  # d beyond the built buckets
   assert g(*bad) == 1002
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libksig_hip.so not built")
+def test_random_warping_series_argument_checks():
+  """Validation of the random warping series entry points runs before any
+  device call."""
+  lib = ctypes.CDLL(LIB)
+  lib.ksig_last_error.restype = ctypes.c_char_p
+  vp, ci, ci64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64
+  f = lib.ksig_random_warping_series
+  f.argtypes = [ci, vp, vp, vp, ci, ci64, ci, vp, vp, ci, ci, vp, ci, vp, vp]
+  ok = [0, None, None, None, 10, 0, 3, None, None, 5, 40, None, 1, None, None]
+  assert f(*ok) == 0  # no sequences: nothing to do
+  bad = list(ok); bad[0] = 4
+  assert f(*bad) == 1001 and b"dtype" in lib.ksig_last_error()
+  bad = list(ok); bad[6] = 0
+  assert f(*bad) == 1001
+  bad = list(ok); bad[6] = 33
+  assert f(*bad) == 1002
+  bad = list(ok); bad[5] = 2  # sequences given, null pointers
+  assert f(*bad) == 1001 and b"null" in lib.ksig_last_error()
+  g = lib.ksig_random_warping_series_gram
+  g.argtypes = [ci, vp, ci64, ci, ci, vp, ci, vp, ci, vp, vp]
+  assert g(0, None, 0, 10, 40, None, 5, None, 1, None, None) == 0
+  assert g(0, None, 3, -1, 40, None, 5, None, 1, None, None) == 1001
+  assert g(0, None, 3, 10, 40, None, 5, None, 1, None, None) == 1001  # null D

@@ -53,3 +53,18 @@ def test_oversubscribed_schedule_covers_every_pair_once():
     sizes = np.diff(wb)
     assert sizes.min() >= 1 and sizes.max() <= 64
     assert len(sizes) >= min(n // 64, 512 * ov) and len(sizes) <= 512 * ov + n // 64 + 1
+
+
+def test_rws_groups_partition_the_series():
+  from ksig_amd._ops import rws_groups
+  rng = np.random.default_rng(5)
+  for n_seq, n_w in [(512, 100), (1, 300), (4096, 100), (3, 1), (7, 2)]:
+    lens = rng.integers(1, 33, n_w)
+    g = rws_groups(lens, n_seq)
+    assert g[0] == 0 and g[-1] == n_w and np.all(np.diff(g) > 0)
+    rows = np.diff(np.concatenate([[0], np.cumsum(lens)])[g])
+    n_g = len(g) - 1
+    # enough groups to fill the chip where the rows allow, at least a band each
+    assert n_g == 1 or rows.min() >= 1
+    assert n_g <= max(1, lens.sum() // 64) and n_g <= n_w
+  assert list(rws_groups(np.zeros(0, np.int64), 4)) == [0]
