@@ -361,6 +361,20 @@ int ksig_random_warping_series_gram(int dtype, const void* D, int64_t n_seq, int
                                     const int32_t* groups, int n_groups, double* out,
                                     void* stream);
 
+/*
+ * Level recursion of the first-order low-rank signature features with a
+ * tensorized random projection: the level loop of
+ * signature_kern_first_order_low_rank (ksig/algorithms.py:218-246) with the
+ * TensorizedRandomProjection outer product (ksig/projections.py:566-584), as
+ * called by SignatureFeatures._compute_features (ksig/kernels.py:330-361).
+ * z: the projected increments [n_levels, n_seq, steps, n_comp] (dtype; level 0
+ * projected alone with its scaling, level m >= 1 the projection of U_m);
+ * out: the level features before normalisation, [n_seq, n_levels, n_comp]
+ * float64.  1 <= n_levels <= 8.
+ */
+int ksig_lowrank_levels(int dtype, const void* z, int n_levels, int64_t n_seq, int steps,
+                        int n_comp, double* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -14,5 +14,6 @@ from . import algorithms
 from . import kernels
 from . import tune
 from . import models
+from . import projections
 
 __version__ = "0.1.0"

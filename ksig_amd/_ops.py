@@ -422,3 +422,17 @@ def random_warping_series_gram(D: torch.Tensor, warp_lens: np.ndarray) -> torch.
       _lib.dtype_code(Dc.dtype), Dc.data_ptr(), n, lx, W, off.data_ptr(), n_w,
       g.data_ptr(), len(groups) - 1, out.data_ptr(), _lib.stream_handle(dev)))
   return out
+
+
+def lowrank_levels(Z: torch.Tensor) -> torch.Tensor:
+  """Low-rank level features [n_seq, n_levels, n_comp] (float64) from the
+  projected increments Z [n_levels, n_seq, steps, n_comp]."""
+  lib = _lib.load()
+  Zc = Z.contiguous()
+  n_levels, n, steps, P = Zc.shape
+  out = torch.empty((n, n_levels, P), device=Z.device, dtype=torch.float64)
+  if n == 0 or P == 0:
+    return out
+  _lib.check(lib.ksig_lowrank_levels(_lib.dtype_code(Zc.dtype), Zc.data_ptr(), n_levels, n,
+                                     steps, P, out.data_ptr(), _lib.stream_handle(Z.device)))
+  return out

@@ -343,3 +343,111 @@ class RandomWarpingSeries(static.features.KernelFeatures):
       nrm = utils.robust_sqrt(utils.squared_norm(P, axis=-1))
       P = P / torch.clamp(nrm, min=_EPS)[..., None]
     return P
+
+
+class SignatureFeatures(SignatureBase, static.features.KernelFeatures):
+  """Low-rank signature features (ksig/kernels.py:263-361): static features
+  per level (``indep_feat``), optional random projection, first-order
+  low-rank recursion (ksig/algorithms.py:175-246).  With a
+  ``TensorizedRandomProjection`` (RFSF-TRP) the projected increments come
+  from library GEMMs and the level recursion (exclusive time cumsum,
+  elementwise product, time sum of every level) is one HIP pass
+  (``ksig_lowrank_levels``).  Without a projection the level-m features are
+  the D^m iterated-sum coordinates (torch ops on the GPU, small D only).
+  Order >= 2 and other projections are not built."""
+
+  def __init__(self, n_levels: int = 5, order: int = 1, difference: bool = True,
+               normalize: bool = True, indep_feat: bool = True,
+               n_features: Optional[int] = None, static_features=None, projection=None):
+    super().__init__(n_levels=n_levels, order=order, difference=difference,
+                     normalize=normalize, n_features=n_features)
+    self.indep_feat = indep_feat
+    self.static_features = static_features
+    self.projection = projection
+
+  def _validate_data(self, X, reset: bool = False):
+    X = SignatureBase._validate_data(self, X, reset=reset)
+    self._validate_format(X, reset=reset)
+    return X
+
+  def _make_feature_components(self, X):
+    """ksig/kernels.py:295-328."""
+    from .projections import TensorizedRandomProjection
+    if self.order != 1:
+      raise NotImplementedError('low-rank signature features are built for order=1')
+    if self.projection is not None and not isinstance(self.projection,
+                                                      TensorizedRandomProjection):
+      raise NotImplementedError('low-rank signature features are built with a '
+                                'TensorizedRandomProjection or no projection')
+    if self.static_features is not None:
+      seq_shape = tuple(X.shape[:-1]) + (-1,)
+      Xf = X.reshape(-1, X.shape[-1])
+      if self.indep_feat:
+        hps = self.static_features.get_params()
+        cls = self.static_features.__class__
+        self.static_features_ = [cls(**hps).fit(Xf) for _ in range(self.n_levels)]
+        U = self.static_features_[0].transform(Xf, return_on_gpu=True)
+      else:
+        self.static_features_ = self.static_features.fit(Xf)
+        U = self.static_features_.transform(Xf, return_on_gpu=True)
+      U = U.reshape(seq_shape)
+    else:
+      U = X
+    if self.projection is not None:
+      hps = self.projection.get_params()
+      cls = self.projection.__class__
+      self.projections_ = [cls(**hps).fit(U)]
+      V = self.projections_[0](U, return_on_gpu=True)
+      self.projections_ += [cls(**hps).fit(V, Y=U) for _ in range(1, self.n_levels)]
+    else:
+      self.projections_ = None
+
+  def _lifted(self, X):
+    """Per-level static features U_m [n, L, D] (or X itself)."""
+    if self.static_features is None:
+      return [X] * self.n_levels
+    seq_shape = tuple(X.shape[:-1]) + (-1,)
+    Xf = X.reshape(-1, X.shape[-1])
+    if isinstance(self.static_features_, list):
+      return [f.transform(Xf, return_on_gpu=True).reshape(seq_shape)
+              for f in self.static_features_]
+    U = self.static_features_.transform(Xf, return_on_gpu=True).reshape(seq_shape)
+    return [U] * self.n_levels
+
+  def _compute_features(self, X) -> torch.Tensor:
+    """ksig/kernels.py:330-361."""
+    if X.ndim != 3:
+      raise ValueError('SignatureFeatures takes sequence arrays of ndim==3.')
+    U = self._lifted(X)
+    if self.difference:
+      U = [torch.diff(u, dim=1) for u in U]
+    n = X.shape[0]
+    ones = torch.ones((n, 1), device=X.device, dtype=torch.float64)
+    if self.projections_ is not None:
+      # projected increments: level 0 alone (scaled), level m >= 1 the
+      # projection of U_m that multiplies the running sums (library GEMMs)
+      Z = [self.projections_[0]._project(U[0])]
+      for u, p in zip(U[1:], self.projections_[1:]):
+        Z.append(p.scaling_ * (u.reshape(-1, u.shape[-1]) @ p.components_.to(u.dtype)
+                               ).reshape(tuple(u.shape[:-1]) + (-1,)))
+      Z = torch.stack(Z, 0)
+      lev = _ops.lowrank_levels(Z)
+      P = [ones] + [lev[:, m] for m in range(self.n_levels)]
+    else:
+      R = U[0].to(torch.float64)
+      P = [ones, R.sum(1)]
+      for i in range(1, self.n_levels):
+        R = torch.cumsum(R, 1) - R  # exclusive cumsum over time
+        R = (R[..., :, None] * U[i].to(torch.float64)[..., None, :]).reshape(n, R.shape[1], -1)
+        P.append(R.sum(1))
+    if not self.normalize:
+      return torch.cat(P, -1).to(X.dtype)
+    P = [p / torch.clamp(utils.robust_sqrt(utils.squared_norm(p, axis=-1)), min=_EPS)[..., None]
+         for p in P]
+    return (torch.cat(P, -1) / np.sqrt(self.n_levels + 1)).to(X.dtype)
+
+  def _K(self, X, Y=None):
+    return static.features.KernelFeatures._K(self, X, Y)
+
+  def _Kdiag(self, X):
+    return static.features.KernelFeatures._Kdiag(self, X)

@@ -1,0 +1,109 @@
+"""Random projections for the low-rank signature features: the
+``TensorizedRandomProjection`` of ksig/projections.py:465-584 (rank 1) over
+the ``RandomProjection`` base (:20-187).  Components are drawn on the host
+with NumPy (the reference draws them with CuPy)."""
+
+from __future__ import annotations
+
+from abc import ABCMeta, abstractmethod
+from typing import Optional
+
+import numpy as np
+import torch
+from sklearn.base import BaseEstimator, TransformerMixin
+from sklearn.utils.validation import check_is_fitted
+
+from . import utils
+from .static.features import check_random_state
+
+
+class RandomProjection(BaseEstimator, TransformerMixin, metaclass=ABCMeta):
+  """Base class of random projections (ksig/projections.py:20-187)."""
+
+  def __init__(self, n_components: int = 100, random_state=None):
+    self.n_components = utils.check_positive_value(n_components, 'n_components')
+    self.random_state = check_random_state(random_state)
+
+  @abstractmethod
+  def _make_projection_components(self, X, Y=None):
+    """Initializes internal variables, called by ``fit``."""
+
+  def _validate_format(self, X, Y=None, reset: bool = False):
+    n_features = X.shape[-1] * (Y.shape[-1] if Y is not None else 1)
+    if reset or not hasattr(self, 'n_features_') or self.n_features_ is None:
+      self.n_features_ = n_features
+    elif n_features != self.n_features_:
+      raise ValueError('Data has a different number of features than during fit.',
+                       f'({n_features} != {self.n_features_})')
+
+  def fit(self, X, Y=None) -> 'RandomProjection':
+    X = utils.as_device(X)
+    Y = utils.as_device(Y) if Y is not None else None
+    self._validate_format(X, Y=Y, reset=True)
+    self._make_projection_components(X, Y=Y)
+    return self
+
+  @abstractmethod
+  def _project(self, X):
+    """Projects X."""
+
+  @abstractmethod
+  def _project_outer_prod(self, X, Y):
+    """Projects the outer product of X and Y along the last axis."""
+
+  def transform(self, X, Y=None, return_on_gpu: bool = False):
+    check_is_fitted(self)
+    X = utils.as_device(X)
+    Y = utils.as_device(Y) if Y is not None else None
+    self._validate_format(X, Y=Y)
+    P = self._project_outer_prod(X, Y) if Y is not None else self._project(X)
+    return P if return_on_gpu else utils.to_host(P)
+
+  def __call__(self, X, Y=None, return_on_gpu: bool = False):
+    return self.transform(X, Y=Y, return_on_gpu=return_on_gpu)
+
+
+class TensorizedRandomProjection(RandomProjection):
+  """Tensorized random projection of rank 1 (ksig/projections.py:465-584):
+  X -> X C / sqrt(n_components) alone, and (X, Y) -> X * (Y C) for an
+  already-projected X (the recursive step of a tensor's projection).  Ranks
+  above 1 are not built (the reference's components hold n_components
+  columns whatever the rank)."""
+
+  def __init__(self, n_components: int = 100, rank: int = 1, random_state=None):
+    super().__init__(n_components=n_components, random_state=random_state)
+    self.rank = utils.check_positive_value(rank, 'rank')
+    if rank != 1:
+      raise NotImplementedError('TensorizedRandomProjection is built for rank 1')
+
+  def _validate_format(self, X, Y=None, reset: bool = False):
+    """ksig/projections.py:500-536."""
+    n_features = Y.shape[-1] if Y is not None else X.shape[-1]
+    if reset or not hasattr(self, 'n_features_') or self.n_features_ is None:
+      self.n_features_ = n_features
+    if Y is not None:
+      if Y.shape[-1] != self.n_features_:
+        raise ValueError('Data `Y` has a different number of features than during fit.',
+                         f'({Y.shape[-1]} != {self.n_features_})')
+      if X.shape[-1] != self.n_components * self.rank:
+        raise ValueError('Data `X` has incorrect number of features.',
+                         f'({X.shape[-1]} != {self.n_components * self.rank})')
+    elif X.shape[-1] != self.n_features_:
+      raise ValueError('Data `X` has a different number of features than during fit.',
+                       f'({X.shape[-1]} != {self.n_features_})')
+
+  def _make_projection_components(self, X, Y=None):
+    """ksig/projections.py:538-548."""
+    c = self.random_state.normal(size=[self.n_features_, self.n_components])
+    self.components_ = torch.as_tensor(c, device=X.device, dtype=X.dtype)
+    self.scaling_ = 1. / np.sqrt(self.n_components) if Y is None else 1.
+
+  def _project(self, X):
+    C = self.components_.to(device=X.device, dtype=X.dtype)
+    return self.scaling_ * (X.reshape(-1, self.n_features_) @ C).reshape(
+        tuple(X.shape[:-1]) + (-1,))
+
+  def _project_outer_prod(self, X, Y):
+    C = self.components_.to(device=Y.device, dtype=Y.dtype)
+    return self.scaling_ * X * (Y.reshape(-1, self.n_features_) @ C).reshape(
+        tuple(Y.shape[:-1]) + (-1,))

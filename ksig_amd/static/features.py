@@ -80,3 +80,35 @@ class KernelFeatures(Kernel, TransformerMixin, metaclass=ABCMeta):
     """ksig/static/features.py:157-177."""
     check_is_fitted(self)
     return super().__call__(X, Y=Y, diag=diag, return_on_gpu=return_on_gpu)
+
+
+class StaticFeatures(KernelFeatures):
+  """Static features flatten the time axis (ksig/static/features.py:180-202)."""
+
+  def _validate_data(self, X, reset: bool = False):
+    if X.ndim > 2:
+      X = X.reshape(tuple(X.shape[:-2]) + (-1,))
+    return super()._validate_data(X, reset=reset)
+
+
+class RandomFourierFeatures(StaticFeatures):
+  """Random Fourier features of the RBF kernel (ksig/static/features.py:288-342):
+  [sin(X W), cos(X W)] / sqrt(n_components) with W ~ N(0, 1/bandwidth^2)
+  of shape [n_features, n_components].  The product X W is a library GEMM;
+  the weights are drawn on the host with NumPy (the reference draws them with
+  CuPy, so equal seeds give different weights)."""
+
+  def __init__(self, bandwidth: float = 1., n_components: int = 100, random_state=None):
+    super().__init__(n_components=n_components, random_state=random_state)
+    self.bandwidth = utils.check_positive_value(bandwidth, 'bandwidth')
+
+  def _make_feature_components(self, X):
+    self.n_components_ = self.n_components
+    w = self.random_state.normal(size=[self.n_features_, self.n_components_])
+    self.random_weights_ = torch.as_tensor(w / self.bandwidth, device=X.device, dtype=X.dtype)
+
+  def _compute_features(self, X):
+    W = self.random_weights_.to(device=X.device, dtype=X.dtype)
+    proj = X.reshape(-1, self.n_features_) @ W
+    F = torch.cat([torch.sin(proj), torch.cos(proj)], -1) / np.sqrt(self.n_components)
+    return F.reshape(tuple(X.shape[:-1]) + (-1,))

@@ -23,6 +23,10 @@ difference it, per-level exclusive 2-D cumulative sums):
                               log-space normalisation :191-209)
 * random warping series ..... ksig/algorithms.py:565-659 (DTW recursion),
                               ksig/kernels.py:459-477 (features)
+* low-rank signature features ksig/algorithms.py:148-246 (first order),
+                              ksig/kernels.py:263-361, random Fourier features
+                              ksig/static/features.py:288-342, tensorized
+                              random projection ksig/projections.py:465-584
 
 Additions that have no reference counterpart (build extensions):
 ``dyadic_order`` for the PDE (each ΔΔ cell refined into 2^λ×2^λ sub-cells
@@ -36,7 +40,10 @@ PDE one-cell closed form, PDE dyadic refinement vs I0(2*sqrt(ab)),
 stuttering / translation invariance, symmetry and the normalised diagonal;
 GA vs explicit enumeration of alignment paths in exp space and vs Delannoy
 numbers (M' = 1); DTW vs explicit enumeration of warping paths and the
-one-point closed forms.
+one-point closed forms; low-rank features without projection vs the
+first-order DP with a linear static kernel on the lifted path, with a
+tensorized projection vs brute-force iterated-sum tensors contracted with
+the rank-1 projection tensors.
 """
 
 from __future__ import annotations
@@ -482,6 +489,63 @@ def dtw_path_min(D: np.ndarray) -> float:
 
   walk(0, 0, 0.0)
   return best
+
+
+# -----------------------------------------------------------------------------
+# Low-rank signature features (ksig/algorithms.py:148-246, ksig/kernels.py:263-361,
+# ksig/static/features.py:288-342, ksig/projections.py:465-584)
+# -----------------------------------------------------------------------------
+
+def rff_features(X: np.ndarray, weights: np.ndarray) -> np.ndarray:
+  """RandomFourierFeatures._compute_features (ksig/static/features.py:322-342)
+  for given weights [d, D]: [sin(XW), cos(XW)] / sqrt(D)."""
+  proj = X @ weights
+  return np.concatenate([np.sin(proj), np.cos(proj)], -1) / np.sqrt(weights.shape[1])
+
+
+def signature_kern_low_rank_fo(U, n_levels, difference=True, projections=None):
+  """First-order low-rank signature features of ksig/algorithms.py:175-246
+  with ``return_levels=True``: U is a list of n_levels arrays [n, L, D] (one
+  per level, independent static features) or one array; ``projections`` is
+  None or a list of TRP (components [D, P], scaling) pairs, level 0 projected
+  alone (:199-201) and level i >= 1 as the elementwise product of the
+  exclusive time cumsum with the projection of U_i (:224-237, TRP outer
+  product ksig/projections.py:566-584).  Returns the list of level
+  features [n, 1], [n, D or P], ..."""
+  Us = U if isinstance(U, list) else [U] * n_levels
+  if difference:
+    Us = [np.diff(u, axis=1) for u in Us]
+  n = Us[0].shape[0]
+  P = [np.ones((n, 1), dtype=Us[0].dtype)]
+  if projections is None:
+    R = Us[0].copy()
+  else:
+    C0, s0 = projections[0]
+    R = s0 * (Us[0] @ C0)
+  P.append(R.sum(1))
+  for i in range(1, n_levels):
+    R = multi_cumsum(R, exclusive=True, axis=1)
+    if projections is None:
+      R = (R[..., :, None] * Us[i][..., None, :]).reshape(R.shape[:2] + (-1,))
+    else:
+      Ci, si = projections[i]
+      R = si * R * (Us[i] @ Ci)
+    P.append(R.sum(1))
+  return P
+
+
+def signature_features(X, n_levels, rff_weights, trp=None, difference=True, normalize=True):
+  """SignatureFeatures._compute_features (ksig/kernels.py:330-361) with
+  RandomFourierFeatures static features (one weight matrix per level,
+  ``indep_feat``) and optionally TensorizedRandomProjection components:
+  level features, each normalised to unit norm, concatenated and divided by
+  sqrt(n_levels + 1)."""
+  U = [rff_features(X, w) for w in rff_weights]
+  P = signature_kern_low_rank_fo(U, n_levels, difference, trp)
+  if not normalize:
+    return np.concatenate(P, -1)
+  P = [p / np.maximum(np.sqrt(np.maximum(np.sum(p * p, -1), EPS)), EPS)[..., None] for p in P]
+  return np.concatenate(P, -1) / np.sqrt(n_levels + 1)
 
 
 # -----------------------------------------------------------------------------

@@ -1,0 +1,69 @@
+"""GPU parity of the low-rank signature features (RFSF-TRP: random Fourier
+features + tensorized random projection, first order) against the FP64
+oracle, given the object's random weights and components.
+
+Tolerances: level features within 1e-11 (float64 inputs) / 1e-5 (float32,
+normwise per sequence and level); normalised features 1e-5."""
+
+import numpy as np
+import pytest
+import torch
+
+import ksig_amd as ksig
+from ksig_amd import _ops
+from oracle import sigkern as O
+from tests.gpu_util import normwise, to_dev
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-11), (np.float32, 1e-5)])
+@pytest.mark.parametrize("n_levels", [1, 3, 5, 8])
+def test_level_scan(dtype, tol, n_levels):
+  rng = np.random.default_rng(n_levels)
+  n, L, D, P = 6, 40, 5, 33
+  U = rng.standard_normal((n, L, D)) * 0.3
+  comps = [rng.standard_normal((D, P)) for _ in range(n_levels)]
+  proj = [(comps[0], 1 / np.sqrt(P))] + [(c, 1.0) for c in comps[1:]]
+  ref = O.signature_kern_low_rank_fo(U, n_levels, True, proj)
+  dU = np.diff(U, axis=1)
+  Z = np.stack([dU @ comps[0] / np.sqrt(P)] + [dU @ c for c in comps[1:]], 0).astype(dtype)
+  got = _ops.lowrank_levels(to_dev(Z)).cpu().numpy()
+  assert got.shape == (n, n_levels, P)
+  for m in range(n_levels):
+    num = np.abs(got[:, m] - ref[m + 1]).max(-1)
+    den = np.maximum(np.abs(ref[m + 1]).max(-1), 1e-300)
+    assert float(np.max(num / den)) < tol, m
+
+
+@pytest.mark.parametrize("normalize", [True, False])
+def test_rfsf_trp_class(normalize):
+  rng = np.random.default_rng(7)
+  X = O.brownian(rng, 9, 30, 4).astype(np.float32)
+  Y = O.brownian(rng, 5, 25, 4).astype(np.float32)
+  sf = ksig.kernels.SignatureFeatures(
+      n_levels=4, normalize=normalize,
+      static_features=ksig.static.features.RandomFourierFeatures(n_components=16, random_state=1),
+      projection=ksig.projections.TensorizedRandomProjection(n_components=24, random_state=2))
+  sf.fit(X)
+  w = [f.random_weights_.double().cpu().numpy() for f in sf.static_features_]
+  trp = [(p.components_.double().cpu().numpy(), p.scaling_) for p in sf.projections_]
+  F = sf.transform(X)
+  ref = O.signature_features(X.astype(np.float64), 4, w, trp, normalize=normalize)
+  assert F.shape == (9, 1 + 4 * 24) and normwise(F, ref) < 1e-5
+  K = sf(X, Y)
+  refK = ref @ O.signature_features(Y.astype(np.float64), 4, w, trp, normalize=normalize).T
+  assert K.shape == (9, 5) and normwise(K, refK) < 1e-5
+  if normalize:
+    np.testing.assert_allclose(sf(X, diag=True), 1.0, rtol=1e-5)
+
+
+def test_unprojected_features_match_oracle():
+  rng = np.random.default_rng(8)
+  X = O.brownian(rng, 4, 12, 2)
+  sf = ksig.kernels.SignatureFeatures(n_levels=3, normalize=False).fit(X)
+  F = sf.transform(X)
+  ref = np.concatenate(O.signature_kern_low_rank_fo(X, 3), -1)
+  assert F.shape == (4, 1 + 2 + 4 + 8) and normwise(F, ref) < 1e-11
+  with pytest.raises(NotImplementedError):
+    ksig.kernels.SignatureFeatures(n_levels=3, order=2).fit(X)

@@ -1,0 +1,75 @@
+"""Pins of the low-rank signature feature oracle (no GPU): without a
+projection the features' inner products are the first-order DP signature
+kernel with a linear static kernel on the lifted path; with a tensorized
+random projection each level is the iterated-sum tensor contracted with the
+rank-1 projection tensors (brute force); the normalised map has unit norm
+per level."""
+
+import itertools
+
+import numpy as np
+import pytest
+
+from oracle import sigkern as O
+
+
+def test_unprojected_features_equal_first_order_dp_linear_kernel():
+  rng = np.random.default_rng(0)
+  X = rng.standard_normal((3, 7, 2))
+  Y = rng.standard_normal((2, 6, 2))
+  for n_levels in (1, 2, 4):
+    Px = O.signature_kern_low_rank_fo(X, n_levels)
+    Py = O.signature_kern_low_rank_fo(Y, n_levels)
+    M = O.pairwise_gram(X, Y, 'linear', 1.0)
+    ref = O.signature_kern(M, n_levels, 1, True, return_levels=True)  # [n_levels+1, 3, 2]
+    for m in range(n_levels + 1):
+      np.testing.assert_allclose(Px[m] @ Py[m].T, ref[m], rtol=1e-11, atol=1e-12)
+
+
+def _iterated_sum_tensor(dU, m):
+  """S_m = sum_{t1 < ... < tm} dU[t1] (x) ... (x) dU[tm] by enumeration."""
+  L, D = dU.shape
+  S = np.zeros((D,) * m)
+  for ts in itertools.combinations(range(L), m):
+    T = dU[ts[0]]
+    for t in ts[1:]:
+      T = np.multiply.outer(T, dU[t])
+    S += T
+  return S
+
+
+@pytest.mark.parametrize("n_levels", [1, 3])
+def test_trp_features_equal_contracted_iterated_sums(n_levels):
+  rng = np.random.default_rng(1)
+  U = rng.standard_normal((2, 6, 3))
+  P = 4
+  comps = [rng.standard_normal((3, P)) for _ in range(n_levels)]
+  proj = [(comps[0], 1.0 / np.sqrt(P))] + [(c, 1.0) for c in comps[1:]]
+  feats = O.signature_kern_low_rank_fo(U, n_levels, True, proj)
+  for a in range(2):
+    dU = np.diff(U[a], axis=0)
+    for m in range(1, n_levels + 1):
+      S = _iterated_sum_tensor(dU, m)
+      for p in range(P):
+        T = comps[0][:, p] / np.sqrt(P)
+        for k in range(1, m):
+          T = np.multiply.outer(T, comps[k][:, p])
+        assert abs(feats[m][a, p] - np.sum(S * T)) < 1e-10 * max(1.0, np.abs(S).max())
+
+
+def test_normalised_feature_map():
+  rng = np.random.default_rng(2)
+  X = O.brownian(rng, 5, 20, 3)
+  w = [rng.standard_normal((3, 8)) for _ in range(4)]
+  c = [rng.standard_normal((16, 6)) for _ in range(4)]
+  trp = [(c[0], 1 / np.sqrt(6))] + [(ci, 1.0) for ci in c[1:]]
+  F = O.signature_features(X, 4, w, trp)
+  assert F.shape == (5, 1 + 4 * 6)
+  np.testing.assert_allclose(np.sum(F * F, -1), 1.0, rtol=1e-12)
+  # the planted mistake: an inclusive cumsum breaks the brute-force pin
+  U = rng.standard_normal((1, 5, 2))
+  good = O.signature_kern_low_rank_fo(U, 2)[2][0]
+  dU = np.diff(U[0], axis=0)
+  bad = np.einsum('ti,tj->ij', np.cumsum(dU, 0), dU).reshape(-1)
+  assert np.abs(good - _iterated_sum_tensor(dU, 2).reshape(-1)).max() < 1e-12
+  assert np.abs(bad - good).max() > 1e-3
