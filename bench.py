@@ -39,7 +39,8 @@ PEAK_HBM_GBS = 8000.0
 # with the kernel version and schedule they were measured on (c3: the
 # autotuned schedule, 2738 waves).
 MEASURED_TRAFFIC_BYTES = {"c1": 38658 * 1024, "c3": (663693 * 2 + 581215) * 1024,
-                          "gak": (21130 * 2 + 4104) * 1024, "rws": (13451 * 2 + 3200) * 1024}
+                          "gak": (21130 * 2 + 4104) * 1024, "rws": (13451 * 2 + 3200) * 1024,
+                          "rfsf": (230276 * 2 + 3906) * 1024}
 
 
 def brownian(gen: torch.Generator, n, L, d, device, dtype=torch.float32):
@@ -159,7 +160,8 @@ def cpu_baseline(X64: np.ndarray, n_levels: int, sigma: float, seconds: float):
 
 def main():
   ap = argparse.ArgumentParser()
-  ap.add_argument("--workload", default="c1", choices=["c1", "c2", "c3", "c4", "gak", "rws"],
+  ap.add_argument("--workload", default="c1",
+                  choices=["c1", "c2", "c3", "c4", "gak", "rws", "rfsf"],
                   help="c1: 512 x L=200 x d=8 symmetric Gram (the headline line); "
                        "c3: 4096 uneven-length pairs, L in [32, 1024], d=16, 8 levels, order 2; "
                        "c4: MMD^2 training step, 1024 x 1024 paths, L=128, d=32, 6 levels; "
@@ -213,6 +215,8 @@ def main():
     return bench_gak(args)
   if args.workload == "rws":
     return bench_rws(args)
+  if args.workload == "rfsf":
+    return bench_rfsf(args)
 
   import ksig_amd
   from ksig_amd import _lib, _ops
@@ -591,6 +595,83 @@ def bench_rws(args):
                            "sample": f"{done} random sequences against the first 10 series, "
                                      f"FP64 NumPy oracle (vectorised over {batch} sequences), "
                                      f"{dt:.1f} s on 1 host core"}
+  print(json.dumps(res), flush=True)
+
+
+def bench_rfsf(args):
+  """Random Fourier signature features with a tensorized random projection
+  (RFSF-TRP), the reference README's example shape: 1000 sequences, L=200,
+  d=100, 5 levels, 100 RFF components (200 features), 100 projection
+  components.  Times the whole transform (RFF and projection GEMMs in
+  hipBLASLt, the HIP level scan) and the level scan alone."""
+  import ksig_amd as ksig
+  from ksig_amd import _ops
+  from oracle import sigkern as O
+  dev = torch.device("cuda", 0)
+  torch.cuda.set_device(dev)
+  n, L, d, nl, D, P = 1000, 200, 100, 5, 100, 100
+  g = torch.Generator(device=dev).manual_seed(0)
+  X = torch.randn((n, L, d), generator=g, device=dev, dtype=torch.float32)
+  sf = ksig.kernels.SignatureFeatures(
+      n_levels=nl, static_features=ksig.static.features.RandomFourierFeatures(
+          n_components=D, random_state=0),
+      projection=ksig.projections.TensorizedRandomProjection(n_components=P, random_state=1))
+  sf.fit(X)
+
+  def step():
+    return sf.transform(X, return_on_gpu=True)
+  for _ in range(args.warmup):
+    step()
+  torch.cuda.synchronize()
+  t0 = time.perf_counter()
+  for _ in range(args.steps):
+    F = step()
+  torch.cuda.synchronize()
+  ms_per_step = (time.perf_counter() - t0) * 1e3 / args.steps
+  # the level scan alone on the projected increments of this input
+  U = [torch.diff(f.transform(X.reshape(-1, d), return_on_gpu=True).reshape(n, L, -1), dim=1)
+       for f in sf.static_features_]
+  Z = [sf.projections_[0]._project(U[0])]
+  for u, p in zip(U[1:], sf.projections_[1:]):
+    Z.append(p.scaling_ * (u.reshape(-1, u.shape[-1]) @ p.components_).reshape(n, L - 1, -1))
+  Z = torch.stack(Z, 0).contiguous()
+  stream = torch.cuda.current_stream(dev)
+  e0 = torch.cuda.Event(enable_timing=True)
+  e1 = torch.cuda.Event(enable_timing=True)
+  _ops.lowrank_levels(Z)
+  e0.record(stream)
+  for _ in range(args.steps):
+    _ops.lowrank_levels(Z)
+  e1.record(stream)
+  torch.cuda.synchronize()
+  scan_ms = e0.elapsed_time(e1) / args.steps
+  scan_bytes = Z.numel() * 4 + n * nl * P * 8
+  # features of 4 sequences vs the FP64 oracle with the same random components
+  w = [f.random_weights_.double().cpu().numpy() for f in sf.static_features_]
+  trp = [(p.components_.double().cpu().numpy(), p.scaling_) for p in sf.projections_]
+  idx = [0, 1, 500, 999]
+  ref = O.signature_features(X[idx].double().cpu().numpy(), nl, w, trp)
+  err = float(np.max(np.abs(F[idx].double().cpu().numpy() - ref).max(-1) / np.abs(ref).max(-1)))
+  achieved = scan_bytes / (scan_ms * 1e-3) / 1e9
+  res = {
+      "metric": "RFSF-TRP feature throughput (sequences x L / s), fp32, whole transform",
+      "value": n * L / (ms_per_step * 1e-3),
+      "unit": "sequence points/s",
+      "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+      "ms_per_step": ms_per_step,
+      "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+      "dtype": "f32", "data": "synthetic (seeded N(0,1) sequences)",
+      "config": {"workload": "RFSF-TRP: 1000 x L=200 x d=100, 5 levels, 100 RFF components, "
+                             "100 projection components (501 features)",
+                 "features": int(F.shape[1])},
+      "max_rel_err_vs_fp64": err,
+      "err_sample": "4 sequences, normwise over their normalised feature vectors",
+      "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                   "frac": achieved / PEAK_HBM_GBS, "traffic": MEASURED_TRAFFIC_BYTES["rfsf"],
+                   "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE)",
+                   "kernel": "lowrank_scan_kernel<float,5>", "kernel_ms": scan_ms,
+                   "bytes_per_launch": scan_bytes},
+  }
   print(json.dumps(res), flush=True)
 
 
