@@ -375,6 +375,17 @@ int ksig_random_warping_series_gram(int dtype, const void* D, int64_t n_seq, int
 int ksig_lowrank_levels(int dtype, const void* z, int n_levels, int64_t n_seq, int steps,
                         int n_comp, double* out, void* stream);
 
+/*
+ * The same level recursion with a DiagonalProjection of internal size 2
+ * (ksig/projections.py:587-674) on random Fourier features (RFSF-DP):
+ * du: the differenced features [n_levels, n_seq, steps, 2 * half] (dtype; sin
+ * half then cos half); out: the level features before normalisation,
+ * [n_seq, sum_i 2^(i+1) * half] float64, level i's block laid out as
+ * [2^(i+1), half].  1 <= n_levels <= 5.
+ */
+int ksig_lowrank_levels_diag(int dtype, const void* du, int n_levels, int64_t n_seq,
+                             int steps, int half, double* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

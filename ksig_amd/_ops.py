@@ -436,3 +436,20 @@ def lowrank_levels(Z: torch.Tensor) -> torch.Tensor:
   _lib.check(lib.ksig_lowrank_levels(_lib.dtype_code(Zc.dtype), Zc.data_ptr(), n_levels, n,
                                      steps, P, out.data_ptr(), _lib.stream_handle(Z.device)))
   return out
+
+
+def lowrank_levels_diag(dU: torch.Tensor) -> torch.Tensor:
+  """RFSF-DP level features [n_seq, sum_i 2^(i+1) * half] (float64) from the
+  differenced random Fourier features dU [n_levels, n_seq, steps, 2 * half]."""
+  lib = _lib.load()
+  dc = dU.contiguous()
+  n_levels, n, steps, width = dc.shape
+  half = width // 2
+  tot = (2 << n_levels) - 2
+  out = torch.empty((n, tot * half), device=dU.device, dtype=torch.float64)
+  if n == 0 or half == 0:
+    return out
+  _lib.check(lib.ksig_lowrank_levels_diag(_lib.dtype_code(dc.dtype), dc.data_ptr(), n_levels, n,
+                                          steps, half, out.data_ptr(),
+                                          _lib.stream_handle(dU.device)))
+  return out

@@ -18,6 +18,8 @@
 namespace ksig {
 hipError_t lowrank_scan_launch(int dtype, const void* z, int n_levels, int64_t n_seq,
                                int steps, int n_comp, double* out, hipStream_t s);
+hipError_t lowrank_diag_scan_launch(int dtype, const void* du, int n_levels, int64_t n_seq,
+                                    int steps, int half, double* out, hipStream_t s);
 }  // namespace ksig
 
 namespace {
@@ -907,6 +909,21 @@ int ksig_lowrank_levels(int dtype, const void* z, int n_levels, int64_t n_seq, i
   hipError_t e = ksig::lowrank_scan_launch(dtype, z, n_levels, n_seq, steps, n_comp, out,
                                           (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "lowrank_scan launch");
+  return 0;
+}
+
+int ksig_lowrank_levels_diag(int dtype, const void* du, int n_levels, int64_t n_seq,
+                             int steps, int half, double* out, void* stream) {
+  if (dtype != KSIG_F32 && dtype != KSIG_F64) return fail(KSIG_EINVAL, "bad dtype %d", dtype);
+  if (n_levels < 1) return fail(KSIG_EINVAL, "n_levels must be positive");
+  if (n_levels > 5) return fail(KSIG_EUNSUPPORTED, "n_levels = %d > 5 is not built", n_levels);
+  if (n_seq < 0 || steps < 0 || half < 0) return fail(KSIG_EINVAL, "negative shape");
+  if (n_seq * (int64_t)half > 0x7fffffffLL * 256LL) return fail(KSIG_EINVAL, "too many sequences");
+  if (n_seq == 0 || half == 0) return 0;
+  if (!out || (steps > 0 && !du)) return fail(KSIG_EINVAL, "null device pointer");
+  hipError_t e = ksig::lowrank_diag_scan_launch(dtype, du, n_levels, n_seq, steps, half, out,
+                                               (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "lowrank_diag_scan launch");
   return 0;
 }
 

@@ -73,6 +73,86 @@ static hipError_t dispatch_scan(const void* z, int n_levels, int64_t n_seq, int 
   return hipErrorInvalidValue;
 }
 
+// The same recursion with a diagonal projection (ksig/projections.py:587-674)
+// of internal size 2 on random Fourier features: level i's increments are
+// the (sin, cos) pair u_i(t) = (dU_i[t, k], dU_i[t, H + k]) of component k,
+// and R_i(t) = sqrt(H) C_{i-1}(t) (x) u_i(t) has 2^(i+1) entries per
+// component (the q-index of level 0 most significant), so the level features
+// are [2^(i+1), H] per sequence.  One thread per (sequence, component k)
+// keeps every level's running sums and time sums in float64 registers.
+template <typename T, int NL>
+__global__ void __launch_bounds__(256)
+lowrank_diag_scan_kernel(const T* __restrict__ du, int64_t n_seq, int steps, int half,
+                         double* __restrict__ out) {
+  constexpr int kTot = (2 << NL) - 2;  // sum over levels of 2^(i+1)
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n_seq * half) return;
+  const int64_t a = idx / half;
+  const int k = (int)(idx - a * half);
+  const int width = 2 * half;
+  const int64_t level_stride = n_seq * (int64_t)steps * width;
+  const T* up = du + a * (int64_t)steps * width + k;
+  const double sc = sqrt((double)half);
+  double C[kTot], S[kTot];
+#pragma unroll
+  for (int j = 0; j < kTot; ++j) C[j] = S[j] = 0.0;
+  for (int t = 0; t < steps; ++t) {
+    double R[kTot];
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+      const T* ut = up + i * level_stride + (int64_t)t * width;
+      const double u0 = (double)ut[0], u1 = (double)ut[half];
+      const int off = (2 << i) - 2;  // first entry of level i
+      if (i == 0) {
+        R[0] = u0;
+        R[1] = u1;
+      } else {
+        const int prev = (2 << (i - 1)) - 2;
+#pragma unroll
+        for (int j = 0; j < (1 << i); ++j) {
+          R[off + 2 * j] = sc * C[prev + j] * u0;
+          R[off + 2 * j + 1] = sc * C[prev + j] * u1;
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kTot; ++j) {
+      C[j] += R[j];
+      S[j] += R[j];
+    }
+  }
+  double* o = out + a * (int64_t)kTot * half + k;
+#pragma unroll
+  for (int j = 0; j < kTot; ++j) o[(int64_t)j * half] = S[j];
+}
+
+template <typename T, int NL>
+static hipError_t launch_diag(const void* du, int64_t n_seq, int steps, int half,
+                              double* out, hipStream_t s) {
+  const int64_t threads = n_seq * (int64_t)half;
+  const unsigned blocks = (unsigned)((threads + 255) / 256);
+  hipLaunchKernelGGL((lowrank_diag_scan_kernel<T, NL>), dim3(blocks), dim3(256), 0, s,
+                     reinterpret_cast<const T*>(du), n_seq, steps, half, out);
+  return hipGetLastError();
+}
+
+hipError_t lowrank_diag_scan_launch(int dtype, const void* du, int n_levels, int64_t n_seq,
+                                    int steps, int half, double* out, hipStream_t s) {
+  switch (n_levels) {
+#define KSIG_DIAG_CASE(L)                                                          \
+    case L:                                                                      \
+      return dtype == KSIG_F32 ? launch_diag<float, L>(du, n_seq, steps, half, out, s) \
+                               : launch_diag<double, L>(du, n_seq, steps, half, out, s);
+    KSIG_DIAG_CASE(1)
+    KSIG_DIAG_CASE(2)
+    KSIG_DIAG_CASE(3)
+    KSIG_DIAG_CASE(4)
+    KSIG_DIAG_CASE(5)
+#undef KSIG_DIAG_CASE
+  }
+  return hipErrorInvalidValue;
+}
+
 hipError_t lowrank_scan_launch(int dtype, const void* z, int n_levels, int64_t n_seq,
                                int steps, int n_comp, double* out, hipStream_t s) {
   if (dtype == KSIG_F32) return dispatch_scan<float>(z, n_levels, n_seq, steps, n_comp, out, s);

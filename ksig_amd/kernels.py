@@ -352,9 +352,11 @@ class SignatureFeatures(SignatureBase, static.features.KernelFeatures):
   ``TensorizedRandomProjection`` (RFSF-TRP) the projected increments come
   from library GEMMs and the level recursion (exclusive time cumsum,
   elementwise product, time sum of every level) is one HIP pass
-  (``ksig_lowrank_levels``).  Without a projection the level-m features are
-  the D^m iterated-sum coordinates (torch ops on the GPU, small D only).
-  Order >= 2 and other projections are not built."""
+  (``ksig_lowrank_levels``).  With a ``DiagonalProjection`` (RFSF-DP) the
+  differenced random Fourier features go straight into the diagonal level
+  scan (``ksig_lowrank_levels_diag``).  Without a projection the level-m
+  features are the D^m iterated-sum coordinates (torch ops on the GPU,
+  small D only).  Order >= 2 and other projections are not built."""
 
   def __init__(self, n_levels: int = 5, order: int = 1, difference: bool = True,
                normalize: bool = True, indep_feat: bool = True,
@@ -372,13 +374,18 @@ class SignatureFeatures(SignatureBase, static.features.KernelFeatures):
 
   def _make_feature_components(self, X):
     """ksig/kernels.py:295-328."""
-    from .projections import TensorizedRandomProjection
+    from .projections import DiagonalProjection, TensorizedRandomProjection
     if self.order != 1:
       raise NotImplementedError('low-rank signature features are built for order=1')
-    if self.projection is not None and not isinstance(self.projection,
-                                                      TensorizedRandomProjection):
+    if self.projection is not None and not isinstance(
+        self.projection, (TensorizedRandomProjection, DiagonalProjection)):
       raise NotImplementedError('low-rank signature features are built with a '
-                                'TensorizedRandomProjection or no projection')
+                                'TensorizedRandomProjection, a DiagonalProjection or '
+                                'no projection')
+    if isinstance(self.projection, DiagonalProjection) and (
+        self.projection.internal_size != 2 or self.n_levels > 5):
+      raise NotImplementedError('the diagonal projection is built for internal_size=2 '
+                                'and at most 5 levels')
     if self.static_features is not None:
       seq_shape = tuple(X.shape[:-1]) + (-1,)
       Xf = X.reshape(-1, X.shape[-1])
@@ -423,7 +430,17 @@ class SignatureFeatures(SignatureBase, static.features.KernelFeatures):
       U = [torch.diff(u, dim=1) for u in U]
     n = X.shape[0]
     ones = torch.ones((n, 1), device=X.device, dtype=torch.float64)
-    if self.projections_ is not None:
+    from .projections import DiagonalProjection
+    if self.projections_ is not None and isinstance(self.projections_[0], DiagonalProjection):
+      # RFSF-DP: the (sin, cos) increments go straight into the level scan
+      lev = _ops.lowrank_levels_diag(torch.stack(U, 0))
+      H = U[0].shape[-1] // 2
+      P, off = [ones], 0
+      for i in range(self.n_levels):
+        w = (2 << i) * H
+        P.append(lev[:, off:off + w])
+        off += w
+    elif self.projections_ is not None:
       # projected increments: level 0 alone (scaled), level m >= 1 the
       # projection of U_m that multiplies the running sums (library GEMMs)
       Z = [self.projections_[0]._project(U[0])]

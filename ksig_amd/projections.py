@@ -107,3 +107,38 @@ class TensorizedRandomProjection(RandomProjection):
     C = self.components_.to(device=Y.device, dtype=Y.dtype)
     return self.scaling_ * X * (Y.reshape(-1, self.n_features_) @ C).reshape(
         tuple(Y.shape[:-1]) + (-1,))
+
+
+class DiagonalProjection(RandomProjection):
+  """Diagonal projection (ksig/projections.py:587-674): only for the low-rank
+  signature features on random Fourier features (RFSF-DP).  ``_project``
+  splits the last axis into [internal_size, -1]; the outer-product step is
+  the Hadamard product over the component axis with an outer product over
+  the internal axis, scaled by sqrt(n_features_).  No random components."""
+
+  def __init__(self, internal_size: int = 2):
+    self.internal_size = internal_size
+
+  def _validate_format(self, X, Y=None, reset: bool = False):
+    """ksig/projections.py:611-637."""
+    q = self.internal_size
+    if reset or not hasattr(self, 'n_features_') or self.n_features_ is None:
+      self.n_features_ = X.shape[-1]
+    if X.shape[-1] != self.n_features_:
+      raise ValueError('Data `X` has a different number of features than during fit.',
+                       f'({X.shape[-1]} != {self.n_features_})')
+    if Y is not None and Y.shape[-1] // q != self.n_features_:
+      raise ValueError('Data `Y` has a different number of features than during fit.',
+                       f'({Y.shape[-1]} != {self.n_features_})')
+
+  def _make_projection_components(self, X, Y=None):
+    pass
+
+  def _project(self, X):
+    return X.reshape(tuple(X.shape[:-1]) + (self.internal_size, -1))
+
+  def _project_outer_prod(self, X, Y):
+    q = self.internal_size
+    Y = Y.reshape(tuple(Y.shape[:-1]) + (q, -1))
+    return np.sqrt(self.n_features_) * (X[..., :, None, :] * Y[..., None, :, :]).reshape(
+        tuple(X.shape[:-2]) + (-1, X.shape[-1]))

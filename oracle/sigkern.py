@@ -517,6 +517,20 @@ def signature_kern_low_rank_fo(U, n_levels, difference=True, projections=None):
     Us = [np.diff(u, axis=1) for u in Us]
   n = Us[0].shape[0]
   P = [np.ones((n, 1), dtype=Us[0].dtype)]
+  if projections == 'diag':
+    # DiagonalProjection of internal size 2 (ksig/projections.py:649-674):
+    # level 0 split into [2, H]; level i the component-wise product with an
+    # outer product over the internal axis, scaled by sqrt(H)
+    H = Us[0].shape[-1] // 2
+    R = Us[0].reshape(Us[0].shape[:-1] + (2, H))
+    P.append(R.sum(1).reshape(n, -1))
+    for i in range(1, n_levels):
+      R = multi_cumsum(R, exclusive=True, axis=1)
+      Y = Us[i].reshape(Us[i].shape[:-1] + (2, H))
+      R = np.sqrt(H) * (R[..., :, None, :] * Y[..., None, :, :]).reshape(
+          R.shape[:2] + (-1, H))
+      P.append(R.sum(1).reshape(n, -1))
+    return P
   if projections is None:
     R = Us[0].copy()
   else:
@@ -541,7 +555,7 @@ def signature_features(X, n_levels, rff_weights, trp=None, difference=True, norm
   level features, each normalised to unit norm, concatenated and divided by
   sqrt(n_levels + 1)."""
   U = [rff_features(X, w) for w in rff_weights]
-  P = signature_kern_low_rank_fo(U, n_levels, difference, trp)
+  P = signature_kern_low_rank_fo(U, n_levels, difference, trp)  # trp: list, 'diag' or None
   if not normalize:
     return np.concatenate(P, -1)
   P = [p / np.maximum(np.sqrt(np.maximum(np.sum(p * p, -1), EPS)), EPS)[..., None] for p in P]

@@ -67,3 +67,36 @@ def test_unprojected_features_match_oracle():
   assert F.shape == (4, 1 + 2 + 4 + 8) and normwise(F, ref) < 1e-11
   with pytest.raises(NotImplementedError):
     ksig.kernels.SignatureFeatures(n_levels=3, order=2).fit(X)
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-11), (np.float32, 1e-5)])
+@pytest.mark.parametrize("n_levels", [1, 3, 5])
+def test_diagonal_level_scan(dtype, tol, n_levels):
+  rng = np.random.default_rng(10 + n_levels)
+  n, L, H = 5, 30, 37
+  U = rng.standard_normal((n, L, 2 * H)) * 0.3
+  ref = np.concatenate(O.signature_kern_low_rank_fo(U, n_levels, True, 'diag')[1:], -1)
+  dU = np.stack([np.diff(U, axis=1)] * n_levels, 0).astype(dtype)
+  got = _ops.lowrank_levels_diag(to_dev(dU)).cpu().numpy()
+  assert got.shape == ref.shape
+  num = np.abs(got - ref).max(-1)
+  assert float(np.max(num / np.abs(ref).max(-1))) < tol
+
+
+def test_rfsf_dp_class():
+  rng = np.random.default_rng(11)
+  X = O.brownian(rng, 7, 25, 3).astype(np.float32)
+  sf = ksig.kernels.SignatureFeatures(
+      n_levels=4, static_features=ksig.static.features.RandomFourierFeatures(
+          n_components=20, random_state=4),
+      projection=ksig.projections.DiagonalProjection())
+  sf.fit(X)
+  w = [f.random_weights_.double().cpu().numpy() for f in sf.static_features_]
+  F = sf.transform(X)
+  ref = O.signature_features(X.astype(np.float64), 4, w, 'diag')
+  assert F.shape == (7, 1 + (2 + 4 + 8 + 16) * 20) and normwise(F, ref) < 1e-5
+  np.testing.assert_allclose(sf(X, diag=True), 1.0, rtol=1e-5)
+  with pytest.raises(NotImplementedError):
+    ksig.kernels.SignatureFeatures(n_levels=6, projection=ksig.projections.DiagonalProjection(),
+                                   static_features=ksig.static.features.RandomFourierFeatures(
+                                       n_components=4)).fit(X)

@@ -73,3 +73,23 @@ def test_normalised_feature_map():
   bad = np.einsum('ti,tj->ij', np.cumsum(dU, 0), dU).reshape(-1)
   assert np.abs(good - _iterated_sum_tensor(dU, 2).reshape(-1)).max() < 1e-12
   assert np.abs(bad - good).max() > 1e-3
+
+
+@pytest.mark.parametrize("n_levels", [1, 2, 3])
+def test_diagonal_projection_picks_matching_component_entries(n_levels):
+  """RFSF-DP level m equals the unprojected iterated-sum tensor at the
+  entries whose m factors share one component k (any sin / cos choice),
+  scaled by sqrt(H)^(m-1)."""
+  rng = np.random.default_rng(3)
+  H = 3
+  U = rng.standard_normal((2, 6, 2 * H))
+  dp = O.signature_kern_low_rank_fo(U, n_levels, True, 'diag')
+  full = O.signature_kern_low_rank_fo(U, n_levels, True, None)
+  for m in range(1, n_levels + 1):
+    S = full[m].reshape((2,) + (2 * H,) * m)
+    D = dp[m].reshape(2, 2 ** m, H)
+    for a in range(2):
+      for j, qs in enumerate(itertools.product(range(2), repeat=m)):
+        for k in range(H):
+          idx = tuple(q * H + k for q in qs)
+          assert abs(D[a, j, k] - np.sqrt(H) ** (m - 1) * S[(a,) + idx]) < 1e-10
