@@ -672,6 +672,38 @@ def bench_rfsf(args):
                    "kernel": "lowrank_scan_kernel<float,5>", "kernel_ms": scan_ms,
                    "bytes_per_launch": scan_bytes},
   }
+  # RFSF-DP on the same input: diagonal projection of the RFF increments
+  dp = ksig.kernels.SignatureFeatures(
+      n_levels=nl, static_features=ksig.static.features.RandomFourierFeatures(
+          n_components=D, random_state=0),
+      projection=ksig.projections.DiagonalProjection())
+  dp.fit(X)
+  for _ in range(args.warmup):
+    dp.transform(X, return_on_gpu=True)
+  torch.cuda.synchronize()
+  t0 = time.perf_counter()
+  for _ in range(args.steps):
+    Fd = dp.transform(X, return_on_gpu=True)
+  torch.cuda.synchronize()
+  dp_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+  dU = torch.stack([torch.diff(f.transform(X.reshape(-1, d), return_on_gpu=True).reshape(n, L, -1),
+                               dim=1) for f in dp.static_features_], 0).contiguous()
+  _ops.lowrank_levels_diag(dU)
+  e0.record(stream)
+  for _ in range(args.steps):
+    _ops.lowrank_levels_diag(dU)
+  e1.record(stream)
+  torch.cuda.synchronize()
+  dscan_ms = e0.elapsed_time(e1) / args.steps
+  dscan_bytes = dU.numel() * 4 + n * ((2 << nl) - 2) * D * 8
+  wd = [f.random_weights_.double().cpu().numpy() for f in dp.static_features_]
+  refd = O.signature_features(X[idx].double().cpu().numpy(), nl, wd, 'diag')
+  errd = float(np.max(np.abs(Fd[idx].double().cpu().numpy() - refd).max(-1) /
+                      np.abs(refd).max(-1)))
+  res["dp"] = {"ms_per_transform": dp_ms, "features": int(Fd.shape[1]),
+               "max_rel_err_vs_fp64": errd, "scan_kernel": "lowrank_diag_scan_kernel<float,5>",
+               "scan_ms": dscan_ms, "scan_bytes": dscan_bytes,
+               "scan_GBps": dscan_bytes / (dscan_ms * 1e-3) / 1e9}
   print(json.dumps(res), flush=True)
 
 
