@@ -28,7 +28,10 @@ import torch
 from . import _lib, _ops
 
 LOG2E = 1.4426950408889634
-_OMEGA_BYTES = 1 << 30  # omega scratch per chunk of pairs
+# omega scratch per chunk of pairs: 8 GiB (fewer, larger adjoint launches
+# fill the chip better; the 16,384-pair chunks of 1 GiB left a partial round
+# of waves per launch)
+_OMEGA_BYTES = 1 << 33
 
 
 def _as_param(p, device) -> torch.Tensor:
@@ -206,7 +209,7 @@ def mmd2_unbiased(Kxx: torch.Tensor, Kyy: torch.Tensor, Kxy: torch.Tensor) -> to
 # Signature-PDE kernel
 # -----------------------------------------------------------------------------
 
-_GM_BYTES = 1 << 30  # dL/dM scratch per chunk of pairs
+_GM_BYTES = 1 << 33  # dL/dM scratch per chunk of pairs (8 GiB, as omega)
 
 
 class _SigPDE(torch.autograd.Function):
