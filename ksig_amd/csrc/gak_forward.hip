@@ -227,12 +227,19 @@ gak_forward_kernel(const GakArgs a) {
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       const int row = row0 + q;
-      // column -1: empty sums, except K(-1, -1) = 1 above-left of (0, 0)
-      const double dq = c0 ? (row == 0 ? 1.0 : 0.0) : dv;
-      const int dqe = c0 ? (row == 0 ? 0 : kGakZeroExp) : de;
+      // column -1 holds empty sums (the left values are reset at each band
+      // start, and rows q >= 1 take their diagonal from them), except
+      // K(-1, -1) = 1 above-left of (0, 0); the first row's diagonal at
+      // column 0 comes from the row above, which is not yet in the band
+      double dq = dv;
+      int dqe = de;
+      if (q == 0) {
+        dq = c0 ? (row == 0 ? 1.0 : 0.0) : dv;
+        dqe = c0 ? (row == 0 ? 0 : kGakZeroExp) : de;
+      }
       double v;
       int e;
-      gak_cell(mp[q], uv, ue, c0 ? 0.0 : lv[q], c0 ? kGakZeroExp : le[q], dq, dqe, v, e);
+      gak_cell(mp[q], uv, ue, lv[q], le[q], dq, dqe, v, e);
       const bool cell = colok && row < Lx;
       v = cell ? v : 0.0;
       e = cell ? e : kGakZeroExp;
@@ -254,10 +261,15 @@ gak_forward_kernel(const GakArgs a) {
     pin_e = in_e;
 
     ++cc;
-    if (cc == Pc) {  // wrap to the next band
+    if (cc == Pc) {  // wrap to the next band: column -1 of the new rows
       cc = 0;
       ++band;
       if (band < nb) load_rows(band);
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        lv[q] = 0.0;
+        le[q] = kGakZeroExp;
+      }
     }
   }
 }

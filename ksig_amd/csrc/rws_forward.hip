@@ -180,10 +180,12 @@ rws_forward_kernel(const RwsArgs a) {
     for (int q = 0; q < R; ++q) {
       // the first row of a series starts from the boundary: P(-1, c) = inf,
       // P(-1, -1) = 0; column -1 is +inf
+      // (left values are reset to +inf at each band start, so column -1
+      // needs no select for them, nor for the diagonal of rows q >= 1)
       const T uq = first[q] ? kInf : u;
-      const T dq = c0 ? (first[q] ? T(0) : kInf) : (first[q] ? kInf : dg);
-      const T lq = c0 ? kInf : left[q];
-      T val = dist[q] + fminf_t(fminf_t(uq, lq), dq);
+      T dq = first[q] ? (c0 ? T(0) : kInf) : dg;
+      if (q == 0 && !first[q]) dq = c0 ? kInf : dg;
+      T val = dist[q] + fminf_t(fminf_t(uq, left[q]), dq);
       const bool cell = colok && rowq[q] < row_end;
       val = cell ? val : kInf;
       if (cell && last[q] && cc == Lx - 1) out[wid[q]] = (double)val;
@@ -196,10 +198,12 @@ rws_forward_kernel(const RwsArgs a) {
     pin = in;
 
     ++cc;
-    if (cc == Pc) {  // wrap to the next band
+    if (cc == Pc) {  // wrap to the next band: column -1 of the new rows
       cc = 0;
       ++band;
       if (band < nb) load_rows(band);
+#pragma unroll
+      for (int q = 0; q < R; ++q) left[q] = kInf;
     }
   }
 }
