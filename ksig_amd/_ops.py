@@ -453,3 +453,19 @@ def lowrank_levels_diag(dU: torch.Tensor) -> torch.Tensor:
                                           steps, half, out.data_ptr(),
                                           _lib.stream_handle(dU.device)))
   return out
+
+
+def rff_increments(proj: torch.Tensor, difference: bool = True) -> torch.Tensor:
+  """[sin, cos](proj) / sqrt(half), differenced along time, from projections
+  proj [n_seq, L, half] in one pass."""
+  lib = _lib.load()
+  pc = proj.contiguous()
+  n, L, half = pc.shape
+  steps = L - 1 if difference else L
+  out = torch.empty((n, max(steps, 0), 2 * half), device=proj.device, dtype=proj.dtype)
+  if out.numel() == 0:
+    return out
+  _lib.check(lib.ksig_rff_increments(_lib.dtype_code(pc.dtype), pc.data_ptr(), n, L, half,
+                                     int(bool(difference)), out.data_ptr(),
+                                     _lib.stream_handle(proj.device)))
+  return out

@@ -421,13 +421,34 @@ class SignatureFeatures(SignatureBase, static.features.KernelFeatures):
     U = self.static_features_.transform(Xf, return_on_gpu=True).reshape(seq_shape)
     return [U] * self.n_levels
 
+  def _increments(self, X):
+    """Per-level lifted paths, differenced along time when ``difference``.
+    Random Fourier features go from the GEMM X W straight to their
+    increments in one HIP pass (``ksig_rff_increments``: sin / cos, scaling
+    and time difference without the intermediate feature tensor)."""
+    RFF = static.features.RandomFourierFeatures
+    fitted = getattr(self, 'static_features_', None) if self.static_features is not None else None
+    fs = fitted if isinstance(fitted, list) else ([fitted] if fitted is not None else [])
+    if fs and all(type(f) is RFF for f in fs):
+      Xf = X.reshape(-1, X.shape[-1])
+      cache = {}
+      out = []
+      for i in range(self.n_levels):
+        f = fs[i] if len(fs) > 1 else fs[0]
+        if id(f) not in cache:
+          proj = (Xf @ f.random_weights_.to(device=X.device, dtype=X.dtype)).reshape(
+              X.shape[0], X.shape[1], -1)
+          cache[id(f)] = _ops.rff_increments(proj, self.difference)
+        out.append(cache[id(f)])
+      return out
+    U = self._lifted(X)
+    return [torch.diff(u, dim=1) for u in U] if self.difference else U
+
   def _compute_features(self, X) -> torch.Tensor:
     """ksig/kernels.py:330-361."""
     if X.ndim != 3:
       raise ValueError('SignatureFeatures takes sequence arrays of ndim==3.')
-    U = self._lifted(X)
-    if self.difference:
-      U = [torch.diff(u, dim=1) for u in U]
+    U = self._increments(X)
     n = X.shape[0]
     ones = torch.ones((n, 1), device=X.device, dtype=torch.float64)
     from .projections import DiagonalProjection

@@ -100,3 +100,31 @@ def test_rfsf_dp_class():
     ksig.kernels.SignatureFeatures(n_levels=6, projection=ksig.projections.DiagonalProjection(),
                                    static_features=ksig.static.features.RandomFourierFeatures(
                                        n_components=4)).fit(X)
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-12), (np.float32, 1e-5)])
+@pytest.mark.parametrize("difference", [True, False])
+def test_rff_increments(dtype, tol, difference):
+  rng = np.random.default_rng(12)
+  proj = (rng.standard_normal((3, 17, 10)) * 2).astype(dtype)
+  got = _ops.rff_increments(to_dev(proj), difference).cpu().numpy()
+  F = np.concatenate([np.sin(proj.astype(np.float64)), np.cos(proj.astype(np.float64))], -1)
+  F /= np.sqrt(10)
+  ref = np.diff(F, axis=1) if difference else F
+  assert got.shape == ref.shape and np.abs(got - ref).max() < tol
+
+
+def test_signature_features_shared_static_map():
+  """indep_feat=False: one Fourier map for every level."""
+  rng = np.random.default_rng(13)
+  X = O.brownian(rng, 6, 20, 3).astype(np.float32)
+  sf = ksig.kernels.SignatureFeatures(
+      n_levels=3, indep_feat=False,
+      static_features=ksig.static.features.RandomFourierFeatures(n_components=8, random_state=5),
+      projection=ksig.projections.TensorizedRandomProjection(n_components=12, random_state=6))
+  sf.fit(X)
+  w = sf.static_features_.random_weights_.double().cpu().numpy()
+  trp = [(p.components_.double().cpu().numpy(), p.scaling_) for p in sf.projections_]
+  F = sf.transform(X)
+  ref = O.signature_features(X.astype(np.float64), 3, [w] * 3, trp)
+  assert normwise(F, ref) < 1e-5
