@@ -388,14 +388,15 @@ int ksig_lowrank_levels_diag(int dtype, const void* du, int n_levels, int64_t n_
 
 /*
  * Random Fourier features of projected points and their time increments in
- * one pass: proj = X W [n_seq, len, half] (dtype), out [n_seq, len - 1, 2 * half]
+ * one pass: proj = X W [n_seq, len, half] (dtype; rows ld >= half elements
+ * apart, so one GEMM may hold several levels' weights), out [n_seq, len - 1, 2 * half]
  * (or [n_seq, len, 2 * half] without difference) = the sin half then the cos
  * half divided by sqrt(half), differenced along time.  Replaces
  * RandomFourierFeatures._compute_features after its GEMM
  * (ksig/static/features.py:322-342) followed by the increments of
  * ksig/algorithms.py:196, as SignatureFeatures._compute_features calls them.
  */
-int ksig_rff_increments(int dtype, const void* proj, int64_t n_seq, int len, int half,
+int ksig_rff_increments(int dtype, const void* proj, int64_t n_seq, int len, int half, int ld,
                         int difference, void* out, void* stream);
 
 #ifdef __cplusplus

@@ -455,17 +455,24 @@ def lowrank_levels_diag(dU: torch.Tensor) -> torch.Tensor:
   return out
 
 
-def rff_increments(proj: torch.Tensor, difference: bool = True) -> torch.Tensor:
+def rff_increments(proj: torch.Tensor, difference: bool = True,
+                   out: Optional[torch.Tensor] = None) -> torch.Tensor:
   """[sin, cos](proj) / sqrt(half), differenced along time, from projections
-  proj [n_seq, L, half] in one pass."""
+  proj [n_seq, L, half] in one pass.  ``proj`` may be a column slice of a
+  wider GEMM output (unit stride along its last axis); ``out`` optionally
+  receives the result [n_seq, L - 1 (or L), 2 half] (contiguous)."""
   lib = _lib.load()
-  pc = proj.contiguous()
-  n, L, half = pc.shape
+  n, L, half = proj.shape
+  if proj.stride(2) != 1 or proj.stride(0) != L * proj.stride(1):
+    proj = proj.contiguous()
+  ld = proj.stride(1)
   steps = L - 1 if difference else L
-  out = torch.empty((n, max(steps, 0), 2 * half), device=proj.device, dtype=proj.dtype)
+  if out is None:
+    out = torch.empty((n, max(steps, 0), 2 * half), device=proj.device, dtype=proj.dtype)
   if out.numel() == 0:
     return out
-  _lib.check(lib.ksig_rff_increments(_lib.dtype_code(pc.dtype), pc.data_ptr(), n, L, half,
-                                     int(bool(difference)), out.data_ptr(),
+  assert out.is_contiguous() and tuple(out.shape) == (n, steps, 2 * half)
+  _lib.check(lib.ksig_rff_increments(_lib.dtype_code(proj.dtype), proj.data_ptr(), n, L, half,
+                                     ld, int(bool(difference)), out.data_ptr(),
                                      _lib.stream_handle(proj.device)))
   return out

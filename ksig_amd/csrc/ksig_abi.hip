@@ -20,7 +20,7 @@ hipError_t lowrank_scan_launch(int dtype, const void* z, int n_levels, int64_t n
                                int steps, int n_comp, double* out, hipStream_t s);
 hipError_t lowrank_diag_scan_launch(int dtype, const void* du, int n_levels, int64_t n_seq,
                                     int steps, int half, double* out, hipStream_t s);
-hipError_t rff_diff_launch(int dtype, const void* p, int64_t n_seq, int L, int half,
+hipError_t rff_diff_launch(int dtype, const void* p, int64_t n_seq, int L, int half, int ld,
                            int difference, void* out, hipStream_t s);
 }  // namespace ksig
 
@@ -929,15 +929,16 @@ int ksig_lowrank_levels_diag(int dtype, const void* du, int n_levels, int64_t n_
   return 0;
 }
 
-int ksig_rff_increments(int dtype, const void* proj, int64_t n_seq, int len, int half,
+int ksig_rff_increments(int dtype, const void* proj, int64_t n_seq, int len, int half, int ld,
                         int difference, void* out, void* stream) {
   if (dtype != KSIG_F32 && dtype != KSIG_F64) return fail(KSIG_EINVAL, "bad dtype %d", dtype);
   if (n_seq < 0 || len < 0 || half < 0) return fail(KSIG_EINVAL, "negative shape");
+  if (ld < half) return fail(KSIG_EINVAL, "row stride %d below the row length %d", ld, half);
   const int steps = difference ? len - 1 : len;
   if (n_seq == 0 || half == 0 || steps <= 0) return 0;
   if (!proj || !out) return fail(KSIG_EINVAL, "null device pointer");
-  hipError_t e = ksig::rff_diff_launch(dtype, proj, n_seq, len, half, difference ? 1 : 0, out,
-                                      (hipStream_t)stream);
+  hipError_t e = ksig::rff_diff_launch(dtype, proj, n_seq, len, half, ld, difference ? 1 : 0,
+                                      out, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "rff_diff launch");
   return 0;
 }

@@ -164,15 +164,16 @@ hipError_t lowrank_scan_launch(int dtype, const void* z, int n_levels, int64_t n
 namespace ksig {
 
 // Random Fourier features and their time increments in one pass: from the
-// projections p = X W [n_seq, L, half] (ksig/static/features.py:322-342),
+// projections p = X W [n_seq, L, half] with rows ld elements apart (several
+// levels' weights may share one GEMM) (ksig/static/features.py:322-342),
 // out[a, t, k] = (f(p[a, t+1, k]) - f(p[a, t, k])) / sqrt(half) with
 // f = sin for k < half and cos for k >= half (difference), or f(p[a, t, k])
 // / sqrt(half) without it — the concatenation, scaling and time difference
 // of ksig/algorithms.py:196 as one HBM read and one write.
 template <typename T>
 __global__ void __launch_bounds__(256)
-rff_diff_kernel(const T* __restrict__ p, int64_t n_seq, int L, int half, int difference,
-                T* __restrict__ out) {
+rff_diff_kernel(const T* __restrict__ p, int64_t n_seq, int L, int half, int ld,
+                int difference, T* __restrict__ out) {
   const int steps = difference ? L - 1 : L;
   const int64_t total = n_seq * (int64_t)steps * half;
   const T sc = T(1) / sqrt(T(half));
@@ -182,11 +183,11 @@ rff_diff_kernel(const T* __restrict__ p, int64_t n_seq, int L, int half, int dif
     const int64_t r = i / half;
     const int t = (int)(r % steps);
     const int64_t a = r / steps;
-    const T* pa = p + (a * L + t) * half + k;
+    const T* pa = p + (a * L + t) * ld + k;
     const T x0 = pa[0];
     T s = sin(x0), c = cos(x0);
     if (difference) {
-      const T x1 = pa[half];
+      const T x1 = pa[ld];
       s = sin(x1) - s;
       c = cos(x1) - c;
     }
@@ -196,7 +197,7 @@ rff_diff_kernel(const T* __restrict__ p, int64_t n_seq, int L, int half, int dif
   }
 }
 
-hipError_t rff_diff_launch(int dtype, const void* p, int64_t n_seq, int L, int half,
+hipError_t rff_diff_launch(int dtype, const void* p, int64_t n_seq, int L, int half, int ld,
                            int difference, void* out, hipStream_t s) {
   const int steps = difference ? L - 1 : L;
   const int64_t total = n_seq * (int64_t)steps * half;
@@ -205,11 +206,11 @@ hipError_t rff_diff_launch(int dtype, const void* p, int64_t n_seq, int L, int h
   if (blocks < 1) blocks = 1;
   if (dtype == KSIG_F32)
     hipLaunchKernelGGL(rff_diff_kernel<float>, dim3((unsigned)blocks), dim3(256), 0, s,
-                       reinterpret_cast<const float*>(p), n_seq, L, half, difference,
+                       reinterpret_cast<const float*>(p), n_seq, L, half, ld, difference,
                        reinterpret_cast<float*>(out));
   else
     hipLaunchKernelGGL(rff_diff_kernel<double>, dim3((unsigned)blocks), dim3(256), 0, s,
-                       reinterpret_cast<const double*>(p), n_seq, L, half, difference,
+                       reinterpret_cast<const double*>(p), n_seq, L, half, ld, difference,
                        reinterpret_cast<double*>(out));
   return hipGetLastError();
 }

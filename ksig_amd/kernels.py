@@ -421,28 +421,29 @@ class SignatureFeatures(SignatureBase, static.features.KernelFeatures):
     U = self.static_features_.transform(Xf, return_on_gpu=True).reshape(seq_shape)
     return [U] * self.n_levels
 
-  def _increments(self, X):
-    """Per-level lifted paths, differenced along time when ``difference``.
-    Random Fourier features go from the GEMM X W straight to their
-    increments in one HIP pass (``ksig_rff_increments``: sin / cos, scaling
-    and time difference without the intermediate feature tensor)."""
+  def _increments(self, X) -> torch.Tensor:
+    """Per-level lifted paths [n_levels, n, steps, D], differenced along time
+    when ``difference``.  Random Fourier features take one GEMM for every
+    level's weights and go from it to their increments in one HIP pass per
+    level (``ksig_rff_increments``: sin / cos, scaling and time difference
+    without the intermediate feature tensor)."""
     RFF = static.features.RandomFourierFeatures
     fitted = getattr(self, 'static_features_', None) if self.static_features is not None else None
     fs = fitted if isinstance(fitted, list) else ([fitted] if fitted is not None else [])
+    n, L = X.shape[0], X.shape[1]
     if fs and all(type(f) is RFF for f in fs):
-      Xf = X.reshape(-1, X.shape[-1])
-      cache = {}
-      out = []
-      for i in range(self.n_levels):
-        f = fs[i] if len(fs) > 1 else fs[0]
-        if id(f) not in cache:
-          proj = (Xf @ f.random_weights_.to(device=X.device, dtype=X.dtype)).reshape(
-              X.shape[0], X.shape[1], -1)
-          cache[id(f)] = _ops.rff_increments(proj, self.difference)
-        out.append(cache[id(f)])
-      return out
+      D = fs[0].random_weights_.shape[1]
+      W = torch.cat([f.random_weights_.to(device=X.device, dtype=X.dtype) for f in fs], 1)
+      proj = (X.reshape(-1, X.shape[-1]) @ W).reshape(n, L, -1)
+      steps = L - 1 if self.difference else L
+      dU = torch.empty((len(fs), n, max(steps, 0), 2 * D), device=X.device, dtype=X.dtype)
+      for i in range(len(fs)):
+        _ops.rff_increments(proj[:, :, i * D:(i + 1) * D], self.difference, out=dU[i])
+      return dU if len(fs) == self.n_levels else dU.expand(self.n_levels, -1, -1, -1)
     U = self._lifted(X)
-    return [torch.diff(u, dim=1) for u in U] if self.difference else U
+    if self.difference:
+      U = [torch.diff(u, dim=1) for u in U]
+    return torch.stack(U, 0)
 
   def _compute_features(self, X) -> torch.Tensor:
     """ksig/kernels.py:330-361."""
@@ -454,7 +455,7 @@ class SignatureFeatures(SignatureBase, static.features.KernelFeatures):
     from .projections import DiagonalProjection
     if self.projections_ is not None and isinstance(self.projections_[0], DiagonalProjection):
       # RFSF-DP: the (sin, cos) increments go straight into the level scan
-      lev = _ops.lowrank_levels_diag(torch.stack(U, 0))
+      lev = _ops.lowrank_levels_diag(U.contiguous())
       H = U[0].shape[-1] // 2
       P, off = [ones], 0
       for i in range(self.n_levels):
@@ -462,13 +463,13 @@ class SignatureFeatures(SignatureBase, static.features.KernelFeatures):
         P.append(lev[:, off:off + w])
         off += w
     elif self.projections_ is not None:
-      # projected increments: level 0 alone (scaled), level m >= 1 the
-      # projection of U_m that multiplies the running sums (library GEMMs)
-      Z = [self.projections_[0]._project(U[0])]
-      for u, p in zip(U[1:], self.projections_[1:]):
-        Z.append(p.scaling_ * (u.reshape(-1, u.shape[-1]) @ p.components_.to(u.dtype)
-                               ).reshape(tuple(u.shape[:-1]) + (-1,)))
-      Z = torch.stack(Z, 0)
+      # projected increments, one batched GEMM over the levels: level 0
+      # alone (scaled), level m >= 1 the projection of U_m that multiplies
+      # the running sums
+      C = torch.stack([p.scaling_ * p.components_.to(device=U.device, dtype=U.dtype)
+                       for p in self.projections_], 0)
+      nl, _, steps, width = U.shape
+      Z = torch.bmm(U.reshape(nl, n * steps, width), C.contiguous()).reshape(nl, n, steps, -1)
       lev = _ops.lowrank_levels(Z)
       P = [ones] + [lev[:, m] for m in range(self.n_levels)]
     else:

@@ -171,8 +171,9 @@ def test_rff_increments_argument_checks():
   lib = ctypes.CDLL(LIB)
   f = lib.ksig_rff_increments
   f.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
-                ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
-  assert f(0, None, 0, 10, 4, 1, None, None) == 0
-  assert f(0, None, 3, 1, 4, 1, None, None) == 0     # one point: no increments
-  assert f(5, None, 3, 10, 4, 1, None, None) == 1001
-  assert f(0, None, 3, 10, 4, 1, None, None) == 1001  # null pointers
+                ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+  assert f(0, None, 0, 10, 4, 4, 1, None, None) == 0
+  assert f(0, None, 3, 1, 4, 4, 1, None, None) == 0     # one point: no increments
+  assert f(5, None, 3, 10, 4, 4, 1, None, None) == 1001
+  assert f(0, None, 3, 10, 4, 3, 1, None, None) == 1001  # row stride below the row
+  assert f(0, None, 3, 10, 4, 8, 1, None, None) == 1001  # null pointers
