@@ -22,6 +22,7 @@ from math import ceil
 from typing import Dict, Optional
 
 import numpy as np
+import torch
 from sklearn.base import BaseEstimator, ClassifierMixin
 from sklearn.model_selection import GridSearchCV, StratifiedKFold
 from sklearn.svm import SVC
@@ -36,6 +37,12 @@ _DEFAULT_SVC_HPARAMS = {
 }
 
 
+def _as_matrix(F) -> torch.Tensor:
+  """A feature block as a 2-D tensor (host arrays of foreign kernels too)."""
+  F = F if isinstance(F, torch.Tensor) else torch.as_tensor(np.asarray(F))
+  return F.reshape(F.shape[0], -1)
+
+
 def _is_library_kernel(kernel) -> bool:
   """Kernels of this library: one launch for any Gram, no L x L tensor."""
   from .kernels import SignatureBase
@@ -47,9 +54,13 @@ class PrecomputedKernelSVC(BaseEstimator, ClassifierMixin):
   (ksig/models/pre_svc.py:25-106 over ksig/models/pre_base.py:17-282).
 
   Same constructor arguments and stored attributes as the reference (sklearn
-  ``get_params`` / ``clone`` read them).  ``has_transform=True`` needs a
-  feature-map kernel (low-rank / random-feature signatures), which this
-  library does not provide: it raises ``NotImplementedError`` at ``fit``."""
+  ``get_params`` / ``clone`` read them).  ``has_transform=True`` takes a
+  feature-map kernel (``SignatureFeatures`` or any object with
+  ``transform(X, return_on_gpu=...)``): the Gram is the product of the
+  feature matrices, one device GEMM in float64 (the reference multiplies
+  blocks of its ``_MAT_MUL_BSIZE`` rows, ksig/models/pre_base.py:172-224, to
+  bound device memory; one [n, n] float64 product fits HBM for any n SVC can
+  take)."""
 
   def __init__(self,
                kernel: Kernel,
@@ -106,14 +117,41 @@ class PrecomputedKernelSVC(BaseEstimator, ClassifierMixin):
         K[i0:i1, j0:j1] = self.kernel(X[i0:i1], X2[j0:j1])
     return K
 
+  def _precompute_feature_mat(self, X):
+    """Feature matrix [n, n_features], on the device when the kernel is this
+    library's (ksig/models/pre_base.py:143-170: in blocks of ``batch_size``
+    sequences when set)."""
+    if not hasattr(self.kernel, 'transform'):
+      raise ValueError('has_transform=True needs a kernel with a transform method')
+    if self.batch_size is None:
+      return _as_matrix(self.kernel.transform(X, return_on_gpu=True))
+    b = int(self.batch_size)
+    parts = [_as_matrix(self.kernel.transform(X[i:i + b], return_on_gpu=True))
+             for i in range(0, X.shape[0], b)]
+    return torch.cat(parts, dim=0)
+
+  @staticmethod
+  def _precompute_matrix_mult(F, F2=None) -> np.ndarray:
+    """F @ F2^T on the host as float64 (ksig/models/pre_base.py:172-224);
+    with F2 None the product is mirrored from its upper triangle so that it
+    is exactly symmetric, as the reference's block mirroring makes it."""
+    A = F.to(torch.float64)
+    if F2 is None:
+      P = A @ A.T
+      P = torch.triu(P) + torch.triu(P, diagonal=1).T
+    else:
+      P = A @ F2.to(device=A.device, dtype=torch.float64).T
+    return P.cpu().numpy()
+
   def _precompute_model_inputs(self, X=None) -> np.ndarray:
-    """ksig/models/pre_svc.py:69-104 (kernel-matrix branch)."""
-    if self.has_transform:
-      raise NotImplementedError(
-          'has_transform=True needs a feature-map kernel; this library computes '
-          'full-rank sequence kernels only')
+    """ksig/models/pre_svc.py:69-104."""
     if X is None:
+      if self.has_transform:
+        self.X_feat = self._precompute_feature_mat(self.X)
+        return self._precompute_matrix_mult(self.X_feat)
       return self._precompute_kernel_mat(self.X)
+    if self.has_transform:
+      return self._precompute_matrix_mult(self._precompute_feature_mat(X), self.X_feat)
     return self._precompute_kernel_mat(X, self.X)
 
   def fit(self, X, y) -> 'PrecomputedKernelSVC':

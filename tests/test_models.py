@@ -59,7 +59,17 @@ def test_fit_predict_and_grid_search():
   assert (g.predict(Xt) == yt).mean() >= 0.9
 
 
-def test_params_and_unsupported_transform():
+class MomentFeatures:
+  """A feature-map kernel on the host: per-channel mean and mean square."""
+
+  def fit(self, X):
+    return self
+
+  def transform(self, X, return_on_gpu=False):
+    return np.concatenate([X.mean(1), (X * X).mean(1)], axis=1)
+
+
+def test_params_and_transform_without_map():
   m = PrecomputedKernelSVC(FlatRBF(), svc_hparams={}, cv=4, batch_size=5)
   p = m.get_params(deep=False)
   assert p["cv"] == 4 and p["batch_size"] == 5 and p["has_transform"] is False
@@ -68,5 +78,24 @@ def test_params_and_unsupported_transform():
   t = PrecomputedKernelSVC(FlatRBF(), svc_hparams={}, has_transform=True)
   rng = np.random.default_rng(2)
   X, y = _toy(rng, 10)
-  with pytest.raises(NotImplementedError):
+  with pytest.raises(ValueError):
     t.fit(X, y)
+
+
+@pytest.mark.parametrize("b", [None, 3, 100])
+def test_transform_gram_is_feature_product(b):
+  """has_transform: Gram = F F^T (train) and F_test F^T (test), exactly
+  symmetric, whatever the feature blocks (ksig/models/pre_base.py:143-224)."""
+  rng = np.random.default_rng(3)
+  X, y = _toy(rng, 20)
+  Xt, yt = _toy(rng, 8)
+  m = PrecomputedKernelSVC(MomentFeatures(), svc_hparams={}, batch_size=b, has_transform=True)
+  m.X = X
+  K = m._precompute_model_inputs()
+  F = MomentFeatures().transform(X)
+  np.testing.assert_allclose(K, F @ F.T, rtol=1e-12, atol=1e-12)
+  assert np.array_equal(K, K.T)
+  Kt = m._precompute_model_inputs(Xt)
+  np.testing.assert_allclose(Kt, MomentFeatures().transform(Xt) @ F.T, rtol=1e-12, atol=1e-12)
+  m.fit(X, y)
+  assert (m.predict(Xt) == yt).mean() >= 0.9
