@@ -669,7 +669,7 @@ def bench_rfsf(args):
       "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                    "frac": achieved / PEAK_HBM_GBS, "traffic": MEASURED_TRAFFIC_BYTES["rfsf"],
                    "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE)",
-                   "kernel": "lowrank_scan_kernel<float,5>", "kernel_ms": scan_ms,
+                   "kernel": "lowrank_scan_kernel<float,5,1>", "kernel_ms": scan_ms,
                    "bytes_per_launch": scan_bytes},
   }
   # RFSF-DP on the same input: diagonal projection of the RFF increments

@@ -362,18 +362,20 @@ int ksig_random_warping_series_gram(int dtype, const void* D, int64_t n_seq, int
                                     void* stream);
 
 /*
- * Level recursion of the first-order low-rank signature features with a
- * tensorized random projection: the level loop of
- * signature_kern_first_order_low_rank (ksig/algorithms.py:218-246) with the
+ * Level recursion of the low-rank signature features with a tensorized
+ * random projection: the level loops of signature_kern_first_order_low_rank
+ * (ksig/algorithms.py:218-246) and signature_kern_higher_order_low_rank
+ * (:296-337), dispatched by signature_kern_low_rank (:146-172), with the
  * TensorizedRandomProjection outer product (ksig/projections.py:566-584), as
  * called by SignatureFeatures._compute_features (ksig/kernels.py:330-361).
  * z: the projected increments [n_levels, n_seq, steps, n_comp] (dtype; level 0
  * projected alone with its scaling, level m >= 1 the projection of U_m);
- * out: the level features before normalisation, [n_seq, n_levels, n_comp]
- * float64.  1 <= n_levels <= 8.
+ * order: the embedding order (<= 0 or >= n_levels: the full recursion, as
+ * :165); out: the level features before normalisation,
+ * [n_seq, n_levels, n_comp] float64.  1 <= n_levels <= 8.
  */
-int ksig_lowrank_levels(int dtype, const void* z, int n_levels, int64_t n_seq, int steps,
-                        int n_comp, double* out, void* stream);
+int ksig_lowrank_levels(int dtype, const void* z, int n_levels, int order, int64_t n_seq,
+                        int steps, int n_comp, double* out, void* stream);
 
 /*
  * The same level recursion with a DiagonalProjection of internal size 2

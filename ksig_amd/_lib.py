@@ -78,7 +78,8 @@ SIGNATURES = {
         _c_int, _vp, _vp]),
     "ksig_random_warping_series_gram": (_c_int, [
         _c_int, _vp, _c_i64, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp, _vp]),
-    "ksig_lowrank_levels": (_c_int, [_c_int, _vp, _c_int, _c_i64, _c_int, _c_int, _vp, _vp]),
+    "ksig_lowrank_levels": (_c_int, [_c_int, _vp, _c_int, _c_int, _c_i64, _c_int, _c_int, _vp,
+                                     _vp]),
     "ksig_lowrank_levels_diag": (_c_int, [_c_int, _vp, _c_int, _c_i64, _c_int, _c_int, _vp,
                                           _vp]),
     "ksig_rff_increments": (_c_int, [_c_int, _vp, _c_i64, _c_int, _c_int, _c_int, _c_int, _vp,

@@ -424,17 +424,19 @@ def random_warping_series_gram(D: torch.Tensor, warp_lens: np.ndarray) -> torch.
   return out
 
 
-def lowrank_levels(Z: torch.Tensor) -> torch.Tensor:
+def lowrank_levels(Z: torch.Tensor, order: int = 1) -> torch.Tensor:
   """Low-rank level features [n_seq, n_levels, n_comp] (float64) from the
-  projected increments Z [n_levels, n_seq, steps, n_comp]."""
+  projected increments Z [n_levels, n_seq, steps, n_comp], embedding order
+  ``order`` (<= 0 or >= n_levels: all orders)."""
   lib = _lib.load()
   Zc = Z.contiguous()
   n_levels, n, steps, P = Zc.shape
   out = torch.empty((n, n_levels, P), device=Z.device, dtype=torch.float64)
   if n == 0 or P == 0:
     return out
-  _lib.check(lib.ksig_lowrank_levels(_lib.dtype_code(Zc.dtype), Zc.data_ptr(), n_levels, n,
-                                     steps, P, out.data_ptr(), _lib.stream_handle(Z.device)))
+  _lib.check(lib.ksig_lowrank_levels(_lib.dtype_code(Zc.dtype), Zc.data_ptr(), n_levels,
+                                     int(order), n, steps, P, out.data_ptr(),
+                                     _lib.stream_handle(Z.device)))
   return out
 
 

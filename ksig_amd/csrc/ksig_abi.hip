@@ -16,8 +16,9 @@
 #include "rws.h"
 
 namespace ksig {
-hipError_t lowrank_scan_launch(int dtype, const void* z, int n_levels, int64_t n_seq,
-                               int steps, int n_comp, double* out, hipStream_t s);
+hipError_t lowrank_scan_launch(int dtype, const void* z, int n_levels, int order,
+                               int64_t n_seq, int steps, int n_comp, double* out,
+                               hipStream_t s);
 hipError_t lowrank_diag_scan_launch(int dtype, const void* du, int n_levels, int64_t n_seq,
                                     int steps, int half, double* out, hipStream_t s);
 hipError_t rff_diff_launch(int dtype, const void* p, int64_t n_seq, int L, int half, int ld,
@@ -899,17 +900,20 @@ int ksig_random_warping_series_gram(int dtype, const void* D, int64_t n_seq, int
   return rws_common(dtype, a, lx, stream);
 }
 
-int ksig_lowrank_levels(int dtype, const void* z, int n_levels, int64_t n_seq, int steps,
-                        int n_comp, double* out, void* stream) {
+int ksig_lowrank_levels(int dtype, const void* z, int n_levels, int order, int64_t n_seq,
+                        int steps, int n_comp, double* out, void* stream) {
   if (dtype != KSIG_F32 && dtype != KSIG_F64) return fail(KSIG_EINVAL, "bad dtype %d", dtype);
   if (n_levels < 1) return fail(KSIG_EINVAL, "n_levels must be positive");
   if (n_levels > 8) return fail(KSIG_EUNSUPPORTED, "n_levels = %d > 8 is not built", n_levels);
+  // ksig/algorithms.py:165: a non-positive order, or one of n_levels or
+  // more, is the full recursion
+  if (order <= 0 || order > n_levels) order = n_levels;
   if (n_seq < 0 || steps < 0 || n_comp < 0) return fail(KSIG_EINVAL, "negative shape");
   if (n_seq * (int64_t)n_comp > 0x7fffffffLL * 256LL) return fail(KSIG_EINVAL, "too many sequences");
   if (n_seq == 0 || n_comp == 0) return 0;
   if (!out || (steps > 0 && !z)) return fail(KSIG_EINVAL, "null device pointer");
-  hipError_t e = ksig::lowrank_scan_launch(dtype, z, n_levels, n_seq, steps, n_comp, out,
-                                          (hipStream_t)stream);
+  hipError_t e = ksig::lowrank_scan_launch(dtype, z, n_levels, order, n_seq, steps, n_comp,
+                                          out, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "lowrank_scan launch");
   return 0;
 }

@@ -1,24 +1,30 @@
-// Level recursion of the first-order low-rank signature features with an
-// elementwise (tensorized random) projection (gfx950).
+// Level recursion of the low-rank signature features with an elementwise
+// (tensorized random) projection, first and higher order (gfx950).
 //
-// Replaces the level loop of ksig/algorithms.py:218-246 for a
-// TensorizedRandomProjection (ksig/projections.py:566-584): with z_m(t) the
-// projected increments of level m (already scaled; level 0 projected alone),
-//   R_1(t) = z_0(t),  R_m(t) = C_{m-1}(t) z_{m-1}(t),  C_m(t) = sum_{t' < t} R_m(t'),
-//   P_m = sum_t R_m(t),
-// i.e. the exclusive time cumsum, the elementwise product and the time sum
-// of every level in one pass over z instead of three whole-tensor passes per
-// level.  One thread per (sequence, component): the time walk is sequential
-// in registers, consecutive threads read consecutive components (coalesced),
-// and the level recursion keeps its running sums in float64.  HBM-bound: z
-// is read once, the features written once.
+// Replaces the level loops of ksig/algorithms.py:218-246 (order 1) and
+// :296-337 (order p > 1) for a TensorizedRandomProjection
+// (ksig/projections.py:566-584): with z_m(t) the projected increments of
+// level m (already scaled; level 0 projected alone) and d_m = min(m+1, p)
+// the embedding orders carried at level m,
+//   R_0[0](t) = z_0(t),
+//   R_m[0](t) = C_{m-1}(t) z_m(t),   C_m(t) = sum_{t' < t} sum_r R_m[r](t'),
+//   R_m[r](t) = R_{m-1}[r-1](t) z_m(t) / (r+1),   1 <= r < d_m,
+//   P_m = sum_t sum_r R_m[r](t),
+// i.e. the exclusive time cumsum, the elementwise products and the time sums
+// of every level and order in one pass over z instead of several
+// whole-tensor passes per level (at p = 1 only r = 0 exists and this is the
+// first-order recursion).  One thread per (sequence, component): the time
+// walk is sequential in registers, consecutive threads read consecutive
+// components (coalesced), and the recursion keeps its running sums and the
+// current R values in float64.  HBM-bound: z is read once, the features
+// written once.
 
 #include "ksig_common.h"
 #include "ksig_hip.h"
 
 namespace ksig {
 
-template <typename T, int NL>
+template <typename T, int NL, int ORD>
 __global__ void __launch_bounds__(256)
 lowrank_scan_kernel(const T* __restrict__ z, int64_t n_seq, int steps, int n_comp,
                     double* __restrict__ out) {
@@ -32,14 +38,25 @@ lowrank_scan_kernel(const T* __restrict__ z, int64_t n_seq, int steps, int n_com
 #pragma unroll
   for (int m = 0; m < NL; ++m) C[m] = S[m] = 0.0;
   for (int t = 0; t < steps; ++t) {
-    double R[NL];
-    R[0] = (double)zp[(int64_t)t * n_comp];
+    // R[m][r] of this time step; entries with r >= min(m+1, ORD) stay unused
+    double R[NL][ORD];
+    R[0][0] = (double)zp[(int64_t)t * n_comp];
 #pragma unroll
-    for (int m = 1; m < NL; ++m) R[m] = C[m - 1] * (double)zp[m * level_stride + (int64_t)t * n_comp];
+    for (int m = 1; m < NL; ++m) {
+      const double zm = (double)zp[m * level_stride + (int64_t)t * n_comp];
+      R[m][0] = C[m - 1] * zm;
+#pragma unroll
+      for (int r = 1; r < ORD; ++r)
+        if (r < m + 1) R[m][r] = R[m - 1][r - 1] * zm * (1.0 / (r + 1));
+    }
 #pragma unroll
     for (int m = 0; m < NL; ++m) {
-      C[m] += R[m];
-      S[m] += R[m];
+      double sum = R[m][0];
+#pragma unroll
+      for (int r = 1; r < ORD; ++r)
+        if (r < m + 1) sum += R[m][r];
+      C[m] += sum;
+      S[m] += sum;
     }
   }
   double* o = out + a * (int64_t)NL * n_comp + p;
@@ -47,28 +64,45 @@ lowrank_scan_kernel(const T* __restrict__ z, int64_t n_seq, int steps, int n_com
   for (int m = 0; m < NL; ++m) o[(int64_t)m * n_comp] = S[m];
 }
 
-template <typename T, int NL>
+template <typename T, int NL, int ORD>
 static hipError_t launch_scan(const void* z, int64_t n_seq, int steps, int n_comp,
                               double* out, hipStream_t s) {
   const int64_t threads = n_seq * (int64_t)n_comp;
   const unsigned blocks = (unsigned)((threads + 255) / 256);
-  hipLaunchKernelGGL((lowrank_scan_kernel<T, NL>), dim3(blocks), dim3(256), 0, s,
+  hipLaunchKernelGGL((lowrank_scan_kernel<T, NL, ORD>), dim3(blocks), dim3(256), 0, s,
                      reinterpret_cast<const T*>(z), n_seq, steps, n_comp, out);
   return hipGetLastError();
 }
 
+// order in [1, NL] (order >= NL carries every order: the full recursion)
+template <typename T, int NL>
+static hipError_t dispatch_order(const void* z, int order, int64_t n_seq, int steps,
+                                 int n_comp, double* out, hipStream_t s) {
+  switch (order < NL ? order : NL) {
+    case 1: return launch_scan<T, NL, 1>(z, n_seq, steps, n_comp, out, s);
+    case 2: if constexpr (NL >= 2) return launch_scan<T, NL, 2>(z, n_seq, steps, n_comp, out, s); break;
+    case 3: if constexpr (NL >= 3) return launch_scan<T, NL, 3>(z, n_seq, steps, n_comp, out, s); break;
+    case 4: if constexpr (NL >= 4) return launch_scan<T, NL, 4>(z, n_seq, steps, n_comp, out, s); break;
+    case 5: if constexpr (NL >= 5) return launch_scan<T, NL, 5>(z, n_seq, steps, n_comp, out, s); break;
+    case 6: if constexpr (NL >= 6) return launch_scan<T, NL, 6>(z, n_seq, steps, n_comp, out, s); break;
+    case 7: if constexpr (NL >= 7) return launch_scan<T, NL, 7>(z, n_seq, steps, n_comp, out, s); break;
+    case 8: if constexpr (NL >= 8) return launch_scan<T, NL, 8>(z, n_seq, steps, n_comp, out, s); break;
+  }
+  return hipErrorInvalidValue;
+}
+
 template <typename T>
-static hipError_t dispatch_scan(const void* z, int n_levels, int64_t n_seq, int steps,
-                                int n_comp, double* out, hipStream_t s) {
+static hipError_t dispatch_scan(const void* z, int n_levels, int order, int64_t n_seq,
+                                int steps, int n_comp, double* out, hipStream_t s) {
   switch (n_levels) {
-    case 1: return launch_scan<T, 1>(z, n_seq, steps, n_comp, out, s);
-    case 2: return launch_scan<T, 2>(z, n_seq, steps, n_comp, out, s);
-    case 3: return launch_scan<T, 3>(z, n_seq, steps, n_comp, out, s);
-    case 4: return launch_scan<T, 4>(z, n_seq, steps, n_comp, out, s);
-    case 5: return launch_scan<T, 5>(z, n_seq, steps, n_comp, out, s);
-    case 6: return launch_scan<T, 6>(z, n_seq, steps, n_comp, out, s);
-    case 7: return launch_scan<T, 7>(z, n_seq, steps, n_comp, out, s);
-    case 8: return launch_scan<T, 8>(z, n_seq, steps, n_comp, out, s);
+    case 1: return dispatch_order<T, 1>(z, order, n_seq, steps, n_comp, out, s);
+    case 2: return dispatch_order<T, 2>(z, order, n_seq, steps, n_comp, out, s);
+    case 3: return dispatch_order<T, 3>(z, order, n_seq, steps, n_comp, out, s);
+    case 4: return dispatch_order<T, 4>(z, order, n_seq, steps, n_comp, out, s);
+    case 5: return dispatch_order<T, 5>(z, order, n_seq, steps, n_comp, out, s);
+    case 6: return dispatch_order<T, 6>(z, order, n_seq, steps, n_comp, out, s);
+    case 7: return dispatch_order<T, 7>(z, order, n_seq, steps, n_comp, out, s);
+    case 8: return dispatch_order<T, 8>(z, order, n_seq, steps, n_comp, out, s);
   }
   return hipErrorInvalidValue;
 }
@@ -153,10 +187,12 @@ hipError_t lowrank_diag_scan_launch(int dtype, const void* du, int n_levels, int
   return hipErrorInvalidValue;
 }
 
-hipError_t lowrank_scan_launch(int dtype, const void* z, int n_levels, int64_t n_seq,
-                               int steps, int n_comp, double* out, hipStream_t s) {
-  if (dtype == KSIG_F32) return dispatch_scan<float>(z, n_levels, n_seq, steps, n_comp, out, s);
-  return dispatch_scan<double>(z, n_levels, n_seq, steps, n_comp, out, s);
+hipError_t lowrank_scan_launch(int dtype, const void* z, int n_levels, int order,
+                               int64_t n_seq, int steps, int n_comp, double* out,
+                               hipStream_t s) {
+  if (dtype == KSIG_F32)
+    return dispatch_scan<float>(z, n_levels, order, n_seq, steps, n_comp, out, s);
+  return dispatch_scan<double>(z, n_levels, order, n_seq, steps, n_comp, out, s);
 }
 
 }  // namespace ksig

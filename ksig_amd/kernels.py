@@ -375,17 +375,15 @@ class SignatureFeatures(SignatureBase, static.features.KernelFeatures):
   def _make_feature_components(self, X):
     """ksig/kernels.py:295-328."""
     from .projections import DiagonalProjection, TensorizedRandomProjection
-    if self.order != 1:
-      raise NotImplementedError('low-rank signature features are built for order=1')
     if self.projection is not None and not isinstance(
         self.projection, (TensorizedRandomProjection, DiagonalProjection)):
       raise NotImplementedError('low-rank signature features are built with a '
                                 'TensorizedRandomProjection, a DiagonalProjection or '
                                 'no projection')
     if isinstance(self.projection, DiagonalProjection) and (
-        self.projection.internal_size != 2 or self.n_levels > 5):
-      raise NotImplementedError('the diagonal projection is built for internal_size=2 '
-                                'and at most 5 levels')
+        self.projection.internal_size != 2 or self.n_levels > 5 or self._order() != 1):
+      raise NotImplementedError('the diagonal projection is built for internal_size=2, '
+                                'at most 5 levels and order=1')
     if self.static_features is not None:
       seq_shape = tuple(X.shape[:-1]) + (-1,)
       Xf = X.reshape(-1, X.shape[-1])
@@ -408,6 +406,11 @@ class SignatureFeatures(SignatureBase, static.features.KernelFeatures):
       self.projections_ += [cls(**hps).fit(V, Y=U) for _ in range(1, self.n_levels)]
     else:
       self.projections_ = None
+
+  def _order(self) -> int:
+    """Embedding order of the recursion (ksig/algorithms.py:165)."""
+    o = self.order
+    return self.n_levels if o <= 0 or o >= self.n_levels else o
 
   def _lifted(self, X):
     """Per-level static features U_m [n, L, D] (or X itself)."""
@@ -470,15 +473,25 @@ class SignatureFeatures(SignatureBase, static.features.KernelFeatures):
                        for p in self.projections_], 0)
       nl, _, steps, width = U.shape
       Z = torch.bmm(U.reshape(nl, n * steps, width), C.contiguous()).reshape(nl, n, steps, -1)
-      lev = _ops.lowrank_levels(Z)
+      lev = _ops.lowrank_levels(Z, self._order())
       P = [ones] + [lev[:, m] for m in range(self.n_levels)]
     else:
-      R = U[0].to(torch.float64)
-      P = [ones, R.sum(1)]
+      # no projection: the exact (exponentially wide) features of
+      # ksig/algorithms.py:218-246 (order 1) and :296-337 (order p > 1),
+      # Rs[r] the order-(r+1) terms of the current level
+      order = self._order()
+      Rs = [U[0].to(torch.float64)]
+      P = [ones, Rs[0].sum(1)]
       for i in range(1, self.n_levels):
-        R = torch.cumsum(R, 1) - R  # exclusive cumsum over time
-        R = (R[..., :, None] * U[i].to(torch.float64)[..., None, :]).reshape(n, R.shape[1], -1)
-        P.append(R.sum(1))
+        Ui = U[i].to(torch.float64)[..., None, :]
+        S = sum(Rs)
+        Q = torch.cumsum(S, 1) - S  # exclusive cumsum over time
+        steps = Q.shape[1]
+        nxt = [(Q[..., :, None] * Ui).reshape(n, steps, -1)]
+        for r in range(1, min(i + 1, order)):
+          nxt.append((Rs[r - 1][..., :, None] * Ui).reshape(n, steps, -1) / (r + 1))
+        Rs = nxt
+        P.append(sum(R.sum(1) for R in Rs))
     if not self.normalize:
       return torch.cat(P, -1).to(X.dtype)
     P = [p / torch.clamp(utils.robust_sqrt(utils.squared_norm(p, axis=-1)), min=_EPS)[..., None]

@@ -548,14 +548,55 @@ def signature_kern_low_rank_fo(U, n_levels, difference=True, projections=None):
   return P
 
 
-def signature_features(X, n_levels, rff_weights, trp=None, difference=True, normalize=True):
+def signature_kern_low_rank(U, n_levels, order=1, difference=True, projections=None):
+  """Low-rank signature features of any embedding order, ksig/algorithms.py:146-172
+  dispatching to :175-246 (order 1) or :249-337 (order p > 1), with
+  ``return_levels=True``.  At level i the order-(r+1) terms R[r], r < min(i+1, p):
+  R[0] is the exclusive time cumsum of the previous level's sum over r times
+  the (projected) U_i (:312-316), R[r] = R_prev[r-1] times U_i / (r+1)
+  (:317-324); the level feature sums over r and time (:325-334).
+  ``projections``: None or TRP (components [D, P], scaling) pairs, as in
+  signature_kern_low_rank_fo."""
+  order = normalize_order(n_levels, order)
+  Us = U if isinstance(U, list) else [U] * n_levels
+  if difference:
+    Us = [np.diff(u, axis=1) for u in Us]
+  n, steps = Us[0].shape[:2]
+  P = [np.ones((n, 1), dtype=Us[0].dtype)]
+  if projections is None:
+    Rs = [Us[0].copy()]
+  else:
+    C0, s0 = projections[0]
+    Rs = [s0 * (Us[0] @ C0)]
+  P.append(Rs[0].sum(1))
+  for i in range(1, n_levels):
+    Q = multi_cumsum(sum(Rs), exclusive=True, axis=1)
+    if projections is None:
+      outer = lambda A: (A[..., :, None] * Us[i][..., None, :]).reshape(n, steps, -1)
+    else:
+      Ci, si = projections[i]
+      Z = si * (Us[i] @ Ci)
+      outer = lambda A: A * Z
+    nxt = [outer(Q)]
+    for r in range(1, min(i + 1, order)):
+      nxt.append(outer(Rs[r - 1]) / (r + 1))
+    Rs = nxt
+    P.append(sum(R.sum(1) for R in Rs))
+  return P
+
+
+def signature_features(X, n_levels, rff_weights, trp=None, difference=True, normalize=True,
+                       order=1):
   """SignatureFeatures._compute_features (ksig/kernels.py:330-361) with
   RandomFourierFeatures static features (one weight matrix per level,
   ``indep_feat``) and optionally TensorizedRandomProjection components:
   level features, each normalised to unit norm, concatenated and divided by
   sqrt(n_levels + 1)."""
   U = [rff_features(X, w) for w in rff_weights]
-  P = signature_kern_low_rank_fo(U, n_levels, difference, trp)  # trp: list, 'diag' or None
+  if normalize_order(n_levels, order) == 1:
+    P = signature_kern_low_rank_fo(U, n_levels, difference, trp)  # trp: list, 'diag' or None
+  else:
+    P = signature_kern_low_rank(U, n_levels, order, difference, trp)
   if not normalize:
     return np.concatenate(P, -1)
   P = [p / np.maximum(np.sqrt(np.maximum(np.sum(p * p, -1), EPS)), EPS)[..., None] for p in P]

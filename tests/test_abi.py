@@ -145,13 +145,14 @@ def test_lowrank_levels_argument_checks():
   lib = ctypes.CDLL(LIB)
   lib.ksig_last_error.restype = ctypes.c_char_p
   f = lib.ksig_lowrank_levels
-  f.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_int,
-                ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
-  assert f(0, None, 5, 0, 10, 4, None, None) == 0          # no sequences
-  assert f(2, None, 5, 3, 10, 4, None, None) == 1001       # dtype
-  assert f(0, None, 0, 3, 10, 4, None, None) == 1001       # n_levels
-  assert f(0, None, 9, 3, 10, 4, None, None) == 1002       # not built
-  assert f(0, None, 5, 3, 10, 4, None, None) == 1001 and b"null" in lib.ksig_last_error()
+  f.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int64,
+                ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+  assert f(0, None, 5, 1, 0, 10, 4, None, None) == 0          # no sequences
+  assert f(2, None, 5, 1, 3, 10, 4, None, None) == 1001       # dtype
+  assert f(0, None, 0, 1, 3, 10, 4, None, None) == 1001       # n_levels
+  assert f(0, None, 9, 1, 3, 10, 4, None, None) == 1002       # not built
+  assert f(0, None, 5, 3, 3, 10, 4, None, None) == 1001 and b"null" in lib.ksig_last_error()
+  assert f(0, None, 5, -1, 3, 10, 4, None, None) == 1001      # order <= 0: all orders
 
 
 @pytest.mark.skipif(not os.path.exists(LIB), reason="libksig_hip.so not built")

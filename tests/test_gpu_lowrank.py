@@ -1,6 +1,6 @@
 """GPU parity of the low-rank signature features (RFSF-TRP: random Fourier
-features + tensorized random projection, first order) against the FP64
-oracle, given the object's random weights and components.
+features + tensorized random projection, first and higher order; RFSF-DP)
+against the FP64 oracle, given the object's random weights and components.
 
 Tolerances: level features within 1e-11 (float64 inputs) / 1e-5 (float32,
 normwise per sequence and level); normalised features 1e-5."""
@@ -65,8 +65,52 @@ def test_unprojected_features_match_oracle():
   F = sf.transform(X)
   ref = np.concatenate(O.signature_kern_low_rank_fo(X, 3), -1)
   assert F.shape == (4, 1 + 2 + 4 + 8) and normwise(F, ref) < 1e-11
+  for order in (2, 3):
+    sf = ksig.kernels.SignatureFeatures(n_levels=3, order=order, normalize=False).fit(X)
+    ref = np.concatenate(O.signature_kern_low_rank(X, 3, order), -1)
+    assert normwise(sf.transform(X), ref) < 1e-11, order
   with pytest.raises(NotImplementedError):
-    ksig.kernels.SignatureFeatures(n_levels=3, order=2).fit(X)
+    ksig.kernels.SignatureFeatures(
+        n_levels=3, order=2,
+        static_features=ksig.static.features.RandomFourierFeatures(n_components=8),
+        projection=ksig.projections.DiagonalProjection()).fit(X)
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-11), (np.float32, 1e-5)])
+@pytest.mark.parametrize("n_levels,order", [(2, 2), (3, 2), (5, 2), (5, 3), (5, 0), (8, 4),
+                                            (8, 8)])
+def test_level_scan_higher_order(dtype, tol, n_levels, order):
+  """ksig_lowrank_levels at embedding order p > 1 (ksig/algorithms.py:249-337)."""
+  rng = np.random.default_rng(10 * n_levels + order)
+  n, L, D, P = 5, 30, 4, 37
+  U = rng.standard_normal((n, L, D)) * 0.3
+  comps = [rng.standard_normal((D, P)) for _ in range(n_levels)]
+  proj = [(comps[0], 1 / np.sqrt(P))] + [(c, 1.0) for c in comps[1:]]
+  ref = O.signature_kern_low_rank(U, n_levels, order, True, proj)
+  dU = np.diff(U, axis=1)
+  Z = np.stack([dU @ comps[0] / np.sqrt(P)] + [dU @ c for c in comps[1:]], 0).astype(dtype)
+  got = _ops.lowrank_levels(to_dev(Z), order).cpu().numpy()
+  for m in range(n_levels):
+    num = np.abs(got[:, m] - ref[m + 1]).max(-1)
+    den = np.maximum(np.abs(ref[m + 1]).max(-1), 1e-300)
+    assert float(np.max(num / den)) < tol, m
+
+
+def test_rfsf_trp_class_higher_order():
+  rng = np.random.default_rng(11)
+  X = O.brownian(rng, 7, 28, 3).astype(np.float32)
+  sf = ksig.kernels.SignatureFeatures(
+      n_levels=4, order=2,
+      static_features=ksig.static.features.RandomFourierFeatures(n_components=16, random_state=3),
+      projection=ksig.projections.TensorizedRandomProjection(n_components=20, random_state=4))
+  sf.fit(X)
+  w = [f.random_weights_.double().cpu().numpy() for f in sf.static_features_]
+  trp = [(p.components_.double().cpu().numpy(), p.scaling_) for p in sf.projections_]
+  F = sf.transform(X)
+  ref = O.signature_features(X.astype(np.float64), 4, w, trp, order=2)
+  first = O.signature_features(X.astype(np.float64), 4, w, trp, order=1)
+  assert F.shape == (7, 1 + 4 * 20) and normwise(F, ref) < 1e-5
+  assert normwise(first, ref) > 1e-3  # the order matters at these sizes
 
 
 @pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-11), (np.float32, 1e-5)])

@@ -93,3 +93,54 @@ def test_diagonal_projection_picks_matching_component_entries(n_levels):
         for k in range(H):
           idx = tuple(q * H + k for q in qs)
           assert abs(D[a, j, k] - np.sqrt(H) ** (m - 1) * S[(a,) + idx]) < 1e-10
+
+
+@pytest.mark.parametrize("n_levels,order", [(2, 2), (3, 2), (3, 3), (4, 2), (4, 3), (4, 4)])
+def test_higher_order_unprojected_features_equal_dp_linear_kernel(n_levels, order):
+  """Without a projection the order-p features' inner products are the
+  order-p DP signature kernel with a linear static kernel on the path; the
+  order changes that kernel, so a recursion that ignored it would fail."""
+  rng = np.random.default_rng(5)
+  X = rng.standard_normal((3, 7, 2))
+  Y = rng.standard_normal((2, 6, 2))
+  Px = O.signature_kern_low_rank(X, n_levels, order)
+  Py = O.signature_kern_low_rank(Y, n_levels, order)
+  M = O.pairwise_gram(X, Y, 'linear', 1.0)
+  ref = O.signature_kern(M, n_levels, order, True, return_levels=True)
+  first = O.signature_kern(M, n_levels, 1, True, return_levels=True)
+  assert np.abs(ref[n_levels] - first[n_levels]).max() > 1e-3
+  for m in range(n_levels + 1):
+    np.testing.assert_allclose(Px[m] @ Py[m].T, ref[m], rtol=1e-11, atol=1e-12)
+
+
+def test_general_recursion_at_order_one_is_first_order():
+  rng = np.random.default_rng(6)
+  U = rng.standard_normal((2, 8, 3))
+  comps = [(rng.standard_normal((3, 5)), 0.4)] + [(rng.standard_normal((3, 5)), 1.0)] * 3
+  for proj in (None, comps):
+    a = O.signature_kern_low_rank(U, 4, 1, True, proj)
+    b = O.signature_kern_low_rank_fo(U, 4, True, proj)
+    for x, y in zip(a, b):
+      np.testing.assert_allclose(x, y, rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("order", [2, 3])
+def test_higher_order_trp_features_contract_unprojected_ones(order):
+  """TRP at order p: component p of level m is the unprojected order-p level
+  feature (a flattened [D]*m tensor) contracted with the rank-1 tensor
+  c_0[:, p] s_0 (x) c_1[:, p] s_1 (x) ... (the projection is linear in each
+  tensor factor)."""
+  rng = np.random.default_rng(7)
+  n_levels, D, P = 3, 3, 4
+  U = rng.standard_normal((2, 6, D))
+  proj = [(rng.standard_normal((D, P)), 0.5)] + [(rng.standard_normal((D, P)), 1.3)
+                                                  for _ in range(n_levels - 1)]
+  feats = O.signature_kern_low_rank(U, n_levels, order, True, proj)
+  full = O.signature_kern_low_rank(U, n_levels, order, True, None)
+  for m in range(1, n_levels + 1):
+    for p in range(P):
+      T = proj[0][0][:, p] * proj[0][1]
+      for k in range(1, m):
+        T = np.multiply.outer(T, proj[k][0][:, p] * proj[k][1])
+      np.testing.assert_allclose(feats[m][:, p], full[m] @ T.reshape(-1), rtol=1e-11,
+                                 atol=1e-12)
