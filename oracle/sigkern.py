@@ -24,6 +24,7 @@ difference it, per-level exclusive 2-D cumulative sums):
 * random warping series ..... ksig/algorithms.py:565-659 (DTW recursion),
                               ksig/kernels.py:459-477 (features)
 * low-rank signature features ksig/algorithms.py:148-246 (first order),
+                              :249-337 (higher order),
                               ksig/kernels.py:263-361, random Fourier features
                               ksig/static/features.py:288-342, tensorized
                               random projection ksig/projections.py:465-584
@@ -43,7 +44,9 @@ numbers (M' = 1); DTW vs explicit enumeration of warping paths and the
 one-point closed forms; low-rank features without projection vs the
 first-order DP with a linear static kernel on the lifted path, with a
 tensorized projection vs brute-force iterated-sum tensors contracted with
-the rank-1 projection tensors.
+the rank-1 projection tensors; higher-order low-rank features without
+projection vs the order-p DP with a linear static kernel, with a tensorized
+projection vs the unprojected ones contracted with the projection tensors.
 """
 
 from __future__ import annotations
