@@ -205,38 +205,62 @@ namespace ksig {
 // out[a, t, k] = (f(p[a, t+1, k]) - f(p[a, t, k])) / sqrt(half) with
 // f = sin for k < half and cos for k >= half (difference), or f(p[a, t, k])
 // / sqrt(half) without it — the concatenation, scaling and time difference
-// of ksig/algorithms.py:196 as one HBM read and one write.
+// of ksig/algorithms.py:196 as one HBM read and one write.  One thread per
+// (sequence, chunk of kTimeChunk time steps, component): it walks its chunk
+// in time, so each point's sin and cos are evaluated once (the difference
+// reuses the previous point's), the chunk's loads are independent and in
+// flight together, and consecutive threads take consecutive components
+// (coalesced rows).  The index is split once per thread, not per output.
+constexpr int kTimeChunk = 8;
+
+__device__ __forceinline__ void sin_cos(float x, float* s, float* c) { sincosf(x, s, c); }
+__device__ __forceinline__ void sin_cos(double x, double* s, double* c) { sincos(x, s, c); }
+
 template <typename T>
 __global__ void __launch_bounds__(256)
 rff_diff_kernel(const T* __restrict__ p, int64_t n_seq, int L, int half, int ld,
                 int difference, T* __restrict__ out) {
   const int steps = difference ? L - 1 : L;
-  const int64_t total = n_seq * (int64_t)steps * half;
+  const int nchunk = (steps + kTimeChunk - 1) / kTimeChunk;
+  const int64_t total = n_seq * (int64_t)nchunk * half;
   const T sc = T(1) / sqrt(T(half));
+  const int64_t w = 2 * (int64_t)half;  // output row
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
        i += (int64_t)gridDim.x * blockDim.x) {
     const int k = (int)(i % half);
     const int64_t r = i / half;
-    const int t = (int)(r % steps);
-    const int64_t a = r / steps;
-    const T* pa = p + (a * L + t) * ld + k;
-    const T x0 = pa[0];
-    T s = sin(x0), c = cos(x0);
-    if (difference) {
-      const T x1 = pa[ld];
-      s = sin(x1) - s;
-      c = cos(x1) - c;
+    const int t0 = (int)(r % nchunk) * kTimeChunk;
+    const int64_t a = r / nchunk;
+    const int nt = min(kTimeChunk, steps - t0);
+    const int lag = difference ? 1 : 0;
+    const T* pa = p + (a * L + t0) * ld + k;
+    T* o = out + (a * steps + t0) * w + k;
+    T x[kTimeChunk];
+#pragma unroll
+    for (int u = 0; u < kTimeChunk; ++u) x[u] = u < nt ? pa[(int64_t)(u + lag) * ld] : T(0);
+    T sp = T(0), cp = T(0);
+    if (difference) sin_cos(pa[0], &sp, &cp);
+#pragma unroll
+    for (int u = 0; u < kTimeChunk; ++u) {
+      if (u < nt) {
+        T sv, cv;
+        sin_cos(x[u], &sv, &cv);
+        o[u * w] = (sv - sp) * sc;
+        o[u * w + half] = (cv - cp) * sc;
+        if (difference) {
+          sp = sv;
+          cp = cv;
+        }
+      }
     }
-    T* o = out + (a * steps + t) * (int64_t)(2 * half) + k;
-    o[0] = s * sc;
-    o[half] = c * sc;
   }
 }
 
 hipError_t rff_diff_launch(int dtype, const void* p, int64_t n_seq, int L, int half, int ld,
                            int difference, void* out, hipStream_t s) {
   const int steps = difference ? L - 1 : L;
-  const int64_t total = n_seq * (int64_t)steps * half;
+  const int64_t total =
+      n_seq * (int64_t)((steps + kTimeChunk - 1) / kTimeChunk) * (int64_t)half;
   int64_t blocks = (total + 255) / 256;
   if (blocks > 65536) blocks = 65536;
   if (blocks < 1) blocks = 1;
