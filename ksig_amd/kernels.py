@@ -388,7 +388,7 @@ class SignatureFeatures(SignatureBase, static.features.KernelFeatures):
       seq_shape = tuple(X.shape[:-1]) + (-1,)
       Xf = X.reshape(-1, X.shape[-1])
       if self.indep_feat:
-        hps = self.static_features.get_params()
+        hps = self.static_features.get_params(deep=False)
         cls = self.static_features.__class__
         self.static_features_ = [cls(**hps).fit(Xf) for _ in range(self.n_levels)]
         U = self.static_features_[0].transform(Xf, return_on_gpu=True)
@@ -399,7 +399,7 @@ class SignatureFeatures(SignatureBase, static.features.KernelFeatures):
     else:
       U = X
     if self.projection is not None:
-      hps = self.projection.get_params()
+      hps = self.projection.get_params(deep=False)
       cls = self.projection.__class__
       self.projections_ = [cls(**hps).fit(U)]
       V = self.projections_[0](U, return_on_gpu=True)

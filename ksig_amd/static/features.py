@@ -1,9 +1,12 @@
 """Featurized kernels: the ``KernelFeatures`` base of
 ksig/static/features.py:21-177 (fit / transform / kernel through the
-Euclidean inner product of the features), used by RandomWarpingSeries."""
+Euclidean inner product of the features), used by RandomWarpingSeries and
+SignatureFeatures, and the static feature maps RandomFourierFeatures and
+NystroemFeatures."""
 
 from __future__ import annotations
 
+import warnings
 from abc import ABCMeta, abstractmethod
 from typing import Optional
 
@@ -111,4 +114,43 @@ class RandomFourierFeatures(StaticFeatures):
     W = self.random_weights_.to(device=X.device, dtype=X.dtype)
     proj = X.reshape(-1, self.n_features_) @ W
     F = torch.cat([torch.sin(proj), torch.cos(proj)], -1) / np.sqrt(self.n_components)
+    return F.reshape(tuple(X.shape[:-1]) + (-1,))
+
+
+class NystroemFeatures(StaticFeatures):
+  """Nystroem feature map (ksig/static/features.py:205-285): ``n_components``
+  basis points drawn without replacement from the fitted points, the basis
+  Gram K_bb = U diag(s) U^T, and features K(X, basis) U_+ diag(s_+)^(-1/2)
+  over the eigenvalues with |s| > 1e-12 (:249-255).  Both Grams run through
+  the base kernel on the device; the small symmetric eigenproblem is solved
+  in float64 (the reference solves it in the data dtype) and the factor is
+  kept in the data dtype.  The basis indices are drawn with NumPy (the
+  reference draws them with CuPy)."""
+
+  def __init__(self, base_kernel: Kernel, n_components: int = 100, random_state=None):
+    super().__init__(n_components=n_components, random_state=random_state)
+    self.base_kernel = base_kernel
+
+  def _make_feature_components(self, X):
+    X = X.reshape(-1, self.n_features_)
+    n_samples = X.shape[0]
+    if self.n_components >= n_samples:
+      warnings.warn('`n_samples <= n_components`, hence `n_components` was set to '
+                    '`n_samples`, which results in inefficient evaluation of the full kernel.')
+      self.n_components_ = n_samples
+    else:
+      self.n_components_ = self.n_components
+    idx = self.random_state.choice(n_samples, size=self.n_components_, replace=False)
+    basis = X[torch.as_tensor(idx, device=X.device)]
+    Kb = self.base_kernel(basis, return_on_gpu=True).to(torch.float64)
+    S, U = torch.linalg.eigh(Kb)
+    keep = utils.robust_nonzero(S)
+    self.normalization_ = (U[:, keep] / utils.robust_sqrt(S[keep])[None, :]).to(X.dtype)
+    self.components_ = basis
+    self.component_idx_ = idx
+
+  def _compute_features(self, X):
+    emb = self.base_kernel(X.reshape(-1, self.n_features_), self.components_,
+                           return_on_gpu=True)
+    F = emb.to(self.normalization_.dtype) @ self.normalization_
     return F.reshape(tuple(X.shape[:-1]) + (-1,))

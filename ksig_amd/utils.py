@@ -82,6 +82,11 @@ def squared_euclid_dist(X: torch.Tensor, Y: Optional[torch.Tensor] = None
   return Xn2[..., :, None] + Yn2[..., None, :] - 2 * matrix_mult(X, Y, transpose_Y=True)
 
 
+def robust_nonzero(X: torch.Tensor) -> torch.Tensor:
+  """ksig/utils.py:199-208: |X| > eps elementwise."""
+  return torch.abs(X) > _EPS
+
+
 def robust_sqrt(X: torch.Tensor) -> torch.Tensor:
   """ksig/utils.py:173-182."""
   return torch.sqrt(torch.clamp(X, min=_EPS))

@@ -24,7 +24,8 @@ difference it, per-level exclusive 2-D cumulative sums):
 * random warping series ..... ksig/algorithms.py:565-659 (DTW recursion),
                               ksig/kernels.py:459-477 (features)
 * low-rank signature features ksig/algorithms.py:148-246 (first order),
-                              :249-337 (higher order),
+                              :249-337 (higher order), Nystroem features
+                              ksig/static/features.py:205-285,
                               ksig/kernels.py:263-361, random Fourier features
                               ksig/static/features.py:288-342, tensorized
                               random projection ksig/projections.py:465-584
@@ -504,6 +505,25 @@ def rff_features(X: np.ndarray, weights: np.ndarray) -> np.ndarray:
   for given weights [d, D]: [sin(XW), cos(XW)] / sqrt(D)."""
   proj = X @ weights
   return np.concatenate([np.sin(proj), np.cos(proj)], -1) / np.sqrt(weights.shape[1])
+
+
+def nystroem_features(X: np.ndarray, basis_idx, kind='rbf', param=1.0) -> np.ndarray:
+  """NystroemFeatures (ksig/static/features.py:227-285) for given basis
+  indices into the flattened points of X [..., d]: K(X, B) U_+ diag(s_+)^-1/2
+  with K(B, B) = U diag(s) U^T and |s| > 1e-12 kept."""
+  Xf = X.reshape(-1, X.shape[-1])
+  B = Xf[np.asarray(basis_idx)]
+  s, U = np.linalg.eigh(static_gram(B, B, kind, param))
+  keep = np.abs(s) > EPS
+  norm = U[:, keep] / np.sqrt(np.maximum(s[keep], EPS))[None, :]
+  return (static_gram(Xf, B, kind, param) @ norm).reshape(X.shape[:-1] + (-1,))
+
+
+def low_rank_features_normalised(P, n_levels):
+  """Level features normalised to unit norm, concatenated, divided by
+  sqrt(n_levels + 1) (ksig/kernels.py:354-361)."""
+  P = [p / np.maximum(np.sqrt(np.maximum(np.sum(p * p, -1), EPS)), EPS)[..., None] for p in P]
+  return np.concatenate(P, -1) / np.sqrt(n_levels + 1)
 
 
 def signature_kern_low_rank_fo(U, n_levels, difference=True, projections=None):

@@ -172,3 +172,51 @@ def test_signature_features_shared_static_map():
   F = sf.transform(X)
   ref = O.signature_features(X.astype(np.float64), 3, [w] * 3, trp)
   assert normwise(F, ref) < 1e-5
+
+
+def test_nystroem_features():
+  """NystroemFeatures (ksig/static/features.py:205-285) vs the oracle given
+  the drawn basis indices; with every point in the basis F F^T = K."""
+  rng = np.random.default_rng(14)
+  X = rng.standard_normal((40, 3))
+  nf = ksig.static.features.NystroemFeatures(ksig.static.kernels.RBFKernel(1.5),
+                                             n_components=12, random_state=0).fit(X)
+  F = nf.transform(X)
+  ref = O.nystroem_features(X, nf.component_idx_, 'rbf', 1.5)
+  # eigenvectors are fixed up to sign (and rotation in repeated eigenspaces):
+  # the approximate Gram F F^T is what the map defines
+  assert F.shape == ref.shape and normwise(F @ F.T, ref @ ref.T) < 1e-9
+  B = X[nf.component_idx_]
+  own = O.static_gram(X, B, 'rbf', 1.5) @ nf.normalization_.double().cpu().numpy()
+  assert normwise(F, own) < 1e-9
+  with pytest.warns(UserWarning):
+    full = ksig.static.features.NystroemFeatures(ksig.static.kernels.RBFKernel(1.5),
+                                                 n_components=64).fit(X[:20])
+  Ff = full.transform(X[:20])
+  assert normwise(Ff @ Ff.T, O.static_gram(X[:20], X[:20], 'rbf', 1.5)) < 1e-8
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-9), (np.float32, 1e-2)])
+def test_signature_features_with_nystroem_and_trp(dtype, tol):
+  """RFSF with Nystroem static features.  float32 is loose: the factor
+  U_+ s_+^(-1/2) keeps eigenvalues down to 1e-12, so float32 rounding of
+  K(X, B) is scaled by s^(-1/2) before the time differences cancel; float64
+  pins the dataflow."""
+  rng = np.random.default_rng(15)
+  X = O.brownian(rng, 6, 20, 2).astype(dtype)
+  sf = ksig.kernels.SignatureFeatures(
+      n_levels=3,
+      static_features=ksig.static.features.NystroemFeatures(ksig.static.kernels.RBFKernel(1.0),
+                                                            n_components=16, random_state=5),
+      projection=ksig.projections.TensorizedRandomProjection(n_components=24, random_state=6))
+  sf.fit(X)
+  Xd = X.astype(np.float64)
+  # the static features with the object's basis and factor (eigenvector
+  # signs are the solver's)
+  U = [(O.static_gram(Xd.reshape(-1, 2), Xd.reshape(-1, 2)[f.component_idx_], 'rbf', 1.0)
+        @ f.normalization_.double().cpu().numpy()).reshape(6, 20, -1)
+       for f in sf.static_features_]
+  trp = [(p.components_.double().cpu().numpy(), p.scaling_) for p in sf.projections_]
+  ref = O.low_rank_features_normalised(O.signature_kern_low_rank_fo(U, 3, True, trp), 3)
+  F = sf.transform(X)
+  assert F.shape == ref.shape and normwise(F, ref) < tol

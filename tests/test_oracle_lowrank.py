@@ -144,3 +144,19 @@ def test_higher_order_trp_features_contract_unprojected_ones(order):
         T = np.multiply.outer(T, proj[k][0][:, p] * proj[k][1])
       np.testing.assert_allclose(feats[m][:, p], full[m] @ T.reshape(-1), rtol=1e-11,
                                  atol=1e-12)
+
+
+def test_nystroem_oracle_reproduces_kernel_on_its_span():
+  """With every point in the basis the Nystroem features reproduce the
+  Gram exactly (F F^T = K for a full-rank K), and any point's features are
+  the projection of its kernel section onto the basis span (so for basis
+  points F_b F^T = K(B, X))."""
+  rng = np.random.default_rng(9)
+  X = rng.standard_normal((2, 6, 3))
+  Xf = X.reshape(-1, 3)
+  F = O.nystroem_features(X, np.arange(12), 'rbf', 1.3).reshape(12, -1)
+  np.testing.assert_allclose(F @ F.T, O.static_gram(Xf, Xf, 'rbf', 1.3), atol=1e-9)
+  idx = rng.choice(12, 5, replace=False)
+  Fb = O.nystroem_features(X, idx, 'rbf', 1.3).reshape(12, -1)
+  np.testing.assert_allclose(Fb[idx] @ Fb.T, O.static_gram(Xf[idx], Xf, 'rbf', 1.3),
+                             atol=1e-9)
