@@ -1,8 +1,8 @@
 """Featurized kernels: the ``KernelFeatures`` base of
 ksig/static/features.py:21-177 (fit / transform / kernel through the
 Euclidean inner product of the features), used by RandomWarpingSeries and
-SignatureFeatures, and the static feature maps RandomFourierFeatures and
-NystroemFeatures."""
+SignatureFeatures, and the static feature maps RandomFourierFeatures,
+RandomFourierFeatures1D and NystroemFeatures."""
 
 from __future__ import annotations
 
@@ -153,4 +153,29 @@ class NystroemFeatures(StaticFeatures):
     emb = self.base_kernel(X.reshape(-1, self.n_features_), self.components_,
                            return_on_gpu=True)
     F = emb.to(self.normalization_.dtype) @ self.normalization_
+    return F.reshape(tuple(X.shape[:-1]) + (-1,))
+
+
+class RandomFourierFeatures1D(StaticFeatures):
+  """Random Fourier features of the RBF kernel with random phases
+  (ksig/static/features.py:344-390): cos(X W + b) sqrt(2 / n_components),
+  W ~ N(0, 1/bandwidth^2) of shape [n_features, n_components], b ~ U(0, 2 pi).
+  The product is a library GEMM; weights and phases are drawn on the host
+  with NumPy (weights first, then phases, as the reference draws them)."""
+
+  def __init__(self, bandwidth: float = 1., n_components: int = 100, random_state=None):
+    super().__init__(n_components=n_components, random_state=random_state)
+    self.bandwidth = utils.check_positive_value(bandwidth, 'bandwidth')
+
+  def _make_feature_components(self, X):
+    self.n_components_ = self.n_components
+    w = self.random_state.normal(size=[self.n_features_, self.n_components_]) / self.bandwidth
+    b = 2 * np.pi * self.random_state.uniform(size=[1, self.n_components_])
+    self.random_weights_ = torch.as_tensor(w, device=X.device, dtype=X.dtype)
+    self.random_shift_ = torch.as_tensor(b, device=X.device, dtype=X.dtype)
+
+  def _compute_features(self, X):
+    W = self.random_weights_.to(device=X.device, dtype=X.dtype)
+    b = self.random_shift_.to(device=X.device, dtype=X.dtype)
+    F = torch.cos(X.reshape(-1, self.n_features_) @ W + b) * np.sqrt(2 / self.n_components)
     return F.reshape(tuple(X.shape[:-1]) + (-1,))

@@ -25,7 +25,8 @@ difference it, per-level exclusive 2-D cumulative sums):
                               ksig/kernels.py:459-477 (features)
 * low-rank signature features ksig/algorithms.py:148-246 (first order),
                               :249-337 (higher order), Nystroem features
-                              ksig/static/features.py:205-285,
+                              ksig/static/features.py:205-285, random
+                              Fourier features with phases :344-390,
                               ksig/kernels.py:263-361, random Fourier features
                               ksig/static/features.py:288-342, tensorized
                               random projection ksig/projections.py:465-584
@@ -505,6 +506,12 @@ def rff_features(X: np.ndarray, weights: np.ndarray) -> np.ndarray:
   for given weights [d, D]: [sin(XW), cos(XW)] / sqrt(D)."""
   proj = X @ weights
   return np.concatenate([np.sin(proj), np.cos(proj)], -1) / np.sqrt(weights.shape[1])
+
+
+def rff1d_features(X: np.ndarray, weights: np.ndarray, shift: np.ndarray) -> np.ndarray:
+  """RandomFourierFeatures1D._compute_features (ksig/static/features.py:381-390)
+  for given weights [d, D] and phases [1, D]: cos(XW + b) sqrt(2/D)."""
+  return np.cos(X @ weights + shift) * np.sqrt(2 / weights.shape[1])
 
 
 def nystroem_features(X: np.ndarray, basis_idx, kind='rbf', param=1.0) -> np.ndarray:

@@ -220,3 +220,27 @@ def test_signature_features_with_nystroem_and_trp(dtype, tol):
   ref = O.low_rank_features_normalised(O.signature_kern_low_rank_fo(U, 3, True, trp), 3)
   F = sf.transform(X)
   assert F.shape == ref.shape and normwise(F, ref) < tol
+
+
+def test_rff1d_features_and_signature_features():
+  rng = np.random.default_rng(16)
+  X = O.brownian(rng, 5, 18, 3).astype(np.float32)
+  rf = ksig.static.features.RandomFourierFeatures1D(bandwidth=1.2, n_components=32,
+                                                    random_state=2).fit(X.reshape(-1, 3))
+  F = rf.transform(X.reshape(-1, 3))
+  ref = O.rff1d_features(X.reshape(-1, 3).astype(np.float64),
+                         rf.random_weights_.double().cpu().numpy(),
+                         rf.random_shift_.double().cpu().numpy())
+  assert normwise(F, ref) < 1e-5
+  sf = ksig.kernels.SignatureFeatures(
+      n_levels=3,
+      static_features=ksig.static.features.RandomFourierFeatures1D(n_components=16,
+                                                                   random_state=3),
+      projection=ksig.projections.TensorizedRandomProjection(n_components=20, random_state=4))
+  sf.fit(X)
+  Xd = X.astype(np.float64)
+  U = [O.rff1d_features(Xd, f.random_weights_.double().cpu().numpy(),
+                        f.random_shift_.double().cpu().numpy()) for f in sf.static_features_]
+  trp = [(p.components_.double().cpu().numpy(), p.scaling_) for p in sf.projections_]
+  ref = O.low_rank_features_normalised(O.signature_kern_low_rank_fo(U, 3, True, trp), 3)
+  assert normwise(sf.transform(X), ref) < 1e-4

@@ -160,3 +160,17 @@ def test_nystroem_oracle_reproduces_kernel_on_its_span():
   Fb = O.nystroem_features(X, idx, 'rbf', 1.3).reshape(12, -1)
   np.testing.assert_allclose(Fb[idx] @ Fb.T, O.static_gram(Xf[idx], Xf, 'rbf', 1.3),
                              atol=1e-9)
+
+
+def test_rff1d_oracle_approximates_rbf_kernel():
+  """Bochner: with W ~ N(0, 1/sigma^2) and b ~ U(0, 2 pi), E[cos(w.x + b)
+  cos(w.y + b)] * 2 = exp(-|x-y|^2 / (2 sigma^2)); at D = 40000 the Monte
+  Carlo error is a few 1e-3 (a dropped sqrt(2), a phase left out or a wrong
+  bandwidth scale is off by far more)."""
+  rng = np.random.default_rng(10)
+  X = rng.standard_normal((8, 3)) * 0.7
+  sigma, D = 1.3, 40000
+  W = rng.standard_normal((3, D)) / sigma
+  b = 2 * np.pi * rng.uniform(size=(1, D))
+  F = O.rff1d_features(X, W, b)
+  np.testing.assert_allclose(F @ F.T, O.static_gram(X, X, 'rbf', sigma), atol=0.03)
