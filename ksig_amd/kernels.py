@@ -374,12 +374,14 @@ class SignatureFeatures(SignatureBase, static.features.KernelFeatures):
 
   def _make_feature_components(self, X):
     """ksig/kernels.py:295-328."""
-    from .projections import DiagonalProjection, TensorizedRandomProjection
+    from .projections import (DiagonalProjection, GaussianRandomProjection,
+                              TensorizedRandomProjection)
     if self.projection is not None and not isinstance(
-        self.projection, (TensorizedRandomProjection, DiagonalProjection)):
+        self.projection,
+        (TensorizedRandomProjection, DiagonalProjection, GaussianRandomProjection)):
       raise NotImplementedError('low-rank signature features are built with a '
-                                'TensorizedRandomProjection, a DiagonalProjection or '
-                                'no projection')
+                                'TensorizedRandomProjection, a DiagonalProjection, a '
+                                'GaussianRandomProjection or no projection')
     if isinstance(self.projection, DiagonalProjection) and (
         self.projection.internal_size != 2 or self.n_levels > 5 or self._order() != 1):
       raise NotImplementedError('the diagonal projection is built for internal_size=2, '
@@ -455,7 +457,7 @@ class SignatureFeatures(SignatureBase, static.features.KernelFeatures):
     U = self._increments(X)
     n = X.shape[0]
     ones = torch.ones((n, 1), device=X.device, dtype=torch.float64)
-    from .projections import DiagonalProjection
+    from .projections import DiagonalProjection, TensorizedRandomProjection
     if self.projections_ is not None and isinstance(self.projections_[0], DiagonalProjection):
       # RFSF-DP: the (sin, cos) increments go straight into the level scan
       lev = _ops.lowrank_levels_diag(U.contiguous())
@@ -465,6 +467,24 @@ class SignatureFeatures(SignatureBase, static.features.KernelFeatures):
         w = (2 << i) * H
         P.append(lev[:, off:off + w])
         off += w
+    elif self.projections_ is not None and not isinstance(self.projections_[0],
+                                                           TensorizedRandomProjection):
+      # any other projection: the reference's recursion (ksig/algorithms.py:
+      # 218-246, :296-337) with the projection called on each (running sum,
+      # increments) pair, in float64
+      proj = self.projections_
+      order = self._order()
+      Ud = U.to(torch.float64)
+      Rs = [proj[0](Ud[0], return_on_gpu=True)]
+      P = [ones, Rs[0].sum(1)]
+      for i in range(1, self.n_levels):
+        S = sum(Rs)
+        Q = torch.cumsum(S, 1) - S  # exclusive cumsum over time
+        nxt = [proj[i](Q, Ud[i], return_on_gpu=True)]
+        for r in range(1, min(i + 1, order)):
+          nxt.append(proj[i](Rs[r - 1], Ud[i], return_on_gpu=True) / (r + 1))
+        Rs = nxt
+        P.append(sum(R.sum(1) for R in Rs))
     elif self.projections_ is not None:
       # projected increments, one batched GEMM over the levels: level 0
       # alone (scaled), level m >= 1 the projection of U_m that multiplies

@@ -1,6 +1,8 @@
 """Random projections for the low-rank signature features: the
-``TensorizedRandomProjection`` of ksig/projections.py:465-584 (rank 1) over
-the ``RandomProjection`` base (:20-187).  Components are drawn on the host
+``GaussianRandomProjection`` of ksig/projections.py:189-242, the
+``TensorizedRandomProjection`` of :465-584 (rank 1) and the
+``DiagonalProjection`` of :587-674 over the ``RandomProjection`` base
+(:20-187).  Components are drawn on the host
 with NumPy (the reference draws them with CuPy)."""
 
 from __future__ import annotations
@@ -61,6 +63,27 @@ class RandomProjection(BaseEstimator, TransformerMixin, metaclass=ABCMeta):
 
   def __call__(self, X, Y=None, return_on_gpu: bool = False):
     return self.transform(X, Y=Y, return_on_gpu=return_on_gpu)
+
+
+class GaussianRandomProjection(RandomProjection):
+  """Gaussian random projection (ksig/projections.py:189-242): components
+  N(0, 1) of shape [n_components, n_features], X -> X C^T / sqrt(n_components)
+  as one library GEMM; (X, Y) projects the outer product X (x) Y flattened
+  with X's index major (ksig/utils.py:160-172), formed explicitly as the
+  reference does."""
+
+  def _make_projection_components(self, X, Y=None):
+    c = self.random_state.normal(size=[self.n_components, self.n_features_])
+    self.components_ = torch.as_tensor(c, device=X.device, dtype=X.dtype)
+    self.scaling_ = 1. / np.sqrt(self.n_components)
+
+  def _project(self, X):
+    C = self.components_.to(device=X.device, dtype=X.dtype)
+    P = self.scaling_ * (X.reshape(-1, self.n_features_) @ C.T)
+    return P.reshape(tuple(X.shape[:-1]) + (self.n_components,))
+
+  def _project_outer_prod(self, X, Y):
+    return self._project((X[..., :, None] * Y[..., None, :]).reshape(tuple(X.shape[:-1]) + (-1,)))
 
 
 class TensorizedRandomProjection(RandomProjection):

@@ -615,6 +615,35 @@ def signature_kern_low_rank(U, n_levels, order=1, difference=True, projections=N
   return P
 
 
+def signature_kern_low_rank_gauss(U, n_levels, comps, order=1, difference=True):
+  """Low-rank signature features (ksig/algorithms.py:146-337, any order) with
+  GaussianRandomProjection components (ksig/projections.py:189-242): comps[i]
+  = (C [P, F], s); level 0 projects U_0 as s C U, level i >= 1 projects the
+  outer product of a running sum (or order term) with U_i, X index major."""
+  order = normalize_order(n_levels, order)
+  Us = U if isinstance(U, list) else [U] * n_levels
+  if difference:
+    Us = [np.diff(u, axis=1) for u in Us]
+  n, steps = Us[0].shape[:2]
+
+  def proj(i, A, B=None):
+    C, sc = comps[i]
+    if B is not None:
+      A = (A[..., :, None] * B[..., None, :]).reshape(n, steps, -1)
+    return sc * (A @ C.T)
+
+  Rs = [proj(0, Us[0])]
+  P = [np.ones((n, 1)), Rs[0].sum(1)]
+  for i in range(1, n_levels):
+    Q = multi_cumsum(sum(Rs), exclusive=True, axis=1)
+    nxt = [proj(i, Q, Us[i])]
+    for r in range(1, min(i + 1, order)):
+      nxt.append(proj(i, Rs[r - 1], Us[i]) / (r + 1))
+    Rs = nxt
+    P.append(sum(R.sum(1) for R in Rs))
+  return P
+
+
 def signature_features(X, n_levels, rff_weights, trp=None, difference=True, normalize=True,
                        order=1):
   """SignatureFeatures._compute_features (ksig/kernels.py:330-361) with

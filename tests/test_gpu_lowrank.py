@@ -244,3 +244,25 @@ def test_rff1d_features_and_signature_features():
   trp = [(p.components_.double().cpu().numpy(), p.scaling_) for p in sf.projections_]
   ref = O.low_rank_features_normalised(O.signature_kern_low_rank_fo(U, 3, True, trp), 3)
   assert normwise(sf.transform(X), ref) < 1e-4
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_signature_features_gaussian_projection(order):
+  """RFSF with GaussianRandomProjection (ksig/projections.py:189-242): the
+  reference's recursion with the projection on each (running sum,
+  increments) outer product, vs the oracle given the object's components."""
+  rng = np.random.default_rng(17)
+  X = O.brownian(rng, 5, 16, 2).astype(np.float32)
+  sf = ksig.kernels.SignatureFeatures(
+      n_levels=3, order=order,
+      static_features=ksig.static.features.RandomFourierFeatures(n_components=8, random_state=1),
+      projection=ksig.projections.GaussianRandomProjection(n_components=12, random_state=2))
+  sf.fit(X)
+  w = [f.random_weights_.double().cpu().numpy() for f in sf.static_features_]
+  U = [O.rff_features(X.astype(np.float64), wi) for wi in w]
+  comps = [(p.components_.double().cpu().numpy(), p.scaling_) for p in sf.projections_]
+  assert comps[0][0].shape == (12, 16) and comps[1][0].shape == (12, 12 * 16)
+  ref = O.low_rank_features_normalised(
+      O.signature_kern_low_rank_gauss(U, 3, comps, order), 3)
+  F = sf.transform(X)
+  assert F.shape == (5, 1 + 3 * 12) and normwise(F, ref) < 1e-5

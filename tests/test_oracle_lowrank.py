@@ -174,3 +174,23 @@ def test_rff1d_oracle_approximates_rbf_kernel():
   b = 2 * np.pi * rng.uniform(size=(1, D))
   F = O.rff1d_features(X, W, b)
   np.testing.assert_allclose(F @ F.T, O.static_gram(X, X, 'rbf', sigma), atol=0.03)
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_gaussian_projection_features_are_linear_images(order):
+  """Gaussian projections: level m is A_m vec(unprojected level m) with
+  A_1 = s_0 C_0 and A_m = s_m C_m (A_{m-1} (x) I_D) (each projection acts on
+  the outer product with the previous level's index major), at every order."""
+  rng = np.random.default_rng(11)
+  n_levels, D, P = 3, 3, 5
+  U = rng.standard_normal((2, 7, D))
+  comps = [(rng.standard_normal((P, D)), 1 / np.sqrt(P))] + [
+      (rng.standard_normal((P, P * D)), 1 / np.sqrt(P)) for _ in range(n_levels - 1)]
+  feats = O.signature_kern_low_rank_gauss(U, n_levels, comps, order)
+  full = O.signature_kern_low_rank(U, n_levels, order, True, None)
+  A = comps[0][1] * comps[0][0]
+  np.testing.assert_allclose(feats[1], full[1] @ A.T, rtol=1e-11, atol=1e-12)
+  for m in range(2, n_levels + 1):
+    C, sc = comps[m - 1]
+    A = sc * C @ np.kron(A, np.eye(D))
+    np.testing.assert_allclose(feats[m], full[m] @ A.T, rtol=1e-11, atol=1e-12)
