@@ -375,13 +375,15 @@ class SignatureFeatures(SignatureBase, static.features.KernelFeatures):
   def _make_feature_components(self, X):
     """ksig/kernels.py:295-328."""
     from .projections import (DiagonalProjection, GaussianRandomProjection,
-                              TensorizedRandomProjection)
+                              SubsamplingProjection, TensorizedRandomProjection,
+                              VerySparseRandomProjection)
     if self.projection is not None and not isinstance(
-        self.projection,
-        (TensorizedRandomProjection, DiagonalProjection, GaussianRandomProjection)):
-      raise NotImplementedError('low-rank signature features are built with a '
-                                'TensorizedRandomProjection, a DiagonalProjection, a '
-                                'GaussianRandomProjection or no projection')
+        self.projection, (TensorizedRandomProjection, DiagonalProjection,
+                          GaussianRandomProjection, SubsamplingProjection,
+                          VerySparseRandomProjection)):
+      raise NotImplementedError('low-rank signature features are built with a tensorized, '
+                                'diagonal, Gaussian, subsampling or very sparse projection, '
+                                'or none')
     if isinstance(self.projection, DiagonalProjection) and (
         self.projection.internal_size != 2 or self.n_levels > 5 or self._order() != 1):
       raise NotImplementedError('the diagonal projection is built for internal_size=2, '

@@ -1,5 +1,7 @@
 """Random projections for the low-rank signature features: the
 ``GaussianRandomProjection`` of ksig/projections.py:189-242, the
+``SubsamplingProjection`` of :245-284, the ``VerySparseRandomProjection`` of
+:287-376, the
 ``TensorizedRandomProjection`` of :465-584 (rank 1) and the
 ``DiagonalProjection`` of :587-674 over the ``RandomProjection`` base
 (:20-187).  Components are drawn on the host
@@ -84,6 +86,68 @@ class GaussianRandomProjection(RandomProjection):
 
   def _project_outer_prod(self, X, Y):
     return self._project((X[..., :, None] * Y[..., None, :]).reshape(tuple(X.shape[:-1]) + (-1,)))
+
+
+class SubsamplingProjection(RandomProjection):
+  """Subsampling projection (ksig/projections.py:245-284): n_components
+  coordinates drawn without replacement, scaled by
+  sqrt(n_features / n_components); on an outer product only the drawn
+  entries are formed (index k = i * d_Y + j takes X_i Y_j, ksig/utils.py
+  subsample_outer_prod)."""
+
+  def _make_projection_components(self, X, Y=None):
+    idx = self.random_state.choice(self.n_features_, size=self.n_components, replace=False)
+    self.sampled_idx_ = torch.as_tensor(idx, device=X.device)
+    self.scaling_ = float(np.sqrt(self.n_features_ / self.n_components))
+
+  def _project(self, X):
+    return self.scaling_ * X[..., self.sampled_idx_.to(X.device)]
+
+  def _project_outer_prod(self, X, Y):
+    idx = self.sampled_idx_.to(X.device)
+    dy = Y.shape[-1]
+    return self.scaling_ * X[..., idx // dy] * Y[..., idx % dy]
+
+
+class VerySparseRandomProjection(RandomProjection):
+  """Very sparse random projection (ksig/projections.py:287-376) as the
+  reference computes it: the Bernoulli mask of density p (log(F)/F or
+  1/sqrt(F)) is drawn and then replaced by the Rademacher draw (:340-345
+  assign rather than multiply), so the components are dense +-1 over every
+  column, scaled by sqrt(1 / (p n_components)); kept as is so that the
+  features equal the reference's."""
+
+  def __init__(self, n_components: int = 100, sparsity: str = 'log', random_state=None):
+    if sparsity not in ('sqrt', 'log'):
+      raise ValueError(f'Unknown sparsity mode ({sparsity}).',
+                       "Possible values are ['sqrt', 'log'].")
+    self.sparsity = sparsity
+    super().__init__(n_components=n_components, random_state=random_state)
+
+  def _make_projection_components(self, X, Y=None):
+    F = self.n_features_
+    prob = np.log(F) / F if self.sparsity == 'log' else 1. / np.sqrt(F)
+    mask = self.random_state.uniform(size=[self.n_components, F]) < prob
+    mask[0, 0] = True
+    full = np.where(self.random_state.uniform(size=[self.n_components, F]) < 0.5, -1., 1.)
+    idx = np.where(np.any(np.abs(full) > 1e-12, axis=0))[0]
+    self.sampled_idx_ = torch.as_tensor(idx, device=X.device)
+    self.n_sampled_ = len(idx)
+    self.components_ = torch.as_tensor(full[:, idx], device=X.device, dtype=X.dtype)
+    self.scaling_ = float(np.sqrt(1. / (prob * self.n_components)))
+
+  def _project(self, X):
+    C = self.components_.to(device=X.device, dtype=X.dtype)
+    Xs = X[..., self.sampled_idx_.to(X.device)].reshape(-1, self.n_sampled_)
+    return (self.scaling_ * (Xs @ C.T)).reshape(tuple(X.shape[:-1]) + (-1,))
+
+  def _project_outer_prod(self, X, Y):
+    idx = self.sampled_idx_.to(X.device)
+    dy = Y.shape[-1]
+    XY = X[..., idx // dy] * Y[..., idx % dy]
+    C = self.components_.to(device=X.device, dtype=X.dtype)
+    return (self.scaling_ * (XY.reshape(-1, self.n_sampled_) @ C.T)).reshape(
+        tuple(XY.shape[:-1]) + (-1,))
 
 
 class TensorizedRandomProjection(RandomProjection):

@@ -266,3 +266,36 @@ def test_signature_features_gaussian_projection(order):
       O.signature_kern_low_rank_gauss(U, 3, comps, order), 3)
   F = sf.transform(X)
   assert F.shape == (5, 1 + 3 * 12) and normwise(F, ref) < 1e-5
+
+
+@pytest.mark.parametrize("kind", ["subsampling", "very_sparse"])
+def test_signature_features_subsampling_and_very_sparse(kind):
+  """SubsamplingProjection (ksig/projections.py:245-284) and
+  VerySparseRandomProjection (:287-376) are linear maps: a row selection of
+  the identity, or the +-1 components on the kept columns, each scaled; the
+  pinned Gaussian-projection oracle takes them as its components."""
+  rng = np.random.default_rng(18)
+  X = O.brownian(rng, 5, 16, 2).astype(np.float32)
+  if kind == "subsampling":
+    proj = ksig.projections.SubsamplingProjection(n_components=10, random_state=2)
+  else:
+    proj = ksig.projections.VerySparseRandomProjection(n_components=10, random_state=2)
+  sf = ksig.kernels.SignatureFeatures(
+      n_levels=3, order=2,
+      static_features=ksig.static.features.RandomFourierFeatures(n_components=6, random_state=1),
+      projection=proj)
+  sf.fit(X)
+  w = [f.random_weights_.double().cpu().numpy() for f in sf.static_features_]
+  U = [O.rff_features(X.astype(np.float64), wi) for wi in w]
+  comps = []
+  for p in sf.projections_:
+    idx = p.sampled_idx_.cpu().numpy()
+    if kind == "subsampling":
+      C = np.eye(p.n_features_)[idx]
+    else:
+      C = np.zeros((p.n_components, p.n_features_))
+      C[:, idx] = p.components_.double().cpu().numpy()
+    comps.append((C, p.scaling_))
+  ref = O.low_rank_features_normalised(O.signature_kern_low_rank_gauss(U, 3, comps, 2), 3)
+  F = sf.transform(X)
+  assert F.shape == (5, 1 + 3 * 10) and normwise(F, ref) < 1e-5
