@@ -1,18 +1,25 @@
 """Benchmark of the signature-kernel hot path on one MI355X.
 
-Workload (default): 512 Brownian paths (cumsum of N(0, 1/L) increments),
-L = 200, d = 8, float32; truncated signature kernel by dynamic programming,
-n_levels = 5, order = 1, RBF bandwidth sqrt(8); full symmetric Gram plus
-diagonal, evaluated as the n(n+1)/2 = 131,328 upper-triangle pairs (the lower
-triangle is the mirror image).  A step is one pass of the fused HIP sweep over
-all those pairs with the inputs already resident in HBM.
+Default workload (the headline line): the batched uneven DP signature kernel,
+4096 sequence pairs whose lengths are drawn independently from U{32..1024},
+d = 16, float32 random walks (cumsum of N(0, 1/L) increments plus a
+per-sequence per-channel offset of 0.1 N(0, 1)), truncated signature kernel
+by dynamic programming with n_levels = 8, order = 2, RBF bandwidth 4 =
+sqrt(d).  A step is one persistent launch of the fused HIP row stream over all
+4096 pairs (length-bucketed slices, launch choices picked by the autotuner
+before the timed region), inputs already resident in HBM.
 
-Metric: DP cell-levels per second = evaluated pairs x (L-1)^2 x n_levels / s.
+Metric: DP cell-levels per second = sum over pairs of (Lx-1)(Ly-1) x n_levels / s.
 
-Prints ONE JSON line (rank 0) with the measured value, a roofline object for
-the sweep kernel (HIP-event timing on the launch stream), the FP64 CPU oracle
-timed on a bounded sample of the same workload, and the reference dataflow
-restated in plain FP32 GPU tensor ops as a comparator.
+The JSON line (rank 0) also carries: the per-pair error against the FP64
+oracle; the reference dataflow restated in plain FP32 GPU tensor ops (the
+comparator of the >= 5x target); the matrix-core Gram variants (bf16 hi/lo
+split and float32 MFMA, per-pair kernel) with their errors; a roofline object
+for the row-stream kernel (HIP-event timing on the launch stream); the FP64
+and FP32 NumPy oracle timed over a process pool on the host cores; the MMD^2
+training step (1024 vs 1024 paths, L = 128, d = 32, 6 levels) as forward+
+adjoint over forward; and the 512 x L = 200 x d = 8 symmetric Gram (5 levels,
+order 1) as a secondary figure.  Other workloads: --workload c1|c2|c4|gak|rws|rfsf.
 """
 
 from __future__ import annotations
@@ -127,55 +134,115 @@ def plain_fp32_levels_ho(X, Y, n_levels, order, sigma):
   return torch.stack(levels, -1)
 
 
-def cpu_baseline(X64: np.ndarray, n_levels: int, sigma: float, seconds: float):
-  """FP64 NumPy oracle (the reference algorithm and dataflow) on one host
-  core, over seeded pairs of the same workload until `seconds` elapse."""
-  from oracle import sigkern as O
+# Work of one process-pool CPU baseline, set before the pool forks (the
+# workers inherit it; nothing large is pickled).
+_CPU_JOB: dict = {}
+
+
+def _cpu_worker(wid: int):
+  """One pool worker: evaluate randomly drawn units of the workload with the
+  NumPy oracle on one thread until the job's time budget ends."""
+  job = _CPU_JOB
   try:
     from threadpoolctl import threadpool_limits
-    lim = threadpool_limits(1)
+    threadpool_limits(1)
   except Exception:  # pragma: no cover
-    lim = None
-  rng = np.random.default_rng(7)
-  n = X64.shape[0]
-  L = X64.shape[1]
-  batch = 8
-  cells = 0
-  pairs = 0
+    pass
+  rng = np.random.default_rng(job["seed"] + 7919 * wid)
   t0 = time.perf_counter()
-  while time.perf_counter() - t0 < seconds:
-    a = rng.integers(0, n, batch)
-    b = rng.integers(0, n, batch)
-    M = O.rbf_gram(X64[a], X64[b], sigma)
-    O.signature_kern(M, n_levels, 1, True, True)
-    cells += batch * (L - 1) ** 2 * n_levels
-    pairs += batch
-  dt = time.perf_counter() - t0
-  if lim is not None:
-    lim.unregister() if hasattr(lim, "unregister") else None
-  return {"value": cells / dt, "unit": "DP cell-levels/s", "cores": 1, "kind": "port",
-          "sample": f"{pairs} random pairs of the workload (L={L}, d={X64.shape[2]}, "
-                    f"{n_levels} levels), FP64 NumPy oracle, {dt:.1f} s on 1 host core"}
+  units = 0
+  work = 0.0
+  done = []
+  while time.perf_counter() - t0 < job["seconds"]:
+    u = job["pick"](rng)
+    val = job["eval"](u)
+    work += job["work"](u)
+    units += 1
+    if job.get("keep"):
+      done.append((u, val))
+  return work, units, time.perf_counter() - t0, done
+
+
+def host_cpu_model() -> str:
+  try:
+    with open("/proc/cpuinfo") as f:
+      for line in f:
+        if line.startswith("model name"):
+          return line.split(":", 1)[1].strip()
+  except OSError:  # pragma: no cover
+    pass
+  return "unknown"
+
+
+def pool_cores() -> int:
+  """Worker processes of the CPU baseline: every host core this process may
+  use, at most 16 (the host share of one GPU)."""
+  try:
+    n = len(os.sched_getaffinity(0))
+  except AttributeError:  # pragma: no cover
+    n = os.cpu_count() or 1
+  return max(1, min(16, n))
+
+
+def cpu_pool_run(pick, evaluate, work, seconds: float, seed: int, cores: int,
+                 keep: bool = False):
+  """Run the oracle over a fork pool of ``cores`` single-threaded workers for
+  ``seconds`` each.  Returns (work per second of the whole pool, units done,
+  work per second of one worker on average, kept (unit, value) results).
+  Must be called before this process initialises the GPU."""
+  import multiprocessing as mp
+  _CPU_JOB.clear()
+  _CPU_JOB.update(pick=pick, eval=evaluate, work=work, seconds=seconds, seed=seed, keep=keep)
+  ctx = mp.get_context("fork")
+  t0 = time.perf_counter()
+  with ctx.Pool(cores) as pool:
+    parts = pool.map(_cpu_worker, range(cores))
+  wall = time.perf_counter() - t0
+  tot_work = sum(p[0] for p in parts)
+  units = sum(p[1] for p in parts)
+  per_core = float(np.mean([p[0] / p[2] for p in parts]))
+  kept = [x for p in parts for x in p[3]]
+  span = max(p[2] for p in parts)
+  return tot_work / max(span, 1e-9), units, per_core, kept, wall
+
+
+def cpu_baseline_pool(unit: str, sample: str, runs: dict, cores: int) -> dict:
+  """The cpu_baseline object: FP64 over the pool is the value; the FP32 run
+  and the mean per-core rate are reported beside it."""
+  f64 = runs["f64"]
+  res = {"value": f64[0], "unit": unit, "cores": cores, "kind": "port",
+         "cpu_model": host_cpu_model(), "threads_per_worker": 1,
+         "sample": sample + f"; FP64: {f64[1]} units in {f64[4]:.1f} s wall"}
+  res["fp64_per_core"] = f64[2]
+  if "f32" in runs:
+    res["fp32_value"] = runs["f32"][0]
+    res["fp32_per_core"] = runs["f32"][2]
+    res["sample"] += f"; FP32: {runs['f32'][1]} units in {runs['f32'][4]:.1f} s wall"
+  return res
 
 
 def main():
   ap = argparse.ArgumentParser()
-  ap.add_argument("--workload", default="c1",
+  ap.add_argument("--workload", default="c3",
                   choices=["c1", "c2", "c3", "c4", "gak", "rws", "rfsf"],
-                  help="c1: 512 x L=200 x d=8 symmetric Gram (the headline line); "
-                       "c3: 4096 uneven-length pairs, L in [32, 1024], d=16, 8 levels, order 2; "
+                  help="c3 (default, the headline line): 4096 uneven-length pairs, L in "
+                       "[32, 1024], d=16, 8 levels, order 2, with the c4 forward+adjoint ratio "
+                       "and the c1 figure attached; c1: 512 x L=200 x d=8 symmetric Gram; "
                        "c4: MMD^2 training step, 1024 x 1024 paths, L=128, d=32, 6 levels; "
                        "c2: signature-PDE MMD^2 of 256 vs 256 paths, L=256, d=4, dyadic 1")
   ap.add_argument("--gpus", type=int, default=1)
   ap.add_argument("--steps", type=int, default=20)
   ap.add_argument("--warmup", type=int, default=3)
-  ap.add_argument("--cpu-seconds", type=float, default=12.0)
+  ap.add_argument("--cpu-seconds", type=float, default=10.0,
+                  help="time budget of each CPU-baseline pool run (FP64, then FP32)")
   ap.add_argument("--no-cpu", action="store_true")
   ap.add_argument("--no-plain", action="store_true")
+  ap.add_argument("--no-extras", action="store_true",
+                  help="c3: leave out the attached c4 and c1 figures")
   ap.add_argument("--no-tune", action="store_true",
                   help="c3: skip the autotuner and time the default schedule")
   ap.add_argument("--gram", default="valu", choices=["valu", "fp32", "bf16x3"],
-                  help="static-kernel Gram evaluation inside the sweep")
+                  help="c1: static-kernel Gram evaluation inside the sweep")
   ap.add_argument("--no-variants", action="store_true")
   ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                   help="only run an elementwise copy of MIB MiB (5 times): the "
@@ -208,7 +275,8 @@ def main():
   if args.workload == "c3":
     return bench_c3(args)
   if args.workload == "c4":
-    return bench_c4(args)
+    print(json.dumps(run_c4(args)), flush=True)
+    return
   if args.workload == "c2":
     return bench_c2(args)
   if args.workload == "gak":
@@ -217,16 +285,53 @@ def main():
     return bench_rws(args)
   if args.workload == "rfsf":
     return bench_rfsf(args)
+  cpu = None if args.no_cpu else c1_cpu_baseline(args.cpu_seconds)
+  res = run_c1(args)
+  if cpu is not None:
+    res["cpu_baseline"] = cpu
+  print(json.dumps(res), flush=True)
 
+
+C1 = dict(n=512, L=200, d=8, n_levels=5, seed=1234)
+
+
+def c1_cpu_baseline(seconds: float) -> dict:
+  """The FP64 and FP32 NumPy oracle over a process pool on the host cores, on
+  random pairs of the 512 x L=200 x d=8 workload (its own seeded paths with
+  the same distribution; the GPU is not touched)."""
+  from oracle import sigkern as O
+  n, L, d, nl = C1["n"], C1["L"], C1["d"], C1["n_levels"]
+  sigma = math.sqrt(d)
+  rng = np.random.default_rng(C1["seed"])
+  X64 = np.cumsum(rng.standard_normal((n, L, d)) / math.sqrt(L), 1)
+  X32 = X64.astype(np.float32)
+  batch = 8
+  cores = pool_cores()
+  runs = {}
+  for tag, X in (("f64", X64), ("f32", X32)):
+    runs[tag] = cpu_pool_run(
+        lambda r: (r.integers(0, n, batch), r.integers(0, n, batch)),
+        lambda u, X=X: O.signature_kern(O.rbf_gram(X[u[0]], X[u[1]], sigma), nl, 1, True, True),
+        lambda u: batch * (L - 1) ** 2 * nl, seconds, 7, cores)
+  return cpu_baseline_pool(
+      "DP cell-levels/s",
+      f"random pairs of the workload (L={L}, d={d}, {nl} levels, order 1), NumPy oracle "
+      f"vectorised over {batch} pairs per call, {cores} single-threaded worker processes, "
+      f"{seconds:.0f} s per precision", runs, cores)
+
+
+def run_c1(args) -> dict:
+  """512 Brownian paths, L=200, d=8, 5 levels, order 1, RBF sigma=sqrt(8):
+  the symmetric Gram as its 131,328 upper-triangle pairs in one launch."""
   import ksig_amd
   from ksig_amd import _lib, _ops
   from oracle import sigkern as O
 
   dev = torch.device("cuda", 0)
   torch.cuda.set_device(dev)
-  n, L, d, n_levels = 512, 200, 8, 5
+  n, L, d, n_levels = C1["n"], C1["L"], C1["d"], C1["n_levels"]
   sigma = math.sqrt(d)
-  gen = torch.Generator(device=dev).manual_seed(1234)
+  gen = torch.Generator(device=dev).manual_seed(C1["seed"])
   X = brownian(gen, n, L, d, dev)
   Xs = _ops.SeqSet.dense(X)
   pairs = _ops.triu_pairs(n, dev)
@@ -305,7 +410,8 @@ def main():
       "max_rel_err_vs_fp64": err,
       "gram": args.gram,
       "gram_variants": variants,
-      "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+      "roofline": {"bound": "valu" if args.gram == "valu" else "mfma", "achieved": achieved,
+                   "peak": PEAK_FP32_TFLOPS,
                    "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
                    "traffic": MEASURED_TRAFFIC_BYTES["c1"] if args.gram == "valu" else None,
                    "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE)",
@@ -331,9 +437,7 @@ def main():
                          "speedup_fused_over_plain": res["value"] / pv,
                          "sample": "8 x 512 pairs per call, torch FP32 ops"}
     del plain_levels
-  if not args.no_cpu:
-    res["cpu_baseline"] = cpu_baseline(X64, n_levels, sigma, args.cpu_seconds)
-  print(json.dumps(res), flush=True)
+  return res
 
 
 # per cell: static kernel (d multiply-adds, norm terms, exp), the transform
@@ -773,10 +877,10 @@ def kernel_probe(args):
                     "n_levels": nl, "launches": args.steps, "ms_per_launch": ms}), flush=True)
 
 
-def ragged_walks(rng: np.random.Generator, lens, d: int, offset_scale: float, dev):
+def ragged_walks_host(rng: np.random.Generator, lens, d: int, offset_scale: float):
   """Random walks of the given lengths (cumsum of N(0, 1/L) increments plus a
-  per-sequence per-channel offset), packed as a ragged float32 SeqSet."""
-  from ksig_amd import _ops
+  per-sequence per-channel offset), packed row-wise in float64: (walk
+  [sum of lengths, d], start row of every sequence)."""
   lens = np.asarray(lens, dtype=np.int64)
   total = int(lens.sum())
   seg = np.repeat(np.arange(len(lens)), lens)
@@ -785,24 +889,84 @@ def ragged_walks(rng: np.random.Generator, lens, d: int, offset_scale: float, de
   starts = np.concatenate([[0], np.cumsum(lens)[:-1]])
   base = np.where((starts > 0)[:, None], csum[np.maximum(starts - 1, 0)], 0.0)
   walk = csum - base[seg] + offset_scale * rng.standard_normal((len(lens), d))[seg]
+  return walk, starts
+
+
+def seqset_from_host(walk: np.ndarray, starts: np.ndarray, lens, dev):
+  """The float32 ragged SeqSet of packed host rows."""
+  from ksig_amd import _ops
+  lens = np.asarray(lens, dtype=np.int64)
   data = torch.as_tensor(walk.astype(np.float32), device=dev)
   off = torch.as_tensor(starts, device=dev, dtype=torch.int64)
   ln = torch.as_tensor(lens, device=dev, dtype=torch.int32)
-  return _ops.SeqSet(data, off, ln, int(lens.max()), len(lens), lens), walk
+  return _ops.SeqSet(data, off, ln, int(lens.max()), len(lens), lens)
 
 
-def bench_c3(args):
-  """4096 pairs of uneven lengths in one persistent row-stream launch."""
+def ragged_walks(rng: np.random.Generator, lens, d: int, offset_scale: float, dev):
+  """ragged_walks_host moved to the device as a float32 SeqSet: (set, walk)."""
+  walk, starts = ragged_walks_host(rng, lens, d, offset_scale)
+  return seqset_from_host(walk, starts, lens, dev), walk
+
+
+def c3_host_problem(seed: int = 2024) -> dict:
+  """The batched uneven workload on the host: 4096 pairs (x_p, y_p), each
+  length drawn from U{32..1024}, d=16, 8 levels, order 2, RBF sigma=4."""
+  n, d = 4096, 16
+  rng = np.random.default_rng(seed)
+  lx = rng.integers(32, 1025, n)
+  ly = rng.integers(32, 1025, n)
+  xw, xs = ragged_walks_host(rng, lx, d, 0.1)
+  yw, ys = ragged_walks_host(rng, ly, d, 0.1)
+  return dict(n=n, d=d, n_levels=8, order=2, sigma=4.0, lx=lx, ly=ly, xw=xw, yw=yw, xs=xs,
+              ys=ys)
+
+
+def c3_pair(P: dict, p: int, dtype):
+  """Pair p of the workload as the GPU sees it (float32 values) in ``dtype``."""
+  x = P["xw"][P["xs"][p]:P["xs"][p] + P["lx"][p]].astype(np.float32).astype(dtype)
+  y = P["yw"][P["ys"][p]:P["ys"][p] + P["ly"][p]].astype(np.float32).astype(dtype)
+  return x, y
+
+
+def c3_cpu_baseline(P: dict, seconds: float):
+  """The FP64 and FP32 NumPy oracle (the reference algorithm and dataflow:
+  materialised Gram, ΔΔ, per-level exclusive cumsums of the higher-order
+  recursion) over a process pool on the host cores, on uniformly drawn pairs
+  of the workload.  Returns (cpu_baseline object, {pair: FP64 levels})."""
+  from oracle import sigkern as O
+  nl, order, sigma = P["n_levels"], P["order"], P["sigma"]
+  cores = pool_cores()
+  runs, ref = {}, {}
+  for tag, dt in (("f64", np.float64), ("f32", np.float32)):
+    def evaluate(p, dt=dt):
+      x, y = c3_pair(P, p, dt)
+      return O.pair_levels(x, y, nl, order, True, "rbf", sigma)
+    runs[tag] = cpu_pool_run(
+        lambda r: int(r.integers(0, P["n"])), evaluate,
+        lambda p: float((P["lx"][p] - 1) * (P["ly"][p] - 1) * nl), seconds,
+        17 if tag == "f64" else 29, cores, keep=(tag == "f64"))
+    if tag == "f64":
+      ref = {int(p): v for p, v in runs[tag][3]}
+  obj = cpu_baseline_pool(
+      "DP cell-levels/s",
+      f"uniformly drawn pairs of the workload ({nl} levels, order {order}, d={P['d']}, "
+      f"lengths U{{32..1024}}), NumPy oracle one pair per call, {cores} single-threaded "
+      f"worker processes, {seconds:.0f} s per precision", runs, cores)
+  return obj, ref
+
+
+def run_c3(args, P: dict, ref64: dict) -> dict:
+  """4096 pairs of uneven lengths in one persistent row-stream launch; the
+  error is taken against the FP64 levels in ``ref64`` (pair -> levels), or
+  against 16 pairs evaluated here when it is empty."""
   from ksig_amd import _lib, _ops
   from oracle import sigkern as O
   dev = torch.device("cuda", 0)
   torch.cuda.set_device(dev)
-  n, d, n_levels, order, sigma = 4096, 16, 8, 2, 4.0
-  rng = np.random.default_rng(2024)
-  lx = rng.integers(32, 1025, n)
-  ly = rng.integers(32, 1025, n)
-  X, xw = ragged_walks(rng, lx, d, 0.1, dev)
-  Y, yw = ragged_walks(rng, ly, d, 0.1, dev)
+  n, d, n_levels, order, sigma = P["n"], P["d"], P["n_levels"], P["order"], P["sigma"]
+  lx, ly = P["lx"], P["ly"]
+  X = seqset_from_host(P["xw"], P["xs"], lx, dev)
+  Y = seqset_from_host(P["yw"], P["ys"], ly, dev)
   pairs = torch.stack([torch.arange(n, device=dev, dtype=torch.int32)] * 2, -1).contiguous()
   cells = float(np.sum((lx - 1) * (ly - 1)))
   cell_levels = cells * n_levels
@@ -844,44 +1008,78 @@ def bench_c3(args):
   kern_ms = e0.elapsed_time(e1) / args.steps
   ms_per_step = wall * 1e3 / args.steps
 
-  # per-pair error vs the FP64 oracle on a sample (inputs rounded to float32)
-  starts_x = np.concatenate([[0], np.cumsum(lx)[:-1]])
-  starts_y = np.concatenate([[0], np.cumsum(ly)[:-1]])
-  xw32 = xw.astype(np.float32).astype(np.float64)
-  yw32 = yw.astype(np.float32).astype(np.float64)
-  sample = np.random.default_rng(5).choice(n, 16, replace=False)
+  # per-pair error vs the FP64 oracle (inputs rounded to float32 as on the GPU)
+  if not ref64:
+    for p in np.random.default_rng(5).choice(n, 16, replace=False):
+      x, y = c3_pair(P, int(p), np.float64)
+      ref64 = dict(ref64)
+      ref64[int(p)] = O.pair_levels(x, y, n_levels, order, True, "rbf", sigma)
   got = out.cpu().numpy()
-  errs = []
-  for p in sample:
-    xa = xw32[starts_x[p]:starts_x[p] + lx[p]]
-    yb = yw32[starts_y[p]:starts_y[p] + ly[p]]
-    ref = O.pair_levels(xa, yb, n_levels, order, True, "rbf", sigma)
-    errs.append(float(np.max(np.abs(got[p] - ref)) / np.max(np.abs(ref))))
+
+  def pair_errors(vals):
+    return np.array([float(np.max(np.abs(vals[p] - r)) / np.max(np.abs(r)))
+                     for p, r in ref64.items()])
+  errs = pair_errors(got)
 
   res = {
-      "metric": "DP-cells/sec (pairs x (Lx-1)(Ly-1) x levels), fused fwd, fp32, uneven lengths",
+      "metric": "DP-cells/sec (pairs×L²×levels) fwd & fwd+bwd vs plain FP32, at ≤1e-5 rel err",
       "value": cell_levels / (ms_per_step * 1e-3),
       "unit": "DP cell-levels/s",
       "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
       "ms_per_step": ms_per_step,
       "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
       "dtype": "f32", "data": "synthetic (seeded random walks with channel offsets)",
-      "config": {"workload": "batched uneven DP kernel: 4096 pairs, L ~ U{32..1024} per "
-                             "sequence, d=16, 8 levels, order 2, RBF sigma=4",
-                 "cells": cells, "n_waves": int(sched[1].shape[0] - 1),
+      "config": {"workload": "batched uneven DP signature kernel: 4096 pairs, L ~ U{32..1024} "
+                             "per sequence, d=16, 8 levels, order 2, RBF sigma=4, one persistent "
+                             "launch (fused FP32 row stream)",
+                 "cells": cells, "cell_levels": cell_levels,
+                 "n_waves": int(sched[1].shape[0] - 1),
                  "global_fifo": bool(sched[3]), "plan_ms": plan_ms, "autotune": tuned},
-      "max_rel_err_vs_fp64": max(errs),
-      "err_sample": "16 random pairs, normwise over each pair's level vector",
+      "max_rel_err_vs_fp64": float(errs.max()),
+      "median_rel_err_vs_fp64": float(np.median(errs)),
+      "err_sample": f"{len(errs)} uniformly drawn pairs, normwise over each pair's level "
+                    "vector (K_0..K_8) against the FP64 oracle on the float32-rounded inputs",
   }
   flops_cell = recursion_flops(n_levels, order) + 3 + 2 * d + 3
   achieved = cells * flops_cell / (kern_ms * 1e-3) / 1e12
-  res["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+  res["roofline"] = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
                      "traffic": MEASURED_TRAFFIC_BYTES["c3"],
                      "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE)",
                      "kernel": "dp_stream_kernel<8,2,16,2,0,true,%s>" %
                                ("true" if sched[3] else "false"),
-                     "kernel_ms": kern_ms, "flops_per_cell": flops_cell}
+                     "kernel_ms": kern_ms, "flops_per_cell": flops_cell,
+                     "peak_note": "157.3 TFLOP/s counts packed FP32 FMAs; this build compiles "
+                                  "with packed FP32 disabled, so the issue ceiling of its "
+                                  "instructions is half of it"}
+  variants = {"valu_row_stream": {"kernel_ms": kern_ms,
+                                  "cell_levels_per_s": cell_levels / (kern_ms * 1e-3),
+                                  "max_rel_err_vs_fp64": float(errs.max())}}
+  if not args.no_variants:
+    # the static Gram on the matrix cores (bf16 hi/lo split with three
+    # products, or float32 MFMA): the per-pair MFMA kernel, one launch over
+    # all 4096 pairs, recursion in FP32 on the VALU
+    for name in ("bf16x3", "fp32"):
+      mode = _lib.GRAM_MODES[name]
+
+      def vstep(mode=mode):
+        return _ops.signature_levels(X, Y, pairs, n_levels, order, True, _lib.STATIC_RBF,
+                                     sigma, mode)
+      vo = vstep()
+      torch.cuda.synchronize()
+      e0.record(stream)
+      reps = 3
+      for _ in range(reps):
+        vo = vstep()
+      e1.record(stream)
+      torch.cuda.synchronize()
+      vms = e0.elapsed_time(e1) / reps
+      verr = pair_errors(vo.cpu().numpy())
+      variants[name + "_mfma_per_pair"] = {
+          "kernel_ms": vms, "cell_levels_per_s": cell_levels / (vms * 1e-3),
+          "max_rel_err_vs_fp64": float(verr.max()),
+          "kernel": "dp_mfma_kernel<8,2,16,%d>" % (1 if name == "bf16x3" else 0)}
+  res["gram_variants"] = variants
   if not args.no_plain:
     # plain FP32: the reference dataflow over length buckets (pairs padded to
     # the bucket's lengths by repeating their last point: ΔΔ of the padding
@@ -896,14 +1094,14 @@ def bench_c3(args):
       groups.setdefault((int(kx), int(ky)), []).append(int(p))
 
     def padded(walk, starts, lens, idx, L):
-      out = np.empty((len(idx), L, d), dtype=np.float32)
+      o = np.empty((len(idx), L, d), dtype=np.float32)
       for k, p in enumerate(idx):
         seq = walk[starts[p]:starts[p] + lens[p]]
-        out[k, :lens[p]] = seq
-        out[k, lens[p]:] = seq[-1]
-      return torch.as_tensor(out, device=dev)
-    batches = [(padded(xw, starts_x, lx, idx, kx), padded(yw, starts_y, ly, idx, ky), idx)
-               for (kx, ky), idx in groups.items()]
+        o[k, :lens[p]] = seq
+        o[k, lens[p]:] = seq[-1]
+      return torch.as_tensor(o, device=dev)
+    batches = [(padded(P["xw"], P["xs"], lx, idx, kx), padded(P["yw"], P["ys"], ly, idx, ky),
+                idx) for (kx, ky), idx in groups.items()]
     plain_out = {}
     for bx, by, idx in batches:   # warm-up + results
       lev = plain_fp32_levels_ho(bx, by, n_levels, order, sigma)
@@ -924,6 +1122,32 @@ def bench_c3(args):
                          "sample": "512 random pairs batched in length buckets of 256 "
                                    "(about 32 pairs per batch), torch FP32 ops",
                          "max_rel_diff_vs_fused": perr}
+    for v in variants.values():
+      v["speedup_over_plain_fp32"] = v["cell_levels_per_s"] / pv
+  return res
+
+
+def bench_c3(args):
+  """The headline line: C3, with the C4 forward+adjoint ratio and the C1
+  figure attached (unless --no-extras)."""
+  P = c3_host_problem()
+  cpu, ref64 = (None, {}) if args.no_cpu else c3_cpu_baseline(P, args.cpu_seconds)
+  res = run_c3(args, P, ref64)
+  if cpu is not None:
+    res["cpu_baseline"] = cpu
+  if not args.no_extras:
+    c4 = run_c4(args)
+    res["fwd_bwd_over_fwd"] = c4["bwd_over_fwd"]
+    res["c4_training_step"] = {
+        k: c4[k] for k in ("config", "ms_per_step", "value", "unit", "fwd_ms",
+                           "fwd_bwd_eager_ms", "fwd_bwd_graph_ms", "bwd_over_fwd",
+                           "graph_replay_max_rel_diff_vs_eager",
+                           "grad_scale_max_rel_err_vs_fp64", "loss_rel_err_vs_fp64",
+                           "grad_check", "plain_fp32") if k in c4}
+    c1 = run_c1(args)
+    res["c1_symmetric_gram"] = {
+        k: c1[k] for k in ("config", "ms_per_step", "value", "unit", "max_rel_err_vs_fp64",
+                           "roofline", "plain_fp32", "gram_variants") if k in c1}
   print(json.dumps(res), flush=True)
 
 
@@ -961,7 +1185,7 @@ def flat_sum(t: torch.Tensor) -> torch.Tensor:
   return v.view(-1, 1024).sum(1).sum()
 
 
-def bench_c4(args):
+def run_c4(args) -> dict:
   """MMD^2 training step: generated paths with a learnable per-channel scale
   against fixed targets; forward + adjoint, eager and HIP-graph captured."""
   from ksig_amd import _lib, _ops, autograd
@@ -1143,7 +1367,7 @@ def bench_c4(args):
     res["plain_fp32"] = {"value": pv, "unit": "DP cell-levels/s (fwd+bwd)",
                          "speedup_fused_over_plain": res["value"] / pv,
                          "sample": "8 x 1024 pairs of K_XY, torch FP32 ops + autograd"}
-  print(json.dumps(res), flush=True)
+  return res
 
 
 def bench_c2(args):

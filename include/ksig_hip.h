@@ -31,7 +31,7 @@ extern "C" {
 #define KSIG_STATIC_LINEAR 1 /* ksig/static/kernels.py:134-166, param = scale */
 
 /* static-kernel Gram evaluation inside the DP sweep (float32 inputs) */
-#define KSIG_GRAM_AUTO -1         /* MFMA float32 when available, else VALU */
+#define KSIG_GRAM_AUTO -1         /* the library's choice: the VALU sweep (faster than both MFMA forms at d <= 32) */
 #define KSIG_GRAM_VALU 0          /* one float32 (or float64) VALU dot product per cell */
 #define KSIG_GRAM_MFMA_F32 1      /* v_mfma_f32_16x16x4f32 tiles */
 #define KSIG_GRAM_MFMA_BF16X3 2   /* bf16 hi/lo split, 3 products, v_mfma_f32_16x16x32_bf16 */

@@ -92,10 +92,12 @@ class SignatureKernel(SignatureBase):
                static_kernel: Optional[StaticKernel] = None,
                gram_precision: str = 'auto'):
     """``gram_precision`` (no reference counterpart) selects how float32
-    static-kernel values are produced inside the sweep: 'auto' (matrix-core
-    float32 when available), 'valu' (one float32 dot product per cell on the
-    vector units), 'fp32' (float32 MFMA tiles), 'bf16x3' (bf16 hi/lo split,
-    three MFMA products).  float64 inputs always use exact float64 arithmetic."""
+    static-kernel values are produced inside the sweep: 'auto' (the library's
+    choice, which is the vector-unit sweep: measured faster than both
+    matrix-core forms for the fused kernels), 'valu' (one float32 dot product
+    per cell on the vector units), 'fp32' (float32 MFMA tiles), 'bf16x3' (bf16
+    hi/lo split, three MFMA products).  float64 inputs always use exact float64
+    arithmetic."""
     super().__init__(n_levels=n_levels, order=order, difference=difference,
                      normalize=normalize, n_features=n_features)
     self.static_kernel = static_kernel or static.kernels.RBFKernel()
