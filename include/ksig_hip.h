@@ -118,6 +118,38 @@ int ksig_signature_levels_batched(const void* x, const int64_t* x_off, const int
                                   int n_waves, int max_slice, void* workspace,
                                   size_t workspace_bytes, double* out, void* stream);
 
+/* Size of the device scratch array behind ksig_signature_levels_batched_noalloc. */
+#define KSIG_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+
+/*
+ * Allocation-free variant of ksig_signature_levels_batched with the band FIFO
+ * in global memory (the long-sequence choice), for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid allocation).  No counterpart in the reference.
+ * Same arguments and results as ksig_signature_levels_batched with a
+ * workspace, except that the FIFO lives in a device array of the library
+ * (KSIG_NOALLOC_SCRATCH_BYTES = 64 MiB per GPU, held until the process ends,
+ * its device address looked up on the first call on each GPU).  Fails with
+ * KSIG_EINVAL, launching nothing, when the FIFO of n_waves slices needs more
+ * than the scratch (n_waves * the *workspace_per_wave of ksig_stream_slots
+ * with global_fifo = 1).  Needs d in {8, 16, 32} and 16-byte aligned inputs.
+ * Every call on one GPU shares that GPU's scratch: calls that overlap on one
+ * GPU (concurrent streams or host threads) race and must be serialised by
+ * the caller.  The FIFO contents need no clearing between calls.  Calls no
+ * hipMalloc, hipFree or allocating library routine.
+ */
+int ksig_signature_levels_batched_noalloc(const void* x, const int64_t* x_off,
+                                          const int32_t* x_len, int max_len_x, const void* y,
+                                          const int64_t* y_off, const int32_t* y_len,
+                                          int max_len_y, int d, const int32_t* pairs,
+                                          int64_t n_pairs, int static_kind, double static_param,
+                                          int n_levels, int order, const int32_t* sched_order,
+                                          const int32_t* wave_beg, int n_waves, int max_slice,
+                                          double* out, void* stream);
+
+/* KSIG_NOALLOC_SCRATCH_BYTES, as compiled into the library. */
+size_t ksig_noalloc_scratch_bytes(void);
+
 /*
  * Training forward: ksig_signature_levels for order 1 with difference, which
  * also writes the boundary state the adjoint needs.  For pair p, save holds

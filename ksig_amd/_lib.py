@@ -37,6 +37,10 @@ SIGNATURES = {
     "ksig_signature_levels_batched": (_c_int, [
         _vp, _vp, _vp, _c_int, _vp, _vp, _vp, _c_int, _c_int, _vp, _c_i64, _c_int,
         _c_dbl, _c_int, _c_int, _vp, _vp, _c_int, _c_int, _vp, _c_size, _vp, _vp]),
+    "ksig_signature_levels_batched_noalloc": (_c_int, [
+        _vp, _vp, _vp, _c_int, _vp, _vp, _vp, _c_int, _c_int, _vp, _c_i64, _c_int,
+        _c_dbl, _c_int, _c_int, _vp, _vp, _c_int, _c_int, _vp, _vp]),
+    "ksig_noalloc_scratch_bytes": (_c_size, []),
     "ksig_signature_levels_fwd_save": (_c_int, [
         _c_int, _vp, _vp, _vp, _c_int, _vp, _vp, _vp, _c_int, _c_int, _vp,
         _c_i64, _c_int, _c_dbl, _c_int, _vp, _vp, _vp, _vp, _c_size, _vp]),

@@ -108,6 +108,38 @@ int check_common(int dtype, const void* x, const void* y, int d, const int32_t* 
   return 0;
 }
 
+// Scratch of the allocation-free entry point
+// (ksig_signature_levels_batched_noalloc): the row stream's global band FIFO.
+// Each GPU the library runs on holds one copy, at its own address, until the
+// process ends.  Host code never passes the array's name to a kernel (on the
+// host it names a stand-in, not device memory): it uses the device address
+// that hipGetSymbolAddress returns, looked up once per device.
+static __device__ __align__(256) unsigned char scratch[KSIG_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+void* g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+// Device address of this GPU's scratch (the current device's), looked up on
+// the first call on that device.
+int scratch_address(void** ptr) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(KSIG_EINVAL, "device %d beyond the %d the scratch table holds", dev, kMaxDevices);
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    if (!p) return fail(KSIG_EINVAL, "hipGetSymbolAddress(scratch) returned null");
+    g_scratch_ptr[dev] = p;
+  }
+  *ptr = g_scratch_ptr[dev];
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -327,14 +359,15 @@ int ksig_stream_slots(int d, int static_kind, int n_levels, int order, int max_l
   return 0;
 }
 
-int ksig_signature_levels_batched(const void* x, const int64_t* x_off, const int32_t* x_len,
-                                  int max_len_x, const void* y, const int64_t* y_off,
-                                  const int32_t* y_len, int max_len_y, int d,
-                                  const int32_t* pairs, int64_t n_pairs, int static_kind,
-                                  double static_param, int n_levels, int order,
-                                  const int32_t* sched_order, const int32_t* wave_beg,
-                                  int n_waves, int max_slice, void* workspace,
-                                  size_t workspace_bytes, double* out, void* stream) {
+// The batched row stream; noalloc: the band FIFO goes to the library's static
+// scratch array (workspace is its device address, workspace_bytes its size).
+static int batched_common(const void* x, const int64_t* x_off, const int32_t* x_len,
+                          int max_len_x, const void* y, const int64_t* y_off,
+                          const int32_t* y_len, int max_len_y, int d, const int32_t* pairs,
+                          int64_t n_pairs, int static_kind, double static_param, int n_levels,
+                          int order, const int32_t* sched_order, const int32_t* wave_beg,
+                          int n_waves, int max_slice, void* workspace, size_t workspace_bytes,
+                          double* out, void* stream, bool noalloc) {
   int rc = check_common(KSIG_F32, x, y, d, pairs, n_pairs, out);
   if (rc) return rc;
   if (n_levels < 1) return fail(KSIG_EINVAL, "n_levels must be positive");
@@ -375,6 +408,11 @@ int ksig_signature_levels_batched(const void* x, const int64_t* x_off, const int
   if (gf) {
     const size_t need =
         ksig::stream_gfifo_bytes(pl.pc, ksig::dp_flow_count(n_levels, order), waves);
+    if (noalloc && need > workspace_bytes)
+      return fail(KSIG_EINVAL, "the band FIFO of %lld waves x %d columns needs %zu bytes, more "
+                  "than the %zu-byte scratch of the allocation-free entry (cut fewer slices, or "
+                  "pass a workspace to ksig_signature_levels_batched)", (long long)waves, pl.pc,
+                  need, workspace_bytes);
     if (workspace_bytes < need || ((uintptr_t)workspace & 15) != 0)
       return fail(KSIG_EINVAL, "workspace of %zu bytes (16-byte aligned) needed, got %zu", need,
                   workspace_bytes);
@@ -382,6 +420,39 @@ int ksig_signature_levels_batched(const void* x, const int64_t* x_off, const int
   return stream_launch(pl, f, static_param, sched_order, wave_beg, (int)waves,
                        wave_beg ? 0 : max_slice, workspace, stream);
 }
+
+int ksig_signature_levels_batched(const void* x, const int64_t* x_off, const int32_t* x_len,
+                                  int max_len_x, const void* y, const int64_t* y_off,
+                                  const int32_t* y_len, int max_len_y, int d,
+                                  const int32_t* pairs, int64_t n_pairs, int static_kind,
+                                  double static_param, int n_levels, int order,
+                                  const int32_t* sched_order, const int32_t* wave_beg,
+                                  int n_waves, int max_slice, void* workspace,
+                                  size_t workspace_bytes, double* out, void* stream) {
+  return batched_common(x, x_off, x_len, max_len_x, y, y_off, y_len, max_len_y, d, pairs,
+                        n_pairs, static_kind, static_param, n_levels, order, sched_order,
+                        wave_beg, n_waves, max_slice, workspace, workspace_bytes, out, stream,
+                        false);
+}
+
+int ksig_signature_levels_batched_noalloc(const void* x, const int64_t* x_off,
+                                          const int32_t* x_len, int max_len_x, const void* y,
+                                          const int64_t* y_off, const int32_t* y_len,
+                                          int max_len_y, int d, const int32_t* pairs,
+                                          int64_t n_pairs, int static_kind, double static_param,
+                                          int n_levels, int order, const int32_t* sched_order,
+                                          const int32_t* wave_beg, int n_waves, int max_slice,
+                                          double* out, void* stream) {
+  void* fifo = nullptr;
+  const int rc = scratch_address(&fifo);
+  if (rc) return rc;
+  return batched_common(x, x_off, x_len, max_len_x, y, y_off, y_len, max_len_y, d, pairs,
+                        n_pairs, static_kind, static_param, n_levels, order, sched_order,
+                        wave_beg, n_waves, max_slice, fifo, KSIG_NOALLOC_SCRATCH_BYTES, out,
+                        stream, true);
+}
+
+size_t ksig_noalloc_scratch_bytes(void) { return KSIG_NOALLOC_SCRATCH_BYTES; }
 
 size_t ksig_signature_levels_fwd_save_workspace(int dtype, int d, int n_levels,
                                                 int max_len_x, int max_len_y,
