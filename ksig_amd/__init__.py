@@ -6,6 +6,9 @@ Drop-in for ``ksig.kernels.SignatureKernel`` / ``SignaturePDEKernel`` with
 
     import ksig_amd as ksig
     K = ksig.kernels.SignatureKernel(5, static_kernel=ksig.static.kernels.RBFKernel())(X)
+
+The package ``ksig`` next to this one registers these modules under the
+reference's own import name (``import ksig``, ``from ksig.models import ...``).
 """
 
 from . import utils
@@ -15,5 +18,6 @@ from . import kernels
 from . import tune
 from . import models
 from . import projections
+from . import preprocessing
 
 __version__ = "0.1.0"

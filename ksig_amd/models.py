@@ -25,7 +25,7 @@ import numpy as np
 import torch
 from sklearn.base import BaseEstimator, ClassifierMixin
 from sklearn.model_selection import GridSearchCV, StratifiedKFold
-from sklearn.svm import SVC
+from sklearn.svm import SVC, LinearSVC
 from sklearn.utils.validation import check_is_fitted
 
 from .static.kernels import Kernel
@@ -34,6 +34,12 @@ from .static.kernels import Kernel
 _DEFAULT_SVC_HPARAMS = {
   'decision_function_shape': 'ovo',
   'max_iter': 10000,
+}
+# ksig/models/pre_lin_svc.py:13-17
+_DEFAULT_LIN_SVC_HPARAMS = {
+  'dual': False,
+  'fit_intercept': False,
+  'tol': 1e-3,
 }
 
 
@@ -178,3 +184,47 @@ class PrecomputedKernelSVC(BaseEstimator, ClassifierMixin):
     check_is_fitted(self.model)
     y_pred = self.model.predict(self._precompute_model_inputs(X))
     return np.asarray(y_pred)
+
+
+class PrecomputedFeatureLinSVC(PrecomputedKernelSVC):
+  """``LinearSVC`` on a precomputed feature matrix with optional
+  ``GridSearchCV`` (ksig/models/pre_lin_svc.py:24-103 over
+  ksig/models/pre_base.py:17-282), for feature-map kernels such as
+  ``SignatureFeatures`` or ``RandomWarpingSeries``.
+
+  The features are computed on the GPU by this library (in blocks of
+  ``batch_size`` sequences when set) and handed to scikit-learn's
+  ``LinearSVC`` on the host.  ``on_gpu=True`` selects cuML's GPU LinearSVC in
+  the reference (ksig/models/pre_lin_svc.py:62-68), a CUDA library with no
+  ROCm build: here it is accepted for drop-in use and the model still runs
+  on the host, with the reference's CPU defaults."""
+
+  def __init__(self,
+               kernel: Kernel,
+               svc_hparams: Dict = {},
+               svc_grid: Optional[Dict] = None,
+               cv: int = 5,
+               n_jobs: int = -1,
+               need_kernel_fit: bool = False,
+               batch_size: Optional[int] = None,
+               on_gpu: bool = False):
+    self.kernel = kernel
+    self.svc_hparams = svc_hparams
+    self.svc_grid = svc_grid
+    self.cv = cv
+    self.n_jobs = 1 if on_gpu else n_jobs  # ksig/models/pre_lin_svc.py:49
+    self.need_kernel_fit = need_kernel_fit
+    self.batch_size = batch_size
+    self.has_transform = True
+    self.on_gpu = on_gpu
+    for key, val in _DEFAULT_LIN_SVC_HPARAMS.items():
+      self.svc_hparams.setdefault(key, val)
+
+  def _get_svc_model(self) -> LinearSVC:
+    return LinearSVC(**self.svc_hparams)
+
+  def _precompute_model_inputs(self, X=None) -> np.ndarray:
+    """ksig/models/pre_lin_svc.py:70-103: the feature matrix of the training
+    sequences (X None) or of X, on the host."""
+    F = self._precompute_feature_mat(self.X if X is None else X)
+    return F.detach().cpu().numpy()

@@ -1,9 +1,11 @@
-"""Static (state-space) kernels: the ksig.static.kernels counterparts used on
-the signature-kernel path (ksig/static/kernels.py:14-166, 210-260).
+"""Static (state-space) kernels: the ksig.static.kernels counterparts
+(ksig/static/kernels.py:14-343).
 
 ``RBFKernel`` and ``LinearKernel`` are recognised by the sequence kernels and
 fused into the HIP sweep (their Gram is never materialised there); called on
-their own they evaluate the Gram with GPU tensor ops.
+their own they evaluate the Gram with GPU tensor ops.  The polynomial, Matern
+and rational quadratic kernels are evaluated with GPU tensor ops and reach the
+signature kernels as a caller-supplied static-kernel tensor.
 """
 
 from __future__ import annotations
@@ -92,3 +94,64 @@ class RBFKernel(StationaryKernel):
   def _K(self, X, Y=None):
     D2_scaled = utils.squared_euclid_dist(X, Y) / max(2 * self.bandwidth**2, _EPS)
     return torch.exp(-D2_scaled)
+
+
+class PolynomialKernel(StaticKernel):
+  """k(x, y) = scale * (<x, y> + gamma)^degree (ksig/static/kernels.py:169-211)."""
+
+  def __init__(self, degree: float = 3., gamma: float = 1., scale: float = 1.):
+    self.degree = utils.check_positive_value(degree, 'degree')
+    self.gamma = gamma
+    self.scale = utils.check_positive_value(scale, 'scale')
+
+  def _K(self, X, Y=None):
+    return self.scale * torch.pow(utils.matrix_mult(X, Y, transpose_Y=True) + self.gamma,
+                                  self.degree)
+
+  def _Kdiag(self, X):
+    return self.scale * torch.pow(utils.squared_norm(X, axis=-1) + self.gamma, self.degree)
+
+
+class Matern12Kernel(StationaryKernel):
+  """k(x, y) = exp(-|x - y| / max(sigma, eps)) (ksig/static/kernels.py:263-277).
+
+  The reference computes |x - y| through ``utils.euclid_dist``, which is
+  declared with a stray ``self`` parameter (ksig/utils.py:185), so its call
+  ``euclid_dist(X, Y)`` evaluates the distances of Y with itself (and fails
+  for Y = None); this class evaluates the intended distances of X and Y."""
+
+  def _K(self, X, Y=None):
+    return torch.exp(-utils.euclid_dist(X, Y) / max(self.bandwidth, _EPS))
+
+
+class Matern32Kernel(StationaryKernel):
+  """k = (1 + r) exp(-r), r = sqrt(3) |x - y| / max(sigma, eps)
+  (ksig/static/kernels.py:280-296; distances of X and Y as in
+  ``Matern12Kernel``)."""
+
+  def _K(self, X, Y=None):
+    D = 3.0 ** 0.5 * utils.euclid_dist(X, Y) / max(self.bandwidth, _EPS)
+    return (1. + D) * torch.exp(-D)
+
+
+class Matern52Kernel(StationaryKernel):
+  """k = (1 + r + r^2 / 3) exp(-r), r^2 = 5 |x - y|^2 / max(sigma^2, eps)
+  with r the robust square root (ksig/static/kernels.py:299-315)."""
+
+  def _K(self, X, Y=None):
+    D2 = 5 * utils.squared_euclid_dist(X, Y) / max(self.bandwidth**2, _EPS)
+    D = utils.robust_sqrt(D2)
+    return (1. + D + D2 / 3.) * torch.exp(-D)
+
+
+class RationalQuadraticKernel(StationaryKernel):
+  """k = (1 + |x - y|^2 / max(2 alpha sigma^2, eps))^(-alpha)
+  (ksig/static/kernels.py:318-343)."""
+
+  def __init__(self, bandwidth: float = 1., alpha: float = 1.):
+    super().__init__(bandwidth=bandwidth)
+    self.alpha = utils.check_positive_value(alpha, 'alpha')
+
+  def _K(self, X, Y=None):
+    D2 = utils.squared_euclid_dist(X, Y) / max(2 * self.alpha * self.bandwidth**2, _EPS)
+    return torch.pow(1 + D2, -self.alpha)

@@ -6,8 +6,8 @@ Arrays are torch tensors on the GPU; host numpy arrays are moved there by
 
 from __future__ import annotations
 
-from numbers import Number
-from typing import Optional
+from numbers import Integral, Number
+from typing import Optional, Sequence, Union
 
 import numpy as np
 import torch
@@ -16,6 +16,9 @@ _EPS = 1e-12  # ksig/utils.py:16
 
 ArrayOnCPU = np.ndarray
 ArrayOnGPU = torch.Tensor
+ArrayOnCPUOrGPU = Union[np.ndarray, torch.Tensor]
+RandomState = np.random.RandomState
+RandomStateOrSeed = Union[Integral, np.random.RandomState]
 
 
 def check_positive_value(scalar: Number, name: str) -> Number:
@@ -90,3 +93,43 @@ def robust_nonzero(X: torch.Tensor) -> torch.Tensor:
 def robust_sqrt(X: torch.Tensor) -> torch.Tensor:
   """ksig/utils.py:173-182."""
   return torch.sqrt(torch.clamp(X, min=_EPS))
+
+
+def euclid_dist(X: torch.Tensor, Y: Optional[torch.Tensor] = None) -> torch.Tensor:
+  """Pairwise Euclidean distances, the robust square root of
+  ``squared_euclid_dist`` (ksig/utils.py:185-196, without its stray ``self``
+  parameter)."""
+  return robust_sqrt(squared_euclid_dist(X, Y))
+
+
+def outer_prod(X: torch.Tensor, Y: torch.Tensor) -> torch.Tensor:
+  """ksig/utils.py:160-170: batched outer product flattened to [..., d1*d2]."""
+  return (X[..., :, None] * Y[..., None, :]).reshape(X.shape[:-1] + (-1,))
+
+
+def multi_cumsum(M: torch.Tensor, exclusive: bool = False, axis=-1) -> torch.Tensor:
+  """ksig/utils.py:63-90: cumulative sum over one or several axes;
+  ``exclusive`` drops the last entry along each axis and pre-pads a zero."""
+  ndim = M.dim()
+  axes = [axis] if isinstance(axis, int) else list(axis)
+  axes = [a % ndim for a in axes]
+  if exclusive:
+    M = M[tuple(slice(0, -1) if a in axes else slice(None) for a in range(ndim))]
+  for a in axes:
+    M = torch.cumsum(M, a)
+  if exclusive:
+    pad = []
+    for a in reversed(range(ndim)):
+      pad += [1, 0] if a in axes else [0, 0]
+    M = torch.nn.functional.pad(M, pad)
+  return M
+
+
+def check_random_state(random_state=None) -> np.random.RandomState:
+  """ksig/utils.py:38-60 with a NumPy RandomState in place of CuPy's: None
+  or an integer seeds a new one, a RandomState is returned as it is."""
+  if random_state is None or isinstance(random_state, Integral):
+    return np.random.RandomState(random_state)
+  if isinstance(random_state, np.random.RandomState):
+    return random_state
+  raise ValueError(f'{random_state} cannot seed a `numpy.random.RandomState` instance')
