@@ -271,6 +271,34 @@ int ksig_signature_levels_gram(int dtype, const void* M, int64_t n_mats,
                                int difference, double* out, void* stream);
 
 /*
+ * Truncated signature kernel from static-kernel dot products computed by a
+ * library GEMM (matrix cores, K = d): the route for inputs the fused kernels
+ * do not take (d > 32, or an (n_levels, order) outside the compiled table,
+ * see ksig_dp_supported).  Replaces, per matrix, the composition of
+ * SignatureKernel._compute_kernel (ksig/kernels.py:147-164): the static Gram
+ * of ksig/static/kernels.py:248-260 (RBF) or :145-155 (linear), whose inner
+ * products ksig/utils.py:105-157 forms with a GEMM, followed by
+ * signature_kern(M, n_levels, order, difference, return_levels=True) of
+ * ksig/algorithms.py:22-140.
+ *
+ * Matrix p holds the values v(i, j) = dots[mat_off[p] + i * ld + j] for
+ * i < lx_p, j < ly_p, with (lx_p, ly_p) = (lens[2p], lens[2p+1]) or, lens
+ * NULL, (lx, ly); max_ly bounds every ly_p (it sizes LDS).  The static-kernel
+ * value is exp2(v - hx[hx_off[p] + i] - hy[hy_off[p] + j]) for
+ * KSIG_STATIC_RBF (v = 2h<x_i, y_j>, hx = h|x_i|^2, hy = h|y_j|^2,
+ * h = log2(e) / max(2 sigma^2, 1e-12)) and v itself for KSIG_STATIC_LINEAR
+ * (v = scale <x_i, y_j>).  out[p, m] = K_m, float64.  Orders outside the
+ * compiled table run a kernel whose level count and order are runtime values
+ * (KSIG_ELDS if its per-lane state does not fit).
+ */
+int ksig_signature_levels_dots(int dtype, const void* dots, const int64_t* mat_off,
+                               int64_t ld, const int32_t* lens, int lx, int ly, int max_ly,
+                               int static_kind, const void* hx, const int64_t* hx_off,
+                               const void* hy, const int64_t* hy_off, int64_t n_mats,
+                               int n_levels, int order, int difference, double* out,
+                               void* stream);
+
+/*
  * Signature-PDE kernel for a list of sequence pairs.
  *
  * Replaces, per pair, SignaturePDEKernel._compute_kernel

@@ -59,6 +59,9 @@ SIGNATURES = {
         _vp, _vp, _c_dbl, _c_dbl, _vp, _vp, _vp]),
     "ksig_signature_levels_gram": (_c_int, [
         _c_int, _vp, _c_i64, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp]),
+    "ksig_signature_levels_dots": (_c_int, [
+        _c_int, _vp, _vp, _c_i64, _vp, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp,
+        _c_i64, _c_int, _c_int, _c_int, _vp, _vp]),
     "ksig_signature_pde": (_c_int, [
         _c_int, _vp, _vp, _vp, _c_int, _vp, _vp, _vp, _c_int, _c_int, _vp,
         _c_i64, _c_int, _c_dbl, _c_int, _c_int, _vp, _vp]),

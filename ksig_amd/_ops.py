@@ -108,13 +108,198 @@ def _check_sets(X: SeqSet, Y: SeqSet):
     raise ValueError("X and Y must live on the same device")
 
 
+# Largest d of the fused sweeps (features in registers / LDS per lane).
+FUSED_MAX_D = 32
+# Dot products per GEMM chunk of the GEMM route (bytes).
+GEMM_CHUNK_BYTES = 1 << 31
+
+
+def _norm_order(n_levels: int, order: int) -> int:
+  return n_levels if (order <= 0 or order >= n_levels) else order
+
+
+def fused_dp_ok(d: int, n_levels: int, order: int) -> bool:
+  """True when the fused sweeps take this shape: d <= 32 and (n_levels,
+  order) in the compiled table; otherwise the GEMM route runs."""
+  lib = _lib.load()
+  return d <= FUSED_MAX_D and bool(lib.ksig_dp_supported(n_levels, _norm_order(n_levels, order)))
+
+
+def _static_factors(kind: int, param: float):
+  """(scale folded into the row operand, norm factor h or None) of the
+  static kernel: RBF exp2(2h<x,y> - h|x|^2 - h|y|^2), h = log2(e) /
+  max(2 sigma^2, 1e-12) (ksig/static/kernels.py:258-260); linear scale<x,y>."""
+  if kind == _lib.STATIC_RBF:
+    if not param > 0:
+      raise ValueError(f"static kernel parameter must be positive (got {param})")
+    h = 1.4426950408889634 / max(2.0 * param * param, 1e-12)
+    return 2.0 * h, h
+  if kind == _lib.STATIC_LINEAR:
+    if not param > 0:
+      raise ValueError(f"static kernel parameter must be positive (got {param})")
+    return float(param), None
+  raise ValueError(f"unknown static kernel kind {kind}")
+
+
+def signature_levels_dots(dots: torch.Tensor, mat_off: torch.Tensor, ld: int,
+                          lens: Optional[torch.Tensor], lx: int, ly: int, max_ly: int,
+                          kind: int, hx: Optional[torch.Tensor], hx_off: Optional[torch.Tensor],
+                          hy: Optional[torch.Tensor], hy_off: Optional[torch.Tensor],
+                          n_levels: int, order: int, difference: bool) -> torch.Tensor:
+  """Level values [n_mats, n_levels+1] (float64) of ksig_signature_levels_dots:
+  matrix p at element offset mat_off[p] of ``dots`` with row stride ``ld``,
+  static-kernel values formed on read (RBF: exp2(v - hx - hy))."""
+  lib = _lib.load()
+  n = mat_off.shape[0]
+  out = torch.empty((n, n_levels + 1), device=dots.device, dtype=torch.float64)
+  if n == 0:
+    return out
+  ptr = lambda t: t.data_ptr() if t is not None else None
+  _lib.check(lib.ksig_signature_levels_dots(
+      _lib.dtype_code(dots.dtype), dots.data_ptr(), mat_off.data_ptr(), int(ld), ptr(lens),
+      int(lx), int(ly), int(max_ly), int(kind), ptr(hx), ptr(hx_off), ptr(hy), ptr(hy_off), n,
+      n_levels, order, int(bool(difference)), out.data_ptr(), _lib.stream_handle(dots.device)))
+  return out
+
+
+def signature_levels_gemm(X: SeqSet, Y: SeqSet, pairs: torch.Tensor, n_levels: int,
+                          order: int, difference: bool, kind: int, param: float
+                          ) -> torch.Tensor:
+  """Level values [n_pairs, n_levels+1] (float64) with the static kernel's
+  inner products from library GEMMs on the matrix cores (K = d) and the
+  recursion in the HIP kernels of ksig_signature_levels_dots: any d, any
+  (n_levels, order).  Dense sets take one GEMM per block of x sequences
+  against the y range its pairs need, read in place; other pair lists gather
+  their sequences and take a batched GEMM per chunk.  The pair orientation
+  puts the longer sequences on the rows (the levels are symmetric in x and
+  y), so the LDS column state holds the shorter side."""
+  _check_sets(X, Y)
+  n_pairs = pairs.shape[0]
+  dev = X.data.device
+  out = torch.empty((n_pairs, n_levels + 1), device=dev, dtype=torch.float64)
+  if n_pairs == 0:
+    return out
+  if Y.max_len > X.max_len:  # rows = the longer side
+    return signature_levels_gemm(Y, X, pairs.flip(-1).contiguous(), n_levels, order,
+                                 difference, kind, param)
+  gdot, h = _static_factors(kind, param)
+  Xd = X.data * gdot
+  hx = (h * (X.data * X.data).sum(-1)).contiguous() if h is not None else None
+  hy = (h * (Y.data * Y.data).sum(-1)).contiguous() if h is not None else None
+  esz = X.data.element_size()
+  pr = pairs.detach().cpu().numpy().astype(np.int64)
+  xoff, yoff = X.off.detach().cpu().numpy(), Y.off.detach().cpu().numpy()
+  lxs, lys = X.lengths()[pr[:, 0]], Y.lengths()[pr[:, 1]]
+
+  def run(dots, idx, mat_off, ld, lens, lx, ly, max_ly):
+    t = lambda a, dt=torch.int64: torch.as_tensor(np.ascontiguousarray(a), device=dev, dtype=dt)
+    hxo = t(xoff[pr[idx, 0]]) if h is not None else None
+    hyo = t(yoff[pr[idx, 1]]) if h is not None else None
+    lv = signature_levels_dots(dots, t(mat_off), ld, None if lens is None else t(lens,
+                               torch.int32), lx, ly, max_ly, kind, hx, hxo, hy, hyo, n_levels,
+                               order, difference)
+    out[torch.as_tensor(idx, device=dev)] = lv
+
+  done = np.zeros(n_pairs, dtype=bool)
+  if X.uniform and Y.uniform:
+    # block GEMMs: x sequences [a0, a1) against y sequences [b0, b1]
+    Lx, Ly = X.max_len, Y.max_len
+    by_a = np.argsort(pr[:, 0], kind="stable")
+    a_sorted = pr[by_a, 0]
+    a0 = 0
+    while a0 < X.n:
+      lo = np.searchsorted(a_sorted, a0)
+      if lo >= n_pairs:
+        break
+      a0 = int(a_sorted[lo])
+      # grow the block while the dot products stay within the chunk
+      nb = 1
+      while True:
+        hi = np.searchsorted(a_sorted, a0 + 2 * nb)
+        bsel = pr[by_a[lo:hi], 1]
+        span = int(bsel.max() - bsel.min() + 1)
+        if a0 + 2 * nb > X.n or 2 * nb * Lx * span * Ly * esz > GEMM_CHUNK_BYTES:
+          break
+        nb *= 2
+      hi = np.searchsorted(a_sorted, a0 + nb)
+      idx = by_a[lo:hi]
+      b = pr[idx, 1]
+      b0, b1 = int(b.min()), int(b.max())
+      span = b1 - b0 + 1
+      if len(idx) * 4 >= nb * span:  # the block GEMM's pairs are mostly needed
+        A = Xd[int(xoff[a0]):int(xoff[a0]) + nb * Lx]
+        B = Y.data[int(yoff[b0]):int(yoff[b0]) + span * Ly]
+        dots = A @ B.T
+        ld = span * Ly
+        mat_off = (pr[idx, 0] - a0) * Lx * ld + (b - b0) * Ly
+        run(dots, idx, mat_off, ld, None, Lx, Ly, Ly)
+        done[idx] = True
+      a0 += nb
+  rest = np.nonzero(~done)[0]
+  # gathered pairs: batched GEMMs over chunks of pairs, padded to the chunk's
+  # longest sides (the kernel reads each pair within its own lengths)
+  pos = 0
+  d = X.d
+  while pos < len(rest):
+    Lxm, Lym = int(lxs[rest[pos]]), int(lys[rest[pos]])
+    end = pos + 1
+    while end < len(rest):
+      nx, ny = max(Lxm, int(lxs[rest[end]])), max(Lym, int(lys[rest[end]]))
+      if (end - pos + 1) * (nx * ny + (nx + ny) * d) * esz > GEMM_CHUNK_BYTES:
+        break
+      Lxm, Lym = nx, ny
+      end += 1
+    idx = rest[pos:end]
+    ar = np.arange(max(Lxm, Lym))
+    ix = xoff[pr[idx, 0]][:, None] + np.minimum(ar[None, :Lxm], lxs[idx][:, None] - 1)
+    iy = yoff[pr[idx, 1]][:, None] + np.minimum(ar[None, :Lym], lys[idx][:, None] - 1)
+    Xp = Xd[torch.as_tensor(ix, device=dev)]
+    Yp = Y.data[torch.as_tensor(iy, device=dev)]
+    dots = torch.bmm(Xp, Yp.transpose(1, 2).contiguous())
+    mat_off = np.arange(len(idx), dtype=np.int64) * Lxm * Lym
+    lens = np.stack([lxs[idx], lys[idx]], -1).astype(np.int32)
+    run(dots, idx, mat_off, Lym, lens, 0, 0, int(lys[idx].max()))
+    pos = end
+  return out
+
+
+def static_gram_chunks(X: SeqSet, Y: SeqSet, pairs: torch.Tensor, kind: int, param: float):
+  """Static-kernel matrices of a pair list over uniform-length sets, in
+  chunks of pairs: yields (pair indices, M [chunk, Lx, Ly]) with M = k(x_i,
+  y_j) formed from a batched library GEMM (the inner products of
+  ksig/utils.py:105-157) and one elementwise pass (RBF exp2(2h<x,y> -
+  h|x|^2 - h|y|^2), ksig/static/kernels.py:248-260; linear scale<x,y>).  The
+  route of the sweeps over caller matrices for d beyond the fused kernels."""
+  if not (X.uniform and Y.uniform):
+    raise NotImplementedError(
+        f"d = {X.d} > {FUSED_MAX_D} is taken for sets of one sequence length only")
+  gdot, h = _static_factors(kind, param)
+  Lx, Ly, d = X.max_len, Y.max_len, X.d
+  X3 = X.data.view(X.n, Lx, d)
+  Y3 = Y.data.view(Y.n, Ly, d)
+  esz = X.data.element_size()
+  chunk = max(1, GEMM_CHUNK_BYTES // ((Lx * Ly + (Lx + Ly) * d) * esz))
+  pl = pairs.long()
+  for p0 in range(0, pairs.shape[0], chunk):
+    pa, pb = pl[p0:p0 + chunk, 0], pl[p0:p0 + chunk, 1]
+    Xp, Yp = X3[pa], Y3[pb]
+    dots = torch.bmm(Xp * gdot, Yp.transpose(1, 2).contiguous())
+    if h is not None:
+      dots = torch.exp2(dots - h * (Xp * Xp).sum(-1)[:, :, None]
+                        - h * (Yp * Yp).sum(-1)[:, None, :])
+    yield p0, dots
+
+
 def signature_levels(X: SeqSet, Y: SeqSet, pairs: torch.Tensor, n_levels: int,
                      order: int, difference: bool, kind: int, param: float,
                      gram_mode: int = -1) -> torch.Tensor:
-  """Level values [n_pairs, n_levels+1] (float64) of the fused DP kernel."""
+  """Level values [n_pairs, n_levels+1] (float64) of the fused DP kernel, or
+  of the GEMM route for shapes the fused kernels do not take."""
   lib = _lib.load()
   _check_sets(X, Y)
   n_pairs = pairs.shape[0]
+  if kind in (_lib.STATIC_RBF, _lib.STATIC_LINEAR) and not fused_dp_ok(X.d, n_levels, order):
+    return signature_levels_gemm(X, Y, pairs, n_levels, order, difference, kind, param)
   if (n_pairs > 0 and X.data.dtype == torch.float32 and difference and gram_mode in (-1, 0)
       and kind in (_lib.STATIC_RBF, _lib.STATIC_LINEAR) and not (X.uniform and Y.uniform)):
     # uneven lengths: length-bucketed schedule for the row stream
@@ -302,6 +487,10 @@ def signature_pde(X: SeqSet, Y: SeqSet, pairs: torch.Tensor, difference: bool,
   out = torch.empty((n_pairs,), device=X.data.device, dtype=torch.float64)
   if n_pairs == 0:
     return out
+  if X.d > FUSED_MAX_D:
+    for p0, M in static_gram_chunks(X, Y, pairs, kind, param):
+      out[p0:p0 + M.shape[0]] = signature_pde_gram(M, difference, dyadic_order)
+    return out
   _lib.check(lib.ksig_signature_pde(
       _lib.dtype_code(X.data.dtype), X.data.data_ptr(), X.off.data_ptr(),
       X.len.data_ptr(), X.max_len, Y.data.data_ptr(), Y.off.data_ptr(),
@@ -334,6 +523,10 @@ def global_align_log(X: SeqSet, Y: SeqSet, pairs: torch.Tensor, kind: int,
   n_pairs = pairs.shape[0]
   out = torch.empty((n_pairs,), device=X.data.device, dtype=torch.float64)
   if n_pairs == 0:
+    return out
+  if X.d > FUSED_MAX_D:
+    for p0, M in static_gram_chunks(X, Y, pairs, kind, param):
+      out[p0:p0 + M.shape[0]] = global_align_log_gram(M)
     return out
   _lib.check(lib.ksig_global_align_log(
       _lib.dtype_code(X.data.dtype), X.data.data_ptr(), X.off.data_ptr(),
@@ -390,6 +583,15 @@ def random_warping_series(X: SeqSet, warp: torch.Tensor, warp_lens: np.ndarray
   out = torch.empty((X.n, n_w), device=dev, dtype=torch.float64)
   if X.n == 0 or n_w == 0:
     return out
+  if X.d > FUSED_MAX_D:
+    # distances from a library GEMM (ksig/kernels.py:466), then the sweep
+    if not X.uniform:
+      raise NotImplementedError(
+          f"d = {X.d} > {FUSED_MAX_D} is taken for sets of one sequence length only")
+    Xf = X.data
+    w = warp.contiguous()
+    D = ((Xf * Xf).sum(-1)[:, None] + (w * w).sum(-1)[None, :] - 2 * Xf @ w.T)
+    return random_warping_series_gram(D.view(X.n, X.max_len, -1), lens)
   off = torch.as_tensor(np.concatenate([[0], np.cumsum(lens)]).astype(np.int32), device=dev)
   groups = rws_groups(lens, X.n)
   g = torch.as_tensor(groups, device=dev)

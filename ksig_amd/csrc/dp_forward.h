@@ -33,6 +33,20 @@ struct DpFwdArgs {
   int n_levels, order, difference;
   const void* gram;      // kind == kGram: [n_pairs, gram_lx, gram_ly]
   int gram_lx, gram_ly;
+  // kind == kGram, general layout: matrix p at element offset gram_off[p]
+  // (null: p * gram_lx * gram_ly), rows gram_ld elements apart (0: gram_ly),
+  // per-pair sizes gram_len[2p], gram_len[2p+1] (null: gram_lx, gram_ly), and
+  // the static-kernel value formed from the stored value v as
+  //   gram_tf == kGramRbfDots: exp2(v - hx[hx_off[p] + i] - hy[hy_off[p] + j])
+  //   gram_tf == kGramPlain:    v
+  const int64_t* gram_off;
+  int64_t gram_ld;
+  const int32_t* gram_len;
+  int gram_tf;
+  const void* gram_hx;
+  const int64_t* gram_hx_off;
+  const void* gram_hy;
+  const int64_t* gram_hy_off;
   double* out;           // [n_pairs, n_levels + 1]
   // Optional boundary state for the adjoint (order 1, difference only):
   // per pair at save_off[p], [n_levels-1][Nr] row-end prefixes P_a(i, Nc)
@@ -154,6 +168,10 @@ int dp_flow_count(int nlev, int order);
 size_t dp_sweep_lds(int dtype, int d, int cols, int n_levels, int order);
 hipError_t dp_sweep_launch(const DpFwdArgs& a, int dtype, size_t lds, hipStream_t s);
 bool dp_supported(int n_levels, int order);
+// any (n_levels, order) on caller matrices (dp_generic.hip): LDS bytes for
+// matrices of at most max_cols columns, and the launch
+size_t dp_generic_lds(int dtype, int n_levels, int order, int max_cols);
+hipError_t dp_generic_launch(const DpFwdArgs& a, int dtype, size_t lds, hipStream_t s);
 hipError_t dp_forward_launch(const DpFwdArgs& a, int dtype, size_t lds, hipStream_t s);
 size_t dp_backward_lds(int dtype, int d, int cols, int n_levels);
 hipError_t dp_backward_launch(const DpBwdArgs& a, int dtype, size_t lds, hipStream_t s);

@@ -45,6 +45,18 @@ __host__ __device__ constexpr int dp_side_off(int m, int order) {
   return n;
 }
 __host__ __device__ constexpr int cmin(int a, int b) { return a < b ? a : b; }
+// Column-flowing values per lane: the static-kernel value, the level column
+// sums and the column side sums.
+__host__ __device__ constexpr int dp_flow_count_hd(int nlev, int order) {
+  return 1 + nlev + dp_num_side(nlev, order);
+}
+
+// RBF static-kernel value from a stored scaled dot product v = 2h<x, y> and
+// the norm terms hx = h|x|^2, hy = h|y|^2 (h = log2(e) / max(2 sigma^2, eps)).
+template <typename T>
+__device__ __forceinline__ T gram_rbf(T v, T hx, T hy) {
+  return fast_exp2(v - hx - hy);
+}
 
 // One cell of the level recursion.  `in` holds the column-flowing inputs
 // (index 1..NLEV: column sums of S_m above this cell; 1+NLEV+..: column sums
@@ -127,11 +139,20 @@ dp_forward_kernel(const DpFwdArgs a) {
   const T* xs = nullptr;
   const T* ys = nullptr;
   const T* gm = nullptr;  // caller-supplied Gram / M for this pair
+  const T* ghx = nullptr;  // RBF-transformed values: row / column norm terms
+  const T* ghy = nullptr;
+  int64_t gld = 0;
   int Lx, Ly;
   if (GRAM) {
-    Lx = a.gram_lx;
-    Ly = a.gram_ly;
-    gm = reinterpret_cast<const T*>(a.gram) + pr * (int64_t)Lx * Ly;
+    Lx = a.gram_len ? a.gram_len[2 * pr] : a.gram_lx;
+    Ly = a.gram_len ? a.gram_len[2 * pr + 1] : a.gram_ly;
+    gm = reinterpret_cast<const T*>(a.gram) +
+         (a.gram_off ? a.gram_off[pr] : pr * (int64_t)a.gram_lx * a.gram_ly);
+    gld = a.gram_ld > 0 ? a.gram_ld : (int64_t)Ly;
+    if (a.gram_tf == kGramRbfDots) {
+      ghx = reinterpret_cast<const T*>(a.gram_hx) + a.gram_hx_off[pr];
+      ghy = reinterpret_cast<const T*>(a.gram_hy) + a.gram_hy_off[pr];
+    }
   } else {
     const int32_t ia = a.pairs[2 * pr];
     const int32_t ib = a.pairs[2 * pr + 1];
@@ -223,6 +244,7 @@ dp_forward_kernel(const DpFwdArgs a) {
       if (diff && s < Ly) {
         if (GRAM) {
           g = gm[s];
+          if (ghx) g = gram_rbf(g, ghx[0], ghy[s]);
         } else {
           T dot = T(0);
 #pragma unroll
@@ -256,7 +278,8 @@ dp_forward_kernel(const DpFwdArgs a) {
   auto gram_at = [&](int rr, int cc) -> T {
     const int xr = rr + rshift, yc = cc + rshift;
     if (xr < 0 || xr >= Lx || yc < 0 || yc >= Ly) return T(0);
-    return gm[(int64_t)xr * Ly + yc];
+    const T v = gm[(int64_t)xr * gld + yc];
+    return ghx ? gram_rbf(v, ghx[xr], ghy[yc]) : v;
   };
   T gpre = T(0);
   if (GRAM) gpre = gram_at(row, c);

@@ -170,6 +170,19 @@ static int dp_levels_common(int dtype, ksig::DpFwdArgs& a, int cols, int dmax,
   return 0;
 }
 
+// Caller-matrix recursion: the register-resident kernel for the compiled
+// (n_levels, order) table, the runtime-levels kernel for every other pair.
+static int gram_levels_dispatch(int dtype, ksig::DpFwdArgs& a, int cols, void* stream) {
+  if (ksig::dp_supported(a.n_levels, a.order)) return dp_levels_common(dtype, a, cols, 8, stream);
+  const size_t lds = ksig::dp_generic_lds(dtype, a.n_levels, a.order, cols);
+  if (lds > max_lds_bytes())
+    return fail(KSIG_ELDS, "(n_levels=%d, order=%d) with %d columns needs %zu bytes of LDS "
+                "state, above the device limit", a.n_levels, a.order, cols, lds);
+  hipError_t e = ksig::dp_generic_launch(a, dtype, lds, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "dp_generic launch");
+  return 0;
+}
+
 // Row-stream launch (float32, difference=1): one persistent pass, every wave a
 // contiguous slice of the pair list.  stream_levels returns -1 when no
 // compiled row-stream kernel or no LDS layout fits, so the caller can fall
@@ -676,8 +689,6 @@ int ksig_signature_levels_gram(int dtype, const void* M, int64_t n_mats, int lx,
   if (lx < 0 || ly < 0) return fail(KSIG_EINVAL, "negative matrix shape");
   if (n_levels < 1) return fail(KSIG_EINVAL, "n_levels must be positive");
   if (order <= 0 || order >= n_levels) order = n_levels;
-  if (!ksig::dp_supported(n_levels, order))
-    return fail(KSIG_EUNSUPPORTED, "(n_levels=%d, order=%d) is not built", n_levels, order);
   if (n_mats == 0) return 0;
   if (!M || !out) return fail(KSIG_EINVAL, "null device pointer");
   ksig::DpFwdArgs a{};
@@ -691,7 +702,49 @@ int ksig_signature_levels_gram(int dtype, const void* M, int64_t n_mats, int lx,
   a.gram_lx = lx;
   a.gram_ly = ly;
   a.out = out;
-  return dp_levels_common(dtype, a, ly < 1 ? 1 : ly, 8, stream);
+  return gram_levels_dispatch(dtype, a, ly < 1 ? 1 : ly, stream);
+}
+
+int ksig_signature_levels_dots(int dtype, const void* dots, const int64_t* mat_off,
+                               int64_t ld, const int32_t* lens, int lx, int ly, int max_ly,
+                               int static_kind, const void* hx, const int64_t* hx_off,
+                               const void* hy, const int64_t* hy_off, int64_t n_mats,
+                               int n_levels, int order, int difference, double* out,
+                               void* stream) {
+  if (dtype != KSIG_F32 && dtype != KSIG_F64) return fail(KSIG_EINVAL, "bad dtype %d", dtype);
+  if (n_mats < 0 || n_mats > 0x7fffffffLL) return fail(KSIG_EINVAL, "bad n_mats");
+  if (n_levels < 1) return fail(KSIG_EINVAL, "n_levels must be positive");
+  if (static_kind != ksig::kRBF && static_kind != ksig::kLinear)
+    return fail(KSIG_EINVAL, "unknown static kernel kind %d", static_kind);
+  if (order <= 0 || order >= n_levels) order = n_levels;
+  if (n_mats == 0) return 0;
+  if (!dots || !out || !mat_off) return fail(KSIG_EINVAL, "null device pointer");
+  if (static_kind == ksig::kRBF && (!hx || !hx_off || !hy || !hy_off))
+    return fail(KSIG_EINVAL, "the RBF transform needs both norm-term arrays and offsets");
+  if (!lens && (lx < 0 || ly < 0)) return fail(KSIG_EINVAL, "negative matrix shape");
+  const int cols = lens ? max_ly : ly;
+  if (cols < 0) return fail(KSIG_EINVAL, "negative maximum column count");
+  if (ld < (lens ? 0 : ly)) return fail(KSIG_EINVAL, "row stride below the column count");
+  ksig::DpFwdArgs a{};
+  a.kind = ksig::kGram;
+  a.d = 1;
+  a.n_pairs = n_mats;
+  a.n_levels = n_levels;
+  a.order = order;
+  a.difference = difference ? 1 : 0;
+  a.gram = dots;
+  a.gram_lx = lx;
+  a.gram_ly = ly;
+  a.gram_off = mat_off;
+  a.gram_ld = ld;
+  a.gram_len = lens;
+  a.gram_tf = static_kind == ksig::kRBF ? ksig::kGramRbfDots : ksig::kGramPlain;
+  a.gram_hx = hx;
+  a.gram_hx_off = hx_off;
+  a.gram_hy = hy;
+  a.gram_hy_off = hy_off;
+  a.out = out;
+  return gram_levels_dispatch(dtype, a, cols < 1 ? 1 : cols, stream);
 }
 
 static int pde_common(int dtype, ksig::PdeFwdArgs& a, int cols, int dmax, void* stream) {

@@ -18,6 +18,10 @@ namespace ksig {
 constexpr int kWave = 64;
 
 enum StaticKind : int { kRBF = 0, kLinear = 1, kGram = 2 };
+// How a caller-matrix kernel forms static-kernel values from the stored ones
+// (DpFwdArgs::gram_tf); zero, the value of a zero-initialised argument block,
+// is the plain matrix.
+enum GramTransform : int { kGramPlain = 0, kGramRbfDots = 1 };
 
 // ---------------------------------------------------------------------------
 // Lane shift: lane i receives lane i-1's value; lane 0 receives `fill`.

@@ -51,8 +51,13 @@ def test_host_only_entry_points():
   lib.ksig_last_error.restype = ctypes.c_char_p
   rc = lib.ksig_signature_levels_gram(7, None, 0, 1, 1, 5, 1, 1, None, None)
   assert rc == 1001 and b"dtype" in lib.ksig_last_error()
-  rc = lib.ksig_signature_levels_gram(0, None, 0, 4, 4, 40, 1, 1, None, None)
-  assert rc == 1002
+  # any (n_levels, order) runs on caller matrices; a state beyond the LDS of
+  # the runtime-levels kernel is refused before launching (dummy pointers
+  # are never dereferenced on the host)
+  assert lib.ksig_signature_levels_gram(0, None, 0, 4, 4, 40, 1, 1, None, None) == 0
+  rc = lib.ksig_signature_levels_gram(0, ctypes.c_void_p(16), 1, 4, 4, 40, 40, 1,
+                                      ctypes.c_void_p(16), None)
+  assert rc == 1003 and b"LDS" in lib.ksig_last_error()
 
 
 @pytest.mark.skipif(not os.path.exists(LIB), reason="libksig_hip.so not built")

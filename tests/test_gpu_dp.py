@@ -217,7 +217,20 @@ def test_symmetric_gram_matches_full_evaluation():
   np.testing.assert_allclose(Kxx, Kxx.T, rtol=0, atol=0)
 
 
-def test_unsupported_combination_raises():
-  X = np.zeros((2, 5, 3))
-  with pytest.raises(NotImplementedError):
-    ksig.kernels.SignatureKernel(40, order=1)(X)
+def test_untabulated_combination_routes_around_the_fused_entry():
+  """(40, 1) is outside the compiled table: the fused entry refuses it, the
+  class takes the GEMM route (runtime-levels kernel) and matches the oracle."""
+  from ksig_amd import _lib, _ops
+  rng = np.random.default_rng(40)
+  X = O.brownian(rng, 2, 5, 3)
+  Xs = _ops.SeqSet.dense(to_dev(X))
+  lib = _lib.load()
+  out = torch.empty((1, 41), device=dev(), dtype=torch.float64)
+  pairs = to_dev(np.array([[0, 1]], dtype=np.int32))
+  rc = lib.ksig_signature_levels(_lib.F64, Xs.data.data_ptr(), Xs.off.data_ptr(),
+                                 Xs.len.data_ptr(), 5, Xs.data.data_ptr(), Xs.off.data_ptr(),
+                                 Xs.len.data_ptr(), 5, 3, pairs.data_ptr(), 1, _lib.STATIC_RBF,
+                                 1.0, 40, 1, 1, -1, out.data_ptr(), _lib.stream_handle(dev()))
+  assert rc == 1002
+  K = ksig.kernels.SignatureKernel(40, order=1, normalize=False)(X)
+  assert normwise(K, O.signature_kernel(X, n_levels=40, normalize=False)) < TOL64
