@@ -1008,6 +1008,22 @@ def run_c3(args, P: dict, ref64: dict) -> dict:
   kern_ms = e0.elapsed_time(e1) / args.steps
   ms_per_step = wall * 1e3 / args.steps
 
+  # host planning of a repeated ragged call (the schedule cache) and the
+  # end-to-end time of the public op on this pair list (outside the timed
+  # region: it includes the planning)
+  from ksig_amd import tune as _tune
+  _tune.tuned_schedule(X, Y, pairs, _lib.STATIC_RBF, n_levels, order)
+  t2 = time.perf_counter()
+  for _ in range(5):
+    _tune.tuned_schedule(X, Y, pairs, _lib.STATIC_RBF, n_levels, order)
+  plan_repeat_ms = (time.perf_counter() - t2) * 1e3 / 5
+  torch.cuda.synchronize()
+  t2 = time.perf_counter()
+  for _ in range(5):
+    _ops.signature_levels(X, Y, pairs, n_levels, order, True, _lib.STATIC_RBF, sigma)
+  torch.cuda.synchronize()
+  call_ms = (time.perf_counter() - t2) * 1e3 / 5
+
   # per-pair error vs the FP64 oracle (inputs rounded to float32 as on the GPU)
   if not ref64:
     for p in np.random.default_rng(5).choice(n, 16, replace=False):
@@ -1034,7 +1050,9 @@ def run_c3(args, P: dict, ref64: dict) -> dict:
                              "launch (fused FP32 row stream)",
                  "cells": cells, "cell_levels": cell_levels,
                  "n_waves": int(sched[1].shape[0] - 1),
-                 "global_fifo": bool(sched[3]), "plan_ms": plan_ms, "autotune": tuned},
+                 "global_fifo": bool(sched[3]), "plan_ms": plan_ms,
+                 "plan_ms_repeat_call": plan_repeat_ms, "public_op_call_ms": call_ms,
+                 "autotune": tuned},
       "max_rel_err_vs_fp64": float(errs.max()),
       "median_rel_err_vs_fp64": float(np.median(errs)),
       "err_sample": f"{len(errs)} uniformly drawn pairs, normwise over each pair's level "

@@ -202,6 +202,15 @@ hipError_t dp_stream_launch(const StreamEntry& e, const DpStreamArgs& a, size_t 
                             hipStream_t s);
 int dp_stream_occupancy(const StreamEntry& e, size_t lds);
 hipError_t pde_forward_launch(const PdeFwdArgs& a, int dtype, size_t lds, hipStream_t s);
+// FIFO slots (band pass length) of the phase-aligned PDE forward (dyadic
+// order 0 and 1) for nc original columns: a multiple of 2^lam, at least
+// 64 * 2^lam + 2.
+__host__ __device__ inline int pde_aligned_pc(int nc, int lam) {
+  const int f = 1 << lam;
+  int pc = (nc + 1) * f;
+  if (pc < 64 * f + 2) pc = 64 * f + 2;
+  return (pc + f - 1) / f * f;
+}
 size_t pde_bwd_edges_lds(int dtype, int d, int ly, int ncr, int nrr);
 int pde_bwd_fifo_cols(int ncr);
 size_t pde_bwd_fifo_bytes(int dtype, int ncr);

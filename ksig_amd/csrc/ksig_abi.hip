@@ -754,7 +754,13 @@ static int pde_common(int dtype, ksig::PdeFwdArgs& a, int cols, int dmax, void* 
   long pc = (long)(nc + 1) << a.dyadic;
   if (pc <= ksig::kWave) pc = ksig::kWave + 1;  // as the kernel's band pass
   const size_t ycols = a.kind == ksig::kGram ? 0 : (size_t)cols;
-  const size_t lds = (ycols * dmax + ycols + (size_t)pc * 4) * esz;
+  size_t lds = (ycols * dmax + ycols + (size_t)pc * 4) * esz;
+  if (a.dyadic <= 1 && nc >= 0) {  // phase-aligned kernel: FIFO of [G, U, V], x rows per lane
+    pc = ksig::pde_aligned_pc(nc, a.dyadic);
+    const size_t xslots =
+        a.kind == ksig::kGram ? 0 : (size_t)ksig::kWave * (4 >> a.dyadic) * (dmax + 1);
+    lds = ((size_t)pc * 3 + xslots) * esz;
+  }
   if (lds > max_lds_bytes())
     return fail(KSIG_ELDS, "per-pair LDS state of %zu bytes exceeds the device limit", lds);
   hipError_t e = ksig::pde_forward_launch(a, dtype, lds, (hipStream_t)stream);

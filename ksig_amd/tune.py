@@ -136,6 +136,7 @@ def autotune(X: _ops.SeqSet, Y: _ops.SeqSet, pairs: torch.Tensor, n_levels: int,
   best = min(times, key=times.get)
   lx, ly = _pair_lengths(X, Y, pairs)
   _cache[signature_key(lx, ly, X.d, kind, n_levels, order)] = best
+  _generation[0] += 1
   return best, times
 
 
@@ -149,19 +150,57 @@ def lookup(X: _ops.SeqSet, Y: _ops.SeqSet, pairs: torch.Tensor, kind: int, n_lev
   return _cache.get(signature_key(lx, ly, X.d, kind, n_levels, order))
 
 
+# Schedules already built, keyed by the identity of the pair list and the
+# sets' host length arrays (a repeated call with the same tensors skips the
+# host planning and the device copies), and by the content of the pair
+# lengths (a new tensor with the same lengths costs one device-to-host copy
+# of the pair list and a hash).
+_SCHED_MAX = 64
+_sched_by_id: Dict[tuple, tuple] = {}
+_sched_by_content: Dict[tuple, tuple] = {}
+_generation = [0]   # bumped whenever the kept launch choices change
+
+
+def _remember(table: dict, key, value) -> None:
+  if len(table) >= _SCHED_MAX:
+    table.pop(next(iter(table)))
+  table[key] = value
+
+
 def tuned_schedule(X: _ops.SeqSet, Y: _ops.SeqSet, pairs: torch.Tensor, kind: int,
                    n_levels: int, order: int):
   """The schedule of the kept choice, or the default length-bucketed schedule
-  when this signature has not been tuned."""
-  cfg = lookup(X, Y, pairs, kind, n_levels, order)
-  if cfg is None:
-    return _ops.make_schedule(X, Y, pairs, kind, n_levels, order)
-  return _ops.make_schedule(X, Y, pairs, kind, n_levels, order, cfg.global_fifo,
-                            cfg.max_slice, cfg.oversub)
+  when this signature has not been tuned; built schedules are cached."""
+  _ensure_loaded()
+  base = (X.d, kind, n_levels, _norm_order(n_levels, order), X.max_len, Y.max_len,
+          str(pairs.device), _generation[0])
+  idkey = base + (id(pairs), id(X.host_len), id(Y.host_len))
+  hit = _sched_by_id.get(idkey)
+  if (hit is not None and hit[0] is pairs and hit[1] == pairs._version
+      and hit[2] is X.host_len and hit[3] is Y.host_len):
+    return hit[4]
+  lx, ly = _pair_lengths(X, Y, pairs)
+  ckey = base + (hash(lx.tobytes()), hash(ly.tobytes()), lx.size)
+  sched = _sched_by_content.get(ckey)
+  if sched is None:
+    cfg = _cache.get(signature_key(lx, ly, X.d, kind, n_levels, order)) if _cache else None
+    if cfg is None:
+      sched = _ops.make_schedule(X, Y, pairs, kind, n_levels, order)
+    else:
+      sched = _ops.make_schedule(X, Y, pairs, kind, n_levels, order, cfg.global_fifo,
+                                 cfg.max_slice, cfg.oversub)
+    _remember(_sched_by_content, ckey, sched)
+  # the entry holds the pair tensor and the length arrays, so their ids stay
+  # theirs while it lives
+  _remember(_sched_by_id, idkey, (pairs, pairs._version, X.host_len, Y.host_len, sched))
+  return sched
 
 
 def clear() -> None:
   _cache.clear()
+  _generation[0] += 1
+  _sched_by_id.clear()
+  _sched_by_content.clear()
 
 
 def save(path: str) -> None:
@@ -174,6 +213,7 @@ def load(path: str) -> None:
     raw = json.load(f)
   for k, v in raw.items():
     _cache[k] = StreamConfig(bool(v["global_fifo"]), int(v["max_slice"]), int(v["oversub"]))
+  _generation[0] += 1
 
 
 def _ensure_loaded() -> None:

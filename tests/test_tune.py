@@ -68,3 +68,41 @@ def test_rws_groups_partition_the_series():
     assert n_g == 1 or rows.min() >= 1
     assert n_g <= max(1, lens.sum() // 64) and n_g <= n_w
   assert list(rws_groups(np.zeros(0, np.int64), 4)) == [0]
+
+
+def test_schedule_cache_skips_repeated_planning(monkeypatch):
+  """A repeated call with the same pair tensor reuses the built schedule
+  without touching the pair list; a new tensor with the same pair lengths
+  reuses it after one content hash; changed lengths rebuild it."""
+  import torch
+  from ksig_amd import _ops
+  tune.clear()
+  builds = []
+
+  def fake_make_schedule(X, Y, pairs, kind, n_levels, order, *cfg):
+    builds.append(cfg)
+    return ("schedule", len(builds))
+  monkeypatch.setattr(_ops, "make_schedule", fake_make_schedule)
+  rng = np.random.default_rng(6)
+  lx = rng.integers(2, 50, 30)
+  ly = rng.integers(2, 50, 20)
+  X = _ops.SeqSet(torch.zeros((int(lx.sum()), 4)), torch.zeros(30, dtype=torch.int64),
+                  torch.as_tensor(lx, dtype=torch.int32), int(lx.max()), 30, lx)
+  Y = _ops.SeqSet(torch.zeros((int(ly.sum()), 4)), torch.zeros(20, dtype=torch.int64),
+                  torch.as_tensor(ly, dtype=torch.int32), int(ly.max()), 20, ly)
+  pairs = torch.as_tensor(np.stack([rng.integers(0, 30, 100), rng.integers(0, 20, 100)], -1),
+                          dtype=torch.int32)
+  s1 = tune.tuned_schedule(X, Y, pairs, 0, 5, 1)
+  s2 = tune.tuned_schedule(X, Y, pairs, 0, 5, 1)
+  assert s1 == s2 and len(builds) == 1
+  s3 = tune.tuned_schedule(X, Y, pairs.clone(), 0, 5, 1)   # same content
+  assert s3 == s1 and len(builds) == 1
+  pairs[:, 0] = (pairs[:, 0] + 1) % 30                     # in place: new lengths
+  tune.tuned_schedule(X, Y, pairs, 0, 5, 1)
+  assert len(builds) == 2
+  tune.tuned_schedule(X, Y, pairs, 0, 6, 1)                # another problem
+  assert len(builds) == 3
+  tune.clear()                                              # kept choices changed
+  tune.tuned_schedule(X, Y, pairs, 0, 6, 1)
+  assert len(builds) == 4
+  tune.clear()
