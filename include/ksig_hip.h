@@ -226,7 +226,8 @@ size_t ksig_signature_levels_bwd_workspace(int dtype, int d, int n_levels, int m
  *   *grad_param += sum over all pairs of dL/dG dG/dparam
  * (RBF: param = sigma; linear: param = scale).  Any of grad_x, grad_y,
  * grad_param may be NULL; grad_x and grad_y may alias.  Device atomics: the
- * result is order-dependent in its last bits.  1 <= d <= 32.
+ * result is order-dependent in its last bits.  Any d >= 1 (float32: MFMA tiles up
+ * to d = 64, a VALU reduction beyond).
  */
 int ksig_signature_pde_grad_contract(int dtype, const void* x, const int64_t* x_off,
                                      const int32_t* x_len, int max_len_x, const void* y,
@@ -250,7 +251,7 @@ int ksig_signature_pde_grad_contract(int dtype, const void* x, const int64_t* x_
  * omega = dL/dG * G; linear: c_cross = scale, c_self = 0, omega = dL/dG.
  * Either gradient may be NULL; both may alias (x and y the same set).
  * Accumulates with device atomics: the result is order-dependent in its last
- * bits.  1 <= d <= 32.
+ * bits.  Any d >= 1 (float32: MFMA tiles up to d = 64, a VALU reduction beyond).
  */
 int ksig_static_grad_contract(int dtype, const void* x, const int64_t* x_off,
                               const int32_t* x_len, int max_len_x, const void* y,

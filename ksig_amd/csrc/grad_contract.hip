@@ -359,47 +359,51 @@ __global__ __launch_bounds__(64) void contract_pde_simple_kernel(const ContractA
     *fac = dot;
     return gG;
   };
-  T acc[32];
+  // features in chunks of 32 accumulators (omega recomputed per chunk: this
+  // is the float64 / very wide fallback)
+  for (int f0 = 0; f0 < d; f0 += 32) {
+    T acc[32];
 #pragma unroll
-  for (int f = 0; f < 32; ++f) acc[f] = T(0);
-  T sum = T(0), gp = T(0);
-  if (side == 0) {
-    if (r >= A) return;
-    for (int b = 0; b < B; ++b) {
-      T fac;
-      const T wv = omega_at(r, b, &fac);
+    for (int f = 0; f < 32; ++f) acc[f] = T(0);
+    T sum = T(0), gp = T(0);
+    if (side == 0) {
+      if (r >= A) return;
+      for (int b = 0; b < B; ++b) {
+        T fac;
+        const T wv = omega_at(r, b, &fac);
 #pragma unroll
-      for (int f = 0; f < 32; ++f)
-        if (f < d) acc[f] += wv * Y[(yo + b) * d + f];
-      sum += wv;
-      gp += wv * fac;
-    }
-    T* G = static_cast<T*>(a.gx);
-    if (G) {
+        for (int f = 0; f < 32; ++f)
+          if (f0 + f < d) acc[f] += wv * Y[(yo + b) * d + f0 + f];
+        sum += wv;
+        gp += wv * fac;
+      }
+      T* G = static_cast<T*>(a.gx);
+      if (G) {
+#pragma unroll
+        for (int f = 0; f < 32; ++f) {
+          if (f0 + f >= d) break;
+          const int64_t o = (xo + r) * d + f0 + f;
+          unsafeAtomicAdd(G + o, cc * acc[f] + cself * sum * X[o]);
+        }
+      }
+      if (a.gparam && f0 == 0) unsafeAtomicAdd(a.gparam, (double)gp * a.gp_scale);
+    } else {
+      T* G = static_cast<T*>(a.gy);
+      if (!G || r >= B) return;
+      for (int q = 0; q < A; ++q) {
+        T fac;
+        const T wv = omega_at(q, r, &fac);
+#pragma unroll
+        for (int f = 0; f < 32; ++f)
+          if (f0 + f < d) acc[f] += wv * X[(xo + q) * d + f0 + f];
+        sum += wv;
+      }
 #pragma unroll
       for (int f = 0; f < 32; ++f) {
-        if (f >= d) break;
-        const int64_t o = (xo + r) * d + f;
-        unsafeAtomicAdd(G + o, cc * acc[f] + cself * sum * X[o]);
+        if (f0 + f >= d) break;
+        const int64_t o = (yo + r) * d + f0 + f;
+        unsafeAtomicAdd(G + o, cc * acc[f] + cself * sum * Y[o]);
       }
-    }
-    if (a.gparam) unsafeAtomicAdd(a.gparam, (double)gp * a.gp_scale);
-  } else {
-    T* G = static_cast<T*>(a.gy);
-    if (!G || r >= B) return;
-    for (int q = 0; q < A; ++q) {
-      T fac;
-      const T wv = omega_at(q, r, &fac);
-#pragma unroll
-      for (int f = 0; f < 32; ++f)
-        if (f < d) acc[f] += wv * X[(xo + q) * d + f];
-      sum += wv;
-    }
-#pragma unroll
-    for (int f = 0; f < 32; ++f) {
-      if (f >= d) break;
-      const int64_t o = (yo + r) * d + f;
-      unsafeAtomicAdd(G + o, cc * acc[f] + cself * sum * Y[o]);
     }
   }
 }
@@ -455,16 +459,24 @@ template <int MODE, int KIND>
 static void launch_f32(const ContractArgs& b, dim3 grid, hipStream_t s) {
   if (b.d <= 16)
     hipLaunchKernelGGL((contract_f32_kernel<1, MODE, KIND>), grid, dim3(256), 0, s, b);
-  else
+  else if (b.d <= 32)
     hipLaunchKernelGGL((contract_f32_kernel<2, MODE, KIND>), grid, dim3(256), 0, s, b);
+  else if (b.d <= 48)
+    hipLaunchKernelGGL((contract_f32_kernel<3, MODE, KIND>), grid, dim3(256), 0, s, b);
+  else
+    hipLaunchKernelGGL((contract_f32_kernel<4, MODE, KIND>), grid, dim3(256), 0, s, b);
 }
+
+// Features the MFMA tiles take (16 per block, up to 4 blocks); wider inputs
+// take the VALU reduction, which loops over any d.
+constexpr int kContractMfmaMaxD = 64;
 
 hipError_t grad_contract_launch(const ContractArgs& a, bool f64, int max_len_x, int max_len_y,
                                 int n_cus, hipStream_t s) {
   if (a.n_pairs <= 0 || (!a.gx && !a.gy && !(a.mode != 0 && a.gparam))) return hipSuccess;
   if (max_len_x < 1 || max_len_y < 1) return hipSuccess;
   const int mx = max_len_x > max_len_y ? max_len_x : max_len_y;
-  if (f64 || (int64_t)max_len_x * max_len_y >= (int64_t(1) << 29)) {
+  if (f64 || a.d > kContractMfmaMaxD || (int64_t)max_len_x * max_len_y >= (int64_t(1) << 29)) {
     if (a.mode == 0) {
       dim3 grid((unsigned)a.n_pairs, (unsigned)((mx + kTile - 1) / kTile), 2);
       if (f64)

@@ -608,7 +608,7 @@ int ksig_signature_pde_grad_contract(int dtype, const void* x, const int64_t* x_
                                      double* grad_param, void* stream) {
   if (dtype != KSIG_F32 && dtype != KSIG_F64) return fail(KSIG_EINVAL, "bad dtype %d", dtype);
   if (n_pairs < 0 || n_pairs > 0x7fffffffLL) return fail(KSIG_EINVAL, "bad n_pairs");
-  if (d < 1 || d > 32) return fail(KSIG_EUNSUPPORTED, "d = %d outside [1, 32]", d);
+  if (d < 1) return fail(KSIG_EINVAL, "d must be >= 1 (got %d)", d);
   if (!(static_param > 0.0))
     return fail(KSIG_EINVAL, "static kernel parameter must be positive (got %g)", static_param);
   if (static_kind != ksig::kRBF && static_kind != ksig::kLinear)
@@ -659,7 +659,7 @@ int ksig_static_grad_contract(int dtype, const void* x, const int64_t* x_off,
                               double c_self, void* grad_x, void* grad_y, void* stream) {
   if (dtype != KSIG_F32 && dtype != KSIG_F64) return fail(KSIG_EINVAL, "bad dtype %d", dtype);
   if (n_pairs < 0 || n_pairs > 0x7fffffffLL) return fail(KSIG_EINVAL, "bad n_pairs");
-  if (d < 1 || d > 32) return fail(KSIG_EUNSUPPORTED, "d = %d outside [1, 32]", d);
+  if (d < 1) return fail(KSIG_EINVAL, "d must be >= 1 (got %d)", d);
   if (n_pairs == 0 || (!grad_x && !grad_y)) return 0;
   if (!x || !x_off || !x_len || !y || !y_off || !y_len || !pairs || !omega || !omega_off)
     return fail(KSIG_EINVAL, "null device pointer");

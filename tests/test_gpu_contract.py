@@ -53,7 +53,7 @@ def _run(X, Y, pairs, Ws, cc, cs, gx, gy, dtype):
 
 
 @pytest.mark.parametrize("dtype,tol", [(torch.float32, 1e-5), (torch.float64, 1e-12)])
-@pytest.mark.parametrize("d", [1, 5, 16, 17, 32])
+@pytest.mark.parametrize("d", [1, 5, 16, 17, 32, 33, 48, 64, 100])
 def test_ragged_pairs(dtype, tol, d):
   """Lengths around the 64-row tiles, unsorted pairs with repeats."""
   rng = np.random.default_rng(900 + d)

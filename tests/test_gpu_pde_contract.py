@@ -60,7 +60,7 @@ def _reference(xs, ys, pairs, gms, difference, kind, param):
 @pytest.mark.parametrize("dtype,tol", [(torch.float32, 1e-5), (torch.float64, 1e-12)])
 @pytest.mark.parametrize("kind,param", [("rbf", 1.3), ("linear", 0.7)])
 @pytest.mark.parametrize("difference", [True, False])
-@pytest.mark.parametrize("d", [4, 21])
+@pytest.mark.parametrize("d", [4, 21, 40, 70])
 def test_pde_grad_contract(dtype, tol, kind, param, difference, d):
   rng = np.random.default_rng(700 + d + 2 * int(difference))
   lx = [1, 40, 65, 130, 7]
