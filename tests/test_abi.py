@@ -80,9 +80,9 @@ def test_workspace_queries_and_adjoint_argument_checks():
   rc = lib.ksig_signature_levels_bwd(
       _lib.F32, p, p, p, 128, p, p, p, 128, 8, p, 4, 0, 1.0, 6, p, p, p, p, p, p, None, 0, None)
   assert rc == 1001 and b"workspace" in lib.ksig_last_error()
-  rc = lib.ksig_static_grad_contract(_lib.F32, p, p, p, 4, p, p, p, 4, 40, p, 1, p, p, 1.0, 0.0,
+  rc = lib.ksig_static_grad_contract(_lib.F32, p, p, p, 4, p, p, p, 4, 0, p, 1, p, p, 1.0, 0.0,
                                      p, p, None)
-  assert rc == 1002
+  assert rc == 1001 and b"d" in lib.ksig_last_error()
   rc = lib.ksig_signature_pde_grad_contract(_lib.F32, p, p, p, 4, p, p, p, 4, 3, p, 1, 7, 1.0,
                                             1, p, p, p, p, None, None)
   assert rc == 1001
