@@ -319,9 +319,12 @@ dp_backward_mfma_kernel(const DpBwdArgs a) {
   constexpr int kSlotB = NFG * 4;  // FIFO slot bytes
   int fbyte = 0;  // lane 0's FIFO slot of the next prefetched step: ((step + 2) mod Pc) * kSlotB
   const int fin_vo = lane == 0 ? 0 : int(kBufOob);
+  // every step issues NFG/4 FIFO loads and NFG/4 FIFO stores
+  constexpr int kFifoWait = fifo_wait_count(bwd_mfma_pc(0) - (kWave + 1), NFG / 2);
   auto prefetch = [&](T& arg_o, T* fin_o) __attribute__((always_inline)) {  // uniform
     arg_o = ring[ring_pos(rslot, lane)];
     rslot = rslot + 1 == W ? 0 : rslot + 1;
+    wait_vmcnt<kFifoWait>();
     const int f0 = fbyte;
     fbyte = fbyte + kSlotB == Pc * kSlotB ? 0 : fbyte + kSlotB;
 #pragma unroll

@@ -231,9 +231,13 @@ dp_fwd_mfma_kernel(const DpFwdArgs a) {
   int fbyte = 0;  // lane 0's FIFO slot of the next prefetched step (bytes)
   const int fin_vo = lane == 0 ? 0 : int(kBufOob);
   int s63b = (Pc - (kWave - 1)) * kSlotB;  // lane 63's FIFO slot ((t - 63) mod Pc, bytes)
+  // lane 63 stored lane 0's slot Pc - 65 >= 15 steps earlier; every step
+  // issues NFG/4 FIFO loads and NFG/4 FIFO stores
+  constexpr int kFifoWait = fifo_wait_count(fwd_mfma_pc(0) - (kWave + 1), NFG / 2);
   auto prefetch = [&](T& g_o, T* fin_o) __attribute__((always_inline)) {  // uniform
     g_o = ring[ring_pos(rslot, lane)];
     rslot = rslot + 1 == W ? 0 : rslot + 1;
+    wait_vmcnt<kFifoWait>();
 #pragma unroll
     for (int k = 0; k < NFG / 4; ++k) {
       const u32x4 q = __builtin_amdgcn_raw_buffer_load_b128(fifo_rs, fin_vo, fbyte + 16 * k, 0);

@@ -346,8 +346,13 @@ dp_stream_kernel(const DpStreamArgs a) {
 
   // FIFO slot of lane 0 two steps ahead (prefetched like the y rows)
   int s2 = 2;
+  // lane 63 stored the slot Pc - 65 >= kStreamMinColsGlobal - 65 steps
+  // earlier; each step issues a y-row load and a FIFO load at least
+  constexpr int kFifoWait =
+      fifo_wait_count(kStreamMinColsGlobal - (kWave + 1), (VEC ? DMAX / 4 : 1) + NFG / 4);
   auto fload = [&](T* dst) {
     if (GF) {
+      wait_vmcnt<kFifoWait>();
 #pragma unroll
       for (int k = 0; k < NFG; k += 4) {
         const float4 t = *reinterpret_cast<const float4*>(gfifo + s2 * NFG + k);
@@ -444,8 +449,8 @@ dp_stream_kernel(const DpStreamArgs a) {
     }
     if (lane == kWave - 1 && T_ >= kWave - 1) {
       if (GF) {
-        // read back by lane 0 at least Pc - 65 >= 4 steps later: by then the
-        // in-order vector memory counter has retired this store
+        // read back by lane 0 at least Pc - 65 steps later, after a vmcnt
+        // wait that completes this store (fload)
 #pragma unroll
         for (int k = 0; k < NFG; k += 4) {
           float4 t;

@@ -24,6 +24,31 @@ enum StaticKind : int { kRBF = 0, kLinear = 1, kGram = 2 };
 enum GramTransform : int { kGramPlain = 0, kGramRbfDots = 1 };
 
 // ---------------------------------------------------------------------------
+// s_waitcnt vmcnt(N), expcnt and lgkmcnt not waited for (gfx9-family
+// encoding: vmcnt in bits [3:0] and [15:14], expcnt [6:4], lgkmcnt [11:8]).
+// These parts retire a wave's vector memory loads and stores on one counter in
+// the order they were issued, so once vmcnt <= N every vector memory
+// instruction of the wave older than its N most recent has completed.
+// ---------------------------------------------------------------------------
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+  static_assert(N >= 0 && N <= 63, "vmcnt holds 6 bits");
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+
+// Band hand-over FIFOs in global memory: lane 63 stores a slot and lane 0
+// loads it back `steps` or more steps later, and every step issues at least
+// `per_step` vector memory instructions.  vmcnt(fifo_wait_count(...)) right
+// before lane 0's load therefore completes the store first (two steps of slack
+// for the compiler's placement of accesses inside an unrolled pair of steps);
+// a wave rarely has that many instructions in flight, so the wait is
+// normally free.
+__host__ __device__ constexpr int fifo_wait_count(int steps, int per_step) {
+  const int n = (steps - 2) * per_step - 1;
+  return n > 62 ? 62 : (n < 0 ? 0 : n);
+}
+
+// ---------------------------------------------------------------------------
 // Lane shift: lane i receives lane i-1's value; lane 0 receives `fill`.
 // DPP wave_shr:1 (dpp_ctrl 0x138) with bound_ctrl off keeps the `old` operand
 // in lane 0, so the FIFO value for lane 0 rides in as `old`.
