@@ -320,11 +320,14 @@ int ksig_signature_pde(int dtype, const void* x, const int64_t* x_off,
 
 /*
  * Training forward of the signature-PDE kernel: ksig_signature_pde, which also
- * writes, per pair at element offset e_off[p] (dtype, kernel orientation:
- * rows = the longer sequence), U of the last refined row (Ncr values) then U
- * of the last refined column (Nrr values), Nrr = (len_rows - difference) << dyadic_order,
- * Ncr likewise.  Same reference interface as ksig_signature_pde; the edges
- * have no reference counterpart (the reference has no gradients).
+ * writes, per pair at element offset e_off[p] of `edges` (float64 whatever
+ * dtype; kernel orientation: rows = the longer sequence), U of the last
+ * refined row (Ncr values) then U of the last refined column (Nrr values),
+ * Nrr = (len_rows - difference) << dyadic_order, Ncr likewise.  At dyadic
+ * order 0 and 1 U is accumulated in float64 here, since the adjoint rebuilds
+ * it backwards from these edges.  Same reference interface as
+ * ksig_signature_pde; the edges have no reference counterpart (the
+ * reference has no gradients).
  */
 int ksig_signature_pde_fwd_edges(int dtype, const void* x, const int64_t* x_off,
                                  const int32_t* x_len, int max_len_x, const void* y,
@@ -339,9 +342,10 @@ int ksig_signature_pde_fwd_edges(int dtype, const void* x, const int64_t* x_off,
  * scheme of ksig/algorithms.py:380-385 is forward-only).  Given grad_out[p] =
  * dL/dK, writes dL/dM for every original ΔΔ cell, [Nr][Nc] in kernel
  * orientation at element offset m_off[p] of grad_m (dtype).  The solution U is
- * rebuilt in the reverse sweep from the edges; the static kernel is
- * recomputed.  workspace: device scratch of at least
- * ksig_signature_pde_bwd_workspace(...) bytes, 16-byte aligned.
+ * rebuilt in float64 in the reverse sweep from the (float64) edges; the
+ * static kernel is recomputed.  workspace: device scratch of at least
+ * ksig_signature_pde_bwd_workspace(...) bytes, 16-byte aligned (none at
+ * dyadic order 0 and 1).
  */
 int ksig_signature_pde_bwd_edges(int dtype, const void* x, const int64_t* x_off,
                                  const int32_t* x_len, int max_len_x, const void* y,

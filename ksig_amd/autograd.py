@@ -214,8 +214,8 @@ _GM_BYTES = 1 << 33  # dL/dM scratch per chunk of pairs (8 GiB, as omega)
 
 class _SigPDE(torch.autograd.Function):
   """The forward also leaves U of the last refined row and column of every
-  pair (O(Lx + Ly) values); the adjoint kernel rebuilds the rest of U from
-  them in its reverse sweep and writes dL/dM per original cell, which
+  pair (O(Lx + Ly) float64 values); the adjoint kernel rebuilds the rest of U
+  from them in its reverse sweep and writes dL/dM per original cell, which
   ksig_signature_pde_grad_contract turns into input and parameter gradients
   (ΔΔ-transpose and static-kernel chain rule, fused with the contraction)."""
 
@@ -238,7 +238,7 @@ class _SigPDE(torch.autograd.Function):
     if n_pairs > 1:
       e_off[1:] = torch.cumsum(esz, 0)[:-1]
     total = int(esz.sum().item()) if n_pairs else 0
-    edges = torch.empty(max(total, 1), device=dev, dtype=Xs.data.dtype)
+    edges = torch.empty(max(total, 1), device=dev, dtype=torch.float64)
     if n_pairs:
       _lib.check(lib.ksig_signature_pde_fwd_edges(
           _lib.dtype_code(Xs.data.dtype), Xs.data.data_ptr(), Xs.off.data_ptr(),

@@ -211,9 +211,9 @@ __host__ __device__ inline int pde_aligned_pc(int nc, int lam) {
   if (pc < 64 * f + 2) pc = 64 * f + 2;
   return (pc + f - 1) / f * f;
 }
-size_t pde_bwd_edges_lds(int dtype, int d, int ly, int ncr, int nrr);
+size_t pde_bwd_edges_lds(int dtype, int d, int ly, int ncr, int nrr, int dyadic);
 int pde_bwd_fifo_cols(int ncr);
-size_t pde_bwd_fifo_bytes(int dtype, int ncr);
+size_t pde_bwd_fifo_bytes(int dtype, int ncr, int dyadic);
 hipError_t pde_bwd_edges_launch(const PdeBwdEdgeArgs& a, int dtype, size_t lds, hipStream_t s);
 hipError_t grad_contract_launch(const ContractArgs& a, bool f64, int max_len_x, int max_len_y,
                                 int n_cus, hipStream_t s);

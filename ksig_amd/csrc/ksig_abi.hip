@@ -759,7 +759,8 @@ static int pde_common(int dtype, ksig::PdeFwdArgs& a, int cols, int dmax, void* 
     pc = ksig::pde_aligned_pc(nc, a.dyadic);
     const size_t xslots =
         a.kind == ksig::kGram ? 0 : (size_t)ksig::kWave * (4 >> a.dyadic) * (dmax + 1);
-    lds = ((size_t)pc * 3 + xslots) * esz;
+    const size_t uesz = a.edges ? 8 : esz;  // U and V in float64 when saving the edges
+    lds = (size_t)pc * (esz + 2 * uesz) + xslots * esz;
   }
   if (lds > max_lds_bytes())
     return fail(KSIG_ELDS, "per-pair LDS state of %zu bytes exceeds the device limit", lds);
@@ -842,7 +843,7 @@ size_t ksig_signature_pde_bwd_workspace(int dtype, int max_len_x, int max_len_y,
   if (n_pairs <= 0 || dyadic_order < 0 || dyadic_order > 6) return 0;
   long ncr, nrr;
   pde_refined(max_len_x, max_len_y, dyadic_order, difference, &ncr, &nrr);
-  return (size_t)n_pairs * ksig::pde_bwd_fifo_bytes(dtype, (int)ncr);
+  return (size_t)n_pairs * ksig::pde_bwd_fifo_bytes(dtype, (int)ncr, dyadic_order);
 }
 
 int ksig_signature_pde_bwd_edges(int dtype, const void* x, const int64_t* x_off,
@@ -882,12 +883,13 @@ int ksig_signature_pde_bwd_edges(int dtype, const void* x, const int64_t* x_off,
   const int ly = max_len_x < max_len_y ? max_len_x : max_len_y;  // the shorter side
   long ncr, nrr;
   pde_refined(max_len_x, max_len_y, dyadic_order, difference, &ncr, &nrr);
-  const size_t lds = ksig::pde_bwd_edges_lds(dtype, d, ly < 1 ? 1 : ly, (int)ncr, (int)nrr);
+  const size_t lds =
+      ksig::pde_bwd_edges_lds(dtype, d, ly < 1 ? 1 : ly, (int)ncr, (int)nrr, dyadic_order);
   if (lds > max_lds_bytes())
     return fail(KSIG_ELDS, "adjoint LDS state of %zu bytes exceeds the device limit", lds);
   const size_t need = ksig_signature_pde_bwd_workspace(dtype, max_len_x, max_len_y,
                                                        dyadic_order, difference, n_pairs);
-  if (!workspace || workspace_bytes < need || ((uintptr_t)workspace & 15))
+  if (need > 0 && (!workspace || workspace_bytes < need || ((uintptr_t)workspace & 15)))
     return fail(KSIG_EINVAL, "workspace of %zu bytes (16-byte aligned) required, got %zu", need,
                 workspace ? workspace_bytes : (size_t)0);
   a.fifo = workspace;

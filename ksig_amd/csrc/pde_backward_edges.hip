@@ -33,6 +33,8 @@
 // cell once (no atomics).  Static-kernel values are recomputed (two per lane
 // per original column), not read back.
 
+#include <type_traits>
+
 #include "ksig_common.h"
 #include "ksig_hip.h"
 #include "dp_forward.h"
@@ -89,8 +91,8 @@ pde_bwd_edges_kernel(const PdeBwdEdgeArgs a) {
   const int nb = (nrows + BR - 1) / BR;
   const T inv4 = T(1) / T(F * F);
   const T w = T(a.gout[pr]);
-  const T* edge_row = reinterpret_cast<const T*>(a.edges) + a.e_off[pr];  // [Ncr]
-  const T* edge_col = edge_row + Ncr;                                       // [Nrr]
+  const double* edge_row = reinterpret_cast<const double*>(a.edges) + a.e_off[pr];  // [Ncr]
+  const double* edge_col = edge_row + Ncr;                                            // [Nrr]
   T* gm = reinterpret_cast<T*>(a.gm) + a.m_off[pr];                         // [Nr][Nc]
   float4* fifo = reinterpret_cast<float4*>(a.fifo) + (size_t)pr * a.fifo_cols * SQ;
 
@@ -112,8 +114,8 @@ pde_bwd_edges_kernel(const PdeBwdEdgeArgs a) {
     }
     ynrm[j] = n2;
   }
-  for (int c = lane; c < Ncr; c += kWave) erow[c] = edge_row[c];
-  for (int c = lane; c < Nrr; c += kWave) ecol[c] = edge_col[c];
+  for (int c = lane; c < Ncr; c += kWave) erow[c] = T(edge_row[c]);
+  for (int c = lane; c < Nrr; c += kWave) ecol[c] = T(edge_col[c]);
 
   auto gval = [&](const T* xv, T xn, int yc) __attribute__((always_inline)) -> T {
     const T* yr = ylds + yc * DMAX;
@@ -382,6 +384,388 @@ pde_bwd_edges_kernel(const PdeBwdEdgeArgs a) {
   if (step < total) step_fn(step, fv0, fu0);
 }
 
+// ---------------------------------------------------------------------------
+// Phase-aligned adjoint for dyadic order 0 and 1 (F = 2^LAM), R = 4 refined
+// rows per lane; the adjoint in the working precision, U rebuilt in float64.
+//
+// On the flipped grid (row index r counted up from the last refined row,
+// column c counted left from the last refined column) the adjoint and the
+// rebuild of U are both forward recursions over one stencil, so the sweep
+// has the shape of the forward: lane i takes flipped refined column k at
+// step k + F*(i+1), F steps behind lane i-1, and every lane is at the same
+// position within its original column.  The row of U index r (1-based,
+// U(0,.) = U(.,0) = 1) at column c receives from the row below (r + 1, the
+// lane below or the previous row of the lane) mu(r+1,c), a(r+1,c) lam(r+1,c),
+// b(r+1,c) lam(r+1,c) and its own V(r,c), U(r,c-1), which the row below
+// rebuilt, and computes
+//   mu(r,c)    = mu(r+1,c) + a(r+1,c) lam(r+1,c) + a(r,c+1) lam(r,c+1) + b(r+1,c+1) lam(r+1,c+1)
+//   lam(r,c)   = lam(r,c+1) + mu(r,c)
+//   S          = U(r-1,c) + U(r,c-1)
+//   V(r-1,c)   = (V(r,c) - S a(r,c) - U(r-1,c) b(r,c)) / (1 - b(r,c))
+//   U(r-1,c-1) = U(r-1,c) - V(r-1,c)
+//   dL/dM'(r,c) = lam(r,c) (S (1/2 + m/6) + U(r-1,c-1) m/6)
+// and hands mu, a lam, b lam, V(r-1,c), U(r-1,c-1) to the row above.  The
+// rebuild is the forward's increment update (V(r,c) = V(r-1,c) + S a +
+// U(r-1,c-1) b, U(r,c) = U(r,c-1) + V(r,c)) solved for the corner it did not
+// know.  It runs in float64 whatever the working precision: solving the
+// Goursat problem backwards from the saved last row and column amplifies
+// rounding by a factor that grows quickly with U (at K ~ 2 a float32
+// rebuild is 1e-5 off, at K ~ 100 it is meaningless; in float64 a pair with
+// K ~ 1e5 still loses about five digits), and a float64 fused multiply-add
+// issues at the rate of an unpacked float32 one on gfx950.  The row below
+// the grid is the saved last refined row (V and U from it, mu = dL/dK at
+// the last column, which seeds lam(Nrr,Ncr)); the top row r = 1 keeps
+// U(0,.) = 1 through a zero rebuild factor.  The static kernel, its ΔΔ and the coefficients a = m/2 +
+// m^2/12, b = m^2/12, 1/(1-b), 1/2 + m/6, m/6 are evaluated once per original
+// cell on wave-uniform steps; they are zero on the priming, padding and
+// out-of-grid cells, which therefore need no per-cell mask.  dL/dM of an
+// original cell is the sum over its F x F refined cells, which sit in one
+// lane, times 1/F^2, written once per original cell.
+// ---------------------------------------------------------------------------
+template <typename T, int DMAX, int KIND, int LAM>
+__global__ void __launch_bounds__(kWave)
+pde_bwd_aligned_kernel(const PdeBwdEdgeArgs a) {
+  constexpr int F = 1 << LAM;
+  constexpr int R = 4;
+  constexpr int NO = R / F;       // original rows per lane
+  constexpr int BR = kWave * R;   // refined rows per band
+  constexpr int NF = 4;           // FIFO slot (T): [mu, a lam, b lam, G]; (double): [V, U]
+  constexpr int XS = DMAX + 1;    // staged x row: DMAX scaled features and the norm
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* smem = reinterpret_cast<T*>(smem_raw);
+
+  const int lane = threadIdx.x;
+  const int64_t pr = blockIdx.x;
+  const int d = a.d;
+  const int32_t ia = a.pairs[2 * pr];
+  const int32_t ib = a.pairs[2 * pr + 1];
+  const T* xs = reinterpret_cast<const T*>(a.x.data) + a.x.off[ia] * d;
+  int Lx = a.x.len[ia];
+  const T* ys = reinterpret_cast<const T*>(a.y.data) + a.y.off[ib] * d;
+  int Ly = a.y.len[ib];
+  if (Ly > Lx) {  // orientation of the forward (rows = longer sequence)
+    const T* tp = xs; xs = ys; ys = tp;
+    int tl = Lx; Lx = Ly; Ly = tl;
+  }
+  const int diff = a.difference;
+  const int Nr = diff ? Lx - 1 : Lx;
+  const int Nc = diff ? Ly - 1 : Ly;
+  if (Nr <= 0 || Nc <= 0) return;
+  const int Nrr = Nr * F, Ncr = Nc * F;
+  const int Pc = pde_aligned_pc(Nc, LAM);
+  const int nb = (Nrr + BR - 1) / BR;
+  const T inv4 = T(1) / T(F * F);
+  const T xscale = T(a.xscale), yscale = T(a.yscale);
+  const double* erow = reinterpret_cast<const double*>(a.edges) + a.e_off[pr];  // U(Nrr, c), c = 1..Ncr
+  const double* ecol = erow + Ncr;                                                // U(r, Ncr), r = 1..Nrr
+  T* gm = reinterpret_cast<T*>(a.gm) + a.m_off[pr];                    // [Nr][Nc]
+  double* fifo_d = reinterpret_cast<double*>(smem_raw);      // [Pc][2]
+  T* fifo = reinterpret_cast<T*>(fifo_d + 2 * Pc);            // [Pc][NF]
+  T* xslot = fifo + Pc * NF + lane * 2 * NO * XS;  // this lane's [2][NO][XS], by band parity
+
+  // Band b of this lane: original rows (64 b + lane) * NO + k counted up from
+  // the last (cell row Nr - 1 - that), whose upper x row is the cell row's
+  // index, scaled into slot b & 1 with its norm; and U(r - 1, Ncr) of its
+  // refined rows r (the rebuild's start) into registers.
+  double ecn[R];
+  auto load_rows = [&](int b) __attribute__((always_inline)) {
+    T* dst = xslot + (b & 1) * NO * XS;
+#pragma unroll
+    for (int k = 0; k < NO; ++k) {
+      const int i = Nr - 1 - ((b * kWave + lane) * NO + k);
+      const bool ok = i >= 0;
+      T n2 = T(0);
+#pragma unroll
+      for (int e = 0; e < DMAX; ++e) {
+        const T v = (ok && e < d) ? xs[(int64_t)i * d + e] * xscale : T(0);
+        dst[k * XS + e] = v;
+        n2 += v * v;
+      }
+      dst[k * XS + DMAX] = n2;
+    }
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int rq = Nrr - ((b * kWave + lane) * R + q);
+      ecn[q] = rq >= 2 ? ecol[rq - 2] : 1.0;
+    }
+  };
+  auto load_y = [&](T* yv, T& yn, int yc) __attribute__((always_inline)) {
+    yc = yc < 0 ? 0 : (yc >= Ly ? Ly - 1 : yc);
+    T n2 = T(0);
+#pragma unroll
+    for (int e = 0; e < DMAX; ++e) {
+      const T v = e < d ? ys[(int64_t)yc * d + e] * yscale : T(0);
+      yv[e] = v;
+      n2 += v * v;
+    }
+    yn = n2;
+  };
+  auto kfused = [&](const T* xv, T xn, const T* yv, T yn) __attribute__((always_inline)) -> T {
+    T dot = T(0);
+#pragma unroll
+    for (int k = 0; k < DMAX; ++k) dot += xv[k] * yv[k];
+    return KIND == kRBF ? fast_exp2(-(xn + yn - T(2) * dot)) : dot;
+  };
+  load_rows(0);
+  double up[R];  // U(r-1, c) of the lane's rows: the rebuild's right neighbour
+#pragma unroll
+  for (int q = 0; q < R; ++q) up[q] = ecn[q];
+  load_rows(1);
+
+  // FIFO initial contents: the row below the grid.  Slot s holds flipped
+  // column k = s - F (U column c = Ncr - k): V(Nrr,c), U(Nrr,c-1), the seed
+  // mu = dL/dK at k = 0, and the static kernel of the last x row at the
+  // original column's left y column (the bottom edge of the lowest cells).
+  {
+    T xl[DMAX], xln = T(0);
+#pragma unroll
+    for (int e = 0; e < DMAX; ++e) {
+      xl[e] = (diff && e < d) ? xs[(int64_t)Nr * d + e] * xscale : T(0);
+      xln += xl[e] * xl[e];
+    }
+    const T w = T(a.gout[pr]);
+    for (int s = lane; s < Pc; s += kWave) {
+      const int k = s - F;
+      T mu = T(0), G = T(0);
+      double V = 0.0, U = 1.0;
+      if (k >= 0 && k < Ncr) {
+        const int c = Ncr - k;
+        const double uc = erow[c - 1];
+        const double ul = c >= 2 ? erow[c - 2] : 1.0;
+        V = uc - ul;
+        U = ul;
+        mu = k == 0 ? w : T(0);
+      }
+      const int yc = Nc - 1 - (k >> LAM);
+      if (diff && yc >= 0 && yc <= Nc) {
+        T yv[DMAX], yn;
+        load_y(yv, yn, yc);
+        G = kfused(xl, xln, yv, yn);
+      }
+      fifo[s * NF + 0] = mu;
+      fifo[s * NF + 1] = T(0);
+      fifo[s * NF + 2] = T(0);
+      fifo[s * NF + 3] = G;
+      fifo_d[2 * s + 0] = V;
+      fifo_d[2 * s + 1] = U;
+    }
+  }
+  __syncthreads();
+
+  // per-row state
+  T lamp[R], alp[R], blb[R];  // lam(r,c+1), a(r,c+1) lam(r,c+1), the row below's b lam of the last step
+#pragma unroll
+  for (int q = 0; q < R; ++q) lamp[q] = alp[q] = blb[q] = T(0);
+  // per original row of the lane: gradient partial, static kernel at the right
+  // y column (lower and upper x row), the scheme's coefficients
+  // (the rebuild's in double: the backward Goursat solve amplifies rounding
+  // by about the growth of U, which float32 cannot absorb beyond K ~ 10)
+  constexpr int kSeg = 8;  // original columns per written row segment
+  T gseg[NO][kSeg];
+#pragma unroll
+  for (int k = 0; k < NO; ++k)
+#pragma unroll
+    for (int w = 0; w < kSeg; ++w) gseg[k][w] = T(0);
+  T gacc[NO], gbr[NO], gtr[NO], ca[NO], cb[NO];
+  double cad[NO], cbd[NO], cgd[NO], chd[NO], rbq[R];
+#pragma unroll
+  for (int k = 0; k < NO; ++k) {
+    gacc[k] = gbr[k] = gtr[k] = ca[k] = cb[k] = T(0);
+    cad[k] = cbd[k] = cgd[k] = chd[k] = 0.0;
+  }
+#pragma unroll
+  for (int q = 0; q < R; ++q) rbq[q] = 1.0;
+  T fl[F][NF];        // outputs of the last F steps, by phase
+  double fd[F][2];    // V, U of the last F steps, by phase
+#pragma unroll
+  for (int p = 0; p < F; ++p) {
+#pragma unroll
+    for (int k = 0; k < NF; ++k) fl[p][k] = T(0);
+    fd[p][0] = 0.0;
+    fd[p][1] = 1.0;
+  }
+  int cc = -F - F * lane;  // flipped refined column of the current step
+  int band = 0;
+  int rr0 = lane * R;      // flipped first refined row of the lane in its band
+  // 1 for the refined rows whose upper neighbour is rebuilt, 0 for r = 1
+  // (U(0, .) = 1 stays exact) and the padding rows above the grid
+  double rmask[R];
+  auto set_rmask = [&]() __attribute__((always_inline)) {
+#pragma unroll
+    for (int q = 0; q < R; ++q) rmask[q] = Nrr - (rr0 + q) >= 2 ? 1.0 : 0.0;
+  };
+  set_rmask();
+
+  T ny[DMAX], nyn = T(0), nf[NF];
+  double nd[2];
+  auto fetch_col = [&](int c) __attribute__((always_inline)) {
+    load_y(ny, nyn, Nc - 1 - (c >> LAM));
+  };
+  auto fetch_fifo = [&](int c) __attribute__((always_inline)) {
+    const int s0 = __builtin_amdgcn_readfirstlane(c + F);  // lane 0's slot (in [0, Pc))
+#pragma unroll
+    for (int k = 0; k < NF; ++k) nf[k] = fifo[s0 * NF + k];
+    nd[0] = fifo_d[2 * s0 + 0];
+    nd[1] = fifo_d[2 * s0 + 1];
+  };
+  fetch_col(cc);
+  fetch_fifo(cc);
+
+  auto step_fn = [&](auto phase_tag) __attribute__((always_inline)) {
+    constexpr int P = decltype(phase_tag)::value;
+    const bool active = cc >= -F && band < nb;
+    T imu = shift_down(fl[P][0], nf[0]);
+    T ial = shift_down(fl[P][1], nf[1]);
+    T ibl = shift_down(fl[P][2], nf[2]);
+    double iv = shift_down(fd[P][0], nd[0]);
+    double iu = shift_down(fd[P][1], nd[1]);
+    T oG = T(0);
+    // a lane leaves its band after the last refined column of an original
+    // column: only on phase F-1 steps
+    const bool wrap = P == F - 1 && cc + 1 == Pc - F;
+    const int orow = Nr - 1 - (rr0 >> LAM);  // cell row of the lane's lowest original row
+    if constexpr (P == 0) {  // a new original column: once per original cell
+      const bool colok = cc >= 0 && cc < Ncr;
+      T gb = shift_down(fl[P][3], nf[3]);  // lower x row of the lowest original row
+      const T* xb = xslot + (band & 1) * NO * XS;
+#pragma unroll
+      for (int k = 0; k < NO; ++k) {
+        T xv[DMAX];
+#pragma unroll
+        for (int e = 0; e < DMAX; ++e) xv[e] = xb[k * XS + e];
+        const T gt = kfused(xv, xb[k * XS + DMAX], ny, nyn);
+        T m = diff ? (gbr[k] - gb) - (gtr[k] - gt) : gt;
+        const bool ok = colok && orow - k >= 0;
+        m = ok ? m * inv4 : T(0);
+        gbr[k] = gb;
+        gtr[k] = gt;
+        // a and b as the forward forms them, so that the rebuild inverts the
+        // very coefficients the forward used
+        cb[k] = m * m * T(1.0 / 12.0);
+        ca[k] = T(0.5) * m + cb[k];
+        cad[k] = (double)ca[k];
+        cbd[k] = (double)cb[k];
+        chd[k] = (double)m * (1.0 / 6.0);
+        cgd[k] = ok ? 0.5 + chd[k] : 0.0;
+        // 1 / (1 - b): the float32 reciprocal and two Newton steps
+        // (relative error 2^-23 -> 2^-46 -> below float64 rounding)
+        const double bb = 1.0 - cbd[k];
+        double rinv = (double)__builtin_amdgcn_rcpf((float)bb);
+        rinv = fma(rinv, fma(-bb, rinv, 1.0), rinv);
+        rinv = fma(rinv, fma(-bb, rinv, 1.0), rinv);
+#pragma unroll
+        for (int f = 0; f < F; ++f) rbq[k * F + f] = rinv * rmask[k * F + f];
+        gb = gt;
+      }
+      oG = gb;
+      // the next original column's y row, F steps ahead (the next band's
+      // priming column after the band's last original column)
+      const bool wrapn = cc + F >= Pc - F;
+      fetch_col(wrapn ? -F : cc + F);
+    }
+    const int cc1 = wrap ? -F : cc + 1;
+    fetch_fifo(cc1);
+
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int k = q / F;
+      const T mu = imu + ial + alp[q] + blb[q];
+      const T lam = lamp[q] + mu;
+      const double S = up[q] + iu;
+      const double vn = (iv - S * cad[k] - up[q] * cbd[k]) * rbq[q];
+      const double un = up[q] - vn;
+      gacc[k] += lam * T(S * cgd[k] + un * chd[k]);
+      const T al = ca[k] * lam, bl = cb[k] * lam;
+      blb[q] = ibl;
+      lamp[q] = lam;
+      alp[q] = al;
+      up[q] = un;
+      imu = mu;
+      ial = al;
+      ibl = bl;
+      iv = vn;
+      iu = un;
+    }
+    if constexpr (P == F - 1) {  // the original column's last refined column
+      // dL/dM of the lane's original cells goes through a shift register of
+      // the last kSeg original columns, written as one contiguous segment of
+      // each row: per-column stores land on 64 rows at once (the lanes are
+      // skewed), which the memory system turns into a partial-line write
+      // each, about ten times the bytes stored
+      const int p = cc >> LAM;  // flipped original column
+#pragma unroll
+      for (int k = 0; k < NO; ++k) {
+#pragma unroll
+        for (int w = kSeg - 1; w > 0; --w) gseg[k][w] = gseg[k][w - 1];
+        gseg[k][0] = gacc[k] * inv4;
+        gacc[k] = T(0);
+      }
+      const bool flush = cc >= 0 && cc < Ncr && ((p & (kSeg - 1)) == kSeg - 1 || p == Nc - 1);
+      if (flush) {
+        const int j0 = Nc - 1 - p;             // the segment's first column
+        const int cnt = (p & (kSeg - 1)) + 1;  // kSeg, or the grid's last partial segment
+#pragma unroll
+        for (int k = 0; k < NO; ++k) {
+          if (orow - k >= 0) {
+            T* g = gm + (int64_t)(orow - k) * Nc + j0;
+#pragma unroll
+            for (int w = 0; w < kSeg; ++w)
+              if (w < cnt) g[w] = gseg[k][w];
+          }
+        }
+      }
+    }
+    fl[P][0] = imu;
+    fl[P][1] = ial;
+    fl[P][2] = ibl;
+    fl[P][3] = oG;
+    fd[P][0] = iv;
+    fd[P][1] = iu;
+    if (lane == kWave - 1 && active) {
+      const int s = cc + F;
+      fifo[s * NF + 0] = imu;
+      fifo[s * NF + 1] = ial;
+      fifo[s * NF + 2] = ibl;
+      fifo[s * NF + 3] = oG;
+      fifo_d[2 * s + 0] = iv;
+      fifo_d[2 * s + 1] = iu;
+    }
+    cc = cc1;
+    if constexpr (P == F - 1) {
+      // a lane entering its next band starts from the right edge with a
+      // zero adjoint; wave-uniform test, since few steps have such a lane
+      if (__builtin_amdgcn_ballot_w64(wrap)) {
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+          blb[q] = wrap ? T(0) : blb[q];
+          lamp[q] = wrap ? T(0) : lamp[q];
+          alp[q] = wrap ? T(0) : alp[q];
+          up[q] = wrap ? ecn[q] : up[q];
+        }
+        band += wrap ? 1 : 0;
+        rr0 += wrap ? BR : 0;
+        if (wrap) set_rmask();
+      }
+    }
+  };
+
+  // The x rows of band b+1 go into the slot of band b-1 once lane 63 has
+  // entered band b (every lane has left band b-1); U(r - 1, Ncr) of band
+  // b+1 waits in registers until each lane's own entry into that band.
+  const int total = nb * Pc + F * (kWave - 1);  // a multiple of F
+  int next_load = F * (kWave - 1) + Pc;
+  int load_band = 2;
+  for (int step = 0; step < total; step += F) {
+    if (step == next_load) {  // wave-uniform
+      load_rows(load_band);
+      ++load_band;
+      next_load += Pc;
+    }
+    step_fn(std::integral_constant<int, 0>{});
+    if constexpr (F == 2) step_fn(std::integral_constant<int, 1>{});
+  }
+}
+
 // Refined rows per lane: 2 where the staged and register x rows fit (up to
 // 16 float32 / 8 float64 features), else 1 (4 rows measured no faster at
 // C2: the adjoint's float64 rebuild keeps it issue bound per row).
@@ -391,8 +775,24 @@ __host__ __device__ constexpr int pde_bwd_rows_for(int dmax, int esz) {
 template <typename T, int DMAX>
 constexpr int pde_bwd_rows() { return pde_bwd_rows_for(DMAX, (int)sizeof(T)); }
 
+template <typename T, int DMAX, int KIND>
+static hipError_t launch_bwd_aligned(const PdeBwdEdgeArgs& a, size_t lds, hipStream_t s) {
+  auto k = a.dyadic == 0 ? pde_bwd_aligned_kernel<T, DMAX, KIND, 0>
+                         : pde_bwd_aligned_kernel<T, DMAX, KIND, 1>;
+  if (lds > 64 * 1024) {
+    hipError_t e = hipFuncSetAttribute((const void*)k,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k, dim3((unsigned)a.n_pairs), dim3(kWave), lds, s, a);
+  return hipGetLastError();
+}
+
 template <typename T, int DMAX>
 static hipError_t launch_bwd_edges(const PdeBwdEdgeArgs& a, size_t lds, hipStream_t s) {
+  if (a.dyadic <= 1)
+    return a.kind == kRBF ? launch_bwd_aligned<T, DMAX, kRBF>(a, lds, s)
+                          : launch_bwd_aligned<T, DMAX, kLinear>(a, lds, s);
   auto k = pde_bwd_edges_kernel<T, DMAX, pde_bwd_rows<T, DMAX>()>;
   if (lds > 64 * 1024) {
     hipError_t e = hipFuncSetAttribute((const void*)k,
@@ -411,9 +811,12 @@ static hipError_t dispatch_bwd_edges(const PdeBwdEdgeArgs& a, size_t lds, hipStr
   return launch_bwd_edges<T, 32>(a, lds, s);
 }
 
-size_t pde_bwd_edges_lds(int dtype, int d, int ly, int ncr, int nrr) {
+size_t pde_bwd_edges_lds(int dtype, int d, int ly, int ncr, int nrr, int dyadic) {
   const size_t esz = dtype == KSIG_F32 ? 4 : 8;
   const int dmax = d <= 4 ? 4 : (d <= 8 ? 8 : (d <= 16 ? 16 : 32));
+  if (dyadic <= 1)  // phase-aligned kernel: [Pc][2] double + [Pc][4] FIFO, two bands of x rows
+    return (size_t)pde_aligned_pc(ncr >> dyadic, dyadic) * (16 + 4 * esz) +
+           (size_t)kWave * 2 * (4 >> dyadic) * (dmax + 1) * esz;
   const int rows = pde_bwd_rows_for(dmax, (int)esz);
   return ((size_t)ly * (dmax + 1) + (size_t)ncr + (size_t)nrr +
           (size_t)kWave * rows * (2 * dmax + 2)) * esz;
@@ -421,7 +824,8 @@ size_t pde_bwd_edges_lds(int dtype, int d, int ly, int ncr, int nrr) {
 
 int pde_bwd_fifo_cols(int ncr) { return pde_bwd_pc(ncr); }
 
-size_t pde_bwd_fifo_bytes(int dtype, int ncr) {
+size_t pde_bwd_fifo_bytes(int dtype, int ncr, int dyadic) {
+  if (dyadic <= 1) return 0;  // the phase-aligned kernel keeps its FIFO in LDS
   const int quads = dtype == KSIG_F32 ? pde_slot_quads<float>() : pde_slot_quads<double>();
   return (size_t)pde_bwd_pc(ncr) * quads * 16;
 }

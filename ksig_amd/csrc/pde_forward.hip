@@ -158,7 +158,7 @@ pde_forward_kernel(const PdeFwdArgs a) {
     load_x(xv2[q], xnrm2[q], ((rr0 + q + BR) >> lam) + rshift);
   }
 
-  T* edges = a.edges ? reinterpret_cast<T*>(a.edges) + a.e_off[pr] : nullptr;
+  double* edges = a.edges ? reinterpret_cast<double*>(a.edges) + a.e_off[pr] : nullptr;
 
   T fl[NF] = {T(0), T(0), T(1), T(0)};
   T gt_old[R], gb_old[R], gt[R], gb[R], M[R], uleft[R], udiag[R];
@@ -321,14 +321,17 @@ pde_forward_kernel(const PdeFwdArgs a) {
 // the bottom edge of its last original row flows down to the next lane as
 // the top edge of that lane's first.
 // ---------------------------------------------------------------------------
-template <typename T, int DMAX, int KIND, int LAM>
+template <typename T, int DMAX, int KIND, int LAM, bool EDGES>
 __global__ void __launch_bounds__(kWave)
 pde_fwd_aligned_kernel(const PdeFwdArgs a) {
+  // U and V in float64 when the edges are saved for the adjoint, whose
+  // backward rebuild amplifies any rounding of the values it starts from
+  using TU = std::conditional_t<EDGES, double, T>;
   constexpr int F = 1 << LAM;
   constexpr int R = 4;
   constexpr int NO = R / F;       // original rows per lane
   constexpr int BR = kWave * R;   // refined rows per band
-  constexpr int NF = 3;           // FIFO slot: [G, U, V]
+  constexpr int NF = 3;           // FIFO slot: G (T), U and V (TU)
   constexpr bool kGramIn = KIND == kGram;
   constexpr int NY = kGramIn ? NO : DMAX;
   constexpr int XS = DMAX + 1;    // staged x row: DMAX scaled features and the norm
@@ -372,8 +375,9 @@ pde_fwd_aligned_kernel(const PdeFwdArgs a) {
   const int rshift = diff ? 1 : 0;
   const T xscale = T(a.xscale), yscale = T(a.yscale);
   // LDS: the FIFO and each lane's x rows of its current band
-  T* fifo = smem;                     // [Pc][NF]
-  T* xslot = fifo + Pc * NF + lane * NO * XS;  // this lane's [NO][XS]
+  TU* fifo_u = reinterpret_cast<TU*>(smem_raw);      // [Pc][2]: U, V
+  T* fifo = reinterpret_cast<T*>(fifo_u + 2 * Pc);   // [Pc]: G
+  T* xslot = fifo + Pc + lane * NO * XS;  // this lane's [NO][XS]
 
   // raw x rows of band b for this lane: original rows (64 b + lane) * NO + k
   T xr[NO][DMAX];
@@ -445,26 +449,31 @@ pde_fwd_aligned_kernel(const PdeFwdArgs a) {
           g = kfused(x0, x0n, yv, yn);
         }
       }
-      fifo[s * NF + 0] = g;
-      fifo[s * NF + 1] = T(1);
-      fifo[s * NF + 2] = T(0);
+      fifo[s] = g;
+      fifo_u[2 * s + 0] = TU(1);
+      fifo_u[2 * s + 1] = TU(0);
     }
   }
   __syncthreads();
 
-  T* edges = a.edges ? reinterpret_cast<T*>(a.edges) + a.e_off[pr] : nullptr;
-  T uleft[R], udiag[R];
+  double* edges = EDGES ? reinterpret_cast<double*>(a.edges) + a.e_off[pr] : nullptr;
+  TU uleft[R], udiag[R];
 #pragma unroll
-  for (int q = 0; q < R; ++q) uleft[q] = udiag[q] = T(1);
-  T gbl[NO], gtl[NO], ca[NO], cb[NO];
+  for (int q = 0; q < R; ++q) uleft[q] = udiag[q] = TU(1);
+  T gbl[NO], gtl[NO];
+  TU ca[NO], cb[NO];
 #pragma unroll
-  for (int k = 0; k < NO; ++k) gbl[k] = gtl[k] = ca[k] = cb[k] = T(0);
-  T fl[F][3];  // outputs of the last F steps, by phase: G, U, V
+  for (int k = 0; k < NO; ++k) {
+    gbl[k] = gtl[k] = T(0);
+    ca[k] = cb[k] = TU(0);
+  }
+  T flg[F];       // outputs of the last F steps, by phase: G
+  TU fl[F][2];    // U, V
 #pragma unroll
   for (int p = 0; p < F; ++p) {
-    fl[p][0] = T(0);
-    fl[p][1] = T(1);
-    fl[p][2] = T(0);
+    flg[p] = T(0);
+    fl[p][0] = TU(1);
+    fl[p][1] = TU(0);
   }
   int cc = -F - F * lane;  // refined column of the current step
   int band = 0;
@@ -474,7 +483,8 @@ pde_fwd_aligned_kernel(const PdeFwdArgs a) {
   // norm from global memory, L2-resident, or the caller's values), fetched
   // right after the current column consumed them; lane 0's FIFO slot for
   // the next step (every lane reads it at one address; only lane 0 uses it).
-  T ny[NY], nyn = T(0), nf[3];
+  T ny[NY], nyn = T(0), nfg;
+  TU nf[2];
   auto fetch_col = [&](int c, int r0) __attribute__((always_inline)) {
     const int yc = (c >> LAM) + rshift;
     if constexpr (kGramIn) {
@@ -489,9 +499,9 @@ pde_fwd_aligned_kernel(const PdeFwdArgs a) {
   };
   auto fetch_fifo = [&](int c) __attribute__((always_inline)) {
     const int s0 = __builtin_amdgcn_readfirstlane(c + F);  // lane 0's slot (in [0, Pc))
-    nf[0] = fifo[s0 * NF + 0];
-    nf[1] = fifo[s0 * NF + 1];
-    nf[2] = fifo[s0 * NF + 2];
+    nfg = fifo[s0];
+    nf[0] = fifo_u[2 * s0 + 0];
+    nf[1] = fifo_u[2 * s0 + 1];
   };
   fetch_col(cc, rr0);
   fetch_fifo(cc);
@@ -499,9 +509,9 @@ pde_fwd_aligned_kernel(const PdeFwdArgs a) {
   auto step_fn = [&](auto phase_tag) __attribute__((always_inline)) {
     constexpr int P = decltype(phase_tag)::value;
     const bool active = cc >= -F && band < nb;
-    const T iG = shift_down(fl[P][0], nf[0]);
-    T iu = shift_down(fl[P][1], nf[1]);
-    T iv = shift_down(fl[P][2], nf[2]);
+    const T iG = shift_down(flg[P], nfg);
+    TU iu = shift_down(fl[P][0], nf[0]);
+    TU iv = shift_down(fl[P][1], nf[1]);
     T oG = iG;
     // the next step's column (per lane: the lanes wrap at different steps)
     const bool wrap = cc + 1 == Pc - F;
@@ -527,8 +537,10 @@ pde_fwd_aligned_kernel(const PdeFwdArgs a) {
         m = cc >= 0 ? m * inv4 : T(0);
         gbl[k] = gb;
         gtl[k] = gt;
-        cb[k] = m * m * T(1.0 / 12.0);
-        ca[k] = T(0.5) * m + cb[k];
+        // a and b formed in T (the adjoint forms the same values)
+        const T bq = m * m * T(1.0 / 12.0);
+        cb[k] = TU(bq);
+        ca[k] = TU(T(0.5) * m + bq);
         gt = gb;
       }
       oG = gt;
@@ -544,15 +556,15 @@ pde_fwd_aligned_kernel(const PdeFwdArgs a) {
     // (V stays 0, U stays at the left value 1 set at the band change);
     // padding rows and columns beyond the grid only feed cells below or to
     // the right of the grid.
-    T ucur[R];
+    TU ucur[R];
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       const int k = q / F;
-      const T delta = (iu + uleft[q]) * ca[k] + udiag[q] * cb[k];
-      const T v = iv + delta;
-      const T u = uleft[q] + v;
+      const TU delta = (iu + uleft[q]) * ca[k] + udiag[q] * cb[k];
+      const TU v = iv + delta;
+      const TU u = uleft[q] + v;
       udiag[q] = iu;
-      uleft[q] = wrap ? T(1) : u;
+      uleft[q] = wrap ? TU(1) : u;
       ucur[q] = u;
       iu = u;
       iv = v;
@@ -563,7 +575,7 @@ pde_fwd_aligned_kernel(const PdeFwdArgs a) {
     // that band's x rows (loaded into registers a band ahead) into its slot
     const int lastq = Nrr - 1 - rr0;
     if (lastq >= 0 && lastq < R && colok) {
-      T ul = ucur[0];
+      TU ul = ucur[0];
 #pragma unroll
       for (int q = 1; q < R; ++q) ul = lastq == q ? ucur[q] : ul;
       if (edges) edges[cc] = ul;
@@ -577,14 +589,14 @@ pde_fwd_aligned_kernel(const PdeFwdArgs a) {
     if constexpr (!kGramIn) {
       if (wrap) store_rows();
     }
-    fl[P][0] = oG;
-    fl[P][1] = iu;
-    fl[P][2] = iv;
+    flg[P] = oG;
+    fl[P][0] = iu;
+    fl[P][1] = iv;
     if (lane == kWave - 1 && active) {
       const int s = cc + F;
-      fifo[s * NF + 0] = oG;
-      fifo[s * NF + 1] = iu;
-      fifo[s * NF + 2] = iv;
+      fifo[s] = oG;
+      fifo_u[2 * s + 0] = iu;
+      fifo_u[2 * s + 1] = iv;
     }
     cc = cc1;
     band = band1;
@@ -616,8 +628,11 @@ constexpr int pde_rows() { return DMAX * (int)sizeof(T) <= 16 ? 4 : (DMAX * (int
 
 template <typename T, int DMAX, int KIND>
 static hipError_t launch_pde(const PdeFwdArgs& a, size_t lds, hipStream_t s) {
-  auto k = a.dyadic == 0   ? pde_fwd_aligned_kernel<T, DMAX, KIND, 0>
-           : a.dyadic == 1 ? pde_fwd_aligned_kernel<T, DMAX, KIND, 1>
+  const bool e = a.edges != nullptr;
+  auto k = a.dyadic == 0   ? (e ? pde_fwd_aligned_kernel<T, DMAX, KIND, 0, true>
+                                : pde_fwd_aligned_kernel<T, DMAX, KIND, 0, false>)
+           : a.dyadic == 1 ? (e ? pde_fwd_aligned_kernel<T, DMAX, KIND, 1, true>
+                                : pde_fwd_aligned_kernel<T, DMAX, KIND, 1, false>)
                            : pde_forward_kernel<T, DMAX, KIND, pde_rows<T, DMAX>()>;
   if (lds > 64 * 1024) {
     hipError_t e = hipFuncSetAttribute((const void*)k,

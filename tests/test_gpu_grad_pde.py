@@ -104,3 +104,70 @@ def test_pde_gradients_float32_c2_shape_subset():
     grads[dt] = (Xt.grad.double().cpu(), float(s.grad))
   assert _rel(grads[torch.float32][0], grads[torch.float64][0]) < 1e-4
   assert abs(grads[torch.float32][1] - grads[torch.float64][1]) < 1e-4 * abs(grads[torch.float64][1])
+
+
+@pytest.mark.parametrize("lam,difference,kind", [
+    (0, True, "rbf"), (1, True, "rbf"), (1, True, "linear"), (0, False, "linear"),
+    (1, False, "linear")])
+def test_pde_gradients_multiband_vs_finite_differences(lam, difference, kind):
+  """Float64 adjoint on ragged sets whose grids span two and three bands of
+  256 refined rows, in both orientations (lengths 300 and 131 against 90
+  and 350), against central differences (h = 1e-6) of the float64 forward:
+  the weighted sum of the four pair values with respect to 10 random input
+  coordinates of each side and the static-kernel parameter; then the
+  float32 adjoint on the same sets against the float64 one.  Without
+  differencing the cells hold the static kernel itself, so the paths are
+  scaled by 0.02 there (K up to about 340).  The adjoint rebuilds U backwards
+  from the saved last row and column, which amplifies rounding with the
+  growth of U: at scale 0.1 (K ~ 2e5) even the float64 rebuild is off by
+  5e-6 in the gradient, which is a limit of the method, not of these
+  tolerances."""
+  rng = np.random.default_rng(40 + 2 * lam + int(difference))
+  d = 3
+  sc = 1.0 if difference else 0.02
+  xs = [torch.tensor(sc * O.brownian(rng, 1, L, d)[0], device=dev()) for L in (300, 131)]
+  ys = [torch.tensor(sc * O.brownian(rng, 1, L, d)[0], device=dev()) for L in (90, 350)]
+  Xs, Ys = _ops.SeqSet.ragged(xs), _ops.SeqSet.ragged(ys)
+  pairs = _ops.dense_pairs(2, 2, dev())
+  code = _lib.STATIC_RBF if kind == "rbf" else _lib.STATIC_LINEAR
+  param = 0.8 if kind == "rbf" else 0.7
+  c = torch.tensor(rng.standard_normal(4), device=dev())
+  xd = Xs.data.clone().requires_grad_(True)
+  yd = Ys.data.clone().requires_grad_(True)
+  p = torch.tensor(param, dtype=torch.float64, device=dev(), requires_grad=True)
+  K = autograd.signature_pde(Xs, Ys, pairs, code, p, lam, difference, xd, yd)
+  (K * c).sum().backward()
+
+  def value(xdata, ydata, pv):
+    Xn = _ops.SeqSet(xdata, Xs.off, Xs.len, Xs.max_len, Xs.n, Xs.host_len)
+    Yn = _ops.SeqSet(ydata, Ys.off, Ys.len, Ys.max_len, Ys.n, Ys.host_len)
+    return float((_ops.signature_pde(Xn, Yn, pairs, difference, code, pv, lam) * c).sum())
+
+  h = 1e-6
+  for side, data, grad in (("x", Xs.data, xd.grad), ("y", Ys.data, yd.grad)):
+    rows = data.shape[0]
+    for _ in range(10):
+      r, k = int(rng.integers(0, rows)), int(rng.integers(0, d))
+      dp, dm = data.clone(), data.clone()
+      dp[r, k] += h
+      dm[r, k] -= h
+      if side == "x":
+        fd = (value(dp, Ys.data, param) - value(dm, Ys.data, param)) / (2 * h)
+      else:
+        fd = (value(Xs.data, dp, param) - value(Xs.data, dm, param)) / (2 * h)
+      g = float(grad[r, k])
+      assert abs(fd - g) < 1e-6 * max(1.0, abs(fd)), (side, r, k, fd, g)
+  fd = (value(Xs.data, Ys.data, param + h) - value(Xs.data, Ys.data, param - h)) / (2 * h)
+  assert abs(fd - float(p.grad)) < 1e-6 * max(1.0, abs(fd)), (fd, float(p.grad))
+
+  X32 = _ops.SeqSet(Xs.data.float(), Xs.off, Xs.len, Xs.max_len, Xs.n, Xs.host_len)
+  Y32 = _ops.SeqSet(Ys.data.float(), Ys.off, Ys.len, Ys.max_len, Ys.n, Ys.host_len)
+  xd32 = X32.data.clone().requires_grad_(True)
+  yd32 = Y32.data.clone().requires_grad_(True)
+  p32 = torch.tensor(param, dtype=torch.float64, device=dev(), requires_grad=True)
+  K32 = autograd.signature_pde(X32, Y32, pairs, code, p32, lam, difference, xd32, yd32)
+  (K32 * c).sum().backward()
+  assert _rel(K32, K) < 1e-5
+  assert _rel(xd32.grad, xd.grad) < 1e-4
+  assert _rel(yd32.grad, yd.grad) < 1e-4
+  assert abs(float(p32.grad) - float(p.grad)) < 1e-4 * max(1.0, abs(float(p.grad)))
