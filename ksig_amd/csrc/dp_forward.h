@@ -211,8 +211,17 @@ __host__ __device__ inline int pde_aligned_pc(int nc, int lam) {
   if (pc < 64 * f + 2) pc = 64 * f + 2;
   return (pc + f - 1) / f * f;
 }
+// Band pass of the phase-aligned PDE adjoint: the forward's, and at least
+// 64 * 2^lam + 32 steps, so that lane 63 writes a global-memory FIFO slot
+// 2^lam + 30 steps before lane 0 reads it.
+__host__ __device__ inline int pde_bwd_aligned_pc(int nc, int lam) {
+  const int f = 1 << lam;
+  const int pc = pde_aligned_pc(nc, lam);
+  const int lo = 64 * f + 32;
+  return pc >= lo ? pc : (lo + f - 1) / f * f;
+}
 size_t pde_bwd_edges_lds(int dtype, int d, int ly, int ncr, int nrr, int dyadic);
-int pde_bwd_fifo_cols(int ncr);
+int pde_bwd_fifo_cols(int ncr, int dyadic);
 size_t pde_bwd_fifo_bytes(int dtype, int ncr, int dyadic);
 hipError_t pde_bwd_edges_launch(const PdeBwdEdgeArgs& a, int dtype, size_t lds, hipStream_t s);
 hipError_t grad_contract_launch(const ContractArgs& a, bool f64, int max_len_x, int max_len_y,

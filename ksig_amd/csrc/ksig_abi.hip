@@ -893,7 +893,7 @@ int ksig_signature_pde_bwd_edges(int dtype, const void* x, const int64_t* x_off,
     return fail(KSIG_EINVAL, "workspace of %zu bytes (16-byte aligned) required, got %zu", need,
                 workspace ? workspace_bytes : (size_t)0);
   a.fifo = workspace;
-  a.fifo_cols = ksig::pde_bwd_fifo_cols((int)ncr);
+  a.fifo_cols = ksig::pde_bwd_fifo_cols((int)ncr, dyadic_order);
   hipError_t e = ksig::pde_bwd_edges_launch(a, dtype, lds, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "pde_bwd_edges launch");
   return 0;

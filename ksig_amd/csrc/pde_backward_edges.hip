@@ -429,7 +429,8 @@ pde_bwd_aligned_kernel(const PdeBwdEdgeArgs a) {
   constexpr int R = 4;
   constexpr int NO = R / F;       // original rows per lane
   constexpr int BR = kWave * R;   // refined rows per band
-  constexpr int NF = 4;           // FIFO slot (T): [mu, a lam, b lam, G]; (double): [V, U]
+  constexpr int NF = 4;           // FIFO slot: [mu, a lam, b lam, G] (T), then [V, U] (double)
+  constexpr int SB = NF * (int)sizeof(T) + 16;  // slot bytes
   constexpr int XS = DMAX + 1;    // staged x row: DMAX scaled features and the norm
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* smem = reinterpret_cast<T*>(smem_raw);
@@ -452,16 +453,20 @@ pde_bwd_aligned_kernel(const PdeBwdEdgeArgs a) {
   const int Nc = diff ? Ly - 1 : Ly;
   if (Nr <= 0 || Nc <= 0) return;
   const int Nrr = Nr * F, Ncr = Nc * F;
-  const int Pc = pde_aligned_pc(Nc, LAM);
+  const int Pc = pde_bwd_aligned_pc(Nc, LAM);
   const int nb = (Nrr + BR - 1) / BR;
   const T inv4 = T(1) / T(F * F);
   const T xscale = T(a.xscale), yscale = T(a.yscale);
   const double* erow = reinterpret_cast<const double*>(a.edges) + a.e_off[pr];  // U(Nrr, c), c = 1..Ncr
   const double* ecol = erow + Ncr;                                                // U(r, Ncr), r = 1..Nrr
   T* gm = reinterpret_cast<T*>(a.gm) + a.m_off[pr];                    // [Nr][Nc]
-  double* fifo_d = reinterpret_cast<double*>(smem_raw);      // [Pc][2]
-  T* fifo = reinterpret_cast<T*>(fifo_d + 2 * Pc);            // [Pc][NF]
-  T* xslot = fifo + Pc * NF + lane * 2 * NO * XS;  // this lane's [2][NO][XS], by band parity
+  // band hand-over FIFO: this pair's [fifo_cols] slots of the workspace
+  unsigned char* fifo = reinterpret_cast<unsigned char*>(a.fifo) + (size_t)pr * a.fifo_cols * SB;
+  auto slot_t = [&](int sl) -> T* { return reinterpret_cast<T*>(fifo + (size_t)sl * SB); };
+  auto slot_d = [&](int sl) -> double* {
+    return reinterpret_cast<double*>(fifo + (size_t)sl * SB + NF * sizeof(T));
+  };
+  T* xslot = smem + lane * 2 * NO * XS;  // this lane's [2][NO][XS] in LDS, by band parity
 
   // Band b of this lane: original rows (64 b + lane) * NO + k counted up from
   // the last (cell row Nr - 1 - that), whose upper x row is the cell row's
@@ -542,12 +547,14 @@ pde_bwd_aligned_kernel(const PdeBwdEdgeArgs a) {
         load_y(yv, yn, yc);
         G = kfused(xl, xln, yv, yn);
       }
-      fifo[s * NF + 0] = mu;
-      fifo[s * NF + 1] = T(0);
-      fifo[s * NF + 2] = T(0);
-      fifo[s * NF + 3] = G;
-      fifo_d[2 * s + 0] = V;
-      fifo_d[2 * s + 1] = U;
+      T* st = slot_t(s);
+      st[0] = mu;
+      st[1] = T(0);
+      st[2] = T(0);
+      st[3] = G;
+      double* sd = slot_d(s);
+      sd[0] = V;
+      sd[1] = U;
     }
   }
   __syncthreads();
@@ -560,7 +567,9 @@ pde_bwd_aligned_kernel(const PdeBwdEdgeArgs a) {
   // y column (lower and upper x row), the scheme's coefficients
   // (the rebuild's in double: the backward Goursat solve amplifies rounding
   // by about the growth of U, which float32 cannot absorb beyond K ~ 10)
-  constexpr int kSeg = 8;  // original columns per written row segment
+  // original columns per written row segment (32 bytes; 16 at F = 1, where
+  // the lane holds four original rows)
+  constexpr int kSeg = F == 2 ? 8 : 4;
   T gseg[NO][kSeg];
 #pragma unroll
   for (int k = 0; k < NO; ++k)
@@ -596,29 +605,39 @@ pde_bwd_aligned_kernel(const PdeBwdEdgeArgs a) {
   };
   set_rmask();
 
-  T ny[DMAX], nyn = T(0), nf[NF];
-  double nd[2];
+  // lane 0's FIFO slot, read two steps ahead into the register set of the
+  // step's parity (every lane reads it at one address; only lane 0 uses it)
+  T ny[DMAX], nyn = T(0), nf[2][NF];
+  double nd[2][2];
   auto fetch_col = [&](int c) __attribute__((always_inline)) {
     load_y(ny, nyn, Nc - 1 - (c >> LAM));
   };
-  auto fetch_fifo = [&](int c) __attribute__((always_inline)) {
+  // lane 63 wrote a slot Pc - 63F - 2 >= F + 30 steps before this read,
+  // and every step issues the two loads of this read at least
+  constexpr int kFifoWait = fifo_wait_count(F + 30, 2);
+  auto fetch_fifo = [&](int Q, int c) __attribute__((always_inline)) {
     const int s0 = __builtin_amdgcn_readfirstlane(c + F);  // lane 0's slot (in [0, Pc))
+    wait_vmcnt<kFifoWait>();
+    const T* st = slot_t(s0);
+    const double* sd = slot_d(s0);
 #pragma unroll
-    for (int k = 0; k < NF; ++k) nf[k] = fifo[s0 * NF + k];
-    nd[0] = fifo_d[2 * s0 + 0];
-    nd[1] = fifo_d[2 * s0 + 1];
+    for (int k = 0; k < NF; ++k) nf[Q][k] = st[k];
+    nd[Q][0] = sd[0];
+    nd[Q][1] = sd[1];
   };
   fetch_col(cc);
-  fetch_fifo(cc);
+  fetch_fifo(0, cc);
+  fetch_fifo(1, cc + 1);
 
-  auto step_fn = [&](auto phase_tag) __attribute__((always_inline)) {
+  auto step_fn = [&](auto phase_tag, auto parity_tag) __attribute__((always_inline)) {
     constexpr int P = decltype(phase_tag)::value;
+    constexpr int Q = decltype(parity_tag)::value;  // register set of the FIFO prefetch
     const bool active = cc >= -F && band < nb;
-    T imu = shift_down(fl[P][0], nf[0]);
-    T ial = shift_down(fl[P][1], nf[1]);
-    T ibl = shift_down(fl[P][2], nf[2]);
-    double iv = shift_down(fd[P][0], nd[0]);
-    double iu = shift_down(fd[P][1], nd[1]);
+    T imu = shift_down(fl[P][0], nf[Q][0]);
+    T ial = shift_down(fl[P][1], nf[Q][1]);
+    T ibl = shift_down(fl[P][2], nf[Q][2]);
+    double iv = shift_down(fd[P][0], nd[Q][0]);
+    double iu = shift_down(fd[P][1], nd[Q][1]);
     T oG = T(0);
     // a lane leaves its band after the last refined column of an original
     // column: only on phase F-1 steps
@@ -626,7 +645,7 @@ pde_bwd_aligned_kernel(const PdeBwdEdgeArgs a) {
     const int orow = Nr - 1 - (rr0 >> LAM);  // cell row of the lane's lowest original row
     if constexpr (P == 0) {  // a new original column: once per original cell
       const bool colok = cc >= 0 && cc < Ncr;
-      T gb = shift_down(fl[P][3], nf[3]);  // lower x row of the lowest original row
+      T gb = shift_down(fl[P][3], nf[Q][3]);  // lower x row of the lowest original row
       const T* xb = xslot + (band & 1) * NO * XS;
 #pragma unroll
       for (int k = 0; k < NO; ++k) {
@@ -664,7 +683,8 @@ pde_bwd_aligned_kernel(const PdeBwdEdgeArgs a) {
       fetch_col(wrapn ? -F : cc + F);
     }
     const int cc1 = wrap ? -F : cc + 1;
-    fetch_fifo(cc1);
+    // this lane's column two steps ahead (only lane 0's is used)
+    fetch_fifo(Q, cc + 2 >= Pc - F ? cc + 2 - Pc : cc + 2);
 
 #pragma unroll
     for (int q = 0; q < R; ++q) {
@@ -723,12 +743,14 @@ pde_bwd_aligned_kernel(const PdeBwdEdgeArgs a) {
     fd[P][1] = iu;
     if (lane == kWave - 1 && active) {
       const int s = cc + F;
-      fifo[s * NF + 0] = imu;
-      fifo[s * NF + 1] = ial;
-      fifo[s * NF + 2] = ibl;
-      fifo[s * NF + 3] = oG;
-      fifo_d[2 * s + 0] = iv;
-      fifo_d[2 * s + 1] = iu;
+      T* st = slot_t(s);
+      st[0] = imu;
+      st[1] = ial;
+      st[2] = ibl;
+      st[3] = oG;
+      double* sd = slot_d(s);
+      sd[0] = iv;
+      sd[1] = iu;
     }
     cc = cc1;
     if constexpr (P == F - 1) {
@@ -755,15 +777,32 @@ pde_bwd_aligned_kernel(const PdeBwdEdgeArgs a) {
   const int total = nb * Pc + F * (kWave - 1);  // a multiple of F
   int next_load = F * (kWave - 1) + Pc;
   int load_band = 2;
-  for (int step = 0; step < total; step += F) {
-    if (step == next_load) {  // wave-uniform
+  // two steps per iteration (both phases at F = 2): the FIFO prefetch
+  // alternates between two register sets; band changes fall on a multiple of
+  // F, and Pc is even at F = 2, so the loads of the next band's rows stay on
+  // an iteration boundary at F = 2 and are checked on both steps at F = 1
+  using Z = std::integral_constant<int, 0>;
+  using O = std::integral_constant<int, 1>;
+  int step = 0;
+  for (; step + 2 <= total; step += 2) {
+    if (step == next_load || (F == 1 && step + 1 == next_load)) {  // wave-uniform
+      if (F == 1 && step + 1 == next_load) {
+        step_fn(Z{}, Z{});
+        load_rows(load_band);
+        ++load_band;
+        next_load += Pc;
+        step_fn(Z{}, O{});
+        continue;
+      }
       load_rows(load_band);
       ++load_band;
       next_load += Pc;
     }
-    step_fn(std::integral_constant<int, 0>{});
-    if constexpr (F == 2) step_fn(std::integral_constant<int, 1>{});
+    step_fn(Z{}, Z{});
+    if constexpr (F == 2) step_fn(O{}, O{});
+    else step_fn(Z{}, O{});
   }
+  if (step < total) step_fn(Z{}, Z{});  // F = 1 with an odd step count
 }
 
 // Refined rows per lane: 2 where the staged and register x rows fit (up to
@@ -814,18 +853,20 @@ static hipError_t dispatch_bwd_edges(const PdeBwdEdgeArgs& a, size_t lds, hipStr
 size_t pde_bwd_edges_lds(int dtype, int d, int ly, int ncr, int nrr, int dyadic) {
   const size_t esz = dtype == KSIG_F32 ? 4 : 8;
   const int dmax = d <= 4 ? 4 : (d <= 8 ? 8 : (d <= 16 ? 16 : 32));
-  if (dyadic <= 1)  // phase-aligned kernel: [Pc][2] double + [Pc][4] FIFO, two bands of x rows
-    return (size_t)pde_aligned_pc(ncr >> dyadic, dyadic) * (16 + 4 * esz) +
-           (size_t)kWave * 2 * (4 >> dyadic) * (dmax + 1) * esz;
+  if (dyadic <= 1)  // phase-aligned kernel: two bands of the lanes' x rows
+    return (size_t)kWave * 2 * (4 >> dyadic) * (dmax + 1) * esz;
   const int rows = pde_bwd_rows_for(dmax, (int)esz);
   return ((size_t)ly * (dmax + 1) + (size_t)ncr + (size_t)nrr +
           (size_t)kWave * rows * (2 * dmax + 2)) * esz;
 }
 
-int pde_bwd_fifo_cols(int ncr) { return pde_bwd_pc(ncr); }
+int pde_bwd_fifo_cols(int ncr, int dyadic) {
+  return dyadic <= 1 ? pde_bwd_aligned_pc(ncr >> dyadic, dyadic) : pde_bwd_pc(ncr);
+}
 
 size_t pde_bwd_fifo_bytes(int dtype, int ncr, int dyadic) {
-  if (dyadic <= 1) return 0;  // the phase-aligned kernel keeps its FIFO in LDS
+  if (dyadic <= 1)  // phase-aligned kernel: [mu, a lam, b lam, G] and [V, U] (float64) per slot
+    return (size_t)pde_bwd_fifo_cols(ncr, dyadic) * (4 * (dtype == KSIG_F32 ? 4 : 8) + 16);
   const int quads = dtype == KSIG_F32 ? pde_slot_quads<float>() : pde_slot_quads<double>();
   return (size_t)pde_bwd_pc(ncr) * quads * 16;
 }

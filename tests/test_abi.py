@@ -73,10 +73,11 @@ def test_workspace_queries_and_adjoint_argument_checks():
   assert lib.ksig_signature_levels_bwd_workspace(_lib.F32, 8, 6, 128, 128, 10) == 10 * 144 * 16 * 4
   assert lib.ksig_signature_levels_bwd_workspace(_lib.F32, 8, 6, 20, 30, 1) == 80 * 16 * 4
   assert lib.ksig_signature_levels_bwd_workspace(_lib.F64, 8, 6, 128, 128, 10) == 0
-  # PDE adjoint: none at dyadic order 0 and 1 (the FIFO is in LDS); beyond,
-  # per pair (refined columns, at least 80) 32-byte (float) slots
-  assert lib.ksig_signature_pde_bwd_workspace(_lib.F32, 256, 256, 1, 1, 3) == 0
-  assert lib.ksig_signature_pde_bwd_workspace(_lib.F64, 40, 10, 0, 0, 2) == 0
+  # PDE adjoint: at dyadic order 0 and 1 per pair one slot per step of the
+  # band pass (at least 64 * 2^order + 32) of 4 values and 2 float64;
+  # beyond, per pair (refined columns, at least 80) 32-byte (float) slots
+  assert lib.ksig_signature_pde_bwd_workspace(_lib.F32, 256, 256, 1, 1, 3) == 3 * 512 * 32
+  assert lib.ksig_signature_pde_bwd_workspace(_lib.F64, 40, 10, 0, 0, 2) == 2 * 96 * 48
   assert lib.ksig_signature_pde_bwd_workspace(_lib.F32, 256, 256, 2, 1, 3) == 3 * 1020 * 32
   assert lib.ksig_signature_pde_bwd_workspace(_lib.F64, 40, 10, 3, 0, 2) == 2 * 80 * 64
   p = ctypes.c_void_p(16)  # never dereferenced: validation fails first
