@@ -247,7 +247,9 @@ def main():
   ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                   help="only run an elementwise copy of MIB MiB (5 times): the "
                        "reference point for scaling HBM byte counters")
-  ap.add_argument("--kernel-probe", default=None, choices=["fwd_save", "adjoint", "contract"],
+  ap.add_argument("--kernel-probe", default=None,
+                  choices=["fwd_save", "adjoint", "contract", "pde_fwd", "pde_fwd_edges",
+                           "pde_adjoint", "pde_contract"],
                   help="only time one kernel of the C4 training step on one 16384-pair "
                        "chunk (128 x 128 pairs, L=128, d=32, 6 levels), --steps launches: "
                        "the unit that counter collection and the per-chunk costs refer to")
@@ -271,6 +273,8 @@ def main():
     return
 
   if args.kernel_probe:
+    if args.kernel_probe.startswith("pde_"):
+      return pde_kernel_probe(args)
     return kernel_probe(args)
   if args.workload == "c3":
     return bench_c3(args)
@@ -875,6 +879,77 @@ def kernel_probe(args):
   ms = e0.elapsed_time(e1) / args.steps
   print(json.dumps({"kernel_probe": args.kernel_probe, "pairs": P, "L": L, "d": d,
                     "n_levels": nl, "launches": args.steps, "ms_per_launch": ms}), flush=True)
+
+
+def pde_kernel_probe(args):
+  """One kernel of the signature-PDE training step on one chunk of the C2
+  shape: 128 Brownian paths (L=256, d=4) against themselves, 16384 pairs, RBF
+  sigma=1, dyadic order 1; the forward (levels only), the forward that also
+  saves the edges, the adjoint (dL/dM per original cell) or the contraction of
+  dL/dM with the inputs, each launched --steps times after one untimed
+  launch, timed with HIP events on the library's stream."""
+  from ksig_amd import _lib, _ops
+  dev = torch.device("cuda", 0)
+  lib = _lib.load()
+  st = _lib.stream_handle(dev)
+  n, L, d, lam = 128, 256, 4, 1
+  g = torch.Generator(device=dev).manual_seed(1)
+  X = brownian(g, n, L, d, dev).reshape(-1, d).contiguous()
+  S = _ops.SeqSet.dense(X.reshape(n, L, d))
+  pairs = _ops.dense_pairs(n, n, dev)
+  P = pairs.shape[0]
+  xa = (S.data.data_ptr(), S.off.data_ptr(), S.len.data_ptr(), L)
+  out = torch.empty(P, device=dev, dtype=torch.float64)
+  per = 2 * ((L - 1) << lam)
+  e_off = torch.arange(P, device=dev, dtype=torch.int64) * per
+  edges = torch.empty(P * per, device=dev, dtype=torch.float64)
+  gout = torch.randn(P, device=dev, dtype=torch.float64, generator=g)
+  m_off = torch.arange(P, device=dev, dtype=torch.int64) * ((L - 1) ** 2)
+  gm = torch.zeros(P * (L - 1) ** 2, device=dev)
+  wsb = lib.ksig_signature_pde_bwd_workspace(0, L, L, lam, 1, P)
+  ws = torch.empty(max(wsb // 4, 1), device=dev)
+  gx = torch.zeros_like(X)
+  gparam = torch.zeros((), device=dev, dtype=torch.float64)
+
+  def fwd():
+    _lib.check(lib.ksig_signature_pde(0, *xa, *xa, d, pairs.data_ptr(), P, _lib.STATIC_RBF,
+                                      1.0, lam, 1, out.data_ptr(), st))
+
+  def fwd_edges():
+    _lib.check(lib.ksig_signature_pde_fwd_edges(
+        0, *xa, *xa, d, pairs.data_ptr(), P, _lib.STATIC_RBF, 1.0, lam, 1, out.data_ptr(),
+        edges.data_ptr(), e_off.data_ptr(), st))
+
+  def adjoint():
+    _lib.check(lib.ksig_signature_pde_bwd_edges(
+        0, *xa, *xa, d, pairs.data_ptr(), P, _lib.STATIC_RBF, 1.0, lam, 1, gout.data_ptr(),
+        edges.data_ptr(), e_off.data_ptr(), gm.data_ptr(), m_off.data_ptr(), ws.data_ptr(),
+        wsb, st))
+
+  def contract():
+    _lib.check(lib.ksig_signature_pde_grad_contract(
+        0, *xa, *xa, d, pairs.data_ptr(), P, _lib.STATIC_RBF, 1.0, 1, gm.data_ptr(),
+        m_off.data_ptr(), gx.data_ptr(), gx.data_ptr(), gparam.data_ptr(), st))
+
+  probes = {"pde_fwd": fwd, "pde_fwd_edges": fwd_edges, "pde_adjoint": adjoint,
+            "pde_contract": contract}
+  if args.kernel_probe in ("pde_adjoint", "pde_contract"):
+    fwd_edges()
+    adjoint()
+  f = probes[args.kernel_probe]
+  f()
+  torch.cuda.synchronize()
+  e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+  e0.record()
+  for _ in range(args.steps):
+    f()
+  e1.record()
+  torch.cuda.synchronize()
+  ms = e0.elapsed_time(e1) / args.steps
+  print(json.dumps({"kernel_probe": args.kernel_probe, "pairs": P, "L": L, "d": d,
+                    "dyadic_order": lam, "launches": args.steps, "ms_per_launch": ms,
+                    "refined_cells_per_s": P * ((L - 1) << lam) ** 2 / (ms * 1e-3)}),
+        flush=True)
 
 
 def ragged_walks_host(rng: np.random.Generator, lens, d: int, offset_scale: float):

@@ -356,9 +356,13 @@ class SignatureFeatures(SignatureBase, static.features.KernelFeatures):
   elementwise product, time sum of every level) is one HIP pass
   (``ksig_lowrank_levels``).  With a ``DiagonalProjection`` (RFSF-DP) the
   differenced random Fourier features go straight into the diagonal level
-  scan (``ksig_lowrank_levels_diag``).  Without a projection the level-m
-  features are the D^m iterated-sum coordinates (torch ops on the GPU,
-  small D only).  Order >= 2 and other projections are not built."""
+  scan (``ksig_lowrank_levels_diag``).  Gaussian, subsampling and very
+  sparse projections run the reference's projected recursion at any order in
+  float64 torch ops on the GPU (one GEMM per projected outer product); TRP
+  also runs at any order.  Without a projection the level-m features are the
+  D^m iterated-sum coordinates (torch ops on the GPU, small D only).  Not
+  built: TensorSketch, TRP with rank > 1, and DiagonalProjection beyond
+  order 1, internal_size 2 and 5 levels."""
 
   def __init__(self, n_levels: int = 5, order: int = 1, difference: bool = True,
                normalize: bool = True, indep_feat: bool = True,

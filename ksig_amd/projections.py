@@ -129,7 +129,7 @@ class VerySparseRandomProjection(RandomProjection):
     prob = np.log(F) / F if self.sparsity == 'log' else 1. / np.sqrt(F)
     mask = self.random_state.uniform(size=[self.n_components, F]) < prob
     mask[0, 0] = True
-    full = np.where(self.random_state.uniform(size=[self.n_components, F]) < 0.5, -1., 1.)
+    full = np.where(self.random_state.uniform(size=[self.n_components, F]) < 0.5, 1., -1.)
     idx = np.where(np.any(np.abs(full) > 1e-12, axis=0))[0]
     self.sampled_idx_ = torch.as_tensor(idx, device=X.device)
     self.n_sampled_ = len(idx)

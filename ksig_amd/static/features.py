@@ -123,8 +123,11 @@ class NystroemFeatures(StaticFeatures):
   Gram K_bb = U diag(s) U^T, and features K(X, basis) U_+ diag(s_+)^(-1/2)
   over the eigenvalues with |s| > 1e-12 (:249-255).  Both Grams run through
   the base kernel on the device; the small symmetric eigenproblem is solved
-  in float64 (the reference solves it in the data dtype) and the factor is
-  kept in the data dtype.  The basis indices are drawn with NumPy (the
+  in float64 (the reference solves it in the data dtype), the factor is kept
+  in float64 and the small product K(X, basis) @ factor runs in float64
+  before the features are cast to the data dtype, so a float32 result
+  carries only the rounding of K(X, basis) and of the final cast.  The basis
+  indices are drawn with NumPy (the
   reference draws them with CuPy)."""
 
   def __init__(self, base_kernel: Kernel, n_components: int = 100, random_state=None):
@@ -145,14 +148,14 @@ class NystroemFeatures(StaticFeatures):
     Kb = self.base_kernel(basis, return_on_gpu=True).to(torch.float64)
     S, U = torch.linalg.eigh(Kb)
     keep = utils.robust_nonzero(S)
-    self.normalization_ = (U[:, keep] / utils.robust_sqrt(S[keep])[None, :]).to(X.dtype)
+    self.normalization_ = U[:, keep] / utils.robust_sqrt(S[keep])[None, :]
     self.components_ = basis
     self.component_idx_ = idx
 
   def _compute_features(self, X):
     emb = self.base_kernel(X.reshape(-1, self.n_features_), self.components_,
                            return_on_gpu=True)
-    F = emb.to(self.normalization_.dtype) @ self.normalization_
+    F = (emb.to(torch.float64) @ self.normalization_.to(emb.device)).to(X.dtype)
     return F.reshape(tuple(X.shape[:-1]) + (-1,))
 
 
