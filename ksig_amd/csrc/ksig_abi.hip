@@ -11,6 +11,7 @@
 #include "ksig_common.h"
 #include "ksig_hip.h"
 #include "dp_forward.h"
+#include "pde_common.h"
 #include "dp_stream.h"
 #include "gak.h"
 #include "rws.h"
@@ -755,12 +756,10 @@ static int pde_common(int dtype, ksig::PdeFwdArgs& a, int cols, int dmax, void* 
   if (pc <= ksig::kWave) pc = ksig::kWave + 1;  // as the kernel's band pass
   const size_t ycols = a.kind == ksig::kGram ? 0 : (size_t)cols;
   size_t lds = (ycols * dmax + ycols + (size_t)pc * 4) * esz;
-  if (a.dyadic <= 1 && nc >= 0) {  // phase-aligned kernel: FIFO of [G, U, V], x rows per lane
-    pc = ksig::pde_aligned_pc(nc, a.dyadic);
-    const size_t xslots =
-        a.kind == ksig::kGram ? 0 : (size_t)ksig::kWave * (4 >> a.dyadic) * (dmax + 1);
-    const size_t uesz = a.edges ? 8 : esz;  // U and V in float64 when saving the edges
-    lds = (size_t)pc * (esz + 2 * uesz) + xslots * esz;
+  if (a.dyadic <= 1 && nc >= 0) {  // phase-aligned kernel
+    const int uesz = a.edges ? 8 : (int)esz;  // U and V in float64 when saving the edges
+    lds = (size_t)ksig::pde_aligned_lds((int)esz, uesz, dmax, a.kind,
+                                        ksig::pde_aligned_pc(nc, a.dyadic), a.dyadic).bytes;
   }
   if (lds > max_lds_bytes())
     return fail(KSIG_ELDS, "per-pair LDS state of %zu bytes exceeds the device limit", lds);

@@ -38,6 +38,7 @@
 #include "ksig_common.h"
 #include "ksig_hip.h"
 #include "dp_forward.h"
+#include "pde_common.h"
 
 namespace ksig {
 
@@ -470,8 +471,9 @@ pde_bwd_aligned_kernel(const PdeBwdEdgeArgs a) {
 
   // Band b of this lane: original rows (64 b + lane) * NO + k counted up from
   // the last (cell row Nr - 1 - that), whose upper x row is the cell row's
-  // index, scaled into slot b & 1 with its norm; and U(r - 1, Ncr) of its
-  // refined rows r (the rebuild's start) into registers.
+  // index, prepared as the forward prepares it (pde_xrow_prep) into slot
+  // b & 1 with its norm term; and U(r - 1, Ncr) of its refined rows r (the
+  // rebuild's start) into registers.
   double ecn[R];
   auto load_rows = [&](int b) __attribute__((always_inline)) {
     T* dst = xslot + (b & 1) * NO * XS;
@@ -479,14 +481,13 @@ pde_bwd_aligned_kernel(const PdeBwdEdgeArgs a) {
     for (int k = 0; k < NO; ++k) {
       const int i = Nr - 1 - ((b * kWave + lane) * NO + k);
       const bool ok = i >= 0;
-      T n2 = T(0);
+      T raw[DMAX], xm[DMAX], nx;
 #pragma unroll
-      for (int e = 0; e < DMAX; ++e) {
-        const T v = (ok && e < d) ? xs[(int64_t)i * d + e] * xscale : T(0);
-        dst[k * XS + e] = v;
-        n2 += v * v;
-      }
-      dst[k * XS + DMAX] = n2;
+      for (int e = 0; e < DMAX; ++e) raw[e] = (ok && e < d) ? xs[(int64_t)i * d + e] : T(0);
+      pde_xrow_prep<T, DMAX, KIND == kRBF>(raw, xm, nx, xscale, yscale, LAM);
+#pragma unroll
+      for (int e = 0; e < DMAX; ++e) dst[k * XS + e] = xm[e];
+      dst[k * XS + DMAX] = nx;
     }
 #pragma unroll
     for (int q = 0; q < R; ++q) {
@@ -494,22 +495,20 @@ pde_bwd_aligned_kernel(const PdeBwdEdgeArgs a) {
       ecn[q] = rq >= 2 ? ecol[rq - 2] : 1.0;
     }
   };
+  // y row as stored and its norm term, as the forward forms them
   auto load_y = [&](T* yv, T& yn, int yc) __attribute__((always_inline)) {
     yc = yc < 0 ? 0 : (yc >= Ly ? Ly - 1 : yc);
     T n2 = T(0);
 #pragma unroll
     for (int e = 0; e < DMAX; ++e) {
-      const T v = e < d ? ys[(int64_t)yc * d + e] * yscale : T(0);
-      yv[e] = v;
-      n2 += v * v;
+      yv[e] = e < d ? ys[(int64_t)yc * d + e] : T(0);
+      const T v = yv[e] * yscale;
+      n2 = pde_fma(v, v, n2);
     }
-    yn = n2;
+    yn = KIND == kRBF ? n2 : T(0);
   };
-  auto kfused = [&](const T* xv, T xn, const T* yv, T yn) __attribute__((always_inline)) -> T {
-    T dot = T(0);
-#pragma unroll
-    for (int k = 0; k < DMAX; ++k) dot += xv[k] * yv[k];
-    return KIND == kRBF ? fast_exp2(-(xn + yn - T(2) * dot)) : dot;
+  auto kfused = [&](const T* xv, T nx, const T* yv, T yn) __attribute__((always_inline)) -> T {
+    return pde_gval<T, DMAX, KIND == kRBF>(xv, nx, yv, yn);
   };
   load_rows(0);
   double up[R];  // U(r-1, c) of the lane's rows: the rebuild's right neighbour
@@ -522,11 +521,12 @@ pde_bwd_aligned_kernel(const PdeBwdEdgeArgs a) {
   // mu = dL/dK at k = 0, and the static kernel of the last x row at the
   // original column's left y column (the bottom edge of the lowest cells).
   {
-    T xl[DMAX], xln = T(0);
+    T xl[DMAX], xln;
+    {
+      T raw[DMAX];
 #pragma unroll
-    for (int e = 0; e < DMAX; ++e) {
-      xl[e] = (diff && e < d) ? xs[(int64_t)Nr * d + e] * xscale : T(0);
-      xln += xl[e] * xl[e];
+      for (int e = 0; e < DMAX; ++e) raw[e] = (diff && e < d) ? xs[(int64_t)Nr * d + e] : T(0);
+      pde_xrow_prep<T, DMAX, KIND == kRBF>(raw, xl, xln, xscale, yscale, LAM);
     }
     const T w = T(a.gout[pr]);
     for (int s = lane; s < Pc; s += kWave) {
@@ -655,13 +655,12 @@ pde_bwd_aligned_kernel(const PdeBwdEdgeArgs a) {
         const T gt = kfused(xv, xb[k * XS + DMAX], ny, nyn);
         T m = diff ? (gbr[k] - gb) - (gtr[k] - gt) : gt;
         const bool ok = colok && orow - k >= 0;
-        m = ok ? m * inv4 : T(0);
+        m = ok ? m : T(0);  // 1/4^lam is folded into the static-kernel values
         gbr[k] = gb;
         gtr[k] = gt;
         // a and b as the forward forms them, so that the rebuild inverts the
         // very coefficients the forward used
-        cb[k] = m * m * T(1.0 / 12.0);
-        ca[k] = T(0.5) * m + cb[k];
+        pde_coef(m, ca[k], cb[k]);
         cad[k] = (double)ca[k];
         cbd[k] = (double)cb[k];
         chd[k] = (double)m * (1.0 / 6.0);

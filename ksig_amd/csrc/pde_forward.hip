@@ -34,6 +34,7 @@
 #include "ksig_common.h"
 #include "ksig_hip.h"
 #include "dp_forward.h"
+#include "pde_fwd_aligned.h"
 
 namespace ksig {
 
@@ -300,326 +301,6 @@ pde_forward_kernel(const PdeFwdArgs a) {
   }
 }
 
-// ---------------------------------------------------------------------------
-// Phase-aligned sweep for dyadic order 0 and 1 (F = 2^LAM = 1 or 2 refined
-// cells per original cell and direction), R = 4 refined rows per lane.
-//
-// Lane i processes refined column c at step c + F*i (+ F: the priming
-// columns), i.e. F steps behind lane i-1 rather than one: every lane is then
-// at the same position within its original column, so the work that belongs
-// to an original cell (one static-kernel value per original row and column,
-// its ΔΔ, the coefficients a = m/2 + m^2/12 and b = m^2/12 of the scheme)
-// runs once per original column, on wave-uniform steps, and a refined cell
-// costs only the increment update
-//   δ = (U(r-1,c) + U(r,c-1)) a + U(r-1,c-1) b,  V = V(r-1,c) + δ,  U = U(r,c-1) + V.
-// The lane-to-lane hand-over reads what lane i-1 produced F steps earlier:
-// at F = 2 the loop is unrolled by two and each phase keeps its own output
-// registers, so the delay costs no moves.  Lane 63's outputs go to an LDS
-// FIFO slot per refined column, which lane 0 of the next band reads a step
-// ahead (a band pass is at least 64F + 2 steps, so the slot is written
-// first).  The lane's R rows hold R / F original rows; the static kernel at
-// the bottom edge of its last original row flows down to the next lane as
-// the top edge of that lane's first.
-// ---------------------------------------------------------------------------
-template <typename T, int DMAX, int KIND, int LAM, bool EDGES>
-__global__ void __launch_bounds__(kWave)
-pde_fwd_aligned_kernel(const PdeFwdArgs a) {
-  // U and V in float64 when the edges are saved for the adjoint, whose
-  // backward rebuild amplifies any rounding of the values it starts from
-  using TU = std::conditional_t<EDGES, double, T>;
-  constexpr int F = 1 << LAM;
-  constexpr int R = 4;
-  constexpr int NO = R / F;       // original rows per lane
-  constexpr int BR = kWave * R;   // refined rows per band
-  constexpr int NF = 3;           // FIFO slot: G (T), U and V (TU)
-  constexpr bool kGramIn = KIND == kGram;
-  constexpr int NY = kGramIn ? NO : DMAX;
-  constexpr int XS = DMAX + 1;    // staged x row: DMAX scaled features and the norm
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* smem = reinterpret_cast<T*>(smem_raw);
-
-  const int lane = threadIdx.x;
-  const int64_t pr = blockIdx.x;
-  const int d = a.d;
-  const T* xs = nullptr;
-  const T* ys = nullptr;
-  int Lx, Ly;
-  const T* gm = nullptr;
-  if constexpr (kGramIn) {
-    Lx = a.gram_lx;
-    Ly = a.gram_ly;
-    gm = reinterpret_cast<const T*>(a.gram) + pr * (int64_t)Lx * Ly;
-  } else {
-    const int32_t ia = a.pairs[2 * pr];
-    const int32_t ib = a.pairs[2 * pr + 1];
-    xs = reinterpret_cast<const T*>(a.x.data) + a.x.off[ia] * d;
-    Lx = a.x.len[ia];
-    ys = reinterpret_cast<const T*>(a.y.data) + a.y.off[ib] * d;
-    Ly = a.y.len[ib];
-    if (Ly > Lx) {  // the Goursat problem is symmetric under transposition
-      const T* tp = xs; xs = ys; ys = tp;
-      int tl = Lx; Lx = Ly; Ly = tl;
-    }
-  }
-  const int diff = a.difference;
-  const int Nr = diff ? Lx - 1 : Lx;
-  const int Nc = diff ? Ly - 1 : Ly;
-  if (Nr <= 0 || Nc <= 0) {
-    if (lane == 0) a.out[pr] = 1.0;
-    return;
-  }
-  const int Nrr = Nr * F, Ncr = Nc * F;
-  const int Pc = pde_aligned_pc(Nc, LAM);
-  const int nb = (Nrr + BR - 1) / BR;
-  const T inv4 = T(1) / T(F * F);
-  const int rshift = diff ? 1 : 0;
-  const T xscale = T(a.xscale), yscale = T(a.yscale);
-  // LDS: the FIFO and each lane's x rows of its current band
-  TU* fifo_u = reinterpret_cast<TU*>(smem_raw);      // [Pc][2]: U, V
-  T* fifo = reinterpret_cast<T*>(fifo_u + 2 * Pc);   // [Pc]: G
-  T* xslot = fifo + Pc + lane * NO * XS;  // this lane's [NO][XS]
-
-  // raw x rows of band b for this lane: original rows (64 b + lane) * NO + k
-  T xr[NO][DMAX];
-  auto load_rows = [&](int b) __attribute__((always_inline)) {
-#pragma unroll
-    for (int k = 0; k < NO; ++k) {
-      const int row = (b * kWave + lane) * NO + k + rshift;
-      const bool ok = !kGramIn && row < Lx;
-#pragma unroll
-      for (int e = 0; e < DMAX; ++e) xr[k][e] = (ok && e < d) ? xs[(int64_t)row * d + e] : T(0);
-    }
-  };
-  auto store_rows = [&]() __attribute__((always_inline)) {
-#pragma unroll
-    for (int k = 0; k < NO; ++k) {
-      T n2 = T(0);
-#pragma unroll
-      for (int e = 0; e < DMAX; ++e) {
-        const T v = xr[k][e] * xscale;
-        xslot[k * XS + e] = v;
-        n2 += v * v;
-      }
-      xslot[k * XS + DMAX] = n2;
-    }
-  };
-  // y row (scaled) and its norm for original column index yc (clamped)
-  auto load_y = [&](T* yv, T& yn, int yc) __attribute__((always_inline)) {
-    yc = yc < 0 ? 0 : (yc >= Ly ? Ly - 1 : yc);
-    T n2 = T(0);
-#pragma unroll
-    for (int e = 0; e < DMAX; ++e) {
-      const T v = e < d ? ys[(int64_t)yc * d + e] * yscale : T(0);
-      yv[e] = v;
-      n2 += v * v;
-    }
-    yn = n2;
-  };
-  auto kfused = [&](const T* xv, T xn, const T* yv, T yn) __attribute__((always_inline)) -> T {
-    T dot = T(0);
-#pragma unroll
-    for (int k = 0; k < DMAX; ++k) dot += xv[k] * yv[k];
-    return KIND == kRBF ? fast_exp2(-(xn + yn - T(2) * dot)) : dot;
-  };
-  if constexpr (!kGramIn) {
-    load_rows(0);
-    store_rows();
-    load_rows(1);
-  }
-
-  // FIFO initial contents: the row above the grid.  Slot s holds refined
-  // column s - F, of original column ((s - F) >> LAM), whose right edge is y
-  // column ((s - F) >> LAM) + 1 (with differencing): G(x_0, y) and U = 1.
-  {
-    T x0[DMAX], x0n = T(0);
-#pragma unroll
-    for (int e = 0; e < DMAX; ++e) {
-      x0[e] = (!kGramIn && e < d) ? xs[e] * xscale : T(0);
-      x0n += x0[e] * x0[e];
-    }
-    for (int s = lane; s < Pc; s += kWave) {
-      const int yc = ((s - F) >> LAM) + rshift;
-      T g = T(0);
-      if (diff && yc >= 0 && yc < Ly) {
-        if constexpr (kGramIn) {
-          g = gm[yc];
-        } else {
-          T yv[DMAX], yn;
-          load_y(yv, yn, yc);
-          g = kfused(x0, x0n, yv, yn);
-        }
-      }
-      fifo[s] = g;
-      fifo_u[2 * s + 0] = TU(1);
-      fifo_u[2 * s + 1] = TU(0);
-    }
-  }
-  __syncthreads();
-
-  double* edges = EDGES ? reinterpret_cast<double*>(a.edges) + a.e_off[pr] : nullptr;
-  TU uleft[R], udiag[R];
-#pragma unroll
-  for (int q = 0; q < R; ++q) uleft[q] = udiag[q] = TU(1);
-  T gbl[NO], gtl[NO];
-  TU ca[NO], cb[NO];
-#pragma unroll
-  for (int k = 0; k < NO; ++k) {
-    gbl[k] = gtl[k] = T(0);
-    ca[k] = cb[k] = TU(0);
-  }
-  T flg[F];       // outputs of the last F steps, by phase: G
-  TU fl[F][2];    // U, V
-#pragma unroll
-  for (int p = 0; p < F; ++p) {
-    flg[p] = T(0);
-    fl[p][0] = TU(1);
-    fl[p][1] = TU(0);
-  }
-  int cc = -F - F * lane;  // refined column of the current step
-  int band = 0;
-  int rr0 = lane * R;      // first refined row of the lane in the current band
-
-  // The static-kernel inputs of the next new original column (y row and
-  // norm from global memory, L2-resident, or the caller's values), fetched
-  // right after the current column consumed them; lane 0's FIFO slot for
-  // the next step (every lane reads it at one address; only lane 0 uses it).
-  T ny[NY], nyn = T(0), nfg;
-  TU nf[2];
-  auto fetch_col = [&](int c, int r0) __attribute__((always_inline)) {
-    const int yc = (c >> LAM) + rshift;
-    if constexpr (kGramIn) {
-#pragma unroll
-      for (int k = 0; k < NO; ++k) {
-        const int xrow = (r0 >> LAM) + k + rshift;
-        ny[k] = (xrow < Lx && yc >= 0 && yc < Ly) ? gm[(int64_t)xrow * Ly + yc] : T(0);
-      }
-    } else {
-      load_y(ny, nyn, yc);
-    }
-  };
-  auto fetch_fifo = [&](int c) __attribute__((always_inline)) {
-    const int s0 = __builtin_amdgcn_readfirstlane(c + F);  // lane 0's slot (in [0, Pc))
-    nfg = fifo[s0];
-    nf[0] = fifo_u[2 * s0 + 0];
-    nf[1] = fifo_u[2 * s0 + 1];
-  };
-  fetch_col(cc, rr0);
-  fetch_fifo(cc);
-
-  auto step_fn = [&](auto phase_tag) __attribute__((always_inline)) {
-    constexpr int P = decltype(phase_tag)::value;
-    const bool active = cc >= -F && band < nb;
-    const T iG = shift_down(flg[P], nfg);
-    TU iu = shift_down(fl[P][0], nf[0]);
-    TU iv = shift_down(fl[P][1], nf[1]);
-    T oG = iG;
-    // the next step's column (per lane: the lanes wrap at different steps)
-    const bool wrap = cc + 1 == Pc - F;
-    const int cc1 = wrap ? -F : cc + 1;
-    const int band1 = band + (wrap ? 1 : 0);
-    const int rr1 = rr0 + (wrap ? BR : 0);
-    if constexpr (P == 0) {  // a new original column: once per original cell
-      T gt = iG;
-#pragma unroll
-      for (int k = 0; k < NO; ++k) {
-        T gb;
-        if constexpr (kGramIn) {
-          gb = ny[k];
-        } else {
-          T xv[DMAX];
-#pragma unroll
-          for (int e = 0; e < DMAX; ++e) xv[e] = xslot[k * XS + e];
-          gb = kfused(xv, xslot[k * XS + DMAX], ny, nyn);
-        }
-        T m = diff ? (gb - gbl[k]) - (gt - gtl[k]) : gb;
-        // the priming column is the boundary: a zero scheme there keeps V at
-        // zero and U at its reset value 1
-        m = cc >= 0 ? m * inv4 : T(0);
-        gbl[k] = gb;
-        gtl[k] = gt;
-        // a and b formed in T (the adjoint forms the same values)
-        const T bq = m * m * T(1.0 / 12.0);
-        cb[k] = TU(bq);
-        ca[k] = TU(T(0.5) * m + bq);
-        gt = gb;
-      }
-      oG = gt;
-      // the next original column's inputs, F steps ahead (the next band's
-      // priming column after the band's last original column)
-      const bool wrapn = cc + F >= Pc - F;
-      fetch_col(wrapn ? -F : cc + F, rr0 + (wrapn ? BR : 0));
-    }
-    fetch_fifo(cc1);
-
-    const bool colok = active && cc >= 0 && cc < Ncr;
-    // Cells need no mask: on the priming columns the coefficients are zero
-    // (V stays 0, U stays at the left value 1 set at the band change);
-    // padding rows and columns beyond the grid only feed cells below or to
-    // the right of the grid.
-    TU ucur[R];
-#pragma unroll
-    for (int q = 0; q < R; ++q) {
-      const int k = q / F;
-      const TU delta = (iu + uleft[q]) * ca[k] + udiag[q] * cb[k];
-      const TU v = iv + delta;
-      const TU u = uleft[q] + v;
-      udiag[q] = iu;
-      uleft[q] = wrap ? TU(1) : u;
-      ucur[q] = u;
-      iu = u;
-      iv = v;
-    }
-    // rare events, off the per-cell path: the lane holding the last refined
-    // row records it (and the result at the last column); every lane records
-    // its rows at the last refined column; a lane entering a new band takes
-    // that band's x rows (loaded into registers a band ahead) into its slot
-    const int lastq = Nrr - 1 - rr0;
-    if (lastq >= 0 && lastq < R && colok) {
-      TU ul = ucur[0];
-#pragma unroll
-      for (int q = 1; q < R; ++q) ul = lastq == q ? ucur[q] : ul;
-      if (edges) edges[cc] = ul;
-      if (cc == Ncr - 1) a.out[pr] = (double)ul;
-    }
-    if (edges && colok && cc == Ncr - 1) {
-#pragma unroll
-      for (int q = 0; q < R; ++q)
-        if (rr0 + q < Nrr) edges[Ncr + rr0 + q] = ucur[q];
-    }
-    if constexpr (!kGramIn) {
-      if (wrap) store_rows();
-    }
-    flg[P] = oG;
-    fl[P][0] = iu;
-    fl[P][1] = iv;
-    if (lane == kWave - 1 && active) {
-      const int s = cc + F;
-      fifo[s] = oG;
-      fifo_u[2 * s + 0] = iu;
-      fifo_u[2 * s + 1] = iv;
-    }
-    cc = cc1;
-    band = band1;
-    rr0 = rr1;
-  };
-
-  // The x rows of band b+1 are loaded into registers once lane 63 has
-  // entered band b (step b*Pc + 63F: every lane has stored band b's rows)
-  // and stored by each lane at its own entry into band b+1.
-  const int total = nb * Pc + F * (kWave - 1);  // a multiple of F
-  int next_load = F * (kWave - 1) + Pc;
-  int load_band = 2;
-  for (int step = 0; step < total; step += F) {
-    if (!kGramIn && step == next_load) {  // wave-uniform
-      load_rows(load_band);
-      ++load_band;
-      next_load += Pc;
-    }
-    step_fn(std::integral_constant<int, 0>{});
-    if constexpr (F == 2) step_fn(std::integral_constant<int, 1>{});
-  }
-}
-
 // Refined rows per lane from register pressure (x rows of this and the next
 // band stay in registers): 4 up to 4 float32 / 2 float64 features, 2 up to
 // 8 float64 / 16 float32 features, else 1.
@@ -628,12 +309,10 @@ constexpr int pde_rows() { return DMAX * (int)sizeof(T) <= 16 ? 4 : (DMAX * (int
 
 template <typename T, int DMAX, int KIND>
 static hipError_t launch_pde(const PdeFwdArgs& a, size_t lds, hipStream_t s) {
-  const bool e = a.edges != nullptr;
-  auto k = a.dyadic == 0   ? (e ? pde_fwd_aligned_kernel<T, DMAX, KIND, 0, true>
-                                : pde_fwd_aligned_kernel<T, DMAX, KIND, 0, false>)
-           : a.dyadic == 1 ? (e ? pde_fwd_aligned_kernel<T, DMAX, KIND, 1, true>
-                                : pde_fwd_aligned_kernel<T, DMAX, KIND, 1, false>)
-                           : pde_forward_kernel<T, DMAX, KIND, pde_rows<T, DMAX>()>;
+  if (a.dyadic <= 1)
+    return sizeof(T) == 4 ? pde_fwd_aligned_launch_f32(a, KIND == kGram ? 8 : DMAX, lds, s)
+                          : pde_fwd_aligned_launch_f64(a, KIND == kGram ? 8 : DMAX, lds, s);
+  auto k = pde_forward_kernel<T, DMAX, KIND, pde_rows<T, DMAX>()>;
   if (lds > 64 * 1024) {
     hipError_t e = hipFuncSetAttribute((const void*)k,
                                        hipFuncAttributeMaxDynamicSharedMemorySize,

@@ -40,6 +40,30 @@ def test_pde_shapes_float64(lx, ly, lam):
   assert normwise(got, ref) < 1e-11
 
 
+@pytest.mark.parametrize("d", [1, 4, 5, 8, 16, 20])
+@pytest.mark.parametrize("lam", [0, 1])
+@pytest.mark.parametrize("kind", ["rbf", "linear"])
+def test_pde_feature_buckets(d, lam, kind):
+  """Every feature bucket of the phase-aligned sweep, with 16-byte y-row loads
+  (d = 4, 8, 16) and per-feature loads (d = 1, 5, 20), in float64 and float32,
+  on pairs over one and several bands."""
+  rng = np.random.default_rng(100 * d + 10 * lam + (kind == "rbf"))
+  X = O.brownian(rng, 3, 150, d)
+  Y = O.brownian(rng, 2, 70, d)
+  static = (ksig.static.kernels.RBFKernel(1.3) if kind == "rbf"
+            else ksig.static.kernels.LinearKernel(0.7))
+  ref = O.signature_pde_kernel(X, Y, normalize=False, dyadic_order=lam, kind=kind,
+                               param=1.3 if kind == "rbf" else 0.7)
+  pk = ksig.kernels.SignaturePDEKernel(normalize=False, dyadic_order=lam, static_kernel=static)
+  assert normwise(pk(X, Y), ref) < 1e-11
+  got32 = pk(X.astype(np.float32), Y.astype(np.float32))
+  ref32 = O.signature_pde_kernel(X.astype(np.float32).astype(np.float64),
+                                 Y.astype(np.float32).astype(np.float64), normalize=False,
+                                 dyadic_order=lam, kind=kind,
+                                 param=1.3 if kind == "rbf" else 0.7)
+  assert normwise(got32, ref32) < 2e-5
+
+
 def test_pde_c2_shape_float32_subset():
   """C2 shapes (L = 256, d = 4, dyadic 1) on 4 x 3 pairs in float32."""
   rng = np.random.default_rng(2)
