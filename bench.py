@@ -249,7 +249,7 @@ def main():
                        "reference point for scaling HBM byte counters")
   ap.add_argument("--kernel-probe", default=None,
                   choices=["fwd_save", "adjoint", "contract", "pde_fwd", "pde_fwd_edges",
-                           "pde_adjoint", "pde_contract"],
+                           "pde_adjoint", "pde_contract", "pde_fused"],
                   help="only time one kernel of the C4 training step on one 16384-pair "
                        "chunk (128 x 128 pairs, L=128, d=32, 6 levels), --steps launches: "
                        "the unit that counter collection and the per-chunk costs refer to")
@@ -931,10 +931,21 @@ def pde_kernel_probe(args):
         0, *xa, *xa, d, pairs.data_ptr(), P, _lib.STATIC_RBF, 1.0, 1, gm.data_ptr(),
         m_off.data_ptr(), gx.data_ptr(), gx.data_ptr(), gparam.data_ptr(), st))
 
+  fwsb = lib.ksig_signature_pde_bwd_fused_workspace(0, d, L, L, lam, 1, P)
+  fws = torch.empty(max(fwsb // 4, 1), device=dev)
+  gpp = torch.empty(P, device=dev, dtype=torch.float64)
+
+  def fused():
+    _lib.check(lib.ksig_signature_pde_bwd_fused(
+        0, *xa, *xa, d, pairs.data_ptr(), P, _lib.STATIC_RBF, 1.0, lam, 1, gout.data_ptr(),
+        edges.data_ptr(), e_off.data_ptr(), gx.data_ptr(), gx.data_ptr(), gpp.data_ptr(),
+        fws.data_ptr(), fwsb, st))
+
   probes = {"pde_fwd": fwd, "pde_fwd_edges": fwd_edges, "pde_adjoint": adjoint,
-            "pde_contract": contract}
-  if args.kernel_probe in ("pde_adjoint", "pde_contract"):
+            "pde_contract": contract, "pde_fused": fused}
+  if args.kernel_probe in ("pde_adjoint", "pde_contract", "pde_fused"):
     fwd_edges()
+  if args.kernel_probe in ("pde_adjoint", "pde_contract"):
     adjoint()
   f = probes[args.kernel_probe]
   f()

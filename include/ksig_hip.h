@@ -366,6 +366,41 @@ size_t ksig_signature_pde_bwd_workspace(int dtype, int max_len_x, int max_len_y,
                                         int dyadic_order, int difference, int64_t n_pairs);
 
 /*
+ * Adjoint of ksig_signature_pde_fwd_edges fused with the chain rule through
+ * ΔΔ and the static kernel: the work of ksig_signature_pde_bwd_edges followed
+ * by ksig_signature_pde_grad_contract in one sweep, with no dL/dM array (no
+ * reference counterpart: ksig/algorithms.py:344-446 and the static kernels
+ * of ksig/static/kernels.py:134-166, 245-260 are forward-only).  Given
+ * grad_out[p] = dL/dK of pair p, ADDS to grad_x / grad_y (dtype, either may
+ * be NULL, both may alias) the gradients of the pairs' sequences and writes
+ * grad_param_pairs[p] (float64, may be NULL) = this pair's part of dL/dparam
+ * (RBF: sigma; linear: scale); the caller sums it over pairs.  Device atomics:
+ * the gradients are order-dependent in their last bits.  Built for float32,
+ * difference = 1, dyadic_order 0 or 1, 1 <= d <= 8, RBF or linear: other
+ * shapes return KSIG_EUNSUPPORTED, launching nothing (the two-call path
+ * takes them).  workspace: at least
+ * ksig_signature_pde_bwd_fused_workspace(...) bytes, 16-byte aligned.
+ */
+int ksig_signature_pde_bwd_fused(int dtype, const void* x, const int64_t* x_off,
+                                 const int32_t* x_len, int max_len_x, const void* y,
+                                 const int64_t* y_off, const int32_t* y_len,
+                                 int max_len_y, int d, const int32_t* pairs,
+                                 int64_t n_pairs, int static_kind, double static_param,
+                                 int dyadic_order, int difference, const double* grad_out,
+                                 const void* edges, const int64_t* e_off, void* grad_x,
+                                 void* grad_y, double* grad_param_pairs, void* workspace,
+                                 size_t workspace_bytes, void* stream);
+
+/*
+ * Workspace bytes of ksig_signature_pde_bwd_fused for n_pairs pairs of at
+ * most max_len_x x max_len_y points (0 for shapes it does not take): one band
+ * hand-over region per pair, contents scratch.  No reference counterpart.
+ */
+size_t ksig_signature_pde_bwd_fused_workspace(int dtype, int d, int max_len_x, int max_len_y,
+                                              int dyadic_order, int difference,
+                                              int64_t n_pairs);
+
+/*
  * Signature-PDE kernel on caller-supplied matrices M [n_mats, lx, ly].
  * Replaces signature_kern_pde(M, difference) of ksig/algorithms.py:402-446.
  */

@@ -73,6 +73,12 @@ SIGNATURES = {
         _c_i64, _c_int, _c_dbl, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _c_size, _vp]),
     "ksig_signature_pde_bwd_workspace": (_c_size, [
         _c_int, _c_int, _c_int, _c_int, _c_int, _c_i64]),
+    "ksig_signature_pde_bwd_fused": (_c_int, [
+        _c_int, _vp, _vp, _vp, _c_int, _vp, _vp, _vp, _c_int, _c_int, _vp,
+        _c_i64, _c_int, _c_dbl, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_size,
+        _vp]),
+    "ksig_signature_pde_bwd_fused_workspace": (_c_size, [
+        _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_i64]),
     "ksig_signature_pde_gram": (_c_int, [
         _c_int, _vp, _c_i64, _c_int, _c_int, _c_int, _c_int, _vp, _vp]),
     "ksig_global_align_log": (_c_int, [

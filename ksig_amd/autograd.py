@@ -210,6 +210,10 @@ def mmd2_unbiased(Kxx: torch.Tensor, Kyy: torch.Tensor, Kxy: torch.Tensor) -> to
 # -----------------------------------------------------------------------------
 
 _GM_BYTES = 1 << 33  # dL/dM scratch per chunk of pairs (8 GiB, as omega)
+# fused adjoint: at most this many pairs and this much FIFO workspace per
+# launch (0 pairs: always the two-call path, as a reference for tests)
+_FUSED_PAIRS = 1 << 30
+_FUSED_WS_BYTES = 1 << 32
 
 
 class _SigPDE(torch.autograd.Function):
@@ -265,6 +269,31 @@ class _SigPDE(torch.autograd.Function):
     gparam = torch.zeros((), device=dev, dtype=torch.float64)
     if n_pairs == 0:
       return gx, gy, gparam, None, None, None, None, None, None
+    stream = _lib.stream_handle(dev)
+    # the fused adjoint (chain rule in the sweep, no dL/dM array) where it is
+    # built: float32, difference, dyadic order 0 / 1, d <= 8
+    fused_pairs = _FUSED_PAIRS if dt == torch.float32 else 0
+    ws_pp = lib.ksig_signature_pde_bwd_fused_workspace(
+        _lib.dtype_code(dt), Xs.d, Xs.max_len, Ys.max_len, int(dyadic), int(difference), 1)
+    if ws_pp > 0 and fused_pairs > 0:
+      need_gp = ctx.needs_input_grad[2]
+      step = max(1, min(fused_pairs, _FUSED_WS_BYTES // ws_pp))
+      for c0 in range(0, n_pairs, step):
+        c1 = min(n_pairs, c0 + step)
+        P = c1 - c0
+        ws = torch.empty(max(ws_pp * P // 4, 1), device=dev, dtype=torch.float32)
+        gpp = torch.empty(P, device=dev, dtype=torch.float64) if need_gp else None
+        _lib.check(lib.ksig_signature_pde_bwd_fused(
+            _lib.dtype_code(dt), Xs.data.data_ptr(), Xs.off.data_ptr(), Xs.len.data_ptr(),
+            Xs.max_len, Ys.data.data_ptr(), Ys.off.data_ptr(), Ys.len.data_ptr(), Ys.max_len,
+            Xs.d, pairs[c0:c1].contiguous().data_ptr(), P, kind, pval, int(dyadic),
+            int(difference), gout[c0:c1].data_ptr(), edges.data_ptr(),
+            e_off[c0:c1].data_ptr(), None if gx is None else gx.data_ptr(),
+            None if gy is None else gy.data_ptr(), None if gpp is None else gpp.data_ptr(),
+            ws.data_ptr(), ws_pp * P, stream))
+        if need_gp:
+          gparam = gparam + gpp.sum()
+      return gx, gy, gparam, None, None, None, None, None, None
     sh = 1 if difference else 0
     nr = torch.clamp(torch.maximum(la, lb) - sh, min=0)   # kernel orientation
     nc = torch.clamp(torch.minimum(la, lb) - sh, min=0)
@@ -273,7 +302,6 @@ class _SigPDE(torch.autograd.Function):
     chunk_id = ((torch.cumsum(msz, 0) - msz) // budget).cpu()
     change = torch.nonzero(chunk_id[1:] != chunk_id[:-1]).flatten() + 1
     bounds = [0] + change.tolist() + [n_pairs]
-    stream = _lib.stream_handle(dev)
     for c0, c1 in zip(bounds[:-1], bounds[1:]):
       pc = pairs[c0:c1].contiguous()
       P = c1 - c0

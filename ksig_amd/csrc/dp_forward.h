@@ -163,6 +163,28 @@ struct PdeBwdEdgeArgs {
   int fifo_cols;
 };
 
+// Signature-PDE adjoint fused with the chain rule (pde_bwd_fused.hip):
+// gradients of the caller's x / y rows (either may be null), the parameter
+// part per pair, the band FIFO workspace.
+struct PdeBwdFusedArgs {
+  SeqSet x, y;
+  int d;
+  const int32_t* pairs;
+  int64_t n_pairs;
+  int kind;
+  double xscale, yscale;
+  int dyadic;
+  const double* gout;    // [n_pairs] dL/dK
+  const void* edges;     // as PdeFwdArgs::edges
+  const int64_t* e_off;
+  void* gx;
+  void* gy;
+  double* gp;            // [n_pairs] (may be null)
+  double c_cross, c_self, gp_scale;
+  void* fifo;            // [n_pairs][fifo_cols] slots of pde_fused_slot_bytes(d)
+  int fifo_cols;
+};
+
 int dp_flow_count(int nlev, int order);
 // fused VALU sweep for difference=1 (dp_sweep.h): LDS bytes for a column side of `cols`
 size_t dp_sweep_lds(int dtype, int d, int cols, int n_levels, int order);
@@ -220,6 +242,19 @@ __host__ __device__ inline int pde_bwd_aligned_pc(int nc, int lam) {
   const int lo = 64 * f + 32;
   return pc >= lo ? pc : (lo + f - 1) / f * f;
 }
+// Band pass of the fused adjoint: priming column, Nc columns and one more
+// (the static-kernel column 0), at least 64 * 2^lam + 32 steps (lane 63
+// writes a global-memory FIFO slot 2^lam + 30 steps before lane 0 reads it).
+__host__ __device__ inline int pde_fused_pc(int nc, int lam) {
+  const int f = 1 << lam;
+  int pc = (nc + 2) * f;
+  if (pc < 64 * f + 32) pc = 64 * f + 32;
+  return (pc + f - 1) / f * f;
+}
+int pde_fused_dmax(int d);  // 4 or 8, 0 when d > 8
+size_t pde_fused_lds(int dtype, int d);
+size_t pde_fused_slot_bytes(int d);
+hipError_t pde_bwd_fused_launch(const PdeBwdFusedArgs& a, int dtype, size_t lds, hipStream_t s);
 size_t pde_bwd_edges_lds(int dtype, int d, int ly, int ncr, int nrr, int dyadic);
 int pde_bwd_fifo_cols(int ncr, int dyadic);
 size_t pde_bwd_fifo_bytes(int dtype, int ncr, int dyadic);

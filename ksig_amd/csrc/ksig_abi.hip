@@ -898,6 +898,86 @@ int ksig_signature_pde_bwd_edges(int dtype, const void* x, const int64_t* x_off,
   return 0;
 }
 
+static bool pde_fused_takes(int dtype, int d, int dyadic_order, int difference,
+                            int static_kind) {
+  return dtype == KSIG_F32 && difference && dyadic_order >= 0 && dyadic_order <= 1 &&
+         ksig::pde_fused_dmax(d) > 0 &&
+         (static_kind == ksig::kRBF || static_kind == ksig::kLinear);
+}
+
+size_t ksig_signature_pde_bwd_fused_workspace(int dtype, int d, int max_len_x, int max_len_y,
+                                              int dyadic_order, int difference,
+                                              int64_t n_pairs) {
+  if (n_pairs <= 0 || d < 1 || !pde_fused_takes(dtype, d, dyadic_order, difference, ksig::kRBF))
+    return 0;
+  const int lo = max_len_x < max_len_y ? max_len_x : max_len_y;
+  const int nc = lo - 1 < 1 ? 1 : lo - 1;
+  return (size_t)n_pairs * ksig::pde_fused_pc(nc, dyadic_order) * ksig::pde_fused_slot_bytes(d);
+}
+
+int ksig_signature_pde_bwd_fused(int dtype, const void* x, const int64_t* x_off,
+                                 const int32_t* x_len, int max_len_x, const void* y,
+                                 const int64_t* y_off, const int32_t* y_len,
+                                 int max_len_y, int d, const int32_t* pairs,
+                                 int64_t n_pairs, int static_kind, double static_param,
+                                 int dyadic_order, int difference, const double* grad_out,
+                                 const void* edges, const int64_t* e_off, void* grad_x,
+                                 void* grad_y, double* grad_param_pairs, void* workspace,
+                                 size_t workspace_bytes, void* stream) {
+  if (dtype != KSIG_F32 && dtype != KSIG_F64) return fail(KSIG_EINVAL, "bad dtype %d", dtype);
+  if (n_pairs < 0 || n_pairs > 0x7fffffffLL) return fail(KSIG_EINVAL, "bad n_pairs");
+  if (d < 1) return fail(KSIG_EINVAL, "d must be >= 1 (got %d)", d);
+  ksig::PdeBwdFusedArgs a{};
+  int rc = static_scales(static_kind, static_param, &a.xscale, &a.yscale);
+  if (rc) return rc;
+  if (dyadic_order < 0 || dyadic_order > 6)
+    return fail(KSIG_EINVAL, "dyadic_order must be in [0, 6]");
+  if (!pde_fused_takes(dtype, d, dyadic_order, difference, static_kind))
+    return fail(KSIG_EUNSUPPORTED,
+                "fused PDE adjoint: float32, difference=1, dyadic order 0/1, d <= 8 only");
+  if (n_pairs == 0) return 0;
+  if (!x || !x_off || !x_len || !y || !y_off || !y_len || !pairs || !grad_out || !edges ||
+      !e_off)
+    return fail(KSIG_EINVAL, "null device pointer");
+  const size_t need = ksig_signature_pde_bwd_fused_workspace(dtype, d, max_len_x, max_len_y,
+                                                             dyadic_order, difference, n_pairs);
+  if (!workspace || workspace_bytes < need || ((uintptr_t)workspace & 15))
+    return fail(KSIG_EINVAL, "workspace of %zu bytes (16-byte aligned) required, got %zu", need,
+                workspace ? workspace_bytes : (size_t)0);
+  a.x = {x, x_off, x_len};
+  a.y = {y, y_off, y_len};
+  a.d = d;
+  a.pairs = pairs;
+  a.n_pairs = n_pairs;
+  a.kind = static_kind;
+  a.dyadic = dyadic_order;
+  a.gout = grad_out;
+  a.edges = edges;
+  a.e_off = e_off;
+  a.gx = grad_x;
+  a.gy = grad_y;
+  a.gp = grad_param_pairs;
+  if (static_kind == ksig::kRBF) {
+    const double den = 2.0 * static_param * static_param;
+    const double sc = 1.0 / (den > 1e-12 ? den : 1e-12);
+    a.c_cross = 2.0 * sc;
+    a.c_self = -2.0 * sc;
+    // d/dsigma of exp(-|x-y|^2 / (2 sigma^2)) = G |x-y|^2 / sigma^3 (0 once clamped)
+    a.gp_scale = den > 1e-12 ? 1.0 / (static_param * static_param * static_param) : 0.0;
+  } else {
+    a.c_cross = static_param;
+    a.c_self = 0.0;
+    a.gp_scale = 1.0;
+  }
+  const int lo = max_len_x < max_len_y ? max_len_x : max_len_y;
+  a.fifo = workspace;
+  a.fifo_cols = ksig::pde_fused_pc(lo - 1 < 1 ? 1 : lo - 1, dyadic_order);
+  const size_t lds = ksig::pde_fused_lds(dtype, d);
+  hipError_t e = ksig::pde_bwd_fused_launch(a, dtype, lds, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "pde_bwd_fused launch");
+  return 0;
+}
+
 int ksig_signature_pde_gram(int dtype, const void* M, int64_t n_mats, int lx,
                             int ly, int dyadic_order, int difference,
                             double* out, void* stream) {
