@@ -252,7 +252,7 @@ __host__ __device__ inline int pde_fused_pc(int nc, int lam) {
   return (pc + f - 1) / f * f;
 }
 int pde_fused_dmax(int d);  // 4 or 8, 0 when d > 8
-size_t pde_fused_lds(int dtype, int d);
+size_t pde_fused_lds(int dtype, int d, int dyadic);
 size_t pde_fused_slot_bytes(int d);
 hipError_t pde_bwd_fused_launch(const PdeBwdFusedArgs& a, int dtype, size_t lds, hipStream_t s);
 size_t pde_bwd_edges_lds(int dtype, int d, int ly, int ncr, int nrr, int dyadic);

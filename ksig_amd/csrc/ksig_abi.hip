@@ -972,7 +972,7 @@ int ksig_signature_pde_bwd_fused(int dtype, const void* x, const int64_t* x_off,
   const int lo = max_len_x < max_len_y ? max_len_x : max_len_y;
   a.fifo = workspace;
   a.fifo_cols = ksig::pde_fused_pc(lo - 1 < 1 ? 1 : lo - 1, dyadic_order);
-  const size_t lds = ksig::pde_fused_lds(dtype, d);
+  const size_t lds = ksig::pde_fused_lds(dtype, d, dyadic_order);
   hipError_t e = ksig::pde_bwd_fused_launch(a, dtype, lds, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "pde_bwd_fused launch");
   return 0;

@@ -51,6 +51,15 @@ namespace {
 template <int DMAX>
 constexpr int fused_nf() { return ((6 + DMAX) + 3) & ~3; }
 
+// LDS per wave: each lane's staged x rows (two bands of 2^(2-lam) + 1 rows of
+// dmax + 1 floats), then each lane's two bands of 4 float64 rebuild starts
+__host__ __device__ constexpr int fused_x_bytes(int dmax, int lam) {
+  return kWave * 2 * ((4 >> lam) + 1) * (dmax + 1) * 4;
+}
+__host__ __device__ constexpr int fused_lds_bytes(int dmax, int lam) {
+  return fused_x_bytes(dmax, lam) + kWave * 2 * 4 * 8;
+}
+
 template <typename T, int DMAX, int KIND, int LAM, bool VEC>
 __global__ void __launch_bounds__(kWave)
 pde_bwd_fused_kernel(const PdeBwdFusedArgs a) {
@@ -107,13 +116,14 @@ pde_bwd_fused_kernel(const PdeBwdFusedArgs a) {
   const int ystride = d * (int)sizeof(T);
   const __amdgpu_buffer_rsrc_t yrs = raw_rsrc(ys, unsigned((int64_t)Ly * ystride));
   T* xslot = reinterpret_cast<T*>(smem_raw) + lane * 2 * NX * XS;  // [2][NX][XS] by band parity
+  // U(r - 1, Ncr) of the lane's refined rows for its next band, by band parity
+  double* eslot = reinterpret_cast<double*>(smem_raw + fused_x_bytes(DMAX, LAM)) + lane * 2 * R;
 
   // ---- band staging --------------------------------------------------------
   // Band b: original row k of this lane is flipped row o = (64 b + lane) NO + k,
   // cell row i_k = Nr - 1 - o, upper x row i_k; slot row NO holds the lower
   // x row of row 0 (x_{i_0 + 1}).  U(r - 1, Ncr) of the refined rows (the
-  // rebuild's start) waits in registers for the lane's own band change.
-  double ecn[R];
+  // rebuild's start) waits in LDS for the lane's own band change.
   auto load_rows = [&](int b) __attribute__((always_inline)) {
     T* dst = xslot + (b & 1) * NX * XS;
     const int o0 = (b * kWave + lane) * NO;
@@ -132,13 +142,13 @@ pde_bwd_fused_kernel(const PdeBwdFusedArgs a) {
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       const int rq = Nrr - ((b * kWave + lane) * R + q);
-      ecn[q] = rq >= 2 ? ecol[rq - 2] : 1.0;
+      eslot[(b & 1) * R + q] = rq >= 2 ? ecol[rq - 2] : 1.0;
     }
   };
   load_rows(0);
   double up[R];  // U(r-1, c) of the lane's refined rows: the rebuild's right neighbour
 #pragma unroll
-  for (int q = 0; q < R; ++q) up[q] = ecn[q];
+  for (int q = 0; q < R; ++q) up[q] = eslot[q];
   load_rows(1);
 
   // FIFO initial contents: the row below the grid.  Slot s holds flipped
@@ -196,7 +206,8 @@ pde_bwd_fused_kernel(const PdeBwdFusedArgs a) {
   // the right y column (lower and upper x row) and one column further right
   // (the point the chain rule finishes), the scheme's coefficients
   T gacc[NO], gbr[NO], gtr[NO], gbo[NO], ca[NO], cb[NO];
-  double cad[NO], cbd[NO], cgd[NO], chd[NO], rbq[R], rmask[R];
+  double cad[NO], cbd[NO], cgd[NO], chd[NO], rbq[R];
+  float rmask[R];  // 1 for the refined rows whose upper neighbour is rebuilt
   T rowm[NO];  // 1 for rows inside the grid
 #pragma unroll
   for (int k = 0; k < NO; ++k) {
@@ -222,7 +233,7 @@ pde_bwd_fused_kernel(const PdeBwdFusedArgs a) {
     for (int k = 0; k < NO; ++k) rowm[k] = Nr - 1 - (o0 + k) >= 0 ? T(1) : T(0);
     const int r0 = (band * kWave + lane) * R;
 #pragma unroll
-    for (int q = 0; q < R; ++q) rmask[q] = Nrr - (r0 + q) >= 2 ? 1.0 : 0.0;
+    for (int q = 0; q < R; ++q) rmask[q] = Nrr - (r0 + q) >= 2 ? 1.f : 0.f;
   };
   set_masks();
 #pragma unroll
@@ -347,7 +358,7 @@ pde_bwd_fused_kernel(const PdeBwdFusedArgs a) {
         rinv = fma(rinv, fma(-bb, rinv, 1.0), rinv);
         rinv = fma(rinv, fma(-bb, rinv, 1.0), rinv);
 #pragma unroll
-        for (int f = 0; f < F; ++f) rbq[k * F + f] = rinv * rmask[k * F + f];
+        for (int f = 0; f < F; ++f) rbq[k * F + f] = rinv * (double)rmask[k * F + f];
         gb = gt;
       }
       flg = gb;
@@ -484,12 +495,12 @@ pde_bwd_fused_kernel(const PdeBwdFusedArgs a) {
             for (int e = 0; e < DMAX; ++e) swy[k][e] = T(0);
           }
           gmbp = T(0);
+          ++band;
 #pragma unroll
           for (int q = 0; q < R; ++q) {
             blb[q] = lamp[q] = alp[q] = T(0);
-            up[q] = ecn[q];
+            up[q] = eslot[(band & 1) * R + q];
           }
-          ++band;
           xpar = xpar == 0 ? NX * XS : 0;
           pcol = -1;
           set_masks();
@@ -574,10 +585,9 @@ hipError_t dispatch_fused(const PdeBwdFusedArgs& a, size_t lds, hipStream_t s) {
 
 int pde_fused_dmax(int d) { return d <= 4 ? 4 : (d <= 8 ? 8 : 0); }
 
-size_t pde_fused_lds(int dtype, int d) {
-  const size_t esz = dtype == KSIG_F32 ? 4 : 8;
-  const int dmax = pde_fused_dmax(d);
-  return (size_t)kWave * 2 * 5 * (dmax + 1) * esz;  // two bands of up to 5 staged x rows
+size_t pde_fused_lds(int dtype, int d, int dyadic) {
+  (void)dtype;  // float32 only
+  return fused_lds_bytes(pde_fused_dmax(d), dyadic);
 }
 
 size_t pde_fused_slot_bytes(int d) {
