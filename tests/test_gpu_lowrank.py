@@ -196,11 +196,12 @@ def test_nystroem_features():
   assert normwise(Ff @ Ff.T, O.static_gram(X[:20], X[:20], 'rbf', 1.5)) < 1e-8
 
 
-@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-9), (np.float32, 1e-2)])
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-9), (np.float32, 2e-3)])
 def test_signature_features_with_nystroem_and_trp(dtype, tol):
-  """RFSF with Nystroem static features.  float32 is loose: the factor
-  U_+ s_+^(-1/2) keeps eigenvalues down to 1e-12, so float32 rounding of
-  K(X, B) is scaled by s^(-1/2) before the time differences cancel; float64
+  """RFSF with Nystroem static features.  The factor U_+ s_+^(-1/2) and the
+  product K(X, B) @ factor are float64; float32 inputs still carry the
+  rounding of K(X, B), scaled by s^(-1/2) (eigenvalues down to 1e-12) before
+  the time differences cancel (2.6e-4 at worst over four seeds); float64
   pins the dataflow."""
   rng = np.random.default_rng(15)
   X = O.brownian(rng, 6, 20, 2).astype(dtype)
