@@ -357,8 +357,11 @@ def stream_schedule(lx: np.ndarray, ly: np.ndarray, slots: int, max_per_wave: in
   Pairs are ordered by their shorter side, longest first (a wave sweeps all
   its pairs with one column count), then cut into contiguous slices of about
   equal work (rows x columns), one slice per resident wave and round, each
-  slice at most ``max_per_wave`` pairs.  Returns (order, wave_beg) as int32
-  numpy arrays."""
+  slice at most ``max_per_wave`` pairs.  The slices are then listed by their
+  sweep length, longest first (the hardware starts waves in launch order and
+  hands each later slice to the first wave slot that frees up, so the slices
+  that set the critical path start at once and the short ones fill the
+  gaps).  Returns (order, wave_beg) as int32 numpy arrays."""
   lx = np.asarray(lx, dtype=np.int64)
   ly = np.asarray(ly, dtype=np.int64)
   n = lx.shape[0]
@@ -385,8 +388,18 @@ def stream_schedule(lx: np.ndarray, ly: np.ndarray, slots: int, max_per_wave: in
   part_r = np.repeat(parts, parts)
   t = np.arange(parts.sum()) - np.repeat(np.cumsum(parts) - parts, parts)
   starts = base + (t * size_r) // part_r
-  wave_beg = np.concatenate([starts, [n]]).astype(np.int32)
-  return order.astype(np.int32), wave_beg
+  wave_beg = np.concatenate([starts, [n]]).astype(np.int64)
+  if len(wave_beg) > 2:
+    # sweep steps of a slice ~ its padded rows x its widest pair's columns
+    so_rows = np.where(live, rows[order], 0)
+    so_cols = np.maximum(cols[order], min_cols)
+    steps = np.add.reduceat(so_rows, wave_beg[:-1]) * so_cols[wave_beg[:-1]]
+    perm = np.argsort(-steps, kind="stable")
+    sizes = np.diff(wave_beg)[perm]
+    idx = np.concatenate([np.arange(wave_beg[k], wave_beg[k + 1]) for k in perm])
+    order = order[idx]
+    wave_beg = np.concatenate([[0], np.cumsum(sizes)])
+  return order.astype(np.int32), wave_beg.astype(np.int32)
 
 
 def signature_levels_batched(X: SeqSet, Y: SeqSet, pairs: torch.Tensor, n_levels: int,

@@ -58,65 +58,110 @@ __device__ __forceinline__ T gram_rbf(T v, T hx, T hy) {
   return fast_exp2(v - hx - hy);
 }
 
-// One cell of the level recursion.  `in` holds the column-flowing inputs
-// (index 1..NLEV: column sums of S_m above this cell; 1+NLEV+..: column sums
-// of the E_i side terms), PS / HB the lane-local row sums.  Writes the
-// outgoing column values into out[1..] and updates PS / HB (reset at the
-// start of a row).
-template <typename T, int NLEV, int ORDER>
-__device__ __forceinline__ void dp_cell(const T M, const T* in, T* out, T* PS, T* HB,
-                                        const bool rowstart) {
-  T R[NLEV + 1][ORDER][ORDER];
-  R[1][0][0] = M;
+__device__ __forceinline__ float dp_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double dp_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// One cell of the level recursion in factored form.  `in` holds the
+// column-flowing inputs (index 1..NLEV: column sums of S_m above this cell;
+// 1+NLEV+..: column sums of the E_i side terms), PS / HB the lane-local row
+// sums (prefix of S_m along the row; the E_j side terms).  Writes the
+// outgoing column values into out[1..] and updates PS / HB through
+// upd(old, increment) (the row-start reset is the caller's).
+//
+// Every term of level m' = m + 1 is R[m'][r][s] = M / ((r+1)(s+1)) * src with
+//   src = PS_m (r = s = 0), column side sum E_i (r = 0), row side sum E_j
+//   (s = 0), or R[m][r-1][s-1] (ksig/algorithms.py:121-129),
+// so a cell forms the few scaled copies of M once, materialises only the
+// terms the next level reads (r, s < min(m'+1, ORDER) - 1), and folds every
+// other term into the sums that consume it as one fused multiply-add:
+// 9 operations per level at order 2 instead of a product per term plus a
+// product per constant plus the additions.
+template <typename T, int NLEV, int ORDER, typename Upd>
+__device__ __forceinline__ void dp_cell_core(const T M, const T* in, T* out, T* PS, T* HB,
+                                             Upd upd) {
+  constexpr int NPS = NLEV > 1 ? NLEV - 1 : 1;
+  constexpr int NSIDE = dp_num_side(NLEV, ORDER);
+  constexpr int NHB = NSIDE > 0 ? NSIDE : 1;
+  T ps[NPS], hb[NHB];  // row sums before this cell
 #pragma unroll
-  for (int m = 1; m < NLEV; ++m) {
-    const int dn = cmin(m + 1, ORDER);
-    R[m + 1][0][0] = M * PS[m - 1];
+  for (int k = 0; k < NPS; ++k) ps[k] = NLEV > 1 ? PS[k] : T(0);
 #pragma unroll
-    for (int s = 0; s < dn - 1; ++s)
-      R[m + 1][0][s + 1] = M * in[1 + NLEV + dp_side_off(m, ORDER) + s] * T(1.0 / (s + 2));
-#pragma unroll
-    for (int r = 0; r < dn - 1; ++r)
-      R[m + 1][r + 1][0] = M * HB[dp_side_off(m, ORDER) + r] * T(1.0 / (r + 2));
-#pragma unroll
-    for (int r = 1; r < dn; ++r)
-#pragma unroll
-      for (int s = 1; s < dn; ++s)
-        R[m + 1][r][s] = M * R[m][r - 1][s - 1] * T(1.0 / ((r + 1) * (s + 1)));
-  }
+  for (int k = 0; k < NHB; ++k) hb[k] = NSIDE > 0 ? HB[k] : T(0);
+  const int kNoMat = ORDER > 1 ? ORDER - 1 : 1;
+  T mat[NLEV + 1][kNoMat][kNoMat];  // materialised R[m][r][s] read by level m + 1
 #pragma unroll
   for (int m = 1; m <= NLEV; ++m) {
     const int dm = cmin(m, ORDER);
-    T S = R[m][0][0];
+    const int nnext = m < NLEV ? cmin(m + 1, ORDER) - 1 : 0;  // side sums / materialised extent
+    auto coef = [&](int r, int s) -> T {
+      return (r == 0 && s == 0) ? M : M * T(1.0 / ((r + 1) * (s + 1)));
+    };
+    auto src = [&](int r, int s) -> T {  // m >= 2
+      if (r == 0 && s == 0) return ps[m - 2];
+      if (r == 0) return in[1 + NLEV + dp_side_off(m - 1, ORDER) + s - 1];
+      if (s == 0) return hb[dp_side_off(m - 1, ORDER) + r - 1];
+      return mat[m - 1][r - 1][s - 1];
+    };
+    auto is_mat = [&](int r, int s) { return r < nnext && s < nnext; };
 #pragma unroll
-    for (int r = 0; r < dm; ++r)
+    for (int r = 0; r < nnext; ++r)
 #pragma unroll
-      for (int s = 0; s < dm; ++s)
-        if (r + s > 0) S += R[m][r][s];
-    out[m] = in[m] + S;
+      for (int s = 0; s < nnext; ++s) mat[m][r][s] = m == 1 ? M : coef(r, s) * src(r, s);
+    auto first = [&](int r, int s) -> T {
+      if (is_mat(r, s)) return mat[m][r][s];
+      return m == 1 ? M : coef(r, s) * src(r, s);
+    };
+    auto add = [&](T acc, int r, int s) -> T {
+      if (is_mat(r, s)) return acc + mat[m][r][s];
+      return m == 1 ? acc + M : dp_fma(coef(r, s), src(r, s), acc);
+    };
+    // side sums of this level: A_s = sum_q R[m][q][s], B_r = sum_q R[m][r][q]
+    T A[ORDER], B[ORDER];
+#pragma unroll
+    for (int s = 0; s < nnext; ++s) {
+      A[s] = first(0, s);
+#pragma unroll
+      for (int q = 1; q < dm; ++q) A[s] = add(A[s], q, s);
+    }
+#pragma unroll
+    for (int r = 0; r < nnext; ++r) {
+      B[r] = first(r, 0);
+#pragma unroll
+      for (int q = 1; q < dm; ++q) B[r] = add(B[r], r, q);
+    }
+    // level total: the column side sums plus the remaining columns
+    T o = in[m];
+#pragma unroll
+    for (int s = 0; s < nnext; ++s) o = o + A[s];
+#pragma unroll
+    for (int s = nnext; s < dm; ++s)
+#pragma unroll
+      for (int q = 0; q < dm; ++q) o = add(o, q, s);
+    out[m] = o;
     if (m < NLEV) {
-      const T ps = PS[m - 1] + in[m];
-      PS[m - 1] = rowstart ? T(0) : ps;
-      const int dn = cmin(m + 1, ORDER);
+      PS[m - 1] = upd(ps[m - 1], in[m]);
 #pragma unroll
-      for (int s = 0; s < dn - 1; ++s) {
-        T A = R[m][0][s];
-#pragma unroll
-        for (int q = 1; q < dm; ++q) A += R[m][q][s];
+      for (int s = 0; s < nnext; ++s) {
         const int k = 1 + NLEV + dp_side_off(m, ORDER) + s;
-        out[k] = in[k] + A;
+        out[k] = in[k] + A[s];
       }
 #pragma unroll
-      for (int r = 0; r < dn - 1; ++r) {
-        T B = R[m][r][0];
-#pragma unroll
-        for (int q = 1; q < dm; ++q) B += R[m][r][q];
+      for (int r = 0; r < nnext; ++r) {
         const int k = dp_side_off(m, ORDER) + r;
-        const T hb = HB[k] + B;
-        HB[k] = rowstart ? T(0) : hb;
+        HB[k] = upd(hb[k], B[r]);
       }
     }
   }
+}
+
+// One cell with the row sums reset to zero at the start of a row.
+template <typename T, int NLEV, int ORDER>
+__device__ __forceinline__ void dp_cell(const T M, const T* in, T* out, T* PS, T* HB,
+                                        const bool rowstart) {
+  dp_cell_core<T, NLEV, ORDER>(M, in, out, PS, HB, [&](T old, T inc) {
+    const T v = old + inc;
+    return rowstart ? T(0) : v;
+  });
 }
 
 template <typename T, int NLEV, int ORDER, int DMAX, bool GRAM>

@@ -325,6 +325,17 @@ dp_stream_kernel(const DpStreamArgs a) {
   int s63 = Pc - (kWave - 1); // lane 63's FIFO slot
   int next_stage = kWave - 1; // band b+1 is staged at step b*Pc + 63
   int stage_band = 1;
+  // Issue priority from the wave's remaining steps, renewed once per band:
+  // on a SIMD shared with short slices, the waves that hold the longest
+  // pairs (the critical path of an uneven batch) issue first, and short
+  // slices fill the gaps.  Results do not depend on it.
+  auto set_prio = [&](int remaining) {  // wave-uniform
+    if (remaining >= 4096) __builtin_amdgcn_s_setprio(3);
+    else if (remaining >= 2048) __builtin_amdgcn_s_setprio(2);
+    else if (remaining >= 1024) __builtin_amdgcn_s_setprio(1);
+    else __builtin_amdgcn_s_setprio(0);
+  };
+  set_prio(total);
 
   // Prefetch cursor, two columns ahead of cp: column a_c of the sequence at
   // a_ptr (last column a_cl); it moves to the next band's sequence when a_c
@@ -377,6 +388,7 @@ dp_stream_kernel(const DpStreamArgs a) {
       if (stage_band < nb) stage(stage_band);
       ++stage_band;
       next_stage += Pc;
+      set_prio(total - T_);
     }
     T in[NF];
     in[0] = shift_down(fl[0], fv[0]);

@@ -39,55 +39,8 @@ namespace ksig {
 template <typename T, int NLEV, int ORDER>
 __device__ __forceinline__ void dp_cell_keep(const T M, const T* in, T* out, T* PS, T* HB,
                                              const T keepf) {
-  T R[NLEV + 1][ORDER][ORDER];
-  R[1][0][0] = M;
-#pragma unroll
-  for (int m = 1; m < NLEV; ++m) {
-    const int dn = cmin(m + 1, ORDER);
-    R[m + 1][0][0] = M * PS[m - 1];
-#pragma unroll
-    for (int s = 0; s < dn - 1; ++s)
-      R[m + 1][0][s + 1] = M * in[1 + NLEV + dp_side_off(m, ORDER) + s] * T(1.0 / (s + 2));
-#pragma unroll
-    for (int r = 0; r < dn - 1; ++r)
-      R[m + 1][r + 1][0] = M * HB[dp_side_off(m, ORDER) + r] * T(1.0 / (r + 2));
-#pragma unroll
-    for (int r = 1; r < dn; ++r)
-#pragma unroll
-      for (int s = 1; s < dn; ++s)
-        R[m + 1][r][s] = M * R[m][r - 1][s - 1] * T(1.0 / ((r + 1) * (s + 1)));
-  }
-#pragma unroll
-  for (int m = 1; m <= NLEV; ++m) {
-    const int dm = cmin(m, ORDER);
-    T S = R[m][0][0];
-#pragma unroll
-    for (int r = 0; r < dm; ++r)
-#pragma unroll
-      for (int s = 0; s < dm; ++s)
-        if (r + s > 0) S += R[m][r][s];
-    out[m] = in[m] + S;
-    if (m < NLEV) {
-      PS[m - 1] = PS[m - 1] * keepf + in[m];
-      const int dn = cmin(m + 1, ORDER);
-#pragma unroll
-      for (int s = 0; s < dn - 1; ++s) {
-        T A = R[m][0][s];
-#pragma unroll
-        for (int q = 1; q < dm; ++q) A += R[m][q][s];
-        const int k = 1 + NLEV + dp_side_off(m, ORDER) + s;
-        out[k] = in[k] + A;
-      }
-#pragma unroll
-      for (int r = 0; r < dn - 1; ++r) {
-        T B = R[m][r][0];
-#pragma unroll
-        for (int q = 1; q < dm; ++q) B += R[m][r][q];
-        const int k = dp_side_off(m, ORDER) + r;
-        HB[k] = HB[k] * keepf + B;
-      }
-    }
-  }
+  dp_cell_core<T, NLEV, ORDER>(M, in, out, PS, HB,
+                               [&](T old, T inc) { return dp_fma(old, keepf, inc); });
 }
 
 template <typename T, int DMAX>

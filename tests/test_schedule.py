@@ -15,7 +15,13 @@ def _check(lx, ly, slots, maxp):
   sizes = np.diff(wb)
   assert sizes.min() >= 1 and sizes.max() <= maxp
   cols = np.minimum(lx, ly)[order]
-  assert np.all(np.diff(cols) <= 0)  # shorter side, longest first
+  rows = np.maximum(lx, ly)[order]
+  for b, e in zip(wb[:-1], wb[1:]):
+    assert np.all(np.diff(cols[b:e]) <= 0)  # shorter side, longest first
+  # slices listed by sweep length (rows x widest pair's columns), longest first
+  steps = [int(np.sum(np.where(cols[b:e] >= 2, rows[b:e], 0))) * max(int(cols[b]), 68)
+           for b, e in zip(wb[:-1], wb[1:])]
+  assert all(steps[k] >= steps[k + 1] for k in range(len(steps) - 1))
   return order, wb
 
 
