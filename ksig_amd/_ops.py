@@ -350,8 +350,17 @@ def stream_slots(X: SeqSet, Y: SeqSet, kind: int, n_levels: int, order: int,
   return s.value, m.value, w.value
 
 
+# Two-wave slices (global FIFO): a slice whose sweep is longer than
+# TWO_WAVE_FACTOR x the mean work per resident wave, with at least
+# TWO_WAVE_MIN_COLS columns and TWO_WAVE_MIN_ROWS rows, is split band by band
+# between the two waves of one block.
+TWO_WAVE_FACTOR = 1.25
+TWO_WAVE_MIN_COLS = 384
+TWO_WAVE_MIN_ROWS = 512
+
+
 def stream_schedule(lx: np.ndarray, ly: np.ndarray, slots: int, max_per_wave: int,
-                    min_cols: int = 68):
+                    min_cols: int = 68, two_wave: bool = False, resident: int = 0):
   """Length-bucketed schedule of a pair list for the row stream (host side).
 
   Pairs are ordered by their shorter side, longest first (a wave sweeps all
@@ -361,7 +370,15 @@ def stream_schedule(lx: np.ndarray, ly: np.ndarray, slots: int, max_per_wave: in
   sweep length, longest first (the hardware starts waves in launch order and
   hands each later slice to the first wave slot that frees up, so the slices
   that set the critical path start at once and the short ones fill the
-  gaps).  Returns (order, wave_beg) as int32 numpy arrays."""
+  gaps).
+
+  ``two_wave`` (global-FIFO launches, two waves per block: block b runs slices
+  2b and 2b + 1): the longest slices become two-wave slices, each followed by
+  an empty slice that marks it, so that the block's second wave sweeps every
+  other band of it; the remaining slices are paired into blocks in the same
+  longest-first order, and blocks are listed by their sweep length.
+  ``resident`` is the resident wave count the mean work is taken over
+  (default ``slots``).  Returns (order, wave_beg) as int32 numpy arrays."""
   lx = np.asarray(lx, dtype=np.int64)
   ly = np.asarray(ly, dtype=np.int64)
   n = lx.shape[0]
@@ -393,10 +410,26 @@ def stream_schedule(lx: np.ndarray, ly: np.ndarray, slots: int, max_per_wave: in
     # sweep steps of a slice ~ its padded rows x its widest pair's columns
     so_rows = np.where(live, rows[order], 0)
     so_cols = np.maximum(cols[order], min_cols)
-    steps = np.add.reduceat(so_rows, wave_beg[:-1]) * so_cols[wave_beg[:-1]]
+    slice_rows = np.add.reduceat(so_rows, wave_beg[:-1])
+    slice_cols = so_cols[wave_beg[:-1]]
+    steps = slice_rows * slice_cols
     perm = np.argsort(-steps, kind="stable")
-    sizes = np.diff(wave_beg)[perm]
-    idx = np.concatenate([np.arange(wave_beg[k], wave_beg[k + 1]) for k in perm])
+    # (block length, slices) with None for the empty slice of a two-wave block
+    tail = []
+    if two_wave:
+      mean = steps.sum() / max(int(resident) if resident > 0 else slots, 1)
+      tw = ((steps[perm] > TWO_WAVE_FACTOR * mean)
+            & (slice_cols[perm] >= TWO_WAVE_MIN_COLS) & (slice_rows[perm] >= TWO_WAVE_MIN_ROWS))
+      solo = [int(k) for k, t in zip(perm, tw) if not t]
+      blocks = [(steps[k] / 2, [int(k), None]) for k, t in zip(perm, tw) if t]
+      blocks += [(steps[solo[i]], solo[i:i + 2]) for i in range(0, len(solo) - 1, 2)]
+      tail = solo[len(solo) - len(solo) % 2:]  # an unpaired slice goes last
+      blocks.sort(key=lambda b: -b[0])
+    else:
+      blocks = [(steps[k], [int(k)]) for k in perm]
+    seq = [k for _, ks in blocks for k in ks] + tail
+    idx = np.concatenate([np.arange(wave_beg[k], wave_beg[k + 1]) for k in seq if k is not None])
+    sizes = np.array([0 if k is None else wave_beg[k + 1] - wave_beg[k] for k in seq])
     order = order[idx]
     wave_beg = np.concatenate([[0], np.cumsum(sizes)])
   return order.astype(np.int32), wave_beg.astype(np.int32)
@@ -442,7 +475,7 @@ def signature_levels_batched(X: SeqSet, Y: SeqSet, pairs: torch.Tensor, n_levels
 
 def make_schedule(X: SeqSet, Y: SeqSet, pairs: torch.Tensor, kind: int, n_levels: int,
                   order: int, global_fifo: Optional[bool] = None, max_slice: int = 0,
-                  oversub: int = 1):
+                  oversub: int = 1, two_wave: Optional[bool] = None):
   """(order, wave_beg, max_slice, global_fifo) of the length-bucketed schedule
   (device int32 tensors, the largest slice and the FIFO placement).
 
@@ -450,7 +483,9 @@ def make_schedule(X: SeqSet, Y: SeqSet, pairs: torch.Tensor, kind: int, n_levels
   ``max_slice`` (0 = the kernel's 128) bounds the pairs of one wave, which
   sizes its LDS pair table; ``oversub`` cuts that many slices per resident
   wave (the hardware hands the later, lighter slices to waves that finish
-  early).  The autotuner (ksig_amd.tune) searches these three."""
+  early); ``two_wave`` (global FIFO only; None = on with it) splits the
+  longest slices between the two waves of a block (see ``stream_schedule``).
+  The autotuner (ksig_amd.tune) searches these four."""
   norm_order = n_levels if (order <= 0 or order >= n_levels) else order
   if global_fifo is None:
     slots, maxp, _ = stream_slots(X, Y, kind, n_levels, norm_order, max_slice)
@@ -469,7 +504,9 @@ def make_schedule(X: SeqSet, Y: SeqSet, pairs: torch.Tensor, kind: int, n_levels
   lx = X.lengths()[pr[:, 0]]
   ly = Y.lengths()[pr[:, 1]]
   so, wb = stream_schedule(lx, ly, slots * max(int(oversub), 1), maxp,
-                           min_cols=96 if global_fifo else 68)
+                           min_cols=96 if global_fifo else 68,
+                           two_wave=(two_wave is None or bool(two_wave)) and bool(global_fifo),
+                           resident=slots)
   dev = X.data.device
   return (torch.as_tensor(so, device=dev), torch.as_tensor(wb, device=dev),
           int(np.diff(wb).max()) if len(wb) > 1 else 1, bool(global_fifo))

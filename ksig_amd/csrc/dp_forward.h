@@ -73,8 +73,9 @@ struct DpStreamArgs {
   int64_t n_pairs;
   const int32_t* order;    // processing order of the pair list (null: identity)
   const int32_t* wave_beg; // [n_waves + 1] or null
-  int n_waves;
+  int n_waves;             // slices; block b runs slices 2b and 2b + 1
   int per_wave;
+  int lds_region;          // dynamic LDS bytes of one wave (16-byte multiple)
   float gdot, gn;          // static-kernel factors folded into the row side
   double* out;             // [n_pairs, n_levels + 1]
   // training forward (order 1): the adjoint's boundary state, as in DpFwdArgs
@@ -220,9 +221,11 @@ StreamEntry dp_stream_entry(int kind, int dmax, bool vec, bool global_fifo, int 
 // training forward (order 1, vector loads, LDS FIFO) that also writes the
 // adjoint's boundary state
 StreamEntry dp_stream_save_entry(int kind, int dmax, int n_levels);
-hipError_t dp_stream_launch(const StreamEntry& e, const DpStreamArgs& a, size_t lds,
+// wpb: waves per block (1, or 2 with the global FIFO: two-wave slices); lds is
+// the block's dynamic LDS; the occupancy is in waves per CU
+hipError_t dp_stream_launch(const StreamEntry& e, const DpStreamArgs& a, size_t lds, int wpb,
                             hipStream_t s);
-int dp_stream_occupancy(const StreamEntry& e, size_t lds);
+int dp_stream_occupancy(const StreamEntry& e, size_t lds, int wpb);
 hipError_t pde_forward_launch(const PdeFwdArgs& a, int dtype, size_t lds, hipStream_t s);
 // FIFO slots (band pass length) of the phase-aligned PDE forward (dyadic
 // order 0 and 1) for nc original columns: a multiple of 2^lam, at least

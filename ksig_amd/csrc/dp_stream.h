@@ -66,6 +66,22 @@ constexpr int kStreamMaxPairs = 128;       // pair-table entries per wave
 constexpr int kStreamMinCols = 68;         // smallest band width Pc (FIFO hazard + staging)
 constexpr int kStreamMinColsGlobal = 96;   // same with the FIFO in global memory
 
+// Two-wave slices (global FIFO only): the waves of a block split one slice's
+// bands, wave 0 the even and wave 1 the odd ones, and hand the band FIFO to
+// each other through global memory.  Each publishes its step count in LDS
+// every kStreamCoopSync steps (release) and reads the other's (acquire):
+// wave 1 stays at least kStreamCoopSync + 65 steps behind wave 0 (it reads
+// what wave 0's lane 63 stored 63 steps after its own lane 0, two steps
+// ahead), wave 0 at most Pc - kStreamCoopSync - 65 steps ahead of wave 1 (the
+// same for the next band).  Both can wait at once only if
+// Pc < 2 kStreamCoopSync + 130, so a two-wave band pass has at least
+// kStreamCoopMinCols slots.  A wait is bounded; past the bound the wave stops
+// waiting and writes NaN for its pairs instead of values.
+constexpr int kStreamCoopSync = 32;
+constexpr int kStreamCoopMinCols = 224;
+constexpr int kStreamCoopSpinLimit = 1 << 22;
+static_assert(kStreamCoopMinCols >= 2 * kStreamCoopSync + 130 + 2, "two-wave deadlock bound");
+
 // Rows per lane of an instantiation: 4 when the per-row registers (x row,
 // norm, row prefixes, side sums, left value) stay small, else 2.
 __host__ __device__ constexpr int stream_rows(int nlev, int order, int dmax) {
@@ -79,6 +95,15 @@ __host__ __device__ inline size_t stream_lds_bytes(int dmax, int r, int pc, int 
   const int xs = ((dmax / 4) % 2 == 0) ? dmax + 4 : dmax + 8;
   return (size_t)4 * kWave * r * xs + (global_fifo ? 0 : (size_t)4 * ((pc * nf + 1) & ~1)) +
          4 * ((size_t)cap + 4) + sizeof(StreamPair) * cap;
+}
+
+// Dynamic LDS of a block of `wpb` waves: one region of `per_wave` bytes
+// (rounded to 16) per wave and the two-wave slice's published step counts.
+__host__ __device__ inline size_t stream_region_bytes(size_t per_wave) {
+  return (per_wave + 15) & ~(size_t)15;
+}
+__host__ __device__ inline size_t stream_block_lds_bytes(size_t per_wave, int wpb) {
+  return wpb * stream_region_bytes(per_wave) + 16;
 }
 
 // Global FIFO workspace bytes: every wave owns [pc][nf rounded up to 4] floats.
@@ -109,7 +134,7 @@ __device__ __forceinline__ void stream_load_row(float* v, const float* p, int d)
 }
 
 template <int NLEV, int ORDER, int DMAX, int R, int KIND, bool VEC, bool GF, bool SAVE>
-__global__ void __launch_bounds__(kWave)
+__global__ void __launch_bounds__(2 * kWave)
 dp_stream_kernel(const DpStreamArgs a) {
   using T = float;
   constexpr int NSIDE = dp_num_side(NLEV, ORDER);
@@ -125,23 +150,37 @@ dp_stream_kernel(const DpStreamArgs a) {
   static_assert(!SAVE || ORDER == 1, "the adjoint boundary state exists for order 1");
 
   extern __shared__ __align__(16) unsigned char smem_raw[];
+  // one or two waves per block (two with the global FIFO), each with its own
+  // LDS region; the published step counts of a two-wave slice follow them
+  const int wpb = blockDim.x >> 6;
+  const int v = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+  const int lane = threadIdx.x & (kWave - 1);
+  int* prog = reinterpret_cast<int*>(smem_raw + wpb * a.lds_region);
+  if (threadIdx.x < 2) prog[threadIdx.x] = 0;
+  __syncthreads();  // the only block-wide barrier: before any wave can leave
+  unsigned char* wsm = smem_raw + ((v * a.lds_region) & ~15);  // regions are 16-byte multiples
   const int cap = a.tab_cap;  // pair-table entries (a multiple of 4)
-  T* xst = reinterpret_cast<T*>(smem_raw);            // [RB][XS], 16-byte rows
+  T* xst = reinterpret_cast<T*>(wsm);                 // [RB][XS], 16-byte rows
   T* fifo = xst + RB * XS;                            // [Pc][NFP] (LDS FIFO)
   // FIFO rounded to an even word count and cap % 4 == 0: the table is 8-byte aligned
   const int lds_fifo_words = GF ? 0 : ((a.fifo_cols * NFP + 1) & ~1);
   int32_t* tstart = reinterpret_cast<int32_t*>(fifo + lds_fifo_words);
   StreamPair* tab = reinterpret_cast<StreamPair*>(tstart + cap + 4);
 
-  const int lane = threadIdx.x;
-  const int w = blockIdx.x;
+  const int w0 = wpb * blockIdx.x;
+  const int w = w0 + v;  // this wave's slot (its FIFO region)
   const int d = a.d;
+  // a two-wave slice: slot w0 holds the slice and slot w0 + 1 is empty
+  bool coop = GF && !SAVE && wpb == 2 && a.wave_beg && w0 + 1 < a.n_waves &&
+              a.wave_beg[w0 + 1] == a.wave_beg[w0 + 2] && a.wave_beg[w0] < a.wave_beg[w0 + 1];
+  const int sw = coop ? w0 : w;  // the slice this wave sweeps
+  if (sw >= a.n_waves) return;
   int kb, ke;
   if (a.wave_beg) {
-    kb = a.wave_beg[w];
-    ke = a.wave_beg[w + 1];
+    kb = a.wave_beg[sw];
+    ke = a.wave_beg[sw + 1];
   } else {
-    kb = w * a.per_wave;
+    kb = sw * a.per_wave;
     ke = kb + a.per_wave;
     ke = ke < a.n_pairs ? ke : (int)a.n_pairs;
   }
@@ -196,13 +235,32 @@ dp_stream_kernel(const DpStreamArgs a) {
   // wave-uniform values live in scalar registers
   const int total_rows = __builtin_amdgcn_readfirstlane(carry);
   if (lane == 0) tstart[nk] = total_rows;
-  __syncthreads();
+  // the table is wave-private: a wave-level barrier orders its LDS writes
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   if (total_rows == 0) return;
   constexpr int kMinCols = GF ? kStreamMinColsGlobal : kStreamMinCols;
-  const int Pc = __builtin_amdgcn_readfirstlane(pcl > kMinCols ? pcl : kMinCols);
-  // global FIFO: this wave's [fifo_cols][NFG] region of the workspace
-  T* gfifo = GF ? reinterpret_cast<T*>(a.gfifo) + (size_t)w * a.fifo_cols * NFG : nullptr;
+  int Pc = __builtin_amdgcn_readfirstlane(pcl > kMinCols ? pcl : kMinCols);
   const int nb = (total_rows + RB - 1) / RB;
+  if (coop) {  // both waves take the same decision from the same table
+    const int pcc = Pc > kStreamCoopMinCols ? Pc : kStreamCoopMinCols;
+    if (pcc <= a.fifo_cols && nb >= 2) {
+      Pc = pcc;
+    } else {  // the first wave sweeps the slice alone
+      coop = false;
+      if (v == 1) return;
+    }
+  }
+  // bands of this wave: all of them, or every other one of a two-wave slice
+  const int nbw = coop ? (nb - v + 1) / 2 : nb;
+  // global FIFO: this wave's [fifo_cols][NFG] region of the workspace (lane
+  // 63 stores), and the region lane 0 loads (the other wave's when two share
+  // the slice)
+  T* gfifo = GF ? reinterpret_cast<T*>(a.gfifo) + (size_t)w * a.fifo_cols * NFG : nullptr;
+  T* gfifo_in =
+      GF ? reinterpret_cast<T*>(a.gfifo) + (size_t)(coop ? w0 + 1 - v : w) * a.fifo_cols * NFG
+         : nullptr;
 
   const T gdot = a.gdot, gn = a.gn;
   const T* xdata = reinterpret_cast<const T*>(a.x.data);
@@ -278,7 +336,7 @@ dp_stream_kernel(const DpStreamArgs a) {
     ccl = pccl;
     q = pq;
     prim = pprim;
-    last = plast && lband < nb;
+    last = plast && lband < nbw;
     if (SAVE) {
       sv = psv;
       sr0 = psr0;
@@ -293,7 +351,7 @@ dp_stream_kernel(const DpStreamArgs a) {
     }
   };
 
-  stage(0);
+  stage(coop ? v : 0);
   enter_band();
 
   T fl[NF];
@@ -320,7 +378,7 @@ dp_stream_kernel(const DpStreamArgs a) {
   }
 
   int cp = -lane;             // this lane's column in its band (< 0: not started)
-  const int total = nb * Pc + kWave - 1;
+  const int total = nbw * Pc + kWave - 1;
   int s0 = 0;                 // lane 0's FIFO slot (= its column)
   int s63 = Pc - (kWave - 1); // lane 63's FIFO slot
   int next_stage = kWave - 1; // band b+1 is staged at step b*Pc + 63
@@ -366,7 +424,7 @@ dp_stream_kernel(const DpStreamArgs a) {
       wait_vmcnt<kFifoWait>();
 #pragma unroll
       for (int k = 0; k < NFG; k += 4) {
-        const float4 t = *reinterpret_cast<const float4*>(gfifo + s2 * NFG + k);
+        const float4 t = *reinterpret_cast<const float4*>(gfifo_in + s2 * NFG + k);
         dst[k] = t.x;
         if (k + 1 < NF) dst[k + 1] = t.y;
         if (k + 2 < NF) dst[k + 2] = t.z;
@@ -380,12 +438,32 @@ dp_stream_kernel(const DpStreamArgs a) {
     s2 = s2 == Pc ? 0 : s2;
   };
 
+  // Two-wave slice hand-over: publish this wave's step count t (its FIFO
+  // stores of steps < t complete first: release), then wait until the other
+  // wave's published count covers the FIFO slots this wave loads in its next
+  // kStreamCoopSync steps (two of them prefetched) and no longer reads a slot
+  // this wave is about to overwrite.
+  bool bail = false;
+  auto coop_sync = [&](int t) {
+    __hip_atomic_store(&prog[v], t, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    const int need = v == 0 ? t + kStreamCoopSync + 65 - Pc : t + kStreamCoopSync + 65;
+    int spins = 0;
+    while (!bail) {
+      const int got = __builtin_amdgcn_readfirstlane(
+          __hip_atomic_load(&prog[1 - v], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
+      if (got >= need) break;
+      __builtin_amdgcn_s_sleep(2);
+      if (++spins >= kStreamCoopSpinLimit) bail = true;
+    }
+  };
+
   // Two register sets per prefetched stream: a step consumes its set, then
   // refills it with the data of the step after next.
   T y0[DMAX], y1[DMAX], f0[NFG], f1[NFG];
   auto step = [&](int T_, T* yv, T* fv) {
+    if (coop && T_ > 0 && (T_ & (kStreamCoopSync - 1)) == 0) coop_sync(T_);  // wave-uniform
     if (T_ == next_stage) {  // wave-uniform
-      if (stage_band < nb) stage(stage_band);
+      if (stage_band < nbw) stage(coop ? v + 2 * stage_band : stage_band);
       ++stage_band;
       next_stage += Pc;
       set_prio(total - T_);
@@ -499,7 +577,7 @@ dp_stream_kernel(const DpStreamArgs a) {
       }
       if (last) {
         double* o = a.out + (int64_t)q * (NLEV + 1);
-        o[0] = 1.0;
+        o[0] = bail ? __builtin_nan("") : 1.0;
 #pragma unroll
         for (int m = 0; m < NLEV; ++m) o[m + 1] = acc[m] + (double)accf[m];
       }
@@ -522,6 +600,8 @@ dp_stream_kernel(const DpStreamArgs a) {
   yload(y0);
   yload(y1);
   // lane 0 starts band 0 with a priming row: slots 0 and 1 are read but unused
+  // (a two-wave slice's second wave waits for its first FIFO slots here)
+  if (coop) coop_sync(0);
   s2 = 0;
   fload(f0);
   fload(f1);
@@ -531,6 +611,8 @@ dp_stream_kernel(const DpStreamArgs a) {
     step(T_ + 1, y1, f1);
   }
   if (T_ < total) step(T_, y0, f0);
+  if (coop)  // done: the other wave no longer waits for this one
+    __hip_atomic_store(&prog[v], 0x7fffffff, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 template <int NLEV, int ORDER, int DMAX, int KIND, bool VEC, bool GF, bool SAVE = false>

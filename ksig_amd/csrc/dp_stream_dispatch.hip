@@ -63,7 +63,7 @@ StreamEntry dp_stream_entry(int kind, int dmax, bool vec, bool global_fifo, int 
   return StreamEntry{nullptr, 0};
 }
 
-hipError_t dp_stream_launch(const StreamEntry& e, const DpStreamArgs& a, size_t lds,
+hipError_t dp_stream_launch(const StreamEntry& e, const DpStreamArgs& a, size_t lds, int wpb,
                             hipStream_t s) {
   if (!e.fn) return hipErrorInvalidValue;
   if (lds > 64 * 1024) {
@@ -73,19 +73,20 @@ hipError_t dp_stream_launch(const StreamEntry& e, const DpStreamArgs& a, size_t 
   }
   DpStreamArgs copy = a;
   void* args[] = {&copy};
-  return hipLaunchKernel(e.fn, dim3((unsigned)a.n_waves), dim3(kWave), args, lds, s);
+  const unsigned blocks = (unsigned)((a.n_waves + wpb - 1) / wpb);
+  return hipLaunchKernel(e.fn, dim3(blocks), dim3(kWave * wpb), args, lds, s);
 }
 
-int dp_stream_occupancy(const StreamEntry& e, size_t lds) {
+int dp_stream_occupancy(const StreamEntry& e, size_t lds, int wpb) {
   if (!e.fn) return 0;
   if (lds > 64 * 1024 &&
       hipFuncSetAttribute(e.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
           hipSuccess)
     return 0;
   int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, e.fn, kWave, lds) != hipSuccess)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, e.fn, kWave * wpb, lds) != hipSuccess)
     return 0;
-  return nb;
+  return nb * wpb;
 }
 
 }  // namespace ksig

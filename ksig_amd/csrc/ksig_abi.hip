@@ -61,17 +61,18 @@ int device_cus() {
   return v > 0 ? v : 1;
 }
 
-// Resident row-stream waves per CU for a (kernel, LDS size), cached.
-int stream_occupancy(const ksig::StreamEntry& e, size_t lds) {
-  struct Slot { const void* fn; size_t lds; int occ; };
+// Resident row-stream waves per CU for a (kernel, block LDS size, waves per
+// block), cached.
+int stream_occupancy(const ksig::StreamEntry& e, size_t lds, int wpb) {
+  struct Slot { const void* fn; size_t lds; int wpb; int occ; };
   static Slot cache[256];
   static int used = 0;
   static std::mutex mu;
   std::lock_guard<std::mutex> g(mu);
   for (int i = 0; i < used; ++i)
-    if (cache[i].fn == e.fn && cache[i].lds == lds) return cache[i].occ;
-  const int occ = ksig::dp_stream_occupancy(e, lds);
-  if (used < 256) cache[used++] = Slot{e.fn, lds, occ};
+    if (cache[i].fn == e.fn && cache[i].lds == lds && cache[i].wpb == wpb) return cache[i].occ;
+  const int occ = ksig::dp_stream_occupancy(e, lds, wpb);
+  if (used < 256) cache[used++] = Slot{e.fn, lds, wpb, occ};
   return occ;
 }
 
@@ -190,7 +191,9 @@ static int gram_levels_dispatch(int dtype, ksig::DpFwdArgs& a, int cols, void* s
 // back.
 struct StreamPlan {
   ksig::StreamEntry e;
-  size_t lds;
+  size_t lds;     // block LDS
+  size_t region;  // LDS of one wave
+  int wpb;        // waves per block (2 with the global FIFO: two-wave slices)
   bool gf;        // FIFO in global memory
   int pc;         // FIFO slots
   int cap;        // pair-table entries
@@ -214,9 +217,11 @@ static bool stream_plan(int d, int kind, int n_levels, int order, int max_len_x,
   p->gf = gf;
   p->cap = (cap + 3) & ~3;
   p->pc = pc;
-  p->lds = ksig::stream_lds_bytes(dmax, p->e.r, pc, nf, p->cap, gf);
+  p->region = ksig::stream_region_bytes(ksig::stream_lds_bytes(dmax, p->e.r, pc, nf, p->cap, gf));
+  p->wpb = gf ? 2 : 1;
+  p->lds = ksig::stream_block_lds_bytes(p->region, p->wpb);
   if (p->lds > max_lds_bytes()) return false;
-  const int occ = stream_occupancy(p->e, p->lds);
+  const int occ = stream_occupancy(p->e, p->lds, p->wpb);
   p->slots = (int64_t)device_cus() * (occ > 0 ? occ : 1);
   return true;
 }
@@ -243,6 +248,7 @@ static int stream_launch(const StreamPlan& pl, const ksig::DpFwdArgs& f, double 
   a.wave_beg = wave_beg;
   a.n_waves = n_waves;
   a.per_wave = (int)per_wave;
+  a.lds_region = (int)pl.region;
   if (f.kind == ksig::kRBF) {
     double den = 2.0 * static_param * static_param;
     if (den < 1e-12) den = 1e-12;
@@ -254,7 +260,7 @@ static int stream_launch(const StreamPlan& pl, const ksig::DpFwdArgs& f, double 
     a.gdot = (float)static_param;
   }
   a.out = f.out;
-  hipError_t err = ksig::dp_stream_launch(pl.e, a, pl.lds, (hipStream_t)stream);
+  hipError_t err = ksig::dp_stream_launch(pl.e, a, pl.lds, pl.wpb, (hipStream_t)stream);
   if (err != hipSuccess) return hip_fail(err, "dp_stream launch");
   return 0;
 }

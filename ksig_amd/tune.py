@@ -13,7 +13,11 @@ reliably:
   CU, at the price of more pipeline fills;
 * ``oversub`` -- slices cut per resident wave: 1 runs every wave once, more
   lets the dispatcher hand the later (shorter) slices to waves that finish
-  early, which evens out the tail set by the largest pairs.
+  early, which evens out the tail set by the largest pairs;
+* ``two_wave`` (global FIFO only) -- the longest slices are split band by
+  band between the two waves of a block, which halves the sweep that sets
+  the critical path, at the price of a second pipeline fill and a step-count
+  hand-shake between the waves.
 
 ``autotune`` times every feasible candidate on the caller's own data with
 events on the launch stream (the kernels run on torch's current stream) and
@@ -47,6 +51,7 @@ class StreamConfig:
   global_fifo: bool
   max_slice: int
   oversub: int
+  two_wave: bool = False
 
 
 _cache: Dict[str, StreamConfig] = {}
@@ -93,6 +98,8 @@ def candidates(X: _ops.SeqSet, Y: _ops.SeqSet, kind: int, n_levels: int,
       seen.add(slots)
       for ov in OVERSUB:
         out.append(StreamConfig(gf, ms, ov))
+        if gf:
+          out.append(StreamConfig(gf, ms, ov, True))
   return out
 
 
@@ -130,7 +137,7 @@ def autotune(X: _ops.SeqSet, Y: _ops.SeqSet, pairs: torch.Tensor, n_levels: int,
   times: Dict[StreamConfig, float] = {}
   for cfg in configs:
     sched = _ops.make_schedule(X, Y, pairs, kind, n_levels, order, cfg.global_fifo,
-                               cfg.max_slice, cfg.oversub)
+                               cfg.max_slice, cfg.oversub, cfg.two_wave)
     times[cfg] = float(timer(lambda: _ops.signature_levels_batched(
         X, Y, pairs, n_levels, order, kind, param, sched), reps))
   best = min(times, key=times.get)
@@ -188,7 +195,7 @@ def tuned_schedule(X: _ops.SeqSet, Y: _ops.SeqSet, pairs: torch.Tensor, kind: in
       sched = _ops.make_schedule(X, Y, pairs, kind, n_levels, order)
     else:
       sched = _ops.make_schedule(X, Y, pairs, kind, n_levels, order, cfg.global_fifo,
-                                 cfg.max_slice, cfg.oversub)
+                                 cfg.max_slice, cfg.oversub, cfg.two_wave)
     _remember(_sched_by_content, ckey, sched)
   # the entry holds the pair tensor and the length arrays, so their ids stay
   # theirs while it lives
@@ -212,7 +219,8 @@ def load(path: str) -> None:
   with open(path) as f:
     raw = json.load(f)
   for k, v in raw.items():
-    _cache[k] = StreamConfig(bool(v["global_fifo"]), int(v["max_slice"]), int(v["oversub"]))
+    _cache[k] = StreamConfig(bool(v["global_fifo"]), int(v["max_slice"]), int(v["oversub"]),
+                             bool(v.get("two_wave", False)))
   _generation[0] += 1
 
 

@@ -190,7 +190,7 @@ def test_autotuned_launch_choices_agree_with_oracle():
   results = []
   for cfg in cands:
     sched = _ops.make_schedule(X, Y, pairs, _lib.STATIC_RBF, 4, 2, cfg.global_fifo,
-                               cfg.max_slice, cfg.oversub)
+                               cfg.max_slice, cfg.oversub, cfg.two_wave)
     assert sched[3] == cfg.global_fifo and sched[2] <= cfg.max_slice
     results.append(_ops.signature_levels_batched(X, Y, pairs, 4, 2, _lib.STATIC_RBF, 2.0,
                                                  sched).cpu().numpy())
@@ -206,3 +206,56 @@ def test_autotuned_launch_choices_agree_with_oracle():
   got = _ops.signature_levels(X, Y, pairs, 4, 2, True, _lib.STATIC_RBF, 2.0).cpu().numpy()
   _check(got[idx], ref)
   tune.clear()
+
+
+def _sched(order, wave_beg, global_fifo=True):
+  wb = np.asarray(wave_beg, np.int32)
+  return (to_dev(np.asarray(order, np.int32)), to_dev(wb), int(np.diff(wb).max()), global_fifo)
+
+
+@pytest.mark.parametrize("kind,param,d,nlev,order", [("rbf", 4.0, 16, 8, 2),
+                                                     ("linear", 0.5, 8, 5, 1),
+                                                     ("rbf", 3.0, 32, 3, 3)])
+def test_two_wave_slices_match_one_wave(kind, param, d, nlev, order):
+  """Slices split band by band between the two waves of a block (an empty
+  slice after a slice marks it): stacked pairs of uneven lengths, pairs that
+  straddle band boundaries, empty grids, odd and even band counts, a slice
+  whose columns are short of the two-wave minimum (padded), and a final
+  block with one wave, against the same slices swept by one wave and
+  against the FP64 oracle."""
+  rng = np.random.default_rng(70 + d)
+  lx = np.array([700, 300, 2, 451, 260, 240, 90, 1, 333, 517, 128, 64, 300, 250])
+  ly = np.array([500, 610, 400, 260, 700, 300, 40, 5, 400, 290, 230, 64, 128, 250])
+  n = len(lx)
+  xs, X = _ragged(rng, lx, d)
+  ys, Y = _ragged(rng, ly, d)
+  pr = np.stack([np.arange(n), np.arange(n)], -1).astype(np.int32)
+  pairs = to_dev(pr)
+  code = _lib.STATIC_RBF if kind == "rbf" else _lib.STATIC_LINEAR
+  # two-wave slices: [0, 3), [3, 6), [12, 14); one-wave blocks: [6, 8) + [8, 10),
+  # [10, 12) alone at the end (odd slice count: the last block has one wave)
+  seq = [(0, 3), None, (3, 6), None, (12, 14), None, (6, 8), (8, 10), (10, 12)]
+  o2, wb2 = [], [0]
+  for sl in seq:
+    if sl is not None:
+      o2 += list(range(*sl))
+    wb2.append(len(o2))
+  one = [0]
+  for sl in seq:
+    if sl is not None:
+      one.append(one[-1] + sl[1] - sl[0])
+  got2 = _ops.signature_levels_batched(X, Y, pairs, nlev, order, code, param,
+                                       _sched(o2, wb2)).cpu().numpy()
+  got1 = _ops.signature_levels_batched(X, Y, pairs, nlev, order, code, param,
+                                       _sched(o2, one)).cpu().numpy()
+  assert np.all(np.isfinite(got2))
+  assert level_err(got2.T, got1.T) < 1e-6
+  ref = _oracle(xs, ys, pr, nlev, order, kind, param)
+  _check(got2, ref)
+  # the schedule builder makes two-wave slices of the longest sweeps
+  so, wb = _ops.stream_schedule(lx, ly, 2048, 128, min_cols=96, two_wave=True)
+  sizes = np.diff(wb)
+  assert np.any(sizes == 0) and np.all(np.nonzero(sizes == 0)[0] % 2 == 1)
+  got3 = _ops.signature_levels_batched(X, Y, pairs, nlev, order, code, param,
+                                       _sched(so, wb)).cpu().numpy()
+  _check(got3, ref)

@@ -60,3 +60,40 @@ def test_degenerate_and_tiny_lists():
   _check(np.array([3, 1, 2]), np.array([2, 2, 9]), 1, 128)
   order, wb = stream_schedule(np.zeros(0, np.int64), np.zeros(0, np.int64), 8, 128)
   assert len(order) == 0 and list(wb) == [0]
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_two_wave_schedule_marks_long_slices(seed):
+  """Two-wave slices: the longest sweeps (above the mean work per resident
+  wave by the factor, with enough columns and rows) each take a block,
+  marked by an empty slice right after them at an odd index; the other
+  slices pair up in blocks; every pair appears once."""
+  from ksig_amd import _ops
+  rng = np.random.default_rng(seed)
+  n = 4096
+  lx = rng.integers(32, 1025, n)
+  ly = rng.integers(32, 1025, n)
+  for resident in (2048, 256):
+    order, wb = stream_schedule(lx, ly, 4 * resident, 128, min_cols=96, two_wave=True,
+                                resident=resident)
+    np.testing.assert_array_equal(np.sort(order), np.arange(n))
+    sizes = np.diff(wb)
+    empty = np.nonzero(sizes == 0)[0]
+    assert (len(empty) > 0) == (resident == 2048)  # 256 waves: no sweep is that long
+    assert np.all(empty % 2 == 1) and np.all(sizes[empty - 1] > 0)
+    cols = np.minimum(lx, ly)[order]
+    rows = np.maximum(lx, ly)[order]
+    steps = np.array([rows[b:e].sum() * max(int(cols[b]), 96) if e > b else 0
+                      for b, e in zip(wb[:-1], wb[1:])])
+    mean = steps.sum() / resident
+    tw = empty - 1
+    assert np.all(steps[tw] > _ops.TWO_WAVE_FACTOR * mean)
+    assert np.all(cols[wb[tw]] >= _ops.TWO_WAVE_MIN_COLS)
+    solo = np.setdiff1d(np.arange(len(sizes)), np.concatenate([tw, empty]))
+    long_solo = (steps[solo] > _ops.TWO_WAVE_FACTOR * mean) & \
+        (cols[wb[solo]] >= _ops.TWO_WAVE_MIN_COLS) & \
+        (np.add.reduceat(rows, wb[:-1])[solo] >= _ops.TWO_WAVE_MIN_ROWS)
+    assert not np.any(long_solo)
+  # without the flag nothing changes
+  o1, w1 = stream_schedule(lx, ly, 8192, 128, min_cols=96)
+  assert np.all(np.diff(w1) > 0)
