@@ -1067,11 +1067,12 @@ def run_c3(args, P: dict, ref64: dict) -> dict:
     t0 = time.perf_counter()
     best, times = tune.autotune(X, Y, pairs, n_levels, order, _lib.STATIC_RBF, sigma, reps=3)
     tune_s = time.perf_counter() - t0
-    default_cfg = tune.StreamConfig(bool(sched[3]), 128, 1, bool(sched[3]))
+    default_cfg = tune.StreamConfig(bool(sched[3]), 128, 1,
+                                    _ops.TWO_WAVE_FACTOR if sched[3] else 0.0)
     tuned = {"best": {"global_fifo": best.global_fifo, "max_slice": best.max_slice,
                       "oversub": best.oversub, "two_wave": best.two_wave},
              "candidates_ms": {f"{'gfifo' if c.global_fifo else 'lds'}/s{c.max_slice}/x{c.oversub}"
-                               f"{'/2w' if c.two_wave else ''}":
+                               f"{f'/2w{c.two_wave:g}' if c.two_wave else ''}":
                                round(t, 4) for c, t in times.items()},
              "default_ms": times.get(default_cfg), "tune_s": tune_s}
     sched = tune.tuned_schedule(X, Y, pairs, _lib.STATIC_RBF, n_levels, order)

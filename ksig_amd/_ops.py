@@ -350,17 +350,17 @@ def stream_slots(X: SeqSet, Y: SeqSet, kind: int, n_levels: int, order: int,
   return s.value, m.value, w.value
 
 
-# Two-wave slices (global FIFO): a slice whose sweep is longer than
-# TWO_WAVE_FACTOR x the mean work per resident wave, with at least
+# Two-wave slices (global FIFO): a slice whose sweep is longer than a factor
+# (default TWO_WAVE_FACTOR) x the mean work per resident wave, with at least
 # TWO_WAVE_MIN_COLS columns and TWO_WAVE_MIN_ROWS rows, is split band by band
 # between the two waves of one block.
-TWO_WAVE_FACTOR = 1.25
+TWO_WAVE_FACTOR = 0.8
 TWO_WAVE_MIN_COLS = 384
 TWO_WAVE_MIN_ROWS = 512
 
 
 def stream_schedule(lx: np.ndarray, ly: np.ndarray, slots: int, max_per_wave: int,
-                    min_cols: int = 68, two_wave: bool = False, resident: int = 0):
+                    min_cols: int = 68, two_wave: float = 0.0, resident: int = 0):
   """Length-bucketed schedule of a pair list for the row stream (host side).
 
   Pairs are ordered by their shorter side, longest first (a wave sweeps all
@@ -372,8 +372,9 @@ def stream_schedule(lx: np.ndarray, ly: np.ndarray, slots: int, max_per_wave: in
   that set the critical path start at once and the short ones fill the
   gaps).
 
-  ``two_wave`` (global-FIFO launches, two waves per block: block b runs slices
-  2b and 2b + 1): the longest slices become two-wave slices, each followed by
+  ``two_wave`` > 0 (global-FIFO launches, two waves per block: block b runs
+  slices 2b and 2b + 1; True means TWO_WAVE_FACTOR): the slices longer than
+  that factor x the mean work per resident wave become two-wave slices, each followed by
   an empty slice that marks it, so that the block's second wave sweeps every
   other band of it; the remaining slices are paired into blocks in the same
   longest-first order, and blocks are listed by their sweep length.
@@ -417,8 +418,9 @@ def stream_schedule(lx: np.ndarray, ly: np.ndarray, slots: int, max_per_wave: in
     # (block length, slices) with None for the empty slice of a two-wave block
     tail = []
     if two_wave:
+      factor = TWO_WAVE_FACTOR if two_wave is True else float(two_wave)
       mean = steps.sum() / max(int(resident) if resident > 0 else slots, 1)
-      tw = ((steps[perm] > TWO_WAVE_FACTOR * mean)
+      tw = ((steps[perm] > factor * mean)
             & (slice_cols[perm] >= TWO_WAVE_MIN_COLS) & (slice_rows[perm] >= TWO_WAVE_MIN_ROWS))
       solo = [int(k) for k, t in zip(perm, tw) if not t]
       blocks = [(steps[k] / 2, [int(k), None]) for k, t in zip(perm, tw) if t]
@@ -475,7 +477,7 @@ def signature_levels_batched(X: SeqSet, Y: SeqSet, pairs: torch.Tensor, n_levels
 
 def make_schedule(X: SeqSet, Y: SeqSet, pairs: torch.Tensor, kind: int, n_levels: int,
                   order: int, global_fifo: Optional[bool] = None, max_slice: int = 0,
-                  oversub: int = 1, two_wave: Optional[bool] = None):
+                  oversub: int = 1, two_wave: Optional[float] = None):
   """(order, wave_beg, max_slice, global_fifo) of the length-bucketed schedule
   (device int32 tensors, the largest slice and the FIFO placement).
 
@@ -483,8 +485,9 @@ def make_schedule(X: SeqSet, Y: SeqSet, pairs: torch.Tensor, kind: int, n_levels
   ``max_slice`` (0 = the kernel's 128) bounds the pairs of one wave, which
   sizes its LDS pair table; ``oversub`` cuts that many slices per resident
   wave (the hardware hands the later, lighter slices to waves that finish
-  early); ``two_wave`` (global FIFO only; None = on with it) splits the
-  longest slices between the two waves of a block (see ``stream_schedule``).
+  early); ``two_wave`` (global FIFO only; the factor of ``stream_schedule``,
+  0 = off, None = TWO_WAVE_FACTOR) splits the longest slices between the two
+  waves of a block.
   The autotuner (ksig_amd.tune) searches these four."""
   norm_order = n_levels if (order <= 0 or order >= n_levels) else order
   if global_fifo is None:
@@ -505,7 +508,8 @@ def make_schedule(X: SeqSet, Y: SeqSet, pairs: torch.Tensor, kind: int, n_levels
   ly = Y.lengths()[pr[:, 1]]
   so, wb = stream_schedule(lx, ly, slots * max(int(oversub), 1), maxp,
                            min_cols=96 if global_fifo else 68,
-                           two_wave=(two_wave is None or bool(two_wave)) and bool(global_fifo),
+                           two_wave=(TWO_WAVE_FACTOR if two_wave is None else float(two_wave))
+                           if global_fifo else 0.0,
                            resident=slots)
   dev = X.data.device
   return (torch.as_tensor(so, device=dev), torch.as_tensor(wb, device=dev),

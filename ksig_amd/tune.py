@@ -14,10 +14,11 @@ reliably:
 * ``oversub`` -- slices cut per resident wave: 1 runs every wave once, more
   lets the dispatcher hand the later (shorter) slices to waves that finish
   early, which evens out the tail set by the largest pairs;
-* ``two_wave`` (global FIFO only) -- the longest slices are split band by
-  band between the two waves of a block, which halves the sweep that sets
-  the critical path, at the price of a second pipeline fill and a step-count
-  hand-shake between the waves.
+* ``two_wave`` (global FIFO only) -- the slices longer than this factor x
+  the mean work per resident wave (0: none) are split band by band between
+  the two waves of a block, which halves the sweeps that set the critical
+  path, at the price of a second pipeline fill and a step-count hand-shake
+  between the waves.
 
 ``autotune`` times every feasible candidate on the caller's own data with
 events on the launch stream (the kernels run on torch's current stream) and
@@ -44,6 +45,7 @@ from . import _ops
 
 MAX_SLICES = (128, 64, 32)
 OVERSUB = (1, 2, 4, 8)
+TWO_WAVE = (0.8, 1.25)
 
 
 @dataclass(frozen=True)
@@ -51,7 +53,7 @@ class StreamConfig:
   global_fifo: bool
   max_slice: int
   oversub: int
-  two_wave: bool = False
+  two_wave: float = 0.0
 
 
 _cache: Dict[str, StreamConfig] = {}
@@ -99,7 +101,7 @@ def candidates(X: _ops.SeqSet, Y: _ops.SeqSet, kind: int, n_levels: int,
       for ov in OVERSUB:
         out.append(StreamConfig(gf, ms, ov))
         if gf:
-          out.append(StreamConfig(gf, ms, ov, True))
+          out += [StreamConfig(gf, ms, ov, f) for f in TWO_WAVE]
   return out
 
 
@@ -220,7 +222,7 @@ def load(path: str) -> None:
     raw = json.load(f)
   for k, v in raw.items():
     _cache[k] = StreamConfig(bool(v["global_fifo"]), int(v["max_slice"]), int(v["oversub"]),
-                             bool(v.get("two_wave", False)))
+                             float(v.get("two_wave", 0.0)))
   _generation[0] += 1
 
 
