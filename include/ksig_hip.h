@@ -94,16 +94,20 @@ int ksig_stream_slots(int d, int static_kind, int n_levels, int order, int max_l
 
 /*
  * ksig_signature_levels for float32 inputs with difference=1 and a caller
- * schedule, in one persistent launch: wave w evaluates the pairs
+ * schedule, in one persistent launch: slice w is the pairs
  * sched_order[wave_beg[w] .. wave_beg[w+1]) (sched_order null: the identity),
  * n_waves slices of at most max_slice pairs each (1 <= max_slice <= the
- * *max_pairs_per_wave of ksig_stream_slots), writing out[q] for each pair
- * index q it takes.  A wave stacks its pairs' rows and sweeps them with one
- * column count (the longest shorter side among them), so slices of similar
- * lengths waste least.  wave_beg null: slices of max_slice consecutive pairs
- * of the order.  workspace non-null selects the global-memory FIFO and must
- * hold n_waves * *workspace_per_wave bytes (16-byte aligned; contents are
- * scratch); null keeps the FIFO in LDS.  Replaces the same reference
+ * *max_pairs_per_wave of ksig_stream_slots); the wave that sweeps a slice
+ * writes out[q] for each pair index q it takes.  A wave stacks its pairs'
+ * rows and sweeps them with one column count (the longest shorter side among
+ * them), so slices of similar lengths waste least.  wave_beg null: slices of
+ * max_slice consecutive pairs of the order.  workspace non-null selects the
+ * global-memory FIFO and must hold n_waves * *workspace_per_wave bytes
+ * (16-byte aligned; contents are scratch); null keeps the FIFO in LDS.  With
+ * the global FIFO a launch runs two waves per block (slices 2b and 2b + 1),
+ * and an empty slice 2b + 1 after a non-empty slice 2b makes both waves
+ * sweep slice 2b, every other band each (slices of at least 224 columns;
+ * shorter ones are swept by the first wave alone).  Replaces the same reference
  * composition as ksig_signature_levels (ksig/kernels.py:147-164 over
  * ksig/algorithms.py:22-140) for pair lists the reference cannot take
  * (uneven lengths; the reference needs one dense [n, L, d] array,
