@@ -44,8 +44,8 @@ PEAK_HBM_GBS = 8000.0
 # from two rocprofv3 --pmc passes (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the
 # factors an elementwise copy of 1 GiB reads on the same counters); recorded
 # with the kernel version and schedule they were measured on (c3: the
-# autotuned schedule, 2738 waves).
-MEASURED_TRAFFIC_BYTES = {"c1": 38658 * 1024, "c3": (663693 * 2 + 581215) * 1024,
+# autotuned schedule with two-wave slices, 3680 wave slots in 1840 blocks).
+MEASURED_TRAFFIC_BYTES = {"c1": 38658 * 1024, "c3": (666357 * 2 + 600020) * 1024,
                           "gak": (21130 * 2 + 4104) * 1024, "rws": (13451 * 2 + 3200) * 1024,
                           "rfsf": (230276 * 2 + 3906) * 1024}
 
