@@ -248,7 +248,7 @@ def main():
                   help="only run an elementwise copy of MIB MiB (5 times): the "
                        "reference point for scaling HBM byte counters")
   ap.add_argument("--kernel-probe", default=None,
-                  choices=["fwd_save", "adjoint", "contract", "pde_fwd", "pde_fwd_edges",
+                  choices=["fwd", "fwd_save", "adjoint", "contract", "pde_fwd", "pde_fwd_edges",
                            "pde_adjoint", "pde_contract", "pde_fused"],
                   help="only time one kernel of the C4 training step on one 16384-pair "
                        "chunk (128 x 128 pairs, L=128, d=32, 6 levels), --steps launches: "
@@ -818,9 +818,10 @@ def bench_rfsf(args):
 def kernel_probe(args):
   """One C4 kernel on one chunk of the training step: 128 Brownian paths
   (L=128, d=32) against themselves, 16384 pairs, 6 levels, RBF sigma=sqrt(32);
-  the training forward (boundary state), the adjoint (omega) or the
-  contraction of omega with the inputs, each launched --steps times after
-  one untimed launch, timed with HIP events on the library's stream."""
+  the forward without (fwd) or with (fwd_save) the boundary state, the
+  adjoint (omega) or the contraction of omega with the inputs, each launched
+  --steps times after one untimed launch, timed with HIP events on the
+  library's stream."""
   from ksig_amd import _lib, _ops
   dev = torch.device("cuda", 0)
   lib = _lib.load()
@@ -852,6 +853,11 @@ def kernel_probe(args):
         0, *xa, *xa, d, pairs.data_ptr(), P, _lib.STATIC_RBF, sig, nl, out.data_ptr(),
         save.data_ptr(), save_off.data_ptr(), fws.data_ptr(), fwsb, st))
 
+  def fwd():  # the same kernel without the boundary state (the forward-only path)
+    _lib.check(lib.ksig_signature_levels_fwd_save(
+        0, *xa, *xa, d, pairs.data_ptr(), P, _lib.STATIC_RBF, sig, nl, out.data_ptr(),
+        None, save_off.data_ptr(), fws.data_ptr(), fwsb, st))
+
   def adjoint():
     _lib.check(lib.ksig_signature_levels_bwd(
         0, *xa, *xa, d, pairs.data_ptr(), P, _lib.STATIC_RBF, sig, nl, gout.data_ptr(),
@@ -865,9 +871,10 @@ def kernel_probe(args):
         gx.data_ptr(), gx.data_ptr(), st))
 
   fwd_save()
-  if args.kernel_probe != "fwd_save":
+  if args.kernel_probe not in ("fwd_save", "fwd"):
     adjoint()
-  f = {"fwd_save": fwd_save, "adjoint": adjoint, "contract": contract}[args.kernel_probe]
+  f = {"fwd": fwd, "fwd_save": fwd_save, "adjoint": adjoint,
+       "contract": contract}[args.kernel_probe]
   f()
   torch.cuda.synchronize()
   e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)

@@ -215,6 +215,13 @@ dp_fwd_mfma_kernel(const DpFwdArgs a) {
   int cp = (Pc - lane) % Pc;  // this lane's column c' (lane i starts i steps late)
   int row = lane;             // grid row of this lane's current band
   int brow = 0;               // band of this lane (counts the wraps of cp)
+  // SAVE: a lane's row-end prefixes, held from the end of its band pass
+  // until every lane of the band has finished it (63 steps later), then
+  // stored by the whole wave at once: one coalesced store per level and
+  // band instead of a one-lane store per level and step
+  T psh[NPS];
+  int hrow = -1;                    // row of the held prefixes (-1: none)
+  int next_flush = Pc + kWave - 1;  // band b is flushed at step (b+1)*Pc + 63
   T* sv = SAVE ? reinterpret_cast<T*>(a.save) + a.save_off[pr] : nullptr;
   // row-end prefixes [NPS][Nr] through a buffer descriptor: one VGPR offset
   // per lane, the level as a scalar offset
@@ -248,8 +255,20 @@ dp_fwd_mfma_kernel(const DpFwdArgs a) {
     }
     fbyte = fbyte + kSlotB == Pc * kSlotB ? 0 : fbyte + kSlotB;
   };
+  auto flush = [&]() __attribute__((always_inline)) {  // wave-uniform
+    if constexpr (SAVE && NLEV > 1) {
+      const int vo = hrow >= 0 ? hrow * 4 : int(kBufOob);
+#pragma unroll
+      for (int m = 0; m < NPS; ++m)
+        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(psh[m]), sv_rs, vo, m * Nr * 4, 0);
+    }
+  };
   auto step_fn = [&](int step, T& g_io, T* fin_io) __attribute__((always_inline)) {
     if (((step + 2) & 15) == 0) produce_chunk((step + 2) >> 4);
+    if (SAVE && NLEV > 1 && step == next_flush) {  // wave-uniform
+      flush();
+      next_flush += Pc;
+    }
     T in[NF];
 #pragma unroll
     for (int k = 0; k < NF; ++k) in[k] = shift_add(fl[k], fin_io[k]);
@@ -286,10 +305,9 @@ dp_fwd_mfma_kernel(const DpFwdArgs a) {
         if constexpr (SAVE && NLEV > 1) {
           // Row-end prefixes P_a(row, Nc): the padding columns past Nc have
           // M = 0 and zero column sums, so PS still holds them here.
-          const int vo = row < Nr && brow < nb ? row * 4 : int(kBufOob);
 #pragma unroll
-          for (int m = 0; m < NPS; ++m)
-            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(PS[m]), sv_rs, vo, m * Nr * 4, 0);
+          for (int m = 0; m < NPS; ++m) psh[m] = PS[m];
+          hrow = row < Nr && brow < nb ? row : -1;
         }
         row += kWave;
         ++brow;
@@ -321,6 +339,7 @@ dp_fwd_mfma_kernel(const DpFwdArgs a) {
     step_fn(step + 1, g1, f1);
   }
   if (step < total) step_fn(step, g0, f0);
+  if (SAVE && NLEV > 1) flush();  // the last band (its flush step is past the sweep)
   __syncthreads();  // lane 63's last FIFO stores retired
 
   double acc[NLEV];
