@@ -259,3 +259,43 @@ def test_two_wave_slices_match_one_wave(kind, param, d, nlev, order):
   got3 = _ops.signature_levels_batched(X, Y, pairs, nlev, order, code, param,
                                        _sched(so, wb)).cpu().numpy()
   _check(got3, ref)
+
+
+def test_two_wave_launch_replays_from_a_graph():
+  """The batched row stream with two-wave slices and its global FIFO
+  workspace captured in a HIP graph: every replay (with eager launches of
+  the same op in between) equals the eager result bit for bit."""
+  rng = np.random.default_rng(90)
+  n = 40
+  lx = rng.integers(200, 1025, n)
+  ly = rng.integers(200, 1025, n)
+  xs = [(O.brownian(rng, 1, L, 16)[0]).astype(np.float32) for L in lx]
+  ys = [(O.brownian(rng, 1, L, 16)[0]).astype(np.float32) for L in ly]
+  X = _ops.SeqSet.ragged([to_dev(s) for s in xs])
+  Y = _ops.SeqSet.ragged([to_dev(s) for s in ys])
+  pairs = to_dev(np.stack([np.arange(n), np.arange(n)], -1).astype(np.int32))
+  sched = _ops.make_schedule(X, Y, pairs, _lib.STATIC_RBF, 8, 2)
+  assert sched[3] and int((np.diff(sched[1].cpu().numpy()) == 0).sum()) > 0
+  run = lambda: _ops.signature_levels_batched(X, Y, pairs, 8, 2, _lib.STATIC_RBF, 4.0, sched)
+  eager = run()
+  torch.cuda.synchronize()
+  stream = torch.cuda.current_stream()
+  side = torch.cuda.Stream()
+  side.wait_stream(stream)
+  with torch.cuda.stream(side):
+    run()
+  stream.wait_stream(side)
+  g = torch.cuda.CUDAGraph()
+  with torch.cuda.graph(g):
+    out = run()
+  for _ in range(3):
+    g.replay()
+    torch.cuda.synchronize()
+    assert torch.equal(out, eager)
+    again = run()
+    torch.cuda.synchronize()
+    assert torch.equal(again, eager)
+  idx = [0, 7, 19]
+  ref = _oracle([xs[i] for i in idx], [ys[i] for i in idx],
+                [(k, k) for k in range(len(idx))], 8, 2, "rbf", 4.0)
+  _check(eager.cpu().numpy()[idx], ref)
