@@ -493,6 +493,48 @@ int ksig_lowrank_levels_diag(int dtype, const void* du, int n_levels, int64_t n_
                              int steps, int half, double* out, void* stream);
 
 /*
+ * The level recursion with a DiagonalProjection of any internal size q
+ * (ksig/projections.py:587-674) at any embedding order, as the reference runs
+ * it (first order ksig/algorithms.py:218-246; higher order :296-337 with its
+ * [d, n, L, q^(i+1), H] terms for this projection, :302-306): du: the
+ * differenced lifted paths [n_levels, n_seq, steps, q * half] (dtype; the
+ * internal index b of feature b * half + h); order as ksig_lowrank_levels;
+ * out: the level features before normalisation, [n_seq, sum_i q^(i+1) *
+ * half] float64, level i's block laid out as [q^(i+1), half] (the internal
+ * index of level 0 most significant).  Built for 1 <= q <= 4 and up to 5
+ * levels where every level's running sums, with two adjacent levels' terms
+ * at order > 1, fit 96 float64 registers per thread (q = 2: 5 levels at
+ * order 1, 4 levels at any order; q = 3: 3 levels; q = 4: 3 levels at order
+ * 1, 2 at any order); other shapes return KSIG_EUNSUPPORTED and launch
+ * nothing.  ksig_lowrank_levels_diag is this entry at q = 2, order 1.
+ */
+int ksig_lowrank_levels_diagonal(int dtype, const void* du, int n_levels, int order,
+                                 int internal_size, int64_t n_seq, int steps, int half,
+                                 double* out, void* stream);
+
+/*
+ * The level recursion with TensorSketch projections (ksig/projections.py:
+ * 379-458: the count sketch of ksig/utils.py:279-298 and the FFT convolution
+ * of :301-313) at any embedding order (ksig/algorithms.py:175-337), as
+ * SignatureFeatures._compute_features calls it with projection=TensorSketch
+ * (ksig/kernels.py:330-361).  du: the (differenced) lifted paths [n_levels,
+ * n_seq, steps, width] (dtype); hash_idx [n_levels, width] int32 and hash_bit
+ * [n_levels, width] (dtype): level i's projection's hash_idx_ and hash_bit_
+ * (a value of hash_idx outside [0, n_comp) selects no component, as the
+ * reference's 0/1 hash mask); order as
+ * ksig_lowrank_levels; out: the level features before normalisation,
+ * [n_seq, n_levels, n_comp] float64.  The convolution of a sketch is
+ * evaluated directly from the sparse count sketch (width products per
+ * component), equal to the FFT route up to rounding.  1 <= n_levels <= 8,
+ * n_comp <= 2048 and the workgroup's LDS ((n_levels + the order terms of
+ * the levels below the top) * 2 * n_comp doubles) within the device's
+ * limit, else KSIG_EUNSUPPORTED and nothing is launched.
+ */
+int ksig_lowrank_levels_sketch(int dtype, const void* du, int n_levels, int order, int64_t n_seq,
+                               int steps, int width, const int32_t* hash_idx,
+                               const void* hash_bit, int n_comp, double* out, void* stream);
+
+/*
  * Random Fourier features of projected points and their time increments in
  * one pass: proj = X W [n_seq, len, half] (dtype; rows ld >= half elements
  * apart, so one GEMM may hold several levels' weights), out [n_seq, len - 1, 2 * half]

@@ -97,6 +97,10 @@ SIGNATURES = {
                                           _vp]),
     "ksig_rff_increments": (_c_int, [_c_int, _vp, _c_i64, _c_int, _c_int, _c_int, _c_int, _vp,
                                      _vp]),
+    "ksig_lowrank_levels_diagonal": (_c_int, [_c_int, _vp, _c_int, _c_int, _c_int, _c_i64,
+                                              _c_int, _c_int, _vp, _vp]),
+    "ksig_lowrank_levels_sketch": (_c_int, [_c_int, _vp, _c_int, _c_int, _c_i64, _c_int, _c_int,
+                                            _vp, _vp, _c_int, _vp, _vp]),
 }
 
 

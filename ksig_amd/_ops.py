@@ -713,6 +713,51 @@ def lowrank_levels_diag(dU: torch.Tensor) -> torch.Tensor:
   return out
 
 
+def lowrank_levels_diagonal(dU: torch.Tensor, internal_size: int = 2,
+                            order: int = 1) -> torch.Tensor:
+  """Level features [n_seq, sum_i q^(i+1) * half] (float64) of the low-rank
+  recursion with a DiagonalProjection of internal size q = internal_size at
+  embedding order ``order`` (<= 0 or >= n_levels: all orders), from the
+  differenced lifted paths dU [n_levels, n_seq, steps, q * half].  Shapes
+  the HIP scan is not built for raise NotImplementedError (nothing is
+  launched)."""
+  lib = _lib.load()
+  dc = dU.contiguous()
+  n_levels, n, steps, width = dc.shape
+  q = int(internal_size)
+  if q < 1 or width % q:
+    raise ValueError(f'cannot split {width} features into internal_size={q} parts')
+  half = width // q
+  tot = sum(q ** (i + 1) for i in range(n_levels))
+  out = torch.empty((n, tot * half), device=dU.device, dtype=torch.float64)
+  _lib.check(lib.ksig_lowrank_levels_diagonal(_lib.dtype_code(dc.dtype), dc.data_ptr(), n_levels,
+                                              int(order), q, n, steps, half, out.data_ptr(),
+                                              _lib.stream_handle(dU.device)))
+  return out
+
+
+def lowrank_levels_sketch(dU: torch.Tensor, hash_idx: torch.Tensor, hash_bit: torch.Tensor,
+                          n_comp: int, order: int = 1) -> torch.Tensor:
+  """Level features [n_seq, n_levels, n_comp] (float64) of the low-rank
+  recursion with TensorSketch projections from the lifted paths dU
+  [n_levels, n_seq, steps, width], each level's hashes hash_idx [n_levels,
+  width] (int32 after conversion) and signs hash_bit [n_levels, width],
+  embedding order ``order``.  Shapes the HIP scan is not built for raise
+  NotImplementedError (nothing is launched)."""
+  lib = _lib.load()
+  dc = dU.contiguous()
+  n_levels, n, steps, width = dc.shape
+  hi = hash_idx.to(device=dU.device, dtype=torch.int32).contiguous()
+  hb = hash_bit.to(device=dU.device, dtype=dc.dtype).contiguous()
+  assert tuple(hi.shape) == (n_levels, width) and tuple(hb.shape) == (n_levels, width)
+  out = torch.empty((n, n_levels, int(n_comp)), device=dU.device, dtype=torch.float64)
+  _lib.check(lib.ksig_lowrank_levels_sketch(_lib.dtype_code(dc.dtype), dc.data_ptr(), n_levels,
+                                            int(order), n, steps, width, hi.data_ptr(),
+                                            hb.data_ptr(), int(n_comp), out.data_ptr(),
+                                            _lib.stream_handle(dU.device)))
+  return out
+
+
 def rff_increments(proj: torch.Tensor, difference: bool = True,
                    out: Optional[torch.Tensor] = None) -> torch.Tensor:
   """[sin, cos](proj) / sqrt(half), differenced along time, from projections

@@ -15,13 +15,20 @@
 #include "dp_stream.h"
 #include "gak.h"
 #include "rws.h"
+#include "lowrank.h"
 
 namespace ksig {
 hipError_t lowrank_scan_launch(int dtype, const void* z, int n_levels, int order,
                                int64_t n_seq, int steps, int n_comp, double* out,
                                hipStream_t s);
-hipError_t lowrank_diag_scan_launch(int dtype, const void* du, int n_levels, int64_t n_seq,
-                                    int steps, int half, double* out, hipStream_t s);
+hipError_t lowrank_diag_scan_launch(int dtype, const void* du, int q, int n_levels, int order,
+                                    int64_t n_seq, int steps, int half, double* out,
+                                    hipStream_t s);
+hipError_t lowrank_sketch_launch(int dtype, const void* du, int n_levels, int order,
+                                 int64_t n_seq, int steps, int width, const int* hidx,
+                                 const void* hbit, int m, double* out, hipStream_t s);
+size_t lowrank_sketch_lds(int n_levels, int order, int m);
+int lowrank_sketch_max_comp();
 hipError_t rff_diff_launch(int dtype, const void* p, int64_t n_seq, int L, int half, int ld,
                            int difference, void* out, hipStream_t s);
 }  // namespace ksig
@@ -1135,18 +1142,63 @@ int ksig_lowrank_levels(int dtype, const void* z, int n_levels, int order, int64
   return 0;
 }
 
-int ksig_lowrank_levels_diag(int dtype, const void* du, int n_levels, int64_t n_seq,
-                             int steps, int half, double* out, void* stream) {
+static int lowrank_diagonal(int dtype, const void* du, int n_levels, int order, int q,
+                            int64_t n_seq, int steps, int half, double* out, void* stream) {
   if (dtype != KSIG_F32 && dtype != KSIG_F64) return fail(KSIG_EINVAL, "bad dtype %d", dtype);
   if (n_levels < 1) return fail(KSIG_EINVAL, "n_levels must be positive");
-  if (n_levels > 5) return fail(KSIG_EUNSUPPORTED, "n_levels = %d > 5 is not built", n_levels);
+  if (q < 1) return fail(KSIG_EINVAL, "internal_size must be positive");
+  if (order <= 0 || order > n_levels) order = n_levels;  // ksig/algorithms.py:165
+  if (!ksig::diag_fits(q, n_levels, order))
+    return fail(KSIG_EUNSUPPORTED,
+                "diagonal scan with internal_size %d, %d levels, order %d is not built", q,
+                n_levels, order);
   if (n_seq < 0 || steps < 0 || half < 0) return fail(KSIG_EINVAL, "negative shape");
   if (n_seq * (int64_t)half > 0x7fffffffLL * 256LL) return fail(KSIG_EINVAL, "too many sequences");
   if (n_seq == 0 || half == 0) return 0;
   if (!out || (steps > 0 && !du)) return fail(KSIG_EINVAL, "null device pointer");
-  hipError_t e = ksig::lowrank_diag_scan_launch(dtype, du, n_levels, n_seq, steps, half, out,
-                                               (hipStream_t)stream);
+  hipError_t e = ksig::lowrank_diag_scan_launch(dtype, du, q, n_levels, order, n_seq, steps, half,
+                                               out, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "lowrank_diag_scan launch");
+  return 0;
+}
+
+int ksig_lowrank_levels_diag(int dtype, const void* du, int n_levels, int64_t n_seq,
+                             int steps, int half, double* out, void* stream) {
+  if (n_levels > 5) return fail(KSIG_EUNSUPPORTED, "n_levels = %d > 5 is not built", n_levels);
+  return lowrank_diagonal(dtype, du, n_levels, 1, 2, n_seq, steps, half, out, stream);
+}
+
+int ksig_lowrank_levels_diagonal(int dtype, const void* du, int n_levels, int order,
+                                 int internal_size, int64_t n_seq, int steps, int half,
+                                 double* out, void* stream) {
+  return lowrank_diagonal(dtype, du, n_levels, order, internal_size, n_seq, steps, half, out,
+                          stream);
+}
+
+int ksig_lowrank_levels_sketch(int dtype, const void* du, int n_levels, int order, int64_t n_seq,
+                               int steps, int width, const int32_t* hash_idx,
+                               const void* hash_bit, int n_comp, double* out, void* stream) {
+  if (dtype != KSIG_F32 && dtype != KSIG_F64) return fail(KSIG_EINVAL, "bad dtype %d", dtype);
+  if (n_levels < 1) return fail(KSIG_EINVAL, "n_levels must be positive");
+  if (n_levels > 8) return fail(KSIG_EUNSUPPORTED, "n_levels = %d > 8 is not built", n_levels);
+  if (order <= 0 || order > n_levels) order = n_levels;  // ksig/algorithms.py:165
+  if (n_seq < 0 || steps < 0 || width < 0) return fail(KSIG_EINVAL, "negative shape");
+  if (n_comp < 1) return fail(KSIG_EINVAL, "n_comp must be positive");
+  if (n_seq > 0x7fffffffLL) return fail(KSIG_EINVAL, "too many sequences");
+  if (n_comp > ksig::lowrank_sketch_max_comp())
+    return fail(KSIG_EUNSUPPORTED, "n_comp = %d > %d is not built", n_comp,
+                ksig::lowrank_sketch_max_comp());
+  const size_t lds = ksig::lowrank_sketch_lds(n_levels, order, n_comp);
+  if (lds > max_lds_bytes())
+    return fail(KSIG_EUNSUPPORTED, "sketch scan needs %zu bytes of LDS (> %zu)", lds,
+                max_lds_bytes());
+  if (n_seq == 0) return 0;
+  if (!out || (steps > 0 && width > 0 && (!du || !hash_idx || !hash_bit)))
+    return fail(KSIG_EINVAL, "null device pointer");
+  hipError_t e = ksig::lowrank_sketch_launch(dtype, du, n_levels, order, n_seq, steps, width,
+                                            hash_idx, hash_bit, n_comp, out,
+                                            (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "lowrank_sketch launch");
   return 0;
 }
 

@@ -21,18 +21,10 @@
 
 #include "ksig_common.h"
 #include "ksig_hip.h"
+#include "lowrank.h"
 
 namespace ksig {
 
-// The diagonal scan below keeps the loads of kAheadDiag time steps in flight
-// in a register ring, refilled as each step is consumed (one thread per
-// sequence and component pair: about one wave per SIMD at 1000 sequences, so
-// the walk is bound by load latency unless several steps are outstanding;
-// 0.428 -> 0.323 ms at 1000 x 199 steps, 50 pairs, 5 levels).  The
-// tensorized scan has twice the threads and is left to the compiler's load
-// scheduling: rings of depth 1 and 4 measured 0.221 and 0.138 ms against its
-// 0.130.
-constexpr int kAheadDiag = 4;
 
 template <typename T, int NL, int ORD>
 __global__ void __launch_bounds__(256)
@@ -119,22 +111,31 @@ static hipError_t dispatch_scan(const void* z, int n_levels, int order, int64_t 
 }
 
 // The same recursion with a diagonal projection (ksig/projections.py:587-674)
-// of internal size 2 on random Fourier features: level i's increments are
-// the (sin, cos) pair u_i(t) = (dU_i[t, k], dU_i[t, H + k]) of component k,
-// and R_i(t) = sqrt(H) C_{i-1}(t) (x) u_i(t) has 2^(i+1) entries per
-// component (the q-index of level 0 most significant), so the level features
-// are [2^(i+1), H] per sequence.  One thread per (sequence, component k)
-// keeps every level's running sums and time sums in float64 registers.
-template <typename T, int NL>
+// of internal size q on random Fourier features, at any embedding order
+// (ksig/algorithms.py:296-337 keeps [d, n, L, q^(i+1), H] terms per level for
+// this projection): level i's increments are the q values u_i(t)[b] =
+// dU_i[t, b H + h] of component h (b < q), and
+//   R_0[0](t) = u_0(t),
+//   R_i[0](t) = sqrt(H) C_{i-1}(t) (x) u_i(t),   C_i(t) = sum_{t' < t} sum_r R_i[r](t'),
+//   R_i[r](t) = sqrt(H) R_{i-1}[r-1](t) (x) u_i(t) / (r+1),   1 <= r < min(i+1, p),
+// each with q^(i+1) entries per component (the internal index of level 0
+// most significant), so the level features are [q^(i+1), H] per sequence.
+// One thread per (sequence, component h) keeps every level's running sums in
+// float64 registers (the projection is diagonal over h: no thread reads
+// another's state); shapes whose state does not fit a register budget are
+// not built (diag_state_doubles below, shared with the host).
+template <typename T, int Q, int NL, int ORD>
 __global__ void __launch_bounds__(256)
 lowrank_diag_scan_kernel(const T* __restrict__ du, int64_t n_seq, int steps, int half,
                          double* __restrict__ out) {
-  constexpr int kTot = (2 << NL) - 2;  // sum over levels of 2^(i+1)
+  constexpr int kTot = diag_total(Q, NL);
+  constexpr int kR = diag_rbuf(Q, NL, ORD);
+  constexpr int kAhead = diag_ahead(Q, NL, ORD, (int)sizeof(T));
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= n_seq * half) return;
   const int64_t a = idx / half;
   const int k = (int)(idx - a * half);
-  const int width = 2 * half;
+  const int width = Q * half;
   const int64_t level_stride = n_seq * (int64_t)steps * width;
   const T* up = du + a * (int64_t)steps * width + k;
   const double sc = sqrt((double)half);
@@ -142,40 +143,86 @@ lowrank_diag_scan_kernel(const T* __restrict__ du, int64_t n_seq, int steps, int
   double C[kTot];
 #pragma unroll
   for (int j = 0; j < kTot; ++j) C[j] = 0.0;
-  T ring[kAheadDiag][2 * NL];
+  T ring[kAhead][Q * NL];
   auto load = [&](T* dst, int t) __attribute__((always_inline)) {
 #pragma unroll
     for (int i = 0; i < NL; ++i) {
       const T* ut = up + i * level_stride + (int64_t)t * width;
-      dst[2 * i] = t < steps ? ut[0] : T(0);
-      dst[2 * i + 1] = t < steps ? ut[half] : T(0);
+#pragma unroll
+      for (int b = 0; b < Q; ++b) dst[Q * i + b] = t < steps ? ut[b * half] : T(0);
     }
   };
 #pragma unroll
-  for (int u = 0; u < kAheadDiag; ++u) load(ring[u], u);
-  for (int t0 = 0; t0 < steps; t0 += kAheadDiag) {
+  for (int u = 0; u < kAhead; ++u) load(ring[u], u);
+  for (int t0 = 0; t0 < steps; t0 += kAhead) {
 #pragma unroll
-    for (int u = 0; u < kAheadDiag; ++u) {
+    for (int u = 0; u < kAhead; ++u) {
       if (t0 + u < steps) {  // uniform over the grid
-        double uv[2 * NL];
+        double uv[Q * NL];
 #pragma unroll
-        for (int j = 0; j < 2 * NL; ++j) uv[j] = (double)ring[u][j];
-        load(ring[u], t0 + u + kAheadDiag);
-        // highest level first, so that level i reads level i-1's running
-        // sums before this step's terms are added to them (exclusive sum)
+        for (int j = 0; j < Q * NL; ++j) uv[j] = (double)ring[u][j];
+        load(ring[u], t0 + u + kAhead);
+        if constexpr (ORD == 1) {
+          // highest level first, so that level i reads level i-1's running
+          // sums before this step's terms are added to them (exclusive sum)
 #pragma unroll
-        for (int i = NL - 1; i >= 1; --i) {
-          const int off = (2 << i) - 2;  // first entry of level i
-          const int prev = (2 << (i - 1)) - 2;
+          for (int i = NL - 1; i >= 1; --i) {
+            const int off = diag_total(Q, i);  // first entry of level i
+            const int prev = diag_total(Q, i - 1);
 #pragma unroll
-          for (int j = 0; j < (1 << i); ++j) {
-            const double cs = sc * C[prev + j];
-            C[off + 2 * j] += cs * uv[2 * i];
-            C[off + 2 * j + 1] += cs * uv[2 * i + 1];
+            for (int j = 0; j < ipow(Q, i); ++j) {
+              const double cs = sc * C[prev + j];
+#pragma unroll
+              for (int b = 0; b < Q; ++b) C[off + Q * j + b] += cs * uv[Q * i + b];
+            }
+          }
+#pragma unroll
+          for (int b = 0; b < Q; ++b) C[b] += uv[b];
+        } else {
+          // lowest level first: level i needs this step's terms of level
+          // i-1 (Rp, its d_{i-1} orders stacked) and C_{i-1} before they are
+          // added to it; the top level's terms go straight into its sums
+          double Rp[kR], Rn[kR];
+#pragma unroll
+          for (int b = 0; b < Q; ++b) Rp[b] = uv[b];
+          if constexpr (NL == 1) {
+#pragma unroll
+            for (int b = 0; b < Q; ++b) C[b] += Rp[b];
+          }
+#pragma unroll
+          for (int i = 1; i < NL; ++i) {
+            const int off = diag_total(Q, i);
+            const int prev = diag_total(Q, i - 1);
+            const int np = ipow(Q, i);                   // entries per order, level i-1
+            const int dp = i < ORD ? i : ORD;            // orders of level i-1
+            const int dn = i + 1 < ORD ? i + 1 : ORD;    // orders of level i
+#pragma unroll
+            for (int j = 0; j < np; ++j) {
+#pragma unroll
+              for (int b = 0; b < Q; ++b) {
+                const double ub = sc * uv[Q * i + b];
+                double v = C[prev + j] * ub;
+                if (i < NL - 1) Rn[Q * j + b] = v;
+                else C[off + Q * j + b] += v;
+#pragma unroll
+                for (int r = 1; r < dn; ++r) {
+                  v = Rp[(r - 1) * np + j] * ub * (1.0 / (r + 1));
+                  if (i < NL - 1) Rn[r * np * Q + Q * j + b] = v;
+                  else C[off + Q * j + b] += v;
+                }
+              }
+            }
+#pragma unroll
+            for (int j = 0; j < np; ++j) {
+#pragma unroll
+              for (int r = 0; r < dp; ++r) C[prev + j] += Rp[r * np + j];
+            }
+            if (i < NL - 1) {
+#pragma unroll
+              for (int j = 0; j < dn * np * Q; ++j) Rp[j] = Rn[j];
+            }
           }
         }
-        C[0] += uv[0];
-        C[1] += uv[1];
       }
     }
   }
@@ -184,31 +231,248 @@ lowrank_diag_scan_kernel(const T* __restrict__ du, int64_t n_seq, int steps, int
   for (int j = 0; j < kTot; ++j) o[(int64_t)j * half] = C[j];
 }
 
-template <typename T, int NL>
+template <typename T, int Q, int NL, int ORD>
 static hipError_t launch_diag(const void* du, int64_t n_seq, int steps, int half,
                               double* out, hipStream_t s) {
-  const int64_t threads = n_seq * (int64_t)half;
-  const unsigned blocks = (unsigned)((threads + 255) / 256);
-  hipLaunchKernelGGL((lowrank_diag_scan_kernel<T, NL>), dim3(blocks), dim3(256), 0, s,
-                     reinterpret_cast<const T*>(du), n_seq, steps, half, out);
+  if constexpr (!diag_fits(Q, NL, ORD)) {
+    return hipErrorInvalidValue;
+  } else {
+    const int64_t threads = n_seq * (int64_t)half;
+    const unsigned blocks = (unsigned)((threads + 255) / 256);
+    hipLaunchKernelGGL((lowrank_diag_scan_kernel<T, Q, NL, ORD>), dim3(blocks), dim3(256), 0, s,
+                       reinterpret_cast<const T*>(du), n_seq, steps, half, out);
+    return hipGetLastError();
+  }
+}
+
+template <typename T, int Q, int NL>
+static hipError_t diag_order(int order, const void* du, int64_t n_seq, int steps, int half,
+                             double* out, hipStream_t s) {
+  switch (order) {
+    case 1: return launch_diag<T, Q, NL, 1>(du, n_seq, steps, half, out, s);
+    case 2: if constexpr (NL >= 2) return launch_diag<T, Q, NL, 2>(du, n_seq, steps, half, out, s); break;
+    case 3: if constexpr (NL >= 3) return launch_diag<T, Q, NL, 3>(du, n_seq, steps, half, out, s); break;
+    case 4: if constexpr (NL >= 4) return launch_diag<T, Q, NL, 4>(du, n_seq, steps, half, out, s); break;
+    case 5: if constexpr (NL >= 5) return launch_diag<T, Q, NL, 5>(du, n_seq, steps, half, out, s); break;
+  }
+  return hipErrorInvalidValue;
+}
+
+template <typename T, int Q>
+static hipError_t diag_levels(int n_levels, int order, const void* du, int64_t n_seq, int steps,
+                              int half, double* out, hipStream_t s) {
+  switch (n_levels) {
+    case 1: return diag_order<T, Q, 1>(order, du, n_seq, steps, half, out, s);
+    case 2: return diag_order<T, Q, 2>(order, du, n_seq, steps, half, out, s);
+    case 3: return diag_order<T, Q, 3>(order, du, n_seq, steps, half, out, s);
+    case 4: return diag_order<T, Q, 4>(order, du, n_seq, steps, half, out, s);
+    case 5: return diag_order<T, Q, 5>(order, du, n_seq, steps, half, out, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+template <typename T>
+static hipError_t diag_q(int q, int n_levels, int order, const void* du, int64_t n_seq,
+                         int steps, int half, double* out, hipStream_t s) {
+  switch (q) {
+    case 1: return diag_levels<T, 1>(n_levels, order, du, n_seq, steps, half, out, s);
+    case 2: return diag_levels<T, 2>(n_levels, order, du, n_seq, steps, half, out, s);
+    case 3: return diag_levels<T, 3>(n_levels, order, du, n_seq, steps, half, out, s);
+    case 4: return diag_levels<T, 4>(n_levels, order, du, n_seq, steps, half, out, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+// order: 1 <= order <= n_levels (the caller folds the full recursion into
+// order = n_levels); diag_fits(q, n_levels, order) must hold
+hipError_t lowrank_diag_scan_launch(int dtype, const void* du, int q, int n_levels, int order,
+                                    int64_t n_seq, int steps, int half, double* out,
+                                    hipStream_t s) {
+  if (!diag_fits(q, n_levels, order)) return hipErrorInvalidValue;
+  return dtype == KSIG_F32 ? diag_q<float>(q, n_levels, order, du, n_seq, steps, half, out, s)
+                           : diag_q<double>(q, n_levels, order, du, n_seq, steps, half, out, s);
+}
+
+// The level recursion with TensorSketch projections (ksig/projections.py:
+// 379-458): level i's projection keeps a hash h_i[j] in [0, m) and a sign
+// s_i[j] per input feature j < width; _project is the count sketch
+// CS_i(u)[k] = sum_{j: h_i[j] = k} s_i[j] u[j] (ksig/utils.py:279-298) and the
+// outer-product step is the circular convolution A (*) CS_i(u) of
+// ksig/utils.py:301-313, which the sparse sketch turns into
+//   (A (*) CS_i(u))[k] = sum_j s_i[j] u[j] A[(k - h_i[j]) mod m]
+// (width terms per component instead of three length-m FFTs), with level 0's
+// count sketch the same sum over the unit vector A = e_0.  The recursion is
+// that of lowrank_scan_kernel with (*) in place of the elementwise product:
+//   R_0[0](t) = CS_0(u_0(t)),  R_i[0](t) = C_{i-1}(t) (*) CS_i(u_i(t)),
+//   R_i[r](t) = R_{i-1}[r-1](t) (*) CS_i(u_i(t)) / (r+1),   P_i = C_i(end).
+// A convolution reads every component of its source, so one workgroup walks
+// one sequence's time steps: each thread owns kSketchKpt components and their
+// running sums (float64 registers); the sources a convolution reads (e_0, the
+// running sums C_0..C_{NL-2} and, at order p > 1, this step's terms of the
+// levels below the top) sit in LDS as rows of 2m doubles holding the vector
+// twice, so that A[(k - h) mod m] is row[k + m - h] with no wrap test.
+// Order 1: every level reads only the previous step's sums, so one pass of
+// convolutions, a barrier, the sum updates and a second barrier per step.
+// Order p > 1: level i reads level i-1's terms of the same step, so a barrier
+// after each level, then the sum updates.  A hash value outside [0, m)
+// selects no component (the reference's 0/1 mask has no column for it).
+constexpr int kSketchKpt = 2;
+
+template <typename T>
+__device__ __forceinline__ void sketch_conv(const double* __restrict__ row, const T* __restrict__ u,
+                                            const int* __restrict__ h, const T* __restrict__ sg,
+                                            int width, int m, const int* kc, double* acc) {
+#pragma unroll
+  for (int c = 0; c < kSketchKpt; ++c) acc[c] = 0.0;
+  for (int j = 0; j < width; ++j) {
+    const double cj = (double)sg[j] * (double)u[j];
+    const int hj = h[j];
+    if ((unsigned)hj >= (unsigned)m) continue;  // no component: the mask has no column
+    const double* r = row + m - hj;
+#pragma unroll
+    for (int c = 0; c < kSketchKpt; ++c)
+      if (kc[c] < m) acc[c] = __builtin_fma(cj, r[kc[c]], acc[c]);
+  }
+}
+
+
+template <typename T, int NL>
+__global__ void __launch_bounds__(1024)
+lowrank_sketch_kernel(const T* __restrict__ du, int64_t n_seq, int steps, int width,
+                      const int* __restrict__ hidx, const T* __restrict__ hbit, int m, int order,
+                      double* __restrict__ out) {
+  extern __shared__ double sk_lds[];
+  const int64_t a = blockIdx.x;
+  const int m2 = 2 * m;
+  const int rows = sketch_rows(NL, order);
+  for (int j = threadIdx.x; j < rows * m2; j += blockDim.x)
+    sk_lds[j] = (j == 0 || j == m) ? 1.0 : 0.0;  // row 0: e_0 twice; the rest zero
+  int kc[kSketchKpt];
+#pragma unroll
+  for (int c = 0; c < kSketchKpt; ++c) kc[c] = threadIdx.x + c * blockDim.x;
+  const int64_t level_stride = n_seq * (int64_t)steps * width;
+  const T* ua = du + a * (int64_t)steps * width;
+  auto crow = [&](int i) { return sk_lds + (1 + i) * m2; };  // C_i, i < NL-1
+  // at order p > 1: level i's order-r terms of this step, i < NL-1
+  auto rrow = [&](int i, int r) {
+    int base = NL;
+    for (int l = 0; l < i; ++l) base += l + 1 < order ? l + 1 : order;
+    return sk_lds + (base + r) * m2;
+  };
+  auto put = [&](double* row, int c, double v) {
+    if (kc[c] < m) {
+      row[kc[c]] = v;
+      row[kc[c] + m] = v;
+    }
+  };
+  double C[NL][kSketchKpt];
+#pragma unroll
+  for (int i = 0; i < NL; ++i)
+#pragma unroll
+    for (int c = 0; c < kSketchKpt; ++c) C[i][c] = 0.0;
+  __syncthreads();
+  for (int t = 0; t < steps; ++t) {
+    double S[NL][kSketchKpt];  // this step's terms summed over the orders
+    if (order == 1) {
+#pragma unroll
+      for (int i = 0; i < NL; ++i)
+        sketch_conv<T>(i == 0 ? sk_lds : crow(i - 1), ua + i * level_stride + (int64_t)t * width,
+                       hidx + i * width, hbit + i * width, width, m, kc, S[i]);
+      __syncthreads();  // every convolution has read the sums
+    } else {
+#pragma unroll
+      for (int i = 0; i < NL; ++i) {
+        const T* u = ua + i * level_stride + (int64_t)t * width;
+        const int d = i + 1 < order ? i + 1 : order;
+#pragma unroll
+        for (int c = 0; c < kSketchKpt; ++c) S[i][c] = 0.0;
+        for (int r = 0; r < d; ++r) {
+          double v[kSketchKpt];
+          const double* src = i == 0 ? sk_lds : (r == 0 ? crow(i - 1) : rrow(i - 1, r - 1));
+          sketch_conv<T>(src, u, hidx + i * width, hbit + i * width, width, m, kc, v);
+          const double w = 1.0 / (r + 1);
+#pragma unroll
+          for (int c = 0; c < kSketchKpt; ++c) {
+            v[c] *= w;
+            S[i][c] += v[c];
+            if (i + 1 < NL) put(rrow(i, r), c, v[c]);
+          }
+        }
+        __syncthreads();  // level i's terms are visible to level i+1
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < NL; ++i)
+#pragma unroll
+      for (int c = 0; c < kSketchKpt; ++c) {
+        C[i][c] += S[i][c];
+        if (i + 1 < NL) put(crow(i), c, C[i][c]);
+      }
+    __syncthreads();  // the sums are updated before the next step reads them
+  }
+  double* o = out + a * (int64_t)NL * m;
+#pragma unroll
+  for (int i = 0; i < NL; ++i)
+#pragma unroll
+    for (int c = 0; c < kSketchKpt; ++c)
+      if (kc[c] < m) o[(int64_t)i * m + kc[c]] = C[i][c];
+}
+
+// threads per workgroup for m components and the LDS bytes of one workgroup
+__host__ inline int sketch_threads(int m) {
+  const int per = (m + kSketchKpt - 1) / kSketchKpt;
+  return ((per + kWave - 1) / kWave) * kWave;
+}
+size_t lowrank_sketch_lds(int n_levels, int order, int m) {
+  return (size_t)sketch_rows(n_levels, order) * 2 * (size_t)m * sizeof(double);
+}
+int lowrank_sketch_max_comp() { return 1024 * kSketchKpt; }
+
+template <typename T, int NL>
+static hipError_t launch_sketch(const void* du, int64_t n_seq, int steps, int width,
+                                const int* hidx, const void* hbit, int m, int order,
+                                double* out, hipStream_t s) {
+  auto k = lowrank_sketch_kernel<T, NL>;
+  const size_t lds = lowrank_sketch_lds(NL, order, m);
+  if (lds > 64 * 1024) {
+    hipError_t e = hipFuncSetAttribute((const void*)k,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k, dim3((unsigned)n_seq), dim3(sketch_threads(m)), lds, s,
+                     reinterpret_cast<const T*>(du), n_seq, steps, width, hidx,
+                     reinterpret_cast<const T*>(hbit), m, order, out);
   return hipGetLastError();
 }
 
-hipError_t lowrank_diag_scan_launch(int dtype, const void* du, int n_levels, int64_t n_seq,
-                                    int steps, int half, double* out, hipStream_t s) {
+template <typename T>
+static hipError_t sketch_levels(int n_levels, const void* du, int64_t n_seq, int steps, int width,
+                                const int* hidx, const void* hbit, int m, int order, double* out,
+                                hipStream_t s) {
   switch (n_levels) {
-#define KSIG_DIAG_CASE(L)                                                          \
-    case L:                                                                      \
-      return dtype == KSIG_F32 ? launch_diag<float, L>(du, n_seq, steps, half, out, s) \
-                               : launch_diag<double, L>(du, n_seq, steps, half, out, s);
-    KSIG_DIAG_CASE(1)
-    KSIG_DIAG_CASE(2)
-    KSIG_DIAG_CASE(3)
-    KSIG_DIAG_CASE(4)
-    KSIG_DIAG_CASE(5)
-#undef KSIG_DIAG_CASE
+#define KSIG_SKETCH_CASE(L) \
+    case L: return launch_sketch<T, L>(du, n_seq, steps, width, hidx, hbit, m, order, out, s);
+    KSIG_SKETCH_CASE(1)
+    KSIG_SKETCH_CASE(2)
+    KSIG_SKETCH_CASE(3)
+    KSIG_SKETCH_CASE(4)
+    KSIG_SKETCH_CASE(5)
+    KSIG_SKETCH_CASE(6)
+    KSIG_SKETCH_CASE(7)
+    KSIG_SKETCH_CASE(8)
+#undef KSIG_SKETCH_CASE
   }
   return hipErrorInvalidValue;
+}
+
+// 1 <= order <= n_levels <= 8, 1 <= m <= lowrank_sketch_max_comp(), the LDS of
+// lowrank_sketch_lds within the device's limit (checked by the caller)
+hipError_t lowrank_sketch_launch(int dtype, const void* du, int n_levels, int order,
+                                 int64_t n_seq, int steps, int width, const int* hidx,
+                                 const void* hbit, int m, double* out, hipStream_t s) {
+  if (dtype == KSIG_F32)
+    return sketch_levels<float>(n_levels, du, n_seq, steps, width, hidx, hbit, m, order, out, s);
+  return sketch_levels<double>(n_levels, du, n_seq, steps, width, hidx, hbit, m, order, out, s);
 }
 
 hipError_t lowrank_scan_launch(int dtype, const void* z, int n_levels, int order,

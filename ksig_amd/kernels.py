@@ -355,14 +355,18 @@ class SignatureFeatures(SignatureBase, static.features.KernelFeatures):
   from library GEMMs and the level recursion (exclusive time cumsum,
   elementwise product, time sum of every level) is one HIP pass
   (``ksig_lowrank_levels``).  With a ``DiagonalProjection`` (RFSF-DP) the
-  differenced random Fourier features go straight into the diagonal level
-  scan (``ksig_lowrank_levels_diag``).  Gaussian, subsampling and very
-  sparse projections run the reference's projected recursion at any order in
-  float64 torch ops on the GPU (one GEMM per projected outer product); TRP
-  also runs at any order.  Without a projection the level-m features are the
-  D^m iterated-sum coordinates (torch ops on the GPU, small D only).  Not
-  built: TensorSketch, TRP with rank > 1, and DiagonalProjection beyond
-  order 1, internal_size 2 and 5 levels."""
+  differenced lifted features go straight into the diagonal level scan
+  (``ksig_lowrank_levels_diagonal``: internal size 1-4, up to 5 levels, any
+  order whose state fits the registers); with a ``TensorSketch`` every
+  level's count sketch and convolution run in one scan per sequence
+  (``ksig_lowrank_levels_sketch``, up to 8 levels and 2048 components).
+  Gaussian, subsampling and very sparse projections, and the diagonal and
+  sketch shapes those scans are not built for, run the reference's
+  projected recursion at any order in float64 torch ops on the GPU (one
+  projection call per outer product); TRP also runs at any order (rank > 1
+  raises the reference's ValueError).  Without a projection the level-m
+  features are the D^m iterated-sum coordinates (torch ops on the GPU, small
+  D only)."""
 
   def __init__(self, n_levels: int = 5, order: int = 1, difference: bool = True,
                normalize: bool = True, indep_feat: bool = True,
@@ -380,20 +384,10 @@ class SignatureFeatures(SignatureBase, static.features.KernelFeatures):
 
   def _make_feature_components(self, X):
     """ksig/kernels.py:295-328."""
-    from .projections import (DiagonalProjection, GaussianRandomProjection,
-                              SubsamplingProjection, TensorizedRandomProjection,
-                              VerySparseRandomProjection)
-    if self.projection is not None and not isinstance(
-        self.projection, (TensorizedRandomProjection, DiagonalProjection,
-                          GaussianRandomProjection, SubsamplingProjection,
-                          VerySparseRandomProjection)):
-      raise NotImplementedError('low-rank signature features are built with a tensorized, '
-                                'diagonal, Gaussian, subsampling or very sparse projection, '
-                                'or none')
-    if isinstance(self.projection, DiagonalProjection) and (
-        self.projection.internal_size != 2 or self.n_levels > 5 or self._order() != 1):
-      raise NotImplementedError('the diagonal projection is built for internal_size=2, '
-                                'at most 5 levels and order=1')
+    from .projections import RandomProjection
+    if self.projection is not None and not isinstance(self.projection, RandomProjection):
+      raise NotImplementedError('low-rank signature features take a projection of '
+                                'ksig_amd.projections, or none')
     if self.static_features is not None:
       seq_shape = tuple(X.shape[:-1]) + (-1,)
       Xf = X.reshape(-1, X.shape[-1])
@@ -465,26 +459,46 @@ class SignatureFeatures(SignatureBase, static.features.KernelFeatures):
     U = self._increments(X)
     n = X.shape[0]
     ones = torch.ones((n, 1), device=X.device, dtype=torch.float64)
-    from .projections import DiagonalProjection, TensorizedRandomProjection
-    if self.projections_ is not None and isinstance(self.projections_[0], DiagonalProjection):
-      # RFSF-DP: the (sin, cos) increments go straight into the level scan
-      lev = _ops.lowrank_levels_diag(U.contiguous())
-      H = U[0].shape[-1] // 2
-      P, off = [ones], 0
-      for i in range(self.n_levels):
-        w = (2 << i) * H
-        P.append(lev[:, off:off + w])
-        off += w
-    elif self.projections_ is not None and not isinstance(self.projections_[0],
-                                                           TensorizedRandomProjection):
-      # any other projection: the reference's recursion (ksig/algorithms.py:
-      # 218-246, :296-337) with the projection called on each (running sum,
-      # increments) pair, in float64
+    from .projections import DiagonalProjection, TensorizedRandomProjection, TensorSketch
+    proj0 = self.projections_[0] if self.projections_ is not None else None
+    P = None
+    if isinstance(proj0, DiagonalProjection):
+      # RFSF-DP: the lifted increments go straight into the diagonal level
+      # scan (any internal size and order whose state fits its registers)
+      q = proj0.internal_size
+      try:
+        lev = _ops.lowrank_levels_diagonal(U.contiguous(), q, self._order())
+      except NotImplementedError:
+        lev = None
+      if lev is not None:
+        H = U.shape[-1] // q
+        P, off = [ones], 0
+        for i in range(self.n_levels):
+          w = q ** (i + 1) * H
+          P.append(lev[:, off:off + w])
+          off += w
+    elif isinstance(proj0, TensorSketch):
+      # every level's count sketch and convolution in one scan per sequence
+      hidx = torch.stack([p.hash_idx_.to(U.device) for p in self.projections_], 0)
+      hbit = torch.stack([p.hash_bit_.to(device=U.device, dtype=U.dtype)
+                          for p in self.projections_], 0)
+      try:
+        lev = _ops.lowrank_levels_sketch(U, hidx, hbit, proj0.n_components, self._order())
+        P = [ones] + [lev[:, m] for m in range(self.n_levels)]
+      except NotImplementedError:
+        P = None
+    if P is not None:
+      pass  # a HIP scan above produced the level features
+    elif proj0 is not None and not isinstance(proj0, TensorizedRandomProjection):
+      # any other projection, and the diagonal / sketch shapes the scans are
+      # not built for: the reference's recursion (ksig/algorithms.py:218-246,
+      # :296-337) with the projection called on each (running sum,
+      # increments) pair, in float64 torch ops on the GPU
       proj = self.projections_
       order = self._order()
       Ud = U.to(torch.float64)
       Rs = [proj[0](Ud[0], return_on_gpu=True)]
-      P = [ones, Rs[0].sum(1)]
+      P = [ones, Rs[0].sum(1).reshape(n, -1)]
       for i in range(1, self.n_levels):
         S = sum(Rs)
         Q = torch.cumsum(S, 1) - S  # exclusive cumsum over time
@@ -492,8 +506,16 @@ class SignatureFeatures(SignatureBase, static.features.KernelFeatures):
         for r in range(1, min(i + 1, order)):
           nxt.append(proj[i](Rs[r - 1], Ud[i], return_on_gpu=True) / (r + 1))
         Rs = nxt
-        P.append(sum(R.sum(1) for R in Rs))
+        P.append(sum(R.sum(1) for R in Rs).reshape(n, -1))
     elif self.projections_ is not None:
+      # the reference reshapes each level's projection to [..., n_components,
+      # rank] (ksig/algorithms.py:210-214), which fails for rank > 1 since the
+      # components hold n_components columns; fitting refuses that rank for
+      # the levels after the first, this is the single-level case
+      p0 = self.projections_[0]
+      if p0.rank != 1:
+        raise ValueError(f'cannot reshape array of size {n * U.shape[2] * p0.n_components} '
+                         f'into shape {(n, U.shape[2], p0.n_components, p0.rank)}')
       # projected increments, one batched GEMM over the levels: level 0
       # alone (scaled), level m >= 1 the projection of U_m that multiplies
       # the running sums

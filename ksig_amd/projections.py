@@ -1,11 +1,10 @@
 """Random projections for the low-rank signature features: the
 ``GaussianRandomProjection`` of ksig/projections.py:189-242, the
 ``SubsamplingProjection`` of :245-284, the ``VerySparseRandomProjection`` of
-:287-376, the
-``TensorizedRandomProjection`` of :465-584 (rank 1) and the
-``DiagonalProjection`` of :587-674 over the ``RandomProjection`` base
-(:20-187).  Components are drawn on the host
-with NumPy (the reference draws them with CuPy)."""
+:287-376, the ``TensorSketch`` of :379-458, the
+``TensorizedRandomProjection`` of :465-584 and the ``DiagonalProjection`` of
+:587-674 over the ``RandomProjection`` base (:20-187).  Components are drawn
+on the host with NumPy (the reference draws them with CuPy)."""
 
 from __future__ import annotations
 
@@ -150,18 +149,67 @@ class VerySparseRandomProjection(RandomProjection):
         tuple(XY.shape[:-1]) + (-1,))
 
 
+class TensorSketch(RandomProjection):
+  """Tensor sketch by convolution of count sketches (ksig/projections.py:
+  379-458): each input feature j gets a hash h[j] in [0, n_components) and a
+  Rademacher sign s[j]; X -> the count sketch CS(X)[k] = sum_{j: h[j] = k}
+  s[j] X[..., j] (ksig/utils.py:279-298), and (X, Y) -> the circular
+  convolution of an already-sketched X with CS(Y) along the last axis
+  (ksig/utils.py:301-313; an FFT of length n_components here as there).
+  Inside SignatureFeatures the whole level recursion runs in one HIP scan
+  (ksig_lowrank_levels_sketch)."""
+
+  def _validate_format(self, X, Y=None, reset: bool = False):
+    """ksig/projections.py:393-418."""
+    n_features = Y.shape[-1] if Y is not None else X.shape[-1]
+    if reset or not hasattr(self, 'n_features_') or self.n_features_ is None:
+      self.n_features_ = n_features
+    elif n_features != self.n_features_:
+      raise ValueError('Received data with a different number of features than at fit time.',
+                       f'({n_features} != {self.n_features_})')
+
+  def _make_projection_components(self, X, Y=None):
+    """ksig/projections.py:420-430: the hashes, then the signs
+    (draw_rademacher_matrix, ksig/utils.py:215-230: +1 where a uniform draw
+    is below 1/2)."""
+    idx = self.random_state.randint(self.n_components, size=[self.n_features_])
+    bit = np.where(self.random_state.uniform(size=[self.n_features_]) < 0.5, 1., -1.)
+    self.hash_idx_ = torch.as_tensor(idx, device=X.device, dtype=torch.int64)
+    self.hash_bit_ = torch.as_tensor(bit, device=X.device, dtype=X.dtype)
+
+  def _count_sketch(self, X):
+    idx = self.hash_idx_.to(X.device)
+    bit = self.hash_bit_.to(device=X.device, dtype=X.dtype)
+    out = torch.zeros(tuple(X.shape[:-1]) + (self.n_components,), device=X.device,
+                      dtype=X.dtype)
+    return out.index_add_(-1, idx, X * bit)
+
+  def _project(self, X):
+    return self._count_sketch(X)
+
+  def _project_outer_prod(self, X, Y):
+    n = self.n_components
+    return torch.fft.irfft(torch.fft.rfft(X, n=n, dim=-1) *
+                           torch.fft.rfft(self._count_sketch(Y), n=n, dim=-1), n=n, dim=-1)
+
+
+CountSketch = TensorSketch
+CountSketchRandomProjection = TensorSketch
+
+
 class TensorizedRandomProjection(RandomProjection):
-  """Tensorized random projection of rank 1 (ksig/projections.py:465-584):
-  X -> X C / sqrt(n_components) alone, and (X, Y) -> X * (Y C) for an
-  already-projected X (the recursive step of a tensor's projection).  Ranks
-  above 1 are not built (the reference's components hold n_components
-  columns whatever the rank)."""
+  """Tensorized random projection (ksig/projections.py:465-584): X -> X C /
+  sqrt(n_components) alone, and (X, Y) -> X * (Y C) for an already-projected
+  X (the recursive step of a tensor's projection).  ``rank`` behaves as in
+  the reference: the components hold n_components columns whatever the rank
+  (:538-548), the outer-product step expects X with n_components * rank
+  features (:527-531), so a rank above 1 projects standalone data and is
+  refused with the reference's ValueError when fitted as a recursive step
+  (the low-rank signature features with more than one level)."""
 
   def __init__(self, n_components: int = 100, rank: int = 1, random_state=None):
     super().__init__(n_components=n_components, random_state=random_state)
     self.rank = utils.check_positive_value(rank, 'rank')
-    if rank != 1:
-      raise NotImplementedError('TensorizedRandomProjection is built for rank 1')
 
   def _validate_format(self, X, Y=None, reset: bool = False):
     """ksig/projections.py:500-536."""

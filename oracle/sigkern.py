@@ -29,7 +29,9 @@ difference it, per-level exclusive 2-D cumulative sums):
                               Fourier features with phases :344-390,
                               ksig/kernels.py:263-361, random Fourier features
                               ksig/static/features.py:288-342, tensorized
-                              random projection ksig/projections.py:465-584
+                              random projection ksig/projections.py:465-584,
+                              diagonal projection :587-674, TensorSketch
+                              :379-458 with ksig/utils.py:279-313
 
 Additions that have no reference counterpart (build extensions):
 ``dyadic_order`` for the PDE (each ΔΔ cell refined into 2^λ×2^λ sub-cells
@@ -48,7 +50,11 @@ first-order DP with a linear static kernel on the lifted path, with a
 tensorized projection vs brute-force iterated-sum tensors contracted with
 the rank-1 projection tensors; higher-order low-rank features without
 projection vs the order-p DP with a linear static kernel, with a tensorized
-projection vs the unprojected ones contracted with the projection tensors.
+projection vs the unprojected ones contracted with the projection tensors;
+diagonal projection (any internal size and order) vs the unprojected
+features of each component's q-dimensional path scaled by sqrt(H)^(m-1);
+TensorSketch (any order) vs the count sketch of the unprojected features
+under the summed hash and the multiplied signs (tests/test_oracle_lowrank.py).
 """
 
 from __future__ import annotations
@@ -639,6 +645,81 @@ def signature_kern_low_rank_gauss(U, n_levels, comps, order=1, difference=True):
     nxt = [proj(i, Q, Us[i])]
     for r in range(1, min(i + 1, order)):
       nxt.append(proj(i, Rs[r - 1], Us[i]) / (r + 1))
+    Rs = nxt
+    P.append(sum(R.sum(1) for R in Rs))
+  return P
+
+
+def signature_kern_low_rank_diag(U, n_levels, internal_size=2, order=1, difference=True):
+  """Low-rank signature features with a DiagonalProjection of any internal
+  size q (ksig/projections.py:587-674) at any embedding order (first order
+  ksig/algorithms.py:175-246; higher order :249-337, whose R_next holds
+  [d, n, L, q^(i+1), H] entries for this projection, :302-306): level 0 is
+  U_0 split into [q, H] (:649-659, H = D / q); the outer-product step is
+  sqrt(H) A[..., :, None, :] * Y[..., None, :, :] flattened over the two
+  internal axes with Y = U_i split into [q, H] (:661-674; H is the fitted
+  step's n_features_, the last axis of the projected level 0).  Level m's
+  features are [n, q^m * H], the internal index of level 0 most significant."""
+  order = normalize_order(n_levels, order)
+  Us = U if isinstance(U, list) else [U] * n_levels
+  if difference:
+    Us = [np.diff(u, axis=1) for u in Us]
+  n, steps, D = Us[0].shape
+  q = internal_size
+  H = D // q
+  split = lambda A: A.reshape(A.shape[:-1] + (q, -1))
+
+  def outer(A, i):
+    Y = split(Us[i])
+    return np.sqrt(H) * (A[..., :, None, :] * Y[..., None, :, :]).reshape(A.shape[:2] + (-1, H))
+
+  Rs = [split(Us[0])]
+  P = [np.ones((n, 1), dtype=Us[0].dtype), Rs[0].sum(1).reshape(n, -1)]
+  for i in range(1, n_levels):
+    Q = multi_cumsum(sum(Rs), exclusive=True, axis=1)
+    nxt = [outer(Q, i)]
+    for r in range(1, min(i + 1, order)):
+      nxt.append(outer(Rs[r - 1], i) / (r + 1))
+    Rs = nxt
+    P.append(sum(R.sum(1) for R in Rs).reshape(n, -1))
+  return P
+
+
+def count_sketch(X, hash_idx, hash_bit, n_components):
+  """compute_count_sketch (ksig/utils.py:279-298): the einsum of X with the
+  0/1 mask [hash_idx[i] == j] and the signs, out[..., j] = sum over i with
+  hash_idx[i] = j of hash_bit[i] X[..., i]."""
+  mask = (np.asarray(hash_idx)[:, None] == np.arange(n_components)[None, :]).astype(X.dtype)
+  return np.einsum('...i,ij,i->...j', X, mask, np.asarray(hash_bit, dtype=X.dtype))
+
+
+def convolve_fft(X, Y):
+  """convolve_fft (ksig/utils.py:301-313): the real part of the inverse FFT
+  of the product of the FFTs along the last axis (circular convolution)."""
+  return np.real(np.fft.ifft(np.fft.fft(X, axis=-1) * np.fft.fft(Y, axis=-1), axis=-1))
+
+
+def signature_kern_low_rank_sketch(U, n_levels, sketches, order=1, difference=True):
+  """Low-rank signature features with TensorSketch projections
+  (ksig/projections.py:379-458) at any embedding order (ksig/algorithms.py:
+  175-246, :249-337): sketches[i] = (hash_idx [D], hash_bit [D], m) of level
+  i's fitted projection; level 0 is the count sketch of U_0 (_project,
+  :438-448); level i >= 1 projects (A, U_i) to convolve_fft(A, count sketch
+  of U_i) (_project_outer_prod, :450-465).  Level features [n, m]."""
+  order = normalize_order(n_levels, order)
+  Us = U if isinstance(U, list) else [U] * n_levels
+  if difference:
+    Us = [np.diff(u, axis=1) for u in Us]
+  n = Us[0].shape[0]
+  cs = lambda i, A: count_sketch(A, *sketches[i])
+  Rs = [cs(0, Us[0])]
+  P = [np.ones((n, 1), dtype=Us[0].dtype), Rs[0].sum(1)]
+  for i in range(1, n_levels):
+    Z = cs(i, Us[i])
+    Q = multi_cumsum(sum(Rs), exclusive=True, axis=1)
+    nxt = [convolve_fft(Q, Z)]
+    for r in range(1, min(i + 1, order)):
+      nxt.append(convolve_fft(Rs[r - 1], Z) / (r + 1))
     Rs = nxt
     P.append(sum(R.sum(1) for R in Rs))
   return P

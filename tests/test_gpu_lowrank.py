@@ -69,11 +69,6 @@ def test_unprojected_features_match_oracle():
     sf = ksig.kernels.SignatureFeatures(n_levels=3, order=order, normalize=False).fit(X)
     ref = np.concatenate(O.signature_kern_low_rank(X, 3, order), -1)
     assert normwise(sf.transform(X), ref) < 1e-11, order
-  with pytest.raises(NotImplementedError):
-    ksig.kernels.SignatureFeatures(
-        n_levels=3, order=2,
-        static_features=ksig.static.features.RandomFourierFeatures(n_components=8),
-        projection=ksig.projections.DiagonalProjection()).fit(X)
 
 
 @pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-11), (np.float32, 1e-5)])
@@ -140,10 +135,6 @@ def test_rfsf_dp_class():
   ref = O.signature_features(X.astype(np.float64), 4, w, 'diag')
   assert F.shape == (7, 1 + (2 + 4 + 8 + 16) * 20) and normwise(F, ref) < 1e-5
   np.testing.assert_allclose(sf(X, diag=True), 1.0, rtol=1e-5)
-  with pytest.raises(NotImplementedError):
-    ksig.kernels.SignatureFeatures(n_levels=6, projection=ksig.projections.DiagonalProjection(),
-                                   static_features=ksig.static.features.RandomFourierFeatures(
-                                       n_components=4)).fit(X)
 
 
 @pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-12), (np.float32, 1e-5)])

@@ -194,3 +194,78 @@ def test_gaussian_projection_features_are_linear_images(order):
     C, sc = comps[m - 1]
     A = sc * C @ np.kron(A, np.eye(D))
     np.testing.assert_allclose(feats[m], full[m] @ A.T, rtol=1e-11, atol=1e-12)
+
+
+@pytest.mark.parametrize("q,n_levels,order", [(2, 3, 1), (2, 3, 2), (3, 3, 3), (1, 4, 2),
+                                              (4, 2, 1)])
+def test_diagonal_features_are_per_component_unprojected_features(q, n_levels, order):
+  """DiagonalProjection of internal size q at order p: for each component k
+  the level-m block [q^m, H] at column k equals the unprojected order-p
+  features of the q-dimensional path (U[..., b H + k])_b, scaled by
+  sqrt(H)^(m-1) (the projection is diagonal over k and the outer product
+  runs over the internal axis only).  A wrong scale, a swapped internal
+  order or an order term dropped fails it."""
+  rng = np.random.default_rng(20 + q + 3 * order)
+  H = 3
+  U = rng.standard_normal((2, 6, q * H))
+  dp = O.signature_kern_low_rank_diag(U, n_levels, q, order)
+  for k in range(H):
+    full = O.signature_kern_low_rank(U[..., k::H], n_levels, order, True, None)
+    for m in range(1, n_levels + 1):
+      got = dp[m].reshape(2, q ** m, H)[..., k]
+      np.testing.assert_allclose(got, np.sqrt(H) ** (m - 1) * full[m], rtol=1e-11, atol=1e-12)
+  if q == 2 and order == 1:
+    fo = O.signature_kern_low_rank_fo(U, n_levels, True, 'diag')
+    for a, b in zip(dp, fo):
+      np.testing.assert_allclose(a, b, rtol=1e-13, atol=1e-13)
+
+
+def _sketch_of_tensor(T, sketches):
+  """Count sketch of a flattened [D0 x D1 x ... x Dm] tensor (index of
+  level 0 most significant) under the summed hash (h_0(i_0) + ... + h_m(i_m))
+  mod n and the multiplied signs, by scatter-add."""
+  h = np.zeros((), dtype=np.int64)
+  s = np.ones(())
+  for idx, bit, n in sketches:
+    h = np.add.outer(h, np.asarray(idx))
+    s = np.multiply.outer(s, np.asarray(bit))
+  n = sketches[0][2]
+  h = (h % n).reshape(-1)
+  s = s.reshape(-1)
+  out = np.zeros(T.shape[:-1] + (n,))
+  for j in range(h.size):
+    out[..., h[j]] += s[j] * T[..., j]
+  return out
+
+
+def test_convolved_count_sketches_are_the_sketch_of_the_outer_product():
+  """convolve_fft(CS_1(x), CS_2(y)) = CS_h(x (x) y) with h = h_1 + h_2 mod n
+  and signs s_1 s_2; a cross-correlation (conjugated FFT) instead fails."""
+  rng = np.random.default_rng(30)
+  n = 7
+  sk = [(rng.integers(0, n, 5), rng.choice([-1.0, 1.0], 5), n),
+        (rng.integers(0, n, 4), rng.choice([-1.0, 1.0], 4), n)]
+  x, y = rng.standard_normal(5), rng.standard_normal(4)
+  got = O.convolve_fft(O.count_sketch(x, *sk[0]), O.count_sketch(y, *sk[1]))
+  want = _sketch_of_tensor(np.multiply.outer(x, y).reshape(-1), sk)
+  np.testing.assert_allclose(got, want, atol=1e-12)
+  a, b = O.count_sketch(x, *sk[0]), O.count_sketch(y, *sk[1])
+  corr = np.real(np.fft.ifft(np.conj(np.fft.fft(a)) * np.fft.fft(b)))
+  assert np.abs(corr - want).max() > 1e-3
+
+
+@pytest.mark.parametrize("n_levels,order", [(1, 1), (3, 1), (3, 2), (4, 3)])
+def test_sketch_features_are_sketches_of_unprojected_features(n_levels, order):
+  """TensorSketch features of level m are the count sketch of the
+  unprojected level-m features (a flattened [D]^m tensor) under the summed
+  hash of levels 0..m-1 and their multiplied signs: every step is linear,
+  and a convolution of count sketches sketches the outer product."""
+  rng = np.random.default_rng(40 + n_levels + order)
+  D, n = 3, 5
+  U = rng.standard_normal((2, 6, D))
+  sk = [(rng.integers(0, n, D), rng.choice([-1.0, 1.0], D), n) for _ in range(n_levels)]
+  feats = O.signature_kern_low_rank_sketch(U, n_levels, sk, order)
+  full = O.signature_kern_low_rank(U, n_levels, order, True, None)
+  for m in range(1, n_levels + 1):
+    np.testing.assert_allclose(feats[m], _sketch_of_tensor(full[m], sk[:m]), rtol=1e-11,
+                               atol=1e-12)
