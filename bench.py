@@ -809,9 +809,50 @@ def bench_rfsf(args):
   errd = float(np.max(np.abs(Fd[idx].double().cpu().numpy() - refd).max(-1) /
                       np.abs(refd).max(-1)))
   res["dp"] = {"ms_per_transform": dp_ms, "features": int(Fd.shape[1]),
-               "max_rel_err_vs_fp64": errd, "scan_kernel": "lowrank_diag_scan_kernel<float,5>",
+               "max_rel_err_vs_fp64": errd,
+               "scan_kernel": "lowrank_diag_scan_kernel<float,2,5,1>",
                "scan_ms": dscan_ms, "scan_bytes": dscan_bytes,
                "scan_GBps": dscan_bytes / (dscan_ms * 1e-3) / 1e9}
+  # RFSF-TS on the same input: TensorSketch of the RFF increments, 100
+  # sketch components
+  ts = ksig.kernels.SignatureFeatures(
+      n_levels=nl, static_features=ksig.static.features.RandomFourierFeatures(
+          n_components=D, random_state=0),
+      projection=ksig.projections.TensorSketch(n_components=P, random_state=2))
+  ts.fit(X)
+  for _ in range(args.warmup):
+    ts.transform(X, return_on_gpu=True)
+  torch.cuda.synchronize()
+  t0 = time.perf_counter()
+  for _ in range(args.steps):
+    Ft = ts.transform(X, return_on_gpu=True)
+  torch.cuda.synchronize()
+  ts_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+  hidx = torch.stack([p.hash_idx_ for p in ts.projections_], 0)
+  hbit = torch.stack([p.hash_bit_.float() for p in ts.projections_], 0)
+  _ops.lowrank_levels_sketch(dU, hidx, hbit, P)
+  e0.record(stream)
+  for _ in range(args.steps):
+    _ops.lowrank_levels_sketch(dU, hidx, hbit, P)
+  e1.record(stream)
+  torch.cuda.synchronize()
+  tscan_ms = e0.elapsed_time(e1) / args.steps
+  # one multiply-add per (input feature, sketch component) of each level's
+  # convolution at every step: 2 * width * P flop per level-step
+  tscan_flop = 2.0 * n * (L - 1) * nl * dU.shape[-1] * P
+  wt = [f.random_weights_.double().cpu().numpy() for f in ts.static_features_]
+  skt = [(p.hash_idx_.cpu().numpy(), p.hash_bit_.double().cpu().numpy(), P)
+         for p in ts.projections_]
+  Ut = [O.rff_features(X[idx].double().cpu().numpy(), wi) for wi in wt]
+  reft = O.low_rank_features_normalised(O.signature_kern_low_rank_sketch(Ut, nl, skt), nl)
+  errt = float(np.max(np.abs(Ft[idx].double().cpu().numpy() - reft).max(-1) /
+                      np.abs(reft).max(-1)))
+  res["ts"] = {"ms_per_transform": ts_ms, "features": int(Ft.shape[1]),
+               "max_rel_err_vs_fp64": errt, "scan_kernel": "lowrank_sketch_kernel<float,5>",
+               "scan_ms": tscan_ms, "scan_flop": tscan_flop,
+               "scan_TFLOPs": tscan_flop / (tscan_ms * 1e-3) / 1e12,
+               "flop_note": "float64 multiply-adds of the direct convolution from the sparse "
+                            "count sketch, 2 * width * n_comp per level-step"}
   print(json.dumps(res), flush=True)
 
 

@@ -527,8 +527,9 @@ int ksig_lowrank_levels_diagonal(int dtype, const void* du, int n_levels, int or
  * evaluated directly from the sparse count sketch (width products per
  * component), equal to the FFT route up to rounding.  1 <= n_levels <= 8,
  * n_comp <= 2048 and the workgroup's LDS ((n_levels + the order terms of
- * the levels below the top) * 2 * n_comp doubles) within the device's
- * limit, else KSIG_EUNSUPPORTED and nothing is launched.
+ * the levels below the top) * 2 * n_comp doubles, plus 20 bytes per level
+ * and input feature for the staged hashes, signs and increments) within the
+ * device's limit, else KSIG_EUNSUPPORTED and nothing is launched.
  */
 int ksig_lowrank_levels_sketch(int dtype, const void* du, int n_levels, int order, int64_t n_seq,
                                int steps, int width, const int32_t* hash_idx,

@@ -26,8 +26,9 @@ hipError_t lowrank_diag_scan_launch(int dtype, const void* du, int q, int n_leve
                                     hipStream_t s);
 hipError_t lowrank_sketch_launch(int dtype, const void* du, int n_levels, int order,
                                  int64_t n_seq, int steps, int width, const int* hidx,
-                                 const void* hbit, int m, double* out, hipStream_t s);
-size_t lowrank_sketch_lds(int n_levels, int order, int m);
+                                 const void* hbit, int m, int ng, double* out, hipStream_t s);
+size_t lowrank_sketch_lds(int n_levels, int order, int m, int width, int ng);
+int lowrank_sketch_groups(int n_levels, int order, int m, int width, size_t lds_limit);
 int lowrank_sketch_max_comp();
 hipError_t rff_diff_launch(int dtype, const void* p, int64_t n_seq, int L, int half, int ld,
                            int difference, void* out, hipStream_t s);
@@ -1188,15 +1189,16 @@ int ksig_lowrank_levels_sketch(int dtype, const void* du, int n_levels, int orde
   if (n_comp > ksig::lowrank_sketch_max_comp())
     return fail(KSIG_EUNSUPPORTED, "n_comp = %d > %d is not built", n_comp,
                 ksig::lowrank_sketch_max_comp());
-  const size_t lds = ksig::lowrank_sketch_lds(n_levels, order, n_comp);
-  if (lds > max_lds_bytes())
-    return fail(KSIG_EUNSUPPORTED, "sketch scan needs %zu bytes of LDS (> %zu)", lds,
-                max_lds_bytes());
+  const size_t limit = max_lds_bytes();
+  const int ng = ksig::lowrank_sketch_groups(n_levels, order, n_comp, width, limit);
+  const size_t lds = ksig::lowrank_sketch_lds(n_levels, order, n_comp, width, ng);
+  if (lds > limit)
+    return fail(KSIG_EUNSUPPORTED, "sketch scan needs %zu bytes of LDS (> %zu)", lds, limit);
   if (n_seq == 0) return 0;
   if (!out || (steps > 0 && width > 0 && (!du || !hash_idx || !hash_bit)))
     return fail(KSIG_EINVAL, "null device pointer");
   hipError_t e = ksig::lowrank_sketch_launch(dtype, du, n_levels, order, n_seq, steps, width,
-                                            hash_idx, hash_bit, n_comp, out,
+                                            hash_idx, hash_bit, n_comp, ng, out,
                                             (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "lowrank_sketch launch");
   return 0;

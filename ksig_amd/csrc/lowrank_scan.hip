@@ -306,57 +306,115 @@ hipError_t lowrank_diag_scan_launch(int dtype, const void* du, int q, int n_leve
 //   R_0[0](t) = CS_0(u_0(t)),  R_i[0](t) = C_{i-1}(t) (*) CS_i(u_i(t)),
 //   R_i[r](t) = R_{i-1}[r-1](t) (*) CS_i(u_i(t)) / (r+1),   P_i = C_i(end).
 // A convolution reads every component of its source, so one workgroup walks
-// one sequence's time steps: each thread owns kSketchKpt components and their
-// running sums (float64 registers); the sources a convolution reads (e_0, the
-// running sums C_0..C_{NL-2} and, at order p > 1, this step's terms of the
-// levels below the top) sit in LDS as rows of 2m doubles holding the vector
-// twice, so that A[(k - h) mod m] is row[k + m - h] with no wrap test.
+// one sequence's time steps.  Its threads form ng groups of w: thread l of a
+// group owns components l + c w (c < kSketchKpt), and group g convolves the
+// g-th slice of the input features (several waves per sequence hide the LDS
+// latency that one wave per sequence exposes); the groups' partial sums meet
+// in LDS and group 0 adds them to the running sums (float64, in the LDS rows
+// the next level reads; the top level's in registers).  The sources a convolution reads (e_0, the running sums
+// C_0..C_{NL-2} and, at order p > 1, this step's terms of the levels below
+// the top) sit in LDS as rows of 2m doubles holding the vector twice, so that
+// A[(k - h) mod m] is row[k + m - h] with no wrap test.  The hashes, the signs
+// and each step's signed increments s_i[j] u_i(t)[j] are staged in LDS too
+// (one coalesced load of the step's rows by the whole workgroup), so the
+// convolution loop reads only LDS: broadcast reads of the features' hashes
+// and signed increments and one gather per feature and component.
 // Order 1: every level reads only the previous step's sums, so one pass of
-// convolutions, a barrier, the sum updates and a second barrier per step.
-// Order p > 1: level i reads level i-1's terms of the same step, so a barrier
-// after each level, then the sum updates.  A hash value outside [0, m)
-// selects no component (the reference's 0/1 mask has no column for it).
+// convolutions, a barrier, the reduction with the sum updates and the next
+// step's staging, and a second barrier per step.  Order p > 1: level i reads
+// level i-1's terms of the same step, so each level's convolutions, a
+// barrier, its reduction into the term rows and a barrier, then the sum
+// updates.  A hash value outside [0, m) selects no component (the
+// reference's 0/1 mask has no column for it): its sign is staged as zero.
 constexpr int kSketchKpt = 2;
 
-template <typename T>
-__device__ __forceinline__ void sketch_conv(const double* __restrict__ row, const T* __restrict__ u,
-                                            const int* __restrict__ h, const T* __restrict__ sg,
-                                            int width, int m, const int* kc, double* acc) {
+// sum over features j in [j0, j1) of cs[j] row[k + m - hs[j]] for the
+// thread's components (two accumulator chains per component)
+__device__ __forceinline__ void sketch_conv(const double* __restrict__ row,
+                                            const double* __restrict__ cs,
+                                            const int* __restrict__ hs, int j0, int j1, int m,
+                                            const int* kr, double* acc) {
+  double e[kSketchKpt], o[kSketchKpt];
 #pragma unroll
-  for (int c = 0; c < kSketchKpt; ++c) acc[c] = 0.0;
-  for (int j = 0; j < width; ++j) {
-    const double cj = (double)sg[j] * (double)u[j];
-    const int hj = h[j];
-    if ((unsigned)hj >= (unsigned)m) continue;  // no component: the mask has no column
-    const double* r = row + m - hj;
+  for (int c = 0; c < kSketchKpt; ++c) e[c] = o[c] = 0.0;
+  const double* rm = row + m;
+  int j = j0;
+  for (; j + 4 <= j1; j += 4) {
+    double cj[4];
+    int hj[4];
 #pragma unroll
-    for (int c = 0; c < kSketchKpt; ++c)
-      if (kc[c] < m) acc[c] = __builtin_fma(cj, r[kc[c]], acc[c]);
+    for (int u = 0; u < 4; ++u) {
+      cj[u] = cs[j + u];
+      hj[u] = hs[j + u];
+    }
+#pragma unroll
+    for (int c = 0; c < kSketchKpt; ++c) {
+      e[c] = __builtin_fma(cj[0], rm[kr[c] - hj[0]], e[c]);
+      o[c] = __builtin_fma(cj[1], rm[kr[c] - hj[1]], o[c]);
+      e[c] = __builtin_fma(cj[2], rm[kr[c] - hj[2]], e[c]);
+      o[c] = __builtin_fma(cj[3], rm[kr[c] - hj[3]], o[c]);
+    }
   }
+  for (; j < j1; ++j)
+#pragma unroll
+    for (int c = 0; c < kSketchKpt; ++c) e[c] = __builtin_fma(cs[j], rm[kr[c] - hs[j]], e[c]);
+#pragma unroll
+  for (int c = 0; c < kSketchKpt; ++c) acc[c] = e[c] + o[c];
 }
 
+// per-level stride of the staged tables: rows of 4-feature groups
+__host__ __device__ inline int sketch_wpad(int width) { return (width + 3) & ~3; }
 
 template <typename T, int NL>
 __global__ void __launch_bounds__(1024)
 lowrank_sketch_kernel(const T* __restrict__ du, int64_t n_seq, int steps, int width,
                       const int* __restrict__ hidx, const T* __restrict__ hbit, int m, int order,
-                      double* __restrict__ out) {
+                      int ng, double* __restrict__ out) {
   extern __shared__ double sk_lds[];
   const int64_t a = blockIdx.x;
+  const int tid = threadIdx.x, bs = blockDim.x;
+  const int w = bs / ng;
+  const int g = __builtin_amdgcn_readfirstlane(tid / w);  // w is a multiple of 64
+  const int l = tid - g * w;
   const int m2 = 2 * m;
   const int rows = sketch_rows(NL, order);
-  for (int j = threadIdx.x; j < rows * m2; j += blockDim.x)
+  const int wpad = sketch_wpad(width);
+  double* sv = sk_lds + rows * m2;   // [NL][wpad] signs, zero for a hash outside [0, m)
+  double* cst = sv + NL * wpad;      // [NL][wpad] this step's signed increments
+  double* red = cst + NL * wpad;     // [ng][NL][m] the groups' partial sums (ng > 1)
+  int* hs = reinterpret_cast<int*>(red + (ng > 1 ? ng * NL * m : 0));  // [NL][wpad]
+  // group g's feature slice, in whole 4-feature groups
+  const int chunk = ((width + 4 * ng - 1) / (4 * ng)) * 4;
+  const int j0 = min(width, g * chunk), j1 = min(width, j0 + chunk);
+  for (int j = tid; j < rows * m2; j += bs)
     sk_lds[j] = (j == 0 || j == m) ? 1.0 : 0.0;  // row 0: e_0 twice; the rest zero
-  int kc[kSketchKpt];
+  for (int e = tid; e < NL * wpad; e += bs) {
+    const int i = e / wpad, j = e - i * wpad;
+    const int h = j < width ? hidx[i * width + j] : -1;
+    const bool ok = (unsigned)h < (unsigned)m;
+    hs[e] = ok ? h : 0;
+    sv[e] = ok ? (double)hbit[i * width + j] : 0.0;
+    cst[e] = 0.0;
+  }
+  int kc[kSketchKpt], kr[kSketchKpt];  // components; the ones read (0 past m)
 #pragma unroll
-  for (int c = 0; c < kSketchKpt; ++c) kc[c] = threadIdx.x + c * blockDim.x;
+  for (int c = 0; c < kSketchKpt; ++c) {
+    kc[c] = l + c * w;
+    kr[c] = kc[c] < m ? kc[c] : 0;
+  }
   const int64_t level_stride = n_seq * (int64_t)steps * width;
   const T* ua = du + a * (int64_t)steps * width;
+  auto stage = [&](int t) {  // signed increments of step t (after the signs)
+    for (int e = tid; e < NL * width; e += bs) {
+      const int i = e / width, j = e - i * width;
+      cst[i * wpad + j] = sv[i * wpad + j] * (double)ua[i * level_stride + (int64_t)t * width + j];
+    }
+  };
   auto crow = [&](int i) { return sk_lds + (1 + i) * m2; };  // C_i, i < NL-1
   // at order p > 1: level i's order-r terms of this step, i < NL-1
   auto rrow = [&](int i, int r) {
     int base = NL;
-    for (int l = 0; l < i; ++l) base += l + 1 < order ? l + 1 : order;
+    for (int q = 0; q < i; ++q) base += q + 1 < order ? q + 1 : order;
     return sk_lds + (base + r) * m2;
   };
   auto put = [&](double* row, int c, double v) {
@@ -365,93 +423,146 @@ lowrank_sketch_kernel(const T* __restrict__ du, int64_t n_seq, int steps, int wi
       row[kc[c] + m] = v;
     }
   };
-  double C[NL][kSketchKpt];
+  // the partial sums v of slot s (a level, or an order term) of this group,
+  // and group 0's total over the groups
+  auto part = [&](int slot, const double* v) {
+    if (ng > 1)
 #pragma unroll
-  for (int i = 0; i < NL; ++i)
+      for (int c = 0; c < kSketchKpt; ++c)
+        if (kc[c] < m) red[(g * NL + slot) * m + kc[c]] = v[c];
+  };
+  auto total = [&](int slot, double* v) {
+    if (ng > 1)
 #pragma unroll
-    for (int c = 0; c < kSketchKpt; ++c) C[i][c] = 0.0;
+      for (int c = 0; c < kSketchKpt; ++c) {
+        double sum = 0.0;
+        for (int q = 0; q < ng; ++q) sum += red[(q * NL + slot) * m + kr[c]];
+        v[c] = sum;
+      }
+  };
+  // the top level's running sums (the lower levels' live in their rows)
+  double Ctop[kSketchKpt];
+#pragma unroll
+  for (int c = 0; c < kSketchKpt; ++c) Ctop[c] = 0.0;
+  __syncthreads();  // the signs are staged
+  if (steps > 0) stage(0);
   __syncthreads();
   for (int t = 0; t < steps; ++t) {
     double S[NL][kSketchKpt];  // this step's terms summed over the orders
     if (order == 1) {
 #pragma unroll
-      for (int i = 0; i < NL; ++i)
-        sketch_conv<T>(i == 0 ? sk_lds : crow(i - 1), ua + i * level_stride + (int64_t)t * width,
-                       hidx + i * width, hbit + i * width, width, m, kc, S[i]);
-      __syncthreads();  // every convolution has read the sums
+      for (int i = 0; i < NL; ++i) {
+        sketch_conv(i == 0 ? sk_lds : crow(i - 1), cst + i * wpad, hs + i * wpad, j0, j1, m, kr,
+                    S[i]);
+        part(i, S[i]);
+      }
+      __syncthreads();  // every convolution has read the sums and the step
+      if (g == 0)
+#pragma unroll
+        for (int i = 0; i < NL; ++i) total(i, S[i]);
     } else {
 #pragma unroll
       for (int i = 0; i < NL; ++i) {
-        const T* u = ua + i * level_stride + (int64_t)t * width;
         const int d = i + 1 < order ? i + 1 : order;
+        auto src = [&](int r) { return i == 0 ? sk_lds : (r == 0 ? crow(i - 1) : rrow(i - 1, r - 1)); };
+        if (ng > 1) {
+          for (int r = 0; r < d; ++r) {
+            double v[kSketchKpt];
+            sketch_conv(src(r), cst + i * wpad, hs + i * wpad, j0, j1, m, kr, v);
+            part(r, v);
+          }
+          __syncthreads();  // the partials of every group are in
+        }
+        if (g == 0) {
 #pragma unroll
-        for (int c = 0; c < kSketchKpt; ++c) S[i][c] = 0.0;
-        for (int r = 0; r < d; ++r) {
-          double v[kSketchKpt];
-          const double* src = i == 0 ? sk_lds : (r == 0 ? crow(i - 1) : rrow(i - 1, r - 1));
-          sketch_conv<T>(src, u, hidx + i * width, hbit + i * width, width, m, kc, v);
-          const double w = 1.0 / (r + 1);
+          for (int c = 0; c < kSketchKpt; ++c) S[i][c] = 0.0;
+          for (int r = 0; r < d; ++r) {
+            double v[kSketchKpt];
+            if (ng > 1) total(r, v);
+            else sketch_conv(src(r), cst + i * wpad, hs + i * wpad, j0, j1, m, kr, v);
+            const double wt = 1.0 / (r + 1);
 #pragma unroll
-          for (int c = 0; c < kSketchKpt; ++c) {
-            v[c] *= w;
-            S[i][c] += v[c];
-            if (i + 1 < NL) put(rrow(i, r), c, v[c]);
+            for (int c = 0; c < kSketchKpt; ++c) {
+              v[c] *= wt;
+              S[i][c] += v[c];
+              if (i + 1 < NL) put(rrow(i, r), c, v[c]);
+            }
           }
         }
         __syncthreads();  // level i's terms are visible to level i+1
       }
     }
+    if (g == 0) {
 #pragma unroll
-    for (int i = 0; i < NL; ++i)
+      for (int i = 0; i + 1 < NL; ++i)
 #pragma unroll
-      for (int c = 0; c < kSketchKpt; ++c) {
-        C[i][c] += S[i][c];
-        if (i + 1 < NL) put(crow(i), c, C[i][c]);
-      }
-    __syncthreads();  // the sums are updated before the next step reads them
+        for (int c = 0; c < kSketchKpt; ++c) put(crow(i), c, crow(i)[kr[c]] + S[i][c]);
+#pragma unroll
+      for (int c = 0; c < kSketchKpt; ++c) Ctop[c] += S[NL - 1][c];
+    }
+    if (t + 1 < steps) stage(t + 1);
+    __syncthreads();  // sums and the next step staged before they are read
   }
+  if (g != 0) return;
   double* o = out + a * (int64_t)NL * m;
 #pragma unroll
-  for (int i = 0; i < NL; ++i)
+  for (int c = 0; c < kSketchKpt; ++c) {
+    if (kc[c] < m) {
 #pragma unroll
-    for (int c = 0; c < kSketchKpt; ++c)
-      if (kc[c] < m) o[(int64_t)i * m + kc[c]] = C[i][c];
+      for (int i = 0; i + 1 < NL; ++i) o[(int64_t)i * m + kc[c]] = crow(i)[kc[c]];
+      o[(int64_t)(NL - 1) * m + kc[c]] = Ctop[c];
+    }
+  }
 }
 
-// threads per workgroup for m components and the LDS bytes of one workgroup
+// threads per group for m components, the groups per workgroup and the LDS
+// bytes of one workgroup
 __host__ inline int sketch_threads(int m) {
   const int per = (m + kSketchKpt - 1) / kSketchKpt;
   return ((per + kWave - 1) / kWave) * kWave;
 }
-size_t lowrank_sketch_lds(int n_levels, int order, int m) {
-  return (size_t)sketch_rows(n_levels, order) * 2 * (size_t)m * sizeof(double);
+size_t lowrank_sketch_lds(int n_levels, int order, int m, int width, int ng) {
+  return (size_t)sketch_rows(n_levels, order) * 2 * (size_t)m * sizeof(double) +
+         (size_t)n_levels * sketch_wpad(width) * (2 * sizeof(double) + sizeof(int)) +
+         (ng > 1 ? (size_t)ng * n_levels * m * sizeof(double) : 0);
+}
+// up to 8 waves per sequence where the components leave room, no more groups
+// than 4-feature slices, and fewer where the LDS would not fit
+int lowrank_sketch_groups(int n_levels, int order, int m, int width, size_t lds_limit) {
+  const int w = sketch_threads(m);
+  int ng = 512 / w;
+  if (ng < 1) ng = 1;
+  const int slices = (width + 3) / 4;
+  if (ng > slices) ng = slices > 0 ? slices : 1;
+  while (ng > 1 && lowrank_sketch_lds(n_levels, order, m, width, ng) > lds_limit) --ng;
+  return ng;
 }
 int lowrank_sketch_max_comp() { return 1024 * kSketchKpt; }
 
 template <typename T, int NL>
 static hipError_t launch_sketch(const void* du, int64_t n_seq, int steps, int width,
-                                const int* hidx, const void* hbit, int m, int order,
+                                const int* hidx, const void* hbit, int m, int order, int ng,
                                 double* out, hipStream_t s) {
   auto k = lowrank_sketch_kernel<T, NL>;
-  const size_t lds = lowrank_sketch_lds(NL, order, m);
+  const size_t lds = lowrank_sketch_lds(NL, order, m, width, ng);
   if (lds > 64 * 1024) {
     hipError_t e = hipFuncSetAttribute((const void*)k,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(k, dim3((unsigned)n_seq), dim3(sketch_threads(m)), lds, s,
+  hipLaunchKernelGGL(k, dim3((unsigned)n_seq), dim3(ng * sketch_threads(m)), lds, s,
                      reinterpret_cast<const T*>(du), n_seq, steps, width, hidx,
-                     reinterpret_cast<const T*>(hbit), m, order, out);
+                     reinterpret_cast<const T*>(hbit), m, order, ng, out);
   return hipGetLastError();
 }
 
 template <typename T>
 static hipError_t sketch_levels(int n_levels, const void* du, int64_t n_seq, int steps, int width,
-                                const int* hidx, const void* hbit, int m, int order, double* out,
-                                hipStream_t s) {
+                                const int* hidx, const void* hbit, int m, int order, int ng,
+                                double* out, hipStream_t s) {
   switch (n_levels) {
 #define KSIG_SKETCH_CASE(L) \
-    case L: return launch_sketch<T, L>(du, n_seq, steps, width, hidx, hbit, m, order, out, s);
+    case L: return launch_sketch<T, L>(du, n_seq, steps, width, hidx, hbit, m, order, ng, out, s);
     KSIG_SKETCH_CASE(1)
     KSIG_SKETCH_CASE(2)
     KSIG_SKETCH_CASE(3)
@@ -465,14 +576,17 @@ static hipError_t sketch_levels(int n_levels, const void* du, int64_t n_seq, int
   return hipErrorInvalidValue;
 }
 
-// 1 <= order <= n_levels <= 8, 1 <= m <= lowrank_sketch_max_comp(), the LDS of
-// lowrank_sketch_lds within the device's limit (checked by the caller)
+// 1 <= order <= n_levels <= 8, 1 <= m <= lowrank_sketch_max_comp(), ng from
+// lowrank_sketch_groups and the LDS of lowrank_sketch_lds within the
+// device's limit (checked by the caller)
 hipError_t lowrank_sketch_launch(int dtype, const void* du, int n_levels, int order,
                                  int64_t n_seq, int steps, int width, const int* hidx,
-                                 const void* hbit, int m, double* out, hipStream_t s) {
+                                 const void* hbit, int m, int ng, double* out, hipStream_t s) {
   if (dtype == KSIG_F32)
-    return sketch_levels<float>(n_levels, du, n_seq, steps, width, hidx, hbit, m, order, out, s);
-  return sketch_levels<double>(n_levels, du, n_seq, steps, width, hidx, hbit, m, order, out, s);
+    return sketch_levels<float>(n_levels, du, n_seq, steps, width, hidx, hbit, m, order, ng, out,
+                                s);
+  return sketch_levels<double>(n_levels, du, n_seq, steps, width, hidx, hbit, m, order, ng, out,
+                               s);
 }
 
 hipError_t lowrank_scan_launch(int dtype, const void* z, int n_levels, int order,
