@@ -19,20 +19,25 @@
 // inside the sweep.  The longer sequence is the row side (GA is symmetric
 // under transposition: the alignment paths and M' transpose together).
 //
-// Representation: the sum over alignments is carried in exp space as a
-// float64 mantissa and an int32 power-of-two exponent, K = v * 2^e, so the
-// cell update is the exp-space form of the same recurrence,
+// Representation: the sum over alignments is carried in exp space, each
+// lane's values as float64 multiples of 2^E with one int32 exponent E per
+// lane, so the cell update is the exp-space form of the same recurrence,
 //   K(i,j) = M'(i,j) (K(i-1,j) + K(i,j-1) + K(i-1,j-1)),
-// with the three terms aligned to the largest exponent by ldexp (exact) and
-// the product renormalised by frexp (exact): one float64 rounding per add and
-// per multiply, no exp / log per cell (the log-sum-exp form needs three exp
-// and a log), and no overflow or underflow at any length because the
-// exponent absorbs the scale.  A term whose exponent lies more than 1074
-// below the largest one flushes to zero, exactly as exp(a - m) underflows in
-// the log-sum-exp.  The result is log v + e ln 2 at the last cell.  The
+// two adds and a multiply per cell.  The value the lane above hands down
+// (with its own exponent) is scaled into this lane's by one ldexp per step
+// (exact), and every kGakRenorm steps a lane rescales its values so that the
+// largest has exponent 0 (exact), so nothing overflows or underflows at any
+// length: between rescalings a value grows at most by (3 M'max)^kGakRenorm
+// and adjacent rows differ by bounded factors.  Power-of-two scalings are
+// exact, so each add and multiply rounds as it would on the unscaled values:
+// one float64 rounding per add and per multiply, no exp / log per cell (the
+// log-sum-exp form needs three exp and a log).  A term more than 1074
+// binades below the lane's scale flushes to zero, as exp(a - m) underflows
+// in the log-sum-exp.  The result is log v + E ln 2 at the last cell.  The
 // static-kernel value and M' = max(M / (2 - M), 1e-12) are computed in the
 // input precision (float32 inputs: v_exp_f32 and v_rcp_f32, ~1 ulp each).
-// An empty sum (log 0) is v = 0 with exponent kGakZeroExp.
+// An empty sum (log 0) is v = 0; a handed-down empty sum carries exponent
+// kGakZeroExp.
 
 #include <math.h>
 
@@ -54,17 +59,13 @@ __device__ __forceinline__ double gak_mprime(double m) {
   return fmax(m / (2.0 - m), 1e-12);
 }
 
-// One cell: M' * (a + b + c) for a = va 2^ea etc., renormalised to a
-// mantissa in [0.5, 1) (or 0) and its exponent.
-__device__ __forceinline__ void gak_cell(double mp, double va, int ea, double vb, int eb,
-                                         double vc, int ec, double& v, int& e) {
-  const int m = max(max(ea, eb), ec);
-  const double s = __builtin_amdgcn_ldexp(va, ea - m) + __builtin_amdgcn_ldexp(vb, eb - m) +
-                   __builtin_amdgcn_ldexp(vc, ec - m);
-  const double p = s * mp;
-  v = __builtin_amdgcn_frexp_mant(p);
-  e = m + __builtin_amdgcn_frexp_exp(p);
-}
+// Steps between a lane's rescalings, and how far (in binades) the row above
+// may outgrow the lane's scale before the lane rescales to it at once: with
+// M' < 2^54 (float64 inputs next to the pole M = 2) values grow at most
+// 2^(55 kGakRenorm) between rescalings, so 2^(55 kGakRenorm) * 2^kGakRescale *
+// 2^(55 kGakRenorm) stays far below the float64 range.
+constexpr int kGakRenorm = 4;
+constexpr int kGakRescale = 128;
 
 template <typename T, int DMAX, int KIND, int R>
 __global__ void __launch_bounds__(kWave)
@@ -189,16 +190,15 @@ gak_forward_kernel(const GakArgs a) {
 
   int band = 0;
   load_rows(0);
-  // K of this lane's rows at column c-1, and K of the row above at c-1
+  // K of this lane's rows at column c-1 and of the row above at c-1 (the
+  // first row's diagonal), as multiples of 2^E; the last row's value handed
+  // down with its exponent
   double lv[R];
-  int le[R];
 #pragma unroll
-  for (int q = 0; q < R; ++q) {
-    lv[q] = 0.0;
-    le[q] = kGakZeroExp;
-  }
+  for (int q = 0; q < R; ++q) lv[q] = 0.0;
+  int E = 0;
   double pin_v = 0.0, up_v = 0.0;
-  int pin_e = kGakZeroExp, up_e = kGakZeroExp;
+  int up_e = kGakZeroExp;
   int cc = -lane;
 
   StepIn cur;
@@ -221,44 +221,60 @@ gak_forward_kernel(const GakArgs a) {
       fetch(cur, wrap ? 0 : cc + 1, band + (wrap ? 1 : 0));
     }
 
+    // the first column of a band pass takes the exponent handed down (the
+    // lane's own values are empty sums there); later the row above may
+    // outgrow the lane's scale, and the lane rescales to it
+    if (c0) E = in_v != 0.0 ? in_e : 0;
+    int al = in_e - E;
+    if (al > kGakRescale) {
+#pragma unroll
+      for (int q = 0; q < R; ++q) lv[q] = __builtin_amdgcn_ldexp(lv[q], -al);
+      pin_v = __builtin_amdgcn_ldexp(pin_v, -al);
+      E = in_e;
+      al = 0;
+    }
+    const double in_s = __builtin_amdgcn_ldexp(in_v, al < -2000 ? -2000 : al);
+
     const int row0 = band * BR + lane * R;
-    double uv = in_v, dv = pin_v;   // K(row-1, c), K(row-1, c-1)
-    int ue = in_e, de = pin_e;
+    double uv = in_s, dv = pin_v;   // K(row-1, c), K(row-1, c-1)
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       const int row = row0 + q;
       // column -1 holds empty sums (the left values are reset at each band
       // start, and rows q >= 1 take their diagonal from them), except
-      // K(-1, -1) = 1 above-left of (0, 0); the first row's diagonal at
-      // column 0 comes from the row above, which is not yet in the band
+      // K(-1, -1) = 1 above-left of (0, 0) (exponent 0: E = 0 there); the
+      // first row's diagonal at column 0 comes from the row above, which is
+      // not yet in the band
       double dq = dv;
-      int dqe = de;
-      if (q == 0) {
-        dq = c0 ? (row == 0 ? 1.0 : 0.0) : dv;
-        dqe = c0 ? (row == 0 ? 0 : kGakZeroExp) : de;
-      }
-      double v;
-      int e;
-      gak_cell(mp[q], uv, ue, lv[q], le[q], dq, dqe, v, e);
+      if (q == 0) dq = c0 ? (row == 0 ? 1.0 : 0.0) : dv;
+      double v = ((uv + lv[q]) + dq) * mp[q];
       const bool cell = colok && row < Lx;
       v = cell ? v : 0.0;
-      e = cell ? e : kGakZeroExp;
-      if (cell && row == Lx - 1 && cc == Ly - 1) a.out[pr] = log(v) + (double)e * kLn2;
+      if (cell && row == Lx - 1 && cc == Ly - 1) a.out[pr] = log(v) + (double)E * kLn2;
       dv = lv[q];
-      de = le[q];
       lv[q] = v;
-      le[q] = e;
       uv = v;
-      ue = e;
     }
+    pin_v = in_s;
+    if ((step & (kGakRenorm - 1)) == kGakRenorm - 1) {
+      double mx = pin_v;
+#pragma unroll
+      for (int q = 0; q < R; ++q) mx = fmax(mx, lv[q]);
+      if (mx > 0.0) {
+        const int sh = __builtin_amdgcn_frexp_exp(mx);
+#pragma unroll
+        for (int q = 0; q < R; ++q) lv[q] = __builtin_amdgcn_ldexp(lv[q], -sh);
+        pin_v = __builtin_amdgcn_ldexp(pin_v, -sh);
+        E += sh;
+      }
+    }
+    uv = lv[R - 1];
     if (lane == kWave - 1 && active) {
       fifo_v[cc] = uv;
-      fifo_e[cc] = ue;
+      fifo_e[cc] = E;
     }
     up_v = uv;
-    up_e = ue;
-    pin_v = in_v;
-    pin_e = in_e;
+    up_e = E;
 
     ++cc;
     if (cc == Pc) {  // wrap to the next band: column -1 of the new rows
@@ -266,10 +282,7 @@ gak_forward_kernel(const GakArgs a) {
       ++band;
       if (band < nb) load_rows(band);
 #pragma unroll
-      for (int q = 0; q < R; ++q) {
-        lv[q] = 0.0;
-        le[q] = kGakZeroExp;
-      }
+      for (int q = 0; q < R; ++q) lv[q] = 0.0;
     }
   }
 }

@@ -119,3 +119,21 @@ def test_generic_static_kernel_goes_through_matrices():
   dg = np.diagonal(logK)
   ref = np.exp(logK - .5 * (dg[:, None] + dg[None, :]))
   assert normwise(K, ref) < 1e-11
+
+
+@pytest.mark.parametrize("dtype,lo,hi,tol", [(np.float64, -40.0, 40.0, 1e-11),
+                                             (np.float32, -30.0, 20.0, 1e-5)])
+def test_wide_dynamic_range(dtype, lo, hi, tol):
+  """Caller-supplied matrices whose M' = M / (2 - M) spans 2^lo .. 2^hi
+  (log-uniform, M = 2 M' / (1 + M')), with a block of rows next to the pole
+  (M' near 2^hi everywhere) and one of M' at the 1e-12 clamp: the values
+  within a lane span hundreds of binades between rescalings and the row
+  above outgrows a lane's scale at the block edges."""
+  rng = np.random.default_rng(7)
+  Mp = np.exp2(rng.uniform(lo, hi, (2, 2, 300, 150)))
+  Mp[:, :, 100:140, :] = np.exp2(hi - rng.uniform(0.0, 1.0, (2, 2, 40, 150)))
+  Mp[:, :, 200:230, 10:90] = 1e-13
+  M = (2.0 * Mp / (1.0 + Mp)).astype(dtype)
+  got = ksig.algorithms.global_align_kern_log(to_dev(M)).cpu().numpy()
+  ref = O.global_align_kern_log(M.astype(np.float64))
+  assert np.all(np.isfinite(got)) and _logerr(got, ref) < tol
