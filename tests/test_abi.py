@@ -187,3 +187,36 @@ def test_rff_increments_argument_checks():
   assert f(5, None, 3, 10, 4, 4, 1, None, None) == 1001
   assert f(0, None, 3, 10, 4, 3, 1, None, None) == 1001  # row stride below the row
   assert f(0, None, 3, 10, 4, 8, 1, None, None) == 1001  # null pointers
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libksig_hip.so not built")
+def test_lowrank_levels_diagonal_argument_checks():
+  lib = ctypes.CDLL(LIB)
+  lib.ksig_last_error.restype = ctypes.c_char_p
+  f = lib.ksig_lowrank_levels_diagonal
+  f.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+  assert f(0, None, 3, 2, 3, 0, 10, 4, None, None) == 0       # no sequences
+  assert f(2, None, 3, 1, 2, 3, 10, 4, None, None) == 1001    # dtype
+  assert f(0, None, 0, 1, 2, 3, 10, 4, None, None) == 1001    # n_levels
+  assert f(0, None, 3, 1, 0, 3, 10, 4, None, None) == 1001    # internal size
+  assert f(0, None, 4, 1, 3, 3, 10, 4, None, None) == 1002    # 120 sums: not built
+  assert f(0, None, 5, 2, 2, 3, 10, 4, None, None) == 1002    # q=2, 5 levels, order 2
+  assert f(0, None, 6, 1, 1, 3, 10, 4, None, None) == 1002    # more than 5 levels
+  assert f(0, None, 4, 0, 2, 3, 10, 4, None, None) == 1001 and b"null" in lib.ksig_last_error()
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libksig_hip.so not built")
+def test_lowrank_levels_sketch_argument_checks():
+  lib = ctypes.CDLL(LIB)
+  lib.ksig_last_error.restype = ctypes.c_char_p
+  f = lib.ksig_lowrank_levels_sketch
+  f.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int64,
+                ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                ctypes.c_void_p, ctypes.c_void_p]
+  assert f(2, None, 3, 1, 3, 10, 8, None, None, 50, None, None) == 1001   # dtype
+  assert f(0, None, 0, 1, 3, 10, 8, None, None, 50, None, None) == 1001   # n_levels
+  assert f(0, None, 9, 1, 3, 10, 8, None, None, 50, None, None) == 1002   # > 8 levels
+  assert f(0, None, 3, 1, 3, 10, 8, None, None, 0, None, None) == 1001    # n_comp
+  assert f(0, None, 3, 1, -1, 10, 8, None, None, 50, None, None) == 1001  # negative shape
+  assert f(0, None, 3, 1, 3, 10, 8, None, None, 4096, None, None) == 1002  # > 2048 components
