@@ -89,33 +89,54 @@ rws_forward_kernel(const RwsArgs a) {
   __syncthreads();
 
   const T* wdata = reinterpret_cast<const T*>(a.warp);
-  // this lane's R rows of band b: warp points, norms and series
+  // this lane's R rows of the current band: warp points, norms, stack row
+  // (row_end past the stack), series and first / last flags; and the same of
+  // its rows in the next band, staged by the whole wave at one step (a
+  // band's worth of steps ahead of the lane's band change, which then only
+  // copies registers: loading there would run the loads and the series
+  // search, under one lane's mask, at each of the 64 steps where some lane
+  // changes band)
   T wv[R][DMAX], wn[R];
-  int rowq[R], wid[R];
-  bool first[R], last[R];
-  auto load_rows = [&](int b) __attribute__((always_inline)) {
+  int wid[R];
+  bool valid[R], first[R], last[R];
+  T pwv[R][DMAX], pwn[R];
+  int pwid[R];
+  bool pvalid[R], pfirst[R], plast[R];
+  auto load_rows = [&](int b) __attribute__((always_inline)) {  // into the pending set
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       const int row = row_beg + b * BR + lane * R + q;
-      const bool valid = row < row_end;
-      rowq[q] = valid ? row : row_end;  // row_end marks padding
+      const bool ok = row < row_end;  // rows past the stack are padding
+      pvalid[q] = ok;
       // series of the row: the last w in [w_beg, w_end) with warp_off[w] <= row
       int lo = w_beg, hi = w_end - 1;
       while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
         if (a.warp_off[mid] <= row) lo = mid; else hi = mid - 1;
       }
-      wid[q] = lo;
-      first[q] = valid && row == a.warp_off[lo];
-      last[q] = valid && row == a.warp_off[lo + 1] - 1;
+      pwid[q] = lo;
+      pfirst[q] = ok && row == a.warp_off[lo];
+      plast[q] = ok && row == a.warp_off[lo + 1] - 1;
       T n2 = T(0);
 #pragma unroll
       for (int k = 0; k < DMAX; ++k) {
-        const T v = (!GRAMIN && valid && k < d) ? wdata[(int64_t)row * d + k] : T(0);
-        wv[q][k] = v;
+        const T v = (!GRAMIN && ok && k < d) ? wdata[(int64_t)row * d + k] : T(0);
+        pwv[q][k] = v;
         n2 += v * v;
       }
-      wn[q] = n2;
+      pwn[q] = n2;
+    }
+  };
+  auto take_rows = [&]() __attribute__((always_inline)) {  // the pending set becomes current
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+#pragma unroll
+      for (int k = 0; k < DMAX; ++k) wv[q][k] = pwv[q][k];
+      wn[q] = pwn[q];
+      valid[q] = pvalid[q];
+      wid[q] = pwid[q];
+      first[q] = pfirst[q];
+      last[q] = plast[q];
     }
   };
 
@@ -146,6 +167,12 @@ rws_forward_kernel(const RwsArgs a) {
 
   int band = 0;
   load_rows(0);
+  take_rows();
+  // band b+1 is staged at step b*Pc + 64, when every lane has taken band b
+  // (lane l takes band b at the end of step b*Pc - 1 + l) and before lane 0
+  // takes band b+1 (end of step (b+1)*Pc - 1; Pc >= 65)
+  int next_stage = kWave;
+  int stage_band = 1;
   T left[R];
 #pragma unroll
   for (int q = 0; q < R; ++q) left[q] = kInf;
@@ -155,6 +182,11 @@ rws_forward_kernel(const RwsArgs a) {
   fetch(cur, cc, band);
   const int total = nb * Pc + kWave - 1;
   for (int step = 0; step < total; ++step) {
+    if (step == next_stage) {  // wave-uniform
+      if (stage_band < nb) load_rows(stage_band);
+      ++stage_band;
+      next_stage += Pc;
+    }
     const bool active = cc >= 0 && band < nb;
     const bool colok = active && cc < Lx;
     const bool c0 = cc == 0;
@@ -186,7 +218,7 @@ rws_forward_kernel(const RwsArgs a) {
       T dq = first[q] ? (c0 ? T(0) : kInf) : dg;
       if (q == 0 && !first[q]) dq = c0 ? kInf : dg;
       T val = dist[q] + fminf_t(fminf_t(uq, left[q]), dq);
-      const bool cell = colok && rowq[q] < row_end;
+      const bool cell = colok && valid[q];
       val = cell ? val : kInf;
       if (cell && last[q] && cc == Lx - 1) out[wid[q]] = (double)val;
       dg = left[q];
@@ -201,7 +233,7 @@ rws_forward_kernel(const RwsArgs a) {
     if (cc == Pc) {  // wrap to the next band: column -1 of the new rows
       cc = 0;
       ++band;
-      if (band < nb) load_rows(band);
+      if (band < nb) take_rows();
 #pragma unroll
       for (int q = 0; q < R; ++q) left[q] = kInf;
     }
