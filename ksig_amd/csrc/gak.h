@@ -17,6 +17,7 @@ struct GakArgs {
   double xscale, yscale; // static-kernel scales folded into the inputs
   const void* gram;      // kGram: [n_pairs][gram_lx][gram_ly] static-kernel values
   int gram_lx, gram_ly;
+  int rows_max;          // the longest row side of any pair (the longer sequence)
   double* out;           // [n_pairs] log GA values
 };
 

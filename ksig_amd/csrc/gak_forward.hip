@@ -67,7 +67,13 @@ __device__ __forceinline__ double gak_mprime(double m) {
 constexpr int kGakRenorm = 4;
 constexpr int kGakRescale = 128;
 
-template <typename T, int DMAX, int KIND, int R>
+// MB: pairs may span several row bands; the next band's x rows are then
+// staged by the whole wave into a second register set at one step a band
+// ahead (lane l changes band at its own step b*Pc - 1 + l; loading there ran
+// the loads and norms under one lane's mask at 64 steps of every band).
+// Launches whose rows all fit one band use MB = false and keep the
+// registers.
+template <typename T, int DMAX, int KIND, int R, bool MB>
 __global__ void __launch_bounds__(kWave)
 gak_forward_kernel(const GakArgs a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -128,12 +134,29 @@ gak_forward_kernel(const GakArgs a) {
   }
   __syncthreads();
 
-  // this lane's R rows of band `b`: x rows scaled, with their norms
+  // this lane's R rows of band `b`: x rows scaled, with their norms (and,
+  // MB, the next band's)
   T xv[R][DMAX], xn[R];
-  auto load_rows = [&](int b) __attribute__((always_inline)) {
+  T pxv[MB ? R : 1][DMAX], pxn[MB ? R : 1];
+  auto load_rows = [&](int b, T (&dv)[MB ? R : 1][DMAX], T (&dn)[MB ? R : 1]) __attribute__((always_inline)) {
+#pragma unroll
+    for (int q = 0; q < (MB ? R : 1); ++q) {
+      const int xr = b * BR + lane * R + q;
+      const bool ok = xr < Lx && !kGramIn;
+      T n2 = T(0);
+#pragma unroll
+      for (int k = 0; k < DMAX; ++k) {
+        const T v = (ok && k < d) ? xs[(int64_t)xr * d + k] * xscale : T(0);
+        dv[q][k] = v;
+        n2 += v * v;
+      }
+      dn[q] = n2;
+    }
+  };
+  auto load_first = [&]() __attribute__((always_inline)) {  // band 0 into the current set
 #pragma unroll
     for (int q = 0; q < R; ++q) {
-      const int xr = b * BR + lane * R + q;
+      const int xr = lane * R + q;
       const bool ok = xr < Lx && !kGramIn;
       T n2 = T(0);
 #pragma unroll
@@ -189,7 +212,11 @@ gak_forward_kernel(const GakArgs a) {
   };
 
   int band = 0;
-  load_rows(0);
+  load_first();
+  // MB: band b+1 is staged at step b*Pc + 64, when every lane has taken band
+  // b and before lane 0 takes band b+1 (Pc >= 65)
+  int next_stage = kWave;
+  int stage_band = 1;
   // K of this lane's rows at column c-1 and of the row above at c-1 (the
   // first row's diagonal), as multiples of 2^E; the last row's value handed
   // down with its exponent
@@ -205,6 +232,13 @@ gak_forward_kernel(const GakArgs a) {
   fetch(cur, cc, band);
   const int total = nb * Pc + kWave - 1;
   for (int step = 0; step < total; ++step) {
+    if constexpr (MB) {
+      if (step == next_stage) {  // wave-uniform
+        if (stage_band < nb) load_rows(stage_band, pxv, pxn);
+        ++stage_band;
+        next_stage += Pc;
+      }
+    }
     const bool active = cc >= 0 && band < nb;
     const bool colok = active && cc < Ly;
     const bool c0 = cc == 0;
@@ -280,16 +314,25 @@ gak_forward_kernel(const GakArgs a) {
     if (cc == Pc) {  // wrap to the next band: column -1 of the new rows
       cc = 0;
       ++band;
-      if (band < nb) load_rows(band);
+      if constexpr (MB) {
+        if (band < nb) {
+#pragma unroll
+          for (int q = 0; q < R; ++q) {
+#pragma unroll
+            for (int k = 0; k < DMAX; ++k) xv[q][k] = pxv[q][k];
+            xn[q] = pxn[q];
+          }
+        }
+      }
 #pragma unroll
       for (int q = 0; q < R; ++q) lv[q] = 0.0;
     }
   }
 }
 
-template <typename T, int DMAX, int KIND, int R>
-static hipError_t launch_gak(const GakArgs& a, size_t lds, hipStream_t s) {
-  auto k = gak_forward_kernel<T, DMAX, KIND, R>;
+template <typename T, int DMAX, int KIND, int R, bool MB>
+static hipError_t launch_gak_mb(const GakArgs& a, size_t lds, hipStream_t s) {
+  auto k = gak_forward_kernel<T, DMAX, KIND, R, MB>;
   if (lds > 64 * 1024) {
     hipError_t e = hipFuncSetAttribute((const void*)k,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -297,6 +340,13 @@ static hipError_t launch_gak(const GakArgs& a, size_t lds, hipStream_t s) {
   }
   hipLaunchKernelGGL(k, dim3((unsigned)a.n_pairs), dim3(kWave), lds, s, a);
   return hipGetLastError();
+}
+
+// one band (every pair's row side within 64R rows) or several
+template <typename T, int DMAX, int KIND, int R>
+static hipError_t launch_gak(const GakArgs& a, size_t lds, hipStream_t s) {
+  if (a.rows_max > kWave * R) return launch_gak_mb<T, DMAX, KIND, R, true>(a, lds, s);
+  return launch_gak_mb<T, DMAX, KIND, R, false>(a, lds, s);
 }
 
 // Rows per lane from register pressure: float32 4 (d <= 16) or 2, float64 2

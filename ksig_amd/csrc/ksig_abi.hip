@@ -1041,8 +1041,10 @@ int ksig_global_align_log(int dtype, const void* x, const int64_t* x_off,
   a.n_pairs = n_pairs;
   a.kind = static_kind;
   a.out = out;
-  // the shorter side of a pair (the LDS side) is at most the smaller maximum
+  // the shorter side of a pair (the LDS side) is at most the smaller maximum,
+  // the longer (the rows) at most the larger
   const int cols = max_len_x < max_len_y ? max_len_x : max_len_y;
+  a.rows_max = max_len_x > max_len_y ? max_len_x : max_len_y;
   return gak_common(dtype, a, cols < 1 ? 1 : cols, stream);
 }
 
@@ -1060,6 +1062,7 @@ int ksig_global_align_log_gram(int dtype, const void* M, int64_t n_mats, int lx,
   a.gram = M;
   a.gram_lx = lx;
   a.gram_ly = ly;
+  a.rows_max = lx;
   a.out = out;
   return gak_common(dtype, a, ly < 1 ? 1 : ly, stream);
 }
