@@ -837,9 +837,14 @@ def bench_rfsf(args):
   e1.record(stream)
   torch.cuda.synchronize()
   tscan_ms = e0.elapsed_time(e1) / args.steps
-  # one multiply-add per (input feature, sketch component) of each level's
-  # convolution at every step: 2 * width * P flop per level-step
-  tscan_flop = 2.0 * n * (L - 1) * nl * dU.shape[-1] * P
+  # the scan convolves each level's running sum with the step's count
+  # sketch: from the sparse sketch (one multiply-add per input feature and
+  # component, 2 * width * P flop per level-step) or, with fewer components
+  # than features, from the dense sketch (width adds, then 2 * P * P flop)
+  width = dU.shape[-1]
+  dense = P < width
+  per = (width + 2.0 * P * P) if dense else 2.0 * width * P
+  tscan_flop = float(n) * (L - 1) * nl * per
   wt = [f.random_weights_.double().cpu().numpy() for f in ts.static_features_]
   skt = [(p.hash_idx_.cpu().numpy(), p.hash_bit_.double().cpu().numpy(), P)
          for p in ts.projections_]
@@ -848,11 +853,14 @@ def bench_rfsf(args):
   errt = float(np.max(np.abs(Ft[idx].double().cpu().numpy() - reft).max(-1) /
                       np.abs(reft).max(-1)))
   res["ts"] = {"ms_per_transform": ts_ms, "features": int(Ft.shape[1]),
-               "max_rel_err_vs_fp64": errt, "scan_kernel": "lowrank_sketch_kernel<float,5>",
+               "max_rel_err_vs_fp64": errt,
+               "scan_kernel": f"lowrank_sketch_kernel<float,5,{'true' if dense else 'false'}>",
+               "scan_mode": "dense count sketch" if dense else "sparse count sketch",
                "scan_ms": tscan_ms, "scan_flop": tscan_flop,
                "scan_TFLOPs": tscan_flop / (tscan_ms * 1e-3) / 1e12,
-               "flop_note": "float64 multiply-adds of the direct convolution from the sparse "
-                            "count sketch, 2 * width * n_comp per level-step"}
+               "flop_note": "float64 flop of the convolutions as evaluated: width adds and "
+                            "2 n_comp^2 per level-step from the dense sketch, or 2 width n_comp "
+                            "from the sparse one"}
   print(json.dumps(res), flush=True)
 
 

@@ -525,11 +525,15 @@ int ksig_lowrank_levels_diagonal(int dtype, const void* du, int n_levels, int or
  * ksig_lowrank_levels; out: the level features before normalisation,
  * [n_seq, n_levels, n_comp] float64.  The convolution of a sketch is
  * evaluated directly from the sparse count sketch (width products per
- * component), equal to the FFT route up to rounding.  1 <= n_levels <= 8,
+ * component; with fewer components than features, each step's count sketch
+ * first and then n_comp products per component), equal to the FFT route up
+ * to rounding.  1 <= n_levels <= 8,
  * n_comp <= 2048 and the workgroup's LDS ((n_levels + the order terms of
  * the levels below the top) * 2 * n_comp doubles, plus 20 bytes per level
- * and input feature for the staged hashes, signs and increments) within the
- * device's limit, else KSIG_EUNSUPPORTED and nothing is launched.
+ * and input feature for the staged hashes, signs and increments, and with
+ * fewer components than features 12 bytes per level and component and 4 per
+ * level and feature for the sketches and their buckets) within the device's
+ * limit, else KSIG_EUNSUPPORTED and nothing is launched.
  */
 int ksig_lowrank_levels_sketch(int dtype, const void* du, int n_levels, int order, int64_t n_seq,
                                int steps, int width, const int32_t* hash_idx,

@@ -329,7 +329,9 @@ hipError_t lowrank_diag_scan_launch(int dtype, const void* du, int q, int n_leve
 constexpr int kSketchKpt = 2;
 
 // sum over features j in [j0, j1) of cs[j] row[k + m - hs[j]] for the
-// thread's components (two accumulator chains per component)
+// thread's components (two accumulator chains per component); ID: hs[j] = j
+// (cs is then a count sketch, indexed by component)
+template <bool ID>
 __device__ __forceinline__ void sketch_conv(const double* __restrict__ row,
                                             const double* __restrict__ cs,
                                             const int* __restrict__ hs, int j0, int j1, int m,
@@ -345,7 +347,7 @@ __device__ __forceinline__ void sketch_conv(const double* __restrict__ row,
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       cj[u] = cs[j + u];
-      hj[u] = hs[j + u];
+      hj[u] = ID ? j + u : hs[j + u];
     }
 #pragma unroll
     for (int c = 0; c < kSketchKpt; ++c) {
@@ -357,7 +359,8 @@ __device__ __forceinline__ void sketch_conv(const double* __restrict__ row,
   }
   for (; j < j1; ++j)
 #pragma unroll
-    for (int c = 0; c < kSketchKpt; ++c) e[c] = __builtin_fma(cs[j], rm[kr[c] - hs[j]], e[c]);
+    for (int c = 0; c < kSketchKpt; ++c)
+      e[c] = __builtin_fma(cs[j], rm[kr[c] - (ID ? j : hs[j])], e[c]);
 #pragma unroll
   for (int c = 0; c < kSketchKpt; ++c) acc[c] = e[c] + o[c];
 }
@@ -365,7 +368,11 @@ __device__ __forceinline__ void sketch_conv(const double* __restrict__ row,
 // per-level stride of the staged tables: rows of 4-feature groups
 __host__ __device__ inline int sketch_wpad(int width) { return (width + 3) & ~3; }
 
-template <typename T, int NL>
+// DENSE (m < width): each step's increments are first summed into their count
+// sketch b_i = CS_i(u_i) (m values, by buckets of features listed once per
+// launch), and the convolution runs over the m components of b instead of
+// the width features: (A (*) b)[k] = sum_p b[p] A[(k - p) mod m].
+template <typename T, int NL, bool DENSE>
 __global__ void __launch_bounds__(1024)
 lowrank_sketch_kernel(const T* __restrict__ du, int64_t n_seq, int steps, int width,
                       const int* __restrict__ hidx, const T* __restrict__ hbit, int m, int order,
@@ -382,10 +389,16 @@ lowrank_sketch_kernel(const T* __restrict__ du, int64_t n_seq, int steps, int wi
   double* sv = sk_lds + rows * m2;   // [NL][wpad] signs, zero for a hash outside [0, m)
   double* cst = sv + NL * wpad;      // [NL][wpad] this step's signed increments
   double* red = cst + NL * wpad;     // [ng][NL][m] the groups' partial sums (ng > 1)
-  int* hs = reinterpret_cast<int*>(red + (ng > 1 ? ng * NL * m : 0));  // [NL][wpad]
-  // group g's feature slice, in whole 4-feature groups
-  const int chunk = ((width + 4 * ng - 1) / (4 * ng)) * 4;
-  const int j0 = min(width, g * chunk), j1 = min(width, j0 + chunk);
+  const int mpad = sketch_wpad(m);
+  double* bsk = red + (ng > 1 ? ng * NL * m : 0);  // DENSE: [NL][mpad] the step's count sketches
+  int* hs = reinterpret_cast<int*>(bsk + (DENSE ? NL * mpad : 0));  // [NL][wpad]
+  int* boff = hs + NL * wpad;        // DENSE: [NL][m+1] bucket offsets
+  int* bidx = boff + (DENSE ? NL * (m + 1) : 0);  // DENSE: [NL][width] features by bucket
+  // group g's slice of the terms (features, or DENSE components), in whole
+  // groups of 4
+  const int nterm = DENSE ? m : width;
+  const int chunk = ((nterm + 4 * ng - 1) / (4 * ng)) * 4;
+  const int j0 = min(nterm, g * chunk), j1 = min(nterm, j0 + chunk);
   for (int j = tid; j < rows * m2; j += bs)
     sk_lds[j] = (j == 0 || j == m) ? 1.0 : 0.0;  // row 0: e_0 twice; the rest zero
   for (int e = tid; e < NL * wpad; e += bs) {
@@ -395,6 +408,21 @@ lowrank_sketch_kernel(const T* __restrict__ du, int64_t n_seq, int steps, int wi
     hs[e] = ok ? h : 0;
     sv[e] = ok ? (double)hbit[i * width + j] : 0.0;
     cst[e] = 0.0;
+  }
+  if constexpr (DENSE) {
+    __syncthreads();  // the hashes are staged
+    if (tid < NL) {   // level tid's features by bucket, in feature order
+      const int i = tid;
+      int* bo = boff + i * (m + 1);
+      int* cur = reinterpret_cast<int*>(bsk + i * mpad);  // scratch until the first sketch
+      for (int p = 0; p <= m; ++p) bo[p] = 0;
+      for (int j = 0; j < width; ++j)
+        if ((unsigned)hidx[i * width + j] < (unsigned)m) ++bo[hs[i * wpad + j] + 1];
+      for (int p = 0; p < m; ++p) bo[p + 1] += bo[p];
+      for (int p = 0; p < m; ++p) cur[p] = bo[p];
+      for (int j = 0; j < width; ++j)
+        if ((unsigned)hidx[i * width + j] < (unsigned)m) bidx[i * width + cur[hs[i * wpad + j]]++] = j;
+    }
   }
   int kc[kSketchKpt], kr[kSketchKpt];  // components; the ones read (0 past m)
 #pragma unroll
@@ -410,6 +438,17 @@ lowrank_sketch_kernel(const T* __restrict__ du, int64_t n_seq, int steps, int wi
       cst[i * wpad + j] = sv[i * wpad + j] * (double)ua[i * level_stride + (int64_t)t * width + j];
     }
   };
+  auto bucket = [&]() {  // DENSE: count sketches of the staged increments
+    for (int e = tid; e < NL * m; e += bs) {
+      const int i = e / m, p = e - i * m;
+      const int* bo = boff + i * (m + 1);
+      double sum = 0.0;
+      for (int k = bo[p]; k < bo[p + 1]; ++k) sum += cst[i * wpad + bidx[i * width + k]];
+      bsk[i * mpad + p] = sum;
+    }
+  };
+  // the coefficients a convolution of level i reads, and their hashes
+  auto coef = [&](int i) { return DENSE ? bsk + i * mpad : cst + i * wpad; };
   auto crow = [&](int i) { return sk_lds + (1 + i) * m2; };  // C_i, i < NL-1
   // at order p > 1: level i's order-r terms of this step, i < NL-1
   auto rrow = [&](int i, int r) {
@@ -444,16 +483,25 @@ lowrank_sketch_kernel(const T* __restrict__ du, int64_t n_seq, int steps, int wi
   double Ctop[kSketchKpt];
 #pragma unroll
   for (int c = 0; c < kSketchKpt; ++c) Ctop[c] = 0.0;
-  __syncthreads();  // the signs are staged
+  __syncthreads();  // the signs (and buckets) are staged
   if (steps > 0) stage(0);
   __syncthreads();
+  if constexpr (DENSE) {
+    if (steps > 0) bucket();
+    __syncthreads();
+  }
   for (int t = 0; t < steps; ++t) {
+    // DENSE: the convolutions read the sketches, so the next step's
+    // increments are staged now and sketched after the mid-step barrier
+    if constexpr (DENSE) {
+      if (t + 1 < steps) stage(t + 1);
+    }
     double S[NL][kSketchKpt];  // this step's terms summed over the orders
     if (order == 1) {
 #pragma unroll
       for (int i = 0; i < NL; ++i) {
-        sketch_conv(i == 0 ? sk_lds : crow(i - 1), cst + i * wpad, hs + i * wpad, j0, j1, m, kr,
-                    S[i]);
+        sketch_conv<DENSE>(i == 0 ? sk_lds : crow(i - 1), coef(i), hs + i * wpad, j0, j1, m, kr,
+                           S[i]);
         part(i, S[i]);
       }
       __syncthreads();  // every convolution has read the sums and the step
@@ -468,7 +516,7 @@ lowrank_sketch_kernel(const T* __restrict__ du, int64_t n_seq, int steps, int wi
         if (ng > 1) {
           for (int r = 0; r < d; ++r) {
             double v[kSketchKpt];
-            sketch_conv(src(r), cst + i * wpad, hs + i * wpad, j0, j1, m, kr, v);
+            sketch_conv<DENSE>(src(r), coef(i), hs + i * wpad, j0, j1, m, kr, v);
             part(r, v);
           }
           __syncthreads();  // the partials of every group are in
@@ -479,7 +527,7 @@ lowrank_sketch_kernel(const T* __restrict__ du, int64_t n_seq, int steps, int wi
           for (int r = 0; r < d; ++r) {
             double v[kSketchKpt];
             if (ng > 1) total(r, v);
-            else sketch_conv(src(r), cst + i * wpad, hs + i * wpad, j0, j1, m, kr, v);
+            else sketch_conv<DENSE>(src(r), coef(i), hs + i * wpad, j0, j1, m, kr, v);
             const double wt = 1.0 / (r + 1);
 #pragma unroll
             for (int c = 0; c < kSketchKpt; ++c) {
@@ -500,7 +548,10 @@ lowrank_sketch_kernel(const T* __restrict__ du, int64_t n_seq, int steps, int wi
 #pragma unroll
       for (int c = 0; c < kSketchKpt; ++c) Ctop[c] += S[NL - 1][c];
     }
-    if (t + 1 < steps) stage(t + 1);
+    if (t + 1 < steps) {
+      if constexpr (DENSE) bucket();
+      else stage(t + 1);
+    }
     __syncthreads();  // sums and the next step staged before they are read
   }
   if (g != 0) return;
@@ -521,29 +572,36 @@ __host__ inline int sketch_threads(int m) {
   const int per = (m + kSketchKpt - 1) / kSketchKpt;
   return ((per + kWave - 1) / kWave) * kWave;
 }
+// the count sketch first when it has fewer terms than the features
+bool lowrank_sketch_dense(int m, int width) { return m < width; }
 size_t lowrank_sketch_lds(int n_levels, int order, int m, int width, int ng) {
+  const bool dense = lowrank_sketch_dense(m, width);
   return (size_t)sketch_rows(n_levels, order) * 2 * (size_t)m * sizeof(double) +
          (size_t)n_levels * sketch_wpad(width) * (2 * sizeof(double) + sizeof(int)) +
-         (ng > 1 ? (size_t)ng * n_levels * m * sizeof(double) : 0);
+         (ng > 1 ? (size_t)ng * n_levels * m * sizeof(double) : 0) +
+         (dense ? (size_t)n_levels * (sketch_wpad(m) * sizeof(double) +
+                                      (size_t)(m + 1 + width) * sizeof(int))
+                : 0);
 }
 // up to 8 waves per sequence where the components leave room, no more groups
-// than 4-feature slices, and fewer where the LDS would not fit
+// than 4-term slices, and fewer where the LDS would not fit
 int lowrank_sketch_groups(int n_levels, int order, int m, int width, size_t lds_limit) {
   const int w = sketch_threads(m);
   int ng = 512 / w;
   if (ng < 1) ng = 1;
-  const int slices = (width + 3) / 4;
+  const int nterm = lowrank_sketch_dense(m, width) ? m : width;
+  const int slices = (nterm + 3) / 4;
   if (ng > slices) ng = slices > 0 ? slices : 1;
   while (ng > 1 && lowrank_sketch_lds(n_levels, order, m, width, ng) > lds_limit) --ng;
   return ng;
 }
 int lowrank_sketch_max_comp() { return 1024 * kSketchKpt; }
 
-template <typename T, int NL>
+template <typename T, int NL, bool DENSE>
 static hipError_t launch_sketch(const void* du, int64_t n_seq, int steps, int width,
                                 const int* hidx, const void* hbit, int m, int order, int ng,
                                 double* out, hipStream_t s) {
-  auto k = lowrank_sketch_kernel<T, NL>;
+  auto k = lowrank_sketch_kernel<T, NL, DENSE>;
   const size_t lds = lowrank_sketch_lds(NL, order, m, width, ng);
   if (lds > 64 * 1024) {
     hipError_t e = hipFuncSetAttribute((const void*)k,
@@ -560,9 +618,14 @@ template <typename T>
 static hipError_t sketch_levels(int n_levels, const void* du, int64_t n_seq, int steps, int width,
                                 const int* hidx, const void* hbit, int m, int order, int ng,
                                 double* out, hipStream_t s) {
+  const bool dense = lowrank_sketch_dense(m, width);
   switch (n_levels) {
-#define KSIG_SKETCH_CASE(L) \
-    case L: return launch_sketch<T, L>(du, n_seq, steps, width, hidx, hbit, m, order, ng, out, s);
+#define KSIG_SKETCH_CASE(L)                                                                   \
+    case L:                                                                                   \
+      return dense ? launch_sketch<T, L, true>(du, n_seq, steps, width, hidx, hbit, m, order, \
+                                               ng, out, s)                                    \
+                   : launch_sketch<T, L, false>(du, n_seq, steps, width, hidx, hbit, m,       \
+                                                order, ng, out, s);
     KSIG_SKETCH_CASE(1)
     KSIG_SKETCH_CASE(2)
     KSIG_SKETCH_CASE(3)

@@ -103,7 +103,12 @@ def _sketches(rng, n_levels, width, m):
 @pytest.mark.parametrize("n_levels,order,m,width", [(1, 1, 50, 7), (3, 1, 100, 24),
                                                     (4, 2, 70, 16), (5, 5, 33, 9),
                                                     (3, 1, 300, 40), (2, 2, 1000, 12),
-                                                    (8, 3, 64, 5)])
+                                                    (8, 3, 64, 5),
+                                                    # fewer components than features:
+                                                    # the count sketch first
+                                                    (3, 1, 20, 50), (4, 2, 33, 100),
+                                                    (5, 5, 8, 40), (8, 2, 64, 200),
+                                                    (1, 1, 1, 9)])
 def test_sketch_scan(dtype, tol, n_levels, order, m, width):
   rng = np.random.default_rng(140 + n_levels + order + m)
   n, L = 4, 21
@@ -156,7 +161,7 @@ def test_tensorsketch_standalone_transform():
   assert ksig.projections.CountSketch is ksig.projections.TensorSketch
 
 
-@pytest.mark.parametrize("n_levels,order,m", [(4, 1, 40), (3, 2, 64), (3, 1, 2500)])
+@pytest.mark.parametrize("n_levels,order,m", [(4, 1, 40), (3, 2, 64), (3, 1, 2500), (4, 3, 10)])
 def test_rfsf_tensorsketch_class(n_levels, order, m):
   """SignatureFeatures with TensorSketch on random Fourier features against
   the oracle on the object's weights, hashes and signs; n_comp = 2500 takes
