@@ -248,7 +248,8 @@ def main():
                   help="only run an elementwise copy of MIB MiB (5 times): the "
                        "reference point for scaling HBM byte counters")
   ap.add_argument("--kernel-probe", default=None,
-                  choices=["fwd", "fwd_save", "adjoint", "contract", "pde_fwd", "pde_fwd_edges",
+                  choices=["fwd", "fwd_save", "adjoint", "adjoint_gp", "contract", "pde_fwd",
+                           "pde_fwd_edges",
                            "pde_adjoint", "pde_contract", "pde_fused"],
                   help="only time one kernel of the C4 training step on one 16384-pair "
                        "chunk (128 x 128 pairs, L=128, d=32, 6 levels), --steps launches: "
@@ -868,7 +869,8 @@ def kernel_probe(args):
   """One C4 kernel on one chunk of the training step: 128 Brownian paths
   (L=128, d=32) against themselves, 16384 pairs, 6 levels, RBF sigma=sqrt(32);
   the forward without (fwd) or with (fwd_save) the boundary state, the
-  adjoint (omega) or the contraction of omega with the inputs, each launched
+  adjoint (omega; adjoint_gp also the bandwidth gradient, which the C4 step
+  does not take) or the contraction of omega with the inputs, each launched
   --steps times after one untimed launch, timed with HIP events on the
   library's stream."""
   from ksig_amd import _lib, _ops
@@ -907,11 +909,11 @@ def kernel_probe(args):
         0, *xa, *xa, d, pairs.data_ptr(), P, _lib.STATIC_RBF, sig, nl, out.data_ptr(),
         None, save_off.data_ptr(), fws.data_ptr(), fwsb, st))
 
-  def adjoint():
+  def adjoint(with_gp=False):  # as the C4 step runs it: no bandwidth gradient
     _lib.check(lib.ksig_signature_levels_bwd(
         0, *xa, *xa, d, pairs.data_ptr(), P, _lib.STATIC_RBF, sig, nl, gout.data_ptr(),
-        save.data_ptr(), save_off.data_ptr(), omega.data_ptr(), w_off.data_ptr(), gp.data_ptr(),
-        ws.data_ptr(), wsb, st))
+        save.data_ptr(), save_off.data_ptr(), omega.data_ptr(), w_off.data_ptr(),
+        gp.data_ptr() if with_gp else None, ws.data_ptr(), wsb, st))
 
   def contract():
     c = 1.0 / (sig * sig)
@@ -923,6 +925,7 @@ def kernel_probe(args):
   if args.kernel_probe not in ("fwd_save", "fwd"):
     adjoint()
   f = {"fwd": fwd, "fwd_save": fwd_save, "adjoint": adjoint,
+       "adjoint_gp": lambda: adjoint(True),
        "contract": contract}[args.kernel_probe]
   f()
   torch.cuda.synchronize()

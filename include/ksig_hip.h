@@ -190,7 +190,9 @@ size_t ksig_signature_levels_fwd_save_workspace(int dtype, int d, int n_levels,
  *   omega[omega_off[p] + a*len_y + b] = dL/dG(a, b) * dG/dz
  * over the static-kernel Gram G(a, b) = k(x_a, y_b) (times G for RBF, plain
  * dL/dG for linear), and grad_param[p] = sum omega * D'(a,b) (RBF,
- * D' = log2(e) |x_a - y_b|^2 / max(2 sigma^2, 1e-12)) or sum omega * G (linear).
+ * D' = log2(e) |x_a - y_b|^2 / max(2 sigma^2, 1e-12)) or sum omega * G (linear);
+ * grad_param may be NULL when no parameter gradient is wanted (the float32
+ * adjoint then carries one flow fewer per step).
  * The caller contracts omega with the inputs to obtain dL/dx, dL/dy and the
  * bandwidth / scale gradient (ksig_static_grad_contract).  workspace: device
  * scratch of at least ksig_signature_levels_bwd_workspace(...) bytes, 16-byte

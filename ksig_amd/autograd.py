@@ -142,7 +142,8 @@ class _SigLevels(torch.autograd.Function):
           Xs.max_len, Ys.data.data_ptr(), Ys.off.data_ptr(), Ys.len.data_ptr(),
           Ys.max_len, Xs.d, pc.data_ptr(), c1 - c0, kind, pval, n_levels,
           gout[c0:c1].data_ptr(), save.data_ptr(), save_off[c0:c1].data_ptr(),
-          omega.data_ptr(), w_off.data_ptr(), gp.data_ptr(), ws.data_ptr(), ws_bytes,
+          omega.data_ptr(), w_off.data_ptr(),
+          gp.data_ptr() if ctx.needs_input_grad[2] else None, ws.data_ptr(), ws_bytes,
           _lib.stream_handle(dev)))
       if ctx.needs_input_grad[2]:
         gparam_total = gparam_total + gp.sum()

@@ -101,7 +101,7 @@ dp_backward_kernel(const DpBwdArgs a) {
   }
   const int Nr = Lx - 1, Nc = Ly - 1;
   if (Nr <= 0 || Nc <= 0) {
-    if (lane == 0) a.gparam[pr] = 0.0;
+    if (lane == 0 && a.gparam) a.gparam[pr] = 0.0;
     return;
   }
   const int nrows = Nr + 1;  // + virtual row above the grid
@@ -364,7 +364,7 @@ dp_backward_kernel(const DpBwdArgs a) {
   if (step < total) step_fn(step, y0);
   gsig += (double)gsig_row;
   gsig = wave_sum(gsig);
-  if (lane == 0) a.gparam[pr] = gsig;
+  if (lane == 0 && a.gparam) a.gparam[pr] = gsig;
 }
 
 template <typename T, int NLEV, int DMAX, bool VEC>

@@ -78,13 +78,18 @@ __host__ __device__ constexpr int bwd_mfma_pc(int nc) {
   return (((nc + 2 > 80 ? nc + 2 : 80) + 15) / 16) * 16;
 }
 
-template <int NLEV, int DMAX, int KIND, bool VEC>
+// GP: the parameter gradient sum omega * D' per pair (grad_param); without
+// it the static-kernel argument D' does not flow and the FIFO slot is a
+// quad shorter at 6 levels.
+template <int NLEV, int DMAX, int KIND, bool VEC, bool GP>
 __global__ void __launch_bounds__(kWave)
 dp_backward_mfma_kernel(const DpBwdArgs a) {
   using T = float;
   constexpr int NL1 = NLEV > 1 ? NLEV - 1 : 1;
   constexpr int NC = NLEV - 1;              // prefix levels 1..NLEV-1
-  constexpr int NF = 3 + 2 * (NLEV - 1);    // [arg, D', gM, C_1.., Dq_1..]
+  // flows [G, (D',) gM, C_1.., Dq_1..]
+  constexpr int kDv = 1, kGM = GP ? 2 : 1, kC = kGM + 1, kD = kC + NC;
+  constexpr int NF = kD + NC;
   constexpr int NFG = (NF + 3) & ~3;        // FIFO slot stride (16-byte quads)
   constexpr int KB = DMAX / 4;
   constexpr int W = kBwdRingW;
@@ -108,7 +113,7 @@ dp_backward_mfma_kernel(const DpBwdArgs a) {
   }
   const int Nr = Lx - 1, Nc = Ly - 1;
   if (Nr <= 0 || Nc <= 0) {
-    if (lane == 0) a.gparam[pr] = 0.0;
+    if (GP && lane == 0) a.gparam[pr] = 0.0;
     return;
   }
   const int nrows = Nr + 1;  // + virtual row above the grid
@@ -237,12 +242,12 @@ dp_backward_mfma_kernel(const DpBwdArgs a) {
       }
       T v[NFG];
       v[0] = g;
-      v[1] = dv;
-      v[2] = T(0);
+      if constexpr (GP) v[kDv] = dv;
+      v[kGM] = T(0);
 #pragma unroll
       for (int m = 0; m < NC; ++m) {
-        v[3 + m] = (j >= 0 && j < Nc) ? colsum[m * Nc + j] : T(0);
-        v[3 + NC + m] = T(0);
+        v[kC + m] = (j >= 0 && j < Nc) ? colsum[m * Nc + j] : T(0);
+        v[kD + m] = T(0);
       }
 #pragma unroll
       for (int k = NF; k < NFG; ++k) v[k] = T(0);
@@ -365,7 +370,7 @@ dp_backward_mfma_kernel(const DpBwdArgs a) {
       T R = M;
 #pragma unroll
       for (int m = 0; m < NC; ++m) {
-        const T C = in[3 + m] - R;
+        const T C = in[kC + m] - R;
         Pn[m] = P[m] - C;
         Cout[m] = C;
         R = M * Pn[m];
@@ -382,34 +387,34 @@ dp_backward_mfma_kernel(const DpBwdArgs a) {
     gM = cell ? gM : T(0);
     T Dout[NL1];
 #pragma unroll
-    for (int m = 0; m < NC; ++m) Dout[m] = in[3 + NC + m] + M * lam[m + 2];
+    for (int m = 0; m < NC; ++m) Dout[m] = in[kD + m] + M * lam[m + 2];
 
     // finalise dL/dG at G point (i+1, j+1)
     const bool fin_ok = active && !prim && i >= -1 && j >= -1;
-    const T gG = (gM - gm_right) - (in[2] - gmup_right);
+    const T gG = (gM - gm_right) - (in[kGM] - gmup_right);
     const T om = rbf ? gG * gup_right : gG;
     __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(om), omega_rs,
                                           fin_ok ? (oi_row + oi_col) * 4 : int(kBufOob), 0, 0);
-    gsig_row += (fin_ok ? om : T(0)) * dup_right;
+    if constexpr (GP) gsig_row += (fin_ok ? om : T(0)) * dup_right;
 
     // state
     g_right = g;
     gup_right = in[0];
-    dup_right = in[1];
+    if constexpr (GP) dup_right = in[kDv];
     gm_right = gM;
-    gmup_right = in[2];
+    gmup_right = in[kGM];
 #pragma unroll
     for (int m = 0; m < NC; ++m) {
       P[m] = Pn[m];
-      S[m] += in[3 + NC + m];
+      S[m] += in[kD + m];
     }
     fl[0] = g;
-    fl[1] = dv;
-    fl[2] = gM;
+    if constexpr (GP) fl[kDv] = dv;
+    fl[kGM] = gM;
 #pragma unroll
     for (int m = 0; m < NC; ++m) {
-      fl[3 + m] = Cout[m];
-      fl[3 + NC + m] = Dout[m];
+      fl[kC + m] = Cout[m];
+      fl[kD + m] = Dout[m];
     }
     {
       const int fo = lane == kWave - 1 && active ? s * kSlotB : int(kBufOob);
@@ -463,14 +468,16 @@ dp_backward_mfma_kernel(const DpBwdArgs a) {
     step_fn(step + 1, arg1, fin1);
   }
   if (step < total) step_fn(step, arg0, fin0);
-  gsig += (double)gsig_row;
-  gsig = wave_sum(gsig);
-  if (lane == 0) a.gparam[pr] = gsig;
+  if constexpr (GP) {
+    gsig += (double)gsig_row;
+    gsig = wave_sum(gsig);
+    if (lane == 0) a.gparam[pr] = gsig;
+  }
 }
 
-template <int NLEV, int DMAX, int KIND, bool VEC>
-hipError_t launch_bwd_mfma(const DpBwdArgs& a, size_t lds, hipStream_t s) {
-  auto k = dp_backward_mfma_kernel<NLEV, DMAX, KIND, VEC>;
+template <int NLEV, int DMAX, int KIND, bool VEC, bool GP>
+hipError_t launch_bwd_mfma_gp(const DpBwdArgs& a, size_t lds, hipStream_t s) {
+  auto k = dp_backward_mfma_kernel<NLEV, DMAX, KIND, VEC, GP>;
   if (lds > 64 * 1024) {
     hipError_t e = hipFuncSetAttribute((const void*)k,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -478,6 +485,12 @@ hipError_t launch_bwd_mfma(const DpBwdArgs& a, size_t lds, hipStream_t s) {
   }
   hipLaunchKernelGGL(k, dim3((unsigned)a.n_pairs), dim3(kWave), lds, s, a);
   return hipGetLastError();
+}
+
+template <int NLEV, int DMAX, int KIND, bool VEC>
+hipError_t launch_bwd_mfma(const DpBwdArgs& a, size_t lds, hipStream_t s) {
+  if (a.gparam) return launch_bwd_mfma_gp<NLEV, DMAX, KIND, VEC, true>(a, lds, s);
+  return launch_bwd_mfma_gp<NLEV, DMAX, KIND, VEC, false>(a, lds, s);
 }
 
 #define KSIG_BWD_MFMA_LEVELS(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(12)

@@ -563,7 +563,8 @@ int ksig_signature_levels_bwd(int dtype, const void* x, const int64_t* x_off,
                               const int64_t* save_off, void* omega,
                               const int64_t* omega_off, double* grad_param,
                               void* workspace, size_t workspace_bytes, void* stream) {
-  int rc = check_common(dtype, x, y, d, pairs, n_pairs, grad_param);
+  // grad_param may be NULL (no parameter gradient); omega is the required output
+  int rc = check_common(dtype, x, y, d, pairs, n_pairs, reinterpret_cast<double*>(omega));
   if (rc) return rc;
   if (n_levels < 1) return fail(KSIG_EINVAL, "n_levels must be positive");
   if (!ksig::dp_backward_supported(n_levels))
