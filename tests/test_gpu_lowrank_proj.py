@@ -201,3 +201,19 @@ def test_tensorized_projection_rank_above_one_follows_the_reference():
     with pytest.raises(ValueError):
       sf.fit(Xs)
       sf.transform(Xs)
+
+
+def test_scans_on_empty_inputs():
+  """No time steps (sequences of one point, differenced): every level
+  feature is an empty sum; no sequences: an empty result."""
+  z = to_dev(np.zeros((3, 4, 0, 12)))
+  dz = _ops.lowrank_levels_diagonal(z, 3, 2)
+  assert dz.shape == (4, (3 + 9 + 27) * 4) and not bool(dz.abs().max() > 0)
+  idx = to_dev(np.zeros((3, 12), dtype=np.int32))
+  bit = to_dev(np.ones((3, 12)))
+  for m in (5, 30):  # the count sketch first, and from the sparse sketch
+    out = _ops.lowrank_levels_sketch(z, idx, bit, m, 2)
+    assert out.shape == (4, 3, m) and not bool(out.abs().max() > 0)
+  e = to_dev(np.zeros((3, 0, 7, 12)))
+  assert _ops.lowrank_levels_sketch(e, idx, bit, 5).shape == (0, 3, 5)
+  assert _ops.lowrank_levels_diagonal(e, 3, 1).shape == (0, (3 + 9 + 27) * 4)
