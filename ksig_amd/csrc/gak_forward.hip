@@ -112,6 +112,10 @@ gak_forward_kernel(const GakArgs a) {
   // one step ahead, comes at least a step after lane 63's write of the slot.
   const int Pc = Ly <= kWave ? kWave + 1 : Ly;
   const int nb = (Lx + BR - 1) / BR;
+  if (!MB && nb > 1) {  // a row side longer than the launch's stated maximum
+    if (lane == 0) a.out[pr] = NAN;
+    return;
+  }
 
   // LDS: FIFO mantissas [Pc] (double) and exponents [Pc] (int), then y rows
   // [Ly][DMAX] (scaled) and norms [Ly].
